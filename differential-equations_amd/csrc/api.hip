@@ -1,0 +1,565 @@
+// C-ABI of the ensemble integrator (include/de_api.h, include/de_workload.h):
+// descriptor validation, host-memory staging, schedule selection and the
+// synthetic workload generators.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/de_api.h"
+#include "../../include/de_workload.h"
+#include "de_launch.h"
+#include "de_noise.h"
+#include "de_spring.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(DE_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+const int kFamilyDim[] = {3, 2, 0, 1, 1, 2, 1, 3};
+const int kFamilyParams[] = {3, 1, 0, 1, 2, 1, 2, 0};
+
+// Work counters for persistent launches: a ring of device slots, one per call
+// in flight (calls on different streams must not share one).
+constexpr int kCounterSlots = 1024;
+unsigned long long* g_counters = nullptr;
+std::atomic<int> g_counter_next(0);
+std::mutex g_counter_mu;
+
+int counter_slot(unsigned long long** out) {
+  std::lock_guard<std::mutex> lk(g_counter_mu);
+  if (!g_counters) {
+    hipError_t e = hipMalloc(&g_counters, sizeof(unsigned long long) * kCounterSlots);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
+  }
+  int k = g_counter_next.fetch_add(1) % kCounterSlots;
+  *out = g_counters + k;
+  return DE_OK;
+}
+
+// Device staging of host-memory descriptors.
+struct Staging {
+  std::vector<void*> bufs;
+  hipError_t err = hipSuccess;
+  ~Staging() {
+    for (void* p : bufs) hipFree(p);
+  }
+  template <class T>
+  T* in(const T* host, size_t count) {
+    if (!host || count == 0 || err != hipSuccess) return nullptr;
+    void* d = nullptr;
+    err = hipMalloc(&d, count * sizeof(T));
+    if (err != hipSuccess) return nullptr;
+    bufs.push_back(d);
+    err = hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice);
+    return (T*)d;
+  }
+  template <class T>
+  T* out(T* host, size_t count) {
+    if (!host || count == 0 || err != hipSuccess) return nullptr;
+    void* d = nullptr;
+    err = hipMalloc(&d, count * sizeof(T));
+    if (err != hipSuccess) return nullptr;
+    bufs.push_back(d);
+    return (T*)d;
+  }
+  template <class T>
+  void back(T* host, const T* dev, size_t count) {
+    if (!host || !dev || count == 0 || err != hipSuccess) return;
+    err = hipMemcpy(host, dev, count * sizeof(T), hipMemcpyDeviceToHost);
+  }
+};
+
+__global__ void fill_nan_kernel(double* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = __longlong_as_double(0x7ff8000000000000LL);
+}
+
+int fill_nan(double* p, int64_t n, hipStream_t s) {
+  if (!p || n <= 0) return DE_OK;
+  int64_t grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(fill_nan_kernel, dim3((unsigned)grid), dim3(256), 0, s, p, n);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DE_OK : hip_fail(e, "fill_nan");
+}
+
+int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
+  if (!b || !o) return fail(DE_ERR_INVALID, "null descriptor");
+  if (b->n < 0) return fail(DE_ERR_INVALID, "n < 0");
+  if (b->family < 0 || b->family > DE_FAMILY_ROBERTSON)
+    return fail(DE_ERR_INVALID, "unknown family %d", b->family);
+  if (b->method < DE_METHOD_DOPRI5 || b->method > DE_METHOD_EULER)
+    return fail(DE_ERR_INVALID, "unknown method %d", b->method);
+  if (b->family != DE_FAMILY_SPRING_CHAIN && b->dim != kFamilyDim[b->family])
+    return fail(DE_ERR_INVALID, "family %d has dimension %d, got dim %d", b->family,
+                kFamilyDim[b->family], b->dim);
+  if (b->family == DE_FAMILY_SPRING_CHAIN &&
+      (!b->sizes || !b->state_offset || !b->param_offset) && b->n > 0)
+    return fail(DE_ERR_INVALID, "spring chain needs sizes, state_offset and param_offset");
+  if (b->n_eval < 0) return fail(DE_ERR_INVALID, "n_eval < 0");
+  if (b->n_eval > 0 && !b->t_eval) return fail(DE_ERR_INVALID, "n_eval > 0 without t_eval");
+  if (b->n > 0 && (!b->y0 || !o->y_final || !o->t_final || !o->accepted || !o->rejected ||
+                   !o->evals || !o->status))
+    return fail(DE_ERR_INVALID, "missing input or output array");
+  if (b->n > 0 && b->family != DE_FAMILY_SPRING_CHAIN && kFamilyParams[b->family] > 0 &&
+      !b->params)
+    return fail(DE_ERR_INVALID, "family %d needs params", b->family);
+  if (b->schedule < DE_SCHED_AUTO || b->schedule > DE_SCHED_PERSISTENT)
+    return fail(DE_ERR_INVALID, "unknown schedule %d", b->schedule);
+  if (b->max_steps < 0) return fail(DE_ERR_INVALID, "max_steps < 0");
+  return DE_OK;
+}
+
+de::OdeArgs make_args(const de_ode_batch* b, const de_ode_result* o) {
+  de::OdeArgs a;
+  memset(&a, 0, sizeof a);
+  a.n = b->n;
+  a.params = b->params;
+  a.y0 = b->y0;
+  a.t0 = b->t0;
+  a.tf = b->tf;
+  a.t0s = b->t0_scalar;
+  a.tfs = b->tf_scalar;
+  a.rtol_traj = b->rtol_traj;
+  a.atol_traj = b->atol_traj;
+  a.rtol = b->rtol;
+  a.atol = b->atol;
+  a.h0 = b->h0;
+  a.h_min = b->h_min;
+  a.h_max = b->h_max;
+  a.safety = b->safety_factor;
+  a.min_scale = b->min_scale;
+  a.max_scale = b->max_scale;
+  a.max_steps = b->max_steps;
+  a.t_eval = b->t_eval;
+  a.n_eval = b->n_eval;
+  a.y_final = o->y_final;
+  a.t_final = o->t_final;
+  a.accepted = o->accepted;
+  a.rejected = o->rejected;
+  a.evals = o->evals;
+  a.status = o->status;
+  a.dense = b->n_eval > 0 ? o->dense : nullptr;
+  a.n_dense = b->n_eval > 0 ? o->n_dense : nullptr;
+  if (!a.dense) a.n_eval = 0;
+  return a;
+}
+
+// Launch on device-resident arguments.
+int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs a,
+                   hipStream_t s) {
+  int sched = b->schedule == DE_SCHED_AUTO ? de::kSchedPersistent : b->schedule;
+  if (b->family == DE_FAMILY_SPRING_CHAIN) {
+    std::vector<int32_t> sizes_copy;
+    if (!host_sizes) {
+      sizes_copy.resize((size_t)b->n);
+      hipError_t e = hipMemcpy(sizes_copy.data(), b->sizes, sizeof(int32_t) * b->n,
+                               hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "reading spring sizes");
+      host_sizes = sizes_copy.data();
+    }
+    int r = de::parity::launch_spring(b, host_sizes, a, sched, s);
+    if (r == 1) return fail(DE_ERR_UNSUPPORTED, "spring-chain mapping not supported");
+    if (r < 0) return r == -4 ? fail(DE_ERR_NOMEM, "spring workspace allocation failed")
+                              : hip_fail(hipGetLastError(), "spring launch");
+    return DE_OK;
+  }
+  if (sched == de::kSchedPersistent) {
+    int rc = counter_slot(&a.counter);
+    if (rc != DE_OK) return rc;
+  }
+  if (a.n_eval > 0) {
+    int rc = fill_nan(a.dense, (int64_t)a.n_eval * b->dim * b->n, s);
+    if (rc != DE_OK) return rc;
+  }
+  int r;
+  if (b->flags & DE_FLAG_FAST_MATH)
+    r = de::fast::launch_ode_lane(b->family, b->method, a, sched, s);
+  else
+    r = de::parity::launch_ode_lane(b->family, b->method, a, sched, s);
+  if (r == 1)
+    return fail(DE_ERR_UNSUPPORTED, "family %d / method %d not built%s", b->family, b->method,
+                (b->flags & DE_FLAG_FAST_MATH) ? " in the fast build" : "");
+  if (r < 0) return hip_fail(hipGetLastError(), "kernel launch");
+  return DE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* de_last_error(void) { return g_err.c_str(); }
+
+const char* de_version(void) { return "de_amd 0.1.0 (gfx950, HIP)"; }
+
+void de_ode_batch_init(de_ode_batch* b) {
+  memset(b, 0, sizeof *b);
+  b->rtol = 1.0e-6;
+  b->atol = 1.0e-6;
+  b->h0 = 0.0;
+  b->h_min = 0.0;
+  b->h_max = INFINITY;
+  b->max_steps = 10000;
+  b->safety_factor = 0.9;
+  b->min_scale = 0.2;
+  b->max_scale = 10.0;
+}
+
+void de_sde_batch_init(de_sde_batch* b) {
+  memset(b, 0, sizeof *b);
+  b->h_min = 0.0;
+  b->h_max = INFINITY;
+  b->max_steps = 10000;
+}
+
+int64_t de_ode_workspace_bytes(const de_ode_batch* b) {
+  if (!b) return 0;
+  if (b->family == DE_FAMILY_SPRING_CHAIN) return de::parity::spring_workspace_bytes(b);
+  return (int64_t)sizeof(unsigned long long);
+}
+
+int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
+  int rc = validate_ode(b, o);
+  if (rc != DE_OK) return rc;
+  if (b->n == 0) return DE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (!(b->flags & DE_FLAG_HOST_MEMORY)) return run_ode_device(b, nullptr, make_args(b, o), s);
+
+  // Host-memory call: stage inputs, run, copy results back.
+  const int64_t n = b->n;
+  const bool ragged = b->family == DE_FAMILY_SPRING_CHAIN;
+  int64_t state_len = ragged ? b->state_offset[n] : (int64_t)b->dim * n;
+  int64_t param_len = ragged ? b->param_offset[n] : (int64_t)kFamilyParams[b->family] * n;
+  Staging st;
+  de_ode_batch db = *b;
+  db.flags &= ~DE_FLAG_HOST_MEMORY;
+  db.sizes = st.in(b->sizes, ragged ? n : 0);
+  db.state_offset = st.in(b->state_offset, ragged ? n + 1 : 0);
+  db.param_offset = st.in(b->param_offset, ragged ? n + 1 : 0);
+  db.params = st.in(b->params, (size_t)param_len);
+  db.y0 = st.in(b->y0, (size_t)state_len);
+  db.t0 = st.in(b->t0, b->t0 ? n : 0);
+  db.tf = st.in(b->tf, b->tf ? n : 0);
+  db.rtol_traj = st.in(b->rtol_traj, b->rtol_traj ? n : 0);
+  db.atol_traj = st.in(b->atol_traj, b->atol_traj ? n : 0);
+  db.t_eval = st.in(b->t_eval, b->n_eval);
+  de_ode_result dr;
+  dr.y_final = st.out(o->y_final, state_len);
+  dr.t_final = st.out(o->t_final, n);
+  dr.accepted = st.out(o->accepted, n);
+  dr.rejected = st.out(o->rejected, n);
+  dr.evals = st.out(o->evals, n);
+  dr.status = st.out(o->status, n);
+  int64_t dense_len = (int64_t)b->n_eval * state_len;
+  dr.dense = b->n_eval > 0 ? st.out(o->dense, dense_len) : nullptr;
+  dr.n_dense = b->n_eval > 0 ? st.out(o->n_dense, n) : nullptr;
+  if (st.err != hipSuccess) return hip_fail(st.err, "staging inputs");
+  rc = run_ode_device(&db, ragged ? b->sizes : nullptr, make_args(&db, &dr), s);
+  if (rc != DE_OK) return rc;
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "kernel execution");
+  st.back(o->y_final, dr.y_final, state_len);
+  st.back(o->t_final, dr.t_final, n);
+  st.back(o->accepted, dr.accepted, n);
+  st.back(o->rejected, dr.rejected, n);
+  st.back(o->evals, dr.evals, n);
+  st.back(o->status, dr.status, n);
+  if (dr.dense) st.back(o->dense, dr.dense, dense_len);
+  if (dr.n_dense) st.back(o->n_dense, dr.n_dense, n);
+  if (st.err != hipSuccess) return hip_fail(st.err, "copying results");
+  return DE_OK;
+}
+
+int de_sde_batch_solve(const de_sde_batch* b, de_sde_result* o, void* stream) {
+  if (!b || !o) return fail(DE_ERR_INVALID, "null descriptor");
+  if (b->n < 0) return fail(DE_ERR_INVALID, "n < 0");
+  if (b->family != DE_SDE_ORNSTEIN_UHLENBECK && b->family != DE_SDE_LINEAR_DIAGONAL)
+    return fail(DE_ERR_INVALID, "unknown SDE family %d", b->family);
+  if (b->scheme != DE_SCHEME_EULER_MARUYAMA && b->scheme != DE_SCHEME_MILSTEIN)
+    return fail(DE_ERR_INVALID, "unknown scheme %d", b->scheme);
+  if (b->dim != 1 && b->dim != 2 && b->dim != 4 && b->dim != 8)
+    return fail(DE_ERR_UNSUPPORTED, "SDE dimension %d not built (1, 2, 4, 8)", b->dim);
+  if (b->noise < DE_NOISE_PHILOX || b->noise > DE_NOISE_CONSTANT)
+    return fail(DE_ERR_INVALID, "unknown noise source %d", b->noise);
+  if (b->noise == DE_NOISE_TABLE && b->noise_steps > 0 && !b->noise_table)
+    return fail(DE_ERR_INVALID, "noise table missing");
+  if (b->path_stride < 0) return fail(DE_ERR_INVALID, "path_stride < 0");
+  if (b->n == 0) return DE_OK;
+  if (!b->params || !b->y0 || !o->y_final || !o->t_final || !o->steps || !o->evals ||
+      !o->status)
+    return fail(DE_ERR_INVALID, "missing input or output array");
+  if (b->path_stride > 0 && !o->path)
+    return fail(DE_ERR_INVALID, "path_stride > 0 needs a path buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = b->n;
+  const int P = b->family == DE_SDE_ORNSTEIN_UHLENBECK ? 3 * b->dim : 2 * b->dim;
+  de::SdeArgs a;
+  memset(&a, 0, sizeof a);
+  a.n = n;
+  a.dim = b->dim;
+  a.noise = b->noise;
+  a.t0 = b->t0;
+  a.tf = b->tf;
+  a.h0 = b->h0;
+  a.h_min = b->h_min;
+  a.h_max = b->h_max;
+  a.max_steps = b->max_steps;
+  a.seed = b->seed;
+  a.path_base = b->path_base;
+  a.noise_steps = b->noise_steps;
+  a.noise_constant = b->noise_constant;
+  a.path_stride = b->path_stride;
+  int64_t n_rec = b->path_stride > 0 ? b->max_steps / b->path_stride + 1 : 0;
+  if (!(b->flags & DE_FLAG_HOST_MEMORY)) {
+    a.params = b->params;
+    a.y0 = b->y0;
+    a.noise_table = b->noise_table;
+    a.y_final = o->y_final;
+    a.t_final = o->t_final;
+    a.steps = o->steps;
+    a.evals = o->evals;
+    a.status = o->status;
+    a.path = o->path;
+    a.path_t = o->path_t;
+    int r = de::parity::launch_sde(b->family, b->scheme, a, s);
+    if (r == 1) return fail(DE_ERR_UNSUPPORTED, "SDE kernel not built");
+    if (r < 0) return hip_fail(hipGetLastError(), "SDE launch");
+    return DE_OK;
+  }
+  Staging st;
+  a.params = st.in(b->params, (size_t)P * n);
+  a.y0 = st.in(b->y0, (size_t)b->dim * n);
+  a.noise_table = st.in(b->noise_table, b->noise == DE_NOISE_TABLE ? (size_t)b->noise_steps * b->dim : 0);
+  a.y_final = st.out(o->y_final, (size_t)b->dim * n);
+  a.t_final = st.out(o->t_final, n);
+  a.steps = st.out(o->steps, n);
+  a.evals = st.out(o->evals, n);
+  a.status = st.out(o->status, n);
+  a.path = st.out(o->path, (size_t)(n_rec * b->dim * n));
+  a.path_t = st.out(o->path_t, (size_t)n_rec);
+  if (st.err != hipSuccess) return hip_fail(st.err, "staging inputs");
+  if (a.path) {
+    int rc = fill_nan(a.path, n_rec * b->dim * n, s);
+    if (rc != DE_OK) return rc;
+  }
+  if (a.path_t) {
+    int rc = fill_nan(a.path_t, n_rec, s);
+    if (rc != DE_OK) return rc;
+  }
+  int r = de::parity::launch_sde(b->family, b->scheme, a, s);
+  if (r == 1) return fail(DE_ERR_UNSUPPORTED, "SDE kernel not built");
+  if (r < 0) return hip_fail(hipGetLastError(), "SDE launch");
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "SDE kernel execution");
+  st.back(o->y_final, a.y_final, (size_t)b->dim * n);
+  st.back(o->t_final, a.t_final, n);
+  st.back(o->steps, a.steps, n);
+  st.back(o->evals, a.evals, n);
+  st.back(o->status, a.status, n);
+  st.back(o->path, a.path, (size_t)(n_rec * b->dim * n));
+  st.back(o->path_t, a.path_t, (size_t)n_rec);
+  if (st.err != hipSuccess) return hip_fail(st.err, "copying results");
+  return DE_OK;
+}
+
+// ------------------------------------------------------------- workloads ---
+void de_gen_lorenz(uint64_t seed, int64_t n, double* params, double* y0) {
+  for (int64_t i = 0; i < n; ++i) {
+    params[i] = 10.0;
+    params[n + i] = 50.0 * de::input_uniform(seed, (uint64_t)i, 0, 0);
+    params[2 * n + i] = 8.0 / 3.0;
+    double z0, z1, z2, z3;
+    de::input_normal_pair(seed, (uint64_t)i, 1, 0, &z0, &z1);
+    de::input_normal_pair(seed, (uint64_t)i, 1, 1, &z2, &z3);
+    y0[i] = 1.0 + 0.1 * z0;
+    y0[n + i] = 1.0 + 0.1 * z1;
+    y0[2 * n + i] = 1.0 + 0.1 * z2;
+  }
+}
+
+void de_gen_vdp(uint64_t seed, int64_t n, double* params, double* y0) {
+  for (int64_t i = 0; i < n; ++i) {
+    double u = de::input_uniform(seed, (uint64_t)i, 2, 0);
+    params[i] = exp(log(0.1) + u * log(100.0));
+    double z0, z1;
+    de::input_normal_pair(seed, (uint64_t)i, 3, 0, &z0, &z1);
+    y0[i] = 2.0 + 0.1 * z0;
+    y0[n + i] = 0.1 * z1;
+  }
+}
+
+void de_gen_spring_sizes(uint64_t seed, int64_t n, int32_t* sizes) {
+  for (int64_t i = 0; i < n; ++i) {
+    double u = de::input_uniform(seed, (uint64_t)i, 4, 0);
+    int32_t m = (int32_t)floor(exp(log(4.0) + u * (log(257.0) - log(4.0))));
+    sizes[i] = m < 4 ? 4 : (m > 256 ? 256 : m);
+  }
+}
+
+void de_spring_offsets(int64_t n, const int32_t* sizes, int64_t* so, int64_t* po) {
+  so[0] = 0;
+  po[0] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    so[i + 1] = so[i] + 2 * (int64_t)sizes[i];
+    po[i + 1] = po[i] + (int64_t)sizes[i] + 2;
+  }
+}
+
+void de_gen_spring(uint64_t seed, int64_t n, const int32_t* sizes, const int64_t* so,
+                   const int64_t* po, double* params, double* y0) {
+  for (int64_t i = 0; i < n; ++i) {
+    const int m = sizes[i];
+    double* p = params + po[i];
+    p[0] = 0.1 * de::input_uniform(seed, (uint64_t)i, 5, 0);
+    for (int j = 0; j <= m; j += 2) {
+      double z0, z1;
+      de::input_normal_pair(seed, (uint64_t)i, 6, (uint32_t)(j >> 1), &z0, &z1);
+      p[1 + j] = exp(0.5 * z0);
+      if (j + 1 <= m) p[2 + j] = exp(0.5 * z1);
+    }
+    double* y = y0 + so[i];
+    for (int j = 0; j < m; j += 2) {
+      double z0, z1;
+      de::input_normal_pair(seed, (uint64_t)i, 7, (uint32_t)(j >> 1), &z0, &z1);
+      y[j] = z0;
+      if (j + 1 < m) y[j + 1] = z1;
+    }
+    for (int j = 0; j < m; ++j) y[m + j] = 0.0;
+  }
+}
+
+void de_gen_ou(uint64_t seed, int64_t n, int32_t dim, double* params, double* y0) {
+  for (int64_t i = 0; i < n; ++i) {
+    for (int c = 0; c < dim; ++c) {
+      de::u32x4 w = de::input_block(seed, (uint64_t)i, 8, (uint32_t)c);
+      double u_theta = de::uniform_co(w.v[0], w.v[1]);
+      double u_sigma = de::uniform_co(w.v[2], w.v[3]);
+      double z0, z1;
+      de::input_normal_pair(seed, (uint64_t)i, 9, (uint32_t)c, &z0, &z1);
+      params[(int64_t)(3 * c + 0) * n + i] = 0.5 + 1.5 * u_theta;
+      params[(int64_t)(3 * c + 1) * n + i] = z0;
+      params[(int64_t)(3 * c + 2) * n + i] = 0.1 + 0.9 * u_sigma;
+      y0[(int64_t)c * n + i] = 0.0;
+    }
+  }
+}
+
+void de_gen_gbm(int64_t n, double mu, double sigma, double s0, double* params, double* y0) {
+  for (int64_t i = 0; i < n; ++i) {
+    params[i] = mu;
+    params[n + i] = sigma;
+    y0[i] = s0;
+  }
+}
+
+void de_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+  de::u32x4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  de::u32x4 w = de::philox4x32_10(c, k0, k1);
+  for (int i = 0; i < 4; ++i) out[i] = w.v[i];
+}
+
+double de_noise_normal(uint64_t seed, uint64_t path, uint32_t step, uint32_t comp) {
+  return de::noise_normal(seed, path, step, comp);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------- self-checks ----
+namespace {
+
+__global__ void diag_pow_kernel(const double* x, const double* y, double* out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = de::pow(x[i], y[i]);
+}
+
+__global__ void diag_arith_kernel(const double* a, const double* b, const double* c,
+                                  double* out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    out[3 * i + 0] = a[i] / b[i];
+    out[3 * i + 1] = sqrt(a[i]);
+    out[3 * i + 2] = fma(a[i], b[i], c[i]);
+  }
+}
+
+__global__ void diag_noise_kernel(uint64_t seed, const uint64_t* path, const uint32_t* step,
+                                  const uint32_t* comp, double* out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = de::noise_normal(seed, path[i], step[i], comp[i]);
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" {
+
+int de_diag_pow(const double* x, const double* y, double* out, int64_t n) {
+  if (n <= 0) return DE_OK;
+  Staging st;
+  const double* dx = st.in(x, n);
+  const double* dy = st.in(y, n);
+  double* dout = st.out(out, n);
+  if (st.err != hipSuccess) return hip_fail(st.err, "staging");
+  hipLaunchKernelGGL(diag_pow_kernel, dim3(grid_for(n)), dim3(256), 0, 0, dx, dy, dout, n);
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "diag_pow");
+  st.back(out, dout, n);
+  return st.err == hipSuccess ? DE_OK : hip_fail(st.err, "copy back");
+}
+
+int de_diag_arith(const double* a, const double* b, const double* c, double* out, int64_t n) {
+  if (n <= 0) return DE_OK;
+  Staging st;
+  const double* da = st.in(a, n);
+  const double* db = st.in(b, n);
+  const double* dc = st.in(c, n);
+  double* dout = st.out(out, 3 * n);
+  if (st.err != hipSuccess) return hip_fail(st.err, "staging");
+  hipLaunchKernelGGL(diag_arith_kernel, dim3(grid_for(n)), dim3(256), 0, 0, da, db, dc, dout, n);
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "diag_arith");
+  st.back(out, dout, 3 * n);
+  return st.err == hipSuccess ? DE_OK : hip_fail(st.err, "copy back");
+}
+
+int de_diag_noise(uint64_t seed, const uint64_t* path, const uint32_t* step,
+                  const uint32_t* comp, double* out, int64_t n) {
+  if (n <= 0) return DE_OK;
+  Staging st;
+  const uint64_t* dp = st.in(path, n);
+  const uint32_t* ds = st.in(step, n);
+  const uint32_t* dc = st.in(comp, n);
+  double* dout = st.out(out, n);
+  if (st.err != hipSuccess) return hip_fail(st.err, "staging");
+  hipLaunchKernelGGL(diag_noise_kernel, dim3(grid_for(n)), dim3(256), 0, 0, seed, dp, ds, dc,
+                     dout, n);
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "diag_noise");
+  st.back(out, dout, n);
+  return st.err == hipSuccess ? DE_OK : hip_fail(st.err, "copy back");
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// Shared host/device helpers for the ensemble Runge-Kutta integrator.
+//
+// This header is compiled three ways: by hipcc for gfx950 device code, by hipcc
+// for the host side of the library, and by g++ for CPU-side consumers.  Every
+// floating-point expression in the integrator must produce the same bits in all
+// three, so:
+//   * all translation units are built with -ffp-contract=off (no implicit FMA);
+//   * explicit fma() is used only where an exactly rounded fused operation is
+//     intended (double-double arithmetic), never as an optimisation;
+//   * max/min use fmax/fmin (IEEE maxNum/minNum: a NaN operand yields the other
+//     operand), which is the semantics of Rust's f64::max/f64::min that the
+//     reference solver uses (src/methods/erk/dormandprince/ordinary.rs:157).
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define DE_HD __host__ __device__ inline
+#define DE_DEV __device__ inline
+#else
+#define DE_HD inline
+#define DE_DEV inline
+#endif
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DE_TABLE __device__ __constant__ static const
+#else
+#define DE_TABLE static const
+#endif
+
+namespace de {
+
+// f64::EPSILON * 10, the end-of-interval threshold of the solve drivers
+// (src/ode/solve_ivp.rs:198,263; src/sde/solve_ivp.rs:217,273).
+static constexpr double kTenEps = 2.220446049250313080847e-16 * 10.0;
+
+DE_HD uint64_t as_u64(double x) {
+  union { double d; uint64_t u; } v;
+  v.d = x;
+  return v.u;
+}
+
+DE_HD double as_f64(uint64_t u) {
+  union { double d; uint64_t u; } v;
+  v.u = u;
+  return v.d;
+}
+
+// Rust f64::signum: 1.0 for +0 and positive values, -1.0 for -0 and negative
+// values, NaN for NaN.
+DE_HD double signum(double x) {
+  if (x != x) return x;
+  return (as_u64(x) >> 63) ? -1.0 : 1.0;
+}
+
+// constrain_step_size (src/utils.rs:22-33): clamp |h| to [h_min, h_max] keeping
+// the sign of h.
+DE_HD double constrain_step_size(double h, double h_min, double h_max) {
+  double sign = signum(h);
+  double a = fabs(h);
+  if (a < h_min) return sign * h_min;
+  if (a > h_max) return sign * h_max;
+  return h;
+}
+
+// Per-trajectory status codes.  0..4 mirror Status::Complete and the Error
+// variants a solve can end with (src/error.rs:13-41); kRunning is internal.
+enum Status : int32_t {
+  kComplete = 0,
+  kMaxSteps = 1,
+  kStepSize = 2,
+  kStiffness = 3,
+  kBadInput = 4,
+  kRunning = -1,
+};
+
+// validate_step_size_parameters (src/utils.rs:60-157).  Returns true when the
+// parameters are valid; false corresponds to Err(Error::BadInput).
+DE_HD bool validate_step_size(double h0, double h_min, double h_max, double t0,
+                              double tf) {
+  if (tf == t0) return false;
+  double sign = signum(tf - t0);
+  if (signum(h0) != sign) return false;
+  if (h_min < 0.0) return false;
+  if (h_max < 0.0) return false;
+  if (h_min > h_max) return false;
+  if (fabs(h0) < h_min) return false;
+  if (fabs(h0) > h_max) return false;
+  if (fabs(h0) > fabs(tf - t0)) return false;
+  if (h0 == 0.0) return false;
+  return true;
+}
+
+}  // namespace de

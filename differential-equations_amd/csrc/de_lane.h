@@ -1,0 +1,475 @@
+// One-lane-per-trajectory solvers for fixed-dimension families (device code).
+//
+// Each lane runs the reference's solve loop for one trajectory with its whole
+// state in registers:
+//   solve_ode driver         src/ode/solve_ivp.rs:116-277
+//   DOPRI5 init / step       src/methods/erk/dormandprince/ordinary.rs:16-270
+//   DOPRI5 dense output      src/methods/erk/dormandprince/ordinary.rs:196-235,298-336
+//   h_init                   src/methods/h_init.rs:45-135
+//   fixed ERK (rk4, euler)   src/methods/erk/fixed/ordinary.rs:16-138, Hermite
+//                            output src/interpolate.rs:40-60
+//   TEvalSolout              src/solout/t_eval.rs:83-134
+// Operation order is the reference's, with one difference: terms whose tableau
+// coefficient is exactly zero (a[6][1], b[1], b[6], er[1], all of bi[4]) and
+// the leading `0 +` of combinations that start from zeros_like() are omitted.
+// For finite stages those terms add a signed zero, which can only change the
+// sign of an exactly-zero component; a non-finite stage makes the attempt's
+// error non-finite either way, so the attempt is rejected either way.  The
+// values produced are therefore the reference's.
+//
+// The caller provides the RHS family F (de_families.h) and the launch
+// arguments; the same code runs under the plain and the persistent schedules.
+#pragma once
+
+#include "de_common.h"
+#include "de_families.h"
+#include "de_pow.h"
+
+// Controller pow: the bit-exact project pow, or the device libm pow in the
+// FMA-contracted fast build (whose results are not bit-matched anyway).
+#if defined(DE_FAST_BUILD)
+#define DE_CTRL_POW(x, y) ::pow((x), (y))
+#else
+#define DE_CTRL_POW(x, y) ::de::pow((x), (y))
+#endif
+
+namespace de {
+
+struct OdeArgs {
+  int64_t n;
+  const double* params;
+  const double* y0;
+  const double* t0;
+  const double* tf;
+  double t0s, tfs;
+  const double* rtol_traj;
+  const double* atol_traj;
+  double rtol, atol;
+  double h0, h_min, h_max, safety, min_scale, max_scale;
+  int64_t max_steps;
+  const double* t_eval;
+  int32_t n_eval;
+  double* y_final;
+  double* t_final;
+  int64_t* accepted;
+  int64_t* rejected;
+  int64_t* evals;
+  int32_t* status;
+  double* dense;
+  int32_t* n_dense;
+  unsigned long long* counter;  // persistent schedule: next trajectory index
+};
+
+// DOPRI5 coefficients (src/tableau/dorman_prince.rs:37-117).
+namespace dp {
+constexpr double c1 = 0.2, c2 = 0.3, c3 = 0.8, c4 = 8.0 / 9.0;
+constexpr double a10 = 0.2;
+constexpr double a20 = 3.0 / 40.0, a21 = 9.0 / 40.0;
+constexpr double a30 = 44.0 / 45.0, a31 = -56.0 / 15.0, a32 = 32.0 / 9.0;
+constexpr double a40 = 19372.0 / 6561.0, a41 = -25360.0 / 2187.0, a42 = 64448.0 / 6561.0,
+                 a43 = -212.0 / 729.0;
+constexpr double a50 = 9017.0 / 3168.0, a51 = -355.0 / 33.0, a52 = 46732.0 / 5247.0,
+                 a53 = 49.0 / 176.0, a54 = -5103.0 / 18656.0;
+constexpr double a60 = 35.0 / 384.0, a62 = 500.0 / 1113.0, a63 = 125.0 / 192.0,
+                 a64 = -2187.0 / 6784.0, a65 = 11.0 / 84.0;
+constexpr double b0 = 35.0 / 384.0, b2 = 500.0 / 1113.0, b3 = 125.0 / 192.0,
+                 b4 = -2187.0 / 6784.0, b5 = 11.0 / 84.0;
+constexpr double e0 = 71.0 / 57600.0, e2 = -71.0 / 16695.0, e3 = 71.0 / 1920.0,
+                 e4 = -17253.0 / 339200.0, e5 = 22.0 / 525.0, e6 = -1.0 / 40.0;
+}  // namespace dp
+
+// TEvalSolout over [t_prev, t_curr]; interp(te, out[D]) evaluates the step's
+// interpolant.  Writes dense[(e * D + c) * n + i].
+template <int D, class Interp>
+DE_HD void teval_emit(const OdeArgs& a, int64_t i, double dir, int32_t& next, int32_t& pushed,
+                      double t_curr, double t_prev, const double* y_curr, Interp&& interp) {
+  int32_t idx = next;
+  while (idx < a.n_eval) {
+    double te = a.t_eval[idx];
+    bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
+                              : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
+    if (in_range) {
+      double out[D];
+      if (te == t_curr) {
+        for (int c = 0; c < D; ++c) out[c] = y_curr[c];
+      } else {
+        interp(te, out);
+      }
+      for (int c = 0; c < D; ++c) a.dense[((int64_t)idx * D + c) * a.n + i] = out[c];
+      pushed += 1;
+      idx += 1;
+    } else {
+      if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+      idx += 1;
+    }
+  }
+  next = idx;
+}
+
+// InitialStepSize::<Ordinary>::compute (src/methods/h_init.rs:45-135).
+template <class F>
+DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, double rtol,
+                         double atol, double h_min, double h_max) {
+  constexpr int D = F::kDim;
+  double posneg = signum(tf - t0);
+  double f0[D], f1[D], sk[D], y1[D];
+  f.diff(t0, y0, f0);
+  double dnf = 0.0, dny = 0.0;
+  for (int c = 0; c < D; ++c) {
+    sk[c] = atol + rtol * fabs(y0[c]);
+    double u = f0[c] / sk[c];
+    dnf += u * u;
+    double v = y0[c] / sk[c];
+    dny += v * v;
+  }
+  double h = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+  h = fmin(h, h_max);
+  h *= posneg;
+  for (int c = 0; c < D; ++c) y1[c] = y0[c] + h * f0[c];
+  f.diff(t0 + h, y1, f1);
+  double der2 = 0.0;
+  for (int c = 0; c < D; ++c) {
+    double d = (f1[c] - f0[c]) / sk[c];
+    der2 += d * d;
+  }
+  der2 = sqrt(der2) / fabs(h);
+  double der12 = fmax(sqrt(dnf), der2);
+  double h1 = der12 <= 1.0e-15 ? fabs(h) * 1.0e-3 : DE_CTRL_POW(0.01 / der12, 1.0 / 5.0);
+  double interval = fabs(tf - t0);
+  h = fmin(fmax(fmin(fmin(fabs(h) * 100.0, h1), h_max), h_min), interval);
+  return h * posneg;
+}
+
+// ------------------------------------------------------------------ DOPRI5 --
+template <class F>
+struct Dopri5Lane {
+  static constexpr int D = F::kDim;
+  F f;
+  double y[D], k0[D];
+  double t, h, h_prev, tf, dir, rtol, atol;
+  int64_t steps, acc, rej, ev;
+  int32_t stiff, nonstiff, countdown, next_eval, pushed;
+  bool rejected_last;
+
+  DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const {
+    for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = y[c];
+    a.t_final[i] = t;
+    a.accepted[i] = acc;
+    a.rejected[i] = rej;
+    a.evals[i] = ev;
+    a.status[i] = st;
+    if (a.n_dense) a.n_dense[i] = pushed;
+  }
+
+  // Driver prologue + DOPRI5 init.  Returns false when the trajectory ended
+  // (BadInput), after writing its results.
+  DE_HD bool init(const OdeArgs& a, int64_t i) {
+    f.load(a.params, a.n, i);
+    double t0 = a.t0 ? a.t0[i] : a.t0s;
+    tf = a.tf ? a.tf[i] : a.tfs;
+    rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
+    atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
+    t = t0;
+    acc = rej = ev = 0;
+    pushed = 0;
+    next_eval = 0;
+    double s = signum(tf - t0);
+    dir = s;
+    bool ok = (s == 1.0 || s == -1.0);
+    double h0 = a.h0;
+    if (ok && h0 == 0.0) {
+      h0 = lane_h_init(f, t0, tf, y, rtol, atol, a.h_min, a.h_max);
+      ev += 2;
+    }
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      ev = 0;
+      finish(a, i, kBadInput);
+      return false;
+    }
+    h = h0;
+    f.diff(t, y, k0);
+    ev += 1;
+    steps = 0;
+    h_prev = 0.0;
+    stiff = 0;
+    nonstiff = 0;
+    countdown = 100;
+    rejected_last = false;
+    if (a.n_eval > 0) {
+      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y, [](double, double*) {});
+    }
+    return true;
+  }
+
+  // One pass of the driver loop: overshoot clip, one DOPRI5 attempt, solout,
+  // end test.  Returns true when the trajectory ended (results written).
+  DE_HD bool advance(const OdeArgs& a, int64_t i) {
+    if ((t + h - tf) * dir > 0.0) {
+      double hn = tf - t;
+      if (fabs(hn) < kTenEps) {
+        finish(a, i, kComplete);
+        return true;
+      }
+      h = hn;
+    }
+    if (fabs(h) < fabs(h_prev) * 1e-14) {
+      finish(a, i, kStepSize);
+      return true;
+    }
+    if (steps >= a.max_steps) {
+      finish(a, i, kMaxSteps);
+      return true;
+    }
+    steps += 1;
+    countdown -= 1;
+
+    double k1[D], k2[D], k3[D], k4[D], k5[D], k6[D], ys[D];
+    const double a10h = dp::a10 * h;
+    for (int c = 0; c < D; ++c) ys[c] = y[c] + a10h * k0[c];
+    f.diff(t + dp::c1 * h, ys, k1);
+    const double a20h = dp::a20 * h, a21h = dp::a21 * h;
+    for (int c = 0; c < D; ++c) ys[c] = (y[c] + a20h * k0[c]) + a21h * k1[c];
+    f.diff(t + dp::c2 * h, ys, k2);
+    const double a30h = dp::a30 * h, a31h = dp::a31 * h, a32h = dp::a32 * h;
+    for (int c = 0; c < D; ++c) ys[c] = ((y[c] + a30h * k0[c]) + a31h * k1[c]) + a32h * k2[c];
+    f.diff(t + dp::c3 * h, ys, k3);
+    const double a40h = dp::a40 * h, a41h = dp::a41 * h, a42h = dp::a42 * h,
+                 a43h = dp::a43 * h;
+    for (int c = 0; c < D; ++c)
+      ys[c] = (((y[c] + a40h * k0[c]) + a41h * k1[c]) + a42h * k2[c]) + a43h * k3[c];
+    f.diff(t + dp::c4 * h, ys, k4);
+    const double a50h = dp::a50 * h, a51h = dp::a51 * h, a52h = dp::a52 * h,
+                 a53h = dp::a53 * h, a54h = dp::a54 * h;
+    for (int c = 0; c < D; ++c)
+      ys[c] = ((((y[c] + a50h * k0[c]) + a51h * k1[c]) + a52h * k2[c]) + a53h * k3[c]) +
+              a54h * k4[c];
+    f.diff(t + 1.0 * h, ys, k5);
+    const double a60h = dp::a60 * h, a62h = dp::a62 * h, a63h = dp::a63 * h,
+                 a64h = dp::a64 * h, a65h = dp::a65 * h;
+    for (int c = 0; c < D; ++c)
+      ys[c] = ((((y[c] + a60h * k0[c]) + a62h * k2[c]) + a63h * k3[c]) + a64h * k4[c]) +
+              a65h * k5[c];
+    f.diff(t + 1.0 * h, ys, k6);
+
+    double yseg[D], ynew[D];
+    double err = 0.0;
+    for (int c = 0; c < D; ++c) {
+      yseg[c] = (((dp::b0 * k0[c] + dp::b2 * k2[c]) + dp::b3 * k3[c]) + dp::b4 * k4[c]) +
+                dp::b5 * k5[c];
+      ynew[c] = y[c] + h * yseg[c];
+      double es = ((((dp::e0 * k0[c] + dp::e2 * k2[c]) + dp::e3 * k3[c]) + dp::e4 * k4[c]) +
+                   dp::e5 * k5[c]) +
+                  dp::e6 * k6[c];
+      double sk = atol + rtol * fmax(fabs(y[c]), fabs(ynew[c]));
+      double e = es / sk;
+      err = c == 0 ? e * e : err + e * e;
+    }
+    double deno = err;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * (double)D));
+    double scale = a.safety * DE_CTRL_POW(err, -0.2);
+    scale = fmin(fmax(scale, a.min_scale), a.max_scale);
+
+    if (err <= 1.0) {
+      double t_new = t + h;
+      double dn[D];
+      f.diff(t_new, ynew, dn);
+      if (countdown == 0) {
+        countdown = 100;
+        double stdnum = 0.0, stden = 0.0;
+        for (int c = 0; c < D; ++c) {
+          double u = yseg[c] - k6[c];
+          stdnum = c == 0 ? u * u : stdnum + u * u;
+          double v = dn[c] - ys[c];
+          stden = c == 0 ? v * v : stden + v * v;
+        }
+        if (stden > 0.0) {
+          double h_lamb = h * sqrt(stdnum / stden);
+          if (h_lamb > 6.1) {
+            nonstiff = 0;
+            stiff += 1;
+            if (stiff == 15) {
+              finish(a, i, kStiffness);
+              return true;
+            }
+          }
+        } else {
+          nonstiff += 1;
+          if (nonstiff == 6) stiff = 0;
+        }
+      }
+      if (a.n_eval > 0) {
+        // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
+        // ordinary.rs:312-333.
+        double cont1[D], cont2[D], cont3[D];
+        for (int c = 0; c < D; ++c) {
+          double ydiff = ynew[c] + (-1.0) * y[c];
+          cont1[c] = ydiff;
+          double bspl = h * k0[c] + (-1.0) * ydiff;
+          cont2[c] = bspl;
+          cont3[c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
+        }
+        const double hp = h, tp = t;
+        const double cont4 = 0.0 * h;
+        const double* y0p = y;
+        teval_emit<D>(a, i, dir, next_eval, pushed, t_new, t, ynew, [&](double te, double* out) {
+          double s = (te - tp) / hp;
+          double s1 = 1.0 - s;
+          for (int c = 0; c < D; ++c) {
+            double acc_ = cont4 * s1 + cont3[c];
+            acc_ = acc_ * s + cont2[c];
+            acc_ = acc_ * s1 + cont1[c];
+            out[c] = y0p[c] + s * acc_;
+          }
+        });
+      }
+      h_prev = h;
+      t = t_new;
+      for (int c = 0; c < D; ++c) {
+        y[c] = ynew[c];
+        k0[c] = dn[c];
+      }
+      if (rejected_last) {
+        rejected_last = false;
+        scale = fmin(scale, 1.0);
+      }
+      acc += 1;
+      ev += 7;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+      if (fabs(tf - t) <= kTenEps) {
+        finish(a, i, kComplete);
+        return true;
+      }
+    } else {
+      rejected_last = true;
+      rej += 1;
+      ev += 6;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    }
+    return false;
+  }
+};
+
+// --------------------------------------------------- fixed-step rk4 / euler --
+// RK4: c = (0, 1/2, 1/2, 1), a10 = a21 = 1/2, a32 = 1, b = (1/6, 1/3, 1/3, 1/6)
+// (src/tableau/runge_kutta.rs:39-70); euler: b = (1) (:289-302).
+template <class F, int STAGES>
+struct FixedLane {
+  static constexpr int D = F::kDim;
+  F f;
+  double y[D], k0[D];
+  double t, h, tf, dir;
+  int64_t steps, acc, ev;
+  int32_t next_eval, pushed;
+
+  DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const {
+    for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = y[c];
+    a.t_final[i] = t;
+    a.accepted[i] = acc;
+    a.rejected[i] = 0;
+    a.evals[i] = ev;
+    a.status[i] = st;
+    if (a.n_dense) a.n_dense[i] = pushed;
+  }
+
+  DE_HD bool init(const OdeArgs& a, int64_t i) {
+    f.load(a.params, a.n, i);
+    double t0 = a.t0 ? a.t0[i] : a.t0s;
+    tf = a.tf ? a.tf[i] : a.tfs;
+    for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
+    t = t0;
+    acc = ev = 0;
+    pushed = 0;
+    next_eval = 0;
+    double s = signum(tf - t0);
+    dir = s;
+    bool ok = (s == 1.0 || s == -1.0);
+    double h0 = a.h0;
+    if (h0 == 0.0) h0 = fabs(tf - t0) / 100.0;
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      finish(a, i, kBadInput);
+      return false;
+    }
+    h = h0;
+    f.diff(t, y, k0);
+    ev = 1;
+    steps = 0;
+    if (a.n_eval > 0) teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y, [](double, double*) {});
+    return true;
+  }
+
+  DE_HD bool advance(const OdeArgs& a, int64_t i) {
+    if ((t + h - tf) * dir > 0.0) {
+      double hn = tf - t;
+      if (fabs(hn) < kTenEps) {
+        finish(a, i, kComplete);
+        return true;
+      }
+      h = hn;
+    }
+    if (steps >= a.max_steps) {
+      finish(a, i, kMaxSteps);
+      return true;
+    }
+    steps += 1;
+    double ynext[D];
+    if (STAGES == 4) {
+      double k1[D], k2[D], k3[D], ys[D];
+      const double hh = 0.5 * h;
+      for (int c = 0; c < D; ++c) ys[c] = y[c] + hh * k0[c];
+      f.diff(t + 0.5 * h, ys, k1);
+      for (int c = 0; c < D; ++c) ys[c] = y[c] + hh * k1[c];
+      f.diff(t + 0.5 * h, ys, k2);
+      const double h1 = 1.0 * h;
+      for (int c = 0; c < D; ++c) ys[c] = y[c] + h1 * k2[c];
+      f.diff(t + 1.0 * h, ys, k3);
+      const double b0h = (1.0 / 6.0) * h, b1h = (1.0 / 3.0) * h;
+      for (int c = 0; c < D; ++c)
+        ynext[c] = (((y[c] + b0h * k0[c]) + b1h * k1[c]) + b1h * k2[c]) + b0h * k3[c];
+      ev += 3;
+    } else {
+      const double b0h = 1.0 * h;
+      for (int c = 0; c < D; ++c) ynext[c] = y[c] + b0h * k0[c];
+    }
+    double tp = t, hp = h;
+    t += h;
+    double dn[D];
+    f.diff(t, ynext, dn);
+    ev += 1;
+    acc += 1;
+    if (a.n_eval > 0) {
+      const double* yp = y;
+      const double* kp = k0;
+      const double tc = t;
+      teval_emit<D>(a, i, dir, next_eval, pushed, t, tp, ynext, [&](double te, double* out) {
+        // cubic_hermite_interpolate(t_prev, t, y_prev, y, dydt_prev, dydt, te)
+        double hh = tc - tp;
+        double s = (te - tp) / hh;
+        double s2 = s * s;
+        double s3 = s2 * s;
+        double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+        double h10 = s3 - 2.0 * s2 + s;
+        double h01 = -2.0 * s3 + 3.0 * s2;
+        double h11 = s3 - s2;
+        double w10 = h10 * hh, w11 = h11 * hh;
+        for (int c = 0; c < D; ++c)
+          out[c] = ((h00 * yp[c] + w10 * kp[c]) + h01 * ynext[c]) + w11 * dn[c];
+      });
+    }
+    (void)hp;
+    for (int c = 0; c < D; ++c) {
+      y[c] = ynext[c];
+      k0[c] = dn[c];
+    }
+    if (fabs(tf - t) <= kTenEps) {
+      finish(a, i, kComplete);
+      return true;
+    }
+    return false;
+  }
+};
+
+}  // namespace de

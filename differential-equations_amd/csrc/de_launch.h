@@ -1,0 +1,45 @@
+// Host-side launch entry points of the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "de_lane.h"
+
+namespace de {
+
+enum : int { kSchedAuto = 0, kSchedPlain = 1, kSchedPersistent = 2 };
+enum : int { kMethodDopri5 = 0, kMethodRk4 = 1, kMethodEuler = 2 };
+enum : int { kSchemeEulerMaruyama = 0, kSchemeMilstein = 1 };
+enum : int { kNoisePhilox = 0, kNoiseTable = 1, kNoiseConstant = 2 };
+
+struct SdeArgs {
+  int64_t n;
+  int32_t dim;
+  int32_t noise;
+  const double* params;
+  const double* y0;
+  double t0, tf, h0, h_min, h_max;
+  int64_t max_steps;
+  uint64_t seed, path_base;
+  const double* noise_table;
+  int64_t noise_steps;
+  double noise_constant;
+  int32_t path_stride;
+  double* y_final;
+  double* t_final;
+  int64_t* steps;
+  int64_t* evals;
+  int32_t* status;
+  double* path;
+  double* path_t;
+};
+
+namespace parity {
+int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+int launch_sde(int family, int scheme, const SdeArgs& a, hipStream_t s);
+}  // namespace parity
+namespace fast {
+int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+}  // namespace fast
+
+}  // namespace de

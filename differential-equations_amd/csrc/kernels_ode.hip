@@ -1,0 +1,137 @@
+// Lane-per-trajectory ODE kernels for fixed-dimension families.
+//
+// Compiled twice: with -ffp-contract=off into namespace de::parity (results
+// bit-identical to the CPU restatement of the reference) and, with
+// -DDE_FAST_BUILD -ffp-contract=fast, into namespace de::fast (FMA-contracted;
+// same algorithm, different rounding).
+//
+// Two schedules:
+//   plain      one thread per trajectory; a wave lives until its slowest lane
+//              finishes.
+//   persistent a grid sized to the resident wave slots; each lane integrates a
+//              trajectory and, when it finishes, takes the next index from a
+//              global counter (one atomic per wave, ballot-aggregated), so
+//              lanes stay busy while step counts vary ~10x across trajectories.
+#include <hip/hip_runtime.h>
+
+#include "de_lane.h"
+#include "de_launch.h"
+
+#if defined(DE_FAST_BUILD)
+#define DE_KNS fast
+#else
+#define DE_KNS parity
+#endif
+
+namespace de {
+namespace DE_KNS {
+
+constexpr int kBlock = 256;
+
+template <class Lane>
+__global__ __launch_bounds__(kBlock) void ode_plain_kernel(OdeArgs a) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  Lane L;
+  if (!L.init(a, i)) return;
+  while (!L.advance(a, i)) {
+  }
+}
+
+template <class Lane>
+__global__ __launch_bounds__(kBlock) void ode_persistent_kernel(OdeArgs a) {
+  Lane L;
+  bool have = false;
+  bool drained = false;
+  int64_t idx = 0;
+  const int lane = __lane_id();
+  const unsigned long long n = (unsigned long long)a.n;
+  for (;;) {
+    if (!drained) {
+      bool need = !have;
+      unsigned long long mask = __ballot(need);
+      if (mask) {
+        int leader = __ffsll((long long)mask) - 1;
+        int cnt = __popcll(mask);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)cnt);
+        base = __shfl(base, leader);
+        if (base + (unsigned long long)cnt >= n) drained = true;
+        if (need) {
+          unsigned long long my = base + (unsigned long long)__popcll(mask & __lanemask_lt());
+          if (my < n) {
+            idx = (int64_t)my;
+            have = L.init(a, idx);
+          }
+        }
+      }
+    }
+    if (!__any(have)) {
+      if (drained) break;
+      continue;
+    }
+    if (have) {
+      if (L.advance(a, idx)) have = false;
+    }
+  }
+}
+
+__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ull; }
+
+template <class Lane>
+static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
+  if (a.n <= 0) return 0;
+  if (schedule == kSchedPlain) {
+    int64_t grid = (a.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(ode_plain_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
+  static int blocks_per_cu = -1;
+  static int num_cu = 0;
+  if (blocks_per_cu < 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return -1;
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ode_persistent_kernel<Lane>, kBlock, 0) !=
+        hipSuccess)
+      return -1;
+    blocks_per_cu = b > 0 ? b : 1;
+  }
+  int64_t want = (int64_t)num_cu * blocks_per_cu;
+  int64_t need = (a.n + kBlock - 1) / kBlock;
+  int64_t grid = want < need ? want : need;
+  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
+  hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <class F>
+static int launch_method(int method, const OdeArgs& a, int schedule, hipStream_t s) {
+  switch (method) {
+    case kMethodDopri5: return launch<Dopri5Lane<F>>(a, schedule, s);
+#if !defined(DE_FAST_BUILD)
+    case kMethodRk4: return launch<FixedLane<F, 4>>(a, schedule, s);
+    case kMethodEuler: return launch<FixedLane<F, 1>>(a, schedule, s);
+#endif
+    default: return 1;
+  }
+}
+
+// Returns 0 on success, 1 if the combination is not built, -1 on HIP error.
+int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s) {
+  switch (family) {
+    case kLorenz: return launch_method<LorenzRhs>(method, a, schedule, s);
+    case kVanDerPol: return launch_method<VanDerPolRhs>(method, a, schedule, s);
+    case kExponential: return launch_method<ExponentialRhs>(method, a, schedule, s);
+    case kLinear: return launch_method<LinearRhs>(method, a, schedule, s);
+    case kHarmonic: return launch_method<HarmonicRhs>(method, a, schedule, s);
+    case kLogistic: return launch_method<LogisticRhs>(method, a, schedule, s);
+    case kRobertson: return launch_method<RobertsonRhs>(method, a, schedule, s);
+    default: return 1;
+  }
+}
+
+}  // namespace DE_KNS
+}  // namespace de

@@ -1,0 +1,475 @@
+"""Python host side of the MI355X ensemble integrator.
+
+Mirrors the reference crate's builder so ensemble calls read like the crate's
+own tests (src/ivp.rs:277-317, 632-706, 781-895):
+
+    sol = IVP.ode(Lorenz(sigma, rho, beta), 0.0, 20.0, y0) \\
+             .method(ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10)) \\
+             .solve()
+
+where every argument that is per trajectory in an ensemble is an array over
+trajectories.  `solve()` runs the HIP library (include/de_api.h); there is no
+CPU fallback: a missing library raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import *  # noqa: F401,F403
+
+
+def lib():
+    return _abi.load()
+
+
+# ----------------------------------------------------------------- methods --
+class ExplicitRungeKutta:
+    """ExplicitRungeKutta::{dopri5, rk4, euler} with the builder setters of
+    src/methods/erk/mod.rs:165-228 and the defaults of :115-158."""
+
+    def __init__(self, method, h0=0.0):
+        self.method_id = method
+        self.settings = dict(rtol=1e-6, atol=1e-6, h0=h0, h_min=0.0, h_max=float("inf"),
+                             max_steps=10000, safety_factor=0.9, min_scale=0.2, max_scale=10.0)
+
+    @classmethod
+    def dopri5(cls):
+        return cls(DE_METHOD_DOPRI5)
+
+    @classmethod
+    def rk4(cls, h0):
+        return cls(DE_METHOD_RK4, h0)
+
+    @classmethod
+    def euler(cls, h0):
+        return cls(DE_METHOD_EULER, h0)
+
+    def _set(self, k, v):
+        self.settings[k] = v
+        return self
+
+    def rtol(self, v):
+        return self._set("rtol", v)
+
+    def atol(self, v):
+        return self._set("atol", v)
+
+    def h0(self, v):
+        return self._set("h0", v)
+
+    def h_min(self, v):
+        return self._set("h_min", v)
+
+    def h_max(self, v):
+        return self._set("h_max", v)
+
+    def max_steps(self, v):
+        return self._set("max_steps", int(v))
+
+    def safety_factor(self, v):
+        return self._set("safety_factor", v)
+
+    def min_scale(self, v):
+        return self._set("min_scale", v)
+
+    def max_scale(self, v):
+        return self._set("max_scale", v)
+
+
+class Milstein:
+    """Milstein::new(h) (src/methods/milstein.rs:37-73)."""
+
+    def __init__(self, h0):
+        self.scheme = DE_SCHEME_MILSTEIN
+        self.settings = dict(h0=h0, h_min=0.0, h_max=float("inf"), max_steps=10000)
+
+    def h_min(self, v):
+        self.settings["h_min"] = v
+        return self
+
+    def h_max(self, v):
+        self.settings["h_max"] = v
+        return self
+
+    def max_steps(self, v):
+        self.settings["max_steps"] = int(v)
+        return self
+
+
+class EulerMaruyama(Milstein):
+    """ExplicitRungeKutta::euler(h) applied to an SDE (stochastic.rs)."""
+
+    def __init__(self, h0):
+        super().__init__(h0)
+        self.scheme = DE_SCHEME_EULER_MARUYAMA
+
+
+# ---------------------------------------------------------------- families --
+def _soa(*cols, n=None):
+    cols = [np.broadcast_to(np.asarray(c, dtype=np.float64), (n,)) for c in cols]
+    return np.ascontiguousarray(np.stack(cols)) if cols else np.zeros((0, n))
+
+
+class _Fixed:
+    family = None
+    dim = None
+
+    def __init__(self, *params):
+        self.params = params
+
+    def pack(self, n):
+        return _soa(*self.params, n=n).reshape(-1)
+
+
+class Lorenz(_Fixed):
+    family, dim = DE_FAMILY_LORENZ, 3
+
+
+class VanDerPol(_Fixed):
+    family, dim = DE_FAMILY_VAN_DER_POL, 2
+
+
+class Exponential(_Fixed):
+    family, dim = DE_FAMILY_EXPONENTIAL, 1
+
+
+class Linear(_Fixed):
+    family, dim = DE_FAMILY_LINEAR, 1
+
+
+class Harmonic(_Fixed):
+    family, dim = DE_FAMILY_HARMONIC, 2
+
+
+class Logistic(_Fixed):
+    family, dim = DE_FAMILY_LOGISTIC, 1
+
+
+class Robertson(_Fixed):
+    family, dim = DE_FAMILY_ROBERTSON, 3
+
+    def pack(self, n):
+        return np.zeros(1)
+
+
+class SpringChain:
+    """Ragged damped spring-mass chains: sizes[i] masses, params packed per
+    system as (c, k_0..k_N), state as (x_0..x_{N-1}, v_0..v_{N-1})."""
+    family = DE_FAMILY_SPRING_CHAIN
+    dim = None
+
+    def __init__(self, sizes, params_packed):
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        n = len(self.sizes)
+        self.state_offset = np.zeros(n + 1, dtype=np.int64)
+        self.param_offset = np.zeros(n + 1, dtype=np.int64)
+        self.state_offset[1:] = np.cumsum(2 * self.sizes.astype(np.int64))
+        self.param_offset[1:] = np.cumsum(self.sizes.astype(np.int64) + 2)
+        self.params_packed = np.ascontiguousarray(params_packed, dtype=np.float64)
+
+
+class OrnsteinUhlenbeck:
+    family = DE_SDE_ORNSTEIN_UHLENBECK
+
+    def __init__(self, dim, params):
+        self.dim = dim
+        self.params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+
+
+class LinearDiagonal:
+    family = DE_SDE_LINEAR_DIAGONAL
+
+    def __init__(self, dim, params):
+        self.dim = dim
+        self.params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):  # torch tensor (device memory)
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- problems --
+class OdeEnsemble:
+    """IVP::ode for an ensemble.  Arrays live in host memory (numpy)."""
+
+    def __init__(self, system, t0, tf, y0, n=None):
+        self.system = system
+        self.t0 = t0
+        self.tf = tf
+        if isinstance(system, SpringChain):
+            self.n = len(system.sizes)
+            self.y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1)
+        else:
+            y0 = np.asarray(y0, dtype=np.float64)
+            if y0.ndim == 1 and system.dim == len(y0) and n is not None:
+                y0 = np.repeat(y0[:, None], n, axis=1)
+            elif y0.ndim == 1:
+                y0 = y0.reshape(system.dim, -1)
+            self.n = y0.shape[1]
+            self.y0 = np.ascontiguousarray(y0).reshape(-1)
+        self._method = ExplicitRungeKutta.dopri5()
+        self._t_eval = None
+        self.schedule = DE_SCHED_AUTO
+        self.mapping = DE_MAP_AUTO
+        self.group_size = 0
+        self.flags = 0
+        self.rtol_traj = None
+        self.atol_traj = None
+
+    def method(self, m):
+        self._method = m
+        return self
+
+    def t_eval(self, points):
+        pts = np.sort(np.asarray(points, dtype=np.float64))
+        t0 = float(np.asarray(self.t0).reshape(-1)[0])
+        tf = float(np.asarray(self.tf).reshape(-1)[0])
+        if tf < t0:
+            pts = pts[::-1].copy()
+        self._t_eval = np.ascontiguousarray(pts)
+        return self
+
+    def options(self, schedule=None, mapping=None, group_size=None, fast_math=None):
+        if schedule is not None:
+            self.schedule = schedule
+        if mapping is not None:
+            self.mapping = mapping
+        if group_size is not None:
+            self.group_size = group_size
+        if fast_math is not None:
+            self.flags = (self.flags | DE_FLAG_FAST_MATH) if fast_math else (
+                self.flags & ~DE_FLAG_FAST_MATH)
+        return self
+
+    def batch(self):
+        """The de_ode_batch descriptor (host pointers) plus result buffers."""
+        s = self.system
+        b = ode_defaults()
+        b.family = s.family
+        b.method = self._method.method_id
+        b.n = self.n
+        keep = [self.y0]
+        if isinstance(s, SpringChain):
+            b.dim = 0
+            b.sizes = s.sizes.ctypes.data_as(C.POINTER(C.c_int32))
+            b.state_offset = s.state_offset.ctypes.data_as(C.POINTER(C.c_int64))
+            b.param_offset = s.param_offset.ctypes.data_as(C.POINTER(C.c_int64))
+            params = s.params_packed
+            state_len = int(s.state_offset[-1])
+            keep += [s.sizes, s.state_offset, s.param_offset]
+        else:
+            b.dim = s.dim
+            params = s.pack(self.n)
+            state_len = s.dim * self.n
+        keep.append(params)
+        b.params = _ptr(params)
+        b.y0 = _ptr(self.y0)
+        for name in ("t0", "tf"):
+            v = np.asarray(getattr(self, name), dtype=np.float64)
+            if v.ndim == 0:
+                setattr(b, name + "_scalar", float(v))
+            else:
+                v = np.ascontiguousarray(v)
+                keep.append(v)
+                setattr(b, name, _ptr(v))
+                setattr(b, name + "_scalar", float(v[0]))
+        for k, v in self._method.settings.items():
+            setattr(b, k, v)
+        for name in ("rtol_traj", "atol_traj"):
+            v = getattr(self, name)
+            if v is not None:
+                v = np.ascontiguousarray(v, dtype=np.float64)
+                keep.append(v)
+                setattr(b, name, _ptr(v))
+        n_eval = 0 if self._t_eval is None else len(self._t_eval)
+        b.n_eval = n_eval
+        if n_eval:
+            keep.append(self._t_eval)
+            b.t_eval = _ptr(self._t_eval)
+        b.schedule = self.schedule
+        b.mapping = self.mapping
+        b.group_size = self.group_size
+        b.flags = self.flags | DE_FLAG_HOST_MEMORY
+        res = Solution(self.n, state_len, n_eval)
+        return b, res, keep
+
+    def solve(self, stream=None):
+        b, res, keep = self.batch()
+        _abi.check(lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), stream))
+        return res
+
+
+class Solution:
+    """Per-trajectory results (Solution / Error of each reference solve)."""
+
+    def __init__(self, n, state_len, n_eval):
+        self.n = n
+        self.y = np.full(state_len, np.nan)
+        self.t = np.full(n, np.nan)
+        self.accepted = np.zeros(n, dtype=np.int64)
+        self.rejected = np.zeros(n, dtype=np.int64)
+        self.evals = np.zeros(n, dtype=np.int64)
+        self.status = np.full(n, -1, dtype=np.int32)
+        self.dense = np.full(state_len * n_eval, np.nan) if n_eval else None
+        self.n_dense = np.zeros(n, dtype=np.int32) if n_eval else None
+
+    def struct(self):
+        r = OdeResult()
+        r.y_final = _ptr(self.y)
+        r.t_final = _ptr(self.t)
+        r.accepted = _ptr(self.accepted)
+        r.rejected = _ptr(self.rejected)
+        r.evals = _ptr(self.evals)
+        r.status = _ptr(self.status)
+        r.dense = _ptr(self.dense)
+        r.n_dense = _ptr(self.n_dense)
+        self._struct = r
+        return r
+
+
+class SdeEnsemble:
+    """IVP::sde for an ensemble of paths."""
+
+    def __init__(self, system, t0, tf, y0, noise=DE_NOISE_PHILOX, seed=0, noise_table=None,
+                 noise_constant=0.0, path_base=0):
+        self.system = system
+        self.t0 = t0
+        self.tf = tf
+        y0 = np.asarray(y0, dtype=np.float64)
+        self.n = y0.size // system.dim
+        self.y0 = np.ascontiguousarray(y0).reshape(-1)
+        self.noise = noise
+        self.seed = seed
+        self.noise_table = None if noise_table is None else np.ascontiguousarray(
+            noise_table, dtype=np.float64).reshape(-1)
+        self.noise_constant = noise_constant
+        self.path_base = path_base
+        self._method = EulerMaruyama(0.0)
+        self.path_stride = 0
+        self.flags = 0
+
+    def method(self, m):
+        self._method = m
+        return self
+
+    def record_every(self, stride):
+        self.path_stride = stride
+        return self
+
+    def batch(self):
+        s = self.system
+        b = sde_defaults()
+        b.family = s.family
+        b.scheme = self._method.scheme
+        b.n = self.n
+        b.dim = s.dim
+        b.noise = self.noise
+        b.params = _ptr(s.params)
+        b.y0 = _ptr(self.y0)
+        b.t0 = self.t0
+        b.tf = self.tf
+        for k, v in self._method.settings.items():
+            setattr(b, k, v)
+        b.seed = self.seed
+        b.path_base = self.path_base
+        keep = [s.params, self.y0]
+        if self.noise_table is not None:
+            keep.append(self.noise_table)
+            b.noise_table = _ptr(self.noise_table)
+            b.noise_steps = len(self.noise_table) // s.dim
+        b.noise_constant = self.noise_constant
+        b.path_stride = self.path_stride
+        b.flags = self.flags | DE_FLAG_HOST_MEMORY
+        n_rec = 0
+        if self.path_stride > 0:
+            n_rec = int(b.max_steps) // self.path_stride + 1
+        res = SdeSolution(self.n, s.dim, n_rec)
+        return b, res, keep
+
+    def solve(self, stream=None):
+        b, res, keep = self.batch()
+        _abi.check(lib().de_sde_batch_solve(C.byref(b), C.byref(res.struct()), stream))
+        return res
+
+
+class SdeSolution:
+    def __init__(self, n, dim, n_rec):
+        self.y = np.full(dim * n, np.nan)
+        self.t = np.full(n, np.nan)
+        self.steps = np.zeros(n, dtype=np.int64)
+        self.evals = np.zeros(n, dtype=np.int64)
+        self.status = np.full(n, -1, dtype=np.int32)
+        self.path = np.full(n_rec * dim * n, np.nan) if n_rec else None
+        self.path_t = np.full(n_rec, np.nan) if n_rec else None
+
+    def struct(self):
+        r = SdeResult()
+        r.y_final = _ptr(self.y)
+        r.t_final = _ptr(self.t)
+        r.steps = _ptr(self.steps)
+        r.evals = _ptr(self.evals)
+        r.status = _ptr(self.status)
+        r.path = _ptr(self.path)
+        r.path_t = _ptr(self.path_t)
+        self._struct = r
+        return r
+
+
+class IVP:
+    """IVP::ode / IVP::sde (src/ivp.rs:277, 502) for ensembles."""
+
+    @staticmethod
+    def ode(system, t0, tf, y0, n=None):
+        return OdeEnsemble(system, t0, tf, y0, n=n)
+
+    @staticmethod
+    def sde(system, t0, tf, y0, **noise):
+        return SdeEnsemble(system, t0, tf, y0, **noise)
+
+
+# ---------------------------------------------------------------- workloads --
+def gen_lorenz(n, seed=0x5EED2026):
+    p = np.empty(3 * n)
+    y0 = np.empty(3 * n)
+    lib().de_gen_lorenz(seed, n, _ptr(p), _ptr(y0))
+    return p.reshape(3, n), y0.reshape(3, n)
+
+
+def gen_vdp(n, seed=0x5EED2026):
+    p = np.empty(n)
+    y0 = np.empty(2 * n)
+    lib().de_gen_vdp(seed, n, _ptr(p), _ptr(y0))
+    return p.reshape(1, n), y0.reshape(2, n)
+
+
+def gen_spring(n, seed=0x5EED2026, sizes=None):
+    if sizes is None:
+        sizes = np.empty(n, dtype=np.int32)
+        lib().de_gen_spring_sizes(seed, n, _ptr(sizes))
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    so = np.empty(n + 1, dtype=np.int64)
+    po = np.empty(n + 1, dtype=np.int64)
+    lib().de_spring_offsets(n, _ptr(sizes), _ptr(so), _ptr(po))
+    params = np.empty(int(po[-1]))
+    y0 = np.empty(int(so[-1]))
+    lib().de_gen_spring(seed, n, _ptr(sizes), _ptr(so), _ptr(po), _ptr(params), _ptr(y0))
+    return SpringChain(sizes, params), y0
+
+
+def gen_ou(n, dim=8, seed=0x5EED2026):
+    p = np.empty(3 * dim * n)
+    y0 = np.empty(dim * n)
+    lib().de_gen_ou(seed, n, dim, _ptr(p), _ptr(y0))
+    return OrnsteinUhlenbeck(dim, p), y0
+
+
+def gen_gbm(n, mu=0.05, sigma=0.2, s0=100.0):
+    p = np.empty(2 * n)
+    y0 = np.empty(n)
+    lib().de_gen_gbm(n, mu, sigma, s0, _ptr(p), _ptr(y0))
+    return LinearDiagonal(1, p), y0

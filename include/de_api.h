@@ -1,0 +1,222 @@
+/* C-ABI of the MI355X ensemble Runge-Kutta integrator.
+ *
+ * The reference crate solves ONE trajectory per call:
+ *   IVP::ode(&sys, t0, tf, y0).method(ExplicitRungeKutta::dopri5()).solve()
+ *     -> solve_ode (src/ode/solve_ivp.rs:116-277)
+ *   IVP::sde(&mut sys, t0, tf, y0).method(ExplicitRungeKutta::euler(dt) |
+ *     Milstein::new(dt)).solve() -> solve_sde (src/sde/solve_ivp.rs:135-287)
+ * Each entry point below runs that same call for a whole ensemble of
+ * trajectories on one GPU.  Per-trajectory outcomes never fail the call: they
+ * come back in the status array (the Ok/Err of each reference solve).  Call
+ * failures (bad descriptor, HIP error) return a negative code and set a
+ * thread-local message readable with de_last_error().
+ *
+ * Memory: by default every pointer is device memory (HBM) and the call is
+ * asynchronous on `stream` (a hipStream_t, NULL = default stream).  With
+ * DE_FLAG_HOST_MEMORY every pointer is host memory; the library stages copies
+ * and returns after the results are back on the host.
+ *
+ * Layouts (structure of arrays, trajectory index fastest, coalesced on HBM):
+ *   fixed-dimension families: y0/y_final[c * n + i], params[p * n + i],
+ *     dense[(e * dim + c) * n + i]
+ *   spring chain (ragged):    system i owns y0/y_final[state_offset[i] ..
+ *     state_offset[i+1]) = [x_0..x_{N-1}, v_0..v_{N-1}], params[param_offset[i]
+ *     .. param_offset[i+1]) = [c, k_0..k_N], dense[n_eval * state_offset[i] +
+ *     e * 2N + c].
+ */
+#ifndef DE_API_H
+#define DE_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Call return codes. */
+#define DE_OK 0
+#define DE_ERR_INVALID (-1)     /* malformed descriptor */
+#define DE_ERR_HIP (-2)         /* HIP runtime failure */
+#define DE_ERR_UNSUPPORTED (-3) /* family/method/mapping combination not built */
+#define DE_ERR_NOMEM (-4)       /* device allocation or budget exceeded */
+
+/* Per-trajectory status (Solution status / Error variant of one solve,
+ * src/status.rs:26, src/error.rs:13-41). */
+#define DE_STATUS_COMPLETE 0
+#define DE_STATUS_MAX_STEPS 1  /* Error::MaxSteps  */
+#define DE_STATUS_STEP_SIZE 2  /* Error::StepSize  */
+#define DE_STATUS_STIFFNESS 3  /* Error::Stiffness */
+#define DE_STATUS_BAD_INPUT 4  /* Error::BadInput  */
+
+/* ODE families (device-compiled right-hand sides). */
+#define DE_FAMILY_LORENZ 0       /* dim 3, params sigma, rho, beta */
+#define DE_FAMILY_VAN_DER_POL 1  /* dim 2, params mu */
+#define DE_FAMILY_SPRING_CHAIN 2 /* dim 2N ragged, params c, k_0..k_N */
+#define DE_FAMILY_EXPONENTIAL 3  /* dim 1, params k */
+#define DE_FAMILY_LINEAR 4       /* dim 1, params a, b */
+#define DE_FAMILY_HARMONIC 5     /* dim 2, params k */
+#define DE_FAMILY_LOGISTIC 6     /* dim 1, params k, m */
+#define DE_FAMILY_ROBERTSON 7    /* dim 3, no params */
+
+/* SDE families. */
+#define DE_SDE_ORNSTEIN_UHLENBECK 0 /* dim D, params theta_c, mu_c, sigma_c */
+#define DE_SDE_LINEAR_DIAGONAL 1    /* dim D, params a_c, b_c (GBM for D=1) */
+
+/* Methods: ExplicitRungeKutta::dopri5() (adaptive), ::rk4(h) and ::euler(h)
+ * (fixed step), src/methods/erk. */
+#define DE_METHOD_DOPRI5 0
+#define DE_METHOD_RK4 1
+#define DE_METHOD_EULER 2
+
+/* SDE schemes: ExplicitRungeKutta::euler(dt) on an SDE (Euler-Maruyama,
+ * src/methods/erk/fixed/stochastic.rs) and Milstein::new(dt)
+ * (src/methods/milstein.rs). */
+#define DE_SCHEME_EULER_MARUYAMA 0
+#define DE_SCHEME_MILSTEIN 1
+
+/* SDE noise sources (what SDE::noise fills dW with). */
+#define DE_NOISE_PHILOX 0   /* sqrt(h) * N(0,1) from Philox4x32-10(seed,path,step,c) */
+#define DE_NOISE_TABLE 1    /* noise_table[step * dim + c], 0 after the table ends */
+#define DE_NOISE_CONSTANT 2 /* noise_constant for every component and step */
+
+/* Launch schedules for the ensemble. */
+#define DE_SCHED_AUTO 0       /* library choice (autotuned) */
+#define DE_SCHED_PLAIN 1      /* one thread per trajectory, one launch */
+#define DE_SCHED_PERSISTENT 2 /* resident lanes refilled from a work counter */
+
+/* Mappings of a system onto lanes (spring chain). */
+#define DE_MAP_AUTO 0  /* autotuner picks per size class */
+#define DE_MAP_LANE 1  /* one lane per system, state in HBM */
+#define DE_MAP_GROUP 2 /* `group_size` lanes per system, state in registers */
+
+/* Flags. */
+#define DE_FLAG_HOST_MEMORY 1u /* all pointers are host memory */
+#define DE_FLAG_FAST_MATH 2u   /* FMA-contracted kernels: faster, not bit-equal */
+
+typedef struct de_ode_batch {
+  int32_t family; /* DE_FAMILY_* */
+  int32_t method; /* DE_METHOD_* */
+  int64_t n;      /* number of trajectories */
+  int32_t dim;    /* state dimension of fixed-size families (ignored for ragged) */
+  int32_t n_eval; /* number of t_eval points (0: final state only) */
+  const int32_t* sizes;        /* spring chain: masses N per system */
+  const int64_t* state_offset; /* spring chain: n+1 prefix offsets */
+  const int64_t* param_offset; /* spring chain: n+1 prefix offsets */
+  const double* params;
+  const double* y0;
+  const double* t0; /* per trajectory, or NULL to use t0_scalar */
+  const double* tf; /* per trajectory, or NULL to use tf_scalar */
+  double t0_scalar;
+  double tf_scalar;
+  /* Method settings; de_ode_batch_init fills the reference defaults of
+   * ExplicitRungeKutta (src/methods/erk/mod.rs:115-158). */
+  double rtol;        /* Tolerance::Scalar(rtol) */
+  double atol;
+  const double* rtol_traj; /* per-trajectory scalar rtol, or NULL */
+  const double* atol_traj;
+  double h0;          /* 0: automatic (h_init for DOPRI5, (tf-t0)/100 for RK4) */
+  double h_min;
+  double h_max;
+  int64_t max_steps;  /* counts attempted steps, as the reference does */
+  double safety_factor;
+  double min_scale;
+  double max_scale;
+  const double* t_eval; /* n_eval points (TEvalSolout semantics), sorted */
+  int32_t schedule;   /* DE_SCHED_* */
+  int32_t mapping;    /* DE_MAP_* */
+  int32_t group_size; /* lanes per system for DE_MAP_GROUP (0: autotune) */
+  uint32_t flags;     /* DE_FLAG_* */
+} de_ode_batch;
+
+typedef struct de_ode_result {
+  double* y_final;   /* final state (state at failure for error statuses) */
+  double* t_final;   /* [n] */
+  int64_t* accepted; /* [n] Steps::accepted */
+  int64_t* rejected; /* [n] Steps::rejected */
+  int64_t* evals;    /* [n] Evals::function */
+  int32_t* status;   /* [n] DE_STATUS_* */
+  double* dense;     /* t_eval output (NaN where a point was not produced), or NULL */
+  int32_t* n_dense;  /* [n] number of t_eval points produced, or NULL */
+} de_ode_result;
+
+typedef struct de_sde_batch {
+  int32_t family; /* DE_SDE_* */
+  int32_t scheme; /* DE_SCHEME_* */
+  int64_t n;      /* number of paths */
+  int32_t dim;
+  int32_t noise;  /* DE_NOISE_* */
+  const double* params; /* [P * n] structure of arrays */
+  const double* y0;     /* [dim * n] */
+  double t0;
+  double tf;
+  double h0;            /* dt; 0: (tf - t0)/100 as the reference */
+  double h_min;
+  double h_max;
+  int64_t max_steps;    /* reference default 10 000 */
+  uint64_t seed;        /* DE_NOISE_PHILOX */
+  uint64_t path_base;   /* global index of path 0 (noise key) */
+  const double* noise_table; /* DE_NOISE_TABLE: [noise_steps * dim] */
+  int64_t noise_steps;
+  double noise_constant;
+  int32_t path_stride;  /* >0: record y after every path_stride-th step */
+  uint32_t flags;
+} de_sde_batch;
+
+typedef struct de_sde_result {
+  double* y_final;  /* [dim * n] */
+  double* t_final;  /* [n] */
+  int64_t* steps;   /* [n] Steps::accepted (every SDE step is accepted) */
+  int64_t* evals;   /* [n] Evals::function */
+  int32_t* status;  /* [n] */
+  double* path;     /* [n_rec * dim * n] with n_rec = steps / path_stride, or NULL */
+  double* path_t;   /* [n_rec] times of the recorded states, or NULL */
+} de_sde_result;
+
+/* Fill a descriptor with the reference defaults: rtol = atol = 1e-6, h0 = 0,
+ * h_min = 0, h_max = inf, max_steps = 10000, safety 0.9, scales [0.2, 10]
+ * (ExplicitRungeKutta::default, src/methods/erk/mod.rs:115-158). */
+void de_ode_batch_init(de_ode_batch* b);
+void de_sde_batch_init(de_sde_batch* b);
+
+/* Batched solve_ode (src/ode/solve_ivp.rs:116-277) with ExplicitRungeKutta
+ * dopri5 (src/methods/erk/dormandprince/ordinary.rs:16-336) or rk4
+ * (src/methods/erk/fixed/ordinary.rs:16-220). */
+int de_ode_batch_solve(const de_ode_batch* batch, de_ode_result* out, void* stream);
+
+/* Batched solve_sde (src/sde/solve_ivp.rs:135-287) with Euler-Maruyama
+ * (src/methods/erk/fixed/stochastic.rs:18-193) or Milstein
+ * (src/methods/milstein.rs:76-224). */
+int de_sde_batch_solve(const de_sde_batch* batch, de_sde_result* out, void* stream);
+
+/* Bytes of device workspace a de_ode_batch_solve call with this descriptor
+ * allocates beyond its inputs and outputs (HBM admission budget). */
+int64_t de_ode_workspace_bytes(const de_ode_batch* batch);
+
+/* Message of the last failed call on this thread. */
+const char* de_last_error(void);
+
+/* Library version string. */
+const char* de_version(void);
+
+/* ---- Serving: continuous batching of ensemble solves on one GPU ----------
+ * Requests (each a de_ode_batch over host memory) are admitted while their
+ * device footprint fits the HBM budget and lanes are free; their state stays
+ * resident on the device until the request completes.  Requests admitted
+ * while others run share the persistent lanes of the running kernels. */
+typedef struct de_server de_server;
+int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out);
+/* Copies the request's inputs; returns a ticket. */
+int de_server_submit(de_server* s, const de_ode_batch* req, uint64_t* ticket);
+/* Returns 1 and fills host-memory `out` when the request is done, 0 if it is
+ * still running (wait = 0) or blocks until done (wait = 1). */
+int de_server_poll(de_server* s, uint64_t ticket, de_ode_result* out, int32_t wait);
+/* Submit-to-completion latency of a finished request, in seconds. */
+double de_server_latency(de_server* s, uint64_t ticket);
+void de_server_destroy(de_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DE_API_H */

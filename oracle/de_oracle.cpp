@@ -1,0 +1,1011 @@
+// CPU oracle for the ensemble integrator -- TEST INFRASTRUCTURE ONLY.
+//
+// A line-by-line restatement, in C++, of the reference crate's explicit
+// Runge-Kutta hot path (Ryan-D-Gast/differential-equations v0.6.1), keeping the
+// reference's floating-point operation order so results can be compared bit for
+// bit.  Only tests, the smoke check and the benchmark's CPU-baseline leg load
+// this library; the product never links it.
+//
+// Built with -O2 -ffp-contract=off (no implicit FMA) like the device code.
+// Operation order follows, function by function:
+//   solve_ode           src/ode/solve_ivp.rs:116-277
+//   solve_sde           src/sde/solve_ivp.rs:135-287
+//   DOPRI5 init/step    src/methods/erk/dormandprince/ordinary.rs:16-270
+//   DOPRI5 interpolate  src/methods/erk/dormandprince/ordinary.rs:298-336
+//   tableau dopri5      src/tableau/dorman_prince.rs:37-117
+//   tableau rk4         src/tableau/runge_kutta.rs:39-70
+//   defaults            src/methods/erk/mod.rs:115-158
+//   h_init (Ordinary)   src/methods/h_init.rs:45-135
+//   validate / clamp    src/utils.rs:22-157
+//   State ops, norm     src/traits.rs:305-411
+//   fixed ERK (RK4)     src/methods/erk/fixed/ordinary.rs:16-220
+//   cubic Hermite       src/interpolate.rs:40-60
+//   stochastic ERK (EM) src/methods/erk/fixed/stochastic.rs:18-146
+//   Milstein            src/methods/milstein.rs:76-180
+//   TEvalSolout         src/solout/t_eval.rs:73-172
+// The tableau's zero coefficients are applied like any other (the reference
+// loops over all j < i), so this restatement does too.
+//
+// pow: the reference calls f64::powf (glibc `pow` on linux-gnu).  pow_mode 0
+// uses the integrator's own double-double pow (shared source with the GPU);
+// pow_mode 1 calls the C library's pow, i.e. exactly what a Rust build of the
+// reference calls on this host.  Comparing the two modes measures the only
+// intended difference between the GPU path and the reference binary.
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "../differential-equations_amd/csrc/de_families.h"
+#include "../differential-equations_amd/csrc/de_noise.h"
+#include "../include/de_api.h"
+
+namespace orc {
+
+static int g_pow_mode = 0;
+
+static inline double ref_powf(double x, double y) {
+  return g_pow_mode ? ::pow(x, y) : de::pow(x, y);
+}
+
+// ---------------------------------------------------------------- systems --
+template <class F>
+struct FixedSys {
+  F f;
+  static constexpr int kMax = F::kDim;
+  int dim() const { return F::kDim; }
+  void diff(double t, const double* y, double* dy) const { f.diff(t, y, dy); }
+};
+
+struct SpringSys {
+  static constexpr int kMax = 512;
+  int nmass;
+  const double* params;
+  int dim() const { return 2 * nmass; }
+  void diff(double, const double* y, double* dy) const {
+    de::spring_chain_diff(nmass, params, y, dy);
+  }
+};
+
+struct Settings {
+  double rtol, atol, h0, h_min, h_max, safety, min_scale, max_scale;
+  int64_t max_steps;
+};
+
+struct TrajResult {
+  double t;
+  int64_t accepted = 0, rejected = 0, evals = 0;
+  int32_t status = DE_STATUS_COMPLETE;
+};
+
+// --------------------------------------------------------------- tableaux --
+// src/tableau/dorman_prince.rs:37-117
+struct Dopri5Tableau {
+  double c[7] = {0.0, 0.2, 0.3, 0.8, 8.0 / 9.0, 1.0, 1.0};
+  double a[7][7] = {};
+  double b[7] = {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0,
+                 -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
+  double er[7] = {71.0 / 57600.0, 0.0, -71.0 / 16695.0, 71.0 / 1920.0,
+                  -17253.0 / 339200.0, 22.0 / 525.0, -1.0 / 40.0};
+  double bi[7][7] = {};
+  Dopri5Tableau() {
+    a[1][0] = 0.2;
+    a[2][0] = 3.0 / 40.0;
+    a[2][1] = 9.0 / 40.0;
+    a[3][0] = 44.0 / 45.0;
+    a[3][1] = -56.0 / 15.0;
+    a[3][2] = 32.0 / 9.0;
+    a[4][0] = 19372.0 / 6561.0;
+    a[4][1] = -25360.0 / 2187.0;
+    a[4][2] = 64448.0 / 6561.0;
+    a[4][3] = -212.0 / 729.0;
+    a[5][0] = 9017.0 / 3168.0;
+    a[5][1] = -355.0 / 33.0;
+    a[5][2] = 46732.0 / 5247.0;
+    a[5][3] = 49.0 / 176.0;
+    a[5][4] = -5103.0 / 18656.0;
+    a[6][0] = 35.0 / 384.0;
+    a[6][1] = 0.0;
+    a[6][2] = 500.0 / 1113.0;
+    a[6][3] = 125.0 / 192.0;
+    a[6][4] = -2187.0 / 6784.0;
+    a[6][5] = 11.0 / 84.0;
+    bi[0][0] = -12715105075.0 / 11282082432.0;
+    bi[0][1] = 0.0;
+    bi[0][2] = 87487479700.0 / 32700410799.0;
+    bi[0][3] = -10690763975.0 / 1880347072.0;
+    bi[0][4] = 701980252875.0 / 199316789632.0;
+    bi[0][5] = -1453857185.0 / 822651844.0;
+    bi[0][6] = 69997945.0 / 29380423.0;
+  }
+};
+static const Dopri5Tableau kDP;
+
+// Fixed-step tableaux: rk4 (src/tableau/runge_kutta.rs:39-70) and euler
+// (src/tableau/runge_kutta.rs:289-302).
+struct FixedTableau {
+  int stages;
+  double c[4];
+  double a[4][4];
+  double b[4];
+};
+static const FixedTableau kRK4 = {
+    4, {0.0, 0.5, 0.5, 1.0},
+    {{0, 0, 0, 0}, {0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0}},
+    {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0}};
+static const FixedTableau kEuler = {1, {0.0}, {{0.0}}, {1.0}};
+
+// ------------------------------------------------------------ State ops ----
+// add_scaled: v_i += alpha * w_i (src/traits.rs:316-320)
+static inline void add_scaled(double* v, double alpha, const double* w, int n) {
+  for (int i = 0; i < n; ++i) v[i] += alpha * w[i];
+}
+static inline void copy(double* d, const double* s, int n) { memcpy(d, s, sizeof(double) * n); }
+static inline void zero(double* d, int n) {
+  for (int i = 0; i < n; ++i) d[i] = 0.0;
+}
+
+// error_norm (src/traits.rs:394-411): sum_i (err_i / sk_i)^2 in index order.
+static inline double error_norm(const double* y, const double* y_new, const double* err,
+                                double atol, double rtol, int n) {
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double sk = atol + rtol * fmax(fabs(y[i]), fabs(y_new[i]));
+    double e = err[i] / sk;
+    sum += e * e;
+  }
+  return sum;
+}
+
+// diff_norm_squared (src/traits.rs:383-391)
+static inline double diff_norm_squared(const double* a, const double* b, int n) {
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double d = a[i] - b[i];
+    sum += d * d;
+  }
+  return sum;
+}
+
+// ------------------------------------------------------------- h_init ------
+// InitialStepSize::<Ordinary>::compute, src/methods/h_init.rs:45-135.
+template <class Sys>
+static double h_init(const Sys& sys, double t0, double tf, const double* y0, int order,
+                     double rtol, double atol, double h_min, double h_max, int64_t* evals) {
+  constexpr int M = Sys::kMax;
+  const int n = sys.dim();
+  double posneg = de::signum(tf - t0);
+  double f0[M], f1[M], sk[M], y1[M];
+  zero(f0, n);
+  zero(f1, n);
+  sys.diff(t0, y0, f0);
+  *evals += 1;
+  double dnf = 0.0, dny = 0.0;
+  for (int i = 0; i < n; ++i) {
+    sk[i] = atol + rtol * fabs(y0[i]);
+    double a = f0[i] / sk[i];
+    dnf += a * a;
+    double b = y0[i] / sk[i];
+    dny += b * b;
+  }
+  double h;
+  if (dnf <= 1.0e-10 || dny <= 1.0e-10) {
+    h = 1.0e-6;
+  } else {
+    h = sqrt(dny / dnf) * 0.01;
+  }
+  h = fmin(h, h_max);
+  h *= posneg;
+  copy(y1, y0, n);
+  add_scaled(y1, h, f0, n);
+  sys.diff(t0 + h, y1, f1);
+  *evals += 1;
+  double der2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double d = (f1[i] - f0[i]) / sk[i];
+    der2 += d * d;
+  }
+  der2 = sqrt(der2) / fabs(h);
+  double der12 = fmax(sqrt(dnf), der2);
+  double h1;
+  if (der12 <= 1.0e-15) {
+    h1 = fabs(h) * fmax(1.0e-3, 1.0e-6);
+  } else {
+    double order_t = (double)order;
+    h1 = ref_powf(0.01 / der12, 1.0 / order_t);
+  }
+  double interval = fabs(tf - t0);
+  h = fmin(fmax(fmin(fmin(fabs(h) * 100.0, h1), h_max), h_min), interval);
+  return h * posneg;
+}
+
+// -------------------------------------------------------------- solouts ----
+struct TEval {
+  const double* t_evals = nullptr;  // sorted in integration direction
+  int n_eval = 0;
+  int next = 0;
+  double dir = 1.0;
+  double* dense = nullptr;  // [n_eval][dim], NaN prefilled
+  int pushed = 0;
+};
+
+// TEvalSolout::solout (src/solout/t_eval.rs:83-134); interp(te, out) is the
+// method's Interpolation::interpolate.
+template <class Interp>
+static void teval_solout(TEval& s, double t_curr, double t_prev, const double* y_curr, int n,
+                         Interp&& interp) {
+  if (!s.t_evals) return;
+  int idx = s.next;
+  while (idx < s.n_eval) {
+    double te = s.t_evals[idx];
+    bool in_range = s.dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
+                                : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
+    if (in_range) {
+      double* out = s.dense + (int64_t)idx * n;
+      if (te == t_curr) {
+        copy(out, y_curr, n);
+      } else {
+        interp(te, out);
+      }
+      s.pushed += 1;
+      idx += 1;
+    } else {
+      if ((s.dir > 0.0 && te > t_curr) || (s.dir < 0.0 && te < t_curr)) break;
+      idx += 1;
+    }
+  }
+  s.next = idx;
+}
+
+// ---------------------------------------------------------------- DOPRI5 ---
+template <class Sys>
+struct Dopri5 {
+  static constexpr int M = Sys::kMax;
+  const Sys& sys;
+  Settings st;
+  int n;
+  double h0, h = 0.0, t = 0.0, h_prev = 0.0, t_prev = 0.0;
+  double y[M], dydt[M], y_prev[M], dydt_prev[M];
+  double k[7][M];
+  double cont[5][M];
+  int64_t steps = 0;
+  int stiffness_counter = 0, non_stiffness_counter = 0;
+  bool rejected_status = false;  // Status::RejectedStep vs Solving
+
+  Dopri5(const Sys& s, const Settings& set) : sys(s), st(set), n(s.dim()), h0(set.h0) {}
+
+  // ordinary.rs:16-61; returns false on BadInput.
+  bool init(double t0, double tf, const double* y0, int64_t* evals) {
+    if (h0 == 0.0) {
+      h0 = h_init(sys, t0, tf, y0, 5, st.rtol, st.atol, st.h_min, st.h_max, evals);
+    }
+    if (!de::validate_step_size(h0, st.h_min, st.h_max, t0, tf)) return false;
+    h = h0;
+    stiffness_counter = 0;
+    t = t0;
+    copy(y, y0, n);
+    zero(dydt, n);
+    copy(y_prev, y0, n);
+    zero(dydt_prev, n);
+    for (int i = 0; i < 7; ++i) zero(k[i], n);
+    for (int i = 0; i < 5; ++i) zero(cont[i], n);
+    sys.diff(t, y, k[0]);
+    copy(dydt, k[0], n);
+    *evals += 1;
+    t_prev = t;
+    copy(y_prev, y, n);
+    copy(dydt_prev, dydt, n);
+    return true;
+  }
+
+  // ordinary.rs:63-270.  Returns the Err status, or -1 for Ok.  *evals gets
+  // this step's function evaluations.
+  int step(int64_t* evals) {
+    if (fabs(h) < fabs(h_prev) * 1e-14) return DE_STATUS_STEP_SIZE;
+    if (steps >= st.max_steps) return DE_STATUS_MAX_STEPS;
+    steps += 1;
+    double y_stage[M];
+    zero(y_stage, n);
+    for (int i = 1; i < 7; ++i) {
+      copy(y_stage, y, n);
+      for (int j = 0; j < i; ++j) add_scaled(y_stage, kDP.a[i][j] * h, k[j], n);
+      sys.diff(t + kDP.c[i] * h, y_stage, k[i]);
+    }
+    double ysti[M];
+    copy(ysti, y_stage, n);
+    double yseg[M];
+    zero(yseg, n);
+    for (int i = 0; i < 7; ++i) add_scaled(yseg, kDP.b[i], k[i], n);
+    double y_new[M];
+    copy(y_new, y, n);
+    add_scaled(y_new, h, yseg, n);
+    double t_new = t + h;
+    *evals += 6;
+    double err2 = 0.0;
+    double err_state[M];
+    zero(err_state, n);
+    for (int j = 0; j < 7; ++j) add_scaled(err_state, kDP.er[j], k[j], n);
+    double err = error_norm(y, y_new, err_state, st.atol, st.rtol, n);
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * (double)n));
+    double error_exponent = 1.0 / 5.0;
+    double scale = st.safety * ref_powf(err, -error_exponent);
+    scale = fmin(fmax(scale, st.min_scale), st.max_scale);
+    if (err <= 1.0) {
+      sys.diff(t_new, y_new, dydt);
+      *evals += 1;
+      if (steps % 100 == 0) {
+        double stdnum = diff_norm_squared(yseg, k[6], n);
+        double stden = diff_norm_squared(dydt, ysti, n);
+        if (stden > 0.0) {
+          double h_lamb = h * sqrt(stdnum / stden);
+          if (h_lamb > 6.1) {
+            non_stiffness_counter = 0;
+            stiffness_counter += 1;
+            if (stiffness_counter == 15) return DE_STATUS_STIFFNESS;
+          }
+        } else {
+          non_stiffness_counter += 1;
+          if (non_stiffness_counter == 6) stiffness_counter = 0;
+        }
+      }
+      // Dense output coefficients (ordinary.rs:196-235).
+      copy(cont[0], y, n);
+      double ydiff[M];
+      copy(ydiff, y_new, n);
+      add_scaled(ydiff, -1.0, y, n);
+      copy(cont[1], ydiff, n);
+      double bspl[M];
+      zero(bspl, n);
+      add_scaled(bspl, h, k[0], n);
+      add_scaled(bspl, -1.0, ydiff, n);
+      copy(cont[2], bspl, n);
+      double cont3[M];
+      copy(cont3, ydiff, n);
+      add_scaled(cont3, -h, dydt, n);
+      add_scaled(cont3, -1.0, bspl, n);
+      copy(cont[3], cont3, n);
+      // bi is Some; I == S so no extra stages; order 5 -> i = 4 only.
+      for (int i = 4; i < 5; ++i) {
+        zero(cont[i], n);
+        for (int j = 0; j < 7; ++j) add_scaled(cont[i], kDP.bi[i][j], k[j], n);
+        for (int c = 0; c < n; ++c) cont[i][c] *= h;
+      }
+      t_prev = t;
+      copy(y_prev, y, n);
+      copy(dydt_prev, k[0], n);
+      h_prev = h;
+      t = t_new;
+      copy(y, y_new, n);
+      copy(k[0], dydt, n);
+      if (rejected_status) {
+        rejected_status = false;
+        scale = fmin(scale, 1.0);
+      }
+    } else {
+      rejected_status = true;
+    }
+    h *= scale;
+    h = de::constrain_step_size(h, st.h_min, st.h_max);
+    return -1;
+  }
+
+  // ordinary.rs:301-336 (in-bounds callers only: TEvalSolout).
+  void interpolate(double t_interp, double* out) const {
+    double s = (t_interp - t_prev) / h_prev;
+    double s1 = 1.0 - s;
+    double acc[M];
+    copy(acc, cont[4], n);
+    for (int i = 3; i >= 1; --i) {
+      double factor = (i % 2 == 1) ? s1 : s;
+      for (int c = 0; c < n; ++c) acc[c] *= factor;
+      add_scaled(acc, 1.0, cont[i], n);
+    }
+    copy(out, cont[0], n);
+    add_scaled(out, s, acc, n);
+  }
+};
+
+// ------------------------------------------------------- fixed-step ERK ----
+// src/methods/erk/fixed/ordinary.rs (rk4, euler; fsal = false, bi = None)
+template <class Sys>
+struct FixedErk {
+  static constexpr int M = Sys::kMax;
+  const Sys& sys;
+  Settings st;
+  int n;
+  double h0, h = 0.0, t = 0.0, h_prev = 0.0, t_prev = 0.0;
+  double y[M], dydt[M], y_prev[M], dydt_prev[M];
+  double k[4][M];
+  int64_t steps = 0;
+  bool rejected_status = false;
+  const FixedTableau& tb;
+
+  FixedErk(const Sys& s, const Settings& set, const FixedTableau& t)
+      : sys(s), st(set), n(s.dim()), h0(set.h0), tb(t) {}
+
+  bool init(double t0, double tf, const double* y0, int64_t* evals) {
+    if (h0 == 0.0) {
+      double duration = fabs(tf - t0);
+      h0 = duration / 100.0;
+    }
+    if (!de::validate_step_size(h0, st.h_min, st.h_max, t0, tf)) return false;
+    h = h0;
+    t = t0;
+    copy(y, y0, n);
+    zero(dydt, n);
+    copy(y_prev, y0, n);
+    zero(dydt_prev, n);
+    for (int i = 0; i < 4; ++i) zero(k[i], n);
+    sys.diff(t, y, dydt);
+    *evals += 1;
+    t_prev = t;
+    copy(y_prev, y, n);
+    copy(dydt_prev, dydt, n);
+    return true;
+  }
+
+  int step(int64_t* evals) {
+    if (steps >= st.max_steps) return DE_STATUS_MAX_STEPS;
+    steps += 1;
+    copy(k[0], dydt, n);
+    for (int i = 1; i < tb.stages; ++i) {
+      double y_stage[M];
+      copy(y_stage, y, n);
+      for (int j = 0; j < i; ++j) add_scaled(y_stage, tb.a[i][j] * h, k[j], n);
+      sys.diff(t + tb.c[i] * h, y_stage, k[i]);
+    }
+    *evals += tb.stages - 1;
+    t_prev = t;
+    copy(y_prev, y, n);
+    copy(dydt_prev, k[0], n);
+    h_prev = h;
+    double y_next[M];
+    copy(y_next, y, n);
+    for (int i = 0; i < tb.stages; ++i) add_scaled(y_next, tb.b[i] * h, k[i], n);
+    t += h;
+    copy(y, y_next, n);
+    sys.diff(t, y, dydt);
+    *evals += 1;
+    return -1;
+  }
+
+  // cubic_hermite_interpolate (src/interpolate.rs:40-60) on
+  // (t_prev, t, y_prev, y, dydt_prev, dydt).
+  void interpolate(double tt, double* out) const {
+    double hh = t - t_prev;
+    double s = (tt - t_prev) / hh;
+    double s2 = s * s;
+    double s3 = s2 * s;
+    double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+    double h10 = s3 - 2.0 * s2 + s;
+    double h01 = -2.0 * s3 + 3.0 * s2;
+    double h11 = s3 - s2;
+    zero(out, n);
+    add_scaled(out, h00, y_prev, n);
+    add_scaled(out, h10 * hh, dydt_prev, n);
+    add_scaled(out, h01, y, n);
+    add_scaled(out, h11 * hh, dydt, n);
+  }
+};
+
+// ------------------------------------------------------------- solve_ode ---
+// src/ode/solve_ivp.rs:116-277 with TEvalSolout (or DefaultSolout when t_eval
+// is absent: only the final state is kept).
+template <class Method>
+static TrajResult solve_ode(Method& m, double t0, double tf, const double* y0, TEval* te,
+                            double* y_out) {
+  TrajResult r;
+  const int n = m.n;
+  double dir;
+  {
+    double s = de::signum(tf - t0);
+    if (s == 1.0) dir = 1.0;
+    else if (s == -1.0) dir = -1.0;
+    else {
+      r.status = DE_STATUS_BAD_INPUT;
+      r.t = t0;
+      copy(y_out, y0, n);
+      return r;
+    }
+  }
+  int64_t ev = 0;
+  if (!m.init(t0, tf, y0, &ev)) {
+    r.status = DE_STATUS_BAD_INPUT;
+    r.t = t0;
+    copy(y_out, y0, n);
+    return r;
+  }
+  r.evals += ev;
+  if (te) {
+    te->dir = dir;
+    teval_solout(*te, m.t, m.t_prev, m.y, n, [&](double tt, double* o) { m.interpolate(tt, o); });
+  }
+  m.rejected_status = false;  // Status::Solving
+  for (;;) {
+    if ((m.t + m.h - tf) * dir > 0.0) {
+      double h_new = tf - m.t;
+      if (fabs(h_new) < de::kTenEps) break;  // Complete
+      m.h = h_new;
+    }
+    int64_t sev = 0;
+    int e = m.step(&sev);
+    if (e >= 0) {
+      r.status = e;
+      r.t = m.t;
+      copy(y_out, m.y, n);
+      return r;
+    }
+    r.evals += sev;
+    if (m.rejected_status) {
+      r.rejected += 1;
+      continue;
+    }
+    r.accepted += 1;
+    if (te) {
+      teval_solout(*te, m.t, m.t_prev, m.y, n, [&](double tt, double* o) { m.interpolate(tt, o); });
+    }
+    if (fabs(tf - m.t) <= de::kTenEps) break;
+  }
+  r.status = DE_STATUS_COMPLETE;
+  r.t = m.t;
+  copy(y_out, m.y, n);
+  return r;
+}
+
+// ---------------------------------------------------------------- SDEs -----
+struct Noise {
+  int32_t kind;
+  uint64_t seed;
+  uint64_t path;
+  const double* table;
+  int64_t table_steps;
+  double constant;
+  int dim;
+  uint32_t calls = 0;
+  // SDE::noise(dt, dw)
+  void fill(double dt, double* dw) {
+    if (kind == DE_NOISE_PHILOX) {
+      double sq = sqrt(dt);
+      for (int c = 0; c < dim; ++c) dw[c] = sq * de::noise_normal(seed, path, calls, (uint32_t)c);
+    } else if (kind == DE_NOISE_TABLE) {
+      for (int c = 0; c < dim; ++c)
+        dw[c] = (int64_t)calls < table_steps ? table[(int64_t)calls * dim + c] : 0.0;
+    } else {
+      for (int c = 0; c < dim; ++c) dw[c] = constant;
+    }
+    calls += 1;
+  }
+};
+
+struct SdeSys {
+  int32_t family;
+  int dim;
+  const double* p;  // this path's parameters, gathered: [P]
+  void drift(double, const double* y, double* f) const {
+    if (family == DE_SDE_ORNSTEIN_UHLENBECK) {
+      for (int c = 0; c < dim; ++c) f[c] = p[3 * c + 0] * (p[3 * c + 1] - y[c]);
+    } else {
+      for (int c = 0; c < dim; ++c) f[c] = p[2 * c + 0] * y[c];
+    }
+  }
+  void diffusion(double, const double* y, double* g) const {
+    if (family == DE_SDE_ORNSTEIN_UHLENBECK) {
+      for (int c = 0; c < dim; ++c) g[c] = p[3 * c + 2];
+    } else {
+      for (int c = 0; c < dim; ++c) g[c] = p[2 * c + 1] * y[c];
+    }
+  }
+};
+
+constexpr int kSdeMax = 64;
+
+struct SdeState {
+  double h0, h = 0.0, t = 0.0, t_prev = 0.0;
+  double y[kSdeMax], dydt[kSdeMax], y_prev[kSdeMax], dydt_prev[kSdeMax];
+  int64_t steps = 0;
+};
+
+// Euler-Maruyama: ExplicitRungeKutta::euler on an SDE,
+// src/methods/erk/fixed/stochastic.rs:18-146.
+static bool em_init(SdeState& s, const SdeSys& sys, double t0, double tf, const double* y0,
+                    double h_min, double h_max, int64_t* evals) {
+  const int n = sys.dim;
+  if (s.h0 == 0.0) {
+    double duration = fabs(tf - t0);
+    s.h0 = duration / 100.0;
+  }
+  if (!de::validate_step_size(s.h0, h_min, h_max, t0, tf)) return false;
+  s.h = s.h0;
+  s.steps = 0;
+  s.t = t0;
+  copy(s.y, y0, n);
+  zero(s.dydt, n);
+  copy(s.y_prev, y0, n);
+  zero(s.dydt_prev, n);
+  sys.drift(s.t, s.y, s.dydt);
+  double diffusion[kSdeMax];
+  zero(diffusion, n);
+  sys.diffusion(s.t, s.y, diffusion);
+  *evals += 2;
+  s.t_prev = s.t;
+  copy(s.y_prev, s.y, n);
+  copy(s.dydt_prev, s.dydt, n);
+  return true;
+}
+
+static int em_step(SdeState& s, const SdeSys& sys, Noise& noise, int64_t max_steps,
+                   int64_t* evals) {
+  const int n = sys.dim;
+  if (s.steps >= max_steps) return DE_STATUS_MAX_STEPS;
+  s.steps += 1;
+  s.t_prev = s.t;
+  copy(s.y_prev, s.y, n);
+  copy(s.dydt_prev, s.dydt, n);
+  double k0[kSdeMax];
+  copy(k0, s.dydt, n);
+  // euler tableau: one stage, b = [1]
+  double drift_increment[kSdeMax];
+  zero(drift_increment, n);
+  add_scaled(drift_increment, 1.0 * s.h, k0, n);
+  double diffusion[kSdeMax];
+  zero(diffusion, n);
+  sys.diffusion(s.t, s.y, diffusion);
+  *evals += 1;
+  double dw[kSdeMax];
+  zero(dw, n);
+  noise.fill(s.h, dw);
+  double diffusion_increment[kSdeMax];
+  copy(diffusion_increment, diffusion, n);
+  for (int c = 0; c < n; ++c) diffusion_increment[c] *= dw[c];
+  double y_next[kSdeMax];
+  copy(y_next, s.y, n);
+  add_scaled(y_next, 1.0, drift_increment, n);
+  add_scaled(y_next, 1.0, diffusion_increment, n);
+  s.t += s.h;
+  copy(s.y, y_next, n);
+  sys.drift(s.t, s.y, s.dydt);
+  *evals += 1;
+  return -1;
+}
+
+// Milstein, src/methods/milstein.rs:76-180.
+static bool mil_init(SdeState& s, const SdeSys& sys, double t0, double tf, const double* y0,
+                     double h_min, double h_max, int64_t* evals) {
+  const int n = sys.dim;
+  if (s.h0 == 0.0) {
+    double duration = fabs(tf - t0);
+    s.h0 = duration / 100.0;
+  }
+  if (!de::validate_step_size(s.h0, h_min, h_max, t0, tf)) return false;
+  s.h = s.h0;
+  s.steps = 0;
+  s.t = t0;
+  copy(s.y, y0, n);
+  zero(s.dydt, n);
+  s.t_prev = t0;
+  copy(s.y_prev, y0, n);
+  sys.drift(s.t, s.y, s.dydt);
+  *evals += 1;
+  return true;
+}
+
+static int mil_step(SdeState& s, const SdeSys& sys, Noise& noise, int64_t max_steps,
+                    int64_t* evals) {
+  const int n = sys.dim;
+  if (s.steps >= max_steps) return DE_STATUS_MAX_STEPS;
+  s.steps += 1;
+  s.t_prev = s.t;
+  copy(s.y_prev, s.y, n);
+  double diffusion[kSdeMax];
+  zero(diffusion, n);
+  sys.diffusion(s.t, s.y, diffusion);
+  *evals += 1;
+  double dw[kSdeMax];
+  zero(dw, n);
+  noise.fill(s.h, dw);
+  double sqrt_h = sqrt(s.h);
+  double y_aux[kSdeMax];
+  copy(y_aux, s.y, n);
+  add_scaled(y_aux, sqrt_h, diffusion, n);
+  double diffusion_aux[kSdeMax];
+  zero(diffusion_aux, n);
+  sys.diffusion(s.t, y_aux, diffusion_aux);
+  *evals += 1;
+  double dw_sq[kSdeMax];
+  copy(dw_sq, dw, n);
+  for (int c = 0; c < n; ++c) dw_sq[c] *= dw[c];
+  double term[kSdeMax];
+  zero(term, n);
+  double factor = 1.0 / (2.0 * sqrt_h);
+  for (int c = 0; c < n; ++c) {
+    double diff = diffusion_aux[c] - diffusion[c];
+    double dws_minus_h = dw_sq[c] - s.h;
+    term[c] = diff * dws_minus_h * factor;
+  }
+  double drift_increment[kSdeMax];
+  copy(drift_increment, s.dydt, n);
+  for (int c = 0; c < n; ++c) drift_increment[c] *= s.h;
+  double diffusion_increment[kSdeMax];
+  copy(diffusion_increment, diffusion, n);
+  for (int c = 0; c < n; ++c) diffusion_increment[c] *= dw[c];
+  double y_next[kSdeMax];
+  copy(y_next, s.y, n);
+  add_scaled(y_next, 1.0, drift_increment, n);
+  add_scaled(y_next, 1.0, diffusion_increment, n);
+  add_scaled(y_next, 1.0, term, n);
+  s.t += s.h;
+  copy(s.y, y_next, n);
+  sys.drift(s.t, s.y, s.dydt);
+  *evals += 1;
+  return -1;
+}
+
+// solve_sde, src/sde/solve_ivp.rs:135-287 (DefaultSolout: every step is
+// recorded; here every path_stride-th state is written to `path`).
+static TrajResult solve_sde_one(int32_t scheme, const SdeSys& sys, Noise& noise, double t0,
+                                double tf, double h0, double h_min, double h_max,
+                                int64_t max_steps, const double* y0, double* y_out,
+                                int32_t path_stride, double* path_rec, int64_t path_stride_elems,
+                                double* path_t) {
+  TrajResult r;
+  const int n = sys.dim;
+  SdeState s;
+  s.h0 = h0;
+  double dirs = de::signum(tf - t0);
+  double dir;
+  if (dirs == 1.0) dir = 1.0;
+  else if (dirs == -1.0) dir = -1.0;
+  else {
+    r.status = DE_STATUS_BAD_INPUT;
+    r.t = t0;
+    copy(y_out, y0, n);
+    return r;
+  }
+  int64_t ev = 0;
+  bool ok = scheme == DE_SCHEME_MILSTEIN
+                ? mil_init(s, sys, t0, tf, y0, h_min, h_max, &ev)
+                : em_init(s, sys, t0, tf, y0, h_min, h_max, &ev);
+  if (!ok) {
+    r.status = DE_STATUS_BAD_INPUT;
+    r.t = t0;
+    copy(y_out, y0, n);
+    return r;
+  }
+  r.evals += ev;
+  for (;;) {
+    if ((s.t + s.h - tf) * dir > 0.0) {
+      double h_new = tf - s.t;
+      if (fabs(h_new) < de::kTenEps) break;
+      s.h = h_new;
+    }
+    int64_t sev = 0;
+    int e = scheme == DE_SCHEME_MILSTEIN ? mil_step(s, sys, noise, max_steps, &sev)
+                                         : em_step(s, sys, noise, max_steps, &sev);
+    if (e >= 0) {
+      r.status = e;
+      r.t = s.t;
+      copy(y_out, s.y, n);
+      return r;
+    }
+    r.evals += sev;
+    r.accepted += 1;
+    if (path_stride > 0 && path_rec && r.accepted % path_stride == 0) {
+      int64_t w = r.accepted / path_stride - 1;
+      for (int c = 0; c < n; ++c) path_rec[w * path_stride_elems + c] = s.y[c];
+      if (path_t) path_t[w] = s.t;
+    }
+    if (fabs(tf - s.t) <= de::kTenEps) break;
+  }
+  r.status = DE_STATUS_COMPLETE;
+  r.t = s.t;
+  copy(y_out, s.y, n);
+  return r;
+}
+
+// --------------------------------------------------------- ensemble glue ---
+template <class Fn>
+static void parallel_for(int64_t n, int threads, Fn&& fn) {
+  if (threads <= 1 || n < 2) {
+    for (int64_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int64_t> next(0);
+  const int64_t chunk = 8;
+  std::vector<std::thread> pool;
+  for (int w = 0; w < threads; ++w) {
+    pool.emplace_back([&]() {
+      for (;;) {
+        int64_t b = next.fetch_add(chunk);
+        if (b >= n) break;
+        int64_t e = std::min(n, b + chunk);
+        for (int64_t i = b; i < e; ++i) fn(i);
+      }
+    });
+  }
+  for (auto& t : pool) t.join();
+}
+
+template <class Sys>
+static void run_ode_traj(const Sys& sys, const de_ode_batch* b, const Settings& st, int64_t i,
+                         const double* y0, double* y_out, double* dense, int32_t* n_dense,
+                         TrajResult* res) {
+  double t0 = b->t0 ? b->t0[i] : b->t0_scalar;
+  double tf = b->tf ? b->tf[i] : b->tf_scalar;
+  TEval te;
+  TEval* tep = nullptr;
+  if (b->n_eval > 0 && dense) {
+    te.t_evals = b->t_eval;
+    te.n_eval = b->n_eval;
+    te.dense = dense;
+    tep = &te;
+  }
+  if (b->method == DE_METHOD_RK4 || b->method == DE_METHOD_EULER) {
+    FixedErk<Sys> m(sys, st, b->method == DE_METHOD_RK4 ? kRK4 : kEuler);
+    *res = solve_ode(m, t0, tf, y0, tep, y_out);
+  } else {
+    Dopri5<Sys> m(sys, st);
+    *res = solve_ode(m, t0, tf, y0, tep, y_out);
+  }
+  if (n_dense) *n_dense = tep ? tep->pushed : 0;
+}
+
+template <class F>
+static void run_fixed_family(const de_ode_batch* b, de_ode_result* out, const Settings& base,
+                             int threads) {
+  const int64_t n = b->n;
+  constexpr int D = F::kDim;
+  parallel_for(n, threads, [&](int64_t i) {
+    FixedSys<F> sys;
+    sys.f.load(b->params, n, i);
+    Settings st = base;
+    if (b->rtol_traj) st.rtol = b->rtol_traj[i];
+    if (b->atol_traj) st.atol = b->atol_traj[i];
+    double y0[D], yo[D];
+    for (int c = 0; c < D; ++c) y0[c] = b->y0[c * n + i];
+    std::vector<double> dense;
+    if (b->n_eval > 0 && out->dense) dense.assign((size_t)b->n_eval * D, NAN);
+    TrajResult r;
+    run_ode_traj(sys, b, st, i, y0, yo, dense.empty() ? nullptr : dense.data(),
+                 out->n_dense ? &out->n_dense[i] : nullptr, &r);
+    for (int c = 0; c < D; ++c) out->y_final[c * n + i] = yo[c];
+    if (!dense.empty())
+      for (int e = 0; e < b->n_eval; ++e)
+        for (int c = 0; c < D; ++c) out->dense[((int64_t)e * D + c) * n + i] = dense[(size_t)e * D + c];
+    out->t_final[i] = r.t;
+    out->accepted[i] = r.accepted;
+    out->rejected[i] = r.rejected;
+    out->evals[i] = r.evals;
+    out->status[i] = r.status;
+  });
+}
+
+static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings& base,
+                       int threads) {
+  const int64_t n = b->n;
+  parallel_for(n, threads, [&](int64_t i) {
+    SpringSys sys;
+    sys.nmass = b->sizes[i];
+    sys.params = b->params + b->param_offset[i];
+    Settings st = base;
+    if (b->rtol_traj) st.rtol = b->rtol_traj[i];
+    if (b->atol_traj) st.atol = b->atol_traj[i];
+    const int64_t off = b->state_offset[i];
+    double* dense = nullptr;
+    if (b->n_eval > 0 && out->dense) {
+      dense = out->dense + (int64_t)b->n_eval * off;
+      for (int64_t e = 0; e < (int64_t)b->n_eval * sys.dim(); ++e) dense[e] = NAN;
+    }
+    TrajResult r;
+    run_ode_traj(sys, b, st, i, b->y0 + off, out->y_final + off, dense,
+                 out->n_dense ? &out->n_dense[i] : nullptr, &r);
+    out->t_final[i] = r.t;
+    out->accepted[i] = r.accepted;
+    out->rejected[i] = r.rejected;
+    out->evals[i] = r.evals;
+    out->status[i] = r.status;
+  });
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Oracle batched ODE solve over host memory.  pow_mode: 0 project pow,
+// 1 C library pow.  threads: worker threads (1 = sequential).
+int de_oracle_ode_solve(const de_ode_batch* b, de_ode_result* out, int pow_mode, int threads) {
+  using namespace orc;
+  g_pow_mode = pow_mode;
+  Settings st{b->rtol, b->atol, b->h0, b->h_min, b->h_max, b->safety_factor,
+              b->min_scale, b->max_scale, b->max_steps};
+  switch (b->family) {
+    case DE_FAMILY_LORENZ: run_fixed_family<de::LorenzRhs>(b, out, st, threads); break;
+    case DE_FAMILY_VAN_DER_POL: run_fixed_family<de::VanDerPolRhs>(b, out, st, threads); break;
+    case DE_FAMILY_EXPONENTIAL: run_fixed_family<de::ExponentialRhs>(b, out, st, threads); break;
+    case DE_FAMILY_LINEAR: run_fixed_family<de::LinearRhs>(b, out, st, threads); break;
+    case DE_FAMILY_HARMONIC: run_fixed_family<de::HarmonicRhs>(b, out, st, threads); break;
+    case DE_FAMILY_LOGISTIC: run_fixed_family<de::LogisticRhs>(b, out, st, threads); break;
+    case DE_FAMILY_ROBERTSON: run_fixed_family<de::RobertsonRhs>(b, out, st, threads); break;
+    case DE_FAMILY_SPRING_CHAIN: run_spring(b, out, st, threads); break;
+    default: return DE_ERR_UNSUPPORTED;
+  }
+  return DE_OK;
+}
+
+// Oracle batched SDE solve over host memory.
+int de_oracle_sde_solve(const de_sde_batch* b, de_sde_result* out, int threads) {
+  using namespace orc;
+  const int64_t n = b->n;
+  const int dim = b->dim;
+  if (dim > kSdeMax) return DE_ERR_UNSUPPORTED;
+  const int P = b->family == DE_SDE_ORNSTEIN_UHLENBECK ? 3 * dim : 2 * dim;
+  parallel_for(n, threads, [&](int64_t i) {
+    std::vector<double> p(P);
+    for (int q = 0; q < P; ++q) p[q] = b->params[(int64_t)q * n + i];
+    SdeSys sys{b->family, dim, p.data()};
+    Noise noise{b->noise, b->seed, b->path_base + (uint64_t)i, b->noise_table, b->noise_steps,
+                b->noise_constant, dim};
+    double y0[kSdeMax], yo[kSdeMax];
+    for (int c = 0; c < dim; ++c) y0[c] = b->y0[(int64_t)c * n + i];
+    double* path_rec = out->path ? out->path + i : nullptr;
+    // path layout [w][c][i]: element (w, c) of path i at (w * dim + c) * n + i;
+    // solve_sde_one writes rec[w * stride_elems + c] so pass a strided view.
+    std::vector<double> rec;
+    TrajResult r;
+    if (path_rec && b->path_stride > 0) {
+      int64_t maxw = b->max_steps / b->path_stride + 1;
+      rec.assign((size_t)(maxw * dim), NAN);
+      r = solve_sde_one(b->scheme, sys, noise, b->t0, b->tf, b->h0, b->h_min, b->h_max,
+                        b->max_steps, y0, yo, b->path_stride, rec.data(), dim,
+                        (i == 0 && out->path_t) ? out->path_t : nullptr);
+      int64_t nw = r.accepted / b->path_stride;
+      for (int64_t w = 0; w < nw; ++w)
+        for (int c = 0; c < dim; ++c) out->path[(w * dim + c) * n + i] = rec[(size_t)(w * dim + c)];
+    } else {
+      r = solve_sde_one(b->scheme, sys, noise, b->t0, b->tf, b->h0, b->h_min, b->h_max,
+                        b->max_steps, y0, yo, 0, nullptr, 0, nullptr);
+    }
+    for (int c = 0; c < dim; ++c) out->y_final[(int64_t)c * n + i] = yo[c];
+    out->t_final[i] = r.t;
+    out->steps[i] = r.accepted;
+    out->evals[i] = r.evals;
+    out->status[i] = r.status;
+  });
+  return DE_OK;
+}
+
+// The oracle's DOPRI5 tableau: c[7], a[7][7], b[7], er[7], bi row 0 [7].
+void de_oracle_dopri5_tableau(double* c, double* a, double* b, double* er, double* bi0) {
+  using orc::kDP;
+  for (int i = 0; i < 7; ++i) {
+    c[i] = kDP.c[i];
+    b[i] = kDP.b[i];
+    er[i] = kDP.er[i];
+    bi0[i] = kDP.bi[0][i];
+    for (int j = 0; j < 7; ++j) a[i * 7 + j] = kDP.a[i][j];
+  }
+}
+
+double de_oracle_pow(double x, double y) { return de::pow(x, y); }
+
+void de_oracle_pow_many(const double* x, const double* y, double* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = de::pow(x[i], y[i]);
+}
+
+double de_oracle_noise_normal(uint64_t seed, uint64_t path, uint32_t step, uint32_t comp) {
+  return de::noise_normal(seed, path, step, comp);
+}
+
+void de_oracle_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+  de::u32x4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  de::u32x4 w = de::philox4x32_10(c, k0, k1);
+  for (int i = 0; i < 4; ++i) out[i] = w.v[i];
+}
+
+}  // extern "C"

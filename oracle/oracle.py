@@ -1,0 +1,94 @@
+"""Loader for the CPU oracle (libde_oracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and the benchmark's CPU-baseline leg
+as the checker; the product package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(REPO, "differential-equations_amd"))
+
+from deamd import _abi  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "_build", "libde_oracle.so")
+POW_PROJECT = 0
+POW_LIBM = 1
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        lib.de_oracle_ode_solve.restype = C.c_int
+        lib.de_oracle_ode_solve.argtypes = [C.POINTER(_abi.OdeBatch), C.POINTER(_abi.OdeResult),
+                                            C.c_int, C.c_int]
+        lib.de_oracle_sde_solve.restype = C.c_int
+        lib.de_oracle_sde_solve.argtypes = [C.POINTER(_abi.SdeBatch), C.POINTER(_abi.SdeResult),
+                                            C.c_int]
+        lib.de_oracle_pow.restype = C.c_double
+        lib.de_oracle_pow.argtypes = [C.c_double, C.c_double]
+        lib.de_oracle_pow_many.restype = None
+        lib.de_oracle_pow_many.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        lib.de_oracle_noise_normal.restype = C.c_double
+        lib.de_oracle_noise_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+        lib.de_oracle_philox.restype = None
+        lib.de_oracle_philox.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        lib.de_oracle_dopri5_tableau.restype = None
+        lib.de_oracle_dopri5_tableau.argtypes = [C.c_void_p] * 5
+        _lib = lib
+    return _lib
+
+
+def dopri5_tableau():
+    import numpy as np
+    c, b, er, bi0 = (np.empty(7) for _ in range(4))
+    a = np.empty(49)
+    load().de_oracle_dopri5_tableau(c.ctypes.data, a.ctypes.data, b.ctypes.data,
+                                    er.ctypes.data, bi0.ctypes.data)
+    return c, a.reshape(7, 7), b, er, bi0
+
+
+def solve_ode(ensemble, pow_mode=POW_PROJECT, threads=1):
+    """Run an deamd.OdeEnsemble through the oracle; returns a deamd.Solution."""
+    b, res, keep = ensemble.batch()
+    rc = load().de_oracle_ode_solve(C.byref(b), C.byref(res.struct()), pow_mode, threads)
+    if rc != 0:
+        raise RuntimeError("oracle ode solve failed: %d" % rc)
+    return res
+
+
+def solve_sde(ensemble, threads=1):
+    b, res, keep = ensemble.batch()
+    rc = load().de_oracle_sde_solve(C.byref(b), C.byref(res.struct()), threads)
+    if rc != 0:
+        raise RuntimeError("oracle sde solve failed: %d" % rc)
+    return res
+
+
+def pow_many(x, y):
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    out = np.empty_like(x)
+    load().de_oracle_pow_many(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def philox(ctr, k0, k1):
+    import numpy as np
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    out = np.empty(4, dtype=np.uint32)
+    load().de_oracle_philox(c.ctypes.data, k0, k1, out.ctypes.data)
+    return out

@@ -1,0 +1,243 @@
+"""GPU parity: the HIP kernels against the CPU oracle on the same inputs.
+
+The bar is bit equality (final state, final time, accepted/rejected/evals
+counts, status, dense output) for every trajectory: the kernels use the
+reference's operation order with contraction off and the same pow, so any
+difference is a bug.  The FMA-contracted fast build is checked separately
+against looser, documented bounds.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import deamd
+from deamd import (DE_NOISE_CONSTANT, DE_NOISE_PHILOX, DE_NOISE_TABLE, DE_SCHED_PERSISTENT,
+                   DE_SCHED_PLAIN, DE_STATUS_BAD_INPUT, DE_STATUS_COMPLETE, DE_STATUS_MAX_STEPS,
+                   EulerMaruyama, ExplicitRungeKutta, IVP, Lorenz, Milstein, VanDerPol)
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def same(a, b):
+    """Bitwise equality of float arrays (NaN == NaN; +0 == -0 as values)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def assert_ode_equal(got, want, dense=False):
+    assert np.array_equal(got.status, want.status)
+    assert np.array_equal(got.accepted, want.accepted)
+    assert np.array_equal(got.rejected, want.rejected)
+    assert np.array_equal(got.evals, want.evals)
+    assert same(got.t, want.t)
+    assert same(got.y, want.y)
+    if dense:
+        assert np.array_equal(got.n_dense, want.n_dense)
+        assert same(got.dense, want.dense)
+
+
+# ------------------------------------------------------- device arithmetic --
+def test_device_pow_matches_host(orc):
+    lib = deamd.lib()
+    rng = np.random.default_rng(1)
+    n = 1 << 20
+    x = np.concatenate([np.exp(rng.uniform(np.log(1e-8), np.log(1e4), n)),
+                        np.exp(rng.uniform(np.log(1e-300), np.log(1e300), n // 4))])
+    for y in (-0.2, 0.2):
+        yy = np.full_like(x, y)
+        out = np.empty_like(x)
+        deamd._abi.check(lib.de_diag_pow(deamd._ptr(x), deamd._ptr(yy), deamd._ptr(out), len(x)))
+        assert np.array_equal(out, orc.pow_many(x, y))
+
+
+def test_device_division_sqrt_fma_are_ieee():
+    lib = deamd.lib()
+    rng = np.random.default_rng(2)
+    n = 1 << 20
+    a = np.exp(rng.uniform(-700, 700, n)) * rng.choice([-1, 1], n)
+    b = np.exp(rng.uniform(-700, 700, n)) * rng.choice([-1, 1], n)
+    c = rng.standard_normal(n)
+    a[:4] = [0.0, 1.0, 4.0, 2.0]
+    b[:4] = [1.0, 3.0, 7.0, 3.0]
+    out = np.empty(3 * n)
+    deamd._abi.check(lib.de_diag_arith(deamd._ptr(a), deamd._ptr(b), deamd._ptr(c),
+                                       deamd._ptr(out), n))
+    out = out.reshape(n, 3)
+    assert np.array_equal(out[:, 0], a / b)
+    pos = a > 0
+    assert np.array_equal(out[pos, 1], np.sqrt(a[pos]))
+    import mpmath  # fma reference: exact product-sum rounded once
+    for i in range(0, n, n // 2000):
+        want = float(mpmath.fadd(mpmath.fmul(a[i], b[i], exact=True), c[i], prec=53,
+                                 rounding="n"))
+        assert out[i, 2] == want
+
+
+def test_device_noise_matches_host():
+    lib = deamd.lib()
+    n = 200000
+    rng = np.random.default_rng(3)
+    path = rng.integers(0, 1 << 40, n, dtype=np.uint64)
+    step = rng.integers(0, 1 << 20, n, dtype=np.uint32)
+    comp = rng.integers(0, 9, n, dtype=np.uint32)
+    out = np.empty(n)
+    deamd._abi.check(lib.de_diag_noise(77, deamd._ptr(path), deamd._ptr(step), deamd._ptr(comp),
+                                       deamd._ptr(out), n))
+    want = np.array([lib.de_noise_normal(77, int(p), int(s), int(c))
+                     for p, s, c in zip(path[:5000], step[:5000], comp[:5000])])
+    assert np.array_equal(out[:5000], want)
+
+
+# ------------------------------------------------------------ Lorenz (C1) --
+def lorenz_c1(n=1024, tf=20.0):
+    p, y0 = deamd.gen_lorenz(n)
+    return IVP.ode(Lorenz(*p), 0.0, tf, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(1000000))
+
+
+@pytest.fixture(scope="module")
+def c1_oracle(orc):
+    return orc.solve_ode(lorenz_c1(), threads=THREADS)
+
+
+@pytest.mark.parametrize("schedule", [DE_SCHED_PLAIN, DE_SCHED_PERSISTENT])
+def test_lorenz_c1_bit_exact(c1_oracle, schedule):
+    got = lorenz_c1().options(schedule=schedule).solve()
+    assert (got.status == DE_STATUS_COMPLETE).all()
+    assert_ode_equal(got, c1_oracle)
+
+
+def test_lorenz_persistent_large_batch_matches_plain():
+    # 65 536 trajectories (several refills per lane) over a short span.
+    p, y0 = deamd.gen_lorenz(65536, seed=11)
+    mk = lambda s: IVP.ode(Lorenz(*p), 0.0, 2.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(100000)).options(schedule=s)
+    a = mk(DE_SCHED_PLAIN).solve()
+    b = mk(DE_SCHED_PERSISTENT).solve()
+    assert_ode_equal(b, a)
+
+
+# ------------------------------------------------ reference test systems ----
+from test_oracle_reference_kats import ACCURACY  # noqa: E402
+
+
+@pytest.mark.parametrize("case", ACCURACY, ids=[c[0] for c in ACCURACY])
+def test_reference_systems_bit_exact(orc, case):
+    name, system, t0, tf, y0, expected, methods = case
+    for make_method, tol in methods:
+        ens = IVP.ode(system, t0, tf, y0, n=3).method(make_method())
+        want = orc.solve_ode(ens)
+        got = ens.solve()
+        assert_ode_equal(got, want)
+        for c, e in enumerate(expected):
+            assert abs(got.y[c * 3] - e) < tol
+
+
+@pytest.mark.parametrize("make_method", [
+    lambda: ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(1000000),
+    lambda: ExplicitRungeKutta.rk4(0.01).max_steps(1000000),
+    lambda: ExplicitRungeKutta.euler(0.001).max_steps(1000000)])
+def test_dense_output_bit_exact(orc, make_method):
+    p, y0 = deamd.gen_lorenz(256, seed=5)
+    pts = np.linspace(0.0, 5.0, 64)
+    ens = IVP.ode(Lorenz(*p), 0.0, 5.0, y0).t_eval(pts).method(make_method())
+    want = orc.solve_ode(ens, threads=THREADS)
+    got = ens.solve()
+    assert_ode_equal(got, want, dense=True)
+
+
+def test_vdp_backward_and_per_trajectory_tolerances(orc):
+    mu, y0 = deamd.gen_vdp(512, seed=9)
+    ens = IVP.ode(VanDerPol(*mu), 5.0, -3.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9).max_steps(200000))
+    ens.rtol_traj = np.where(np.arange(512) % 2 == 0, 1e-6, 1e-8)
+    ens.atol_traj = ens.rtol_traj * 1e-2
+    want = orc.solve_ode(ens, threads=THREADS)
+    got = ens.solve()
+    assert_ode_equal(got, want)
+
+
+def test_error_statuses_bit_exact(orc):
+    p, y0 = deamd.gen_lorenz(256, seed=13)
+    p[1, :8] = np.nan                      # NaN rho: rejected forever -> StepSize
+    y0[:, 8:16] = np.inf
+    ens = IVP.ode(Lorenz(*p), 0.0, 20.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(300))
+    want = orc.solve_ode(ens, threads=THREADS)
+    got = ens.solve()
+    assert_ode_equal(got, want)
+    assert (got.status == DE_STATUS_MAX_STEPS).sum() > 100
+    # BadInput: h0 larger than the interval, and tf == t0 per trajectory
+    tf = np.full(256, 1.0)
+    tf[::3] = 0.0
+    ens = IVP.ode(Lorenz(*p), 0.0, tf, y0).method(ExplicitRungeKutta.dopri5().h0(0.5))
+    want = orc.solve_ode(ens, threads=THREADS)
+    got = ens.solve()
+    assert_ode_equal(got, want)
+    assert (got.status[::3] == DE_STATUS_BAD_INPUT).all()
+
+
+def test_stiffness_detection_bit_exact(orc):
+    # The reference's stiffness test (ordinary.rs:165-194) compares
+    # |yseg - k6|^2 with |f(y_new) - y_stage6|^2; for f(y) = y the second is
+    # rounding-level, so h_lamb exceeds 6.1 on most checks and the solve stops
+    # with Error::Stiffness after 15 of them.
+    ens = IVP.ode(deamd.Exponential(1.0), 0.0, 40.0, [1.0], n=8).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-13).atol(1e-13).max_steps(200000))
+    want = orc.solve_ode(ens)
+    got = ens.solve()
+    assert (want.status == deamd.DE_STATUS_STIFFNESS).all()
+    assert_ode_equal(got, want)
+
+
+# -------------------------------------------------------------------- SDE ---
+@pytest.mark.parametrize("scheme", [EulerMaruyama, Milstein])
+@pytest.mark.parametrize("family", ["ou8", "gbm"])
+def test_sde_bit_exact(orc, scheme, family):
+    n = 4096
+    if family == "ou8":
+        sysm, y0 = deamd.gen_ou(n, 8, seed=21)
+    else:
+        sysm, y0 = deamd.gen_gbm(n)
+    ens = IVP.sde(sysm, 0.0, 1.0, y0, noise=DE_NOISE_PHILOX, seed=0x5EED2026, path_base=1000) \
+        .method(scheme(1e-3).max_steps(2000)).record_every(100)
+    want = orc.solve_sde(ens, threads=THREADS)
+    got = ens.solve()
+    assert np.array_equal(got.status, want.status)
+    assert (got.status == DE_STATUS_COMPLETE).all()
+    assert np.array_equal(got.steps, want.steps) and (got.steps == 1000).all()
+    assert np.array_equal(got.evals, want.evals)
+    assert same(got.t, want.t) and same(got.y, want.y)
+    assert same(got.path, want.path) and same(got.path_t, want.path_t)
+
+
+def test_sde_table_and_constant_noise(orc):
+    dw = np.array([[0.1, -0.1], [-0.05, 0.2], [0.15, -0.05], [0.0, 0.1]])
+    sysm = deamd.LinearDiagonal(2, np.tile(np.array([-1.0, 0.5, -2.0, 0.1])[:, None], 8))
+    ens = IVP.sde(sysm, 0.0, 1.0, np.ones(16), noise=DE_NOISE_TABLE, noise_table=dw) \
+        .method(Milstein(0.25))
+    want = orc.solve_sde(ens)
+    got = ens.solve()
+    assert same(got.y, want.y) and (got.steps == 4).all()
+    sysm = deamd.LinearDiagonal(1, np.tile(np.array([0.5, 0.1])[:, None], 4))
+    ens = IVP.sde(sysm, 0.0, 0.1, np.ones(4), noise=DE_NOISE_CONSTANT, noise_constant=0.1) \
+        .method(EulerMaruyama(0.01))
+    want = orc.solve_sde(ens)
+    got = ens.solve()
+    assert same(got.y, want.y) and (np.abs(got.t - 0.1) < 1e-10).all()
+
+
+# -------------------------------------------------------------- fast build --
+def test_fast_build_close_on_nonchaotic_family(orc):
+    mu, y0 = deamd.gen_vdp(2048, seed=4)
+    ens = IVP.ode(VanDerPol(*mu), 0.0, 10.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(1000000))
+    want = orc.solve_ode(ens, threads=THREADS)
+    got = ens.options(fast_math=True).solve()
+    assert (got.status == DE_STATUS_COMPLETE).all()
+    rel = np.abs(got.y - want.y) / np.maximum(np.abs(want.y), 1e-3)
+    assert np.percentile(rel, 99) < 1e-6

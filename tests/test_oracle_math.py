@@ -1,0 +1,298 @@
+"""Pins of the CPU oracle by mathematics and independent restatement.
+
+Each check is chosen so that a plausible mistake in the oracle (a wrong
+coefficient, a dropped term, a transposed index, a wrong operation order)
+fails it:
+  * DOPRI5 tableau: classical order conditions (order 5 for b, order 4 for the
+    embedded weights b - er) in exact rationals;
+  * one step on y' = lam*y equals R(z) y0 with R the DOPRI5 stability
+    polynomial 1 + z + ... + z^5/120 + z^6/600 (textbook);
+  * RK4 / Euler steps equal the Taylor polynomials of degree 4 / 1;
+  * dense output on a single DOPRI5 step is the cubic Hermite interpolant of
+    the step's end values and derivatives (the reference's bi[4] row is zero);
+  * evaluation-count invariants of the driver;
+  * bit equality with an independent pure-Python restatement (tests/pyref_dopri5.py)
+    on Lorenz trajectories, using the C library pow as the reference does;
+  * SDE: closed-form recurrences for linear SDEs, Milstein == Euler-Maruyama
+    for additive noise, the clipped last step of [0, 1] at dt = 1e-3;
+  * Philox4x32-10 known-answer vectors (Random123), pow correctly rounded
+    against mpmath, normal-transform moments.
+"""
+import math
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+import pyref_dopri5
+from deamd import (DE_NOISE_PHILOX, DE_NOISE_TABLE, DE_STATUS_COMPLETE, EulerMaruyama,
+                   ExplicitRungeKutta, Exponential, IVP, Lorenz, LinearDiagonal, Milstein,
+                   OrnsteinUhlenbeck)
+
+# DOPRI5 rationals (Dormand & Prince 1980; src/tableau/dorman_prince.rs:44-101)
+DP_C = [Fr(0), Fr(1, 5), Fr(3, 10), Fr(4, 5), Fr(8, 9), Fr(1), Fr(1)]
+DP_A = [[Fr(0)] * 7 for _ in range(7)]
+DP_A[1][0] = Fr(1, 5)
+DP_A[2][:2] = [Fr(3, 40), Fr(9, 40)]
+DP_A[3][:3] = [Fr(44, 45), Fr(-56, 15), Fr(32, 9)]
+DP_A[4][:4] = [Fr(19372, 6561), Fr(-25360, 2187), Fr(64448, 6561), Fr(-212, 729)]
+DP_A[5][:5] = [Fr(9017, 3168), Fr(-355, 33), Fr(46732, 5247), Fr(49, 176), Fr(-5103, 18656)]
+DP_A[6][:6] = [Fr(35, 384), Fr(0), Fr(500, 1113), Fr(125, 192), Fr(-2187, 6784), Fr(11, 84)]
+DP_B = [Fr(35, 384), Fr(0), Fr(500, 1113), Fr(125, 192), Fr(-2187, 6784), Fr(11, 84), Fr(0)]
+DP_BHAT = [Fr(5179, 57600), Fr(0), Fr(7571, 16695), Fr(393, 640), Fr(-92097, 339200),
+           Fr(187, 2100), Fr(1, 40)]
+
+
+def order_conditions(b, A, c, order):
+    """Butcher order conditions up to `order` (all rooted trees up to order 5)."""
+    s = len(b)
+    Ac = [sum(A[i][j] * c[j] for j in range(s)) for i in range(s)]
+    Ac2 = [sum(A[i][j] * c[j] ** 2 for j in range(s)) for i in range(s)]
+    AAc = [sum(A[i][j] * Ac[j] for j in range(s)) for i in range(s)]
+    Ac3 = [sum(A[i][j] * c[j] ** 3 for j in range(s)) for i in range(s)]
+    cAc = [c[i] * Ac[i] for i in range(s)]
+    AcAc = [sum(A[i][j] * cAc[j] for j in range(s)) for i in range(s)]
+    AAc2 = [sum(A[i][j] * Ac2[j] for j in range(s)) for i in range(s)]
+    AAAc = [sum(A[i][j] * AAc[j] for j in range(s)) for i in range(s)]
+    dot = lambda u, v: sum(x * y for x, y in zip(u, v))
+    conds = [
+        (1, dot(b, [1] * s), Fr(1)),
+        (2, dot(b, c), Fr(1, 2)),
+        (3, dot(b, [x ** 2 for x in c]), Fr(1, 3)),
+        (3, dot(b, Ac), Fr(1, 6)),
+        (4, dot(b, [x ** 3 for x in c]), Fr(1, 4)),
+        (4, dot(b, [c[i] * Ac[i] for i in range(s)]), Fr(1, 8)),
+        (4, dot(b, Ac2), Fr(1, 12)),
+        (4, dot(b, AAc), Fr(1, 24)),
+        (5, dot(b, [x ** 4 for x in c]), Fr(1, 5)),
+        (5, dot(b, [c[i] ** 2 * Ac[i] for i in range(s)]), Fr(1, 10)),
+        (5, dot(b, [c[i] * Ac2[i] for i in range(s)]), Fr(1, 15)),
+        (5, dot(b, [c[i] * AAc[i] for i in range(s)]), Fr(1, 30)),
+        (5, dot(b, [Ac[i] ** 2 for i in range(s)]), Fr(1, 20)),
+        (5, dot(b, Ac3), Fr(1, 20)),
+        (5, dot(b, AcAc), Fr(1, 40)),
+        (5, dot(b, AAc2), Fr(1, 60)),
+        (5, dot(b, AAAc), Fr(1, 120)),
+    ]
+    return [(o, got, want) for (o, got, want) in conds if o <= order]
+
+
+def test_dopri5_rationals_satisfy_order_conditions():
+    for o, got, want in order_conditions(DP_B, DP_A, DP_C, 5):
+        assert got == want, (o, got, want)
+    for o, got, want in order_conditions(DP_BHAT, DP_A, DP_C, 4):
+        assert got == want, (o, got, want)
+    # row sums = c
+    for i in range(7):
+        assert sum(DP_A[i]) == DP_C[i]
+
+
+def test_oracle_tableau_is_dopri5(orc):
+    c, a, b, er, bi0 = orc.dopri5_tableau()
+    for i in range(7):
+        assert c[i] == float(DP_C[i])
+        assert b[i] == float(DP_B[i])
+        assert er[i] == float(DP_B[i] - DP_BHAT[i])
+        for j in range(7):
+            assert a[i, j] == float(DP_A[i][j])
+
+
+def stability_poly(b, A):
+    """R(z) = 1 + z b^T (I - zA)^{-1} 1 as exact coefficients (explicit A)."""
+    s = len(b)
+    coeffs = [Fr(1)]
+    # b^T A^k 1 for k = 0..s-1
+    v = [Fr(1)] * s
+    for k in range(s):
+        coeffs.append(sum(b[i] * v[i] for i in range(s)))
+        v = [sum(A[i][j] * v[j] for j in range(s)) for i in range(s)]
+    return coeffs
+
+
+def test_dopri5_stability_polynomial_is_textbook():
+    R = stability_poly(DP_B, DP_A)
+    assert R == [Fr(1), Fr(1), Fr(1, 2), Fr(1, 6), Fr(1, 24), Fr(1, 120), Fr(1, 600), Fr(0)]
+
+
+@pytest.mark.parametrize("lam,h", [(1.0, 0.1), (-2.0, 0.05), (0.5, 0.3)])
+def test_dopri5_single_step_is_R_of_z(orc, lam, h):
+    # tf = t0 + h with h0 = h: exactly one attempt; loose tolerances accept it.
+    ens = IVP.ode(Exponential(lam), 0.0, h, [1.0], n=1) \
+        .method(ExplicitRungeKutta.dopri5().h0(h).rtol(1.0).atol(1.0))
+    sol = orc.solve_ode(ens)
+    assert sol.accepted[0] == 1 and sol.rejected[0] == 0
+    z = Fr(lam) * Fr(h)
+    R = sum(c * z ** k for k, c in enumerate(stability_poly(DP_B, DP_A)))
+    assert abs(sol.y[0] - float(R)) <= 4e-16 * abs(float(R))
+
+
+@pytest.mark.parametrize("method,taylor", [
+    (lambda h: ExplicitRungeKutta.rk4(h), 4), (lambda h: ExplicitRungeKutta.euler(h), 1)])
+def test_fixed_single_step_is_taylor(orc, method, taylor):
+    lam, h = -1.5, 0.1
+    ens = IVP.ode(Exponential(lam), 0.0, h, [1.0], n=1).method(method(h))
+    sol = orc.solve_ode(ens)
+    assert sol.accepted[0] == 1
+    z = Fr(lam) * Fr(h)
+    R = sum(z ** k / math.factorial(k) for k in range(taylor + 1))
+    assert abs(sol.y[0] - float(R)) <= 4e-16
+
+
+def test_dopri5_dense_output_is_cubic_hermite(orc):
+    lam, h = 0.7, 0.2
+    pts = [0.0, 0.05, 0.1, 0.13, 0.2]
+    ens = IVP.ode(Exponential(lam), 0.0, h, [1.0], n=1).t_eval(pts) \
+        .method(ExplicitRungeKutta.dopri5().h0(h).rtol(1.0).atol(1.0))
+    sol = orc.solve_ode(ens)
+    assert sol.accepted[0] == 1 and sol.n_dense[0] == len(pts)
+    y0 = Fr(1)
+    y1 = Fr(sol.y[0])
+    f0, f1 = Fr(lam) * y0, Fr(lam) * y1
+    H = Fr(h)
+    for e, tp in enumerate(pts):
+        s = Fr(tp) / H
+        herm = ((2 * s ** 3 - 3 * s ** 2 + 1) * y0 + (s ** 3 - 2 * s ** 2 + s) * H * f0 +
+                (-2 * s ** 3 + 3 * s ** 2) * y1 + (s ** 3 - s ** 2) * H * f1)
+        assert abs(sol.dense[e] - float(herm)) <= 1e-15, (tp, sol.dense[e], float(herm))
+
+
+def lorenz_ensemble(n, tf, rtol=1e-8, atol=1e-10, seed=7):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0, 50, n)
+    y0 = 1.0 + 0.1 * rng.standard_normal((3, n))
+    return IVP.ode(Lorenz(10.0, rho, 8.0 / 3.0), 0.0, tf, y0) \
+        .method(ExplicitRungeKutta.dopri5().rtol(rtol).atol(atol).max_steps(100000))
+
+
+def test_evaluation_count_invariants(orc):
+    sol = orc.solve_ode(lorenz_ensemble(64, 2.0))
+    assert (sol.status == DE_STATUS_COMPLETE).all()
+    assert (sol.evals == 3 + 6 * (sol.accepted + sol.rejected) + sol.accepted).all()
+    assert (np.abs(sol.t - 2.0) <= 10 * 2.220446049250313e-16).all()
+    ens = lorenz_ensemble(16, 1.0).method(ExplicitRungeKutta.rk4(0.01))
+    sol = orc.solve_ode(ens)
+    assert (sol.evals == 1 + 4 * sol.accepted).all()
+    ens = lorenz_ensemble(16, 1.0).method(ExplicitRungeKutta.euler(0.01))
+    sol = orc.solve_ode(ens)
+    assert (sol.evals == 1 + sol.accepted).all()
+
+
+@pytest.mark.parametrize("traj", range(6))
+def test_oracle_bit_equals_python_restatement(orc, traj):
+    rng = np.random.default_rng(100 + traj)
+    rho = float(rng.uniform(0, 50))
+    y0 = list(1.0 + 0.1 * rng.standard_normal(3))
+    tf = 1.5
+
+    def f(t, y, dy):
+        x, yy, z = y
+        dy[0] = 10.0 * (yy - x)
+        dy[1] = x * (rho - z) - yy
+        dy[2] = x * yy - (8.0 / 3.0) * z
+
+    want = pyref_dopri5.solve(f, 0.0, tf, y0, 1e-8, 1e-10)
+    ens = IVP.ode(Lorenz(10.0, rho, 8.0 / 3.0), 0.0, tf, np.array(y0).reshape(3, 1)) \
+        .method(ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(100000))
+    got = orc.solve_ode(ens, pow_mode=1)
+    assert list(got.y) == want[0]
+    assert got.t[0] == want[1]
+    assert (got.accepted[0], got.rejected[0], got.evals[0], got.status[0]) == want[2:]
+
+
+def test_sde_last_step_is_clipped(orc):
+    # [0, 1] at dt = 1e-3: 1000 steps, the last one h = 1 - t_999.
+    t = 0.0
+    for _ in range(999):
+        t += 1e-3
+    h_last = 1.0 - t
+    assert h_last == 0.0009999999999992237
+    sys_ = LinearDiagonal(1, [0.0, 0.0])
+    ens = IVP.sde(sys_, 0.0, 1.0, [1.0], noise=DE_NOISE_PHILOX, seed=1) \
+        .method(EulerMaruyama(1e-3)).record_every(1)
+    sol = orc.solve_sde(ens)
+    assert sol.steps[0] == 1000 and sol.t[0] == 1.0
+    assert sol.path_t[998] == t and sol.path_t[999] == 1.0
+    assert sol.evals[0] == 2 + 2 * 1000
+    sol = orc.solve_sde(ens.method(Milstein(1e-3)))
+    assert sol.evals[0] == 1 + 3 * 1000
+
+
+def test_milstein_equals_euler_maruyama_for_additive_noise(orc):
+    n, d = 64, 8
+    rng = np.random.default_rng(3)
+    p = np.stack([rng.uniform(0.5, 2, (d, n)), rng.standard_normal((d, n)),
+                  rng.uniform(0.1, 1, (d, n))], axis=1).reshape(3 * d, n)
+    sys_ = OrnsteinUhlenbeck(d, p)
+    y0 = np.zeros(d * n)
+    em = orc.solve_sde(IVP.sde(sys_, 0.0, 0.5, y0, seed=11).method(EulerMaruyama(1e-2)))
+    mil = orc.solve_sde(IVP.sde(sys_, 0.0, 0.5, y0, seed=11).method(Milstein(1e-2)))
+    assert np.array_equal(em.y, mil.y)
+
+
+def test_gbm_euler_maruyama_closed_form(orc):
+    # y_{n+1} = y_n (1 + mu h + sigma dW_n) in exact arithmetic
+    mu, sigma, h = 0.05, 0.2, 0.01
+    dw = np.random.default_rng(5).standard_normal((50, 1)) * math.sqrt(h)
+    sys_ = LinearDiagonal(1, [mu, sigma])
+    ens = IVP.sde(sys_, 0.0, 0.5, [100.0], noise=DE_NOISE_TABLE, noise_table=dw) \
+        .method(EulerMaruyama(h))
+    sol = orc.solve_sde(ens)
+    assert sol.steps[0] == 50
+    y = Fr(100)
+    for k in range(50):
+        y = y * (1 + Fr(mu) * Fr(h) + Fr(sigma) * Fr(dw[k, 0]))
+    assert abs(sol.y[0] - float(y)) <= 1e-13 * float(y)
+
+
+KAT_PHILOX = [
+    ([0, 0, 0, 0], 0, 0, [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, 0xffffffff, 0xffffffff, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], 0xa4093822, 0x299f31d0,
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+]
+
+
+@pytest.mark.parametrize("ctr,k0,k1,want", KAT_PHILOX)
+def test_philox_known_answers(orc, ctr, k0, k1, want):
+    assert orc.philox(ctr, k0, k1).tolist() == want
+
+
+@pytest.mark.parametrize("y", [-0.2, 0.2])
+def test_pow_is_correctly_rounded(orc, y):
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 160
+    rng = np.random.default_rng(42 if y < 0 else 43)
+    xs = np.concatenate([np.exp(rng.uniform(np.log(1e-7), np.log(5e3), 6000)),
+                         np.exp(rng.uniform(np.log(1e-300), np.log(1e300), 2000)),
+                         1.0 + (np.arange(-500, 500) * 2.0 ** -52),
+                         [5e-324, 1e-310, 2.2250738585072014e-308, 1.7976931348623157e308]])
+    got = orc.pow_many(xs, y)
+    libm_mis = 0
+    for x, g in zip(xs, got):
+        want = float(mpmath.power(mpmath.mpf(float(x)), mpmath.mpf(y)))
+        assert g == want, (x, g, want)
+        libm_mis += math.pow(float(x), y) != want
+    # The C library's pow is not correctly rounded; record that it differs rarely.
+    assert libm_mis < 0.01 * len(xs)
+
+
+def test_pow_special_cases(orc):
+    assert orc.pow_many(np.array([0.0]), -0.2)[0] == math.inf
+    assert orc.pow_many(np.array([0.0]), 0.2)[0] == 0.0
+    assert orc.pow_many(np.array([math.inf]), -0.2)[0] == 0.0
+    assert orc.pow_many(np.array([1.0]), -0.2)[0] == 1.0
+    assert math.isnan(orc.pow_many(np.array([math.nan]), -0.2)[0])
+    assert math.isnan(orc.pow_many(np.array([-1.0]), -0.2)[0])
+
+
+def test_noise_normals_are_standard_normal(orc):
+    from scipy import stats
+    lib = orc.load()
+    z = np.array([lib.de_oracle_noise_normal(0x5EED2026, p, s, c)
+                  for p in range(50) for s in range(100) for c in range(8)])
+    assert abs(z.mean()) < 0.03
+    assert abs(z.var() - 1.0) < 0.04
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    # deterministic, and keyed by (seed, path, step, component)
+    assert lib.de_oracle_noise_normal(1, 2, 3, 4) == lib.de_oracle_noise_normal(1, 2, 3, 4)
+    assert lib.de_oracle_noise_normal(1, 2, 3, 4) != lib.de_oracle_noise_normal(1, 2, 3, 5)
