@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Benchmark: accepted steps x trajectories per second, FP64 DOPRI5, one MI355X.
+
+Default workload (the configuration the metric is quoted on): 1 048 576 Lorenz
+trajectories (sigma = 10, beta = 8/3, rho ~ U[0, 50], y0 ~ N((1,1,1), 0.1^2 I)),
+DOPRI5 rtol = 1e-8, atol = 1e-10, t in [0, 50], final state and step
+statistics per trajectory.  One "step" of the benchmark is one full ensemble
+solve with inputs already resident in HBM.  The batched (persistent
+lane-refilling) schedule is the headline `value`; the plain one-thread-per-
+trajectory launch is reported beside it, with p50/p99 per-solve latency.
+
+Other workloads (--workload): spring (65 536 uneven spring chains, dense
+output), sde (OU-8 + GBM paths, Euler-Maruyama and Milstein), serve (Poisson
+request stream through the serving loop).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "differential-equations_amd"))
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (AMD spec), FMA = 2 flops
+HBM_PEAK_GBS = 8000.0
+LORENZ_FLOP_ACC = 263     # 7F + 69n, F = 8, n = 3 (per accepted step)
+LORENZ_FLOP_REJ = 255     # 6F + 69n (per rejected step)
+SEED = 0x5EED2026
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="lorenz",
+                    choices=["lorenz", "spring", "sde", "serve"])
+    ap.add_argument("--n", type=int, default=0, help="override ensemble size")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-sample-stride", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (PMC traffic calibration)")
+    return ap.parse_args()
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q / 100.0
+    lo = int(k)
+    hi = min(lo + 1, len(xs) - 1)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def load_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+class Device:
+    """Device buffers (torch owns HBM allocations and the stream)."""
+
+    def __init__(self):
+        import torch
+        if not torch.cuda.is_available():
+            raise SystemExit("bench.py needs a GPU (torch.cuda.is_available() is False)")
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+        self.stream = torch.cuda.current_stream()
+
+    def put(self, a, dtype=None):
+        import numpy as np
+        t = self.torch.from_numpy(np.ascontiguousarray(a))
+        return t.to(self.dev)
+
+    def empty(self, n, dtype):
+        return self.torch.empty(n, dtype=dtype, device=self.dev)
+
+    def event(self):
+        return self.torch.cuda.Event(enable_timing=True)
+
+    def sync(self):
+        self.torch.cuda.synchronize()
+
+
+def calibrate_copy(dv):
+    t = dv.torch
+    src = t.ones(1 << 27, dtype=t.float64, device=dv.dev)   # 1 GiB
+    dst = t.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    dv.sync()
+
+
+# --------------------------------------------------------------- Lorenz ---
+def bench_lorenz(args, dv):
+    import ctypes as C
+    import numpy as np
+    import deamd
+    from deamd import _abi
+    torch = dv.torch
+    n = args.n or (1 << 20)
+    p, y0 = deamd.gen_lorenz(n, seed=SEED)
+    d_p = dv.put(p.reshape(-1))
+    d_y0 = dv.put(y0.reshape(-1))
+    out = dict(y=dv.empty(3 * n, torch.float64), t=dv.empty(n, torch.float64),
+               acc=dv.empty(n, torch.int64), rej=dv.empty(n, torch.int64),
+               ev=dv.empty(n, torch.int64), st=dv.empty(n, torch.int32))
+    lib = deamd.lib()
+    stream = C.c_void_p(dv.stream.cuda_stream)
+
+    def descriptor(schedule, fast):
+        b = _abi.ode_defaults()
+        b.family = _abi.DE_FAMILY_LORENZ
+        b.method = _abi.DE_METHOD_DOPRI5
+        b.n = n
+        b.dim = 3
+        b.params = C.c_void_p(d_p.data_ptr())
+        b.y0 = C.c_void_p(d_y0.data_ptr())
+        b.t0_scalar = 0.0
+        b.tf_scalar = 50.0
+        b.rtol = 1e-8
+        b.atol = 1e-10
+        b.max_steps = 1000000
+        b.schedule = schedule
+        b.flags = _abi.DE_FLAG_FAST_MATH if fast else 0
+        r = _abi.OdeResult()
+        r.y_final = C.c_void_p(out["y"].data_ptr())
+        r.t_final = C.c_void_p(out["t"].data_ptr())
+        r.accepted = C.c_void_p(out["acc"].data_ptr())
+        r.rejected = C.c_void_p(out["rej"].data_ptr())
+        r.evals = C.c_void_p(out["ev"].data_ptr())
+        r.status = C.c_void_p(out["st"].data_ptr())
+        return b, r
+
+    def run(schedule, fast, steps, warmup):
+        b, r = descriptor(schedule, fast)
+        for _ in range(warmup):
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+        dv.sync()
+        evs = [(dv.event(), dv.event()) for _ in range(steps)]
+        dv.sync()
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(dv.stream)
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+            e1.record(dv.stream)
+        dv.sync()
+        wall = time.perf_counter() - t0
+        lat = [e0.elapsed_time(e1) for e0, e1 in evs]   # ms, on the launch stream
+        A = int(out["acc"].sum().item())
+        R = int(out["rej"].sum().item())
+        status_ok = bool((out["st"] == 0).all().item())
+        return dict(wall=wall, lat=lat, A=A, R=R, ok=status_ok)
+
+    res = run(_abi.DE_SCHED_PERSISTENT, False, args.steps, args.warmup)
+    plain = None if args.no_plain else run(_abi.DE_SCHED_PLAIN, False, max(3, args.steps // 2), 1)
+    fast = run(_abi.DE_SCHED_PERSISTENT, True, max(3, args.steps // 2), 1)
+    if args.calibrate_copy:
+        calibrate_copy(dv)
+
+    def summarize(r):
+        ms = 1000.0 * r["wall"] / len(r["lat"])
+        return dict(value=r["A"] / (ms / 1000.0), ms_per_step=ms,
+                    p50_ms=percentile(r["lat"], 50), p99_ms=percentile(r["lat"], 99))
+
+    main = summarize(res)
+    kernel_ms = sum(res["lat"]) / len(res["lat"])
+    flops = LORENZ_FLOP_ACC * res["A"] + LORENZ_FLOP_REJ * res["R"]
+    achieved = flops / (kernel_ms / 1000.0) / 1e12
+    line = {
+        "metric": "accepted steps*traj/s FP64 DOPRI5, batched vs plain, 1 GPU; p50/p99 latency",
+        "value": main["value"],
+        "unit": "accepted_steps*traj/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Philox4x32-10 inputs, seed 0x5EED2026)",
+        "config": {"workload": "lorenz_dopri5_1M", "n_traj": n, "t_span": [0.0, 50.0],
+                   "rtol": 1e-8, "atol": 1e-10, "schedule": "persistent",
+                   "build": "parity (bit-exact, no FMA contraction)",
+                   "accepted_steps_per_solve": res["A"], "rejected_steps_per_solve": res["R"],
+                   "all_complete": res["ok"], "parallelism": "none (1 GPU)"},
+        "latency_ms": {"p50": main["p50_ms"], "p99": main["p99_ms"]},
+        "roofline": {"bound": "fp64_vector", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": load_traffic("lorenz_dopri5_1M"),
+                     "flop_model": "263 per accepted + 255 per rejected step (7F+69n, 6F+69n)",
+                     "kernel_ms": kernel_ms},
+    }
+    if plain is not None:
+        ps = summarize(plain)
+        line["plain"] = {"value": ps["value"], "ms_per_step": ps["ms_per_step"],
+                         "p50_ms": ps["p50_ms"], "p99_ms": ps["p99_ms"]}
+        line["batched_over_plain"] = main["value"] / ps["value"]
+    fs = summarize(fast)
+    fast_ms = sum(fast["lat"]) / len(fast["lat"])
+    fast_flops = LORENZ_FLOP_ACC * fast["A"] + LORENZ_FLOP_REJ * fast["R"]
+    line["fast_math"] = {"value": fs["value"], "ms_per_step": fs["ms_per_step"],
+                         "tflops": fast_flops / (fast_ms / 1000.0) / 1e12,
+                         "frac": fast_flops / (fast_ms / 1000.0) / 1e12 / FP64_PEAK_TFLOPS,
+                         "note": "FMA-contracted build: same algorithm, not bit-equal"}
+    return line, (p, y0, n)
+
+
+def cpu_baseline_lorenz(args, p, y0, n):
+    """The oracle (pow = C library pow, as the reference) on every k-th trajectory."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import deamd
+    import oracle
+    k = args.cpu_sample_stride
+    idx = np.arange(0, n, k)
+    ens = deamd.IVP.ode(deamd.Lorenz(*p[:, idx]), 0.0, 50.0, y0[:, idx]).method(
+        deamd.ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(1000000))
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 16))
+    t0 = time.perf_counter()
+    sol = oracle.solve_ode(ens, pow_mode=oracle.POW_LIBM, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": float(sol.accepted.sum()) / dt, "unit": "accepted_steps*traj/s",
+            "cores": threads, "kind": "port",
+            "sample": "every %d-th trajectory of the 1M Lorenz ensemble (%d trajectories, "
+                      "%d accepted steps, %.1f s wall on %d threads); C++ restatement of the "
+                      "reference solver with the C library pow" % (k, len(idx),
+                                                                    int(sol.accepted.sum()), dt,
+                                                                    threads)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(trajectories shard by index ranges across GPUs; see DESIGN.md)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    dv = Device()
+    if args.workload == "lorenz":
+        line, (p, y0, n) = bench_lorenz(args, dv)
+        if not args.no_cpu_baseline:
+            line["cpu_baseline"] = cpu_baseline_lorenz(args, p, y0, n)
+    else:
+        import bench_workloads
+        line = bench_workloads.run(args, dv)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
