@@ -242,6 +242,199 @@ def cpu_baseline_lorenz(args, p, y0, n):
                                                                     threads)}
 
 
+# --------------------------------------------------------------- spring ---
+def bench_spring(args, dv):
+    """65 536 damped spring chains, N log-uniform in [4, 256], DOPRI5 rtol 1e-6
+    atol 1e-9, t in [0, 10], dense output at 100 uniform times."""
+    import ctypes as C
+    import numpy as np
+    import deamd
+    from deamd import _abi
+    torch = dv.torch
+    n = args.n or 65536
+    sysm, y0 = deamd.gen_spring(n, seed=SEED)
+    pts = np.linspace(0.0, 10.0, 100)
+    state_len = int(sysm.state_offset[-1])
+    d = dict(sizes=dv.put(sysm.sizes), so=dv.put(sysm.state_offset), po=dv.put(sysm.param_offset),
+             p=dv.put(sysm.params_packed), y0=dv.put(y0), te=dv.put(pts))
+    out = dict(y=dv.empty(state_len, torch.float64), t=dv.empty(n, torch.float64),
+               acc=dv.empty(n, torch.int64), rej=dv.empty(n, torch.int64),
+               ev=dv.empty(n, torch.int64), st=dv.empty(n, torch.int32),
+               dense=dv.empty(100 * state_len, torch.float64), nd=dv.empty(n, torch.int32))
+    lib = deamd.lib()
+    stream = C.c_void_p(dv.stream.cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+
+    def descriptor(mapping):
+        b = _abi.ode_defaults()
+        b.family = _abi.DE_FAMILY_SPRING_CHAIN
+        b.method = _abi.DE_METHOD_DOPRI5
+        b.n = n
+        b.sizes = C.cast(ptr(d["sizes"]), C.POINTER(C.c_int32))
+        b.state_offset = C.cast(ptr(d["so"]), C.POINTER(C.c_int64))
+        b.param_offset = C.cast(ptr(d["po"]), C.POINTER(C.c_int64))
+        b.params, b.y0, b.t_eval = ptr(d["p"]), ptr(d["y0"]), ptr(d["te"])
+        b.n_eval = 100
+        b.t0_scalar, b.tf_scalar = 0.0, 10.0
+        b.rtol, b.atol = 1e-6, 1e-9
+        b.max_steps = 1000000
+        b.mapping = mapping
+        r = _abi.OdeResult()
+        r.y_final, r.t_final = ptr(out["y"]), ptr(out["t"])
+        r.accepted, r.rejected = ptr(out["acc"]), ptr(out["rej"])
+        r.evals, r.status = ptr(out["ev"]), ptr(out["st"])
+        r.dense, r.n_dense = ptr(out["dense"]), ptr(out["nd"])
+        return b, r
+
+    def run(mapping, steps, warmup):
+        b, r = descriptor(mapping)
+        for _ in range(warmup):
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+        dv.sync()
+        evs = [(dv.event(), dv.event()) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(dv.stream)
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+            e1.record(dv.stream)
+        dv.sync()
+        wall = time.perf_counter() - t0
+        lat = [e0.elapsed_time(e1) for e0, e1 in evs]
+        acc = out["acc"].cpu().numpy()
+        rej = out["rej"].cpu().numpy()
+        return dict(wall=wall, lat=lat, acc=acc, rej=rej,
+                    ok=bool((out["st"] == 0).all().item()))
+
+    res = run(_abi.DE_MAP_AUTO, args.steps, max(1, args.warmup))
+    tuned = {}
+    for cap in (4, 8, 16, 32, 64, 128, 256):
+        g, c, blk = C.c_int32(), C.c_int32(), C.c_int32()
+        if lib.de_spring_tuned(cap, C.byref(g), C.byref(c), C.byref(blk)) == 0:
+            tuned[str(cap)] = {"lanes_per_system": g.value, "masses_per_lane": c.value,
+                               "block": blk.value}
+    plain = None if args.no_plain else run(_abi.DE_MAP_LANE, 2, 1)
+    N = sysm.sizes.astype(np.float64)
+    ms = 1000.0 * res["wall"] / len(res["lat"])
+    A = float(res["acc"].sum())
+    flops = float((187 * N * res["acc"] + 180 * N * res["rej"] + 12 * N * res["acc"]).sum()
+                  + 12 * N.sum() * 100)
+    dense_bytes = 100 * state_len * 8.0
+    kern_s = sum(res["lat"]) / len(res["lat"]) / 1000.0
+    tf = flops / kern_s / 1e12
+    gbs = dense_bytes / kern_s / 1e9
+    line = {
+        "metric": "accepted steps*traj/s FP64 DOPRI5, batched vs plain, 1 GPU; p50/p99 latency",
+        "value": A / (ms / 1000.0), "unit": "accepted_steps*traj/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Philox4x32-10 inputs, seed 0x5EED2026)",
+        "config": {"workload": "spring_chain_dopri5_64k", "n_systems": n,
+                   "masses": [int(sysm.sizes.min()), int(sysm.sizes.max())],
+                   "mean_masses": float(N.mean()), "t_span": [0.0, 10.0], "rtol": 1e-6,
+                   "atol": 1e-9, "n_eval": 100, "mapping": "auto (autotuned per size class)",
+                   "autotuned": tuned, "all_complete": res["ok"],
+                   "accepted_steps_per_solve": int(A)},
+        "latency_ms": {"p50": percentile(res["lat"], 50), "p99": percentile(res["lat"], 99)},
+        "roofline": {"bound": "fp64_vector" if tf / FP64_PEAK_TFLOPS > gbs / HBM_PEAK_GBS else "hbm",
+                     "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tf / FP64_PEAK_TFLOPS, "traffic": load_traffic("spring_chain_dopri5_64k"),
+                     "dense_write_GBps": gbs, "dense_write_frac": gbs / HBM_PEAK_GBS,
+                     "flop_model": "187N per accepted, 180N per rejected step, +12N dense prep "
+                                   "per accepted step, +12N per output point"},
+    }
+    if plain is not None:
+        pms = 1000.0 * plain["wall"] / len(plain["lat"])
+        line["plain"] = {"value": float(plain["acc"].sum()) / (pms / 1000.0), "ms_per_step": pms,
+                         "mapping": "one thread per system, state in HBM"}
+        line["batched_over_plain"] = line["value"] / line["plain"]["value"]
+    return line
+
+
+# ------------------------------------------------------------------ sde ---
+def bench_sde(args, dv):
+    """4 194 304 paths each of OU-8 and GBM, Euler-Maruyama and Milstein, dt 1e-3,
+    1000 steps, Philox noise; every 100th state recorded."""
+    import ctypes as C
+    import numpy as np
+    import deamd
+    from deamd import _abi
+    torch = dv.torch
+    n = args.n or (1 << 22)
+    ou, ou_y0 = deamd.gen_ou(n, 8, seed=SEED)
+    gbm, gbm_y0 = deamd.gen_gbm(n)
+    lib = deamd.lib()
+    stream = C.c_void_p(dv.stream.cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    stride = 100
+    n_rec = 2000 // stride + 1
+    runs = []
+    for fam, sysm, y0, dim in ((_abi.DE_SDE_ORNSTEIN_UHLENBECK, ou, ou_y0, 8),
+                               (_abi.DE_SDE_LINEAR_DIAGONAL, gbm, gbm_y0, 1)):
+        dp_, dy0 = dv.put(sysm.params), dv.put(y0)
+        o = dict(y=dv.empty(dim * n, torch.float64), t=dv.empty(n, torch.float64),
+                 steps=dv.empty(n, torch.int64), ev=dv.empty(n, torch.int64),
+                 st=dv.empty(n, torch.int32), path=dv.empty(n_rec * dim * n, torch.float64),
+                 pt=dv.empty(n_rec, torch.float64))
+        for scheme in (_abi.DE_SCHEME_EULER_MARUYAMA, _abi.DE_SCHEME_MILSTEIN):
+            b = _abi.sde_defaults()
+            b.family, b.scheme, b.n, b.dim = fam, scheme, n, dim
+            b.noise = _abi.DE_NOISE_PHILOX
+            b.params, b.y0 = ptr(dp_), ptr(dy0)
+            b.t0, b.tf, b.h0 = 0.0, 1.0, 1e-3
+            b.max_steps = 2000
+            b.seed = SEED
+            b.path_stride = stride
+            r = _abi.SdeResult()
+            r.y_final, r.t_final, r.steps = ptr(o["y"]), ptr(o["t"]), ptr(o["steps"])
+            r.evals, r.status, r.path, r.path_t = ptr(o["ev"]), ptr(o["st"]), ptr(o["path"]), ptr(o["pt"])
+            runs.append((fam, scheme, dim, b, r, o, (dp_, dy0)))
+
+    def one_pass():
+        for (_, _, _, b, r, _, _) in runs:
+            _abi.check(lib.de_sde_batch_solve(C.byref(b), C.byref(r), stream))
+
+    for _ in range(max(1, args.warmup)):
+        one_pass()
+    dv.sync()
+    per = {}
+    evs = [[(dv.event(), dv.event()) for _ in runs] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        for j, (_, _, _, b, r, _, _) in enumerate(runs):
+            evs[k][j][0].record(dv.stream)
+            _abi.check(lib.de_sde_batch_solve(C.byref(b), C.byref(r), stream))
+            evs[k][j][1].record(dv.stream)
+    dv.sync()
+    wall = time.perf_counter() - t0
+    total_steps = 0
+    names = {(0, 0): "ou8_em", (0, 1): "ou8_milstein", (1, 0): "gbm_em", (1, 1): "gbm_milstein"}
+    for j, (fam, scheme, dim, b, r, o, _) in enumerate(runs):
+        lat = [evs[k][j][0].elapsed_time(evs[k][j][1]) for k in range(args.steps)]
+        steps = int(o["steps"].sum().item())
+        total_steps += steps
+        kms = sum(lat) / len(lat)
+        path_bytes = (steps // n // stride) * dim * n * 8.0
+        per[names[(fam, scheme)]] = {"kernel_ms": kms, "steps_paths_per_s": steps / (kms / 1000),
+                                     "path_write_GBps": path_bytes / (kms / 1000) / 1e9,
+                                     "all_complete": bool((o["st"] == 0).all().item())}
+    ms = 1000.0 * wall / args.steps
+    ou_em = per["ou8_em"]
+    return {
+        "metric": "SDE path-steps*paths/s FP64 (Euler-Maruyama, Milstein), 1 GPU",
+        "value": total_steps / (ms / 1000.0), "unit": "steps*paths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Philox4x32-10 noise keyed by seed, path, step, component)",
+        "config": {"workload": "sde_ou8_gbm_4M", "n_paths": n, "dt": 1e-3, "steps": 1000,
+                   "record_every": stride, "per_run": per},
+        "roofline": {"bound": "hbm", "achieved": ou_em["path_write_GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ou_em["path_write_GBps"] / HBM_PEAK_GBS,
+                     "traffic": load_traffic("sde_ou8_em_4M"),
+                     "note": "OU-8 Euler-Maruyama path writes; the kernel is bound by noise "
+                             "generation (Philox rounds and the Box-Muller log/sincos)"},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -254,9 +447,12 @@ def main():
         line, (p, y0, n) = bench_lorenz(args, dv)
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = cpu_baseline_lorenz(args, p, y0, n)
+    elif args.workload == "spring":
+        line = bench_spring(args, dv)
+    elif args.workload == "sde":
+        line = bench_sde(args, dv)
     else:
-        import bench_workloads
-        line = bench_workloads.run(args, dv)
+        raise SystemExit("workload %s is not available yet" % args.workload)
     print(json.dumps(line), flush=True)
 
 

@@ -484,6 +484,17 @@ double de_noise_normal(uint64_t seed, uint64_t path, uint32_t step, uint32_t com
   return de::noise_normal(seed, path, step, comp);
 }
 
+int de_spring_tuned(int32_t cap, int32_t* g, int32_t* c, int32_t* block) {
+  int gg = 0, cc = 0, bb = 0;
+  int r = de::parity::spring_tuned(cap, &gg, &cc, &bb);
+  if (r == 0) {
+    *g = gg;
+    *c = cc;
+    *block = bb;
+  }
+  return r;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------- self-checks ----

@@ -15,6 +15,9 @@ namespace parity {
 int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArgs& a, int sched,
                   hipStream_t s);
 int64_t spring_workspace_bytes(const de_ode_batch* b);
+// Autotuner decision for the size class with cap masses: 0 and (G, C, block),
+// or -1 if that class has not been tuned in this process.
+int spring_tuned(int cap, int* g, int* c, int* block);
 
 }  // namespace parity
 }  // namespace de

@@ -1,0 +1,686 @@
+// DOPRI5 solvers for ragged damped spring-mass chains (device code).
+//
+// Same algorithm and operation order as the lane solvers of de_lane.h
+// (solve_ode src/ode/solve_ivp.rs:116-277, DOPRI5 ordinary.rs:16-336, h_init
+// h_init.rs:45-135, TEvalSolout t_eval.rs:83-134), for a state of runtime
+// length n = 2N.  Two mappings:
+//
+//   VecDopri5     one lane per system; vectors live in an HBM workspace.  The
+//                 error norm is the reference's sequential sum, so results are
+//                 bit-identical to the CPU restatement.
+//   GroupDopri5   G lanes per system, C masses per lane held in registers.
+//                 Neighbour displacements cross lanes with __shfl_up/down and
+//                 the norms are fixed butterfly reductions, identical on every
+//                 lane of the group.  The butterfly sums in a different order
+//                 than the reference's sequential loop (traits.rs:398-409), so
+//                 results agree to rounding rather than bit for bit.
+#pragma once
+
+#include "de_lane.h"
+
+namespace de {
+
+struct SpringArgs {
+  OdeArgs o;                   // common settings and per-trajectory outputs
+  const int32_t* sizes;        // N per system
+  const int64_t* state_offset; // n + 1
+  const int64_t* param_offset; // n + 1
+  double* work;                // VecDopri5: 9 vectors per system at 9*state_offset
+  const int32_t* list;         // GroupDopri5: system indices of this size class
+  int64_t list_len;
+};
+
+// ------------------------------------------------------------- VecDopri5 --
+struct VecDopri5 {
+  const SpringArgs* A;
+  int64_t i;
+  int nm, n;  // masses, state length
+  const double* p;
+  double* y;   // y_final slot doubles as the working state
+  double *k[7], *ys, *ynew;
+  double t, h, h_prev, tf, dir, rtol, atol;
+  int64_t acc, rej, ev, steps;
+  int32_t stiff, nonstiff, countdown, next_eval, pushed;
+  bool rejected_last;
+
+  DE_HD void diff(double, const double* yy, double* dy) const { spring_chain_diff(nm, p, yy, dy); }
+
+  DE_HD void finish(int32_t st) {
+    const OdeArgs& a = A->o;
+    a.t_final[i] = t;
+    a.accepted[i] = acc;
+    a.rejected[i] = rej;
+    a.evals[i] = ev;
+    a.status[i] = st;
+    if (a.n_dense) a.n_dense[i] = pushed;
+  }
+
+  DE_HD void emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
+                  double hp, const double* y_old, const double* c1, const double* c2,
+                  const double* c3) {
+    const OdeArgs& a = A->o;
+    int32_t idx = next_eval;
+    double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
+    while (idx < a.n_eval) {
+      double te = a.t_eval[idx];
+      bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
+                                : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
+      if (in_range) {
+        double* out = dbase + (int64_t)idx * n;
+        if (te == t_curr || !have_cont) {
+          for (int c = 0; c < n; ++c) out[c] = ycur[c];
+        } else {
+          double s = (te - t_prev) / hp;
+          double s1 = 1.0 - s;
+          double cont4 = 0.0 * hp;
+          for (int c = 0; c < n; ++c) {
+            double acc_ = cont4 * s1 + c3[c];
+            acc_ = acc_ * s + c2[c];
+            acc_ = acc_ * s1 + c1[c];
+            out[c] = y_old[c] + s * acc_;
+          }
+        }
+        pushed += 1;
+        idx += 1;
+      } else {
+        if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+        idx += 1;
+      }
+    }
+    next_eval = idx;
+  }
+
+  DE_HD bool init(const SpringArgs* args, int64_t sys) {
+    A = args;
+    i = sys;
+    const OdeArgs& a = A->o;
+    nm = A->sizes[i];
+    n = 2 * nm;
+    p = a.params + A->param_offset[i];
+    const int64_t so = A->state_offset[i];
+    y = a.y_final + so;
+    double* w = A->work + 9 * so;
+    for (int v = 0; v < 7; ++v) k[v] = w + (int64_t)v * n;
+    ys = w + 7 * (int64_t)n;
+    ynew = w + 8 * (int64_t)n;
+    const double* y0 = a.y0 + so;
+    for (int c = 0; c < n; ++c) y[c] = y0[c];
+    double t0 = a.t0 ? a.t0[i] : a.t0s;
+    tf = a.tf ? a.tf[i] : a.tfs;
+    rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
+    atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    t = t0;
+    acc = rej = ev = steps = 0;
+    pushed = next_eval = 0;
+    dir = signum(tf - t0);
+    bool ok = dir == 1.0 || dir == -1.0;
+    double h0 = a.h0;
+    if (ok && h0 == 0.0) {
+      // h_init with f0 in k[0], f1 in k[1], sk in ys, y1 in ynew.
+      double* f0 = k[0];
+      double* f1 = k[1];
+      double* sk = ys;
+      double* y1 = ynew;
+      diff(t0, y, f0);
+      double dnf = 0.0, dny = 0.0;
+      for (int c = 0; c < n; ++c) {
+        sk[c] = atol + rtol * fabs(y[c]);
+        double u = f0[c] / sk[c];
+        dnf += u * u;
+        double w2 = y[c] / sk[c];
+        dny += w2 * w2;
+      }
+      double hh = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+      hh = fmin(hh, a.h_max);
+      hh *= dir;
+      for (int c = 0; c < n; ++c) y1[c] = y[c] + hh * f0[c];
+      diff(t0 + hh, y1, f1);
+      double der2 = 0.0;
+      for (int c = 0; c < n; ++c) {
+        double d = (f1[c] - f0[c]) / sk[c];
+        der2 += d * d;
+      }
+      der2 = sqrt(der2) / fabs(hh);
+      double der12 = fmax(sqrt(dnf), der2);
+      double h1 = der12 <= 1.0e-15 ? fabs(hh) * 1.0e-3 : DE_CTRL_POW(0.01 / der12, 1.0 / 5.0);
+      double interval = fabs(tf - t0);
+      hh = fmin(fmax(fmin(fmin(fabs(hh) * 100.0, h1), a.h_max), a.h_min), interval);
+      h0 = hh * dir;
+      ev += 2;
+    }
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      ev = 0;
+      finish(kBadInput);
+      return false;
+    }
+    h = h0;
+    diff(t, y, k[0]);
+    ev += 1;
+    h_prev = 0.0;
+    stiff = nonstiff = 0;
+    countdown = 100;
+    rejected_last = false;
+    if (a.n_eval > 0) emit(t, t, y, false, 0.0, nullptr, nullptr, nullptr, nullptr);
+    return true;
+  }
+
+  DE_HD bool advance() {
+    const OdeArgs& a = A->o;
+    if ((t + h - tf) * dir > 0.0) {
+      double hn = tf - t;
+      if (fabs(hn) < kTenEps) {
+        finish(kComplete);
+        return true;
+      }
+      h = hn;
+    }
+    if (fabs(h) < fabs(h_prev) * 1e-14) {
+      finish(kStepSize);
+      return true;
+    }
+    if (steps >= a.max_steps) {
+      finish(kMaxSteps);
+      return true;
+    }
+    steps += 1;
+    countdown -= 1;
+    const double a10h = dp::a10 * h;
+    for (int c = 0; c < n; ++c) ys[c] = y[c] + a10h * k[0][c];
+    diff(t + dp::c1 * h, ys, k[1]);
+    const double a20h = dp::a20 * h, a21h = dp::a21 * h;
+    for (int c = 0; c < n; ++c) ys[c] = (y[c] + a20h * k[0][c]) + a21h * k[1][c];
+    diff(t + dp::c2 * h, ys, k[2]);
+    const double a30h = dp::a30 * h, a31h = dp::a31 * h, a32h = dp::a32 * h;
+    for (int c = 0; c < n; ++c)
+      ys[c] = ((y[c] + a30h * k[0][c]) + a31h * k[1][c]) + a32h * k[2][c];
+    diff(t + dp::c3 * h, ys, k[3]);
+    const double a40h = dp::a40 * h, a41h = dp::a41 * h, a42h = dp::a42 * h,
+                 a43h = dp::a43 * h;
+    for (int c = 0; c < n; ++c)
+      ys[c] = (((y[c] + a40h * k[0][c]) + a41h * k[1][c]) + a42h * k[2][c]) + a43h * k[3][c];
+    diff(t + dp::c4 * h, ys, k[4]);
+    const double a50h = dp::a50 * h, a51h = dp::a51 * h, a52h = dp::a52 * h,
+                 a53h = dp::a53 * h, a54h = dp::a54 * h;
+    for (int c = 0; c < n; ++c)
+      ys[c] = ((((y[c] + a50h * k[0][c]) + a51h * k[1][c]) + a52h * k[2][c]) + a53h * k[3][c]) +
+              a54h * k[4][c];
+    diff(t + 1.0 * h, ys, k[5]);
+    const double a60h = dp::a60 * h, a62h = dp::a62 * h, a63h = dp::a63 * h,
+                 a64h = dp::a64 * h, a65h = dp::a65 * h;
+    for (int c = 0; c < n; ++c)
+      ys[c] = ((((y[c] + a60h * k[0][c]) + a62h * k[2][c]) + a63h * k[3][c]) + a64h * k[4][c]) +
+              a65h * k[5][c];
+    diff(t + 1.0 * h, ys, k[6]);
+    // yseg is kept in k[1] (k1 has a zero weight in b and er and is not used
+    // again this attempt); ynew in ynew.
+    double err = 0.0;
+    for (int c = 0; c < n; ++c) {
+      double sg = (((dp::b0 * k[0][c] + dp::b2 * k[2][c]) + dp::b3 * k[3][c]) +
+                   dp::b4 * k[4][c]) +
+                  dp::b5 * k[5][c];
+      double yn = y[c] + h * sg;
+      double es = ((((dp::e0 * k[0][c] + dp::e2 * k[2][c]) + dp::e3 * k[3][c]) +
+                    dp::e4 * k[4][c]) +
+                   dp::e5 * k[5][c]) +
+                  dp::e6 * k[6][c];
+      double sk = atol + rtol * fmax(fabs(y[c]), fabs(yn));
+      double e = es / sk;
+      err = c == 0 ? e * e : err + e * e;
+      k[1][c] = sg;
+      ynew[c] = yn;
+    }
+    double deno = err;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * (double)n));
+    double scale = a.safety * DE_CTRL_POW(err, -0.2);
+    scale = fmin(fmax(scale, a.min_scale), a.max_scale);
+    if (err <= 1.0) {
+      double t_new = t + h;
+      // f(t_new, y_new) goes to k[3] (no longer needed); ys still holds the
+      // stage-6 state for the stiffness test.
+      double* dn = k[3];
+      diff(t_new, ynew, dn);
+      if (countdown == 0) {
+        countdown = 100;
+        double stdnum = 0.0, stden = 0.0;
+        for (int c = 0; c < n; ++c) {
+          double u = k[1][c] - k[6][c];
+          stdnum = c == 0 ? u * u : stdnum + u * u;
+          double w2 = dn[c] - ys[c];
+          stden = c == 0 ? w2 * w2 : stden + w2 * w2;
+        }
+        if (stden > 0.0) {
+          double h_lamb = h * sqrt(stdnum / stden);
+          if (h_lamb > 6.1) {
+            nonstiff = 0;
+            stiff += 1;
+            if (stiff == 15) {
+              finish(kStiffness);
+              return true;
+            }
+          }
+        } else {
+          nonstiff += 1;
+          if (nonstiff == 6) stiff = 0;
+        }
+      }
+      if (a.n_eval > 0) {
+        // cont1 = ydiff (k[4]), cont2 = bspl (k[5]), cont3 (k[6]).
+        for (int c = 0; c < n; ++c) {
+          double ydiff = ynew[c] + (-1.0) * y[c];
+          double bspl = h * k[0][c] + (-1.0) * ydiff;
+          double c3 = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
+          k[4][c] = ydiff;
+          k[5][c] = bspl;
+          k[6][c] = c3;
+        }
+        emit(t_new, t, ynew, true, h, y, k[4], k[5], k[6]);
+      }
+      h_prev = h;
+      t = t_new;
+      for (int c = 0; c < n; ++c) {
+        y[c] = ynew[c];
+        k[0][c] = dn[c];
+      }
+      if (rejected_last) {
+        rejected_last = false;
+        scale = fmin(scale, 1.0);
+      }
+      acc += 1;
+      ev += 7;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+      if (fabs(tf - t) <= kTenEps) {
+        finish(kComplete);
+        return true;
+      }
+    } else {
+      rejected_last = true;
+      rej += 1;
+      ev += 6;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    }
+    return false;
+  }
+};
+
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+// ----------------------------------------------------------- GroupDopri5 --
+// G lanes per system (G a power of two <= 64), C masses per lane:
+// lane l of the group owns masses j = l*C + q, q < C, and keeps x_j, v_j of
+// every vector in registers.
+template <int G, int C>
+struct GroupDopri5 {
+  double x[C], v[C];      // y
+  double k0x[C], k0v[C];  // f(t, y)
+  double kk[C + 1];       // springs k_{lC} .. k_{lC+C}
+  double cd;              // damping
+  int nm, l;
+  int64_t sys, so;
+  double t, h, h_prev, tf, dir, rtol, atol;
+  int32_t acc, rej, ev, steps, stiff, nonstiff, countdown, next_eval, pushed;
+  bool rejected_last;
+
+  DE_DEV double gsum(double s) const {
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) s += __shfl_xor(s, m, G);
+    return s;
+  }
+
+  // (dx, dv) = f(x, v): dx_j = v_j, dv_j = spring_accel(...), zero beyond N.
+  DE_DEV void rhs(const double* xs, const double* vs, double* dx, double* dv) const {
+    double left = __shfl_up(xs[C - 1], 1, G);
+    double right = __shfl_down(xs[0], 1, G);
+    if (l == 0) left = 0.0;
+    if (l == G - 1) right = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j = l * C + q;
+      double xm = q == 0 ? left : xs[q - 1];
+      double xp = q == C - 1 ? right : xs[q + 1];
+      if (j + 1 >= nm) xp = 0.0;
+      bool real = j < nm;
+      dx[q] = real ? vs[q] : 0.0;
+      dv[q] = real ? spring_accel(xm, xs[q], xp, vs[q], kk[q], kk[q + 1], cd) : 0.0;
+    }
+  }
+
+  DE_DEV void write_state(double* dst, const double* xs, const double* vs) const {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j = l * C + q;
+      if (j < nm) {
+        dst[j] = xs[q];
+        dst[nm + j] = vs[q];
+      }
+    }
+  }
+
+  DE_DEV void finish(const SpringArgs& A, int32_t st) const {
+    const OdeArgs& a = A.o;
+    write_state(a.y_final + so, x, v);
+    if (l == 0) {
+      a.t_final[sys] = t;
+      a.accepted[sys] = acc;
+      a.rejected[sys] = rej;
+      a.evals[sys] = ev;
+      a.status[sys] = st;
+      if (a.n_dense) a.n_dense[sys] = pushed;
+    }
+  }
+
+  template <class Interp>
+  DE_DEV void emit(const SpringArgs& A, double t_curr, double t_prev, const double* xc,
+                   const double* vc, Interp&& interp) {
+    const OdeArgs& a = A.o;
+    int32_t idx = next_eval;
+    double* dbase = a.dense + (int64_t)a.n_eval * so;
+    while (idx < a.n_eval) {
+      double te = a.t_eval[idx];
+      bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
+                                : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
+      if (in_range) {
+        double* out = dbase + (int64_t)idx * 2 * nm;
+        if (te == t_curr) {
+          write_state(out, xc, vc);
+        } else {
+          double ox[C], ov[C];
+          interp(te, ox, ov);
+          write_state(out, ox, ov);
+        }
+        pushed += 1;
+        idx += 1;
+      } else {
+        if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+        idx += 1;
+      }
+    }
+    next_eval = idx;
+  }
+
+  // Load system `s`; returns false (after writing results) on BadInput.
+  DE_DEV bool init(const SpringArgs& A, int64_t s, int lane_in_group) {
+    const OdeArgs& a = A.o;
+    l = lane_in_group;
+    sys = s;
+    nm = A.sizes[s];
+    so = A.state_offset[s];
+    const double* p = a.params + A.param_offset[s];
+    cd = p[0];
+#pragma unroll
+    for (int q = 0; q <= C; ++q) {
+      const int j = l * C + q;
+      kk[q] = j <= nm ? p[1 + j] : 0.0;
+    }
+    const double* y0 = a.y0 + so;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j = l * C + q;
+      x[q] = j < nm ? y0[j] : 0.0;
+      v[q] = j < nm ? y0[nm + j] : 0.0;
+    }
+    double t0 = a.t0 ? a.t0[s] : a.t0s;
+    tf = a.tf ? a.tf[s] : a.tfs;
+    rtol = a.rtol_traj ? a.rtol_traj[s] : a.rtol;
+    atol = a.atol_traj ? a.atol_traj[s] : a.atol;
+    t = t0;
+    acc = rej = ev = steps = 0;
+    pushed = next_eval = 0;
+    dir = signum(tf - t0);
+    bool ok = dir == 1.0 || dir == -1.0;
+    double h0 = a.h0;
+    if (ok && h0 == 0.0) {
+      double f0x[C], f0v[C], skx[C], skv[C], y1x[C], y1v[C], f1x[C], f1v[C];
+      rhs(x, v, f0x, f0v);
+      double dnf = 0.0, dny = 0.0;
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        if (l * C + q < nm) {
+          skx[q] = atol + rtol * fabs(x[q]);
+          skv[q] = atol + rtol * fabs(v[q]);
+          double u = f0x[q] / skx[q], u2 = f0v[q] / skv[q];
+          dnf += u * u + u2 * u2;
+          double w = x[q] / skx[q], w2 = v[q] / skv[q];
+          dny += w * w + w2 * w2;
+        } else {
+          skx[q] = skv[q] = 1.0;
+        }
+      }
+      dnf = gsum(dnf);
+      dny = gsum(dny);
+      double hh = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+      hh = fmin(hh, a.h_max);
+      hh *= dir;
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        y1x[q] = x[q] + hh * f0x[q];
+        y1v[q] = v[q] + hh * f0v[q];
+      }
+      rhs(y1x, y1v, f1x, f1v);
+      double der2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        if (l * C + q < nm) {
+          double d = (f1x[q] - f0x[q]) / skx[q], d2 = (f1v[q] - f0v[q]) / skv[q];
+          der2 += d * d + d2 * d2;
+        }
+      }
+      der2 = sqrt(gsum(der2)) / fabs(hh);
+      double der12 = fmax(sqrt(dnf), der2);
+      double h1 = der12 <= 1.0e-15 ? fabs(hh) * 1.0e-3 : DE_CTRL_POW(0.01 / der12, 1.0 / 5.0);
+      double interval = fabs(tf - t0);
+      hh = fmin(fmax(fmin(fmin(fabs(hh) * 100.0, h1), a.h_max), a.h_min), interval);
+      h0 = hh * dir;
+      ev += 2;
+    }
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      ev = 0;
+      finish(A, kBadInput);
+      return false;
+    }
+    h = h0;
+    rhs(x, v, k0x, k0v);
+    ev += 1;
+    h_prev = 0.0;
+    stiff = nonstiff = 0;
+    countdown = 100;
+    rejected_last = false;
+    if (a.n_eval > 0) emit(A, t, t, x, v, [](double, double*, double*) {});
+    return true;
+  }
+
+  DE_DEV bool advance(const SpringArgs& A) {
+    const OdeArgs& a = A.o;
+    if ((t + h - tf) * dir > 0.0) {
+      double hn = tf - t;
+      if (fabs(hn) < kTenEps) {
+        finish(A, kComplete);
+        return true;
+      }
+      h = hn;
+    }
+    if (fabs(h) < fabs(h_prev) * 1e-14) {
+      finish(A, kStepSize);
+      return true;
+    }
+    if (steps >= a.max_steps) {
+      finish(A, kMaxSteps);
+      return true;
+    }
+    steps += 1;
+    countdown -= 1;
+    double k1x[C], k1v[C], k2x[C], k2v[C], k3x[C], k3v[C], k4x[C], k4v[C], k5x[C], k5v[C],
+        k6x[C], k6v[C], sx[C], sv[C];
+    const double a10h = dp::a10 * h;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      sx[q] = x[q] + a10h * k0x[q];
+      sv[q] = v[q] + a10h * k0v[q];
+    }
+    rhs(sx, sv, k1x, k1v);
+    const double a20h = dp::a20 * h, a21h = dp::a21 * h;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      sx[q] = (x[q] + a20h * k0x[q]) + a21h * k1x[q];
+      sv[q] = (v[q] + a20h * k0v[q]) + a21h * k1v[q];
+    }
+    rhs(sx, sv, k2x, k2v);
+    const double a30h = dp::a30 * h, a31h = dp::a31 * h, a32h = dp::a32 * h;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      sx[q] = ((x[q] + a30h * k0x[q]) + a31h * k1x[q]) + a32h * k2x[q];
+      sv[q] = ((v[q] + a30h * k0v[q]) + a31h * k1v[q]) + a32h * k2v[q];
+    }
+    rhs(sx, sv, k3x, k3v);
+    const double a40h = dp::a40 * h, a41h = dp::a41 * h, a42h = dp::a42 * h,
+                 a43h = dp::a43 * h;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      sx[q] = (((x[q] + a40h * k0x[q]) + a41h * k1x[q]) + a42h * k2x[q]) + a43h * k3x[q];
+      sv[q] = (((v[q] + a40h * k0v[q]) + a41h * k1v[q]) + a42h * k2v[q]) + a43h * k3v[q];
+    }
+    rhs(sx, sv, k4x, k4v);
+    const double a50h = dp::a50 * h, a51h = dp::a51 * h, a52h = dp::a52 * h,
+                 a53h = dp::a53 * h, a54h = dp::a54 * h;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      sx[q] = ((((x[q] + a50h * k0x[q]) + a51h * k1x[q]) + a52h * k2x[q]) + a53h * k3x[q]) +
+              a54h * k4x[q];
+      sv[q] = ((((v[q] + a50h * k0v[q]) + a51h * k1v[q]) + a52h * k2v[q]) + a53h * k3v[q]) +
+              a54h * k4v[q];
+    }
+    rhs(sx, sv, k5x, k5v);
+    const double a60h = dp::a60 * h, a62h = dp::a62 * h, a63h = dp::a63 * h,
+                 a64h = dp::a64 * h, a65h = dp::a65 * h;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      sx[q] = ((((x[q] + a60h * k0x[q]) + a62h * k2x[q]) + a63h * k3x[q]) + a64h * k4x[q]) +
+              a65h * k5x[q];
+      sv[q] = ((((v[q] + a60h * k0v[q]) + a62h * k2v[q]) + a63h * k3v[q]) + a64h * k4v[q]) +
+              a65h * k5v[q];
+    }
+    rhs(sx, sv, k6x, k6v);
+    double segx[C], segv[C], nx[C], nv[C];
+    double err = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      segx[q] = (((dp::b0 * k0x[q] + dp::b2 * k2x[q]) + dp::b3 * k3x[q]) + dp::b4 * k4x[q]) +
+                dp::b5 * k5x[q];
+      segv[q] = (((dp::b0 * k0v[q] + dp::b2 * k2v[q]) + dp::b3 * k3v[q]) + dp::b4 * k4v[q]) +
+                dp::b5 * k5v[q];
+      nx[q] = x[q] + h * segx[q];
+      nv[q] = v[q] + h * segv[q];
+      if (l * C + q < nm) {
+        double ex = ((((dp::e0 * k0x[q] + dp::e2 * k2x[q]) + dp::e3 * k3x[q]) +
+                      dp::e4 * k4x[q]) +
+                     dp::e5 * k5x[q]) +
+                    dp::e6 * k6x[q];
+        double evv = ((((dp::e0 * k0v[q] + dp::e2 * k2v[q]) + dp::e3 * k3v[q]) +
+                       dp::e4 * k4v[q]) +
+                      dp::e5 * k5v[q]) +
+                     dp::e6 * k6v[q];
+        double e1 = ex / (atol + rtol * fmax(fabs(x[q]), fabs(nx[q])));
+        double e2 = evv / (atol + rtol * fmax(fabs(v[q]), fabs(nv[q])));
+        err += e1 * e1 + e2 * e2;
+      }
+    }
+    err = gsum(err);
+    double deno = err;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * (double)(2 * nm)));
+    double scale = a.safety * DE_CTRL_POW(err, -0.2);
+    scale = fmin(fmax(scale, a.min_scale), a.max_scale);
+    if (err <= 1.0) {
+      double t_new = t + h;
+      double dnx[C], dnv[C];
+      rhs(nx, nv, dnx, dnv);
+      if (countdown == 0) {
+        countdown = 100;
+        double stdnum = 0.0, stden = 0.0;
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+          double u = segx[q] - k6x[q], u2 = segv[q] - k6v[q];
+          stdnum += u * u + u2 * u2;
+          double w = dnx[q] - sx[q], w2 = dnv[q] - sv[q];
+          stden += w * w + w2 * w2;
+        }
+        stdnum = gsum(stdnum);
+        stden = gsum(stden);
+        if (stden > 0.0) {
+          double h_lamb = h * sqrt(stdnum / stden);
+          if (h_lamb > 6.1) {
+            nonstiff = 0;
+            stiff += 1;
+            if (stiff == 15) {
+              finish(A, kStiffness);
+              return true;
+            }
+          }
+        } else {
+          nonstiff += 1;
+          if (nonstiff == 6) stiff = 0;
+        }
+      }
+      if (a.n_eval > 0) {
+        double c1x[C], c1v[C], c2x[C], c2v[C], c3x[C], c3v[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+          double dx = nx[q] + (-1.0) * x[q], dv2 = nv[q] + (-1.0) * v[q];
+          double bx = h * k0x[q] + (-1.0) * dx, bv = h * k0v[q] + (-1.0) * dv2;
+          c1x[q] = dx;
+          c1v[q] = dv2;
+          c2x[q] = bx;
+          c2v[q] = bv;
+          c3x[q] = (dx + (-h) * dnx[q]) + (-1.0) * bx;
+          c3v[q] = (dv2 + (-h) * dnv[q]) + (-1.0) * bv;
+        }
+        const double hp = h, tp = t, cont4 = 0.0 * h;
+        emit(A, t_new, t, nx, nv, [&](double te, double* ox, double* ov) {
+          double s = (te - tp) / hp;
+          double s1 = 1.0 - s;
+#pragma unroll
+          for (int q = 0; q < C; ++q) {
+            double ax = cont4 * s1 + c3x[q];
+            ax = ax * s + c2x[q];
+            ax = ax * s1 + c1x[q];
+            ox[q] = x[q] + s * ax;
+            double av = cont4 * s1 + c3v[q];
+            av = av * s + c2v[q];
+            av = av * s1 + c1v[q];
+            ov[q] = v[q] + s * av;
+          }
+        });
+      }
+      h_prev = h;
+      t = t_new;
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        x[q] = nx[q];
+        v[q] = nv[q];
+        k0x[q] = dnx[q];
+        k0v[q] = dnv[q];
+      }
+      if (rejected_last) {
+        rejected_last = false;
+        scale = fmin(scale, 1.0);
+      }
+      acc += 1;
+      ev += 7;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+      if (fabs(tf - t) <= kTenEps) {
+        finish(A, kComplete);
+        return true;
+      }
+    } else {
+      rejected_last = true;
+      rej += 1;
+      ev += 6;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    }
+    return false;
+  }
+};
+#endif
+
+}  // namespace de

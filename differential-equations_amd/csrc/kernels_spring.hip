@@ -1,12 +1,333 @@
+// Spring-chain ensembles: uneven system sizes (2N in [8, 512]).
+//
+// Mappings (selected by de_ode_batch.mapping):
+//   DE_MAP_LANE   one thread per system, vectors in an HBM workspace (plain
+//                 launch; bit-identical to the CPU restatement).
+//   DE_MAP_GROUP  G lanes per system with C masses per lane in registers.
+//   DE_MAP_AUTO   systems are bucketed into size classes (N <= 4, 8, ..., 256);
+//                 for each class an autotuner times every (G, C, block) with
+//                 G*C equal to the class cap on a sample of the class and keeps
+//                 the fastest.  Classes run concurrently on their own streams,
+//                 each as a persistent kernel whose groups take the next system
+//                 of the class from a work counter when they finish one.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "de_launch.h"
 #include "de_spring.h"
+#include "de_spring_solvers.h"
 
 namespace de {
 namespace parity {
 
-int launch_spring(const de_ode_batch*, const int32_t*, const OdeArgs&, int, hipStream_t) {
-  return 1;
+constexpr int kSpringBlock = 256;
+
+__global__ void fill_nan_dense(double* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = __longlong_as_double(0x7ff8000000000000LL);
 }
-int64_t spring_workspace_bytes(const de_ode_batch*) { return 0; }
+
+__global__ __launch_bounds__(kSpringBlock) void spring_lane_kernel(SpringArgs A) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.o.n) return;
+  VecDopri5 s;
+  if (!s.init(&A, i)) return;
+  while (!s.advance()) {
+  }
+}
+
+template <int G, int C>
+__global__ __launch_bounds__(kSpringBlock) void spring_group_kernel(SpringArgs A) {
+  const int lane = __lane_id();
+  const int l = lane & (G - 1);
+  const int leader_lane = lane - l;
+  GroupDopri5<G, C> S;
+  bool have = false;
+  bool drained = false;
+  const unsigned long long len = (unsigned long long)A.list_len;
+  for (;;) {
+    if (!drained) {
+      bool need = !have;
+      unsigned long long mask = __ballot(need && l == 0);
+      if (mask) {
+        int leader = __ffsll((long long)mask) - 1;
+        int cnt = __popcll(mask);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(A.o.counter, (unsigned long long)cnt);
+        base = __shfl(base, leader);
+        if (base + (unsigned long long)cnt >= len) drained = true;
+        if (need) {
+          unsigned long long below =
+              leader_lane == 0 ? 0ull : (mask & ((1ull << leader_lane) - 1ull));
+          unsigned long long my = base + (unsigned long long)__popcll(below);
+          if (my < len) have = S.init(A, (int64_t)A.list[my], l);
+        }
+      }
+    }
+    if (!__any(have)) {
+      if (drained) break;
+      continue;
+    }
+    if (have) {
+      if (S.advance(A)) have = false;
+    }
+  }
+}
+
+__global__ void spring_zero_counter(unsigned long long* c) { *c = 0ull; }
+
+typedef void (*GroupKernel)(SpringArgs);
+
+struct Config {
+  int g, c;
+  GroupKernel k;
+};
+
+#define DE_CFG(G, C) {G, C, spring_group_kernel<G, C>}
+static const Config kConfigs[] = {
+    DE_CFG(4, 1),  DE_CFG(8, 1),  DE_CFG(16, 1), DE_CFG(32, 1), DE_CFG(64, 1),
+    DE_CFG(2, 2),  DE_CFG(4, 2),  DE_CFG(8, 2),  DE_CFG(16, 2), DE_CFG(32, 2), DE_CFG(64, 2),
+    DE_CFG(1, 4),  DE_CFG(2, 4),  DE_CFG(4, 4),  DE_CFG(8, 4),  DE_CFG(16, 4), DE_CFG(32, 4),
+    DE_CFG(64, 4),
+};
+#undef DE_CFG
+static const int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
+static const int kCaps[] = {4, 8, 16, 32, 64, 128, 256};
+static const int kNumCaps = 7;
+static const int kBlocks[] = {64, 256};
+
+struct Choice {
+  int cfg = -1;
+  int block = kSpringBlock;
+};
+
+// Process-wide state: autotune decisions per size class, streams, workspace.
+static std::mutex g_mu;
+static std::map<int, Choice> g_tuned;
+static hipStream_t g_streams[kNumCaps] = {};
+static hipEvent_t g_fork = nullptr, g_join[kNumCaps] = {};
+static unsigned long long* g_counters = nullptr;
+static double* g_work = nullptr;
+static int64_t g_work_len = 0;
+static int g_num_cu = 0;
+
+static int cap_index(int nm) {
+  for (int k = 0; k < kNumCaps; ++k)
+    if (nm <= kCaps[k]) return k;
+  return -1;
+}
+
+static int ensure_runtime() {
+  if (g_fork) return 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return -1;
+  for (int k = 0; k < kNumCaps; ++k) {
+    if (hipStreamCreateWithFlags(&g_streams[k], hipStreamNonBlocking) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&g_join[k], hipEventDisableTiming) != hipSuccess) return -1;
+  }
+  if (hipEventCreateWithFlags(&g_fork, hipEventDisableTiming) != hipSuccess) return -1;
+  if (hipMalloc(&g_counters, sizeof(unsigned long long) * (kNumCaps + 1)) != hipSuccess) return -1;
+  return 0;
+}
+
+static int blocks_per_cu(GroupKernel k, int block) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, block, 0) != hipSuccess || b < 1)
+    b = 1;
+  return b;
+}
+
+static int launch_class(const SpringArgs& base, const int32_t* d_list, int64_t len,
+                        int64_t work_share_blocks, const Choice& ch, unsigned long long* counter,
+                        hipStream_t s) {
+  const Config& cf = kConfigs[ch.cfg];
+  SpringArgs A = base;
+  A.list = d_list;
+  A.list_len = len;
+  A.o.counter = counter;
+  int64_t groups_per_block = ch.block / cf.g;
+  int64_t need = (len + groups_per_block - 1) / groups_per_block;
+  int64_t grid = work_share_blocks < need ? work_share_blocks : need;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(spring_zero_counter, dim3(1), dim3(1), 0, s, counter);
+  hipLaunchKernelGGL(cf.k, dim3((unsigned)grid), dim3(ch.block), 0, s, A);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Time every candidate for size class k on (up to) `sample` systems.
+static Choice autotune(const SpringArgs& base, int k, const int32_t* d_list, int64_t len,
+                       hipStream_t s) {
+  Choice best;
+  float best_ms = 1e30f;
+  const int64_t sample = len < 1024 ? len : 1024;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return best;
+  for (int ci = 0; ci < kNumConfigs; ++ci) {
+    if (kConfigs[ci].g * kConfigs[ci].c != kCaps[k]) continue;
+    for (int block : kBlocks) {
+      if (block < kConfigs[ci].g) continue;
+      Choice ch;
+      ch.cfg = ci;
+      ch.block = block;
+      int64_t full = (int64_t)g_num_cu * blocks_per_cu(kConfigs[ci].k, block);
+      (void)hipEventRecord(e0, s);
+      if (launch_class(base, d_list, sample, full, ch, g_counters + kNumCaps, s) != 0) continue;
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) continue;
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = ch;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+int64_t spring_workspace_bytes(const de_ode_batch* b) {
+  if (!b) return 0;
+  if (b->mapping == DE_MAP_LANE) {
+    // 9 state-length vectors per system; offsets are device memory here, so
+    // the caller's bound uses 512 doubles per system.
+    return (int64_t)9 * 512 * 8 * b->n;
+  }
+  return (int64_t)sizeof(int32_t) * b->n + (int64_t)sizeof(unsigned long long) * (kNumCaps + 1);
+}
+
+int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArgs& a,
+                  int sched, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ensure_runtime() != 0) return -1;
+  const int64_t n = b->n;
+  SpringArgs base;
+  base.o = a;
+  base.sizes = b->sizes;
+  base.state_offset = b->state_offset;
+  base.param_offset = b->param_offset;
+  base.work = nullptr;
+  base.list = nullptr;
+  base.list_len = 0;
+  // Host-side size scan: validity, total state length, classes.
+  int64_t state_len = 0;
+  std::vector<std::vector<int32_t>> lists(kNumCaps);
+  std::vector<int64_t> masses(kNumCaps, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    int k = cap_index(host_sizes[i]);
+    if (host_sizes[i] < 1 || k < 0) return 1;
+    lists[k].push_back((int32_t)i);
+    masses[k] += host_sizes[i];
+    state_len += 2 * (int64_t)host_sizes[i];
+  }
+  if (a.n_eval > 0 && a.dense) {
+    int64_t len = (int64_t)a.n_eval * state_len;
+    int64_t grid = (len + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(fill_nan_dense, dim3((unsigned)grid), dim3(256), 0, s, a.dense, len);
+  }
+  if (b->mapping == DE_MAP_LANE) {
+    if (g_work_len < 9 * state_len) {
+      if (g_work) (void)hipFree(g_work);
+      g_work = nullptr;
+      g_work_len = 0;
+      if (hipMalloc(&g_work, sizeof(double) * 9 * state_len) != hipSuccess) return -4;
+      g_work_len = 9 * state_len;
+    }
+    base.work = g_work;
+    int64_t grid = (n + kSpringBlock - 1) / kSpringBlock;
+    hipLaunchKernelGGL(spring_lane_kernel, dim3((unsigned)grid), dim3(kSpringBlock), 0, s, base);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
+  // Group mapping: upload class lists (one buffer, classes back to back).
+  std::vector<int32_t> all;
+  std::vector<int64_t> start(kNumCaps + 1, 0);
+  for (int k = 0; k < kNumCaps; ++k) {
+    start[k] = (int64_t)all.size();
+    all.insert(all.end(), lists[k].begin(), lists[k].end());
+  }
+  start[kNumCaps] = (int64_t)all.size();
+  int32_t* d_list = nullptr;
+  if (hipMalloc(&d_list, sizeof(int32_t) * (all.size() + 1)) != hipSuccess) return -4;
+  if (hipMemcpy(d_list, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    (void)hipFree(d_list);
+    return -1;
+  }
+  // Choose a configuration per class.
+  std::vector<Choice> choice(kNumCaps);
+  for (int k = 0; k < kNumCaps; ++k) {
+    if (lists[k].empty()) continue;
+    Choice ch;
+    if (b->mapping == DE_MAP_GROUP && b->group_size > 0) {
+      for (int ci = 0; ci < kNumConfigs; ++ci)
+        if (kConfigs[ci].g == b->group_size && kConfigs[ci].g * kConfigs[ci].c == kCaps[k])
+          ch.cfg = ci;
+      if (ch.cfg < 0) {
+        // group size too small for this class: largest C that covers the cap
+        for (int ci = 0; ci < kNumConfigs; ++ci)
+          if (kConfigs[ci].g * kConfigs[ci].c == kCaps[k] &&
+              (ch.cfg < 0 || kConfigs[ci].c > kConfigs[ch.cfg].c))
+            ch.cfg = ci;
+      }
+    } else {
+      auto it = g_tuned.find(k);
+      if (it != g_tuned.end()) {
+        ch = it->second;
+      } else {
+        ch = autotune(base, k, d_list + start[k], (int64_t)lists[k].size(), s);
+        if (ch.cfg < 0) {
+          (void)hipFree(d_list);
+          return -1;
+        }
+        g_tuned[k] = ch;
+      }
+    }
+    choice[k] = ch;
+  }
+  // Launch classes concurrently, largest first, with blocks shared by work.
+  int64_t total_mass = 0;
+  for (int k = 0; k < kNumCaps; ++k) total_mass += masses[k];
+  if (hipEventRecord(g_fork, s) != hipSuccess) return -1;
+  for (int k = kNumCaps - 1; k >= 0; --k) {
+    if (lists[k].empty()) continue;
+    const Choice& ch = choice[k];
+    int64_t full = (int64_t)g_num_cu * blocks_per_cu(kConfigs[ch.cfg].k, ch.block);
+    int64_t share = (int64_t)((double)full * (double)masses[k] / (double)total_mass) + 1;
+    if (hipStreamWaitEvent(g_streams[k], g_fork, 0) != hipSuccess) return -1;
+    if (launch_class(base, d_list + start[k], (int64_t)lists[k].size(), share, ch,
+                     g_counters + k, g_streams[k]) != 0)
+      return -1;
+    if (hipEventRecord(g_join[k], g_streams[k]) != hipSuccess) return -1;
+    if (hipStreamWaitEvent(s, g_join[k], 0) != hipSuccess) return -1;
+  }
+  // The list buffer is released once the classes are done.
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  (void)hipFree(d_list);
+  return 0;
+}
+
+// Autotuner introspection: chosen (G, C, block) for size class cap, or -1.
+int spring_tuned(int cap, int* g, int* c, int* block) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int k = 0; k < kNumCaps; ++k) {
+    if (kCaps[k] != cap) continue;
+    auto it = g_tuned.find(k);
+    if (it == g_tuned.end()) return -1;
+    *g = kConfigs[it->second.cfg].g;
+    *c = kConfigs[it->second.cfg].c;
+    *block = it->second.block;
+    return 0;
+  }
+  return -1;
+}
 
 }  // namespace parity
 }  // namespace de

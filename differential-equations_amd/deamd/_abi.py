@@ -163,6 +163,8 @@ SIGNATURES = {
     "de_noise_normal": (C.c_double, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]),
     "de_diag_pow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "de_diag_arith": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "de_spring_tuned": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)]),
     "de_diag_noise": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int64]),
 }

@@ -62,6 +62,11 @@ int de_diag_arith(const double* a, const double* b, const double* c, double* out
 int de_diag_noise(uint64_t seed, const uint64_t* path, const uint32_t* step,
                   const uint32_t* comp, double* out, int64_t n);
 
+/* Spring-chain autotuner decision for the size class whose largest chain has
+ * `cap` masses (4, 8, ..., 256): lanes per system G, masses per lane C and
+ * block size.  Returns 0, or -1 if the class has not been tuned yet. */
+int de_spring_tuned(int32_t cap, int32_t* g, int32_t* c, int32_t* block);
+
 #ifdef __cplusplus
 }
 #endif
