@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-sample-stride", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window")
+    ap.add_argument("--serve-graph", action="store_true", help="replay chunks from a hipGraph")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC traffic calibration)")
     return ap.parse_args()
@@ -435,6 +438,88 @@ def bench_sde(args, dv):
     }
 
 
+# ---------------------------------------------------------------- serve ---
+def bench_serve(args, dv):
+    """Poisson request stream through the serving loop.  Each request: one
+    family (Lorenz, Van der Pol mu ~ LogUniform[0.1, 10], spring chain
+    N in [4, 256]), ensemble size log-uniform in [1, 65 536], rtol in
+    {1e-6, 1e-8} with atol = rtol/100, t_final ~ U[1, 50]."""
+    import threading
+    import numpy as np
+    import deamd
+    rng = np.random.default_rng(SEED)
+    big = 65536
+    lp, ly0 = deamd.gen_lorenz(big, seed=SEED)
+    vp, vy0 = deamd.gen_vdp(big, seed=SEED)
+    sp, sy0 = deamd.gen_spring(big, seed=SEED)
+    n_req = int(args.serve_rate * args.serve_seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n_req))
+    reqs = []
+    for k in range(n_req):
+        fam = int(rng.integers(0, 3))
+        n = int(np.exp(rng.uniform(0, np.log(big))))
+        a = int(rng.integers(0, big - n + 1))
+        rtol = float(rng.choice([1e-6, 1e-8]))
+        tf = float(rng.uniform(1.0, 50.0))
+        m = deamd.ExplicitRungeKutta.dopri5().rtol(rtol).atol(rtol * 1e-2).max_steps(1000000)
+        if fam == 0:
+            e = deamd.IVP.ode(deamd.Lorenz(*lp[:, a:a + n]), 0.0, tf, ly0[:, a:a + n]).method(m)
+        elif fam == 1:
+            e = deamd.IVP.ode(deamd.VanDerPol(*vp[:, a:a + n]), 0.0, tf, vy0[:, a:a + n]).method(m)
+        else:
+            sizes = sp.sizes[a:a + n]
+            params = sp.params_packed[sp.param_offset[a]:sp.param_offset[a + n]]
+            y0 = sy0[sp.state_offset[a]:sp.state_offset[a + n]]
+            e = deamd.IVP.ode(deamd.SpringChain(sizes, params), 0.0, tf, y0).method(m)
+        reqs.append((fam, n, e))
+    srv = deamd.Server(128 << 30, graph=args.serve_graph)
+    tickets = [None] * n_req
+    t_start = time.perf_counter()
+
+    def submitter():
+        for k, (fam, n, e) in enumerate(reqs):
+            delay = t_start + arrivals[k] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[k] = srv.submit(e)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    steps = 0
+    lat = []
+    fam_lat = {0: [], 1: [], 2: []}
+    for k in range(n_req):
+        while tickets[k] is None:
+            time.sleep(0.0005)
+        sol = srv.poll(tickets[k])
+        steps += int(sol.accepted.sum())
+        l = srv.latency(tickets[k])
+        lat.append(l)
+        fam_lat[reqs[k][0]].append(l)
+    t_end = time.perf_counter()
+    th.join()
+    srv.close()
+    elapsed = t_end - t_start
+    ms = lambda v: 1000.0 * v if v is not None else None
+    traj = sum(r[1] for r in reqs)
+    return {
+        "metric": "served accepted steps*traj/s FP64 DOPRI5 with p50/p99 request latency, 1 GPU",
+        "value": steps / elapsed, "unit": "accepted_steps*traj/s", "n_gpus": 1,
+        "steps": n_req, "warmup": 0, "ms_per_step": 1000.0 * elapsed / n_req,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Poisson arrivals and Philox4x32-10 inputs)",
+        "config": {"workload": "serve_poisson", "rate_req_per_s": args.serve_rate,
+                   "arrival_window_s": args.serve_seconds, "requests": n_req,
+                   "trajectories": traj, "graph": bool(args.serve_graph),
+                   "wall_s": elapsed, "accepted_steps": steps,
+                   "latency_ms_by_family": {
+                       name: {"p50": ms(percentile(fam_lat[i], 50)),
+                              "p99": ms(percentile(fam_lat[i], 99)), "n": len(fam_lat[i])}
+                       for i, name in ((0, "lorenz"), (1, "vdp"), (2, "spring"))}},
+        "latency_ms": {"p50": ms(percentile(lat, 50)), "p99": ms(percentile(lat, 99))},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -451,6 +536,8 @@ def main():
         line = bench_spring(args, dv)
     elif args.workload == "sde":
         line = bench_sde(args, dv)
+    elif args.workload == "serve":
+        line = bench_serve(args, dv)
     else:
         raise SystemExit("workload %s is not available yet" % args.workload)
     print(json.dumps(line), flush=True)

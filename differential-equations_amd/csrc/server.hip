@@ -1,17 +1,786 @@
-// Serving loop (continuous batching) -- entry points.
+// Serving: continuous batching of ensemble solves on one GPU.
+//
+// The reference has no serving layer; this is the request loop that admits
+// ensemble solves (IVP::ode(...).method(dopri5()).solve() batches) of
+// different families, sizes and tolerances into one running device.
+//
+// Device side.  Each family (Lorenz, Van der Pol, and one per spring-chain size
+// class) owns a pool of solver slots whose complete state (a Dopri5Lane or a
+// GroupDopri5 per lane) stays in HBM between launches, and a work queue: an
+// append-only array of ranges (request, trajectories) whose trajectories are
+// numbered by a running ticket.  A chunk kernel runs every slot for up to K
+// attempts; a slot that finishes (or is idle) takes the next ticket with one
+// wave-aggregated atomic and starts that trajectory in the same launch, so
+// requests admitted while others run share the lanes immediately.  A ticket
+// beyond the published range is held until the host publishes more work.
+//
+// Host side.  A scheduler thread admits requests first-come-first-served while
+// their device footprint (inputs, outputs, dense output) fits the HBM budget
+// (first-fit arena inside one allocation), uploads inputs, appends ranges and
+// publishes them, launches one chunk per family with work (concurrent streams,
+// optionally replayed from a captured hipGraph), reads back the per-request
+// remaining counts, and retires finished requests by copying their results to
+// host memory.  Every trajectory's arithmetic depends only on its own inputs,
+// so a request's results do not depend on what else is being served.
 #include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 #include "../../include/de_api.h"
+#include "de_launch.h"
+#include "de_spring_solvers.h"
+
+namespace de {
+namespace serve {
+
+constexpr int kMaxReq = 8192;     // requests resident at once
+constexpr int kMaxRanges = 1 << 20;
+constexpr int kChunk = 256;       // attempts per slot per launch
+constexpr int kBlock = 256;
+
+struct Range {
+  int64_t start;       // first ticket
+  int32_t req;         // request slot
+  int32_t first;       // first trajectory (index into list when list != null)
+  int32_t count;
+  int32_t pad;
+  const int32_t* list; // spring: system indices of this size class
+};
+
+struct Queue {
+  unsigned long long next;  // next ticket to hand out
+  int64_t published;        // tickets below this are backed by a range
+  int64_t nranges;
+};
+
+struct ReqDev {
+  SpringArgs a;  // lane families use a.o only
+  int32_t remaining;
+  int32_t pad;
+};
+
+template <class L>
+struct LaneSlot {
+  L s;
+  int64_t ticket;
+  int32_t req, traj, state;  // state 0 idle, 1 holding ticket, 2 running
+};
+
+DE_DEV int find_range(const Range* r, int64_t n, int64_t ticket) {
+  int64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (r[mid].start <= ticket) lo = mid;
+    else hi = mid - 1;
+  }
+  return (int)lo;
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void chunk_lane(Queue* q, const Range* ranges, ReqDev* reqs,
+                                                    LaneSlot<Dopri5Lane<F>>* slots, int nslots,
+                                                    int k_iters) {
+  const int slot = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = slot < nslots;
+  const int lane = __lane_id();
+  LaneSlot<Dopri5Lane<F>> S;
+  if (valid) S = slots[slot];
+  else S.state = -1;
+  const int64_t published = *(volatile int64_t*)&q->published;
+  const int64_t nranges = *(volatile int64_t*)&q->nranges;
+  for (int it = 0; it < k_iters; ++it) {
+    bool need = S.state == 0;
+    unsigned long long mask = __ballot(need);
+    if (mask) {
+      int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(&q->next, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader);
+      if (need) {
+        S.ticket = (int64_t)(base + __popcll(mask & __lanemask_lt()));
+        S.state = 1;
+      }
+    }
+    if (S.state == 1 && S.ticket < published) {
+      int r = find_range(ranges, nranges, S.ticket);
+      S.req = ranges[r].req;
+      S.traj = ranges[r].first + (int32_t)(S.ticket - ranges[r].start);
+      if (S.s.init(reqs[S.req].a.o, S.traj)) {
+        S.state = 2;
+      } else {
+        atomicSub(&reqs[S.req].remaining, 1);
+        S.state = 0;
+      }
+    }
+    bool act = S.state == 2;
+    if (!__any(act)) {
+      if (!__any(S.state == 0 || (S.state == 1 && S.ticket < published))) break;
+      continue;
+    }
+    if (act && S.s.advance(reqs[S.req].a.o, S.traj)) {
+      atomicSub(&reqs[S.req].remaining, 1);
+      S.state = 0;
+    }
+  }
+  if (valid) slots[slot] = S;
+}
+
+template <int G, int C>
+__global__ __launch_bounds__(kBlock) void chunk_group(Queue* q, const Range* ranges,
+                                                     ReqDev* reqs,
+                                                     LaneSlot<GroupDopri5<G, C>>* slots,
+                                                     int nslots, int k_iters) {
+  const int slot = blockIdx.x * kBlock + threadIdx.x;  // one per lane; G lanes per system
+  const bool valid = slot < nslots;
+  const int lane = __lane_id();
+  const int l = lane & (G - 1);
+  const int leader_lane = lane - l;
+  LaneSlot<GroupDopri5<G, C>> S;
+  if (valid) S = slots[slot];
+  else S.state = -1;
+  const int64_t published = *(volatile int64_t*)&q->published;
+  const int64_t nranges = *(volatile int64_t*)&q->nranges;
+  for (int it = 0; it < k_iters; ++it) {
+    bool need = S.state == 0;
+    unsigned long long mask = __ballot(need && l == 0);
+    if (mask) {
+      int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(&q->next, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader);
+      if (need) {
+        unsigned long long below =
+            leader_lane == 0 ? 0ull : (mask & ((1ull << leader_lane) - 1ull));
+        S.ticket = (int64_t)(base + __popcll(below));
+        S.state = 1;
+      }
+    }
+    if (S.state == 1 && S.ticket < published) {
+      int r = find_range(ranges, nranges, S.ticket);
+      S.req = ranges[r].req;
+      S.traj = ranges[r].list[ranges[r].first + (int32_t)(S.ticket - ranges[r].start)];
+      if (S.s.init(reqs[S.req].a, S.traj, l)) {
+        S.state = 2;
+      } else {
+        if (l == 0) atomicSub(&reqs[S.req].remaining, 1);
+        S.state = 0;
+      }
+    }
+    bool act = S.state == 2;
+    if (!__any(act)) {
+      if (!__any(S.state == 0 || (S.state == 1 && S.ticket < published))) break;
+      continue;
+    }
+    if (act && S.s.advance(reqs[S.req].a)) {
+      if (l == 0) atomicSub(&reqs[S.req].remaining, 1);
+      S.state = 0;
+    }
+  }
+  if (valid) slots[slot] = S;
+}
+
+template <class Slot>
+__global__ void init_slots(Slot* s, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    s[i].state = 0;
+    s[i].ticket = 0;
+    s[i].req = 0;
+    s[i].traj = 0;
+  }
+}
+
+// ---------------------------------------------------------------- host -----
+typedef void (*ChunkFn)(Queue*, const Range*, ReqDev*, void*, int, int);
+
+struct Family {
+  const char* name;
+  int cap;         // spring: class cap; 0 for lane families
+  int g;           // lanes per system (1 for lane families)
+  size_t slot_bytes;
+  int nslots;      // lanes
+  void* slots;
+  Queue* q;        // device
+  Range* ranges;   // device
+  std::vector<Range> host_ranges;
+  int64_t published = 0;
+  int64_t active_tickets = 0;  // published but possibly unfinished
+  hipStream_t stream;
+  void (*launch)(Family&, ReqDev*, int, hipStream_t);
+};
+
+template <class F>
+void launch_lane(Family& f, ReqDev* reqs, int k, hipStream_t s) {
+  int grid = (f.nslots + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(chunk_lane<F>, dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
+                     (LaneSlot<Dopri5Lane<F>>*)f.slots, f.nslots, k);
+}
+
+template <int G, int C>
+void launch_group(Family& f, ReqDev* reqs, int k, hipStream_t s) {
+  int grid = (f.nslots + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL((chunk_group<G, C>), dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
+                     (LaneSlot<GroupDopri5<G, C>>*)f.slots, f.nslots, k);
+}
+
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+  bool init(size_t bytes) {
+    if (hipMalloc(&base, bytes) != hipSuccess) return false;
+    size = bytes;
+    free_[0] = bytes;
+    return true;
+  }
+  void* alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        size_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > bytes) free_[off + bytes] = len - bytes;
+        return base + off;
+      }
+    }
+    return nullptr;
+  }
+  void release(void* p, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    size_t off = (char*)p - base;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Request {
+  uint64_t ticket;
+  de_ode_batch b;            // host copy; pointers below
+  std::vector<double> params, y0, t0, tf, rtol, atol, t_eval;
+  std::vector<int32_t> sizes;
+  std::vector<int64_t> so, po;
+  int64_t state_len = 0;
+  // results (host)
+  std::vector<double> y, t, dense;
+  std::vector<int64_t> acc, rej, ev;
+  std::vector<int32_t> status, n_dense;
+  // device
+  void* dmem = nullptr;
+  size_t dbytes = 0;
+  int slot = -1;
+  bool done = false;
+  int error = 0;
+  double t_submit = 0, t_done = 0;
+};
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace serve
+}  // namespace de
+
+using namespace de::serve;
+
+struct de_server {
+  size_t budget = 0;
+  int32_t flags = 0;
+  Arena arena;
+  std::vector<Family> fams;
+  de::serve::ReqDev* d_reqs = nullptr;
+  std::vector<de::serve::ReqDev> h_reqs;
+  int32_t* h_remaining = nullptr;  // pinned mirror
+  std::vector<int> free_slots;
+  std::map<int, Request*> active;  // request slot -> request
+  std::deque<Request*> pending;
+  std::map<uint64_t, Request*> all;
+  uint64_t next_ticket = 1;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  std::thread thread;
+  hipStream_t ctl;
+  hipEvent_t fork, join_ev;
+  std::vector<hipEvent_t> fam_done;
+  hipGraphExec_t graph = nullptr;
+  int iterations = 0;
+  int error = 0;
+};
+
+namespace {
+
+enum : int32_t { kServerGraph = 1 };
+
+int fam_index_for(de_server* s, int family, int nm) {
+  if (family == DE_FAMILY_LORENZ) return 0;
+  if (family == DE_FAMILY_VAN_DER_POL) return 1;
+  if (family == DE_FAMILY_SPRING_CHAIN) {
+    for (size_t k = 2; k < s->fams.size(); ++k)
+      if (nm <= s->fams[k].cap) return (int)k;
+  }
+  return -1;
+}
+
+size_t footprint(const Request* r) {
+  const de_ode_batch& b = r->b;
+  size_t n = (size_t)b.n;
+  size_t bytes = 0;
+  bytes += r->params.size() * 8 + r->y0.size() * 8 + r->t0.size() * 8 + r->tf.size() * 8;
+  bytes += r->rtol.size() * 8 + r->atol.size() * 8 + r->t_eval.size() * 8;
+  bytes += r->sizes.size() * 4 + (r->so.size() + r->po.size()) * 8 + n * 4;  // + class lists
+  bytes += (size_t)r->state_len * 8 + n * (8 * 4 + 4 + 4);
+  bytes += (size_t)b.n_eval * r->state_len * 8;
+  return bytes + 64 * 256;
+}
+
+// Upload a request into the arena and build its device descriptor.
+bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_lists,
+            int32_t** d_lists) {
+  size_t bytes = footprint(r);
+  char* p = (char*)s->arena.alloc(bytes);
+  if (!p) return false;
+  r->dmem = p;
+  r->dbytes = bytes;
+  auto put = [&](const void* src, size_t nb) -> void* {
+    if (!nb) return nullptr;
+    void* d = p;
+    (void)hipMemcpyAsync(d, src, nb, hipMemcpyHostToDevice, s->ctl);
+    p += (nb + 255) & ~size_t(255);
+    return d;
+  };
+  auto take = [&](size_t nb) -> void* {
+    if (!nb) return nullptr;
+    void* d = p;
+    p += (nb + 255) & ~size_t(255);
+    return d;
+  };
+  const de_ode_batch& b = r->b;
+  const int64_t n = b.n;
+  ReqDev rd;
+  memset(&rd, 0, sizeof rd);
+  de::OdeArgs& a = rd.a.o;
+  a.n = n;
+  a.params = (const double*)put(r->params.data(), r->params.size() * 8);
+  a.y0 = (const double*)put(r->y0.data(), r->y0.size() * 8);
+  a.t0 = (const double*)put(r->t0.data(), r->t0.size() * 8);
+  a.tf = (const double*)put(r->tf.data(), r->tf.size() * 8);
+  a.t0s = b.t0_scalar;
+  a.tfs = b.tf_scalar;
+  a.rtol_traj = (const double*)put(r->rtol.data(), r->rtol.size() * 8);
+  a.atol_traj = (const double*)put(r->atol.data(), r->atol.size() * 8);
+  a.rtol = b.rtol;
+  a.atol = b.atol;
+  a.h0 = b.h0;
+  a.h_min = b.h_min;
+  a.h_max = b.h_max;
+  a.safety = b.safety_factor;
+  a.min_scale = b.min_scale;
+  a.max_scale = b.max_scale;
+  a.max_steps = b.max_steps;
+  a.t_eval = (const double*)put(r->t_eval.data(), r->t_eval.size() * 8);
+  a.n_eval = b.n_eval;
+  a.y_final = (double*)take((size_t)r->state_len * 8);
+  a.t_final = (double*)take(n * 8);
+  a.accepted = (int64_t*)take(n * 8);
+  a.rejected = (int64_t*)take(n * 8);
+  a.evals = (int64_t*)take(n * 8);
+  a.status = (int32_t*)take(n * 4);
+  a.n_dense = b.n_eval > 0 ? (int32_t*)take(n * 4) : nullptr;
+  a.dense = b.n_eval > 0 ? (double*)take((size_t)b.n_eval * r->state_len * 8) : nullptr;
+  if (a.dense) {
+    std::vector<double> nanv((size_t)b.n_eval * r->state_len, __builtin_nan(""));
+    (void)hipMemcpyAsync(a.dense, nanv.data(), nanv.size() * 8, hipMemcpyHostToDevice, s->ctl);
+    (void)hipStreamSynchronize(s->ctl);
+  }
+  if (b.family == DE_FAMILY_SPRING_CHAIN) {
+    rd.a.sizes = (const int32_t*)put(r->sizes.data(), r->sizes.size() * 4);
+    rd.a.state_offset = (const int64_t*)put(r->so.data(), r->so.size() * 8);
+    rd.a.param_offset = (const int64_t*)put(r->po.data(), r->po.size() * 8);
+    for (size_t k = 0; k < class_lists.size(); ++k)
+      d_lists[k] = (int32_t*)put(class_lists[k].data(), class_lists[k].size() * 4);
+  }
+  rd.remaining = (int32_t)n;
+  s->h_reqs[r->slot] = rd;
+  (void)hipMemcpyAsync(s->d_reqs + r->slot, &s->h_reqs[r->slot], sizeof rd, hipMemcpyHostToDevice,
+                       s->ctl);
+  s->h_remaining[r->slot] = (int32_t)n;
+  return true;
+}
+
+void publish(de_server* s, Family& f, int req_slot, int32_t first, int32_t count,
+             const int32_t* list) {
+  Range rg;
+  rg.start = f.published;
+  rg.req = req_slot;
+  rg.first = first;
+  rg.count = count;
+  rg.pad = 0;
+  rg.list = list;
+  int64_t idx = (int64_t)f.host_ranges.size();
+  f.host_ranges.push_back(rg);
+  (void)hipMemcpyAsync(f.ranges + idx, &f.host_ranges[idx], sizeof rg, hipMemcpyHostToDevice,
+                       s->ctl);
+  f.published += count;
+  de::serve::Queue qh;
+  qh.published = f.published;
+  qh.nranges = idx + 1;
+  (void)hipMemcpyAsync(&f.q->published, &qh.published, 16, hipMemcpyHostToDevice, s->ctl);
+  (void)hipStreamSynchronize(s->ctl);
+}
+
+void retire(de_server* s, Request* r) {
+  const ReqDev& rd = s->h_reqs[r->slot];
+  const de::OdeArgs& a = rd.a.o;
+  const int64_t n = r->b.n;
+  r->y.resize(r->state_len);
+  r->t.resize(n);
+  r->acc.resize(n);
+  r->rej.resize(n);
+  r->ev.resize(n);
+  r->status.resize(n);
+  (void)hipMemcpyAsync(r->y.data(), a.y_final, r->state_len * 8, hipMemcpyDeviceToHost, s->ctl);
+  (void)hipMemcpyAsync(r->t.data(), a.t_final, n * 8, hipMemcpyDeviceToHost, s->ctl);
+  (void)hipMemcpyAsync(r->acc.data(), a.accepted, n * 8, hipMemcpyDeviceToHost, s->ctl);
+  (void)hipMemcpyAsync(r->rej.data(), a.rejected, n * 8, hipMemcpyDeviceToHost, s->ctl);
+  (void)hipMemcpyAsync(r->ev.data(), a.evals, n * 8, hipMemcpyDeviceToHost, s->ctl);
+  (void)hipMemcpyAsync(r->status.data(), a.status, n * 4, hipMemcpyDeviceToHost, s->ctl);
+  if (a.dense) {
+    r->dense.resize((size_t)r->b.n_eval * r->state_len);
+    r->n_dense.resize(n);
+    (void)hipMemcpyAsync(r->dense.data(), a.dense, r->dense.size() * 8, hipMemcpyDeviceToHost,
+                         s->ctl);
+    (void)hipMemcpyAsync(r->n_dense.data(), a.n_dense, n * 4, hipMemcpyDeviceToHost, s->ctl);
+  }
+  if (hipStreamSynchronize(s->ctl) != hipSuccess) r->error = DE_ERR_HIP;
+  s->arena.release(r->dmem, r->dbytes);
+  r->dmem = nullptr;
+}
+
+void launch_all(de_server* s) {
+  (void)hipEventRecord(s->fork, s->ctl);
+  for (size_t k = 0; k < s->fams.size(); ++k) {
+    Family& f = s->fams[k];
+    (void)hipStreamWaitEvent(f.stream, s->fork, 0);
+    f.launch(f, s->d_reqs, kChunk, f.stream);
+    (void)hipEventRecord(s->fam_done[k], f.stream);
+    (void)hipStreamWaitEvent(s->ctl, s->fam_done[k], 0);
+  }
+}
+
+// Capture one iteration (chunk of every family) into a graph.
+bool capture(de_server* s) {
+  hipGraph_t g;
+  if (hipStreamBeginCapture(s->ctl, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  launch_all(s);
+  if (hipStreamEndCapture(s->ctl, &g) != hipSuccess) return false;
+  bool ok = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0) == hipSuccess;
+  (void)hipGraphDestroy(g);
+  return ok;
+}
+
+void scheduler(de_server* s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    // Admission: first come, first served, while the footprint fits.
+    while (!s->pending.empty() && !s->free_slots.empty()) {
+      bool ranges_full = false;
+      for (auto& f : s->fams)
+        if ((int64_t)f.host_ranges.size() + 1 >= kMaxRanges) ranges_full = true;
+      if (ranges_full) break;
+      Request* r = s->pending.front();
+      std::vector<std::vector<int32_t>> lists(s->fams.size());
+      const de_ode_batch& b = r->b;
+      if (b.family == DE_FAMILY_SPRING_CHAIN) {
+        for (int64_t i = 0; i < b.n; ++i)
+          lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i])].push_back((int32_t)i);
+      }
+      r->slot = s->free_slots.back();
+      std::vector<int32_t*> d_lists(s->fams.size(), nullptr);
+      if (!upload(s, r, lists, d_lists.data())) {
+        r->slot = -1;
+        if (s->active.empty()) {  // can never fit
+          s->pending.pop_front();
+          r->error = DE_ERR_NOMEM;
+          r->done = true;
+          r->t_done = now_s();
+          s->cv_done.notify_all();
+          continue;
+        }
+        break;
+      }
+      s->free_slots.pop_back();
+      s->pending.pop_front();
+      s->active[r->slot] = r;
+      if (b.family == DE_FAMILY_SPRING_CHAIN) {
+        for (size_t k = 2; k < s->fams.size(); ++k)
+          if (!lists[k].empty())
+            publish(s, s->fams[k], r->slot, 0, (int32_t)lists[k].size(), d_lists[k]);
+      } else {
+        publish(s, s->fams[fam_index_for(s, b.family, 0)], r->slot, 0, (int32_t)b.n, nullptr);
+      }
+    }
+    if (s->active.empty()) {
+      if (s->stop) break;
+      s->cv_work.wait(lk);
+      continue;
+    }
+    lk.unlock();
+    if (s->graph) {
+      (void)hipGraphLaunch(s->graph, s->ctl);
+    } else {
+      launch_all(s);
+    }
+    // Remaining counts of the active requests.
+    for (auto& kv : s->active)
+      (void)hipMemcpyAsync(&s->h_remaining[kv.first], &s->d_reqs[kv.first].remaining, 4,
+                           hipMemcpyDeviceToHost, s->ctl);
+    hipError_t e = hipStreamSynchronize(s->ctl);
+    lk.lock();
+    s->iterations += 1;
+    if (e != hipSuccess) {
+      s->error = DE_ERR_HIP;
+      for (auto& kv : s->active) {
+        kv.second->error = DE_ERR_HIP;
+        kv.second->done = true;
+      }
+      s->active.clear();
+      s->cv_done.notify_all();
+      break;
+    }
+    std::vector<int> finished;
+    for (auto& kv : s->active)
+      if (s->h_remaining[kv.first] <= 0) finished.push_back(kv.first);
+    for (int slot : finished) {
+      Request* r = s->active[slot];
+      retire(s, r);
+      r->done = true;
+      r->t_done = now_s();
+      s->active.erase(slot);
+      s->free_slots.push_back(slot);
+    }
+    if (!finished.empty()) s->cv_done.notify_all();
+  }
+}
+
+template <class Slot>
+bool make_pool(Family& f, int nslots) {
+  f.nslots = nslots;
+  f.slot_bytes = sizeof(Slot);
+  if (hipMalloc(&f.slots, sizeof(Slot) * (size_t)nslots) != hipSuccess) return false;
+  hipLaunchKernelGGL(init_slots<Slot>, dim3((nslots + 255) / 256), dim3(256), 0, 0,
+                     (Slot*)f.slots, nslots);
+  if (hipMalloc(&f.q, sizeof(de::serve::Queue)) != hipSuccess) return false;
+  if (hipMemset(f.q, 0, sizeof(de::serve::Queue)) != hipSuccess) return false;
+  if (hipMalloc(&f.ranges, sizeof(Range) * kMaxRanges) != hipSuccess) return false;
+  if (hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking) != hipSuccess) return false;
+  return true;
+}
+
+}  // namespace
 
 extern "C" {
 
-int de_server_create(size_t, int32_t, de_server** out) {
-  if (out) *out = nullptr;
-  return DE_ERR_UNSUPPORTED;
+int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
+  if (!out) return DE_ERR_INVALID;
+  *out = nullptr;
+  de_server* s = new de_server();
+  s->budget = hbm_budget_bytes;
+  s->flags = flags;
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  // Slot pools: Lorenz and Van der Pol lanes, and one pool per spring class.
+  struct Spec {
+    const char* name;
+    int cap, g;
+  };
+  const Spec specs[] = {{"lorenz", 0, 1},     {"vdp", 0, 1},       {"spring4", 4, 4},
+                        {"spring8", 8, 8},    {"spring16", 16, 16}, {"spring32", 32, 32},
+                        {"spring64", 64, 64}, {"spring128", 128, 64}, {"spring256", 256, 64}};
+  s->fams.resize(sizeof specs / sizeof specs[0]);
+  const int lane_slots = ncu * 4 * 256;  // 4 blocks of 256 lanes per CU
+  const int group_lanes = ncu * 2 * 256;
+  bool ok = true;
+  for (size_t k = 0; k < s->fams.size() && ok; ++k) {
+    Family& f = s->fams[k];
+    f.name = specs[k].name;
+    f.cap = specs[k].cap;
+    f.g = specs[k].g;
+    switch (k) {
+      case 0: ok = make_pool<LaneSlot<de::Dopri5Lane<de::LorenzRhs>>>(f, lane_slots);
+        f.launch = launch_lane<de::LorenzRhs>; break;
+      case 1: ok = make_pool<LaneSlot<de::Dopri5Lane<de::VanDerPolRhs>>>(f, lane_slots);
+        f.launch = launch_lane<de::VanDerPolRhs>; break;
+      case 2: ok = make_pool<LaneSlot<de::GroupDopri5<4, 1>>>(f, group_lanes);
+        f.launch = launch_group<4, 1>; break;
+      case 3: ok = make_pool<LaneSlot<de::GroupDopri5<8, 1>>>(f, group_lanes);
+        f.launch = launch_group<8, 1>; break;
+      case 4: ok = make_pool<LaneSlot<de::GroupDopri5<16, 1>>>(f, group_lanes);
+        f.launch = launch_group<16, 1>; break;
+      case 5: ok = make_pool<LaneSlot<de::GroupDopri5<32, 1>>>(f, group_lanes);
+        f.launch = launch_group<32, 1>; break;
+      case 6: ok = make_pool<LaneSlot<de::GroupDopri5<64, 1>>>(f, group_lanes);
+        f.launch = launch_group<64, 1>; break;
+      case 7: ok = make_pool<LaneSlot<de::GroupDopri5<64, 2>>>(f, group_lanes);
+        f.launch = launch_group<64, 2>; break;
+      default: ok = make_pool<LaneSlot<de::GroupDopri5<64, 4>>>(f, group_lanes);
+        f.launch = launch_group<64, 4>; break;
+    }
+  }
+  size_t pools = 0;
+  for (auto& f : s->fams) pools += f.slot_bytes * f.nslots + sizeof(Range) * kMaxRanges;
+  ok = ok && hipMalloc(&s->d_reqs, sizeof(de::serve::ReqDev) * kMaxReq) == hipSuccess;
+  ok = ok && hipHostMalloc(&s->h_remaining, sizeof(int32_t) * kMaxReq, 0) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&s->ctl, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) == hipSuccess;
+  s->fam_done.resize(s->fams.size());
+  for (auto& e : s->fam_done) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  if (ok && hbm_budget_bytes <= pools) ok = false;
+  ok = ok && s->arena.init(hbm_budget_bytes - pools);
+  ok = ok && hipDeviceSynchronize() == hipSuccess;
+  if (!ok) {
+    de_server_destroy(s);
+    return DE_ERR_NOMEM;
+  }
+  s->h_reqs.resize(kMaxReq);
+  for (int i = kMaxReq - 1; i >= 0; --i) s->free_slots.push_back(i);
+  if (flags & kServerGraph) {
+    if (!capture(s)) {
+      de_server_destroy(s);
+      return DE_ERR_HIP;
+    }
+  }
+  s->thread = std::thread(scheduler, s);
+  *out = s;
+  return DE_OK;
 }
-int de_server_submit(de_server*, const de_ode_batch*, uint64_t*) { return DE_ERR_UNSUPPORTED; }
-int de_server_poll(de_server*, uint64_t, de_ode_result*, int32_t) { return DE_ERR_UNSUPPORTED; }
-double de_server_latency(de_server*, uint64_t) { return -1.0; }
-void de_server_destroy(de_server*) {}
+
+int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
+  if (!s || !b || !ticket) return DE_ERR_INVALID;
+  if (b->method != DE_METHOD_DOPRI5) return DE_ERR_UNSUPPORTED;
+  if (b->family != DE_FAMILY_LORENZ && b->family != DE_FAMILY_VAN_DER_POL &&
+      b->family != DE_FAMILY_SPRING_CHAIN)
+    return DE_ERR_UNSUPPORTED;
+  if (b->n <= 0 || !b->y0 || (b->n_eval > 0 && !b->t_eval)) return DE_ERR_INVALID;
+  const bool ragged = b->family == DE_FAMILY_SPRING_CHAIN;
+  if (ragged && (!b->sizes || !b->state_offset || !b->param_offset)) return DE_ERR_INVALID;
+  if (!ragged && b->dim != (b->family == DE_FAMILY_LORENZ ? 3 : 2)) return DE_ERR_INVALID;
+  Request* r = new Request();
+  r->b = *b;
+  const int64_t n = b->n;
+  if (ragged) {
+    r->sizes.assign(b->sizes, b->sizes + n);
+    for (int64_t i = 0; i < n; ++i)
+      if (r->sizes[i] < 1 || r->sizes[i] > 256) {
+        delete r;
+        return DE_ERR_UNSUPPORTED;
+      }
+    r->so.assign(b->state_offset, b->state_offset + n + 1);
+    r->po.assign(b->param_offset, b->param_offset + n + 1);
+    r->state_len = r->so[n];
+    r->params.assign(b->params, b->params + r->po[n]);
+  } else {
+    r->state_len = (int64_t)b->dim * n;
+    int P = b->family == DE_FAMILY_LORENZ ? 3 : 1;
+    r->params.assign(b->params, b->params + P * n);
+  }
+  r->y0.assign(b->y0, b->y0 + r->state_len);
+  if (b->t0) r->t0.assign(b->t0, b->t0 + n);
+  if (b->tf) r->tf.assign(b->tf, b->tf + n);
+  if (b->rtol_traj) r->rtol.assign(b->rtol_traj, b->rtol_traj + n);
+  if (b->atol_traj) r->atol.assign(b->atol_traj, b->atol_traj + n);
+  if (b->n_eval > 0) r->t_eval.assign(b->t_eval, b->t_eval + b->n_eval);
+  r->t_submit = now_s();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->error) {
+    delete r;
+    return s->error;
+  }
+  r->ticket = s->next_ticket++;
+  s->all[r->ticket] = r;
+  s->pending.push_back(r);
+  *ticket = r->ticket;
+  s->cv_work.notify_one();
+  return DE_OK;
+}
+
+int de_server_poll(de_server* s, uint64_t ticket, de_ode_result* out, int32_t wait) {
+  if (!s) return DE_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(ticket);
+  if (it == s->all.end()) return DE_ERR_INVALID;
+  Request* r = it->second;
+  while (!r->done) {
+    if (!wait) return 0;
+    s->cv_done.wait(lk);
+  }
+  if (r->error) return r->error;
+  if (out) {
+    const int64_t n = r->b.n;
+    if (out->y_final) memcpy(out->y_final, r->y.data(), r->y.size() * 8);
+    if (out->t_final) memcpy(out->t_final, r->t.data(), n * 8);
+    if (out->accepted) memcpy(out->accepted, r->acc.data(), n * 8);
+    if (out->rejected) memcpy(out->rejected, r->rej.data(), n * 8);
+    if (out->evals) memcpy(out->evals, r->ev.data(), n * 8);
+    if (out->status) memcpy(out->status, r->status.data(), n * 4);
+    if (out->dense && !r->dense.empty()) memcpy(out->dense, r->dense.data(), r->dense.size() * 8);
+    if (out->n_dense && !r->n_dense.empty()) memcpy(out->n_dense, r->n_dense.data(), n * 4);
+  }
+  return 1;
+}
+
+double de_server_latency(de_server* s, uint64_t ticket) {
+  if (!s) return -1.0;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->all.find(ticket);
+  if (it == s->all.end() || !it->second->done) return -1.0;
+  return it->second->t_done - it->second->t_submit;
+}
+
+void de_server_destroy(de_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->thread.joinable()) s->thread.join();
+  (void)hipDeviceSynchronize();
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  for (auto& f : s->fams) {
+    if (f.slots) (void)hipFree(f.slots);
+    if (f.q) (void)hipFree(f.q);
+    if (f.ranges) (void)hipFree(f.ranges);
+    if (f.stream) (void)hipStreamDestroy(f.stream);
+  }
+  for (auto& kv : s->all) delete kv.second;
+  if (s->arena.base) (void)hipFree(s->arena.base);
+  if (s->d_reqs) (void)hipFree(s->d_reqs);
+  if (s->h_remaining) (void)hipHostFree(s->h_remaining);
+  for (auto& e : s->fam_done)
+    if (e) (void)hipEventDestroy(e);
+  if (s->fork) (void)hipEventDestroy(s->fork);
+  if (s->ctl) (void)hipStreamDestroy(s->ctl);
+  delete s;
+}
 
 }  // extern "C"

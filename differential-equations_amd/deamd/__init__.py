@@ -473,3 +473,49 @@ def gen_gbm(n, mu=0.05, sigma=0.2, s0=100.0):
     y0 = np.empty(n)
     lib().de_gen_gbm(n, mu, sigma, s0, _ptr(p), _ptr(y0))
     return LinearDiagonal(1, p), y0
+
+
+# ----------------------------------------------------------------- serving --
+DE_SERVER_GRAPH = 1
+
+
+class Server:
+    """de_server_*: continuous batching of ensemble solves (host memory in/out)."""
+
+    def __init__(self, hbm_budget_bytes, graph=False):
+        h = C.c_void_p()
+        _abi.check(lib().de_server_create(int(hbm_budget_bytes), DE_SERVER_GRAPH if graph else 0,
+                                          C.byref(h)))
+        self.h = h
+        self._pending = {}
+
+    def submit(self, ensemble):
+        b, res, keep = ensemble.batch()
+        t = C.c_uint64()
+        _abi.check(lib().de_server_submit(self.h, C.byref(b), C.byref(t)))
+        self._pending[t.value] = res
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        res = self._pending[ticket]
+        rc = lib().de_server_poll(self.h, ticket, C.byref(res.struct()), 1 if wait else 0)
+        if rc < 0:
+            _abi.check(rc)
+        if rc == 0:
+            return None
+        del self._pending[ticket]
+        return res
+
+    def latency(self, ticket):
+        return lib().de_server_latency(self.h, ticket)
+
+    def close(self):
+        if self.h:
+            lib().de_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

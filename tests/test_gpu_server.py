@@ -1,0 +1,103 @@
+"""Serving loop: continuous batching of ensemble requests on one GPU.
+
+Each request's results must not depend on what else is being served: the
+same request gives the same bits alone and inside a mixed concurrent stream,
+and (for the lane-mapped families) the same bits as a direct batch solve.
+"""
+import numpy as np
+import pytest
+
+import deamd
+from deamd import ExplicitRungeKutta, IVP, Lorenz, Server, VanDerPol
+
+pytestmark = pytest.mark.gpu
+GIB = 1 << 30
+
+
+def lorenz_req(n, seed, tf=5.0, rtol=1e-8):
+    p, y0 = deamd.gen_lorenz(n, seed=seed)
+    return IVP.ode(Lorenz(*p), 0.0, tf, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(rtol).atol(rtol * 1e-2).max_steps(1000000))
+
+
+def vdp_req(n, seed, tf=8.0, rtol=1e-6):
+    mu, y0 = deamd.gen_vdp(n, seed=seed)
+    return IVP.ode(VanDerPol(*mu), 0.0, tf, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(rtol).atol(rtol * 1e-2).max_steps(1000000))
+
+
+def spring_req(n, seed, tf=3.0):
+    sysm, y0 = deamd.gen_spring(n, seed=seed)
+    return IVP.ode(sysm, 0.0, tf, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-6).atol(1e-8).max_steps(1000000))
+
+
+FIELDS = ("status", "accepted", "rejected", "evals", "t", "y")
+
+
+def assert_same(a, b):
+    for f in FIELDS:
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_server_matches_direct_solve():
+    req = lorenz_req(3000, seed=1)
+    want = req.solve()
+    with Server(8 * GIB) as srv:
+        t = srv.submit(req)
+        got = srv.poll(t)
+        assert srv.latency(t) > 0
+    assert_same(got, want)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_result_alone_equals_result_in_mix(graph):
+    probes = [lorenz_req(700, seed=11), vdp_req(500, seed=12), spring_req(300, seed=13)]
+    alone = []
+    with Server(8 * GIB, graph=graph) as srv:
+        for r in probes:
+            alone.append(srv.poll(srv.submit(r)))
+    rng = np.random.default_rng(5)
+    with Server(8 * GIB, graph=graph) as srv:
+        tickets = []
+        for k in range(24):
+            kind = k % 3
+            n = int(np.exp(rng.uniform(0, np.log(4096))))
+            mk = (lorenz_req, vdp_req, spring_req)[kind]
+            tickets.append(srv.submit(mk(max(n, 1), seed=100 + k)))
+            if k in (5, 11, 17):
+                tickets.append(("probe", k // 6, srv.submit(probes[k // 6])))
+        results = {}
+        for t in tickets:
+            if isinstance(t, tuple):
+                results[t[1]] = srv.poll(t[2])
+            else:
+                sol = srv.poll(t)
+                assert (sol.status == 0).all()
+    for i in range(3):
+        assert_same(results[i], alone[i])
+
+
+def test_spring_requests_within_bound(orc):
+    req = spring_req(400, seed=21)
+    want = orc.solve_ode(req, threads=16)
+    with Server(8 * GIB) as srv:
+        got = srv.poll(srv.submit(req))
+    assert (got.status == 0).all()
+    err = np.abs(got.y - want.y).max() / np.abs(want.y).max()
+    assert err <= 1e-10
+    assert ((got.accepted == want.accepted) & (got.rejected == want.rejected)).mean() >= 0.999
+
+
+def test_admission_respects_budget():
+    # Budget a little above the slot pools (about 0.6 GB): a 4M-trajectory
+    # request cannot fit and is rejected; small ones are admitted
+    # progressively as earlier ones retire, and all complete.
+    with Server(1 * GIB) as srv:
+        big = lorenz_req(1 << 22, seed=3, tf=0.1)
+        t_big = srv.submit(big)
+        with pytest.raises(RuntimeError):
+            srv.poll(t_big)
+        ts = [srv.submit(lorenz_req(20000, seed=40 + k, tf=1.0)) for k in range(12)]
+        for t in ts:
+            assert (srv.poll(t).status == 0).all()
