@@ -1,0 +1,269 @@
+// C++ host mirror of the reference crate's builder API over the C-ABI
+// (include/de_api.h), for ensembles.  Header-only; link libde_amd.so.
+//
+// The reference (Rust) reads
+//     let sol = IVP::ode(&lorenz, t0, tf, y0)
+//         .method(ExplicitRungeKutta::dopri5().rtol(1e-8).atol(1e-10))
+//         .t_eval(points)
+//         .solve()?;                                   // src/ivp.rs:277,632,656,781
+// and this header reads
+//     auto sol = IVP::ode(Lorenz{sigma, rho, beta}, t0, tf, y0s)
+//         .method(ExplicitRungeKutta::dopri5().rtol(1e-8).atol(1e-10))
+//         .t_eval(points)
+//         .solve();                                    // one GPU solve of the ensemble
+//     Result r = sol.at(i);    // r.ok ? r.value (Solution) : r.error (Error)
+// with the same names, argument meanings and defaults
+// (ExplicitRungeKutta::default, src/methods/erk/mod.rs:115-158).  Per-trajectory
+// failures come back as Error values (src/error.rs:13-41); a failing call (bad
+// descriptor, HIP error) throws std::runtime_error with de_last_error().
+#pragma once
+
+#include <stdint.h>
+
+#include <algorithm>
+#include <array>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/de_api.h"
+
+namespace differential_equations {
+
+// ------------------------------------------------------------ results ----
+enum class ErrorKind { BadInput, MaxSteps, StepSize, Stiffness };
+
+struct Error {  // Error::{BadInput, MaxSteps{t,y}, StepSize{t,y}, Stiffness{t,y}}
+  ErrorKind kind;
+  double t;
+  std::vector<double> y;
+};
+
+struct Steps {  // src/stats.rs:69
+  int64_t accepted = 0, rejected = 0;
+  int64_t total() const { return accepted + rejected; }
+};
+
+struct Evals {  // src/stats.rs:16
+  int64_t function = 0;
+};
+
+struct Solution {  // src/solution.rs:30-53 (points kept by the solout)
+  std::vector<double> t;
+  std::vector<std::vector<double>> y;
+  Steps steps;
+  Evals evals;
+};
+
+struct Result {  // Result<Solution, Error>
+  bool ok;
+  Solution value;
+  Error error;
+};
+
+// ------------------------------------------------------------ methods ----
+class ExplicitRungeKutta {
+ public:
+  static ExplicitRungeKutta dopri5() { return ExplicitRungeKutta(DE_METHOD_DOPRI5, 0.0); }
+  static ExplicitRungeKutta rk4(double h0) { return ExplicitRungeKutta(DE_METHOD_RK4, h0); }
+  static ExplicitRungeKutta euler(double h0) { return ExplicitRungeKutta(DE_METHOD_EULER, h0); }
+  ExplicitRungeKutta rtol(double v) const { auto m = *this; m.b_.rtol = v; return m; }
+  ExplicitRungeKutta atol(double v) const { auto m = *this; m.b_.atol = v; return m; }
+  ExplicitRungeKutta h0(double v) const { auto m = *this; m.b_.h0 = v; return m; }
+  ExplicitRungeKutta h_min(double v) const { auto m = *this; m.b_.h_min = v; return m; }
+  ExplicitRungeKutta h_max(double v) const { auto m = *this; m.b_.h_max = v; return m; }
+  ExplicitRungeKutta max_steps(int64_t v) const { auto m = *this; m.b_.max_steps = v; return m; }
+  ExplicitRungeKutta safety_factor(double v) const { auto m = *this; m.b_.safety_factor = v; return m; }
+  ExplicitRungeKutta min_scale(double v) const { auto m = *this; m.b_.min_scale = v; return m; }
+  ExplicitRungeKutta max_scale(double v) const { auto m = *this; m.b_.max_scale = v; return m; }
+  const de_ode_batch& settings() const { return b_; }
+
+ private:
+  ExplicitRungeKutta(int32_t method, double h0) {
+    de_ode_batch_init(&b_);
+    b_.method = method;
+    b_.h0 = h0;
+  }
+  de_ode_batch b_;
+};
+
+// ------------------------------------------------------------ systems ----
+// Device-compiled RHS families; each parameter is given per trajectory (a
+// vector of length n) or once for all (a vector of length 1).
+struct Lorenz {  // tests/ode/systems.rs:85-101
+  static constexpr int32_t family = DE_FAMILY_LORENZ;
+  static constexpr int dim = 3;
+  std::vector<double> sigma, rho, beta;
+  std::vector<std::vector<double>> params() const { return {sigma, rho, beta}; }
+};
+struct VanDerPol {  // tests/ode/systems.rs:66-78
+  static constexpr int32_t family = DE_FAMILY_VAN_DER_POL;
+  static constexpr int dim = 2;
+  std::vector<double> mu;
+  std::vector<std::vector<double>> params() const { return {mu}; }
+};
+struct ExponentialGrowth {  // tests/ode/systems.rs:9-17
+  static constexpr int32_t family = DE_FAMILY_EXPONENTIAL;
+  static constexpr int dim = 1;
+  std::vector<double> k;
+  std::vector<std::vector<double>> params() const { return {k}; }
+};
+struct LinearEquation {  // tests/ode/systems.rs:22-32
+  static constexpr int32_t family = DE_FAMILY_LINEAR;
+  static constexpr int dim = 1;
+  std::vector<double> a, b;
+  std::vector<std::vector<double>> params() const { return {a, b}; }
+};
+struct HarmonicOscillator {  // tests/ode/systems.rs:37-48
+  static constexpr int32_t family = DE_FAMILY_HARMONIC;
+  static constexpr int dim = 2;
+  std::vector<double> k;
+  std::vector<std::vector<double>> params() const { return {k}; }
+};
+struct LogisticEquation {  // tests/ode/systems.rs:53-61
+  static constexpr int32_t family = DE_FAMILY_LOGISTIC;
+  static constexpr int dim = 1;
+  std::vector<double> k, m;
+  std::vector<std::vector<double>> params() const { return {k, m}; }
+};
+struct RobertsonProblem {  // tests/ode/systems.rs:155-173
+  static constexpr int32_t family = DE_FAMILY_ROBERTSON;
+  static constexpr int dim = 3;
+  std::vector<std::vector<double>> params() const { return {}; }
+};
+
+// ------------------------------------------------------- ensemble result --
+class EnsembleSolution {
+ public:
+  int64_t n = 0;
+  int dim = 0;
+  double t0 = 0.0;
+  std::vector<double> t_eval;
+  std::vector<double> y, t, dense;
+  std::vector<int64_t> accepted, rejected, evals;
+  std::vector<int32_t> status, n_dense;
+
+  // The reference's Result<Solution, Error> of trajectory i.
+  Result at(int64_t i) const {
+    Result r;
+    std::vector<double> yi(dim);
+    for (int c = 0; c < dim; ++c) yi[c] = y[(size_t)c * n + i];
+    if (status[i] != DE_STATUS_COMPLETE) {
+      r.ok = false;
+      r.error.kind = status[i] == DE_STATUS_MAX_STEPS   ? ErrorKind::MaxSteps
+                     : status[i] == DE_STATUS_STEP_SIZE ? ErrorKind::StepSize
+                     : status[i] == DE_STATUS_STIFFNESS ? ErrorKind::Stiffness
+                                                        : ErrorKind::BadInput;
+      r.error.t = t[i];
+      r.error.y = yi;
+      return r;
+    }
+    r.ok = true;
+    r.value.steps.accepted = accepted[i];
+    r.value.steps.rejected = rejected[i];
+    r.value.evals.function = evals[i];
+    if (t_eval.empty()) {  // final state (the last point DefaultSolout keeps)
+      r.value.t.push_back(t[i]);
+      r.value.y.push_back(yi);
+    } else {               // TEvalSolout points that were produced
+      for (size_t e = 0; e < t_eval.size(); ++e) {
+        double v0 = dense[(e * dim) * n + i];
+        if (v0 != v0) continue;
+        std::vector<double> ye(dim);
+        for (int c = 0; c < dim; ++c) ye[c] = dense[(e * dim + c) * n + i];
+        r.value.t.push_back(t_eval[e]);
+        r.value.y.push_back(ye);
+      }
+    }
+    return r;
+  }
+};
+
+// ------------------------------------------------------------------ IVP ---
+template <class System>
+class IVP {
+ public:
+  // IVP::ode(system, t0, tf, y0) with one initial state per trajectory.
+  static IVP ode(const System& sys, double t0, double tf,
+                 const std::vector<std::array<double, System::dim>>& y0) {
+    return IVP(sys, t0, tf, y0);
+  }
+  IVP method(const ExplicitRungeKutta& m) const { IVP p = *this; p.m_ = m; return p; }
+  IVP t_eval(std::vector<double> points) const {
+    IVP p = *this;
+    p.t_eval_ = points;
+    return p;
+  }
+  // Launch schedule (DE_SCHED_*): persistent lane refill by default.
+  IVP schedule(int32_t s) const { IVP p = *this; p.schedule_ = s; return p; }
+
+  EnsembleSolution solve() const {
+    const int64_t n = (int64_t)y0_.size();
+    de_ode_batch b = m_.settings();
+    b.family = System::family;
+    b.dim = System::dim;
+    b.n = n;
+    b.t0_scalar = t0_;
+    b.tf_scalar = tf_;
+    b.schedule = schedule_;
+    b.flags = DE_FLAG_HOST_MEMORY;
+    std::vector<double> params;
+    for (const auto& col : sys_.params()) {
+      if (col.size() != 1 && (int64_t)col.size() != n)
+        throw std::runtime_error("parameter vector length must be 1 or n");
+      for (int64_t i = 0; i < n; ++i) params.push_back(col.size() == 1 ? col[0] : col[i]);
+    }
+    std::vector<double> y0((size_t)System::dim * n);
+    for (int64_t i = 0; i < n; ++i)
+      for (int c = 0; c < System::dim; ++c) y0[(size_t)c * n + i] = y0_[i][c];
+    std::vector<double> te = t_eval_;
+    if (!te.empty()) {
+      std::sort(te.begin(), te.end());
+      if (tf_ < t0_) std::reverse(te.begin(), te.end());
+    }
+    b.params = params.empty() ? nullptr : params.data();
+    b.y0 = y0.data();
+    b.n_eval = (int32_t)te.size();
+    b.t_eval = te.empty() ? nullptr : te.data();
+    EnsembleSolution s;
+    s.n = n;
+    s.dim = System::dim;
+    s.t0 = t0_;
+    s.t_eval = te;
+    s.y.resize(y0.size());
+    s.t.resize(n);
+    s.accepted.resize(n);
+    s.rejected.resize(n);
+    s.evals.resize(n);
+    s.status.resize(n);
+    if (!te.empty()) {
+      s.dense.resize(te.size() * y0.size());
+      s.n_dense.resize(n);
+    }
+    de_ode_result r;
+    r.y_final = s.y.data();
+    r.t_final = s.t.data();
+    r.accepted = s.accepted.data();
+    r.rejected = s.rejected.data();
+    r.evals = s.evals.data();
+    r.status = s.status.data();
+    r.dense = te.empty() ? nullptr : s.dense.data();
+    r.n_dense = te.empty() ? nullptr : s.n_dense.data();
+    int rc = de_ode_batch_solve(&b, &r, nullptr);
+    if (rc != DE_OK) throw std::runtime_error(std::string("de_ode_batch_solve: ") + de_last_error());
+    return s;
+  }
+
+ private:
+  IVP(const System& sys, double t0, double tf,
+      const std::vector<std::array<double, System::dim>>& y0)
+      : sys_(sys), t0_(t0), tf_(tf), y0_(y0), m_(ExplicitRungeKutta::dopri5()) {}
+  System sys_;
+  double t0_, tf_;
+  std::vector<std::array<double, System::dim>> y0_;
+  ExplicitRungeKutta m_;
+  std::vector<double> t_eval_;
+  int32_t schedule_ = DE_SCHED_AUTO;
+};
+
+}  // namespace differential_equations

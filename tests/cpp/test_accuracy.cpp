@@ -1,0 +1,128 @@
+// The reference's DOPRI5 / RK4 / Euler accuracy and error tests
+// (tests/ode/accuracy.rs, tests/ode/interpolation.rs, tests/ode/errors.rs,
+// tests/ode/from_fn.rs) written against the C++ builder mirror and run on the
+// GPU through the C-ABI.  Each case solves a small ensemble (the same problem
+// in every lane) and checks every lane like the reference checks its one solve.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../../differential-equations_amd/cpp/differential_equations.hpp"
+
+using namespace differential_equations;
+
+static int failures = 0;
+
+#define CHECK(cond, ...)                 \
+  do {                                   \
+    if (!(cond)) {                       \
+      std::printf("FAIL: " __VA_ARGS__); \
+      std::printf("\n");                 \
+      ++failures;                        \
+    }                                    \
+  } while (0)
+
+template <class System>
+static void test_ode(const char* name, const System& sys, double t0, double tf,
+                     std::array<double, System::dim> y0,
+                     std::array<double, System::dim> expected, const ExplicitRungeKutta& m,
+                     double tolerance) {
+  std::vector<std::array<double, System::dim>> y0s(8, y0);
+  auto sol = IVP<System>::ode(sys, t0, tf, y0s).method(m).solve();
+  for (int64_t i = 0; i < sol.n; ++i) {
+    Result r = sol.at(i);
+    CHECK(r.ok, "%s: trajectory %ld did not complete", name, (long)i);
+    if (!r.ok) continue;
+    const auto& yf = r.value.y.back();
+    for (int c = 0; c < System::dim; ++c)
+      CHECK(std::fabs(yf[c] - expected[c]) < tolerance, "%s: component %d: expected %.10g got %.10g",
+            name, c, expected[c], yf[c]);
+  }
+  std::printf("%s passed\n", name);
+}
+
+int main() {
+  // tests/ode/accuracy.rs:70-103
+  test_ode("DOPRI5 exponential_growth_positive_direction", ExponentialGrowth{{1.0}}, 0.0, 10.0,
+           {1.0}, {22026.46579479}, ExplicitRungeKutta::dopri5(), 1e1);
+  test_ode("RK4 exponential_growth_positive_direction", ExponentialGrowth{{1.0}}, 0.0, 10.0,
+           {1.0}, {22026.46579479}, ExplicitRungeKutta::rk4(0.01), 1e3);
+  test_ode("Euler exponential_growth_positive_direction", ExponentialGrowth{{1.0}}, 0.0, 10.0,
+           {1.0}, {22026.46579479}, ExplicitRungeKutta::euler(0.01), 2e3);
+  // tests/ode/accuracy.rs:177-213
+  test_ode("DOPRI5 exponential_growth_negative_direction", ExponentialGrowth{{1.0}}, 0.0, -10.0,
+           {22026.46579479}, {1.0}, ExplicitRungeKutta::dopri5(), 1e-2);
+  test_ode("RK4 exponential_growth_negative_direction", ExponentialGrowth{{1.0}}, 0.0, -10.0,
+           {22026.46579479}, {1.0}, ExplicitRungeKutta::rk4(-0.01), 1e-2);
+  // tests/ode/accuracy.rs:286-321
+  test_ode("DOPRI5 linear_equation", LinearEquation{{1.0}, {1.0}}, 0.0, 10.0, {1.0},
+           {44051.93158958}, ExplicitRungeKutta::dopri5(), 1e1);
+  // tests/ode/accuracy.rs:395-430
+  test_ode("DOPRI5 harmonic_oscillator", HarmonicOscillator{{1.0}}, 0.0, 10.0, {1.0, 0.0},
+           {-0.83907153, 0.54402111}, ExplicitRungeKutta::dopri5(), 1e-2);
+  test_ode("RK4 harmonic_oscillator", HarmonicOscillator{{1.0}}, 0.0, 10.0, {1.0, 0.0},
+           {-0.83907153, 0.54402111}, ExplicitRungeKutta::rk4(0.01), 1e-1);
+  // tests/ode/accuracy.rs:503-539
+  test_ode("DOPRI5 logistic_equation", LogisticEquation{{1.0}, {10.0}}, 0.0, 10.0, {0.1},
+           {9.95525518}, ExplicitRungeKutta::dopri5(), 1e-3);
+  // tests/ode/accuracy.rs:622-671
+  test_ode("DOPRI5 robertson_stiff_problem", RobertsonProblem{}, 0.0, 0.4, {1.0, 0.0, 0.0},
+           {0.9851721139, 3.3863953790e-05, 0.0147940222},
+           ExplicitRungeKutta::dopri5().rtol(1e-8).atol(1e-10), 5e-1);
+  // tests/ode/from_fn.rs:3-18
+  test_ode("Euler from_fn", ExponentialGrowth{{1.0}}, 0.0, 1.0, {1.0}, {2.5937},
+           ExplicitRungeKutta::euler(0.1), 1e-3);
+
+  // tests/ode/interpolation.rs:22-72
+  for (auto m : {ExplicitRungeKutta::dopri5(), ExplicitRungeKutta::rk4(0.01)}) {
+    std::vector<std::array<double, 1>> y0s(4, {1.0});
+    auto sol = IVP<ExponentialGrowth>::ode(ExponentialGrowth{{1.0}}, 0.0, 2.0, y0s)
+                   .t_eval({0.5, 1.0, 1.69})
+                   .method(m)
+                   .solve();
+    Result r = sol.at(0);
+    CHECK(r.ok && r.value.y.size() == 3, "interpolation: expected 3 points");
+    if (r.ok) CHECK(std::fabs(r.value.y[0][0] - std::exp(0.5)) < 1e-3, "interpolation value");
+  }
+  std::printf("interpolation passed\n");
+
+  // tests/ode/errors.rs:87-130
+  {
+    std::vector<std::array<double, 1>> y0s(2, {1.0});
+    auto a = IVP<ExponentialGrowth>::ode(ExponentialGrowth{{1.0}}, 0.0, 0.0, y0s).solve();
+    CHECK(!a.at(0).ok && a.at(0).error.kind == ErrorKind::BadInput, "invalid_time_span");
+    auto b = IVP<ExponentialGrowth>::ode(ExponentialGrowth{{1.0}}, 0.0, 1.0, y0s)
+                 .method(ExplicitRungeKutta::dopri5().h0(10.0))
+                 .solve();
+    CHECK(!b.at(0).ok && b.at(0).error.kind == ErrorKind::BadInput, "initial_step_size_too_big");
+    auto c = IVP<ExponentialGrowth>::ode(ExponentialGrowth{{1.0}}, 0.0, 1.0, y0s)
+                 .method(ExplicitRungeKutta::rk4(10.0))
+                 .solve();
+    CHECK(!c.at(0).ok && c.at(0).error.kind == ErrorKind::BadInput, "rk4 h0 too big");
+  }
+  std::printf("errors passed\n");
+
+  // Ensemble: per-trajectory parameters (the builder's vectors) on Lorenz.
+  {
+    const int n = 256;
+    std::vector<double> rho(n);
+    std::vector<std::array<double, 3>> y0s(n);
+    for (int i = 0; i < n; ++i) {
+      rho[i] = 50.0 * i / n;
+      y0s[i] = {1.0, 1.0, 1.0};
+    }
+    auto sol = IVP<Lorenz>::ode(Lorenz{{10.0}, rho, {8.0 / 3.0}}, 0.0, 2.0, y0s)
+                   .method(ExplicitRungeKutta::dopri5().rtol(1e-8).atol(1e-10).max_steps(100000))
+                   .solve();
+    for (int i = 0; i < n; ++i) {
+      Result r = sol.at(i);
+      CHECK(r.ok, "lorenz trajectory %d", i);
+      if (r.ok)
+        CHECK(r.value.evals.function == 3 + 6 * r.value.steps.total() + r.value.steps.accepted,
+              "lorenz evals invariant %d", i);
+    }
+    std::printf("lorenz ensemble passed\n");
+  }
+  std::printf("%s (%d failures)\n", failures ? "FAILED" : "ALL PASSED", failures);
+  return failures ? 1 : 0;
+}
