@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--n", type=int, default=0, help="override ensemble size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--only", choices=["persistent", "plain", "fast"], default=None,
+                    help="run only one Lorenz variant (profiling)")
     ap.add_argument("--cpu-sample-stride", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window")
@@ -166,6 +168,11 @@ def bench_lorenz(args, dv):
         status_ok = bool((out["st"] == 0).all().item())
         return dict(wall=wall, lat=lat, A=A, R=R, ok=status_ok)
 
+    if args.only:
+        sched = _abi.DE_SCHED_PLAIN if args.only == "plain" else _abi.DE_SCHED_PERSISTENT
+        r = run(sched, args.only == "fast", args.steps, args.warmup)
+        return {"variant": args.only, "accepted": r["A"], "rejected": r["R"],
+                "kernel_ms": sum(r["lat"]) / len(r["lat"])}, (p, y0, n)
     res = run(_abi.DE_SCHED_PERSISTENT, False, args.steps, args.warmup)
     plain = None if args.no_plain else run(_abi.DE_SCHED_PLAIN, False, max(3, args.steps // 2), 1)
     fast = run(_abi.DE_SCHED_PERSISTENT, True, max(3, args.steps // 2), 1)
@@ -530,7 +537,7 @@ def main():
     dv = Device()
     if args.workload == "lorenz":
         line, (p, y0, n) = bench_lorenz(args, dv)
-        if not args.no_cpu_baseline:
+        if not args.no_cpu_baseline and not args.only:
             line["cpu_baseline"] = cpu_baseline_lorenz(args, p, y0, n)
     elif args.workload == "spring":
         line = bench_spring(args, dv)

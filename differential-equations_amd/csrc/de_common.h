@@ -19,9 +19,13 @@
 #include <hip/hip_runtime.h>
 #define DE_HD __host__ __device__ inline
 #define DE_DEV __device__ inline
+// Run once per trajectory (init, result write-back): kept out of line so the
+// step loop's registers are not spent on their values.
+#define DE_COLD __host__ __device__ __attribute__((noinline))
 #else
 #define DE_HD inline
 #define DE_DEV inline
+#define DE_COLD inline
 #endif
 
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -141,24 +141,43 @@ DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, dou
 }
 
 // ------------------------------------------------------------------ DOPRI5 --
-template <class F>
+// DENSE selects the t_eval output path at compile time.  The attempt counter
+// of the reference (`steps`, which counts attempts) is acc + rej at every
+// check, and evals = base + 6 rej + 7 acc on every exit (base = 3 with the
+// automatic initial step, 1 without), so neither is carried in the loop.
+template <class F, bool DENSE = false>
 struct Dopri5Lane {
   static constexpr int D = F::kDim;
   F f;
   double y[D], k0[D];
   double t, h, h_prev, tf, dir, rtol, atol;
-  int64_t steps, acc, rej, ev;
-  int32_t stiff, nonstiff, countdown, next_eval, pushed;
+  int32_t acc, rej, base_ev, stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
 
-  DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const {
-    for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = y[c];
-    a.t_final[i] = t;
-    a.accepted[i] = acc;
-    a.rejected[i] = rej;
-    a.evals[i] = ev;
+  // Result write-back and trajectory start run once per trajectory; they are
+  // out-of-line functions taking the lane by value so that the step loop's
+  // copy of the lane never has its address taken (it stays in registers).
+  DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, Dopri5Lane L) {
+    for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
+    a.t_final[i] = L.t;
+    a.accepted[i] = L.acc;
+    a.rejected[i] = L.rej;
+    a.evals[i] =
+        st == kBadInput ? 0 : (int64_t)L.base_ev + 6 * (int64_t)L.rej + 7 * (int64_t)L.acc;
     a.status[i] = st;
-    if (a.n_dense) a.n_dense[i] = pushed;
+    if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
+  }
+  DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
+
+  DE_COLD static Dopri5Lane start(const OdeArgs& a, int64_t i, bool* ok) {
+    Dopri5Lane L;
+    *ok = L.init(a, i);
+    return L;
+  }
+  DE_HD bool begin(const OdeArgs& a, int64_t i) {
+    bool ok;
+    *this = start(a, i, &ok);
+    return ok;
   }
 
   // Driver prologue + DOPRI5 init.  Returns false when the trajectory ended
@@ -171,58 +190,51 @@ struct Dopri5Lane {
     atol = a.atol_traj ? a.atol_traj[i] : a.atol;
     for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
     t = t0;
-    acc = rej = ev = 0;
+    acc = rej = 0;
     pushed = 0;
     next_eval = 0;
     double s = signum(tf - t0);
     dir = s;
     bool ok = (s == 1.0 || s == -1.0);
     double h0 = a.h0;
+    base_ev = 1;
     if (ok && h0 == 0.0) {
       h0 = lane_h_init(f, t0, tf, y, rtol, atol, a.h_min, a.h_max);
-      ev += 2;
+      base_ev = 3;
     }
     ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
     if (!ok) {
-      ev = 0;
       finish(a, i, kBadInput);
       return false;
     }
     h = h0;
     f.diff(t, y, k0);
-    ev += 1;
-    steps = 0;
     h_prev = 0.0;
     stiff = 0;
     nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (a.n_eval > 0) {
+    if (DENSE && a.n_eval > 0) {
       teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y, [](double, double*) {});
     }
     return true;
   }
 
   // One pass of the driver loop: overshoot clip, one DOPRI5 attempt, solout,
-  // end test.  Returns true when the trajectory ended (results written).
-  DE_HD bool advance(const OdeArgs& a, int64_t i) {
-    if ((t + h - tf) * dir > 0.0) {
-      double hn = tf - t;
-      if (fabs(hn) < kTenEps) {
-        finish(a, i, kComplete);
-        return true;
-      }
-      h = hn;
+  // end test.  Returns -1 while running, else the final status (the caller
+  // writes the results with finish()).
+  DE_HD int32_t step(const OdeArgs& a, int64_t i) {
+    {
+      // Overshoot clip and the two pre-step error checks of the reference,
+      // evaluated as flags so the loop has a single early exit.
+      const bool clip = (t + h - tf) * dir > 0.0;
+      const double hn = tf - t;
+      const bool done = clip && fabs(hn) < kTenEps;
+      h = clip ? hn : h;
+      const bool small = fabs(h) < fabs(h_prev) * 1e-14;
+      const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
+      if (done | small | maxed) return done ? kComplete : (small ? kStepSize : kMaxSteps);
     }
-    if (fabs(h) < fabs(h_prev) * 1e-14) {
-      finish(a, i, kStepSize);
-      return true;
-    }
-    if (steps >= a.max_steps) {
-      finish(a, i, kMaxSteps);
-      return true;
-    }
-    steps += 1;
     countdown -= 1;
 
     double k1[D], k2[D], k3[D], k4[D], k5[D], k6[D], ys[D];
@@ -272,6 +284,7 @@ struct Dopri5Lane {
     double scale = a.safety * DE_CTRL_POW(err, -0.2);
     scale = fmin(fmax(scale, a.min_scale), a.max_scale);
 
+    int32_t result = -1;
     if (err <= 1.0) {
       double t_new = t + h;
       double dn[D];
@@ -290,17 +303,15 @@ struct Dopri5Lane {
           if (h_lamb > 6.1) {
             nonstiff = 0;
             stiff += 1;
-            if (stiff == 15) {
-              finish(a, i, kStiffness);
-              return true;
-            }
           }
         } else {
           nonstiff += 1;
           if (nonstiff == 6) stiff = 0;
         }
       }
-      if (a.n_eval > 0) {
+      // Error::Stiffness leaves (t, y) at the start of the step.
+      if (stiff == 15) return kStiffness;
+      if (DENSE && a.n_eval > 0) {
         // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
         // ordinary.rs:312-333.
         double cont1[D], cont2[D], cont3[D];
@@ -336,19 +347,21 @@ struct Dopri5Lane {
         scale = fmin(scale, 1.0);
       }
       acc += 1;
-      ev += 7;
-      h = constrain_step_size(h * scale, a.h_min, a.h_max);
-      if (fabs(tf - t) <= kTenEps) {
-        finish(a, i, kComplete);
-        return true;
-      }
+      if (fabs(tf - t) <= kTenEps) result = kComplete;
     } else {
       rejected_last = true;
       rej += 1;
-      ev += 6;
-      h = constrain_step_size(h * scale, a.h_min, a.h_max);
     }
-    return false;
+    h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    return result;
+  }
+
+  // step() + finish(): returns true when the trajectory ended.
+  DE_HD bool advance(const OdeArgs& a, int64_t i) {
+    int32_t st = step(a, i);
+    if (st < 0) return false;
+    finish(a, i, st);
+    return true;
   }
 };
 
@@ -373,6 +386,8 @@ struct FixedLane {
     a.status[i] = st;
     if (a.n_dense) a.n_dense[i] = pushed;
   }
+
+  DE_HD bool begin(const OdeArgs& a, int64_t i) { return init(a, i); }
 
   DE_HD bool init(const OdeArgs& a, int64_t i) {
     f.load(a.params, a.n, i);
@@ -401,19 +416,14 @@ struct FixedLane {
     return true;
   }
 
-  DE_HD bool advance(const OdeArgs& a, int64_t i) {
+  // One driver pass; -1 while running, else the final status.
+  DE_HD int32_t step(const OdeArgs& a, int64_t i) {
     if ((t + h - tf) * dir > 0.0) {
       double hn = tf - t;
-      if (fabs(hn) < kTenEps) {
-        finish(a, i, kComplete);
-        return true;
-      }
+      if (fabs(hn) < kTenEps) return kComplete;
       h = hn;
     }
-    if (steps >= a.max_steps) {
-      finish(a, i, kMaxSteps);
-      return true;
-    }
+    if (steps >= a.max_steps) return kMaxSteps;
     steps += 1;
     double ynext[D];
     if (STAGES == 4) {
@@ -464,11 +474,15 @@ struct FixedLane {
       y[c] = ynext[c];
       k0[c] = dn[c];
     }
-    if (fabs(tf - t) <= kTenEps) {
-      finish(a, i, kComplete);
-      return true;
-    }
-    return false;
+    if (fabs(tf - t) <= kTenEps) return kComplete;
+    return -1;
+  }
+
+  DE_HD bool advance(const OdeArgs& a, int64_t i) {
+    int32_t st = step(a, i);
+    if (st < 0) return false;
+    finish(a, i, st);
+    return true;
   }
 };
 

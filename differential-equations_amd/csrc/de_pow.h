@@ -147,17 +147,23 @@ DE_HD double pow_exp_dd(dd Z) {
 }
 
 // pow(x, y) with C99 special cases for the arguments the integrator can see.
+// The general path runs on a substitute argument for the special cases and
+// their results are selected at the end (no divergent early returns).
 DE_HD double pow(double x, double y) {
-  if (y == 0.0) return 1.0;
-  if (x == 1.0) return 1.0;
-  if (x != x || y != y) return x + y;
-  if (x == 0.0) return y < 0.0 ? as_f64(0x7ff0000000000000ULL) : 0.0;
-  if (x < 0.0) return as_f64(0x7ff8000000000000ULL);  // non-integer y only
-  if (x == as_f64(0x7ff0000000000000ULL)) return y < 0.0 ? 0.0 : x;
-  dd L = pow_log_dd(x);
+  const double inf = as_f64(0x7ff0000000000000ULL);
+  const bool special = !(x > 0.0 && x < inf) || x == 1.0 || y == 0.0 || y != y;
+  dd L = pow_log_dd(special ? 2.0 : x);
   dd zp = two_prod(y, L.hi);
   dd Z = fast_two_sum(zp.hi, zp.lo + y * L.lo);
-  return pow_exp_dd(Z);
+  double r = pow_exp_dd(special ? dd{0.0, 0.0} : Z);
+  if (special) {
+    if (y == 0.0 || x == 1.0) r = 1.0;
+    else if (x != x || y != y) r = x + y;
+    else if (x == 0.0) r = y < 0.0 ? inf : 0.0;
+    else if (x < 0.0) r = as_f64(0x7ff8000000000000ULL);  // non-integer y only
+    else r = y < 0.0 ? 0.0 : inf;                          // x = +inf
+  }
+  return r;
 }
 
 }  // namespace de

@@ -33,9 +33,12 @@ __global__ __launch_bounds__(kBlock) void ode_plain_kernel(OdeArgs a) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.n) return;
   Lane L;
-  if (!L.init(a, i)) return;
-  while (!L.advance(a, i)) {
-  }
+  if (!L.begin(a, i)) return;
+  int32_t st;
+  do {
+    st = L.step(a, i);
+  } while (st < 0);
+  L.finish(a, i, st);
 }
 
 template <class Lane>
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void ode_persistent_kernel(OdeArgs a) {
           unsigned long long my = base + (unsigned long long)__popcll(mask & __lanemask_lt());
           if (my < n) {
             idx = (int64_t)my;
-            have = L.init(a, idx);
+            have = L.begin(a, idx);
           }
         }
       }
@@ -71,7 +74,11 @@ __global__ __launch_bounds__(kBlock) void ode_persistent_kernel(OdeArgs a) {
       continue;
     }
     if (have) {
-      if (L.advance(a, idx)) have = false;
+      int32_t st = L.step(a, idx);
+      if (st >= 0) {
+        L.finish(a, idx, st);
+        have = false;
+      }
     }
   }
 }
@@ -110,7 +117,9 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
 template <class F>
 static int launch_method(int method, const OdeArgs& a, int schedule, hipStream_t s) {
   switch (method) {
-    case kMethodDopri5: return launch<Dopri5Lane<F>>(a, schedule, s);
+    case kMethodDopri5:
+      return a.n_eval > 0 ? launch<Dopri5Lane<F, true>>(a, schedule, s)
+                          : launch<Dopri5Lane<F, false>>(a, schedule, s);
 #if !defined(DE_FAST_BUILD)
     case kMethodRk4: return launch<FixedLane<F, 4>>(a, schedule, s);
     case kMethodEuler: return launch<FixedLane<F, 1>>(a, schedule, s);
