@@ -21,7 +21,7 @@
 #define DE_DEV __device__ inline
 // Run once per trajectory (init, result write-back): kept out of line so the
 // step loop's registers are not spent on their values.
-#define DE_COLD __host__ __device__ __attribute__((noinline))
+#define DE_COLD __host__ __device__ inline __attribute__((noinline))
 #else
 #define DE_HD inline
 #define DE_DEV inline
@@ -54,19 +54,14 @@ DE_HD double as_f64(uint64_t u) {
 
 // Rust f64::signum: 1.0 for +0 and positive values, -1.0 for -0 and negative
 // values, NaN for NaN.
-DE_HD double signum(double x) {
-  if (x != x) return x;
-  return (as_u64(x) >> 63) ? -1.0 : 1.0;
-}
+DE_HD double signum(double x) { return x != x ? x : copysign(1.0, x); }
 
 // constrain_step_size (src/utils.rs:22-33): clamp |h| to [h_min, h_max] keeping
-// the sign of h.
+// the sign of h (written with selects; a NaN h is returned unchanged).
 DE_HD double constrain_step_size(double h, double h_min, double h_max) {
-  double sign = signum(h);
-  double a = fabs(h);
-  if (a < h_min) return sign * h_min;
-  if (a > h_max) return sign * h_max;
-  return h;
+  const double sign = signum(h);
+  const double a = fabs(h);
+  return a < h_min ? sign * h_min : (a > h_max ? sign * h_max : h);
 }
 
 // Per-trajectory status codes.  0..4 mirror Status::Complete and the Error

@@ -25,13 +25,9 @@
 #include "de_families.h"
 #include "de_pow.h"
 
-// Controller pow: the bit-exact project pow, or the device libm pow in the
-// FMA-contracted fast build (whose results are not bit-matched anyway).
-#if defined(DE_FAST_BUILD)
-#define DE_CTRL_POW(x, y) ::pow((x), (y))
-#else
+// Controller pow: the project pow (correctly rounded but for rare cases, and
+// cheaper than the device libm pow for the -0.2 exponent) in both builds.
 #define DE_CTRL_POW(x, y) ::de::pow((x), (y))
-#endif
 
 namespace de {
 
@@ -235,7 +231,11 @@ struct Dopri5Lane {
       const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
       if (done | small | maxed) return done ? kComplete : (small ? kStepSize : kMaxSteps);
     }
+    // The stiffness test runs on accepted attempts whose number is a multiple
+    // of 100 (ordinary.rs:167); the countdown restarts on every such attempt.
     countdown -= 1;
+    const bool stiff_check = countdown == 0;
+    if (stiff_check) countdown = 100;
 
     double k1[D], k2[D], k3[D], k4[D], k5[D], k6[D], ys[D];
     const double a10h = dp::a10 * h;
@@ -284,74 +284,74 @@ struct Dopri5Lane {
     double scale = a.safety * DE_CTRL_POW(err, -0.2);
     scale = fmin(fmax(scale, a.min_scale), a.max_scale);
 
-    int32_t result = -1;
-    if (err <= 1.0) {
-      double t_new = t + h;
-      double dn[D];
-      f.diff(t_new, ynew, dn);
-      if (countdown == 0) {
-        countdown = 100;
-        double stdnum = 0.0, stden = 0.0;
-        for (int c = 0; c < D; ++c) {
-          double u = yseg[c] - k6[c];
-          stdnum = c == 0 ? u * u : stdnum + u * u;
-          double v = dn[c] - ys[c];
-          stden = c == 0 ? v * v : stden + v * v;
-        }
-        if (stden > 0.0) {
-          double h_lamb = h * sqrt(stdnum / stden);
-          if (h_lamb > 6.1) {
-            nonstiff = 0;
-            stiff += 1;
-          }
-        } else {
-          nonstiff += 1;
-          if (nonstiff == 6) stiff = 0;
-        }
-      }
-      // Error::Stiffness leaves (t, y) at the start of the step.
-      if (stiff == 15) return kStiffness;
-      if (DENSE && a.n_eval > 0) {
-        // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
-        // ordinary.rs:312-333.
-        double cont1[D], cont2[D], cont3[D];
-        for (int c = 0; c < D; ++c) {
-          double ydiff = ynew[c] + (-1.0) * y[c];
-          cont1[c] = ydiff;
-          double bspl = h * k0[c] + (-1.0) * ydiff;
-          cont2[c] = bspl;
-          cont3[c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
-        }
-        const double hp = h, tp = t;
-        const double cont4 = 0.0 * h;
-        const double* y0p = y;
-        teval_emit<D>(a, i, dir, next_eval, pushed, t_new, t, ynew, [&](double te, double* out) {
-          double s = (te - tp) / hp;
-          double s1 = 1.0 - s;
-          for (int c = 0; c < D; ++c) {
-            double acc_ = cont4 * s1 + cont3[c];
-            acc_ = acc_ * s + cont2[c];
-            acc_ = acc_ * s1 + cont1[c];
-            out[c] = y0p[c] + s * acc_;
-          }
-        });
-      }
-      h_prev = h;
-      t = t_new;
+    // Accept/reject.  f(t_new, y_new) is evaluated for every attempt (a wave
+    // evaluates it whenever one lane accepts) and the state is updated by
+    // selects, so the loop has no divergent update paths.
+    const bool accept = err <= 1.0;
+    const double t_new = t + h;
+    double dn[D];
+    f.diff(t_new, ynew, dn);
+    if (accept && stiff_check) {
+      double stdnum = 0.0, stden = 0.0;
       for (int c = 0; c < D; ++c) {
-        y[c] = ynew[c];
-        k0[c] = dn[c];
+        double u = yseg[c] - k6[c];
+        stdnum = c == 0 ? u * u : stdnum + u * u;
+        double v = dn[c] - ys[c];
+        stden = c == 0 ? v * v : stden + v * v;
       }
-      if (rejected_last) {
-        rejected_last = false;
-        scale = fmin(scale, 1.0);
+      if (stden > 0.0) {
+        double h_lamb = h * sqrt(stdnum / stden);
+        if (h_lamb > 6.1) {
+          nonstiff = 0;
+          stiff += 1;
+        }
+      } else {
+        nonstiff += 1;
+        if (nonstiff == 6) stiff = 0;
       }
-      acc += 1;
-      if (fabs(tf - t) <= kTenEps) result = kComplete;
-    } else {
-      rejected_last = true;
-      rej += 1;
     }
+    // Error::Stiffness (stiff reaches 15 only on an accepted attempt) ends the
+    // solve with (t, y) at the start of the step: the attempt is not taken.
+    const bool stop = stiff == 15;
+    const bool take = accept && !stop;
+    if (DENSE && take && a.n_eval > 0) {
+      // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
+      // ordinary.rs:312-333.
+      double cont1[D], cont2[D], cont3[D];
+      for (int c = 0; c < D; ++c) {
+        double ydiff = ynew[c] + (-1.0) * y[c];
+        cont1[c] = ydiff;
+        double bspl = h * k0[c] + (-1.0) * ydiff;
+        cont2[c] = bspl;
+        cont3[c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
+      }
+      const double hp = h, tp = t;
+      const double cont4 = 0.0 * h;
+      const double* y0p = y;
+      teval_emit<D>(a, i, dir, next_eval, pushed, t_new, t, ynew, [&](double te, double* out) {
+        double s = (te - tp) / hp;
+        double s1 = 1.0 - s;
+        for (int c = 0; c < D; ++c) {
+          double acc_ = cont4 * s1 + cont3[c];
+          acc_ = acc_ * s + cont2[c];
+          acc_ = acc_ * s1 + cont1[c];
+          out[c] = y0p[c] + s * acc_;
+        }
+      });
+    }
+    h_prev = take ? h : h_prev;
+    t = take ? t_new : t;
+    for (int c = 0; c < D; ++c) {
+      y[c] = take ? ynew[c] : y[c];
+      k0[c] = take ? dn[c] : k0[c];
+    }
+    // After a rejection the next accepted step may not grow h.
+    scale = take && rejected_last ? fmin(scale, 1.0) : scale;
+    rejected_last = !accept;
+    acc += take ? 1 : 0;
+    rej += accept ? 0 : 1;
+    const int32_t result =
+        stop ? kStiffness : (take && fabs(tf - t) <= kTenEps ? kComplete : -1);
     h = constrain_step_size(h * scale, a.h_min, a.h_max);
     return result;
   }

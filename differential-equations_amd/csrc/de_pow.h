@@ -18,10 +18,25 @@
 //   Z = y log x (dd);  Z = m ln2/128 + s;  x^y = 2^(m>>7) 2^((m&127)/128) e^s
 // Domain: finite y with |y| <= 0.5 (the controller uses +-1/order <= 0.2), any x.
 // For those arguments the result is always a normal double.
+//
+// pow(x, -0.2), the exponent of every DOPRI5 step, has a cheaper first try:
+// a table-driven fifth root whose result is checked by Ziv's rounding test
+// (in the form of the CRlibm documentation) and returned only when it is
+// certainly the correctly rounded value; otherwise, and for non-normal x, the
+// double-double evaluation above runs.  Both parts use only exactly rounded
+// operations, so host and device still agree bit for bit.
 #pragma once
 
 #include "de_common.h"
 #include "de_pow_tables.h"
+
+// The error-free transformations below need every product and sum rounded
+// on its own, also in translation units compiled with FMA contraction.
+#if defined(__clang__)
+#define DE_NO_CONTRACT _Pragma("clang fp contract(off)")
+#else
+#define DE_NO_CONTRACT
+#endif
 
 namespace de {
 
@@ -34,6 +49,7 @@ struct dd {
 
 // Error-free transformations.
 DE_HD dd two_sum(double a, double b) {
+  DE_NO_CONTRACT
   double s = a + b;
   double bb = s - a;
   double e = (a - (s - bb)) + (b - bb);
@@ -42,12 +58,14 @@ DE_HD dd two_sum(double a, double b) {
 
 // Requires exponent(a) >= exponent(b) or a == 0.
 DE_HD dd fast_two_sum(double a, double b) {
+  DE_NO_CONTRACT
   double s = a + b;
   double e = b - (s - a);
   return {s, e};
 }
 
 DE_HD dd two_prod(double a, double b) {
+  DE_NO_CONTRACT
   double p = a * b;
   double e = fma(a, b, -p);
   return {p, e};
@@ -55,6 +73,7 @@ DE_HD dd two_prod(double a, double b) {
 
 // Round-to-nearest-even integer of |v| < 2^51 using plain IEEE adds.
 DE_HD double round_to_int(double v) {
+  DE_NO_CONTRACT
   const double shift = 0x1.8p52;
   return (v + shift) - shift;
 }
@@ -62,6 +81,7 @@ DE_HD double round_to_int(double v) {
 // log(x) as a double-double for finite x > 0.  Absolute error < 2^-74 for
 // |log x| < 745.
 DE_HD dd pow_log_dd(double x) {
+  DE_NO_CONTRACT
   uint64_t ix = as_u64(x);
   int64_t kadj = 0;
   if ((ix >> 52) == 0) {  // subnormal: scale into the normal range (exact)
@@ -107,6 +127,7 @@ DE_HD dd pow_log_dd(double x) {
 
 // exp(Z) for a double-double Z with |Z| < 700, rounded once to double.
 DE_HD double pow_exp_dd(dd Z) {
+  DE_NO_CONTRACT
   double kz = round_to_int(Z.hi * DE_POW_EXP_INV_C);
   int64_t ki = (int64_t)kz;
   double sh = Z.hi - kz * DE_POW_EXP_C_HI;
@@ -149,7 +170,8 @@ DE_HD double pow_exp_dd(dd Z) {
 // pow(x, y) with C99 special cases for the arguments the integrator can see.
 // The general path runs on a substitute argument for the special cases and
 // their results are selected at the end (no divergent early returns).
-DE_HD double pow(double x, double y) {
+DE_HD double pow_dd(double x, double y) {
+  DE_NO_CONTRACT
   const double inf = as_f64(0x7ff0000000000000ULL);
   const bool special = !(x > 0.0 && x < inf) || x == 1.0 || y == 0.0 || y != y;
   dd L = pow_log_dd(special ? 2.0 : x);
@@ -164,6 +186,85 @@ DE_HD double pow(double x, double y) {
     else r = y < 0.0 ? 0.0 : inf;                          // x = +inf
   }
   return r;
+}
+
+DE_TABLE double kPow5InvTable[128 * 2] = DE_POW5_INV_TABLE;
+DE_TABLE double kPow5ATable[640 * 2] = DE_POW5_A_TABLE;
+
+// x^(-0.2) for positive normal x.  Returns false when the result could not be
+// certified (then *out is meaningless); for other x the return is false too.
+//
+// Let D = -0.2 = -(1/5 + delta), delta = 2^-54/5.  With x = 2^E m, m in [1,2),
+// E = 5q + s (0 <= s < 5), r = m*invc_i - 1 (exact, |r| < 2^-7.48):
+//   x^D = 2^-q * [2^(-s/5) invc_i^(1/5)] * (1+r)^(-1/5) * exp(-delta log x)
+// The bracket is the table entry A = ah + al.  (1+r)^(-1/5) is its Taylor
+// series to r^9 (truncation < 2^-79), kept as 1 + p1 + ulo with p1 + e1 = -r/5
+// exact to 2^-118.  corr = -delta log x = -delta (E ln2 - log invc_i + r) is at
+// most 2^-46.8 and is needed to 2^-70 only (the dropped -delta r^2/2 is below
+// 2^-71.3).  Every rounding after the exact steps is at most 2^-70 relative to
+// the result and there are fewer than six of them, so s + w below is within
+// eps < 2^-67.4 (relative) of x^D.  Rounding test: res = RN(s + w) with
+// residual d = s + w - res (exact, Fast2Sum); if RN(res + d*(1 + 2^-12)) ==
+// res then |x^D - res| < ulp(res)/2 (ulp/4 below a power of two), because
+// 2^54 eps < 1 - 1/(1 + 2^-12).  The test fails for about 2^-12 of all x.
+DE_HD bool pow_m02_fast(double x, double* out) {
+  DE_NO_CONTRACT
+  const uint64_t ix = as_u64(x);
+  const uint32_t hx = (uint32_t)(ix >> 32);
+  const uint32_t i = (hx >> 13) & 127u;
+  const uint32_t u = ((hx >> 20) & 0x7ffu) + 2u;  // E + 1025 = 5 (q + 205) + s
+  const uint32_t fq = (u * 52429u) >> 18;         // floor(u / 5), exact for u < 81920
+  const uint32_t s = u - 5u * fq;
+  const uint32_t row = s * 128u + i;              // < 640 for every x
+  const double m = as_f64((ix & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+  const double invc = kPow5InvTable[2 * i];
+  const double dlinv = kPow5InvTable[2 * i + 1];  // delta log(invc_i)
+  const double ah = kPow5ATable[2 * row];
+  const double al = kPow5ATable[2 * row + 1];
+
+  const double r = fma(m, invc, -1.0);
+  const double e = (double)((int32_t)u - 1025);
+  const double corr = fma(DE_POW5_NEG_DELTA, r, fma(e, DE_POW5_NEG_DELTA_LN2, dlinv));
+
+  const double c[8] = DE_POW5_POLY;  // binom(-1/5, k), k = 9 .. 2
+  double t = c[0];
+  t = fma(r, t, c[1]);
+  t = fma(r, t, c[2]);
+  t = fma(r, t, c[3]);
+  t = fma(r, t, c[4]);
+  t = fma(r, t, c[5]);
+  t = fma(r, t, c[6]);
+  t = fma(r, t, c[7]);
+  const double p1 = r * DE_POW5_C1_HI;
+  const double e1 = fma(r, DE_POW5_C1_HI, -p1);
+  const double v = fma(r, t, DE_POW5_C1_LO);
+  const double ulo = fma(r, v, e1);               // (1+r)^(-1/5) = 1 + p1 + ulo
+  const double lt = ulo + fma(corr, p1 + ulo, corr);  // (..)(1+corr) = 1 + p1 + lt
+  const double alt = fma(al, p1 + lt, al);
+  const double m1 = ah * p1;
+  const double m1e = fma(ah, p1, -m1);
+  const double lo = fma(ah, lt, m1e + alt);
+  const double sh = ah + m1;                      // |ah| > |m1|
+  const double st = m1 - (sh - ah);
+  const double w = st + lo;
+  const double res = sh + w;
+  const double d = w - (res - sh);
+  const bool normal = (hx - 0x00100000u) < 0x7fe00000u;  // 0 < x < inf, not subnormal
+  *out = res * as_f64((uint64_t)(1228u - fq) << 52);     // 2^-q, q in [-205, 204]
+  return normal && fma(d, 0x1.001p+0, res) == res;
+}
+
+// Kept out of line on the device so that the rare fallback is a call in a
+// branch, not code scheduled into the step loop.
+DE_COLD double pow_dd_cold(double x, double y) { return pow_dd(x, y); }
+
+DE_HD double pow(double x, double y) {
+  if (y == -0.2) {
+    double r;
+    if (__builtin_expect(pow_m02_fast(x, &r), 1)) return r;
+    return pow_dd_cold(x, y);
+  }
+  return pow_dd(x, y);
 }
 
 }  // namespace de

@@ -184,7 +184,11 @@ struct VecDopri5 {
       return true;
     }
     steps += 1;
+    // The stiffness test runs on accepted attempts whose number is a multiple
+    // of 100 (ordinary.rs:167); the countdown restarts on every such attempt.
     countdown -= 1;
+    const bool stiff_check = countdown == 0;
+    if (stiff_check) countdown = 100;
     const double a10h = dp::a10 * h;
     for (int c = 0; c < n; ++c) ys[c] = y[c] + a10h * k[0][c];
     diff(t + dp::c1 * h, ys, k[1]);
@@ -241,8 +245,7 @@ struct VecDopri5 {
       // stage-6 state for the stiffness test.
       double* dn = k[3];
       diff(t_new, ynew, dn);
-      if (countdown == 0) {
-        countdown = 100;
+      if (stiff_check) {
         double stdnum = 0.0, stden = 0.0;
         for (int c = 0; c < n; ++c) {
           double u = k[1][c] - k[6][c];
@@ -509,7 +512,11 @@ struct GroupDopri5 {
       return true;
     }
     steps += 1;
+    // The stiffness test runs on accepted attempts whose number is a multiple
+    // of 100 (ordinary.rs:167); the countdown restarts on every such attempt.
     countdown -= 1;
+    const bool stiff_check = countdown == 0;
+    if (stiff_check) countdown = 100;
     double k1x[C], k1v[C], k2x[C], k2v[C], k3x[C], k3v[C], k4x[C], k4v[C], k5x[C], k5v[C],
         k6x[C], k6v[C], sx[C], sv[C];
     const double a10h = dp::a10 * h;
@@ -595,8 +602,7 @@ struct GroupDopri5 {
       double t_new = t + h;
       double dnx[C], dnv[C];
       rhs(nx, nv, dnx, dnv);
-      if (countdown == 0) {
-        countdown = 100;
+      if (stiff_check) {
         double stdnum = 0.0, stden = 0.0;
 #pragma unroll
         for (int q = 0; q < C; ++q) {

@@ -41,8 +41,11 @@ __global__ __launch_bounds__(kBlock) void ode_plain_kernel(OdeArgs a) {
   L.finish(a, i, st);
 }
 
+// Three waves per SIMD: the DOPRI5 lane needs about 165 VGPRs, and the
+// occupancy hint keeps the allocation under the 168-register limit for 3.
 template <class Lane>
-__global__ __launch_bounds__(kBlock) void ode_persistent_kernel(OdeArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void
+ode_persistent_kernel(OdeArgs a) {
   Lane L;
   bool have = false;
   bool drained = false;

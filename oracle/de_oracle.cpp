@@ -998,6 +998,12 @@ void de_oracle_pow_many(const double* x, const double* y, double* out, int64_t n
   for (int64_t i = 0; i < n; ++i) out[i] = de::pow(x[i], y[i]);
 }
 
+// The first try of pow(x, -0.2) alone: ok[i] = 1 when its rounding test
+// certified out[i], else 0 (and pow falls back to the double-double path).
+void de_oracle_pow_m02_fast(const double* x, double* out, uint8_t* ok, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) ok[i] = de::pow_m02_fast(x[i], &out[i]) ? 1 : 0;
+}
+
 double de_oracle_noise_normal(uint64_t seed, uint64_t path, uint32_t step, uint32_t comp) {
   return de::noise_normal(seed, path, step, comp);
 }

@@ -41,6 +41,8 @@ def load():
         lib.de_oracle_pow.argtypes = [C.c_double, C.c_double]
         lib.de_oracle_pow_many.restype = None
         lib.de_oracle_pow_many.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        lib.de_oracle_pow_m02_fast.restype = None
+        lib.de_oracle_pow_m02_fast.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         lib.de_oracle_noise_normal.restype = C.c_double
         lib.de_oracle_noise_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
         lib.de_oracle_philox.restype = None
@@ -84,6 +86,16 @@ def pow_many(x, y):
     out = np.empty_like(x)
     load().de_oracle_pow_many(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
     return out
+
+
+def pow_m02_fast(x):
+    """First try of pow(x, -0.2): (values, certified flags)."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    ok = np.empty(x.shape, dtype=np.uint8)
+    load().de_oracle_pow_m02_fast(x.ctypes.data, out.ctypes.data, ok.ctypes.data, x.size)
+    return out, ok.astype(bool)
 
 
 def philox(ctr, k0, k1):

@@ -194,6 +194,26 @@ def test_stiffness_detection_bit_exact(orc):
     assert_ode_equal(got, want)
 
 
+def test_stiffness_check_after_rejected_attempt(orc):
+    # The stiffness test runs on accepted attempts whose number is a multiple
+    # of 100 (ordinary.rs:167).  With safety factor 1 and an absolute-only
+    # tolerance the controller rejects about two attempts in three, so some
+    # multiples of 100 fall on rejected attempts; the checks at later
+    # multiples must still happen (a countdown that only restarted on accepted
+    # attempts missed them and let these solves run to the end).
+    n = 256
+    rng = np.random.default_rng(5)
+    ens = IVP.ode(deamd.Exponential(1.0), 0.0, 20.0, np.ones((1, n))).method(
+        ExplicitRungeKutta.dopri5().rtol(0.0).atol(1e-9).safety_factor(1.0).max_steps(20000))
+    ens.atol_traj = 1e-9 * np.exp(rng.uniform(-3, 3, n))
+    ens.rtol_traj = np.zeros(n)
+    want = orc.solve_ode(ens)
+    got = ens.solve()
+    assert (want.status == deamd.DE_STATUS_STIFFNESS).sum() > n // 4
+    assert (want.status == deamd.DE_STATUS_COMPLETE).sum() > n // 4
+    assert_ode_equal(got, want)
+
+
 # -------------------------------------------------------------------- SDE ---
 @pytest.mark.parametrize("scheme", [EulerMaruyama, Milstein])
 @pytest.mark.parametrize("family", ["ou8", "gbm"])

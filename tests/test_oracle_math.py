@@ -276,6 +276,34 @@ def test_pow_is_correctly_rounded(orc, y):
     assert libm_mis < 0.01 * len(xs)
 
 
+def test_pow_fifth_root_path_certifies_only_correct_roundings(orc):
+    # pow(x, -0.2) first tries a table-driven fifth root and keeps it only when
+    # its rounding test certifies the correctly rounded value; the test is
+    # designed to fail for about 2^-12 of arguments, which then take the
+    # double-double path.  Check every failure and a sample of successes
+    # against a 160-bit evaluation, and the rate of failures.
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 160
+    rng = np.random.default_rng(44)
+    xs = np.concatenate([np.exp(rng.uniform(np.log(1e-12), np.log(1e6), 150000)),
+                         2.0 ** rng.uniform(-1022, 1023, 50000)])
+    fast, ok = orc.pow_m02_fast(xs)
+    full = orc.pow_many(xs, -0.2)
+    assert 0 < (~ok).sum() < 1e-3 * len(xs)
+    assert np.array_equal(full[ok], fast[ok])
+    cr = lambda x: float(mpmath.power(mpmath.mpf(float(x)), mpmath.mpf(-0.2)))
+    for x, g in zip(xs[~ok], full[~ok]):
+        assert g == cr(x), x
+    idx = np.flatnonzero(ok)[:: max(1, ok.sum() // 3000)]
+    for x, g in zip(xs[idx], fast[idx]):
+        assert g == cr(x), x
+    # Zero, subnormal, infinite, NaN and negative arguments are never certified.
+    _, ok_special = orc.pow_m02_fast(np.array([0.0, 5e-324, 1e-310, math.inf, math.nan, -1.0, -0.0]))
+    assert not ok_special.any()
+    _, ok_edges = orc.pow_m02_fast(np.array([2.2250738585072014e-308, 1.7976931348623157e308, 1.0]))
+    assert ok_edges.all()
+
+
 def test_pow_special_cases(orc):
     assert orc.pow_many(np.array([0.0]), -0.2)[0] == math.inf
     assert orc.pow_many(np.array([0.0]), 0.2)[0] == 0.0
