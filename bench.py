@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--n", type=int, default=0, help="override ensemble size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--only", choices=["persistent", "plain", "fast"], default=None,
+    ap.add_argument("--only", choices=["persistent", "fifo", "plain", "fast"], default=None,
                     help="run only one Lorenz variant (profiling)")
     ap.add_argument("--cpu-sample-stride", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
@@ -169,12 +169,14 @@ def bench_lorenz(args, dv):
         return dict(wall=wall, lat=lat, A=A, R=R, ok=status_ok)
 
     if args.only:
-        sched = _abi.DE_SCHED_PLAIN if args.only == "plain" else _abi.DE_SCHED_PERSISTENT
+        sched = {"plain": _abi.DE_SCHED_PLAIN,
+                 "fifo": _abi.DE_SCHED_PERSISTENT_FIFO}.get(args.only, _abi.DE_SCHED_PERSISTENT)
         r = run(sched, args.only == "fast", args.steps, args.warmup)
         return {"variant": args.only, "accepted": r["A"], "rejected": r["R"],
                 "kernel_ms": sum(r["lat"]) / len(r["lat"])}, (p, y0, n)
     res = run(_abi.DE_SCHED_PERSISTENT, False, args.steps, args.warmup)
     plain = None if args.no_plain else run(_abi.DE_SCHED_PLAIN, False, max(3, args.steps // 2), 1)
+    fifo = run(_abi.DE_SCHED_PERSISTENT_FIFO, False, max(3, args.steps // 2), 1)
     fast = run(_abi.DE_SCHED_PERSISTENT, True, max(3, args.steps // 2), 1)
     if args.calibrate_copy:
         calibrate_copy(dv)
@@ -211,8 +213,14 @@ def bench_lorenz(args, dv):
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": load_traffic("lorenz_dopri5_1M"),
                      "flop_model": "263 per accepted + 255 per rejected step (7F+69n, 6F+69n)",
-                     "kernel_ms": kernel_ms},
+                     "kernel_ms": kernel_ms,
+                     "kernel_ms_note": "HIP events around each solve on its stream: the "
+                                       "longest-first ordering pre-pass plus the persistent "
+                                       "kernel"},
     }
+    fi = summarize(fifo)
+    line["persistent_fifo"] = {"value": fi["value"], "ms_per_step": fi["ms_per_step"],
+                               "note": "same kernel, trajectories in index order"}
     if plain is not None:
         ps = summarize(plain)
         line["plain"] = {"value": ps["value"], "ms_per_step": ps["ms_per_step"],

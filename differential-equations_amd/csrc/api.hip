@@ -126,7 +126,7 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->n > 0 && b->family != DE_FAMILY_SPRING_CHAIN && kFamilyParams[b->family] > 0 &&
       !b->params)
     return fail(DE_ERR_INVALID, "family %d needs params", b->family);
-  if (b->schedule < DE_SCHED_AUTO || b->schedule > DE_SCHED_PERSISTENT)
+  if (b->schedule < DE_SCHED_AUTO || b->schedule > DE_SCHED_PERSISTENT_FIFO)
     return fail(DE_ERR_INVALID, "unknown schedule %d", b->schedule);
   if (b->max_steps < 0) return fail(DE_ERR_INVALID, "max_steps < 0");
   return DE_OK;
@@ -186,7 +186,7 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
                               : hip_fail(hipGetLastError(), "spring launch");
     return DE_OK;
   }
-  if (sched == de::kSchedPersistent) {
+  if (sched == de::kSchedPersistent || sched == de::kSchedPersistentFifo) {
     int rc = counter_slot(&a.counter);
     if (rc != DE_OK) return rc;
   }
@@ -237,7 +237,10 @@ void de_sde_batch_init(de_sde_batch* b) {
 int64_t de_ode_workspace_bytes(const de_ode_batch* b) {
   if (!b) return 0;
   if (b->family == DE_FAMILY_SPRING_CHAIN) return de::parity::spring_workspace_bytes(b);
-  return (int64_t)sizeof(unsigned long long);
+  // Work counter, plus the longest-first order of the persistent schedule
+  // (256 bucket counts, a 4-byte position and a 1-byte bucket per trajectory).
+  const bool ordered = b->schedule == DE_SCHED_AUTO || b->schedule == DE_SCHED_PERSISTENT;
+  return (int64_t)sizeof(unsigned long long) + (ordered ? 1024 + 5 * b->n : 0);
 }
 
 int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {

@@ -54,7 +54,20 @@ struct OdeArgs {
   double* dense;
   int32_t* n_dense;
   unsigned long long* counter;  // persistent schedule: next trajectory index
+  const uint32_t* order;        // persistent schedule: trajectory at each position, or null
 };
+
+// Bucket of a predicted cost c (attempts) for the longest-first order of the
+// persistent schedule: 16 buckets per octave over [1, 2^16), 0 = longest; NaN,
+// c < 1 and ended trajectories go to the last bucket (255).
+DE_HD uint32_t cost_bucket(double c) {
+  if (!(c >= 1.0)) return 255u;
+  if (c >= 65536.0) return 0u;
+  const uint64_t u = as_u64(c);
+  const uint32_t e = (uint32_t)(u >> 52) - 1023u;  // 0 .. 15
+  const uint32_t m = (uint32_t)(u >> 48) & 15u;    // top 4 fraction bits
+  return 255u - (e * 16u + m);
+}
 
 // DOPRI5 coefficients (src/tableau/dorman_prince.rs:37-117).
 namespace dp {
@@ -174,6 +187,23 @@ struct Dopri5Lane {
     bool ok;
     *this = start(a, i, &ok);
     return ok;
+  }
+
+  // Predicted number of attempts, |tf - t0| / |h0|, with h0 the initial step
+  // the solve will use (the automatic one when a.h0 == 0).  Only orders work.
+  DE_HD static double predicted_cost(const OdeArgs& a, int64_t i) {
+    F f;
+    f.load(a.params, a.n, i);
+    const double t0 = a.t0 ? a.t0[i] : a.t0s;
+    const double tf = a.tf ? a.tf[i] : a.tfs;
+    double h0 = a.h0;
+    if (h0 == 0.0 && tf != t0) {
+      double y[D];
+      for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
+      h0 = lane_h_init(f, t0, tf, y, a.rtol_traj ? a.rtol_traj[i] : a.rtol,
+                       a.atol_traj ? a.atol_traj[i] : a.atol, a.h_min, a.h_max);
+    }
+    return fabs(tf - t0) / fabs(h0);
   }
 
   // Driver prologue + DOPRI5 init.  Returns false when the trajectory ended
@@ -388,6 +418,14 @@ struct FixedLane {
   }
 
   DE_HD bool begin(const OdeArgs& a, int64_t i) { return init(a, i); }
+
+  // Number of steps |tf - t0| / |h0| (only orders work).
+  DE_HD static double predicted_cost(const OdeArgs& a, int64_t i) {
+    const double t0 = a.t0 ? a.t0[i] : a.t0s;
+    const double tf = a.tf ? a.tf[i] : a.tfs;
+    const double h0 = a.h0 == 0.0 ? fabs(tf - t0) / 100.0 : a.h0;
+    return fabs(tf - t0) / fabs(h0);
+  }
 
   DE_HD bool init(const OdeArgs& a, int64_t i) {
     f.load(a.params, a.n, i);

@@ -9,9 +9,11 @@
 //   plain      one thread per trajectory; a wave lives until its slowest lane
 //              finishes.
 //   persistent a grid sized to the resident wave slots; each lane integrates a
-//              trajectory and, when it finishes, takes the next index from a
+//              trajectory and, when it finishes, takes the next position from a
 //              global counter (one atomic per wave, ballot-aggregated), so
 //              lanes stay busy while step counts vary ~10x across trajectories.
+//              Positions map to trajectories longest-predicted first (see
+//              launch_order) or, for PERSISTENT_FIFO, in index order.
 #include <hip/hip_runtime.h>
 
 #include "de_lane.h"
@@ -27,6 +29,7 @@ namespace de {
 namespace DE_KNS {
 
 constexpr int kBlock = 256;
+constexpr int64_t kOrderMinN = 4096;
 
 template <class Lane>
 __global__ __launch_bounds__(kBlock) void ode_plain_kernel(OdeArgs a) {
@@ -66,7 +69,7 @@ ode_persistent_kernel(OdeArgs a) {
         if (need) {
           unsigned long long my = base + (unsigned long long)__popcll(mask & __lanemask_lt());
           if (my < n) {
-            idx = (int64_t)my;
+            idx = a.order ? (int64_t)a.order[my] : (int64_t)my;
             have = L.begin(a, idx);
           }
         }
@@ -87,6 +90,78 @@ ode_persistent_kernel(OdeArgs a) {
 }
 
 __global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ull; }
+
+// ---- Longest-first order for the persistent schedule ----------------------
+// A counting sort of trajectories by cost_bucket(predicted_cost): the resident
+// lanes then start the longest trajectories first and finish on the short
+// ones, so few lanes idle while their wave waits for its last trajectory.
+// Order within a bucket is arbitrary (it follows atomic arrival); results do
+// not depend on it, each trajectory being solved on its own.
+constexpr int kBuckets = 256;
+static_assert(kBlock == kBuckets, "cost_kernel and scatter_kernel use one thread per bucket");
+
+__global__ __launch_bounds__(kBuckets) void zero_hist_kernel(uint32_t* hist) {
+  hist[threadIdx.x] = 0u;
+}
+
+template <class Lane>
+__global__ __launch_bounds__(kBlock) void cost_kernel(OdeArgs a, uint8_t* bucket,
+                                                      uint32_t* hist) {
+  __shared__ uint32_t h[kBuckets];
+  h[threadIdx.x] = 0u;  // kBlock == kBuckets
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < a.n) {
+    const uint32_t b = cost_bucket(Lane::predicted_cost(a, i));
+    bucket[i] = (uint8_t)b;
+    atomicAdd(&h[b], 1u);
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// Exclusive prefix sum of the 256 counts, in place (one block).
+__global__ __launch_bounds__(kBuckets) void scan_hist_kernel(uint32_t* hist) {
+  __shared__ uint32_t v[kBuckets];
+  const int t = threadIdx.x;
+  v[t] = hist[t];
+  __syncthreads();
+  for (int d = 1; d < kBuckets; d <<= 1) {
+    const uint32_t add = t >= d ? v[t - d] : 0u;
+    __syncthreads();
+    v[t] += add;
+    __syncthreads();
+  }
+  hist[t] = v[t] - hist[t];
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t n, const uint8_t* bucket,
+                                                         uint32_t* offset, uint32_t* order) {
+  __shared__ uint32_t cnt[kBuckets], base[kBuckets];
+  cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint32_t b = 0u, local = 0u;
+  if (i < n) {
+    b = bucket[i];
+    local = atomicAdd(&cnt[b], 1u);
+  }
+  __syncthreads();
+  if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&offset[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (i < n) order[base[b] + local] = (uint32_t)i;
+}
+
+// Fills a.order (n entries) using `bucket` (n bytes) and `hist` (256 words).
+template <class Lane>
+static void launch_order(const OdeArgs& a, uint8_t* bucket, uint32_t* hist, uint32_t* order,
+                         hipStream_t s) {
+  const unsigned grid = (unsigned)((a.n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(zero_hist_kernel, dim3(1), dim3(kBuckets), 0, s, hist);
+  hipLaunchKernelGGL(cost_kernel<Lane>, dim3(grid), dim3(kBlock), 0, s, a, bucket, hist);
+  hipLaunchKernelGGL(scan_hist_kernel, dim3(1), dim3(kBuckets), 0, s, hist);
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kBlock), 0, s, a.n, bucket, hist, order);
+}
 
 template <class Lane>
 static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
@@ -112,8 +187,23 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
   int64_t want = (int64_t)num_cu * blocks_per_cu;
   int64_t need = (a.n + kBlock - 1) / kBlock;
   int64_t grid = want < need ? want : need;
+  OdeArgs pa = a;
+  pa.order = nullptr;
+  void* work = nullptr;
+  // Longest-first order (schedule PERSISTENT) for batches of a few thousand
+  // trajectories or more; index order (PERSISTENT_FIFO, or small batches).
+  if (schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL) {
+    const size_t bytes = kBuckets * sizeof(uint32_t) + (size_t)a.n * (sizeof(uint32_t) + 1);
+    if (hipMallocAsync(&work, bytes, s) != hipSuccess) return -1;
+    uint32_t* hist = (uint32_t*)work;
+    uint32_t* order = hist + kBuckets;
+    uint8_t* bucket = (uint8_t*)(order + a.n);
+    launch_order<Lane>(a, bucket, hist, order, s);
+    pa.order = order;
+  }
   hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-  hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s, pa);
+  if (work && hipFreeAsync(work, s) != hipSuccess) return -1;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

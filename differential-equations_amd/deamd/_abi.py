@@ -48,6 +48,7 @@ DE_NOISE_CONSTANT = 2
 DE_SCHED_AUTO = 0
 DE_SCHED_PLAIN = 1
 DE_SCHED_PERSISTENT = 2
+DE_SCHED_PERSISTENT_FIFO = 3
 
 DE_MAP_AUTO = 0
 DE_MAP_LANE = 1

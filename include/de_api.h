@@ -83,7 +83,9 @@ extern "C" {
 /* Launch schedules for the ensemble. */
 #define DE_SCHED_AUTO 0       /* library choice (autotuned) */
 #define DE_SCHED_PLAIN 1      /* one thread per trajectory, one launch */
-#define DE_SCHED_PERSISTENT 2 /* resident lanes refilled from a work counter */
+#define DE_SCHED_PERSISTENT 2 /* resident lanes refilled from a work counter, longest
+                                 predicted trajectories first */
+#define DE_SCHED_PERSISTENT_FIFO 3 /* as PERSISTENT, trajectories taken in index order */
 
 /* Mappings of a system onto lanes (spring chain). */
 #define DE_MAP_AUTO 0  /* autotuner picks per size class */

@@ -121,6 +121,86 @@ def test_lorenz_persistent_large_batch_matches_plain():
     assert_ode_equal(b, a)
 
 
+class HipBuffers:
+    """Device buffers through the HIP runtime the library itself links (a
+    second runtime, such as the one bundled with torch, is not initialised
+    after the library's in the same process)."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.ptrs = []
+
+    def alloc(self, nbytes):
+        p = self.C.c_void_p()
+        assert self.hip.hipMalloc(self.C.byref(p), nbytes) == 0
+        self.ptrs.append(p)
+        return p
+
+    def put(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(arr.nbytes)
+        assert self.hip.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) == 0
+        return p
+
+    def poisoned(self, nbytes):
+        p = self.alloc(nbytes)
+        assert self.hip.hipMemset(p, 0xFF, nbytes) == 0   # NaN doubles, -1 integers
+        return p
+
+    def get(self, p, shape, dtype):
+        out = np.empty(shape, dtype)
+        assert self.hip.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
+def test_persistent_orders_cover_every_trajectory():
+    # PERSISTENT takes trajectories longest-predicted first through a device
+    # counting sort; PERSISTENT_FIFO in index order.  Outputs are device
+    # buffers poisoned beforehand, so a trajectory that a wrong permutation
+    # skipped (or solved twice in place of another) cannot match PLAIN.
+    import ctypes as C
+    from deamd import _abi
+    lib = deamd.lib()
+    mem = HipBuffers()
+    n = 50000
+    p, y0 = deamd.gen_lorenz(n, seed=17)
+    d_p, d_y0 = mem.put(p.reshape(-1)), mem.put(y0.reshape(-1))
+    d_tf = mem.put(np.linspace(0.5, 8.0, n))   # uneven lengths
+    fields = [("y", 3 * n, np.float64), ("t", n, np.float64), ("acc", n, np.int64),
+              ("rej", n, np.int64), ("ev", n, np.int64), ("st", n, np.int32)]
+    results = {}
+    try:
+        for sched in (DE_SCHED_PLAIN, DE_SCHED_PERSISTENT, _abi.DE_SCHED_PERSISTENT_FIFO):
+            out = {k: mem.poisoned(m * np.dtype(t).itemsize) for k, m, t in fields}
+            b = _abi.ode_defaults()
+            b.family, b.method, b.n, b.dim = _abi.DE_FAMILY_LORENZ, _abi.DE_METHOD_DOPRI5, n, 3
+            b.params, b.y0, b.tf = d_p, d_y0, d_tf
+            b.rtol, b.atol, b.max_steps, b.schedule = 1e-8, 1e-10, 1000000, sched
+            r = _abi.OdeResult()
+            r.y_final, r.t_final, r.accepted = out["y"], out["t"], out["acc"]
+            r.rejected, r.evals, r.status = out["rej"], out["ev"], out["st"]
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), None))
+            results[sched] = {k: mem.get(out[k], m, t) for k, m, t in fields}
+    finally:
+        mem.free()
+    want = results[DE_SCHED_PLAIN]
+    assert (want["st"] == DE_STATUS_COMPLETE).all()
+    for sched in (DE_SCHED_PERSISTENT, _abi.DE_SCHED_PERSISTENT_FIFO):
+        for k in want:
+            assert np.array_equal(results[sched][k], want[k]), (sched, k)
+
+
 # ------------------------------------------------ reference test systems ----
 from test_oracle_reference_kats import ACCURACY  # noqa: E402
 
