@@ -237,10 +237,13 @@ void de_sde_batch_init(de_sde_batch* b) {
 int64_t de_ode_workspace_bytes(const de_ode_batch* b) {
   if (!b) return 0;
   if (b->family == DE_FAMILY_SPRING_CHAIN) return de::parity::spring_workspace_bytes(b);
-  // Work counter, plus the longest-first order of the persistent schedule
-  // (256 bucket counts, a 4-byte position and a 1-byte bucket per trajectory).
+  // Work counter; for the persistent schedules the launch arguments in device
+  // memory and, for the longest-first order, 256 bucket counts plus a 4-byte
+  // position and a 1-byte bucket per trajectory.
   const bool ordered = b->schedule == DE_SCHED_AUTO || b->schedule == DE_SCHED_PERSISTENT;
-  return (int64_t)sizeof(unsigned long long) + (ordered ? 1024 + 5 * b->n : 0);
+  const bool persistent = ordered || b->schedule == DE_SCHED_PERSISTENT_FIFO;
+  return (int64_t)sizeof(unsigned long long) + (persistent ? 512 : 0) +
+         (ordered ? 1024 + 5 * b->n : 0);
 }
 
 int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {

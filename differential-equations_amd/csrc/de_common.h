@@ -19,9 +19,11 @@
 #include <hip/hip_runtime.h>
 #define DE_HD __host__ __device__ inline
 #define DE_DEV __device__ inline
-// Run once per trajectory (init, result write-back): kept out of line so the
-// step loop's registers are not spent on their values.
-#define DE_COLD __host__ __device__ inline __attribute__((noinline))
+// Run once per trajectory (init, result write-back), inside branches a wave
+// takes only when one of its lanes starts or ends a trajectory.  Forced
+// inline: as calls they would pass the lane state through scratch memory,
+// and the partial-wave scratch writes would multiply the kernel's HBM traffic.
+#define DE_COLD __host__ __device__ inline __attribute__((always_inline))
 #else
 #define DE_HD inline
 #define DE_DEV inline

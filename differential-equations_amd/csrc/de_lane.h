@@ -163,9 +163,9 @@ struct Dopri5Lane {
   int32_t acc, rej, base_ev, stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
 
-  // Result write-back and trajectory start run once per trajectory; they are
-  // out-of-line functions taking the lane by value so that the step loop's
-  // copy of the lane never has its address taken (it stays in registers).
+  // Result write-back and trajectory start run once per trajectory; they take
+  // and return the lane by value so that the step loop's copy of the lane
+  // never has its address taken (it stays in registers).
   DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, Dopri5Lane L) {
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
     a.t_final[i] = L.t;

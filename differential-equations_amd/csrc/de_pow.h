@@ -34,8 +34,10 @@
 // on its own, also in translation units compiled with FMA contraction.
 #if defined(__clang__)
 #define DE_NO_CONTRACT _Pragma("clang fp contract(off)")
+#define DE_FORCE_INLINE __attribute__((always_inline))
 #else
 #define DE_NO_CONTRACT
+#define DE_FORCE_INLINE
 #endif
 
 namespace de {
@@ -80,7 +82,7 @@ DE_HD double round_to_int(double v) {
 
 // log(x) as a double-double for finite x > 0.  Absolute error < 2^-74 for
 // |log x| < 745.
-DE_HD dd pow_log_dd(double x) {
+DE_HD DE_FORCE_INLINE dd pow_log_dd(double x) {
   DE_NO_CONTRACT
   uint64_t ix = as_u64(x);
   int64_t kadj = 0;
@@ -126,7 +128,7 @@ DE_HD dd pow_log_dd(double x) {
 }
 
 // exp(Z) for a double-double Z with |Z| < 700, rounded once to double.
-DE_HD double pow_exp_dd(dd Z) {
+DE_HD DE_FORCE_INLINE double pow_exp_dd(dd Z) {
   DE_NO_CONTRACT
   double kz = round_to_int(Z.hi * DE_POW_EXP_INV_C);
   int64_t ki = (int64_t)kz;
@@ -170,7 +172,7 @@ DE_HD double pow_exp_dd(dd Z) {
 // pow(x, y) with C99 special cases for the arguments the integrator can see.
 // The general path runs on a substitute argument for the special cases and
 // their results are selected at the end (no divergent early returns).
-DE_HD double pow_dd(double x, double y) {
+DE_HD DE_FORCE_INLINE double pow_dd(double x, double y) {
   DE_NO_CONTRACT
   const double inf = as_f64(0x7ff0000000000000ULL);
   const bool special = !(x > 0.0 && x < inf) || x == 1.0 || y == 0.0 || y != y;
@@ -254,15 +256,13 @@ DE_HD bool pow_m02_fast(double x, double* out) {
   return normal && fma(d, 0x1.001p+0, res) == res;
 }
 
-// Kept out of line on the device so that the rare fallback is a call in a
-// branch, not code scheduled into the step loop.
-DE_COLD double pow_dd_cold(double x, double y) { return pow_dd(x, y); }
-
+// The double-double path is forced inline (above): as a call inside the step
+// loop it would save and restore live registers through scratch memory.
 DE_HD double pow(double x, double y) {
   if (y == -0.2) {
     double r;
     if (__builtin_expect(pow_m02_fast(x, &r), 1)) return r;
-    return pow_dd_cold(x, y);
+    return pow_dd(x, y);
   }
   return pow_dd(x, y);
 }

@@ -46,9 +46,14 @@ __global__ __launch_bounds__(kBlock) void ode_plain_kernel(OdeArgs a) {
 
 // Three waves per SIMD: the DOPRI5 lane needs about 165 VGPRs, and the
 // occupancy hint keeps the allocation under the 168-register limit for 3.
+// The arguments live in device memory and are read through a restrict
+// pointer: the fields used only when a trajectory starts or ends are then
+// loaded inside those branches instead of occupying SGPRs for the whole loop,
+// which leaves SGPRs for the step's constants.
 template <class Lane>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void
-ode_persistent_kernel(OdeArgs a) {
+ode_persistent_kernel(const OdeArgs* __restrict__ ap) {
+  const OdeArgs& a = *ap;
   Lane L;
   bool have = false;
   bool drained = false;
@@ -90,6 +95,8 @@ ode_persistent_kernel(OdeArgs a) {
 }
 
 __global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ull; }
+
+__global__ void store_args_kernel(OdeArgs a, OdeArgs* dst) { *dst = a; }
 
 // ---- Longest-first order for the persistent schedule ----------------------
 // A counting sort of trajectories by cost_bucket(predicted_cost): the resident
@@ -187,23 +194,31 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
   int64_t want = (int64_t)num_cu * blocks_per_cu;
   int64_t need = (a.n + kBlock - 1) / kBlock;
   int64_t grid = want < need ? want : need;
+  // Stream-ordered workspace: the launch arguments and, for the longest-first
+  // order (schedule PERSISTENT, batches of a few thousand trajectories or
+  // more), 256 bucket counts, the order and one bucket byte per trajectory.
+  const bool ordered = schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL;
+  const size_t args_bytes = (sizeof(OdeArgs) + 255) & ~(size_t)255;
+  const size_t bytes =
+      args_bytes + (ordered ? kBuckets * sizeof(uint32_t) + (size_t)a.n * (sizeof(uint32_t) + 1)
+                            : 0);
+  void* work = nullptr;
+  if (hipMallocAsync(&work, bytes, s) != hipSuccess) return -1;
+  OdeArgs* d_args = (OdeArgs*)work;
   OdeArgs pa = a;
   pa.order = nullptr;
-  void* work = nullptr;
-  // Longest-first order (schedule PERSISTENT) for batches of a few thousand
-  // trajectories or more; index order (PERSISTENT_FIFO, or small batches).
-  if (schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL) {
-    const size_t bytes = kBuckets * sizeof(uint32_t) + (size_t)a.n * (sizeof(uint32_t) + 1);
-    if (hipMallocAsync(&work, bytes, s) != hipSuccess) return -1;
-    uint32_t* hist = (uint32_t*)work;
+  if (ordered) {
+    uint32_t* hist = (uint32_t*)((char*)work + args_bytes);
     uint32_t* order = hist + kBuckets;
     uint8_t* bucket = (uint8_t*)(order + a.n);
     launch_order<Lane>(a, bucket, hist, order, s);
     pa.order = order;
   }
+  hipLaunchKernelGGL(store_args_kernel, dim3(1), dim3(1), 0, s, pa, d_args);
   hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-  hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s, pa);
-  if (work && hipFreeAsync(work, s) != hipSuccess) return -1;
+  hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s,
+                     (const OdeArgs*)d_args);
+  if (hipFreeAsync(work, s) != hipSuccess) return -1;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
