@@ -40,8 +40,12 @@ __global__ __launch_bounds__(kSpringBlock) void spring_lane_kernel(SpringArgs A)
   }
 }
 
+// Occupancy hints: the rarely taken double-double pow fallback raises the
+// peak register count past a wave-slot boundary; the hints trade that for a
+// few spills outside the step block.
 template <int G, int C>
-__global__ __launch_bounds__(kSpringBlock) void spring_group_kernel(SpringArgs A) {
+__global__ __launch_bounds__(kSpringBlock) __attribute__((amdgpu_waves_per_eu(C == 1 ? 3 : 2))) void
+spring_group_kernel(SpringArgs A) {
   const int lane = __lane_id();
   const int l = lane & (G - 1);
   const int leader_lane = lane - l;

@@ -133,8 +133,12 @@ __global__ __launch_bounds__(kBlock) void chunk_lane(Queue* q, const Range* rang
   if (valid) slots[slot] = S;
 }
 
+// Occupancy hints: the rarely taken double-double pow fallback raises the
+// peak register count past a wave-slot boundary; the hints trade that for a
+// few spills outside the step block.
 template <int G, int C>
-__global__ __launch_bounds__(kBlock) void chunk_group(Queue* q, const Range* ranges,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(C == 1 ? 3 : 2))) void
+chunk_group(Queue* q, const Range* ranges,
                                                      ReqDev* reqs,
                                                      LaneSlot<GroupDopri5<G, C>>* slots,
                                                      int nslots, int k_iters) {
