@@ -67,8 +67,11 @@ class ExplicitRungeKutta {
   static ExplicitRungeKutta dopri5() { return ExplicitRungeKutta(DE_METHOD_DOPRI5, 0.0); }
   static ExplicitRungeKutta rk4(double h0) { return ExplicitRungeKutta(DE_METHOD_RK4, h0); }
   static ExplicitRungeKutta euler(double h0) { return ExplicitRungeKutta(DE_METHOD_EULER, h0); }
-  ExplicitRungeKutta rtol(double v) const { auto m = *this; m.b_.rtol = v; return m; }
-  ExplicitRungeKutta atol(double v) const { auto m = *this; m.b_.atol = v; return m; }
+  ExplicitRungeKutta rtol(double v) const { auto m = *this; m.b_.rtol = v; m.rtol_v_.clear(); return m; }
+  ExplicitRungeKutta atol(double v) const { auto m = *this; m.b_.atol = v; m.atol_v_.clear(); return m; }
+  // Tolerance::Vector: one value per state component (src/tolerance.rs:11-68).
+  ExplicitRungeKutta rtol(std::vector<double> v) const { auto m = *this; m.rtol_v_ = v; return m; }
+  ExplicitRungeKutta atol(std::vector<double> v) const { auto m = *this; m.atol_v_ = v; return m; }
   ExplicitRungeKutta h0(double v) const { auto m = *this; m.b_.h0 = v; return m; }
   ExplicitRungeKutta h_min(double v) const { auto m = *this; m.b_.h_min = v; return m; }
   ExplicitRungeKutta h_max(double v) const { auto m = *this; m.b_.h_max = v; return m; }
@@ -77,6 +80,8 @@ class ExplicitRungeKutta {
   ExplicitRungeKutta min_scale(double v) const { auto m = *this; m.b_.min_scale = v; return m; }
   ExplicitRungeKutta max_scale(double v) const { auto m = *this; m.b_.max_scale = v; return m; }
   const de_ode_batch& settings() const { return b_; }
+  const std::vector<double>& rtol_vector() const { return rtol_v_; }
+  const std::vector<double>& atol_vector() const { return atol_v_; }
 
  private:
   ExplicitRungeKutta(int32_t method, double h0) {
@@ -85,6 +90,7 @@ class ExplicitRungeKutta {
     b_.h0 = h0;
   }
   de_ode_batch b_;
+  std::vector<double> rtol_v_, atol_v_;
 };
 
 // ------------------------------------------------------------ systems ----
@@ -216,6 +222,19 @@ class IVP {
     std::vector<double> y0((size_t)System::dim * n);
     for (int64_t i = 0; i < n; ++i)
       for (int c = 0; c < System::dim; ++c) y0[(size_t)c * n + i] = y0_[i][c];
+    // Tolerance vectors become per-component arrays in the layout of y0.
+    auto expand = [&](const std::vector<double>& v, std::vector<double>& out) -> const double* {
+      if (v.empty()) return nullptr;
+      if ((int)v.size() != System::dim)
+        throw std::runtime_error("tolerance vector length must equal the system dimension");
+      out.resize((size_t)System::dim * n);
+      for (int c = 0; c < System::dim; ++c)
+        for (int64_t i = 0; i < n; ++i) out[(size_t)c * n + i] = v[c];
+      return out.data();
+    };
+    std::vector<double> rtol_comp, atol_comp;
+    b.rtol_comp = expand(m_.rtol_vector(), rtol_comp);
+    b.atol_comp = expand(m_.atol_vector(), atol_comp);
     std::vector<double> te = t_eval_;
     if (!te.empty()) {
       std::sort(te.begin(), te.end());

@@ -144,6 +144,8 @@ de::OdeArgs make_args(const de_ode_batch* b, const de_ode_result* o) {
   a.tfs = b->tf_scalar;
   a.rtol_traj = b->rtol_traj;
   a.atol_traj = b->atol_traj;
+  a.rtol_comp = b->rtol_comp;
+  a.atol_comp = b->atol_comp;
   a.rtol = b->rtol;
   a.atol = b->atol;
   a.h0 = b->h0;
@@ -270,6 +272,8 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   db.tf = st.in(b->tf, b->tf ? n : 0);
   db.rtol_traj = st.in(b->rtol_traj, b->rtol_traj ? n : 0);
   db.atol_traj = st.in(b->atol_traj, b->atol_traj ? n : 0);
+  db.rtol_comp = st.in(b->rtol_comp, b->rtol_comp ? (size_t)state_len : 0);
+  db.atol_comp = st.in(b->atol_comp, b->atol_comp ? (size_t)state_len : 0);
   db.t_eval = st.in(b->t_eval, b->n_eval);
   de_ode_result dr;
   dr.y_final = st.out(o->y_final, state_len);

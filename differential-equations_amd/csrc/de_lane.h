@@ -55,6 +55,35 @@ struct OdeArgs {
   int32_t* n_dense;
   unsigned long long* counter;  // persistent schedule: next trajectory index
   const uint32_t* order;        // persistent schedule: trajectory at each position, or null
+  const double* rtol_comp;      // per-component tolerances [c*n + i] (or state layout), or null
+  const double* atol_comp;
+};
+
+// Tolerances of one trajectory (Tolerance, src/tolerance.rs:11-68): a scalar
+// pair, or one pair per component when V (per-component arrays were given).
+template <int D, bool V>
+struct LaneTol {
+  double r, a;
+  DE_HD void load(const OdeArgs& A, int64_t i) {
+    r = A.rtol_traj ? A.rtol_traj[i] : A.rtol;
+    a = A.atol_traj ? A.atol_traj[i] : A.atol;
+  }
+  DE_HD double rt(int) const { return r; }
+  DE_HD double at(int) const { return a; }
+};
+template <int D>
+struct LaneTol<D, true> {
+  double r[D], a[D];
+  DE_HD void load(const OdeArgs& A, int64_t i) {
+    const double rs = A.rtol_traj ? A.rtol_traj[i] : A.rtol;
+    const double as = A.atol_traj ? A.atol_traj[i] : A.atol;
+    for (int c = 0; c < D; ++c) {
+      r[c] = A.rtol_comp ? A.rtol_comp[c * A.n + i] : rs;
+      a[c] = A.atol_comp ? A.atol_comp[c * A.n + i] : as;
+    }
+  }
+  DE_HD double rt(int c) const { return r[c]; }
+  DE_HD double at(int c) const { return a[c]; }
 };
 
 // Bucket of a predicted cost c (attempts) for the longest-first order of the
@@ -116,16 +145,16 @@ DE_HD void teval_emit(const OdeArgs& a, int64_t i, double dir, int32_t& next, in
 }
 
 // InitialStepSize::<Ordinary>::compute (src/methods/h_init.rs:45-135).
-template <class F>
-DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, double rtol,
-                         double atol, double h_min, double h_max) {
+template <class F, class Tol>
+DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, const Tol& tol,
+                         double h_min, double h_max) {
   constexpr int D = F::kDim;
   double posneg = signum(tf - t0);
   double f0[D], f1[D], sk[D], y1[D];
   f.diff(t0, y0, f0);
   double dnf = 0.0, dny = 0.0;
   for (int c = 0; c < D; ++c) {
-    sk[c] = atol + rtol * fabs(y0[c]);
+    sk[c] = tol.at(c) + tol.rt(c) * fabs(y0[c]);
     double u = f0[c] / sk[c];
     dnf += u * u;
     double v = y0[c] / sk[c];
@@ -150,16 +179,18 @@ DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, dou
 }
 
 // ------------------------------------------------------------------ DOPRI5 --
-// DENSE selects the t_eval output path at compile time.  The attempt counter
+// DENSE selects the t_eval output path and VTOL per-component tolerances at
+// compile time.  The attempt counter
 // of the reference (`steps`, which counts attempts) is acc + rej at every
 // check, and evals = base + 6 rej + 7 acc on every exit (base = 3 with the
 // automatic initial step, 1 without), so neither is carried in the loop.
-template <class F, bool DENSE = false>
+template <class F, bool DENSE = false, bool VTOL = false>
 struct Dopri5Lane {
   static constexpr int D = F::kDim;
   F f;
   double y[D], k0[D];
-  double t, h, h_prev, tf, dir, rtol, atol;
+  double t, h, h_prev, tf, dir;
+  LaneTol<D, VTOL> tol;
   int32_t acc, rej, base_ev, stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
 
@@ -200,8 +231,9 @@ struct Dopri5Lane {
     if (h0 == 0.0 && tf != t0) {
       double y[D];
       for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
-      h0 = lane_h_init(f, t0, tf, y, a.rtol_traj ? a.rtol_traj[i] : a.rtol,
-                       a.atol_traj ? a.atol_traj[i] : a.atol, a.h_min, a.h_max);
+      LaneTol<D, VTOL> tl;
+      tl.load(a, i);
+      h0 = lane_h_init(f, t0, tf, y, tl, a.h_min, a.h_max);
     }
     return fabs(tf - t0) / fabs(h0);
   }
@@ -212,8 +244,7 @@ struct Dopri5Lane {
     f.load(a.params, a.n, i);
     double t0 = a.t0 ? a.t0[i] : a.t0s;
     tf = a.tf ? a.tf[i] : a.tfs;
-    rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
-    atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    tol.load(a, i);
     for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
     t = t0;
     acc = rej = 0;
@@ -225,7 +256,7 @@ struct Dopri5Lane {
     double h0 = a.h0;
     base_ev = 1;
     if (ok && h0 == 0.0) {
-      h0 = lane_h_init(f, t0, tf, y, rtol, atol, a.h_min, a.h_max);
+      h0 = lane_h_init(f, t0, tf, y, tol, a.h_min, a.h_max);
       base_ev = 3;
     }
     ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
@@ -304,7 +335,7 @@ struct Dopri5Lane {
       double es = ((((dp::e0 * k0[c] + dp::e2 * k2[c]) + dp::e3 * k3[c]) + dp::e4 * k4[c]) +
                    dp::e5 * k5[c]) +
                   dp::e6 * k6[c];
-      double sk = atol + rtol * fmax(fabs(y[c]), fabs(ynew[c]));
+      double sk = tol.at(c) + tol.rt(c) * fmax(fabs(y[c]), fabs(ynew[c]));
       double e = es / sk;
       err = c == 0 ? e * e : err + e * e;
     }

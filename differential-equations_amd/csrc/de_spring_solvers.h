@@ -39,7 +39,11 @@ struct VecDopri5 {
   double* y;   // y_final slot doubles as the working state
   double *k[7], *ys, *ynew;
   double t, h, h_prev, tf, dir, rtol, atol;
+  const double *rtv, *atv;  // per-component tolerances of this system, or null
   int64_t acc, rej, ev, steps;
+
+  DE_HD double rt(int c) const { return rtv ? rtv[c] : rtol; }
+  DE_HD double at(int c) const { return atv ? atv[c] : atol; }
   int32_t stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
 
@@ -109,6 +113,8 @@ struct VecDopri5 {
     tf = a.tf ? a.tf[i] : a.tfs;
     rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
     atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    rtv = a.rtol_comp ? a.rtol_comp + so : nullptr;
+    atv = a.atol_comp ? a.atol_comp + so : nullptr;
     t = t0;
     acc = rej = ev = steps = 0;
     pushed = next_eval = 0;
@@ -124,7 +130,7 @@ struct VecDopri5 {
       diff(t0, y, f0);
       double dnf = 0.0, dny = 0.0;
       for (int c = 0; c < n; ++c) {
-        sk[c] = atol + rtol * fabs(y[c]);
+        sk[c] = at(c) + rt(c) * fabs(y[c]);
         double u = f0[c] / sk[c];
         dnf += u * u;
         double w2 = y[c] / sk[c];
@@ -228,7 +234,7 @@ struct VecDopri5 {
                     dp::e4 * k[4][c]) +
                    dp::e5 * k[5][c]) +
                   dp::e6 * k[6][c];
-      double sk = atol + rtol * fmax(fabs(y[c]), fabs(yn));
+      double sk = at(c) + rt(c) * fmax(fabs(y[c]), fabs(yn));
       double e = es / sk;
       err = c == 0 ? e * e : err + e * e;
       k[1][c] = sg;
@@ -323,6 +329,11 @@ struct GroupDopri5 {
   double t, h, h_prev, tf, dir, rtol, atol;
   int32_t acc, rej, ev, steps, stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
+
+  // Tolerances of state component c (x_j: c = j, v_j: c = nm + j); the
+  // per-component arrays are read where used (their pointer is uniform).
+  DE_DEV double rt(const OdeArgs& a, int c) const { return a.rtol_comp ? a.rtol_comp[so + c] : rtol; }
+  DE_DEV double at(const OdeArgs& a, int c) const { return a.atol_comp ? a.atol_comp[so + c] : atol; }
 
   DE_DEV double gsum(double s) const {
 #pragma unroll
@@ -439,8 +450,9 @@ struct GroupDopri5 {
 #pragma unroll
       for (int q = 0; q < C; ++q) {
         if (l * C + q < nm) {
-          skx[q] = atol + rtol * fabs(x[q]);
-          skv[q] = atol + rtol * fabs(v[q]);
+          const int j = l * C + q;
+          skx[q] = at(a, j) + rt(a, j) * fabs(x[q]);
+          skv[q] = at(a, nm + j) + rt(a, nm + j) * fabs(v[q]);
           double u = f0x[q] / skx[q], u2 = f0v[q] / skv[q];
           dnf += u * u + u2 * u2;
           double w = x[q] / skx[q], w2 = v[q] / skv[q];
@@ -587,8 +599,9 @@ struct GroupDopri5 {
                        dp::e4 * k4v[q]) +
                       dp::e5 * k5v[q]) +
                      dp::e6 * k6v[q];
-        double e1 = ex / (atol + rtol * fmax(fabs(x[q]), fabs(nx[q])));
-        double e2 = evv / (atol + rtol * fmax(fabs(v[q]), fabs(nv[q])));
+        const int j = l * C + q;
+        double e1 = ex / (at(a, j) + rt(a, j) * fmax(fabs(x[q]), fabs(nx[q])));
+        double e2 = evv / (at(a, nm + j) + rt(a, nm + j) * fmax(fabs(v[q]), fabs(nv[q])));
         err += e1 * e1 + e2 * e2;
       }
     }

@@ -225,9 +225,14 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
 template <class F>
 static int launch_method(int method, const OdeArgs& a, int schedule, hipStream_t s) {
   switch (method) {
-    case kMethodDopri5:
-      return a.n_eval > 0 ? launch<Dopri5Lane<F, true>>(a, schedule, s)
-                          : launch<Dopri5Lane<F, false>>(a, schedule, s);
+    case kMethodDopri5: {
+      const bool vtol = a.rtol_comp || a.atol_comp;
+      if (a.n_eval > 0)
+        return vtol ? launch<Dopri5Lane<F, true, true>>(a, schedule, s)
+                    : launch<Dopri5Lane<F, true, false>>(a, schedule, s);
+      return vtol ? launch<Dopri5Lane<F, false, true>>(a, schedule, s)
+                  : launch<Dopri5Lane<F, false, false>>(a, schedule, s);
+    }
 #if !defined(DE_FAST_BUILD)
     case kMethodRk4: return launch<FixedLane<F, 4>>(a, schedule, s);
     case kMethodEuler: return launch<FixedLane<F, 1>>(a, schedule, s);

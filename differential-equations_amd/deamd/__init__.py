@@ -50,9 +50,12 @@ class ExplicitRungeKutta:
         return self
 
     def rtol(self, v):
+        """Tolerance::Scalar (a number) or Tolerance::Vector (one value per state
+        component, src/tolerance.rs:11-68)."""
         return self._set("rtol", v)
 
     def atol(self, v):
+        """As rtol."""
         return self._set("atol", v)
 
     def h0(self, v):
@@ -220,6 +223,9 @@ class OdeEnsemble:
         self.flags = 0
         self.rtol_traj = None
         self.atol_traj = None
+        # Per-component tolerances of every trajectory, in the layout of y0.
+        self.rtol_comp = None
+        self.atol_comp = None
 
     def method(self, m):
         self._method = m
@@ -278,8 +284,27 @@ class OdeEnsemble:
                 keep.append(v)
                 setattr(b, name, _ptr(v))
                 setattr(b, name + "_scalar", float(v[0]))
+        comp = {}
         for k, v in self._method.settings.items():
+            if k in ("rtol", "atol") and np.ndim(v) > 0:
+                vec = np.asarray(v, dtype=np.float64).reshape(-1)
+                if isinstance(s, SpringChain) or len(vec) != s.dim:
+                    raise ValueError("a %s vector needs one value per component of a fixed-size "
+                                     "system; give spring chains %s_comp in the state layout"
+                                     % (k, k))
+                comp[k + "_comp"] = np.repeat(vec[:, None], self.n, axis=1).reshape(-1)
+                continue
             setattr(b, k, v)
+        for name in ("rtol_comp", "atol_comp"):
+            v = getattr(self, name)
+            v = comp.get(name) if v is None else v
+            if v is not None:
+                v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+                if len(v) != state_len:
+                    raise ValueError("%s needs %d values (one per state component)"
+                                     % (name, state_len))
+                keep.append(v)
+                setattr(b, name, _ptr(v))
         for name in ("rtol_traj", "atol_traj"):
             v = getattr(self, name)
             if v is not None:

@@ -79,6 +79,7 @@ class OdeBatch(C.Structure):
         ("t_eval", C.c_void_p),
         ("schedule", C.c_int32), ("mapping", C.c_int32), ("group_size", C.c_int32),
         ("flags", C.c_uint32),
+        ("rtol_comp", C.c_void_p), ("atol_comp", C.c_void_p),
     ]
 
 

@@ -129,6 +129,13 @@ typedef struct de_ode_batch {
   int32_t mapping;    /* DE_MAP_* */
   int32_t group_size; /* lanes per system for DE_MAP_GROUP (0: autotune) */
   uint32_t flags;     /* DE_FLAG_* */
+  /* Per-component tolerances, Tolerance::Vector (src/tolerance.rs:11-68): one value
+   * per state component of every trajectory, in the layout of y0 ([c*n + i], or at
+   * state_offset[i] for spring chains), or NULL.  They take precedence over the
+   * per-trajectory and scalar values; the norm uses atol[c] + rtol[c]*max(|y|,|y_new|)
+   * (src/traits.rs:394-411) and h_init atol[c] + rtol[c]*|y0| (src/methods/h_init.rs:80). */
+  const double* rtol_comp;
+  const double* atol_comp;
 } de_ode_batch;
 
 typedef struct de_ode_result {

@@ -75,6 +75,10 @@ struct SpringSys {
 struct Settings {
   double rtol, atol, h0, h_min, h_max, safety, min_scale, max_scale;
   int64_t max_steps;
+  // Tolerance indexed per component (src/tolerance.rs:33-51): this trajectory's
+  // rtol[c], atol[c], the scalar replicated when no vector was given.
+  const double* rtolv = nullptr;
+  const double* atolv = nullptr;
 };
 
 struct TrajResult {
@@ -152,10 +156,10 @@ static inline void zero(double* d, int n) {
 
 // error_norm (src/traits.rs:394-411): sum_i (err_i / sk_i)^2 in index order.
 static inline double error_norm(const double* y, const double* y_new, const double* err,
-                                double atol, double rtol, int n) {
+                                const double* atol, const double* rtol, int n) {
   double sum = 0.0;
   for (int i = 0; i < n; ++i) {
-    double sk = atol + rtol * fmax(fabs(y[i]), fabs(y_new[i]));
+    double sk = atol[i] + rtol[i] * fmax(fabs(y[i]), fabs(y_new[i]));
     double e = err[i] / sk;
     sum += e * e;
   }
@@ -176,7 +180,8 @@ static inline double diff_norm_squared(const double* a, const double* b, int n) 
 // InitialStepSize::<Ordinary>::compute, src/methods/h_init.rs:45-135.
 template <class Sys>
 static double h_init(const Sys& sys, double t0, double tf, const double* y0, int order,
-                     double rtol, double atol, double h_min, double h_max, int64_t* evals) {
+                     const double* rtol, const double* atol, double h_min, double h_max,
+                     int64_t* evals) {
   constexpr int M = Sys::kMax;
   const int n = sys.dim();
   double posneg = de::signum(tf - t0);
@@ -187,7 +192,7 @@ static double h_init(const Sys& sys, double t0, double tf, const double* y0, int
   *evals += 1;
   double dnf = 0.0, dny = 0.0;
   for (int i = 0; i < n; ++i) {
-    sk[i] = atol + rtol * fabs(y0[i]);
+    sk[i] = atol[i] + rtol[i] * fabs(y0[i]);
     double a = f0[i] / sk[i];
     dnf += a * a;
     double b = y0[i] / sk[i];
@@ -282,7 +287,7 @@ struct Dopri5 {
   // ordinary.rs:16-61; returns false on BadInput.
   bool init(double t0, double tf, const double* y0, int64_t* evals) {
     if (h0 == 0.0) {
-      h0 = h_init(sys, t0, tf, y0, 5, st.rtol, st.atol, st.h_min, st.h_max, evals);
+      h0 = h_init(sys, t0, tf, y0, 5, st.rtolv, st.atolv, st.h_min, st.h_max, evals);
     }
     if (!de::validate_step_size(h0, st.h_min, st.h_max, t0, tf)) return false;
     h = h0;
@@ -330,7 +335,7 @@ struct Dopri5 {
     double err_state[M];
     zero(err_state, n);
     for (int j = 0; j < 7; ++j) add_scaled(err_state, kDP.er[j], k[j], n);
-    double err = error_norm(y, y_new, err_state, st.atol, st.rtol, n);
+    double err = error_norm(y, y_new, err_state, st.atolv, st.rtolv, n);
     double deno = err + 0.01 * err2;
     if (deno <= 0.0) deno = 1.0;
     err = fabs(h) * err * sqrt(1.0 / (deno * (double)n));
@@ -867,6 +872,13 @@ static void run_fixed_family(const de_ode_batch* b, de_ode_result* out, const Se
     Settings st = base;
     if (b->rtol_traj) st.rtol = b->rtol_traj[i];
     if (b->atol_traj) st.atol = b->atol_traj[i];
+    double rt[D], at[D];
+    for (int c = 0; c < D; ++c) {
+      rt[c] = b->rtol_comp ? b->rtol_comp[c * n + i] : st.rtol;
+      at[c] = b->atol_comp ? b->atol_comp[c * n + i] : st.atol;
+    }
+    st.rtolv = rt;
+    st.atolv = at;
     double y0[D], yo[D];
     for (int c = 0; c < D; ++c) y0[c] = b->y0[c * n + i];
     std::vector<double> dense;
@@ -897,6 +909,13 @@ static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings
     if (b->rtol_traj) st.rtol = b->rtol_traj[i];
     if (b->atol_traj) st.atol = b->atol_traj[i];
     const int64_t off = b->state_offset[i];
+    std::vector<double> rt(sys.dim()), at(sys.dim());
+    for (int c = 0; c < sys.dim(); ++c) {
+      rt[c] = b->rtol_comp ? b->rtol_comp[off + c] : st.rtol;
+      at[c] = b->atol_comp ? b->atol_comp[off + c] : st.atol;
+    }
+    st.rtolv = rt.data();
+    st.atolv = at.data();
     double* dense = nullptr;
     if (b->n_eval > 0 && out->dense) {
       dense = out->dense + (int64_t)b->n_eval * off;

@@ -69,6 +69,12 @@ def add_scaled(v, alpha, w):
         v[i] += alpha * w[i]
 
 
+def tol(v, i):
+    """Tolerance indexing (src/tolerance.rs:33-51): a scalar for every
+    component, or the component's entry of a vector."""
+    return v[i] if isinstance(v, (list, tuple)) else v
+
+
 def h_init(f, t0, tf, y0, order, rtol, atol, h_min, h_max):
     posneg = signum(tf - t0)
     n = len(y0)
@@ -79,7 +85,7 @@ def h_init(f, t0, tf, y0, order, rtol, atol, h_min, h_max):
     dny = 0.0
     sk = []
     for i in range(n):
-        s = atol + rtol * abs(y0[i])
+        s = tol(atol, i) + tol(rtol, i) * abs(y0[i])
         sk.append(s)
         dnf += (f0[i] / s) * (f0[i] / s)
         dny += (y0[i] / s) * (y0[i] / s)
@@ -157,7 +163,7 @@ def solve(f, t0, tf, y0, rtol, atol, max_steps=100000, powf=math.pow):
             add_scaled(err_state, ER[j], k[j])
         err = 0.0
         for i in range(n):
-            sk = atol + rtol * fmax(abs(y[i]), abs(y_new[i]))
+            sk = tol(atol, i) + tol(rtol, i) * fmax(abs(y[i]), abs(y_new[i]))
             e = err_state[i] / sk
             err += e * e
         deno = err + 0.01 * 0.0

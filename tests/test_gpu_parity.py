@@ -241,6 +241,24 @@ def test_vdp_backward_and_per_trajectory_tolerances(orc):
     assert_ode_equal(got, want)
 
 
+@pytest.mark.parametrize("schedule", [DE_SCHED_PLAIN, DE_SCHED_PERSISTENT])
+def test_per_component_tolerances_bit_exact(orc, schedule):
+    # Tolerance::Vector (src/tolerance.rs): per component of every trajectory,
+    # and the builder's one vector for all trajectories, with dense output.
+    n = 8192
+    p, y0 = deamd.gen_lorenz(n, seed=29)
+    rng = np.random.default_rng(29)
+    ens = IVP.ode(Lorenz(*p), 0.0, 3.0, y0).method(
+        ExplicitRungeKutta.dopri5().max_steps(1000000)).options(schedule=schedule)
+    ens.rtol_comp = 10.0 ** rng.uniform(-9, -5, 3 * n)
+    ens.atol_comp = ens.rtol_comp * 10.0 ** rng.uniform(-3, 0, 3 * n)
+    assert_ode_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS))
+    ens = IVP.ode(Lorenz(*p[:, :512]), 0.0, 3.0, y0[:, :512]).t_eval(np.linspace(0, 3, 31)) \
+        .method(ExplicitRungeKutta.dopri5().rtol([1e-8, 1e-6, 1e-9]).atol([1e-10, 1e-8, 1e-12])
+                .max_steps(1000000)).options(schedule=schedule)
+    assert_ode_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS), dense=True)
+
+
 def test_error_statuses_bit_exact(orc):
     p, y0 = deamd.gen_lorenz(256, seed=13)
     p[1, :8] = np.nan                      # NaN rho: rejected forever -> StepSize

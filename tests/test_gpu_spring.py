@@ -100,3 +100,26 @@ def test_mixed_sizes_results_do_not_depend_on_batch_composition(case):
         sa, sb = sub.state_offset[k], sub.state_offset[k + 1]
         assert np.array_equal(part.y[sa:sb], full.y[a:b])
         assert part.accepted[k] == full.accepted[i]
+
+
+def test_per_component_tolerances(orc):
+    # Tolerance::Vector on ragged systems: one (rtol, atol) per state component
+    # in the state layout; velocities held 100x tighter than displacements.
+    sysm, ens = chains(512, seed=23)
+    rt = np.empty(len(ens.y0))
+    at = np.empty(len(ens.y0))
+    for i, nm in enumerate(sysm.sizes):
+        a = sysm.state_offset[i]
+        rt[a:a + nm], rt[a + nm:a + 2 * nm] = 1e-6, 1e-8
+        at[a:a + nm], at[a + nm:a + 2 * nm] = 1e-9, 1e-11
+    ens.rtol_comp, ens.atol_comp = rt, at
+    want = orc.solve_ode(ens, threads=THREADS)
+    scalar = orc.solve_ode(chains(512, seed=23)[1], threads=THREADS)
+    assert (want.accepted > scalar.accepted).mean() > 0.5   # the tighter components bind
+    got = ens.options(mapping=DE_MAP_LANE).solve()
+    for f in ("status", "accepted", "rejected", "evals", "t", "y", "n_dense"):
+        assert np.array_equal(getattr(got, f), getattr(want, f)), f
+    got = ens.options(mapping=DE_MAP_GROUP, group_size=16).solve()
+    assert rel_state_error(sysm, got, want).max() <= 1e-10
+    same_counts = (got.accepted == want.accepted) & (got.rejected == want.rejected)
+    assert same_counts.mean() >= 0.999

@@ -199,6 +199,45 @@ def test_oracle_bit_equals_python_restatement(orc, traj):
     assert (got.accepted[0], got.rejected[0], got.evals[0], got.status[0]) == want[2:]
 
 
+@pytest.mark.parametrize("traj", range(3))
+def test_oracle_per_component_tolerance_bit_equals_python_restatement(orc, traj):
+    # Tolerance::Vector: atol[i] + rtol[i]*max(|y_i|, |y_new_i|) in the norm and
+    # atol[i] + rtol[i]*|y0_i| in h_init (src/traits.rs:400, src/methods/h_init.rs:80).
+    rng = np.random.default_rng(200 + traj)
+    rho = float(rng.uniform(0, 50))
+    y0 = list(1.0 + 0.1 * rng.standard_normal(3))
+    rtol = [1e-8, 1e-5, 1e-9]
+    atol = [1e-10, 1e-7, 1e-12]
+
+    def f(t, y, dy):
+        x, yy, z = y
+        dy[0] = 10.0 * (yy - x)
+        dy[1] = x * (rho - z) - yy
+        dy[2] = x * yy - (8.0 / 3.0) * z
+
+    want = pyref_dopri5.solve(f, 0.0, 1.5, y0, rtol, atol)
+    ens = IVP.ode(Lorenz(10.0, rho, 8.0 / 3.0), 0.0, 1.5, np.array(y0).reshape(3, 1)) \
+        .method(ExplicitRungeKutta.dopri5().rtol(rtol).atol(atol).max_steps(100000))
+    got = orc.solve_ode(ens, pow_mode=1)
+    assert list(got.y) == want[0]
+    assert got.t[0] == want[1]
+    assert (got.accepted[0], got.rejected[0], got.evals[0], got.status[0]) == want[2:]
+    # A tighter component must cost steps: the scalar loosest setting takes fewer.
+    loose = pyref_dopri5.solve(f, 0.0, 1.5, y0, 1e-5, 1e-7)
+    assert loose[2] < want[2]
+
+
+def test_oracle_vector_tolerance_of_equal_entries_is_the_scalar(orc):
+    import deamd
+    p, y0 = deamd.gen_lorenz(64, seed=3)
+    mk = lambda r, a: IVP.ode(Lorenz(*p), 0.0, 5.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(r).atol(a).max_steps(100000))
+    a = orc.solve_ode(mk(1e-7, 1e-9))
+    b = orc.solve_ode(mk([1e-7] * 3, [1e-9] * 3))
+    assert np.array_equal(a.y, b.y) and np.array_equal(a.accepted, b.accepted)
+    assert np.array_equal(a.rejected, b.rejected) and np.array_equal(a.t, b.t)
+
+
 def test_sde_last_step_is_clipped(orc):
     # [0, 1] at dt = 1e-3: 1000 steps, the last one h = 1 - t_999.
     t = 0.0
