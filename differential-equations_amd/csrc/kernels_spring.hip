@@ -296,15 +296,18 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     }
     choice[k] = ch;
   }
-  // Launch classes concurrently, largest first, with blocks shared by work.
-  int64_t total_mass = 0;
-  for (int k = 0; k < kNumCaps; ++k) total_mass += masses[k];
+  // Launch classes concurrently, largest first.  Every class kernel gets a
+  // full-device grid (capped by its number of systems): the kernels are
+  // persistent and exit when their class is drained, so blocks of the classes
+  // launched later are dispatched as those of earlier ones retire, and the
+  // dispatcher balances the classes.  (A fixed share per class left the GPU
+  // idle whenever classes waited behind others for one of the few hardware
+  // queues a process gets.)
   if (hipEventRecord(g_fork, s) != hipSuccess) return -1;
   for (int k = kNumCaps - 1; k >= 0; --k) {
     if (lists[k].empty()) continue;
     const Choice& ch = choice[k];
-    int64_t full = (int64_t)g_num_cu * blocks_per_cu(kConfigs[ch.cfg].k, ch.block);
-    int64_t share = (int64_t)((double)full * (double)masses[k] / (double)total_mass) + 1;
+    int64_t share = (int64_t)g_num_cu * blocks_per_cu(kConfigs[ch.cfg].k, ch.block);
     if (hipStreamWaitEvent(g_streams[k], g_fork, 0) != hipSuccess) return -1;
     if (launch_class(base, d_list + start[k], (int64_t)lists[k].size(), share, ch,
                      g_counters + k, g_streams[k]) != 0)
