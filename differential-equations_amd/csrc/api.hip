@@ -192,10 +192,8 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
     int rc = counter_slot(&a.counter);
     if (rc != DE_OK) return rc;
   }
-  if (a.n_eval > 0) {
-    int rc = fill_nan(a.dense, (int64_t)a.n_eval * b->dim * b->n, s);
-    if (rc != DE_OK) return rc;
-  }
+  // Dense output needs no pre-fill: each trajectory writes every one of its
+  // n_eval rows, with NaN for the points it does not produce.
   int r;
   if (b->flags & DE_FLAG_FAST_MATH)
     r = de::fast::launch_ode_lane(b->family, b->method, a, sched, s);

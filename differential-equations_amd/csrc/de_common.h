@@ -42,6 +42,9 @@ namespace de {
 // (src/ode/solve_ivp.rs:198,263; src/sde/solve_ivp.rs:217,273).
 static constexpr double kTenEps = 2.220446049250313080847e-16 * 10.0;
 
+// Quiet NaN: dense-output points that a solve does not produce.
+static constexpr uint64_t kNaNBits = 0x7ff8000000000000ULL;
+
 DE_HD uint64_t as_u64(double x) {
   union { double d; uint64_t u; } v;
   v.d = x;

@@ -117,7 +117,9 @@ constexpr double e0 = 71.0 / 57600.0, e2 = -71.0 / 16695.0, e3 = 71.0 / 1920.0,
 }  // namespace dp
 
 // TEvalSolout over [t_prev, t_curr]; interp(te, out[D]) evaluates the step's
-// interpolant.  Writes dense[(e * D + c) * n + i].
+// interpolant.  Writes dense[(e * D + c) * n + i]; a point the solout passes
+// over without producing it (the reference pushes nothing) gets NaN, as do
+// the points after the last one reached (teval_tail_nan at the end).
 template <int D, class Interp>
 DE_HD void teval_emit(const OdeArgs& a, int64_t i, double dir, int32_t& next, int32_t& pushed,
                       double t_curr, double t_prev, const double* y_curr, Interp&& interp) {
@@ -138,10 +140,17 @@ DE_HD void teval_emit(const OdeArgs& a, int64_t i, double dir, int32_t& next, in
       idx += 1;
     } else {
       if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+      for (int c = 0; c < D; ++c) a.dense[((int64_t)idx * D + c) * a.n + i] = as_f64(kNaNBits);
       idx += 1;
     }
   }
   next = idx;
+}
+
+template <int D>
+DE_HD void teval_tail_nan(const OdeArgs& a, int64_t i, int32_t from) {
+  for (int32_t e = from; e < a.n_eval; ++e)
+    for (int c = 0; c < D; ++c) a.dense[((int64_t)e * D + c) * a.n + i] = as_f64(kNaNBits);
 }
 
 // InitialStepSize::<Ordinary>::compute (src/methods/h_init.rs:45-135).
@@ -206,6 +215,7 @@ struct Dopri5Lane {
         st == kBadInput ? 0 : (int64_t)L.base_ev + 6 * (int64_t)L.rej + 7 * (int64_t)L.acc;
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
+    if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
   }
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
 
@@ -446,6 +456,7 @@ struct FixedLane {
     a.evals[i] = ev;
     a.status[i] = st;
     if (a.n_dense) a.n_dense[i] = pushed;
+    if (a.n_eval > 0) teval_tail_nan<D>(a, i, next_eval);
   }
 
   DE_HD bool begin(const OdeArgs& a, int64_t i) { return init(a, i); }

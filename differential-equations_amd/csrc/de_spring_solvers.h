@@ -57,6 +57,11 @@ struct VecDopri5 {
     a.evals[i] = ev;
     a.status[i] = st;
     if (a.n_dense) a.n_dense[i] = pushed;
+    if (a.n_eval > 0) {  // points not reached (all of them on BadInput)
+      double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
+      for (int64_t e = (int64_t)next_eval * n; e < (int64_t)a.n_eval * n; ++e)
+        dbase[e] = as_f64(kNaNBits);
+    }
   }
 
   DE_HD void emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
@@ -88,6 +93,8 @@ struct VecDopri5 {
         idx += 1;
       } else {
         if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+        double* out = dbase + (int64_t)idx * n;  // passed over: not produced
+        for (int c = 0; c < n; ++c) out[c] = as_f64(kNaNBits);
         idx += 1;
       }
     }
@@ -381,6 +388,14 @@ struct GroupDopri5 {
       a.status[sys] = st;
       if (a.n_dense) a.n_dense[sys] = pushed;
     }
+    if (a.n_eval > 0) {  // points not reached (all of them on BadInput)
+      double nan_[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) nan_[q] = as_f64(kNaNBits);
+      double* dbase = a.dense + (int64_t)a.n_eval * so;
+      for (int32_t e = next_eval; e < a.n_eval; ++e)
+        write_state(dbase + (int64_t)e * 2 * nm, nan_, nan_);
+    }
   }
 
   template <class Interp>
@@ -406,6 +421,10 @@ struct GroupDopri5 {
         idx += 1;
       } else {
         if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+        double nan_[C];  // passed over: not produced
+#pragma unroll
+        for (int q = 0; q < C; ++q) nan_[q] = as_f64(kNaNBits);
+        write_state(dbase + (int64_t)idx * 2 * nm, nan_, nan_);
         idx += 1;
       }
     }

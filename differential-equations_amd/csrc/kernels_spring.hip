@@ -25,12 +25,6 @@ namespace parity {
 
 constexpr int kSpringBlock = 256;
 
-__global__ void fill_nan_dense(double* p, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = __longlong_as_double(0x7ff8000000000000LL);
-}
-
 __global__ __launch_bounds__(kSpringBlock) void spring_lane_kernel(SpringArgs A) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.o.n) return;
@@ -230,12 +224,6 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     lists[k].push_back((int32_t)i);
     masses[k] += host_sizes[i];
     state_len += 2 * (int64_t)host_sizes[i];
-  }
-  if (a.n_eval > 0 && a.dense) {
-    int64_t len = (int64_t)a.n_eval * state_len;
-    int64_t grid = (len + 255) / 256;
-    if (grid > 65536) grid = 65536;
-    hipLaunchKernelGGL(fill_nan_dense, dim3((unsigned)grid), dim3(256), 0, s, a.dense, len);
   }
   if (b->mapping == DE_MAP_LANE) {
     if (g_work_len < 9 * state_len) {

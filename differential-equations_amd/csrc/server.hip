@@ -86,12 +86,12 @@ DE_DEV int find_range(const Range* r, int64_t n, int64_t ticket) {
 
 template <class F>
 __global__ __launch_bounds__(kBlock) void chunk_lane(Queue* q, const Range* ranges, ReqDev* reqs,
-                                                    LaneSlot<Dopri5Lane<F>>* slots, int nslots,
+                                                    LaneSlot<Dopri5Lane<F, true>>* slots, int nslots,
                                                     int k_iters) {
   const int slot = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = slot < nslots;
   const int lane = __lane_id();
-  LaneSlot<Dopri5Lane<F>> S;
+  LaneSlot<Dopri5Lane<F, true>> S;
   if (valid) S = slots[slot];
   else S.state = -1;
   const int64_t published = *(volatile int64_t*)&q->published;
@@ -225,7 +225,7 @@ template <class F>
 void launch_lane(Family& f, ReqDev* reqs, int k, hipStream_t s) {
   int grid = (f.nslots + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(chunk_lane<F>, dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
-                     (LaneSlot<Dopri5Lane<F>>*)f.slots, f.nslots, k);
+                     (LaneSlot<Dopri5Lane<F, true>>*)f.slots, f.nslots, k);
 }
 
 template <int G, int C>
@@ -410,11 +410,6 @@ bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_l
   a.status = (int32_t*)take(n * 4);
   a.n_dense = b.n_eval > 0 ? (int32_t*)take(n * 4) : nullptr;
   a.dense = b.n_eval > 0 ? (double*)take((size_t)b.n_eval * r->state_len * 8) : nullptr;
-  if (a.dense) {
-    std::vector<double> nanv((size_t)b.n_eval * r->state_len, __builtin_nan(""));
-    (void)hipMemcpyAsync(a.dense, nanv.data(), nanv.size() * 8, hipMemcpyHostToDevice, s->ctl);
-    (void)hipStreamSynchronize(s->ctl);
-  }
   if (b.family == DE_FAMILY_SPRING_CHAIN) {
     rd.a.sizes = (const int32_t*)put(r->sizes.data(), r->sizes.size() * 4);
     rd.a.state_offset = (const int64_t*)put(r->so.data(), r->so.size() * 8);
@@ -630,9 +625,9 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
     f.cap = specs[k].cap;
     f.g = specs[k].g;
     switch (k) {
-      case 0: ok = make_pool<LaneSlot<de::Dopri5Lane<de::LorenzRhs>>>(f, lane_slots);
+      case 0: ok = make_pool<LaneSlot<de::Dopri5Lane<de::LorenzRhs, true>>>(f, lane_slots);
         f.launch = launch_lane<de::LorenzRhs>; break;
-      case 1: ok = make_pool<LaneSlot<de::Dopri5Lane<de::VanDerPolRhs>>>(f, lane_slots);
+      case 1: ok = make_pool<LaneSlot<de::Dopri5Lane<de::VanDerPolRhs, true>>>(f, lane_slots);
         f.launch = launch_lane<de::VanDerPolRhs>; break;
       case 2: ok = make_pool<LaneSlot<de::GroupDopri5<4, 1>>>(f, group_lanes);
         f.launch = launch_group<4, 1>; break;

@@ -223,7 +223,9 @@ def test_reference_systems_bit_exact(orc, case):
     lambda: ExplicitRungeKutta.euler(0.001).max_steps(1000000)])
 def test_dense_output_bit_exact(orc, make_method):
     p, y0 = deamd.gen_lorenz(256, seed=5)
-    pts = np.linspace(0.0, 5.0, 64)
+    # Points before t0 and after tf are passed over (NaN rows), as is t0 when
+    # it is not the first point (the idx == 0 rule of t_eval.rs:106).
+    pts = np.concatenate([[-0.5, -0.1], np.linspace(0.0, 5.0, 64), [5.5]])
     ens = IVP.ode(Lorenz(*p), 0.0, 5.0, y0).t_eval(pts).method(make_method())
     want = orc.solve_ode(ens, threads=THREADS)
     got = ens.solve()

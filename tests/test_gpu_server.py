@@ -50,6 +50,23 @@ def test_server_matches_direct_solve():
     assert_same(got, want)
 
 
+def test_served_dense_output_matches_direct_solve():
+    # t_eval on the lane-mapped families: the served request writes the same
+    # dense rows (NaN where a point is not produced) as a direct solve.
+    pts = np.linspace(-1.0, 6.0, 29)        # the first points precede t0
+    req = lorenz_req(1500, seed=3).t_eval(pts)
+    want = req.solve()
+    with Server(8 * GIB) as srv:
+        got = srv.poll(srv.submit(req))
+    assert_same(got, want)
+    assert np.array_equal(got.n_dense, want.n_dense)
+    assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
+    assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
+    rows = want.dense.reshape(29, 3, -1)
+    assert np.isnan(rows[:4]).all() and np.isnan(rows[-4:]).all()   # before t0, after tf
+    assert (want.n_dense > 0).all()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_result_alone_equals_result_in_mix(graph):
     probes = [lorenz_req(700, seed=11), vdp_req(500, seed=12), spring_req(300, seed=13)]

@@ -123,3 +123,22 @@ def test_per_component_tolerances(orc):
     assert rel_state_error(sysm, got, want).max() <= 1e-10
     same_counts = (got.accepted == want.accepted) & (got.rejected == want.rejected)
     assert same_counts.mean() >= 0.999
+
+
+def test_dense_rows_of_unfinished_systems_are_nan(orc):
+    # max_steps stops the solves part-way: points never reached stay NaN (the
+    # reference pushes nothing for them), in both mappings.
+    sysm, y0 = deamd.gen_spring(256, seed=31)
+    ens = IVP.ode(sysm, 0.0, 10.0, y0).t_eval(PTS).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-6).atol(1e-9).max_steps(60))
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.status == deamd.DE_STATUS_MAX_STEPS).mean() > 0.5
+    assert (want.n_dense < len(PTS)).mean() > 0.5
+    got = ens.options(mapping=DE_MAP_LANE).solve()
+    for f in ("status", "accepted", "rejected", "evals", "t", "y", "n_dense"):
+        assert np.array_equal(getattr(got, f), getattr(want, f)), f
+    assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
+    assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
+    got = ens.options(mapping=DE_MAP_GROUP, group_size=8).solve()
+    assert np.array_equal(got.n_dense, want.n_dense)
+    assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
