@@ -124,10 +124,10 @@ def bench_lorenz(args, dv):
     lib = deamd.lib()
     stream = C.c_void_p(dv.stream.cuda_stream)
 
-    def descriptor(schedule, fast):
+    def descriptor(schedule, fast, method=_abi.DE_METHOD_DOPRI5):
         b = _abi.ode_defaults()
         b.family = _abi.DE_FAMILY_LORENZ
-        b.method = _abi.DE_METHOD_DOPRI5
+        b.method = method
         b.n = n
         b.dim = 3
         b.params = C.c_void_p(d_p.data_ptr())
@@ -148,8 +148,8 @@ def bench_lorenz(args, dv):
         r.status = C.c_void_p(out["st"].data_ptr())
         return b, r
 
-    def run(schedule, fast, steps, warmup):
-        b, r = descriptor(schedule, fast)
+    def run(schedule, fast, steps, warmup, method=_abi.DE_METHOD_DOPRI5):
+        b, r = descriptor(schedule, fast, method)
         for _ in range(warmup):
             _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
         dv.sync()
@@ -177,6 +177,7 @@ def bench_lorenz(args, dv):
     res = run(_abi.DE_SCHED_PERSISTENT, False, args.steps, args.warmup)
     plain = None if args.no_plain else run(_abi.DE_SCHED_PLAIN, False, max(3, args.steps // 2), 1)
     fifo = run(_abi.DE_SCHED_PERSISTENT_FIFO, False, max(3, args.steps // 2), 1)
+    d853 = run(_abi.DE_SCHED_PERSISTENT, False, max(3, args.steps // 2), 1, _abi.DE_METHOD_DOP853)
     fast = run(_abi.DE_SCHED_PERSISTENT, True, max(3, args.steps // 2), 1)
     if args.calibrate_copy:
         calibrate_copy(dv)
@@ -221,6 +222,11 @@ def bench_lorenz(args, dv):
     fi = summarize(fifo)
     line["persistent_fifo"] = {"value": fi["value"], "ms_per_step": fi["ms_per_step"],
                                "note": "same kernel, trajectories in index order"}
+    d8 = summarize(d853)
+    line["dop853"] = {"value": d8["value"], "ms_per_step": d8["ms_per_step"],
+                      "accepted_steps_per_solve": d853["A"], "all_complete": d853["ok"],
+                      "note": "ExplicitRungeKutta::dop853 on the same ensemble and tolerances "
+                              "(parity build)"}
     if plain is not None:
         ps = summarize(plain)
         line["plain"] = {"value": ps["value"], "ms_per_step": ps["ms_per_step"],

@@ -65,6 +65,7 @@ struct Result {  // Result<Solution, Error>
 class ExplicitRungeKutta {
  public:
   static ExplicitRungeKutta dopri5() { return ExplicitRungeKutta(DE_METHOD_DOPRI5, 0.0); }
+  static ExplicitRungeKutta dop853() { return ExplicitRungeKutta(DE_METHOD_DOP853, 0.0); }
   static ExplicitRungeKutta rk4(double h0) { return ExplicitRungeKutta(DE_METHOD_RK4, h0); }
   static ExplicitRungeKutta euler(double h0) { return ExplicitRungeKutta(DE_METHOD_EULER, h0); }
   ExplicitRungeKutta rtol(double v) const { auto m = *this; m.b_.rtol = v; m.rtol_v_.clear(); return m; }

@@ -110,7 +110,7 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->n < 0) return fail(DE_ERR_INVALID, "n < 0");
   if (b->family < 0 || b->family > DE_FAMILY_ROBERTSON)
     return fail(DE_ERR_INVALID, "unknown family %d", b->family);
-  if (b->method < DE_METHOD_DOPRI5 || b->method > DE_METHOD_EULER)
+  if (b->method < DE_METHOD_DOPRI5 || b->method > DE_METHOD_DOP853)
     return fail(DE_ERR_INVALID, "unknown method %d", b->method);
   if (b->family != DE_FAMILY_SPRING_CHAIN && b->dim != kFamilyDim[b->family])
     return fail(DE_ERR_INVALID, "family %d has dimension %d, got dim %d", b->family,
@@ -126,6 +126,9 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->n > 0 && b->family != DE_FAMILY_SPRING_CHAIN && kFamilyParams[b->family] > 0 &&
       !b->params)
     return fail(DE_ERR_INVALID, "family %d needs params", b->family);
+  if (b->family == DE_FAMILY_SPRING_CHAIN && b->method != DE_METHOD_DOPRI5)
+    return fail(DE_ERR_UNSUPPORTED, "spring chains are built for DOPRI5 only (method %d)",
+                b->method);
   if (b->schedule < DE_SCHED_AUTO || b->schedule > DE_SCHED_PERSISTENT_FIFO)
     return fail(DE_ERR_INVALID, "unknown schedule %d", b->schedule);
   if (b->max_steps < 0) return fail(DE_ERR_INVALID, "max_steps < 0");

@@ -22,6 +22,7 @@
 #pragma once
 
 #include "de_common.h"
+#include "de_dop853_table.h"
 #include "de_families.h"
 #include "de_pow.h"
 
@@ -154,7 +155,7 @@ DE_HD void teval_tail_nan(const OdeArgs& a, int64_t i, int32_t from) {
 }
 
 // InitialStepSize::<Ordinary>::compute (src/methods/h_init.rs:45-135).
-template <class F, class Tol>
+template <class F, class Tol, int ORDER = 5>
 DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, const Tol& tol,
                          double h_min, double h_max) {
   constexpr int D = F::kDim;
@@ -181,7 +182,8 @@ DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, con
   }
   der2 = sqrt(der2) / fabs(h);
   double der12 = fmax(sqrt(dnf), der2);
-  double h1 = der12 <= 1.0e-15 ? fabs(h) * 1.0e-3 : DE_CTRL_POW(0.01 / der12, 1.0 / 5.0);
+  double h1 = der12 <= 1.0e-15 ? fabs(h) * 1.0e-3
+                               : DE_CTRL_POW(0.01 / der12, 1.0 / (double)ORDER);
   double interval = fabs(tf - t0);
   h = fmin(fmax(fmin(fmin(fabs(h) * 100.0, h1), h_max), h_min), interval);
   return h * posneg;
@@ -428,6 +430,297 @@ struct Dopri5Lane {
   }
 
   // step() + finish(): returns true when the trajectory ended.
+  DE_HD bool advance(const OdeArgs& a, int64_t i) {
+    int32_t st = step(a, i);
+    if (st < 0) return false;
+    finish(a, i, st);
+    return true;
+  }
+};
+
+// ------------------------------------------------------------------ DOP853 --
+// The reference's generic Dormand-Prince step (dormandprince/ordinary.rs:
+// 63-270) with the DOP853 tableau (src/tableau/dorman_prince.rs:155-380): 12
+// stages, the second error estimate bh (deno = err + 0.01 err2), order 8 in
+// the controller and h_init, stiffness test on k[11].  On every accepted step
+// the reference evaluates three more stages for dense output, so evals =
+// base + 11 (acc + rej) + 4 acc; the lane evaluates them only when t_eval is
+// requested (they do not enter the solution).  Zero coefficients are skipped
+// as in the DOPRI5 lane: each of k1..k4 (whose b and er weights are zero)
+// feeds, through nonzero a entries, a stage with nonzero b and er weights, so
+// a non-finite stage still makes the attempt's error non-finite.
+namespace d8 {
+constexpr double C[16] = DE_DOP853_C;
+constexpr double A[16][16] = DE_DOP853_A;
+constexpr double B[12] = DE_DOP853_B;
+constexpr double BH[12] = DE_DOP853_BH;
+constexpr double ER[12] = DE_DOP853_ER;
+constexpr double BI[16][16] = DE_DOP853_BI;
+}  // namespace d8
+
+template <class F, bool DENSE = false, bool VTOL = false>
+struct Dop853Lane {
+  static constexpr int D = F::kDim;
+  F f;
+  double y[D], k0[D];
+  double t, h, h_prev, tf, dir;
+  LaneTol<D, VTOL> tol;
+  int32_t acc, rej, base_ev, stiff, nonstiff, countdown, next_eval, pushed;
+  bool rejected_last;
+
+  DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, Dop853Lane L) {
+    for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
+    a.t_final[i] = L.t;
+    a.accepted[i] = L.acc;
+    a.rejected[i] = L.rej;
+    a.evals[i] =
+        st == kBadInput ? 0 : (int64_t)L.base_ev + 11 * (int64_t)(L.acc + L.rej) + 4 * (int64_t)L.acc;
+    a.status[i] = st;
+    if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
+    if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
+  }
+  DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
+
+  DE_COLD static Dop853Lane start(const OdeArgs& a, int64_t i, bool* ok) {
+    Dop853Lane L;
+    *ok = L.init(a, i);
+    return L;
+  }
+  DE_HD bool begin(const OdeArgs& a, int64_t i) {
+    bool ok;
+    *this = start(a, i, &ok);
+    return ok;
+  }
+
+  DE_HD static double predicted_cost(const OdeArgs& a, int64_t i) {
+    F f;
+    f.load(a.params, a.n, i);
+    const double t0 = a.t0 ? a.t0[i] : a.t0s;
+    const double tf = a.tf ? a.tf[i] : a.tfs;
+    double h0 = a.h0;
+    if (h0 == 0.0 && tf != t0) {
+      double y[D];
+      for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
+      LaneTol<D, VTOL> tl;
+      tl.load(a, i);
+      h0 = lane_h_init<F, LaneTol<D, VTOL>, 8>(f, t0, tf, y, tl, a.h_min, a.h_max);
+    }
+    return fabs(tf - t0) / fabs(h0);
+  }
+
+  DE_HD bool init(const OdeArgs& a, int64_t i) {
+    f.load(a.params, a.n, i);
+    double t0 = a.t0 ? a.t0[i] : a.t0s;
+    tf = a.tf ? a.tf[i] : a.tfs;
+    tol.load(a, i);
+    for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
+    t = t0;
+    acc = rej = 0;
+    pushed = 0;
+    next_eval = 0;
+    double s = signum(tf - t0);
+    dir = s;
+    bool ok = (s == 1.0 || s == -1.0);
+    double h0 = a.h0;
+    base_ev = 1;
+    if (ok && h0 == 0.0) {
+      h0 = lane_h_init<F, LaneTol<D, VTOL>, 8>(f, t0, tf, y, tol, a.h_min, a.h_max);
+      base_ev = 3;
+    }
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      finish(a, i, kBadInput);
+      return false;
+    }
+    h = h0;
+    f.diff(t, y, k0);
+    h_prev = 0.0;
+    stiff = 0;
+    nonstiff = 0;
+    countdown = 100;
+    rejected_last = false;
+    if (DENSE && a.n_eval > 0) {
+      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y, [](double, double*) {});
+    }
+    return true;
+  }
+
+  // y + sum_{j<i} (a[i][j] h) k[j], in j order, zero coefficients skipped.
+  template <int I>
+  DE_HD static void stage_input(const double* y, const double (*k)[D], double h, double* ys) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) ys[c] = y[c];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+      if (d8::A[I][j] != 0.0) {
+        const double ah = d8::A[I][j] * h;
+#pragma unroll
+        for (int c = 0; c < D; ++c) ys[c] = ys[c] + ah * k[j][c];
+      }
+    }
+  }
+
+  template <int I>
+  DE_HD void stages(double (*k)[D], double* ys) const {
+    if constexpr (I < 12) {
+      stage_input<I>(y, k, h, ys);
+      f.diff(t + d8::C[I] * h, ys, k[I]);
+      stages<I + 1>(k, ys);
+    }
+  }
+
+  template <int I>
+  DE_HD void dense_stages(double (*k)[D], const double* yy, double tt, double hh) const {
+    if constexpr (I < 16) {
+      double ys[D];
+      stage_input<I>(yy, k, hh, ys);
+      f.diff(tt + d8::C[I] * hh, ys, k[I]);
+      dense_stages<I + 1>(k, yy, tt, hh);
+    }
+  }
+
+  // sum_j w[j] k[j] starting from the first nonzero weight (0 + x == x).
+  template <int N>
+  DE_HD static double comb(const double* w, const double (*k)[D], int c) {
+    double s = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (w[j] != 0.0) {
+        s = first ? w[j] * k[j][c] : s + w[j] * k[j][c];
+        first = false;
+      }
+    }
+    return s;
+  }
+
+  DE_HD int32_t step(const OdeArgs& a, int64_t i) {
+    {
+      const bool clip = (t + h - tf) * dir > 0.0;
+      const double hn = tf - t;
+      const bool done = clip && fabs(hn) < kTenEps;
+      h = clip ? hn : h;
+      const bool small = fabs(h) < fabs(h_prev) * 1e-14;
+      const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
+      if (done | small | maxed) return done ? kComplete : (small ? kStepSize : kMaxSteps);
+    }
+    countdown -= 1;
+    const bool stiff_check = countdown == 0;
+    if (stiff_check) countdown = 100;
+
+    double k[DENSE ? 16 : 12][D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) k[0][c] = k0[c];
+    double ys[D];
+    stages<1>(k, ys);  // ys ends as the stage-11 input (ysti)
+
+    double yseg[D], ynew[D];
+    double err = 0.0, err2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      yseg[c] = comb<12>(d8::B, k, c);
+      ynew[c] = y[c] + h * yseg[c];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double es = comb<12>(d8::ER, k, c);
+      double e2s = yseg[c];
+#pragma unroll
+      for (int j = 0; j < 12; ++j)
+        if (d8::BH[j] != 0.0) e2s = e2s + (-d8::BH[j]) * k[j][c];
+      const double sk = tol.at(c) + tol.rt(c) * fmax(fabs(y[c]), fabs(ynew[c]));
+      const double e = es / sk;
+      const double e2 = e2s / sk;
+      err = c == 0 ? e * e : err + e * e;
+      err2 = c == 0 ? e2 * e2 : err2 + e2 * e2;
+    }
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * (double)D));
+    double scale = a.safety * DE_CTRL_POW(err, -0.125);
+    scale = fmin(fmax(scale, a.min_scale), a.max_scale);
+
+    const bool accept = err <= 1.0;
+    const double t_new = t + h;
+    double dn[D];
+    f.diff(t_new, ynew, dn);
+    if (accept && stiff_check) {
+      double stdnum = 0.0, stden = 0.0;
+      for (int c = 0; c < D; ++c) {
+        double u = yseg[c] - k[11][c];
+        stdnum = c == 0 ? u * u : stdnum + u * u;
+        double v = dn[c] - ys[c];
+        stden = c == 0 ? v * v : stden + v * v;
+      }
+      if (stden > 0.0) {
+        double h_lamb = h * sqrt(stdnum / stden);
+        if (h_lamb > 6.1) {
+          nonstiff = 0;
+          stiff += 1;
+        }
+      } else {
+        nonstiff += 1;
+        if (nonstiff == 6) stiff = 0;
+      }
+    }
+    const bool stop = stiff == 15;
+    const bool take = accept && !stop;
+    if constexpr (DENSE) {
+      if (take && a.n_eval > 0) {
+        // cont[0..3] as DOPRI5; k[12] = f(t_new, y_new); k[13..15] at c[13..15];
+        // cont[i] = h * sum_j bi[i][j] k[j] for i = 4..7.
+        double cont[8][D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          double ydiff = ynew[c] + (-1.0) * y[c];
+          cont[0][c] = y[c];
+          cont[1][c] = ydiff;
+          double bspl = h * k[0][c] + (-1.0) * ydiff;
+          cont[2][c] = bspl;
+          cont[3][c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
+          k[12][c] = dn[c];
+        }
+        dense_stages<13>(k, y, t, h);
+#pragma unroll
+        for (int r = 4; r < 8; ++r)
+#pragma unroll
+          for (int c = 0; c < D; ++c) cont[r][c] = comb<16>(d8::BI[r], k, c) * h;
+        const double hp = h, tp = t;
+        teval_emit<D>(a, i, dir, next_eval, pushed, t_new, t, ynew, [&](double te, double* out) {
+          // ordinary.rs:318-333 with 8 coefficients: factors s1, s, s1 for
+          // cont[6], cont[5], cont[4], then s1, s, s1 for cont[3..1].
+          const double s = (te - tp) / hp;
+          const double s1 = 1.0 - s;
+          for (int c = 0; c < D; ++c) {
+            double q = cont[7][c];
+            q = q * s1 + cont[6][c];
+            q = q * s + cont[5][c];
+            q = q * s1 + cont[4][c];
+            q = q * s1 + cont[3][c];
+            q = q * s + cont[2][c];
+            q = q * s1 + cont[1][c];
+            out[c] = cont[0][c] + s * q;
+          }
+        });
+      }
+    }
+    h_prev = take ? h : h_prev;
+    t = take ? t_new : t;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      y[c] = take ? ynew[c] : y[c];
+      k0[c] = take ? dn[c] : k0[c];
+    }
+    scale = take && rejected_last ? fmin(scale, 1.0) : scale;
+    rejected_last = !accept;
+    acc += take ? 1 : 0;
+    rej += accept ? 0 : 1;
+    const int32_t result =
+        stop ? kStiffness : (take && fabs(tf - t) <= kTenEps ? kComplete : -1);
+    h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    return result;
+  }
+
   DE_HD bool advance(const OdeArgs& a, int64_t i) {
     int32_t st = step(a, i);
     if (st < 0) return false;

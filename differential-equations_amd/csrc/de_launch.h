@@ -8,7 +8,7 @@
 namespace de {
 
 enum : int { kSchedAuto = 0, kSchedPlain = 1, kSchedPersistent = 2, kSchedPersistentFifo = 3 };
-enum : int { kMethodDopri5 = 0, kMethodRk4 = 1, kMethodEuler = 2 };
+enum : int { kMethodDopri5 = 0, kMethodRk4 = 1, kMethodEuler = 2, kMethodDop853 = 3 };
 enum : int { kSchemeEulerMaruyama = 0, kSchemeMilstein = 1 };
 enum : int { kNoisePhilox = 0, kNoiseTable = 1, kNoiseConstant = 2 };
 

@@ -236,6 +236,14 @@ static int launch_method(int method, const OdeArgs& a, int schedule, hipStream_t
 #if !defined(DE_FAST_BUILD)
     case kMethodRk4: return launch<FixedLane<F, 4>>(a, schedule, s);
     case kMethodEuler: return launch<FixedLane<F, 1>>(a, schedule, s);
+    case kMethodDop853: {
+      const bool vtol = a.rtol_comp || a.atol_comp;
+      if (a.n_eval > 0)
+        return vtol ? launch<Dop853Lane<F, true, true>>(a, schedule, s)
+                    : launch<Dop853Lane<F, true, false>>(a, schedule, s);
+      return vtol ? launch<Dop853Lane<F, false, true>>(a, schedule, s)
+                  : launch<Dop853Lane<F, false, false>>(a, schedule, s);
+    }
 #endif
     default: return 1;
   }

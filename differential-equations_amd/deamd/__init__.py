@@ -25,7 +25,7 @@ def lib():
 
 # ----------------------------------------------------------------- methods --
 class ExplicitRungeKutta:
-    """ExplicitRungeKutta::{dopri5, rk4, euler} with the builder setters of
+    """ExplicitRungeKutta::{dopri5, dop853, rk4, euler} with the builder setters of
     src/methods/erk/mod.rs:165-228 and the defaults of :115-158."""
 
     def __init__(self, method, h0=0.0):
@@ -36,6 +36,11 @@ class ExplicitRungeKutta:
     @classmethod
     def dopri5(cls):
         return cls(DE_METHOD_DOPRI5)
+
+    @classmethod
+    def dop853(cls):
+        """ExplicitRungeKutta::dop853 (src/methods/erk/dormandprince/mod.rs:46-52)."""
+        return cls(DE_METHOD_DOP853)
 
     @classmethod
     def rk4(cls, h0):

@@ -37,6 +37,7 @@ DE_SDE_LINEAR_DIAGONAL = 1
 DE_METHOD_DOPRI5 = 0
 DE_METHOD_RK4 = 1
 DE_METHOD_EULER = 2
+DE_METHOD_DOP853 = 3
 
 DE_SCHEME_EULER_MARUYAMA = 0
 DE_SCHEME_MILSTEIN = 1

@@ -68,6 +68,7 @@ extern "C" {
 #define DE_METHOD_DOPRI5 0
 #define DE_METHOD_RK4 1
 #define DE_METHOD_EULER 2
+#define DE_METHOD_DOP853 3 /* ExplicitRungeKutta::dop853 (dormandprince/mod.rs:46-52) */
 
 /* SDE schemes: ExplicitRungeKutta::euler(dt) on an SDE (Euler-Maruyama,
  * src/methods/erk/fixed/stochastic.rs) and Milstein::new(dt)

@@ -41,6 +41,7 @@
 #include <thread>
 #include <vector>
 
+#include "../differential-equations_amd/csrc/de_dop853_table.h"
 #include "../differential-equations_amd/csrc/de_families.h"
 #include "../differential-equations_amd/csrc/de_noise.h"
 #include "../include/de_api.h"
@@ -48,6 +49,8 @@
 namespace orc {
 
 static int g_pow_mode = 0;
+// Test switch: run DOPRI5 through the generic Dormand-Prince class.
+static int g_generic_dopri5 = 0;
 
 static inline double ref_powf(double x, double y) {
   return g_pow_mode ? ::pow(x, y) : de::pow(x, y);
@@ -409,6 +412,208 @@ struct Dopri5 {
     copy(acc, cont[4], n);
     for (int i = 3; i >= 1; --i) {
       double factor = (i % 2 == 1) ? s1 : s;
+      for (int c = 0; c < n; ++c) acc[c] *= factor;
+      add_scaled(acc, 1.0, cont[i], n);
+    }
+    copy(out, cont[0], n);
+    add_scaled(out, s, acc, n);
+  }
+};
+
+// --------------------------------------------- generic Dormand-Prince -----
+// The reference's step is generic over the tableau, ExplicitRungeKutta<
+// Ordinary, DormandPrince, T, Y, O, S, I> (dormandprince/ordinary.rs:13-336):
+// S stages, I stages including dense-output ones, order O, an optional second
+// error estimate bh.  This restatement follows that generic code; DOP853
+// (tableau src/tableau/dorman_prince.rs:155-380) runs through it, and with the
+// DOPRI5 tableau it reproduces the DOPRI5 class above bit for bit.
+struct Dop853Tableau {
+  static constexpr int S = 12, I = 16, O = 8;
+  static constexpr bool kHasBh = true;
+  double c[16] = DE_DOP853_C;
+  double a[16][16] = DE_DOP853_A;
+  double b[12] = DE_DOP853_B;
+  double bh[12] = DE_DOP853_BH;
+  double er[12] = DE_DOP853_ER;
+  double bi[16][16] = DE_DOP853_BI;
+};
+static const Dop853Tableau kDOP853;
+
+struct Dopri5TableauG {
+  static constexpr int S = 7, I = 7, O = 5;
+  static constexpr bool kHasBh = false;
+  double c[7], a[7][7], b[7], bh[7] = {}, er[7], bi[7][7];
+  Dopri5TableauG() {
+    for (int i = 0; i < 7; ++i) {
+      c[i] = kDP.c[i];
+      b[i] = kDP.b[i];
+      er[i] = kDP.er[i];
+      for (int j = 0; j < 7; ++j) {
+        a[i][j] = kDP.a[i][j];
+        bi[i][j] = kDP.bi[i][j];
+      }
+    }
+  }
+};
+static const Dopri5TableauG kDP5G;
+
+template <class Sys, class Tab>
+struct DormandPrince {
+  static constexpr int M = Sys::kMax;
+  static constexpr int S = Tab::S, I = Tab::I, O = Tab::O;
+  const Sys& sys;
+  const Tab& tb;
+  Settings st;
+  int n;
+  double h0, h = 0.0, t = 0.0, h_prev = 0.0, t_prev = 0.0;
+  double y[M], dydt[M], y_prev[M], dydt_prev[M];
+  double k[I][M];
+  double cont[O][M];
+  int64_t steps = 0;
+  int stiffness_counter = 0, non_stiffness_counter = 0;
+  bool rejected_status = false;
+
+  DormandPrince(const Sys& s, const Tab& t_, const Settings& set)
+      : sys(s), tb(t_), st(set), n(s.dim()), h0(set.h0) {}
+
+  bool init(double t0, double tf, const double* y0, int64_t* evals) {
+    if (h0 == 0.0) {
+      h0 = h_init(sys, t0, tf, y0, O, st.rtolv, st.atolv, st.h_min, st.h_max, evals);
+    }
+    if (!de::validate_step_size(h0, st.h_min, st.h_max, t0, tf)) return false;
+    h = h0;
+    stiffness_counter = 0;
+    t = t0;
+    copy(y, y0, n);
+    zero(dydt, n);
+    copy(y_prev, y0, n);
+    zero(dydt_prev, n);
+    for (int i = 0; i < I; ++i) zero(k[i], n);
+    for (int i = 0; i < O; ++i) zero(cont[i], n);
+    sys.diff(t, y, k[0]);
+    copy(dydt, k[0], n);
+    *evals += 1;
+    t_prev = t;
+    copy(y_prev, y, n);
+    copy(dydt_prev, dydt, n);
+    return true;
+  }
+
+  int step(int64_t* evals) {
+    if (fabs(h) < fabs(h_prev) * 1e-14) return DE_STATUS_STEP_SIZE;
+    if (steps >= st.max_steps) return DE_STATUS_MAX_STEPS;
+    steps += 1;
+    double y_stage[M];
+    zero(y_stage, n);
+    for (int i = 1; i < S; ++i) {
+      copy(y_stage, y, n);
+      for (int j = 0; j < i; ++j) add_scaled(y_stage, tb.a[i][j] * h, k[j], n);
+      sys.diff(t + tb.c[i] * h, y_stage, k[i]);
+    }
+    double ysti[M];
+    copy(ysti, y_stage, n);
+    double yseg[M];
+    zero(yseg, n);
+    for (int i = 0; i < S; ++i) add_scaled(yseg, tb.b[i], k[i], n);
+    double y_new[M];
+    copy(y_new, y, n);
+    add_scaled(y_new, h, yseg, n);
+    double t_new = t + h;
+    *evals += S - 1;
+    double err2 = 0.0;
+    double err_state[M];
+    zero(err_state, n);
+    for (int j = 0; j < S; ++j) add_scaled(err_state, tb.er[j], k[j], n);
+    double err = error_norm(y, y_new, err_state, st.atolv, st.rtolv, n);
+    if (Tab::kHasBh) {
+      double err2_state[M];
+      copy(err2_state, yseg, n);
+      for (int j = 0; j < S; ++j) add_scaled(err2_state, -tb.bh[j], k[j], n);
+      err2 = error_norm(y, y_new, err2_state, st.atolv, st.rtolv, n);
+    }
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * (double)n));
+    double error_exponent = 1.0 / (double)O;
+    double scale = st.safety * ref_powf(err, -error_exponent);
+    scale = fmin(fmax(scale, st.min_scale), st.max_scale);
+    if (err <= 1.0) {
+      sys.diff(t_new, y_new, dydt);
+      *evals += 1;
+      if (steps % 100 == 0) {
+        double stdnum = diff_norm_squared(yseg, k[S - 1], n);
+        double stden = diff_norm_squared(dydt, ysti, n);
+        if (stden > 0.0) {
+          double h_lamb = h * sqrt(stdnum / stden);
+          if (h_lamb > 6.1) {
+            non_stiffness_counter = 0;
+            stiffness_counter += 1;
+            if (stiffness_counter == 15) return DE_STATUS_STIFFNESS;
+          }
+        } else {
+          non_stiffness_counter += 1;
+          if (non_stiffness_counter == 6) stiffness_counter = 0;
+        }
+      }
+      copy(cont[0], y, n);
+      double ydiff[M];
+      copy(ydiff, y_new, n);
+      add_scaled(ydiff, -1.0, y, n);
+      copy(cont[1], ydiff, n);
+      double bspl[M];
+      zero(bspl, n);
+      add_scaled(bspl, h, k[0], n);
+      add_scaled(bspl, -1.0, ydiff, n);
+      copy(cont[2], bspl, n);
+      double cont3[M];
+      copy(cont3, ydiff, n);
+      add_scaled(cont3, -h, dydt, n);
+      add_scaled(cont3, -1.0, bspl, n);
+      copy(cont[3], cont3, n);
+      // bi is Some for both tableaux: extra stages (I > S), then cont[4..O).
+      if (I > S) {
+        copy(k[S], dydt, n);
+        for (int i = S + 1; i < I; ++i) {
+          copy(y_stage, y, n);
+          for (int j = 0; j < i; ++j) add_scaled(y_stage, tb.a[i][j] * h, k[j], n);
+          sys.diff(t + tb.c[i] * h, y_stage, k[i]);
+          *evals += 1;
+        }
+      }
+      for (int i = 4; i < O; ++i) {
+        zero(cont[i], n);
+        for (int j = 0; j < I; ++j) add_scaled(cont[i], tb.bi[i][j], k[j], n);
+        for (int c = 0; c < n; ++c) cont[i][c] *= h;
+      }
+      t_prev = t;
+      copy(y_prev, y, n);
+      copy(dydt_prev, k[0], n);
+      h_prev = h;
+      t = t_new;
+      copy(y, y_new, n);
+      copy(k[0], dydt, n);
+      if (rejected_status) {
+        rejected_status = false;
+        scale = fmin(scale, 1.0);
+      }
+    } else {
+      rejected_status = true;
+    }
+    h *= scale;
+    h = de::constrain_step_size(h, st.h_min, st.h_max);
+    return -1;
+  }
+
+  // ordinary.rs:301-336: fold from cont[O-1] down to cont[1] with factors s1
+  // or s (indices >= 4 alternate by distance to the last, 1..3 by parity).
+  void interpolate(double t_interp, double* out) const {
+    double s = (t_interp - t_prev) / h_prev;
+    double s1 = 1.0 - s;
+    const int ilast = O - 1;
+    double acc[M];
+    copy(acc, cont[ilast], n);
+    for (int i = ilast - 1; i >= 1; --i) {
+      double factor = i >= 4 ? (((ilast - i) % 2 == 1) ? s1 : s) : ((i % 2 == 1) ? s1 : s);
       for (int c = 0; c < n; ++c) acc[c] *= factor;
       add_scaled(acc, 1.0, cont[i], n);
     }
@@ -854,6 +1059,12 @@ static void run_ode_traj(const Sys& sys, const de_ode_batch* b, const Settings& 
   if (b->method == DE_METHOD_RK4 || b->method == DE_METHOD_EULER) {
     FixedErk<Sys> m(sys, st, b->method == DE_METHOD_RK4 ? kRK4 : kEuler);
     *res = solve_ode(m, t0, tf, y0, tep, y_out);
+  } else if (b->method == DE_METHOD_DOP853) {
+    DormandPrince<Sys, Dop853Tableau> m(sys, kDOP853, st);
+    *res = solve_ode(m, t0, tf, y0, tep, y_out);
+  } else if (g_generic_dopri5) {
+    DormandPrince<Sys, Dopri5TableauG> m(sys, kDP5G, st);
+    *res = solve_ode(m, t0, tf, y0, tep, y_out);
   } else {
     Dopri5<Sys> m(sys, st);
     *res = solve_ode(m, t0, tf, y0, tep, y_out);
@@ -1011,11 +1222,33 @@ void de_oracle_dopri5_tableau(double* c, double* a, double* b, double* er, doubl
   }
 }
 
+// The DOP853 tableau the oracle uses: c[16], a[16*16], b/bh/er[12], bi[16*16].
+void de_oracle_dop853_tableau(double* c, double* a, double* b, double* bh, double* er,
+                              double* bi) {
+  using orc::kDOP853;
+  for (int i = 0; i < 16; ++i) {
+    c[i] = kDOP853.c[i];
+    for (int j = 0; j < 16; ++j) {
+      a[i * 16 + j] = kDOP853.a[i][j];
+      bi[i * 16 + j] = kDOP853.bi[i][j];
+    }
+  }
+  for (int i = 0; i < 12; ++i) {
+    b[i] = kDOP853.b[i];
+    bh[i] = kDOP853.bh[i];
+    er[i] = kDOP853.er[i];
+  }
+}
+
 double de_oracle_pow(double x, double y) { return de::pow(x, y); }
 
 void de_oracle_pow_many(const double* x, const double* y, double* out, int64_t n) {
   for (int64_t i = 0; i < n; ++i) out[i] = de::pow(x[i], y[i]);
 }
+
+// Route DOPRI5 solves through the generic Dormand-Prince restatement (1) or
+// the dedicated DOPRI5 one (0, default); both must give the same bits.
+void de_oracle_set_generic_dopri5(int on) { orc::g_generic_dopri5 = on; }
 
 // The first try of pow(x, -0.2) alone: ok[i] = 1 when its rounding test
 // certified out[i], else 0 (and pow falls back to the double-double path).

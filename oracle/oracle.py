@@ -41,6 +41,10 @@ def load():
         lib.de_oracle_pow.argtypes = [C.c_double, C.c_double]
         lib.de_oracle_pow_many.restype = None
         lib.de_oracle_pow_many.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        lib.de_oracle_dop853_tableau.restype = None
+        lib.de_oracle_dop853_tableau.argtypes = [C.c_void_p] * 6
+        lib.de_oracle_set_generic_dopri5.restype = None
+        lib.de_oracle_set_generic_dopri5.argtypes = [C.c_int]
         lib.de_oracle_pow_m02_fast.restype = None
         lib.de_oracle_pow_m02_fast.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         lib.de_oracle_noise_normal.restype = C.c_double
@@ -60,6 +64,16 @@ def dopri5_tableau():
     load().de_oracle_dopri5_tableau(c.ctypes.data, a.ctypes.data, b.ctypes.data,
                                     er.ctypes.data, bi0.ctypes.data)
     return c, a.reshape(7, 7), b, er, bi0
+
+
+def dop853_tableau():
+    """(c[16], a[16,16], b[12], bh[12], er[12], bi[16,16]) as the oracle uses them."""
+    import numpy as np
+    c, b, bh, er = np.empty(16), np.empty(12), np.empty(12), np.empty(12)
+    a, bi = np.empty(256), np.empty(256)
+    load().de_oracle_dop853_tableau(c.ctypes.data, a.ctypes.data, b.ctypes.data,
+                                    bh.ctypes.data, er.ctypes.data, bi.ctypes.data)
+    return c, a.reshape(16, 16), b, bh, er, bi.reshape(16, 16)
 
 
 def solve_ode(ensemble, pow_mode=POW_PROJECT, threads=1):
@@ -86,6 +100,11 @@ def pow_many(x, y):
     out = np.empty_like(x)
     load().de_oracle_pow_many(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
     return out
+
+
+def set_generic_dopri5(on):
+    """Run DOPRI5 through the generic Dormand-Prince restatement (test switch)."""
+    load().de_oracle_set_generic_dopri5(1 if on else 0)
 
 
 def pow_m02_fast(x):

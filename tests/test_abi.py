@@ -65,6 +65,17 @@ def test_invalid_descriptors_are_rejected_before_any_device_work(mutate, code):
     assert deamd.lib().de_last_error()
 
 
+def test_spring_chains_accept_dopri5_only():
+    # The spring-chain solvers implement DOPRI5; other methods are refused
+    # rather than silently solved with DOPRI5.
+    sysm, y0 = deamd.gen_spring(8, seed=1)
+    for m in (deamd.ExplicitRungeKutta.rk4(0.01), deamd.ExplicitRungeKutta.dop853()):
+        ens = deamd.IVP.ode(sysm, 0.0, 1.0, y0).method(m)
+        b, res, keep = ens.batch()
+        assert deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None) == \
+            _abi.DE_ERR_UNSUPPORTED
+
+
 def test_empty_batch_is_a_no_op():
     ens = deamd.IVP.ode(deamd.Lorenz(10.0, 28.0, 8.0 / 3.0), 0.0, 1.0, np.zeros((3, 0)))
     b, res, keep = ens.batch()

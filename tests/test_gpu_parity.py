@@ -217,7 +217,26 @@ def test_reference_systems_bit_exact(orc, case):
             assert abs(got.y[c * 3] - e) < tol
 
 
+@pytest.mark.parametrize("schedule", [DE_SCHED_PLAIN, DE_SCHED_PERSISTENT])
+def test_dop853_lorenz_bit_exact(orc, schedule):
+    p, y0 = deamd.gen_lorenz(8192, seed=41)
+    ens = IVP.ode(Lorenz(*p), 0.0, 10.0, y0).method(
+        ExplicitRungeKutta.dop853().rtol(1e-9).atol(1e-11).max_steps(1000000)) \
+        .options(schedule=schedule)
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.status == DE_STATUS_COMPLETE).all()
+    assert_ode_equal(ens.solve(), want)
+    # early stops and per-component tolerances
+    ens = IVP.ode(Lorenz(*p[:, :1024]), 0.0, 10.0, y0[:, :1024]).method(
+        ExplicitRungeKutta.dop853().rtol([1e-9, 1e-7, 1e-10]).atol([1e-11, 1e-9, 1e-12])
+        .max_steps(120)).options(schedule=schedule)
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.status == DE_STATUS_MAX_STEPS).any()
+    assert_ode_equal(ens.solve(), want)
+
+
 @pytest.mark.parametrize("make_method", [
+    lambda: ExplicitRungeKutta.dop853().rtol(1e-9).atol(1e-11).max_steps(1000000),
     lambda: ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(1000000),
     lambda: ExplicitRungeKutta.rk4(0.01).max_steps(1000000),
     lambda: ExplicitRungeKutta.euler(0.001).max_steps(1000000)])

@@ -363,3 +363,156 @@ def test_noise_normals_are_standard_normal(orc):
     # deterministic, and keyed by (seed, path, step, component)
     assert lib.de_oracle_noise_normal(1, 2, 3, 4) == lib.de_oracle_noise_normal(1, 2, 3, 4)
     assert lib.de_oracle_noise_normal(1, 2, 3, 4) != lib.de_oracle_noise_normal(1, 2, 3, 5)
+
+
+# ------------------------------------------------------------------ DOP853 --
+def _scipy_dop853():
+    """SciPy's independent copy of the published DOP853 coefficients, as the
+    tableau dict of pyref_dopri5.solve_dp (bh = b - E3, er = E5, bi rows 4..7
+    = D; c[12] is unused: k[12] is the derivative at the new point)."""
+    d = pytest.importorskip("scipy.integrate._ivp.dop853_coefficients")
+    A = [[float(x) for x in row] for row in d.A[:16, :16]]
+    A[12] = [0.0] * 16
+    BI = [[0.0] * 16 for _ in range(16)]
+    for r in range(4):
+        BI[4 + r] = [float(x) for x in d.D[r, :16]]
+    return dict(S=12, I=16, O=8, C=[float(x) for x in d.C[:16]], A=A,
+                B=[float(x) for x in d.B], ER=[float(x) for x in d.E5[:12]],
+                BH=[float(x) for x in (d.B - d.E3[:12])], BI=BI)
+
+
+def test_dop853_tableau_equals_published_coefficients(orc):
+    c, a, b, bh, er, bi = orc.dop853_tableau()
+    ref = _scipy_dop853()
+    for i in range(16):
+        for j in range(16):
+            assert bi[i, j] == ref["BI"][i][j], (i, j)
+        if i == 12:  # unused by the reference (k[12] = f(t_new, y_new)); SciPy's is b
+            assert c[i] == 0.0 and not a[i].any()
+            continue
+        assert c[i] == ref["C"][i], i
+        for j in range(16):
+            assert a[i, j] == ref["A"][i][j], (i, j)
+    assert list(b) == ref["B"] and list(er) == ref["ER"] and list(bh) == ref["BH"]
+
+
+def test_dop853_tableau_order_conditions(orc):
+    # Quadrature and tall-tree conditions of order 8 (b^T c^q = 1/(q+1),
+    # b^T A^k 1 = 1/(k+1)!), row sums = c, the 5th-order embedded weights
+    # b - er to order 5; in floating point the decimal coefficients meet them
+    # to ~1e-15, a transcription error by far more.
+    c, a, b, bh, er, bi = orc.dop853_tableau()
+    A, C = a[:12, :12], c[:12]
+    for q in range(8):
+        assert abs(b @ C ** q - 1.0 / (q + 1)) < 1e-14, q
+    v = np.ones(12)
+    for k in range(8):
+        assert abs(b @ v - 1.0 / math.factorial(k + 1)) < 1e-14, k
+        v = A @ v
+    assert np.abs(a.sum(1)[:12] - C).max() < 1e-14
+    assert np.abs(a.sum(1)[13:] - c[13:]).max() < 1e-14
+    b5 = b - er
+    for q in range(5):
+        assert abs(b5 @ C ** q - 1.0 / (q + 1)) < 1e-14, q
+
+
+def test_dop853_single_step_is_R_of_z(orc):
+    # One accepted step on y' = lam y equals R(z) y0 with R computed exactly
+    # from the tableau's doubles: R(z) = 1 + z b^T (I - zA)^{-1} 1.
+    c, a, b, bh, er, bi = orc.dop853_tableau()
+    Af = [[Fr(float(x)) for x in row] for row in a[:12, :12]]
+    bf = [Fr(float(x)) for x in b]
+    for lam, h in [(1.0, 0.1), (-2.0, 0.05)]:
+        ens = IVP.ode(Exponential(lam), 0.0, h, [1.0], n=1) \
+            .method(ExplicitRungeKutta.dop853().h0(h).rtol(1.0).atol(1.0))
+        sol = orc.solve_ode(ens)
+        assert sol.accepted[0] == 1 and sol.rejected[0] == 0
+        z = Fr(lam) * Fr(h)
+        R = Fr(1) + z * sum(bf[i] * x for i, x in enumerate(_apply_resolvent(Af, z)))
+        assert abs(sol.y[0] - float(R)) <= 4e-16 * abs(float(R))
+        assert abs(sol.y[0] - math.exp(lam * h)) < 1e-14
+
+
+def _apply_resolvent(A, z):
+    """(I - zA)^{-1} 1 for strictly lower-triangular A, exactly."""
+    s = len(A)
+    x = [Fr(0)] * s
+    for i in range(s):
+        x[i] = Fr(1) + z * sum(A[i][j] * x[j] for j in range(i))
+    return x
+
+
+def test_dop853_dense_output_bit_equals_python_restatement(orc):
+    # The continuous extension (three extra stages, cont[4..8) from bi rows
+    # 4..7) and the reference's interpolation fold, bit for bit against the
+    # Python transcription fed SciPy's coefficients.  The fold alternates s and
+    # s1 in the high-order part in the opposite phase from Hairer's contd8, so
+    # the interpolant is less accurate than DOP853's published 7th-order one;
+    # it is the reference's.
+    rho = 28.0
+    y0 = [1.0, 1.0, 1.0]
+    pts = list(np.linspace(0.0, 1.0, 23))
+
+    def f(t, y, dy):
+        x, yy, z = y
+        dy[0] = 10.0 * (yy - x)
+        dy[1] = x * (rho - z) - yy
+        dy[2] = x * yy - (8.0 / 3.0) * z
+
+    want = pyref_dopri5.solve_dp(f, 0.0, 1.0, y0, 1e-9, 1e-11, _scipy_dop853(), t_eval=pts)
+    ens = IVP.ode(Lorenz(10.0, rho, 8.0 / 3.0), 0.0, 1.0, np.array(y0).reshape(3, 1)).t_eval(pts) \
+        .method(ExplicitRungeKutta.dop853().rtol(1e-9).atol(1e-11).max_steps(100000))
+    got = orc.solve_ode(ens, pow_mode=1)
+    assert got.n_dense[0] == len(want[6]) == len(pts)
+    assert got.dense.reshape(len(pts), 3).tolist() == want[6]
+    lam, h = 0.7, 0.2   # and the step's ends on y' = lam y
+    ens = IVP.ode(Exponential(lam), 0.0, h, [1.0], n=1).t_eval([0.0, 0.1, 0.2]) \
+        .method(ExplicitRungeKutta.dop853().h0(h).rtol(1.0).atol(1.0))
+    sol = orc.solve_ode(ens)
+    assert sol.dense[0] == 1.0 and sol.dense[2] == sol.y[0]
+    assert abs(sol.dense[1] - math.exp(lam * 0.1)) < 1e-8
+
+
+def test_dop853_evaluation_counts(orc):
+    # 11 evaluations per attempt, plus f(t_new, y_new) and the three
+    # dense-output stages on every accepted step; 3 for init with h_init.
+    sol = orc.solve_ode(lorenz_ensemble(64, 2.0).method(
+        ExplicitRungeKutta.dop853().rtol(1e-9).atol(1e-11).max_steps(100000)))
+    assert (sol.status == DE_STATUS_COMPLETE).all()
+    assert (sol.evals == 3 + 11 * (sol.accepted + sol.rejected) + 4 * sol.accepted).all()
+
+
+@pytest.mark.parametrize("traj", range(4))
+def test_oracle_dop853_bit_equals_python_restatement(orc, traj):
+    rng = np.random.default_rng(300 + traj)
+    rho = float(rng.uniform(0, 50))
+    y0 = list(1.0 + 0.1 * rng.standard_normal(3))
+
+    def f(t, y, dy):
+        x, yy, z = y
+        dy[0] = 10.0 * (yy - x)
+        dy[1] = x * (rho - z) - yy
+        dy[2] = x * yy - (8.0 / 3.0) * z
+
+    want = pyref_dopri5.solve_dp(f, 0.0, 2.0, y0, 1e-9, 1e-11, _scipy_dop853())
+    ens = IVP.ode(Lorenz(10.0, rho, 8.0 / 3.0), 0.0, 2.0, np.array(y0).reshape(3, 1)) \
+        .method(ExplicitRungeKutta.dop853().rtol(1e-9).atol(1e-11).max_steps(100000))
+    got = orc.solve_ode(ens, pow_mode=1)
+    assert list(got.y) == want[0]
+    assert got.t[0] == want[1]
+    assert (got.accepted[0], got.rejected[0], got.evals[0], got.status[0]) == want[2:]
+
+
+def test_generic_dormand_prince_step_reproduces_dopri5(orc):
+    # The oracle's generic Dormand-Prince class with the DOPRI5 tableau gives
+    # the dedicated DOPRI5 class's bits (dense output included).
+    ens = lorenz_ensemble(48, 3.0).t_eval(np.linspace(0, 3, 17))
+    a = orc.solve_ode(ens)
+    orc.set_generic_dopri5(True)
+    try:
+        b = orc.solve_ode(ens)
+    finally:
+        orc.set_generic_dopri5(False)
+    for f in ("y", "t", "accepted", "rejected", "evals", "status", "n_dense"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert np.array_equal(a.dense, b.dense)

@@ -18,28 +18,30 @@ from deamd import (ExplicitRungeKutta, Exponential, Harmonic, IVP, Linear, Logis
 POW_MODES = [0, 1]
 
 # (name, system, t0, tf, y0, expected, [(method, tolerance), ...])
-# tests/ode/accuracy.rs:70-103, 177-213, 286-321, 395-430, 503-539, 622-671
+# tests/ode/accuracy.rs:70-103, 177-213, 286-321, 395-430, 503-539, 622-671 (DOP853
+# rows at :78, 187, 296, 405, 514, 666)
 ACCURACY = [
     ("exponential_growth_positive_direction", Exponential(1.0), 0.0, 10.0, [1.0],
      [22026.46579479],
-     [(ExplicitRungeKutta.dopri5, 1e1), (lambda: ExplicitRungeKutta.rk4(0.01), 1e3),
-      (lambda: ExplicitRungeKutta.euler(0.01), 2e3)]),
+     [(lambda: ExplicitRungeKutta.dop853().rtol(1e-12).atol(1e-12), 1e-3), (ExplicitRungeKutta.dopri5, 1e1),
+      (lambda: ExplicitRungeKutta.rk4(0.01), 1e3), (lambda: ExplicitRungeKutta.euler(0.01), 2e3)]),
     ("exponential_growth_negative_direction", Exponential(1.0), 0.0, -10.0, [22026.46579479],
      [1.0],
-     [(ExplicitRungeKutta.dopri5, 1e-2), (lambda: ExplicitRungeKutta.rk4(-0.01), 1e-2),
-      (lambda: ExplicitRungeKutta.euler(-0.01), 1e-1)]),
+     [(lambda: ExplicitRungeKutta.dop853().rtol(1e-12).atol(1e-12), 1e-3), (ExplicitRungeKutta.dopri5, 1e-2),
+      (lambda: ExplicitRungeKutta.rk4(-0.01), 1e-2), (lambda: ExplicitRungeKutta.euler(-0.01), 1e-1)]),
     ("linear_equation", Linear(1.0, 1.0), 0.0, 10.0, [1.0], [44051.93158958],
-     [(ExplicitRungeKutta.dopri5, 1e1), (lambda: ExplicitRungeKutta.rk4(0.01), 1e3),
-      (lambda: ExplicitRungeKutta.euler(0.01), 1e4)]),
+     [(lambda: ExplicitRungeKutta.dop853().rtol(1e-12).atol(1e-12), 1e-3), (ExplicitRungeKutta.dopri5, 1e1),
+      (lambda: ExplicitRungeKutta.rk4(0.01), 1e3), (lambda: ExplicitRungeKutta.euler(0.01), 1e4)]),
     ("harmonic_oscillator", Harmonic(1.0), 0.0, 10.0, [1.0, 0.0], [-0.83907153, 0.54402111],
-     [(ExplicitRungeKutta.dopri5, 1e-2), (lambda: ExplicitRungeKutta.rk4(0.01), 1e-1),
-      (lambda: ExplicitRungeKutta.euler(0.01), 1e-1)]),
+     [(lambda: ExplicitRungeKutta.dop853().rtol(1e-12).atol(1e-12), 1e-3), (ExplicitRungeKutta.dopri5, 1e-2),
+      (lambda: ExplicitRungeKutta.rk4(0.01), 1e-1), (lambda: ExplicitRungeKutta.euler(0.01), 1e-1)]),
     ("logistic_equation", Logistic(1.0, 10.0), 0.0, 10.0, [0.1], [9.95525518],
-     [(ExplicitRungeKutta.dopri5, 1e-3), (lambda: ExplicitRungeKutta.rk4(0.01), 1e3),
-      (lambda: ExplicitRungeKutta.euler(0.01), 1e-2)]),
+     [(lambda: ExplicitRungeKutta.dop853().rtol(1e-12).atol(1e-12), 1e-3), (ExplicitRungeKutta.dopri5, 1e-3),
+      (lambda: ExplicitRungeKutta.rk4(0.01), 1e3), (lambda: ExplicitRungeKutta.euler(0.01), 1e-2)]),
     ("robertson_stiff_problem", Robertson(), 0.0, 0.4, [1.0, 0.0, 0.0],
      [0.9851721139, 3.3863953790e-05, 0.0147940222],
-     [(lambda: ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10), 5e-1)]),
+     [(lambda: ExplicitRungeKutta.dop853().rtol(1e-10).atol(1e-12), 1e-1),
+      (lambda: ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10), 5e-1)]),
 ]
 
 
@@ -55,7 +57,7 @@ def test_accuracy_rows(orc, case, pow_mode):
             assert abs(sol.y[c] - e) < tol, (name, c, sol.y[c], e)
 
 
-@pytest.mark.parametrize("make_method", [ExplicitRungeKutta.dopri5,
+@pytest.mark.parametrize("make_method", [ExplicitRungeKutta.dop853, ExplicitRungeKutta.dopri5,
                                          lambda: ExplicitRungeKutta.rk4(0.01)])
 def test_interpolation(orc, make_method):
     # tests/ode/interpolation.rs:22-72: y' = y on [0, 2], t_eval [0.5, 1.0, 1.69]
@@ -69,7 +71,7 @@ def test_interpolation(orc, make_method):
     assert abs(sol.dense[2] - math.exp(1.69)) < 1e-3
 
 
-@pytest.mark.parametrize("make_method", [ExplicitRungeKutta.dopri5,
+@pytest.mark.parametrize("make_method", [ExplicitRungeKutta.dop853, ExplicitRungeKutta.dopri5,
                                          lambda: ExplicitRungeKutta.rk4(0.1),
                                          lambda: ExplicitRungeKutta.euler(0.1)])
 def test_invalid_time_span(orc, make_method):
@@ -78,7 +80,8 @@ def test_invalid_time_span(orc, make_method):
     assert orc.solve_ode(ens).status[0] == DE_STATUS_BAD_INPUT
 
 
-@pytest.mark.parametrize("make_method", [lambda: ExplicitRungeKutta.dopri5().h0(10.0),
+@pytest.mark.parametrize("make_method", [lambda: ExplicitRungeKutta.dop853().h0(10.0),
+                                         lambda: ExplicitRungeKutta.dopri5().h0(10.0),
                                          lambda: ExplicitRungeKutta.rk4(10.0),
                                          lambda: ExplicitRungeKutta.euler(10.0)])
 def test_initial_step_size_too_big(orc, make_method):
