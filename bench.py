@@ -9,9 +9,11 @@ solve with inputs already resident in HBM.  The batched (persistent
 lane-refilling) schedule is the headline `value`; the plain one-thread-per-
 trajectory launch is reported beside it, with p50/p99 per-solve latency.
 
-Other workloads (--workload): spring (65 536 uneven spring chains, dense
-output), sde (OU-8 + GBM paths, Euler-Maruyama and Milstein), serve (Poisson
-request stream through the serving loop).
+Other workloads (--workload): events (the Lorenz ensemble with a Poincare
+section event z = rho - 1 per trajectory, and the crossing / hyperplane
+solouts), spring (65 536 uneven spring chains, dense output), sde (OU-8 + GBM
+paths, Euler-Maruyama and Milstein), serve (Poisson request stream through the
+serving loop).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -37,11 +39,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lorenz",
-                    choices=["lorenz", "spring", "sde", "serve"])
+                    choices=["lorenz", "events", "spring", "sde", "serve"])
     ap.add_argument("--n", type=int, default=0, help="override ensemble size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--only", choices=["persistent", "fifo", "plain", "fast"], default=None,
+    ap.add_argument("--only", choices=["persistent", "fifo", "plain", "fast", "event",
+                                       "crossing"], default=None,
                     help="run only one Lorenz variant (profiling)")
     ap.add_argument("--cpu-sample-stride", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
@@ -266,6 +269,132 @@ def cpu_baseline_lorenz(args, p, y0, n):
                       "%d accepted steps, median %.2f s wall of 3 runs on %d threads); C++ "
                       "restatement of the reference solver with the C library pow"
                       % (k, len(idx), int(sol.accepted.sum()), dt, threads)}
+
+
+# --------------------------------------------------------------- events ---
+def bench_events(args, dv):
+    """The Lorenz ensemble of the headline workload with event solouts:
+    IVP::event with g = z - (rho - 1) (the Poincare section through the two
+    non-trivial fixed points, one value per trajectory, both directions),
+    IVP::crossing(x = 0) and IVP::hyperplane_crossing; the same solve without
+    a solout is timed beside them for the overhead."""
+    import ctypes as C
+    import numpy as np
+    import deamd
+    from deamd import _abi
+    torch = dv.torch
+    n = args.n or (1 << 20)
+    cap = 128
+    p, y0 = deamd.gen_lorenz(n, seed=SEED)
+    d_p = dv.put(p.reshape(-1))
+    d_y0 = dv.put(y0.reshape(-1))
+    d_val = dv.put(np.ascontiguousarray(p[1] - 1.0))
+    d_pt = dv.put(np.array([0.0, 0.0, 25.0]))
+    d_nrm = dv.put(np.array([1.0, 1.0, 3.0]))
+    out = dict(y=dv.empty(3 * n, torch.float64), t=dv.empty(n, torch.float64),
+               acc=dv.empty(n, torch.int64), rej=dv.empty(n, torch.int64),
+               ev=dv.empty(n, torch.int64), st=dv.empty(n, torch.int32),
+               et=dv.empty(cap * n, torch.float64), ey=dv.empty(cap * 3 * n, torch.float64),
+               ne=dv.empty(n, torch.int32))
+    lib = deamd.lib()
+    stream = C.c_void_p(dv.stream.cuda_stream)
+
+    def descriptor(kind):
+        b = _abi.ode_defaults()
+        b.family = _abi.DE_FAMILY_LORENZ
+        b.method = _abi.DE_METHOD_DOPRI5
+        b.n, b.dim = n, 3
+        b.params = C.c_void_p(d_p.data_ptr())
+        b.y0 = C.c_void_p(d_y0.data_ptr())
+        b.t0_scalar, b.tf_scalar = 0.0, 50.0
+        b.rtol, b.atol, b.max_steps = 1e-8, 1e-10, 1000000
+        b.schedule = _abi.DE_SCHED_PERSISTENT
+        if kind == "event":
+            b.event_solout = _abi.DE_SOLOUT_EVENT
+            b.event_fn = _abi.DE_EVENT_COMPONENT
+            b.event_component = 2
+            b.event_value_traj = C.c_void_p(d_val.data_ptr())
+        elif kind == "crossing":
+            b.event_solout = _abi.DE_SOLOUT_CROSSING
+            b.event_component = 0
+        elif kind == "hyperplane":
+            b.event_solout = _abi.DE_SOLOUT_HYPERPLANE
+            b.event_point = C.c_void_p(d_pt.data_ptr())
+            b.event_normal = C.c_void_p(d_nrm.data_ptr())
+        b.max_events = cap if kind != "none" else 0
+        r = _abi.OdeResult()
+        r.y_final = C.c_void_p(out["y"].data_ptr())
+        r.t_final = C.c_void_p(out["t"].data_ptr())
+        r.accepted = C.c_void_p(out["acc"].data_ptr())
+        r.rejected = C.c_void_p(out["rej"].data_ptr())
+        r.evals = C.c_void_p(out["ev"].data_ptr())
+        r.status = C.c_void_p(out["st"].data_ptr())
+        if kind != "none":
+            r.event_t = C.c_void_p(out["et"].data_ptr())
+            r.event_y = C.c_void_p(out["ey"].data_ptr())
+            r.n_events = C.c_void_p(out["ne"].data_ptr())
+        return b, r
+
+    def run(kind, steps, warmup):
+        b, r = descriptor(kind)
+        for _ in range(warmup):
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+        dv.sync()
+        evs = [(dv.event(), dv.event()) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(dv.stream)
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+            e1.record(dv.stream)
+        dv.sync()
+        wall = time.perf_counter() - t0
+        lat = [e0.elapsed_time(e1) for e0, e1 in evs]
+        A = int(out["acc"].sum().item())
+        R = int(out["rej"].sum().item())
+        E = int(out["ne"].sum().item()) if kind != "none" else 0
+        ok = bool((out["st"] == 0).all().item())
+        ms = 1000.0 * wall / steps
+        return dict(value=A / (ms / 1000.0), ms_per_step=ms, kernel_ms=sum(lat) / len(lat),
+                    A=A, R=R, events=E, events_per_s=E / (ms / 1000.0), ok=ok)
+
+    if args.only in ("event", "crossing"):
+        r = run(args.only, args.steps, args.warmup)
+        return {"variant": args.only, "accepted": r["A"], "events": r["events"],
+                "kernel_ms": r["kernel_ms"]}
+    ev = run("event", args.steps, args.warmup)
+    base = run("none", max(3, args.steps // 2), 1)
+    cr = run("crossing", max(3, args.steps // 2), 1)
+    hp = run("hyperplane", max(3, args.steps // 2), 1)
+    flops = LORENZ_FLOP_ACC * ev["A"] + LORENZ_FLOP_REJ * ev["R"]
+    achieved = flops / (ev["kernel_ms"] / 1000.0) / 1e12
+    return {
+        "metric": "accepted steps*traj/s FP64 DOPRI5 with an event solout, 1 GPU",
+        "value": ev["value"],
+        "unit": "accepted_steps*traj/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ev["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Philox4x32-10 inputs, seed 0x5EED2026)",
+        "config": {"workload": "lorenz_event_dopri5_1M", "n_traj": n, "t_span": [0.0, 50.0],
+                   "rtol": 1e-8, "atol": 1e-10, "schedule": "persistent",
+                   "solout": "IVP::event, g = z - (rho - 1), both directions",
+                   "max_events": cap, "events_per_solve": ev["events"],
+                   "accepted_steps_per_solve": ev["A"], "all_complete": ev["ok"],
+                   "parallelism": "none (1 GPU)"},
+        "events_per_s": ev["events_per_s"],
+        "without_solout": {"value": base["value"], "ms_per_step": base["ms_per_step"],
+                           "event_overhead": ev["ms_per_step"] / base["ms_per_step"] - 1.0},
+        "crossing_x0": {"value": cr["value"], "ms_per_step": cr["ms_per_step"],
+                        "events_per_solve": cr["events"], "all_complete": cr["ok"]},
+        "hyperplane": {"value": hp["value"], "ms_per_step": hp["ms_per_step"],
+                       "events_per_solve": hp["events"], "all_complete": hp["ok"]},
+        "roofline": {"bound": "fp64_vector", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": load_traffic("lorenz_event_dopri5_1M"),
+                     "flop_model": "the step's 263 / 255 flops per accepted / rejected step; "
+                                   "event evaluation and root search not counted",
+                     "kernel_ms": ev["kernel_ms"]},
+    }
 
 
 # --------------------------------------------------------------- spring ---
@@ -555,6 +684,8 @@ def main():
         line, (p, y0, n) = bench_lorenz(args, dv)
         if not args.no_cpu_baseline and not args.only:
             line["cpu_baseline"] = cpu_baseline_lorenz(args, p, y0, n)
+    elif args.workload == "events":
+        line = bench_events(args, dv)
     elif args.workload == "spring":
         line = bench_spring(args, dv)
     elif args.workload == "sde":

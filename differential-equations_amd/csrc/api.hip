@@ -132,6 +132,30 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->schedule < DE_SCHED_AUTO || b->schedule > DE_SCHED_PERSISTENT_FIFO)
     return fail(DE_ERR_INVALID, "unknown schedule %d", b->schedule);
   if (b->max_steps < 0) return fail(DE_ERR_INVALID, "max_steps < 0");
+  if (b->event_solout < DE_SOLOUT_DEFAULT || b->event_solout > DE_SOLOUT_HYPERPLANE)
+    return fail(DE_ERR_INVALID, "unknown event solout %d", b->event_solout);
+  if (b->event_solout != DE_SOLOUT_DEFAULT) {
+    if (b->family == DE_FAMILY_SPRING_CHAIN)
+      return fail(DE_ERR_UNSUPPORTED, "event solouts are built for fixed-dimension families");
+    if (b->event_solout != DE_SOLOUT_EVENT && b->n_eval > 0)
+      return fail(DE_ERR_INVALID, "a crossing solout replaces the t_eval output (n_eval > 0)");
+    if (b->event_direction < DE_CROSS_BOTH || b->event_direction > DE_CROSS_NEGATIVE)
+      return fail(DE_ERR_INVALID, "unknown crossing direction %d", b->event_direction);
+    if (b->event_terminate < 0) return fail(DE_ERR_INVALID, "event_terminate < 0");
+    if (b->max_events < 0) return fail(DE_ERR_INVALID, "max_events < 0");
+    if (b->n > 0 && b->max_events > 0 && (!o->event_t || !o->event_y))
+      return fail(DE_ERR_INVALID, "max_events > 0 needs event_t and event_y");
+    const int fn = b->event_solout == DE_SOLOUT_CROSSING     ? DE_EVENT_COMPONENT
+                   : b->event_solout == DE_SOLOUT_HYPERPLANE ? DE_EVENT_PLANE
+                                                              : b->event_fn;
+    if (fn < DE_EVENT_COMPONENT || fn > DE_EVENT_PLANE)
+      return fail(DE_ERR_INVALID, "unknown event function %d", b->event_fn);
+    if (fn == DE_EVENT_COMPONENT && (b->event_component < 0 || b->event_component >= b->dim))
+      return fail(DE_ERR_INVALID, "event component %d outside [0, %d)", b->event_component,
+                  b->dim);
+    if (fn == DE_EVENT_PLANE && (!b->event_point || !b->event_normal))
+      return fail(DE_ERR_INVALID, "a plane event needs event_point and event_normal");
+  }
   return DE_OK;
 }
 
@@ -169,6 +193,21 @@ de::OdeArgs make_args(const de_ode_batch* b, const de_ode_result* o) {
   a.dense = b->n_eval > 0 ? o->dense : nullptr;
   a.n_dense = b->n_eval > 0 ? o->n_dense : nullptr;
   if (!a.dense) a.n_eval = 0;
+  a.ev.solout = b->event_solout;
+  a.ev.fn = b->event_solout == DE_SOLOUT_CROSSING     ? de::kEventComponent
+            : b->event_solout == DE_SOLOUT_HYPERPLANE ? de::kEventPlane
+                                                       : b->event_fn;
+  a.ev.comp = b->event_component;
+  a.ev.direction = b->event_direction;
+  a.ev.terminate = b->event_terminate;
+  a.ev.max_events = b->event_solout != DE_SOLOUT_DEFAULT ? b->max_events : 0;
+  a.ev.value = b->event_value;
+  a.ev.value_traj = b->event_value_traj;
+  a.ev.point = b->event_point;
+  a.ev.normal = b->event_normal;
+  a.ev.t_out = o->event_t;
+  a.ev.y_out = o->event_y;
+  a.ev.n_out = o->n_events;
   return a;
 }
 
@@ -191,14 +230,18 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
                               : hip_fail(hipGetLastError(), "spring launch");
     return DE_OK;
   }
-  if (sched == de::kSchedPersistent || sched == de::kSchedPersistentFifo) {
+  if (sched == de::kSchedPersistent || sched == de::kSchedPersistentFifo ||
+      b->event_solout != DE_SOLOUT_DEFAULT) {
     int rc = counter_slot(&a.counter);
     if (rc != DE_OK) return rc;
   }
   // Dense output needs no pre-fill: each trajectory writes every one of its
   // n_eval rows, with NaN for the points it does not produce.
+  // Event solouts run the parity kernels, with or without DE_FLAG_FAST_MATH.
   int r;
-  if (b->flags & DE_FLAG_FAST_MATH)
+  if (b->event_solout != DE_SOLOUT_DEFAULT)
+    r = de::parity::launch_ode_event(b->family, b->method, a, sched, s);
+  else if (b->flags & DE_FLAG_FAST_MATH)
     r = de::fast::launch_ode_lane(b->family, b->method, a, sched, s);
   else
     r = de::parity::launch_ode_lane(b->family, b->method, a, sched, s);
@@ -243,9 +286,10 @@ int64_t de_ode_workspace_bytes(const de_ode_batch* b) {
   // Work counter; for the persistent schedules the launch arguments in device
   // memory and, for the longest-first order, 256 bucket counts plus a 4-byte
   // position and a 1-byte bucket per trajectory.
+  const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
   const bool ordered = b->schedule == DE_SCHED_AUTO || b->schedule == DE_SCHED_PERSISTENT;
-  const bool persistent = ordered || b->schedule == DE_SCHED_PERSISTENT_FIFO;
-  return (int64_t)sizeof(unsigned long long) + (persistent ? 512 : 0) +
+  const bool persistent = ordered || b->schedule == DE_SCHED_PERSISTENT_FIFO || events;
+  return (int64_t)sizeof(unsigned long long) + (persistent ? 512 : 0) + (events ? 256 : 0) +
          (ordered ? 1024 + 5 * b->n : 0);
 }
 
@@ -276,7 +320,12 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   db.rtol_comp = st.in(b->rtol_comp, b->rtol_comp ? (size_t)state_len : 0);
   db.atol_comp = st.in(b->atol_comp, b->atol_comp ? (size_t)state_len : 0);
   db.t_eval = st.in(b->t_eval, b->n_eval);
+  const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
+  db.event_value_traj = st.in(b->event_value_traj, events && b->event_value_traj ? n : 0);
+  db.event_point = st.in(b->event_point, events && b->event_point ? b->dim : 0);
+  db.event_normal = st.in(b->event_normal, events && b->event_normal ? b->dim : 0);
   de_ode_result dr;
+  memset(&dr, 0, sizeof dr);
   dr.y_final = st.out(o->y_final, state_len);
   dr.t_final = st.out(o->t_final, n);
   dr.accepted = st.out(o->accepted, n);
@@ -286,6 +335,10 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   int64_t dense_len = (int64_t)b->n_eval * state_len;
   dr.dense = b->n_eval > 0 ? st.out(o->dense, dense_len) : nullptr;
   dr.n_dense = b->n_eval > 0 ? st.out(o->n_dense, n) : nullptr;
+  const int64_t ev_cap = events ? (int64_t)b->max_events * n : 0;
+  dr.event_t = st.out(o->event_t, ev_cap);
+  dr.event_y = st.out(o->event_y, ev_cap * b->dim);
+  dr.n_events = st.out(o->n_events, events ? n : 0);
   if (st.err != hipSuccess) return hip_fail(st.err, "staging inputs");
   rc = run_ode_device(&db, ragged ? b->sizes : nullptr, make_args(&db, &dr), s);
   if (rc != DE_OK) return rc;
@@ -299,6 +352,9 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   st.back(o->status, dr.status, n);
   if (dr.dense) st.back(o->dense, dr.dense, dense_len);
   if (dr.n_dense) st.back(o->n_dense, dr.n_dense, n);
+  if (dr.event_t) st.back(o->event_t, dr.event_t, ev_cap);
+  if (dr.event_y) st.back(o->event_y, dr.event_y, ev_cap * b->dim);
+  if (dr.n_events) st.back(o->n_events, dr.n_events, n);
   if (st.err != hipSuccess) return hip_fail(st.err, "copying results");
   return DE_OK;
 }

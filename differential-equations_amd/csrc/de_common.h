@@ -70,13 +70,17 @@ DE_HD double constrain_step_size(double h, double h_min, double h_max) {
 }
 
 // Per-trajectory status codes.  0..4 mirror Status::Complete and the Error
-// variants a solve can end with (src/error.rs:13-41); kRunning is internal.
+// variants a solve can end with (src/error.rs:13-41), 5 Status::Interrupted
+// (an event solout's Terminate), 6 a solout's unwrap of a failed
+// interpolation (the reference panics); kRunning is internal.
 enum Status : int32_t {
   kComplete = 0,
   kMaxSteps = 1,
   kStepSize = 2,
   kStiffness = 3,
   kBadInput = 4,
+  kInterrupted = 5,
+  kOutOfBounds = 6,
   kRunning = -1,
 };
 

@@ -1,0 +1,341 @@
+// Event solouts of one lane (device code), restated from the reference:
+//   EventWrappedSolout   src/solout/event.rs:320-523 (detect_event :335-393,
+//                        brent_dekker :395-490, solout :499-522)
+//   CrossingSolout       src/solout/crossing.rs:181-320
+//   HyperplaneCrossing   src/solout/hyperplane.rs:126-340
+// A lane calls pre() after every accepted step with the step's end point; when
+// it returns true the step needs the interpolant (a bracketed root), and the
+// lane builds it before calling post().  post() also runs when pre() returned
+// false, to advance the solout's state.
+//
+// Interpolants are passed as callables interp(t, out[D]) -> bool that return
+// false for the reference's Err(OutOfBounds) (t outside [t_prev, t] of the
+// step).  The reference's brent_dekker treats that as "no root" (`.ok()?`);
+// the other interpolations unwrap it, i.e. the reference panics, which ends
+// the trajectory here with kOutOfBounds.
+#pragma once
+
+#include "de_common.h"
+
+namespace de {
+
+enum : int32_t { kSoloutDefault = 0, kSoloutEvent = 1, kSoloutCrossing = 2, kSoloutHyperplane = 3 };
+enum : int32_t { kEventComponent = 0, kEventTime = 1, kEventPlane = 2 };
+enum : int32_t { kCrossBoth = 0, kCrossPositive = 1, kCrossNegative = 2 };
+
+struct EventArgs {
+  int32_t solout;      // kSolout*
+  int32_t fn;          // kEvent*: kEventComponent for crossings, kEventPlane for hyperplanes
+  int32_t comp;
+  int32_t direction;   // kCross*
+  int32_t terminate;   // EventConfig::terminate, 0 = None
+  int32_t max_events;  // points kept per trajectory
+  double value;
+  const double* value_traj;
+  const double* point;
+  const double* normal;  // the hyperplane solout's is normalised before the launch
+  double* t_out;         // [max_events * n]: t_out[k * n + i]
+  double* y_out;         // [(k * D + c) * n + i]
+  int32_t* n_out;        // [n] points pushed, or null
+};
+
+// g(t, y) of the event function: y[comp] - value, t - value, or
+// y.minus(&point).dot(&normal) (State::minus = plus_scaled(-1, ..), dot folds
+// from zero; src/traits.rs:336-354,551-555).
+template <int D>
+DE_HD double event_g(const EventArgs& e, double value, double t, const double* y) {
+  if (e.fn == kEventTime) return t - value;
+  if (e.fn == kEventComponent) {
+    // y[comp] as an OR of masked bit patterns: a select chain over y[c] is
+    // turned back into an indexed load by the compiler, which moves y from
+    // registers to scratch memory.
+    uint64_t bits = 0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) bits |= (e.comp == c ? ~0ull : 0ull) & as_u64(y[c]);
+    return as_f64(bits) - value;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) s = s + (y[c] + (-1.0) * e.point[c]) * e.normal[c];
+  return s;
+}
+
+// f64::signum(a) != f64::signum(b); a NaN signum differs from everything.
+DE_HD bool sign_differs(double a, double b) { return !(signum(a) == signum(b)); }
+
+// Writes point k of trajectory i when it fits.
+template <int D>
+DE_HD void event_store(const EventArgs& e, int64_t n, int64_t i, int32_t k, double t,
+                       const double* y) {
+  if (k < e.max_events) {
+    e.t_out[(int64_t)k * n + i] = t;
+#pragma unroll
+    for (int c = 0; c < D; ++c) e.y_out[((int64_t)k * D + c) * n + i] = y[c];
+  }
+}
+
+// Count and NaN for the slots past the last point.
+template <int D>
+DE_HD void event_finish(const EventArgs& e, int64_t n, int64_t i, int32_t pushed) {
+  if (e.n_out) e.n_out[i] = pushed;
+  for (int32_t k = pushed; k < e.max_events; ++k) {
+    e.t_out[(int64_t)k * n + i] = as_f64(kNaNBits);
+    for (int c = 0; c < D; ++c) e.y_out[((int64_t)k * D + c) * n + i] = as_f64(kNaNBits);
+  }
+}
+
+// EventWrappedSolout::brent_dekker (src/solout/event.rs:395-490) as written.
+// After every iteration it sets c = a (the previous iterate), so the bracket
+// is not maintained and the iteration can stop short of the root; the values
+// are the reference's.  interp(b, yb) false ends the search without a root.
+template <int D, class Interp, class G>
+DE_HD bool brent_dekker(double a, double b, double fa, double fb, Interp& interp, G& g,
+                        double* root) {
+  if (fabs(fa) < fabs(fb)) {
+    double x = a;
+    a = b;
+    b = x;
+    x = fa;
+    fa = fb;
+    fb = x;
+  }
+  double c = a, fc = fa;
+  double d = b - a;
+  double e = d;
+  for (int it = 0; it < 50; ++it) {
+    if (fb == 0.0) {
+      *root = b;
+      return true;
+    }
+    if (signum(fa) == signum(fb)) {
+      a = c;
+      fa = fc;
+      c = b;
+      fc = fb;
+      d = b - a;
+      e = d;
+    }
+    if (fabs(fa) < fabs(fb)) {
+      c = b;
+      b = a;
+      a = c;
+      fc = fb;
+      fb = fa;
+      fa = fc;
+    }
+    const double tol = fmax(1e-14, 1e-12 * fabs(b));
+    const double m = 0.5 * (a - b);
+    if (fabs(m) <= tol || fb == 0.0) {
+      *root = b;
+      return true;
+    }
+    bool bisect = true;
+    if (fabs(e) > tol && fabs(fa) > fabs(fb)) {
+      const double s = fb / fa;
+      double p, q;
+      if (a == c) {
+        p = 2.0 * m * s;
+        q = 1.0 - s;
+      } else {
+        const double q1 = fa / fc;
+        const double r = fb / fc;
+        p = s * (2.0 * m * q1 * (q1 - r) - (b - a) * (r - 1.0));
+        q = (q1 - 1.0) * (r - 1.0) * (s - 1.0);
+      }
+      if (q > 0.0) p = -p;
+      else q = -q;
+      if (fabs(2.0 * p) < (3.0 * m * q - fabs(tol * q)) && p < fabs(e * 0.5 * q)) {
+        e = d;
+        d = p / q;
+        bisect = false;
+      }
+    }
+    if (bisect) {
+      d = m;
+      e = m;
+    }
+    a = b;
+    fa = fb;
+    b = fabs(d) > tol ? b + d : b + (m > 0.0 ? tol : -tol);
+    double yb[D];
+    if (!interp(b, yb)) return false;
+    fb = g(b, yb);
+    c = a;
+    fc = fa;
+  }
+  return false;
+}
+
+// Event state of a lane; EV selects the solout at compile time:
+// 0 none, 1 EventWrappedSolout, 2 CrossingSolout / HyperplaneCrossingSolout.
+template <int D, int EV>
+struct LaneEvent {
+  DE_HD void reset() {}
+  DE_HD void start(const EventArgs&, int64_t, double, const double*) {}
+  DE_HD void finish(const EventArgs&, int64_t, int64_t) const {}
+};
+
+// EventWrappedSolout around DefaultSolout or TEvalSolout.
+template <int D>
+struct LaneEvent<D, 1> {
+  double value, last_g, g_new;
+  int32_t count, pushed;
+
+  DE_HD void reset() { pushed = 0; }
+  // The solout call after init: only records g(t0, y0) (event.rs:346-355).
+  DE_HD void start(const EventArgs& e, int64_t i, double t0, const double* y0) {
+    value = e.value_traj ? e.value_traj[i] : e.value;
+    last_g = event_g<D>(e, value, t0, y0);
+    count = 0;
+  }
+  // Sign change in the configured direction, with a bracket to search.
+  DE_HD bool pre(const EventArgs& e, double t_curr, const double* y_curr) {
+    g_new = event_g<D>(e, value, t_curr, y_curr);
+    const double g_prev = last_g;
+    const bool change = sign_differs(g_prev, g_new);
+    const bool ok = e.direction == kCrossPositive   ? change && g_prev < 0.0 && g_new >= 0.0
+                    : e.direction == kCrossNegative ? change && g_prev > 0.0 && g_new <= 0.0
+                                                    : change;
+    return ok && g_prev * g_new <= 0.0;
+  }
+  // detect_event after pre(): the root, the push unless within 1e-14 of the
+  // Solution's last time `tail` (NaN: no point yet), the terminate count.
+  // Returns -1 to continue, else the trajectory's final status.
+  template <class Interp>
+  DE_HD int32_t post(const EventArgs& e, int64_t n, int64_t i, double dir, bool need,
+                     double t_curr, double t_prev, double& tail, Interp& interp) {
+    int32_t st = -1;
+    if (need) {
+      double a = t_prev, b = t_curr, fa = last_g, fb = g_new;
+      if ((dir > 0.0 && a > b) || (dir < 0.0 && a < b)) {
+        double x = a;
+        a = b;
+        b = x;
+        x = fa;
+        fa = fb;
+        fb = x;
+      }
+      const EventArgs& ee = e;
+      const double v = value;
+      auto g = [&](double t, const double* y) { return event_g<D>(ee, v, t, y); };
+      double t_event;
+      if (brent_dekker<D>(a, b, fa, fb, interp, g, &t_event)) {
+        double ye[D];
+        if (!interp(t_event, ye)) {
+          st = kOutOfBounds;
+        } else {
+          if (tail != tail || fabs(t_event - tail) > 1e-14) {
+            event_store<D>(e, n, i, pushed, t_event, ye);
+            pushed += 1;
+            tail = t_event;
+          }
+          count += 1;
+          if (e.terminate > 0 && count >= e.terminate) st = kInterrupted;
+        }
+      }
+    }
+    last_g = g_new;
+    return st;
+  }
+  DE_HD void finish(const EventArgs& e, int64_t n, int64_t i) const {
+    event_finish<D>(e, n, i, pushed);
+  }
+};
+
+// CrossingSolout (offset y[comp] - threshold) and HyperplaneCrossingSolout
+// (signed distance to the plane, normal normalised), with their Newton
+// iterations (crossing.rs:253-320, hyperplane.rs:277-340).
+template <int D>
+struct LaneEvent<D, 2> {
+  double value, last, off_new;
+  int32_t pushed;
+
+  DE_HD void reset() { pushed = 0; }
+  DE_HD void start(const EventArgs& e, int64_t i, double t0, const double* y0) {
+    value = e.value_traj ? e.value_traj[i] : e.value;
+    last = event_g<D>(e, value, t0, y0);
+  }
+  DE_HD bool pre(const EventArgs& e, double t_curr, const double* y_curr) {
+    off_new = event_g<D>(e, value, t_curr, y_curr);
+    const double lo = last, off = off_new;
+    bool crossing = sign_differs(lo, off);
+    if (e.solout == kSoloutHyperplane)
+      crossing = crossing || (lo == 0.0 && off != 0.0) || (lo != 0.0 && off == 0.0);
+    const bool rec = e.direction == kCrossPositive   ? lo < 0.0 && off >= 0.0
+                     : e.direction == kCrossNegative ? lo > 0.0 && off <= 0.0
+                                                     : true;
+    return crossing && rec;
+  }
+  // find_crossing_newton; returns false when one of its unwrapped
+  // interpolations fails (the reference panics).
+  template <class Interp>
+  DE_HD bool newton(const EventArgs& e, double tl, double tu, double ol, double ou,
+                    Interp& interp, bool* found, double* t_out) const {
+    const bool hyper = e.solout == kSoloutHyperplane;
+    const double eps = 2.220446049250313080847e-16;
+    const double tolerance = eps * 100.0;
+    double t = tl - ol * (tu - tl) / (ou - ol);
+    double yt[D];
+    for (int it = 0; it < 10; ++it) {
+      if (!interp(t, yt)) return false;
+      const double offset = event_g<D>(e, value, t, yt);
+      if (fabs(offset) < tolerance) {
+        *found = true;
+        *t_out = t;
+        return true;
+      }
+      const double delta_t = (tu - tl) * 1e-6;
+      const double t_plus = t + delta_t;
+      double yp[D];
+      if (!interp(t_plus, yp)) return false;
+      const double derivative = (event_g<D>(e, value, t_plus, yp) - offset) / delta_t;
+      if (fabs(derivative) < (hyper ? eps : eps * 10.0)) break;
+      const double t_new = t - offset / derivative;
+      if (t_new < tl || t_new > tu) {
+        t = (tl + tu) / 2.0;
+      } else {
+        if (!hyper && fabs(t_new - t) < tolerance * 0.1) {
+          t = t_new;
+          break;
+        }
+        t = t_new;
+      }
+    }
+    if (!interp(t, yt)) return false;
+    *found = fabs(event_g<D>(e, value, t, yt)) < tolerance * 10.0;
+    *t_out = t;
+    return true;
+  }
+  template <class Interp>
+  DE_HD int32_t post(const EventArgs& e, int64_t n, int64_t i, double, bool need,
+                     double t_curr, double t_prev, double&, Interp& interp) {
+    int32_t st = -1;
+    if (need) {
+      const double lo = last, off = off_new;
+      bool found = false;
+      double tc = 0.0;
+      if (!newton(e, t_prev, t_curr, lo, off, interp, &found, &tc)) {
+        st = kOutOfBounds;
+      } else {
+        if (!found) {
+          const double frac = -lo / (off - lo);
+          tc = t_prev + frac * (t_curr - t_prev);
+        }
+        double yc[D];
+        if (!interp(tc, yc)) {
+          st = kOutOfBounds;
+        } else {
+          event_store<D>(e, n, i, pushed, tc, yc);
+          pushed += 1;
+        }
+      }
+    }
+    last = off_new;
+    return st;
+  }
+  DE_HD void finish(const EventArgs& e, int64_t n, int64_t i) const {
+    event_finish<D>(e, n, i, pushed);
+  }
+};
+
+}  // namespace de

@@ -9,6 +9,8 @@
 //   fixed ERK (rk4, euler)   src/methods/erk/fixed/ordinary.rs:16-138, Hermite
 //                            output src/interpolate.rs:40-60
 //   TEvalSolout              src/solout/t_eval.rs:83-134
+//   event solouts            de_event.h (src/solout/event.rs, crossing.rs,
+//                            hyperplane.rs)
 // Operation order is the reference's, with one difference: terms whose tableau
 // coefficient is exactly zero (a[6][1], b[1], b[6], er[1], all of bi[4]) and
 // the leading `0 +` of combinations that start from zeros_like() are omitted.
@@ -23,6 +25,7 @@
 
 #include "de_common.h"
 #include "de_dop853_table.h"
+#include "de_event.h"
 #include "de_families.h"
 #include "de_pow.h"
 
@@ -58,6 +61,7 @@ struct OdeArgs {
   const uint32_t* order;        // persistent schedule: trajectory at each position, or null
   const double* rtol_comp;      // per-component tolerances [c*n + i] (or state layout), or null
   const double* atol_comp;
+  EventArgs ev;                 // event solout (lanes built with EV != 0)
 };
 
 // Tolerances of one trajectory (Tolerance, src/tolerance.rs:11-68): a scalar
@@ -78,6 +82,7 @@ struct LaneTol<D, true> {
   DE_HD void load(const OdeArgs& A, int64_t i) {
     const double rs = A.rtol_traj ? A.rtol_traj[i] : A.rtol;
     const double as = A.atol_traj ? A.atol_traj[i] : A.atol;
+#pragma unroll
     for (int c = 0; c < D; ++c) {
       r[c] = A.rtol_comp ? A.rtol_comp[c * A.n + i] : rs;
       a[c] = A.atol_comp ? A.atol_comp[c * A.n + i] : as;
@@ -117,14 +122,20 @@ constexpr double e0 = 71.0 / 57600.0, e2 = -71.0 / 16695.0, e3 = 71.0 / 1920.0,
                  e4 = -17253.0 / 339200.0, e5 = 22.0 / 525.0, e6 = -1.0 / 40.0;
 }  // namespace dp
 
-// TEvalSolout over [t_prev, t_curr]; interp(te, out[D]) evaluates the step's
-// interpolant.  Writes dense[(e * D + c) * n + i]; a point the solout passes
-// over without producing it (the reference pushes nothing) gets NaN, as do
-// the points after the last one reached (teval_tail_nan at the end).
+// TEvalSolout over [t_prev, t_curr]; interp(te, out) evaluates the step's
+// interpolant and returns false for Err(OutOfBounds) (te outside the step,
+// which is every interior point of a backward step), on which the
+// reference's unwrap panics: teval_emit then returns false and the
+// trajectory ends with kOutOfBounds.  Writes dense[(e * D + c) * n + i]; a
+// point the solout passes over without producing it (the reference pushes
+// nothing) gets NaN, as do the points after the last one reached
+// (teval_tail_nan at the end).  `tail` is the time of the last point pushed.
 template <int D, class Interp>
-DE_HD void teval_emit(const OdeArgs& a, int64_t i, double dir, int32_t& next, int32_t& pushed,
-                      double t_curr, double t_prev, const double* y_curr, Interp&& interp) {
+DE_HD bool teval_emit(const OdeArgs& a, int64_t i, double dir, int32_t& next, int32_t& pushed,
+                      double t_curr, double t_prev, const double* y_curr, Interp&& interp,
+                      double& tail) {
   int32_t idx = next;
+  bool ok = true;
   while (idx < a.n_eval) {
     double te = a.t_eval[idx];
     bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
@@ -132,25 +143,32 @@ DE_HD void teval_emit(const OdeArgs& a, int64_t i, double dir, int32_t& next, in
     if (in_range) {
       double out[D];
       if (te == t_curr) {
+#pragma unroll
         for (int c = 0; c < D; ++c) out[c] = y_curr[c];
-      } else {
-        interp(te, out);
+      } else if (!interp(te, out)) {
+        ok = false;
+        break;
       }
+#pragma unroll
       for (int c = 0; c < D; ++c) a.dense[((int64_t)idx * D + c) * a.n + i] = out[c];
       pushed += 1;
+      tail = te;
       idx += 1;
     } else {
       if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+#pragma unroll
       for (int c = 0; c < D; ++c) a.dense[((int64_t)idx * D + c) * a.n + i] = as_f64(kNaNBits);
       idx += 1;
     }
   }
   next = idx;
+  return ok;
 }
 
 template <int D>
 DE_HD void teval_tail_nan(const OdeArgs& a, int64_t i, int32_t from) {
   for (int32_t e = from; e < a.n_eval; ++e)
+#pragma unroll
     for (int c = 0; c < D; ++c) a.dense[((int64_t)e * D + c) * a.n + i] = as_f64(kNaNBits);
 }
 
@@ -163,6 +181,7 @@ DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, con
   double f0[D], f1[D], sk[D], y1[D];
   f.diff(t0, y0, f0);
   double dnf = 0.0, dny = 0.0;
+#pragma unroll
   for (int c = 0; c < D; ++c) {
     sk[c] = tol.at(c) + tol.rt(c) * fabs(y0[c]);
     double u = f0[c] / sk[c];
@@ -173,9 +192,11 @@ DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, con
   double h = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
   h = fmin(h, h_max);
   h *= posneg;
+#pragma unroll
   for (int c = 0; c < D; ++c) y1[c] = y0[c] + h * f0[c];
   f.diff(t0 + h, y1, f1);
   double der2 = 0.0;
+#pragma unroll
   for (int c = 0; c < D; ++c) {
     double d = (f1[c] - f0[c]) / sk[c];
     der2 += d * d;
@@ -190,18 +211,22 @@ DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, con
 }
 
 // ------------------------------------------------------------------ DOPRI5 --
-// DENSE selects the t_eval output path and VTOL per-component tolerances at
-// compile time.  The attempt counter
+// DENSE selects the t_eval output path, VTOL per-component tolerances and EV
+// the event solout (de_event.h) at compile time.  The attempt counter
 // of the reference (`steps`, which counts attempts) is acc + rej at every
 // check, and evals = base + 6 rej + 7 acc on every exit (base = 3 with the
 // automatic initial step, 1 without), so neither is carried in the loop.
-template <class F, bool DENSE = false, bool VTOL = false>
+template <class F, bool DENSE = false, bool VTOL = false, int EV = 0>
 struct Dopri5Lane {
   static constexpr int D = F::kDim;
+  static constexpr int kEv = EV;
+  static constexpr int kWavesPerEu = 3;  // occupancy hint of the persistent kernel
   F f;
   double y[D], k0[D];
   double t, h, h_prev, tf, dir;
+  double tail;  // time of the Solution's last point (NaN: none), for the event solout
   LaneTol<D, VTOL> tol;
+  LaneEvent<D, EV> ev;
   int32_t acc, rej, base_ev, stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
 
@@ -209,6 +234,7 @@ struct Dopri5Lane {
   // and return the lane by value so that the step loop's copy of the lane
   // never has its address taken (it stays in registers).
   DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, Dopri5Lane L) {
+#pragma unroll
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
     a.t_final[i] = L.t;
     a.accepted[i] = L.acc;
@@ -218,6 +244,7 @@ struct Dopri5Lane {
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
+    if constexpr (EV != 0) L.ev.finish(a.ev, a.n, i);
   }
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
 
@@ -242,6 +269,7 @@ struct Dopri5Lane {
     double h0 = a.h0;
     if (h0 == 0.0 && tf != t0) {
       double y[D];
+#pragma unroll
       for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
       LaneTol<D, VTOL> tl;
       tl.load(a, i);
@@ -257,11 +285,13 @@ struct Dopri5Lane {
     double t0 = a.t0 ? a.t0[i] : a.t0s;
     tf = a.tf ? a.tf[i] : a.tfs;
     tol.load(a, i);
+#pragma unroll
     for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
     t = t0;
     acc = rej = 0;
     pushed = 0;
     next_eval = 0;
+    ev.reset();
     double s = signum(tf - t0);
     dir = s;
     bool ok = (s == 1.0 || s == -1.0);
@@ -283,10 +313,20 @@ struct Dopri5Lane {
     nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (DENSE && a.n_eval > 0) {
-      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y, [](double, double*) {});
-    }
+    solout_start(a, i);
     return true;
+  }
+
+  // The solout call after init (solve_ivp.rs:152-184): no interpolation, no
+  // event detection; the DefaultSolout pushes (t0, y0).
+  DE_HD void solout_start(const OdeArgs& a, int64_t i) {
+    tail = as_f64(kNaNBits);
+    if (DENSE && a.n_eval > 0)
+      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
+                    [](double, double*) { return false; }, tail);
+    else
+      tail = t;
+    if constexpr (EV != 0) ev.start(a.ev, i, t, y);
   }
 
   // One pass of the driver loop: overshoot clip, one DOPRI5 attempt, solout,
@@ -312,27 +352,33 @@ struct Dopri5Lane {
 
     double k1[D], k2[D], k3[D], k4[D], k5[D], k6[D], ys[D];
     const double a10h = dp::a10 * h;
+#pragma unroll
     for (int c = 0; c < D; ++c) ys[c] = y[c] + a10h * k0[c];
     f.diff(t + dp::c1 * h, ys, k1);
     const double a20h = dp::a20 * h, a21h = dp::a21 * h;
+#pragma unroll
     for (int c = 0; c < D; ++c) ys[c] = (y[c] + a20h * k0[c]) + a21h * k1[c];
     f.diff(t + dp::c2 * h, ys, k2);
     const double a30h = dp::a30 * h, a31h = dp::a31 * h, a32h = dp::a32 * h;
+#pragma unroll
     for (int c = 0; c < D; ++c) ys[c] = ((y[c] + a30h * k0[c]) + a31h * k1[c]) + a32h * k2[c];
     f.diff(t + dp::c3 * h, ys, k3);
     const double a40h = dp::a40 * h, a41h = dp::a41 * h, a42h = dp::a42 * h,
                  a43h = dp::a43 * h;
+#pragma unroll
     for (int c = 0; c < D; ++c)
       ys[c] = (((y[c] + a40h * k0[c]) + a41h * k1[c]) + a42h * k2[c]) + a43h * k3[c];
     f.diff(t + dp::c4 * h, ys, k4);
     const double a50h = dp::a50 * h, a51h = dp::a51 * h, a52h = dp::a52 * h,
                  a53h = dp::a53 * h, a54h = dp::a54 * h;
+#pragma unroll
     for (int c = 0; c < D; ++c)
       ys[c] = ((((y[c] + a50h * k0[c]) + a51h * k1[c]) + a52h * k2[c]) + a53h * k3[c]) +
               a54h * k4[c];
     f.diff(t + 1.0 * h, ys, k5);
     const double a60h = dp::a60 * h, a62h = dp::a62 * h, a63h = dp::a63 * h,
                  a64h = dp::a64 * h, a65h = dp::a65 * h;
+#pragma unroll
     for (int c = 0; c < D; ++c)
       ys[c] = ((((y[c] + a60h * k0[c]) + a62h * k2[c]) + a63h * k3[c]) + a64h * k4[c]) +
               a65h * k5[c];
@@ -340,6 +386,7 @@ struct Dopri5Lane {
 
     double yseg[D], ynew[D];
     double err = 0.0;
+#pragma unroll
     for (int c = 0; c < D; ++c) {
       yseg[c] = (((dp::b0 * k0[c] + dp::b2 * k2[c]) + dp::b3 * k3[c]) + dp::b4 * k4[c]) +
                 dp::b5 * k5[c];
@@ -366,6 +413,7 @@ struct Dopri5Lane {
     f.diff(t_new, ynew, dn);
     if (accept && stiff_check) {
       double stdnum = 0.0, stden = 0.0;
+#pragma unroll
       for (int c = 0; c < D; ++c) {
         double u = yseg[c] - k6[c];
         stdnum = c == 0 ? u * u : stdnum + u * u;
@@ -387,33 +435,42 @@ struct Dopri5Lane {
     // solve with (t, y) at the start of the step: the attempt is not taken.
     const bool stop = stiff == 15;
     const bool take = accept && !stop;
-    if (DENSE && take && a.n_eval > 0) {
-      // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
-      // ordinary.rs:312-333.
-      double cont1[D], cont2[D], cont3[D];
-      for (int c = 0; c < D; ++c) {
-        double ydiff = ynew[c] + (-1.0) * y[c];
-        cont1[c] = ydiff;
-        double bspl = h * k0[c] + (-1.0) * ydiff;
-        cont2[c] = bspl;
-        cont3[c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
-      }
-      const double hp = h, tp = t;
-      const double cont4 = 0.0 * h;
-      const double* y0p = y;
-      teval_emit<D>(a, i, dir, next_eval, pushed, t_new, t, ynew, [&](double te, double* out) {
-        double s = (te - tp) / hp;
-        double s1 = 1.0 - s;
+    int32_t sol = -1;  // solout outcome: -1 continue, else the final status
+    if constexpr (DENSE || EV != 0) {
+      if (take) {
+        // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
+        // ordinary.rs:312-333, on [t, t_new] (ordinary.rs:303-309).
+        double cont1[D], cont2[D], cont3[D];
+#pragma unroll
         for (int c = 0; c < D; ++c) {
-          double acc_ = cont4 * s1 + cont3[c];
-          acc_ = acc_ * s + cont2[c];
-          acc_ = acc_ * s1 + cont1[c];
-          out[c] = y0p[c] + s * acc_;
+          double ydiff = ynew[c] + (-1.0) * y[c];
+          cont1[c] = ydiff;
+          double bspl = h * k0[c] + (-1.0) * ydiff;
+          cont2[c] = bspl;
+          cont3[c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
         }
-      });
+        const double hp = h, tp = t;
+        const double cont4 = 0.0 * h;
+        const double* y0p = y;
+        auto interp = [&](double te, double* out) -> bool {
+          if (te < tp || te > t_new) return false;
+          double s = (te - tp) / hp;
+          double s1 = 1.0 - s;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            double acc_ = cont4 * s1 + cont3[c];
+            acc_ = acc_ * s + cont2[c];
+            acc_ = acc_ * s1 + cont1[c];
+            out[c] = y0p[c] + s * acc_;
+          }
+          return true;
+        };
+        sol = solout_step(a, i, t_new, tp, ynew, interp);
+      }
     }
     h_prev = take ? h : h_prev;
     t = take ? t_new : t;
+#pragma unroll
     for (int c = 0; c < D; ++c) {
       y[c] = take ? ynew[c] : y[c];
       k0[c] = take ? dn[c] : k0[c];
@@ -424,9 +481,37 @@ struct Dopri5Lane {
     acc += take ? 1 : 0;
     rej += accept ? 0 : 1;
     const int32_t result =
-        stop ? kStiffness : (take && fabs(tf - t) <= kTenEps ? kComplete : -1);
+        stop ? kStiffness
+             : (sol >= 0 ? sol : (take && fabs(tf - t) <= kTenEps ? kComplete : -1));
     h = constrain_step_size(h * scale, a.h_min, a.h_max);
     return result;
+  }
+
+  // The solout chain after an accepted step (solve_ivp.rs:242-259): TEvalSolout
+  // or DefaultSolout, then the event detection of EventWrappedSolout
+  // (event.rs:499-522); or the crossing solout alone.
+  template <class Interp>
+  DE_HD int32_t solout_step(const OdeArgs& a, int64_t i, double t_new, double tp,
+                            const double* ynew, Interp& interp) {
+    int32_t sol = -1;
+    if constexpr (EV == 2) {
+      const bool need = ev.pre(a.ev, t_new, ynew);
+      sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
+    } else {
+      if (DENSE && a.n_eval > 0) {
+        if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
+          sol = kOutOfBounds;
+      } else {
+        tail = t_new;
+      }
+      if constexpr (EV == 1) {
+        if (sol < 0) {
+          const bool need = ev.pre(a.ev, t_new, ynew);
+          sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
+        }
+      }
+    }
+    return sol;
   }
 
   // step() + finish(): returns true when the trajectory ended.
@@ -458,17 +543,24 @@ constexpr double ER[12] = DE_DOP853_ER;
 constexpr double BI[16][16] = DE_DOP853_BI;
 }  // namespace d8
 
-template <class F, bool DENSE = false, bool VTOL = false>
+template <class F, bool DENSE = false, bool VTOL = false, int EV = 0>
 struct Dop853Lane {
   static constexpr int D = F::kDim;
+  static constexpr int kEv = EV;
+  // Occupancy hint: with an event solout the 16-stage lane does not fit the
+  // 168 registers of three waves per SIMD without spilling in the step.
+  static constexpr int kWavesPerEu = EV != 0 ? 2 : 3;
   F f;
   double y[D], k0[D];
   double t, h, h_prev, tf, dir;
+  double tail;
   LaneTol<D, VTOL> tol;
+  LaneEvent<D, EV> ev;
   int32_t acc, rej, base_ev, stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
 
   DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, Dop853Lane L) {
+#pragma unroll
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
     a.t_final[i] = L.t;
     a.accepted[i] = L.acc;
@@ -478,6 +570,7 @@ struct Dop853Lane {
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
+    if constexpr (EV != 0) L.ev.finish(a.ev, a.n, i);
   }
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
 
@@ -500,6 +593,7 @@ struct Dop853Lane {
     double h0 = a.h0;
     if (h0 == 0.0 && tf != t0) {
       double y[D];
+#pragma unroll
       for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
       LaneTol<D, VTOL> tl;
       tl.load(a, i);
@@ -513,11 +607,13 @@ struct Dop853Lane {
     double t0 = a.t0 ? a.t0[i] : a.t0s;
     tf = a.tf ? a.tf[i] : a.tfs;
     tol.load(a, i);
+#pragma unroll
     for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
     t = t0;
     acc = rej = 0;
     pushed = 0;
     next_eval = 0;
+    ev.reset();
     double s = signum(tf - t0);
     dir = s;
     bool ok = (s == 1.0 || s == -1.0);
@@ -539,9 +635,13 @@ struct Dop853Lane {
     nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (DENSE && a.n_eval > 0) {
-      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y, [](double, double*) {});
-    }
+    tail = as_f64(kNaNBits);
+    if (DENSE && a.n_eval > 0)
+      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
+                    [](double, double*) { return false; }, tail);
+    else
+      tail = t;
+    if constexpr (EV != 0) ev.start(a.ev, i, t, y);
     return true;
   }
 
@@ -608,7 +708,7 @@ struct Dop853Lane {
     const bool stiff_check = countdown == 0;
     if (stiff_check) countdown = 100;
 
-    double k[DENSE ? 16 : 12][D];
+    double k[(DENSE || EV != 0) ? 16 : 12][D];
 #pragma unroll
     for (int c = 0; c < D; ++c) k[0][c] = k0[c];
     double ys[D];
@@ -646,6 +746,7 @@ struct Dop853Lane {
     f.diff(t_new, ynew, dn);
     if (accept && stiff_check) {
       double stdnum = 0.0, stden = 0.0;
+#pragma unroll
       for (int c = 0; c < D; ++c) {
         double u = yseg[c] - k[11][c];
         stdnum = c == 0 ? u * u : stdnum + u * u;
@@ -665,32 +766,42 @@ struct Dop853Lane {
     }
     const bool stop = stiff == 15;
     const bool take = accept && !stop;
-    if constexpr (DENSE) {
-      if (take && a.n_eval > 0) {
-        // cont[0..3] as DOPRI5; k[12] = f(t_new, y_new); k[13..15] at c[13..15];
-        // cont[i] = h * sum_j bi[i][j] k[j] for i = 4..7.
+    int32_t sol = -1;
+    if constexpr (DENSE || EV != 0) {
+      if (take) {
+        // The continuous extension is built only when a t_eval point or an
+        // event bracket needs it: cont[0..3] as DOPRI5; k[12] = f(t_new,
+        // y_new); k[13..15] at c[13..15]; cont[i] = h sum_j bi[i][j] k[j] for
+        // i = 4..7.
+        bool need = false;
+        if constexpr (EV != 0) need = ev.pre(a.ev, t_new, ynew);
+        const bool tev = DENSE && EV != 2 && a.n_eval > 0;
         double cont[8][D];
+        if (tev || need) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) {
-          double ydiff = ynew[c] + (-1.0) * y[c];
-          cont[0][c] = y[c];
-          cont[1][c] = ydiff;
-          double bspl = h * k[0][c] + (-1.0) * ydiff;
-          cont[2][c] = bspl;
-          cont[3][c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
-          k[12][c] = dn[c];
+          for (int c = 0; c < D; ++c) {
+            double ydiff = ynew[c] + (-1.0) * y[c];
+            cont[0][c] = y[c];
+            cont[1][c] = ydiff;
+            double bspl = h * k[0][c] + (-1.0) * ydiff;
+            cont[2][c] = bspl;
+            cont[3][c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
+            k[12][c] = dn[c];
+          }
+          dense_stages<13>(k, y, t, h);
+#pragma unroll
+          for (int r = 4; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) cont[r][c] = comb<16>(d8::BI[r], k, c) * h;
         }
-        dense_stages<13>(k, y, t, h);
-#pragma unroll
-        for (int r = 4; r < 8; ++r)
-#pragma unroll
-          for (int c = 0; c < D; ++c) cont[r][c] = comb<16>(d8::BI[r], k, c) * h;
         const double hp = h, tp = t;
-        teval_emit<D>(a, i, dir, next_eval, pushed, t_new, t, ynew, [&](double te, double* out) {
-          // ordinary.rs:318-333 with 8 coefficients: factors s1, s, s1 for
+        auto interp = [&](double te, double* out) -> bool {
+          // ordinary.rs:303-333 with 8 coefficients: factors s1, s, s1 for
           // cont[6], cont[5], cont[4], then s1, s, s1 for cont[3..1].
+          if (te < tp || te > t_new) return false;
           const double s = (te - tp) / hp;
           const double s1 = 1.0 - s;
+#pragma unroll
           for (int c = 0; c < D; ++c) {
             double q = cont[7][c];
             q = q * s1 + cont[6][c];
@@ -701,7 +812,21 @@ struct Dop853Lane {
             q = q * s1 + cont[1][c];
             out[c] = cont[0][c] + s * q;
           }
-        });
+          return true;
+        };
+        if constexpr (EV == 2) {
+          sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
+        } else {
+          if (tev) {
+            if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
+              sol = kOutOfBounds;
+          } else {
+            tail = t_new;
+          }
+          if constexpr (EV == 1) {
+            if (sol < 0) sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
+          }
+        }
       }
     }
     h_prev = take ? h : h_prev;
@@ -716,7 +841,8 @@ struct Dop853Lane {
     acc += take ? 1 : 0;
     rej += accept ? 0 : 1;
     const int32_t result =
-        stop ? kStiffness : (take && fabs(tf - t) <= kTenEps ? kComplete : -1);
+        stop ? kStiffness
+             : (sol >= 0 ? sol : (take && fabs(tf - t) <= kTenEps ? kComplete : -1));
     h = constrain_step_size(h * scale, a.h_min, a.h_max);
     return result;
   }
@@ -732,16 +858,21 @@ struct Dop853Lane {
 // --------------------------------------------------- fixed-step rk4 / euler --
 // RK4: c = (0, 1/2, 1/2, 1), a10 = a21 = 1/2, a32 = 1, b = (1/6, 1/3, 1/3, 1/6)
 // (src/tableau/runge_kutta.rs:39-70); euler: b = (1) (:289-302).
-template <class F, int STAGES>
+template <class F, int STAGES, int EV = 0>
 struct FixedLane {
   static constexpr int D = F::kDim;
+  static constexpr int kEv = EV;
+  static constexpr int kWavesPerEu = 3;  // occupancy hint of the persistent kernel
   F f;
   double y[D], k0[D];
   double t, h, tf, dir;
+  double tail;
   int64_t steps, acc, ev;
+  LaneEvent<D, EV> evs;
   int32_t next_eval, pushed;
 
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const {
+#pragma unroll
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = y[c];
     a.t_final[i] = t;
     a.accepted[i] = acc;
@@ -750,6 +881,7 @@ struct FixedLane {
     a.status[i] = st;
     if (a.n_dense) a.n_dense[i] = pushed;
     if (a.n_eval > 0) teval_tail_nan<D>(a, i, next_eval);
+    if constexpr (EV != 0) evs.finish(a.ev, a.n, i);
   }
 
   DE_HD bool begin(const OdeArgs& a, int64_t i) { return init(a, i); }
@@ -766,11 +898,13 @@ struct FixedLane {
     f.load(a.params, a.n, i);
     double t0 = a.t0 ? a.t0[i] : a.t0s;
     tf = a.tf ? a.tf[i] : a.tfs;
+#pragma unroll
     for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
     t = t0;
     acc = ev = 0;
     pushed = 0;
     next_eval = 0;
+    evs.reset();
     double s = signum(tf - t0);
     dir = s;
     bool ok = (s == 1.0 || s == -1.0);
@@ -785,7 +919,13 @@ struct FixedLane {
     f.diff(t, y, k0);
     ev = 1;
     steps = 0;
-    if (a.n_eval > 0) teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y, [](double, double*) {});
+    tail = as_f64(kNaNBits);
+    if (EV != 2 && a.n_eval > 0)
+      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
+                    [](double, double*) { return false; }, tail);
+    else
+      tail = t;
+    if constexpr (EV != 0) evs.start(a.ev, i, t, y);
     return true;
   }
 
@@ -802,19 +942,24 @@ struct FixedLane {
     if (STAGES == 4) {
       double k1[D], k2[D], k3[D], ys[D];
       const double hh = 0.5 * h;
+#pragma unroll
       for (int c = 0; c < D; ++c) ys[c] = y[c] + hh * k0[c];
       f.diff(t + 0.5 * h, ys, k1);
+#pragma unroll
       for (int c = 0; c < D; ++c) ys[c] = y[c] + hh * k1[c];
       f.diff(t + 0.5 * h, ys, k2);
       const double h1 = 1.0 * h;
+#pragma unroll
       for (int c = 0; c < D; ++c) ys[c] = y[c] + h1 * k2[c];
       f.diff(t + 1.0 * h, ys, k3);
       const double b0h = (1.0 / 6.0) * h, b1h = (1.0 / 3.0) * h;
+#pragma unroll
       for (int c = 0; c < D; ++c)
         ynext[c] = (((y[c] + b0h * k0[c]) + b1h * k1[c]) + b1h * k2[c]) + b0h * k3[c];
       ev += 3;
     } else {
       const double b0h = 1.0 * h;
+#pragma unroll
       for (int c = 0; c < D; ++c) ynext[c] = y[c] + b0h * k0[c];
     }
     double tp = t, hp = h;
@@ -823,12 +968,15 @@ struct FixedLane {
     f.diff(t, ynext, dn);
     ev += 1;
     acc += 1;
-    if (a.n_eval > 0) {
+    int32_t sol = -1;
+    if (EV != 0 || a.n_eval > 0) {
       const double* yp = y;
       const double* kp = k0;
       const double tc = t;
-      teval_emit<D>(a, i, dir, next_eval, pushed, t, tp, ynext, [&](double te, double* out) {
+      auto interp = [&](double te, double* out) -> bool {
         // cubic_hermite_interpolate(t_prev, t, y_prev, y, dydt_prev, dydt, te)
+        // on [t_prev, t] (fixed/ordinary.rs:170-177).
+        if (te < tp || te > tc) return false;
         double hh = tc - tp;
         double s = (te - tp) / hh;
         double s2 = s * s;
@@ -838,15 +986,36 @@ struct FixedLane {
         double h01 = -2.0 * s3 + 3.0 * s2;
         double h11 = s3 - s2;
         double w10 = h10 * hh, w11 = h11 * hh;
+#pragma unroll
         for (int c = 0; c < D; ++c)
           out[c] = ((h00 * yp[c] + w10 * kp[c]) + h01 * ynext[c]) + w11 * dn[c];
-      });
+        return true;
+      };
+      if constexpr (EV == 2) {
+        const bool need = evs.pre(a.ev, t, ynext);
+        sol = evs.post(a.ev, a.n, i, dir, need, t, tp, tail, interp);
+      } else {
+        if (a.n_eval > 0) {
+          if (!teval_emit<D>(a, i, dir, next_eval, pushed, t, tp, ynext, interp, tail))
+            sol = kOutOfBounds;
+        } else {
+          tail = t;
+        }
+        if constexpr (EV == 1) {
+          if (sol < 0) {
+            const bool need = evs.pre(a.ev, t, ynext);
+            sol = evs.post(a.ev, a.n, i, dir, need, t, tp, tail, interp);
+          }
+        }
+      }
     }
     (void)hp;
+#pragma unroll
     for (int c = 0; c < D; ++c) {
       y[c] = ynext[c];
       k0[c] = dn[c];
     }
+    if (sol >= 0) return sol;
     if (fabs(tf - t) <= kTenEps) return kComplete;
     return -1;
   }

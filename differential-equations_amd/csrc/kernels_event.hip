@@ -1,0 +1,53 @@
+// Lane-per-trajectory ODE kernels with an event solout (parity build only).
+//
+// IVP::event (EventWrappedSolout around the final-state or t_eval output) and
+// IVP::crossing / ::hyperplane_crossing (src/ivp.rs:662-705) for every
+// fixed-dimension family and method.  The lanes are de_lane.h's with EV = 1
+// (Brent-Dekker event) or EV = 2 (Newton crossing); they carry per-component
+// tolerances (VTOL), which also cover the scalar case.  Events always run on
+// the persistent schedule: a PLAIN request runs as PERSISTENT_FIFO (same
+// results, trajectories in index order).
+#include <hip/hip_runtime.h>
+
+#define DE_KNS parity
+#include "de_ode_sched.h"
+
+namespace de {
+namespace parity {
+
+template <class F, int EV>
+static int launch_event_lane(int method, const OdeArgs& a, int schedule, hipStream_t s) {
+  switch (method) {
+    case kMethodDopri5: return launch<Dopri5Lane<F, EV == 1, true, EV>>(a, schedule, s);
+    case kMethodDop853: return launch<Dop853Lane<F, EV == 1, true, EV>>(a, schedule, s);
+    case kMethodRk4: return launch<FixedLane<F, 4, EV>>(a, schedule, s);
+    case kMethodEuler: return launch<FixedLane<F, 1, EV>>(a, schedule, s);
+    default: return 1;
+  }
+}
+
+template <class F>
+static int launch_event_family(int method, const OdeArgs& a, int schedule, hipStream_t s) {
+  if (a.ev.solout == kSoloutEvent) return launch_event_lane<F, 1>(method, a, schedule, s);
+  if (a.ev.solout == kSoloutCrossing || a.ev.solout == kSoloutHyperplane)
+    return launch_event_lane<F, 2>(method, a, schedule, s);
+  return 1;
+}
+
+// Returns 0 on success, 1 if the combination is not built, -1 on HIP error.
+int launch_ode_event(int family, int method, const OdeArgs& a, int schedule, hipStream_t s) {
+  if (schedule == kSchedPlain) schedule = kSchedPersistentFifo;
+  switch (family) {
+    case kLorenz: return launch_event_family<LorenzRhs>(method, a, schedule, s);
+    case kVanDerPol: return launch_event_family<VanDerPolRhs>(method, a, schedule, s);
+    case kExponential: return launch_event_family<ExponentialRhs>(method, a, schedule, s);
+    case kLinear: return launch_event_family<LinearRhs>(method, a, schedule, s);
+    case kHarmonic: return launch_event_family<HarmonicRhs>(method, a, schedule, s);
+    case kLogistic: return launch_event_family<LogisticRhs>(method, a, schedule, s);
+    case kRobertson: return launch_event_family<RobertsonRhs>(method, a, schedule, s);
+    default: return 1;
+  }
+}
+
+}  // namespace parity
+}  // namespace de

@@ -5,19 +5,8 @@
 // -DDE_FAST_BUILD -ffp-contract=fast, into namespace de::fast (FMA-contracted;
 // same algorithm, different rounding).
 //
-// Two schedules:
-//   plain      one thread per trajectory; a wave lives until its slowest lane
-//              finishes.
-//   persistent a grid sized to the resident wave slots; each lane integrates a
-//              trajectory and, when it finishes, takes the next position from a
-//              global counter (one atomic per wave, ballot-aggregated), so
-//              lanes stay busy while step counts vary ~10x across trajectories.
-//              Positions map to trajectories longest-predicted first (see
-//              launch_order) or, for PERSISTENT_FIFO, in index order.
+// The plain and persistent schedules are in de_ode_sched.h.
 #include <hip/hip_runtime.h>
-
-#include "de_lane.h"
-#include "de_launch.h"
 
 #if defined(DE_FAST_BUILD)
 #define DE_KNS fast
@@ -25,202 +14,10 @@
 #define DE_KNS parity
 #endif
 
+#include "de_ode_sched.h"
+
 namespace de {
 namespace DE_KNS {
-
-constexpr int kBlock = 256;
-constexpr int64_t kOrderMinN = 4096;
-
-template <class Lane>
-__global__ __launch_bounds__(kBlock) void ode_plain_kernel(OdeArgs a) {
-  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
-  Lane L;
-  if (!L.begin(a, i)) return;
-  int32_t st;
-  do {
-    st = L.step(a, i);
-  } while (st < 0);
-  L.finish(a, i, st);
-}
-
-// Three waves per SIMD: the DOPRI5 lane needs about 165 VGPRs, and the
-// occupancy hint keeps the allocation under the 168-register limit for 3.
-// The arguments live in device memory and are read through a restrict
-// pointer: the fields used only when a trajectory starts or ends are then
-// loaded inside those branches instead of occupying SGPRs for the whole loop,
-// which leaves SGPRs for the step's constants.
-template <class Lane>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void
-ode_persistent_kernel(const OdeArgs* __restrict__ ap) {
-  const OdeArgs& a = *ap;
-  Lane L;
-  bool have = false;
-  bool drained = false;
-  int64_t idx = 0;
-  const int lane = __lane_id();
-  const unsigned long long n = (unsigned long long)a.n;
-  for (;;) {
-    if (!drained) {
-      bool need = !have;
-      unsigned long long mask = __ballot(need);
-      if (mask) {
-        int leader = __ffsll((long long)mask) - 1;
-        int cnt = __popcll(mask);
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)cnt);
-        base = __shfl(base, leader);
-        if (base + (unsigned long long)cnt >= n) drained = true;
-        if (need) {
-          unsigned long long my = base + (unsigned long long)__popcll(mask & __lanemask_lt());
-          if (my < n) {
-            idx = a.order ? (int64_t)a.order[my] : (int64_t)my;
-            have = L.begin(a, idx);
-          }
-        }
-      }
-    }
-    if (!__any(have)) {
-      if (drained) break;
-      continue;
-    }
-    if (have) {
-      int32_t st = L.step(a, idx);
-      if (st >= 0) {
-        L.finish(a, idx, st);
-        have = false;
-      }
-    }
-  }
-}
-
-__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ull; }
-
-__global__ void store_args_kernel(OdeArgs a, OdeArgs* dst) { *dst = a; }
-
-// ---- Longest-first order for the persistent schedule ----------------------
-// A counting sort of trajectories by cost_bucket(predicted_cost): the resident
-// lanes then start the longest trajectories first and finish on the short
-// ones, so few lanes idle while their wave waits for its last trajectory.
-// Order within a bucket is arbitrary (it follows atomic arrival); results do
-// not depend on it, each trajectory being solved on its own.
-constexpr int kBuckets = 256;
-static_assert(kBlock == kBuckets, "cost_kernel and scatter_kernel use one thread per bucket");
-
-__global__ __launch_bounds__(kBuckets) void zero_hist_kernel(uint32_t* hist) {
-  hist[threadIdx.x] = 0u;
-}
-
-template <class Lane>
-__global__ __launch_bounds__(kBlock) void cost_kernel(OdeArgs a, uint8_t* bucket,
-                                                      uint32_t* hist) {
-  __shared__ uint32_t h[kBuckets];
-  h[threadIdx.x] = 0u;  // kBlock == kBuckets
-  __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < a.n) {
-    const uint32_t b = cost_bucket(Lane::predicted_cost(a, i));
-    bucket[i] = (uint8_t)b;
-    atomicAdd(&h[b], 1u);
-  }
-  __syncthreads();
-  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
-}
-
-// Exclusive prefix sum of the 256 counts, in place (one block).
-__global__ __launch_bounds__(kBuckets) void scan_hist_kernel(uint32_t* hist) {
-  __shared__ uint32_t v[kBuckets];
-  const int t = threadIdx.x;
-  v[t] = hist[t];
-  __syncthreads();
-  for (int d = 1; d < kBuckets; d <<= 1) {
-    const uint32_t add = t >= d ? v[t - d] : 0u;
-    __syncthreads();
-    v[t] += add;
-    __syncthreads();
-  }
-  hist[t] = v[t] - hist[t];
-}
-
-__global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t n, const uint8_t* bucket,
-                                                         uint32_t* offset, uint32_t* order) {
-  __shared__ uint32_t cnt[kBuckets], base[kBuckets];
-  cnt[threadIdx.x] = 0u;
-  __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  uint32_t b = 0u, local = 0u;
-  if (i < n) {
-    b = bucket[i];
-    local = atomicAdd(&cnt[b], 1u);
-  }
-  __syncthreads();
-  if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&offset[threadIdx.x], cnt[threadIdx.x]);
-  __syncthreads();
-  if (i < n) order[base[b] + local] = (uint32_t)i;
-}
-
-// Fills a.order (n entries) using `bucket` (n bytes) and `hist` (256 words).
-template <class Lane>
-static void launch_order(const OdeArgs& a, uint8_t* bucket, uint32_t* hist, uint32_t* order,
-                         hipStream_t s) {
-  const unsigned grid = (unsigned)((a.n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(zero_hist_kernel, dim3(1), dim3(kBuckets), 0, s, hist);
-  hipLaunchKernelGGL(cost_kernel<Lane>, dim3(grid), dim3(kBlock), 0, s, a, bucket, hist);
-  hipLaunchKernelGGL(scan_hist_kernel, dim3(1), dim3(kBuckets), 0, s, hist);
-  hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kBlock), 0, s, a.n, bucket, hist, order);
-}
-
-template <class Lane>
-static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
-  if (a.n <= 0) return 0;
-  if (schedule == kSchedPlain) {
-    int64_t grid = (a.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(ode_plain_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-  }
-  static int blocks_per_cu = -1;
-  static int num_cu = 0;
-  if (blocks_per_cu < 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return -1;
-    if (hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      return -1;
-    int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ode_persistent_kernel<Lane>, kBlock, 0) !=
-        hipSuccess)
-      return -1;
-    blocks_per_cu = b > 0 ? b : 1;
-  }
-  int64_t want = (int64_t)num_cu * blocks_per_cu;
-  int64_t need = (a.n + kBlock - 1) / kBlock;
-  int64_t grid = want < need ? want : need;
-  // Stream-ordered workspace: the launch arguments and, for the longest-first
-  // order (schedule PERSISTENT, batches of a few thousand trajectories or
-  // more), 256 bucket counts, the order and one bucket byte per trajectory.
-  const bool ordered = schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL;
-  const size_t args_bytes = (sizeof(OdeArgs) + 255) & ~(size_t)255;
-  const size_t bytes =
-      args_bytes + (ordered ? kBuckets * sizeof(uint32_t) + (size_t)a.n * (sizeof(uint32_t) + 1)
-                            : 0);
-  void* work = nullptr;
-  if (hipMallocAsync(&work, bytes, s) != hipSuccess) return -1;
-  OdeArgs* d_args = (OdeArgs*)work;
-  OdeArgs pa = a;
-  pa.order = nullptr;
-  if (ordered) {
-    uint32_t* hist = (uint32_t*)((char*)work + args_bytes);
-    uint32_t* order = hist + kBuckets;
-    uint8_t* bucket = (uint8_t*)(order + a.n);
-    launch_order<Lane>(a, bucket, hist, order, s);
-    pa.order = order;
-  }
-  hipLaunchKernelGGL(store_args_kernel, dim3(1), dim3(1), 0, s, pa, d_args);
-  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-  hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s,
-                     (const OdeArgs*)d_args);
-  if (hipFreeAsync(work, s) != hipSuccess) return -1;
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
 
 template <class F>
 static int launch_method(int method, const OdeArgs& a, int schedule, hipStream_t s) {

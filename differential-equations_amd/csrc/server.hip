@@ -680,9 +680,11 @@ int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
       b->family != DE_FAMILY_SPRING_CHAIN)
     return DE_ERR_UNSUPPORTED;
   if (b->n <= 0 || !b->y0 || (b->n_eval > 0 && !b->t_eval)) return DE_ERR_INVALID;
-  // Served requests carry scalar or per-trajectory tolerances; per-component
-  // vectors go through de_ode_batch_solve.
+  // Served requests carry scalar or per-trajectory tolerances and the final
+  // state / t_eval output; per-component vectors and event solouts go through
+  // de_ode_batch_solve.
   if (b->rtol_comp || b->atol_comp) return DE_ERR_UNSUPPORTED;
+  if (b->event_solout != DE_SOLOUT_DEFAULT) return DE_ERR_UNSUPPORTED;
   const bool ragged = b->family == DE_FAMILY_SPRING_CHAIN;
   if (ragged && (!b->sizes || !b->state_offset || !b->param_offset)) return DE_ERR_INVALID;
   if (!ragged && b->dim != (b->family == DE_FAMILY_LORENZ ? 3 : 2)) return DE_ERR_INVALID;

@@ -201,6 +201,68 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+# ------------------------------------------------------------------ events --
+class CrossingDirection:
+    """CrossingDirection (src/solout/mod.rs:38-55)."""
+    Both = DE_CROSS_BOTH
+    Positive = DE_CROSS_POSITIVE
+    Negative = DE_CROSS_NEGATIVE
+
+
+class EventConfig:
+    """EventConfig (src/solout/event.rs:8-50): crossing direction and the
+    number of events after which the solve terminates (None: never)."""
+
+    def __init__(self, direction=CrossingDirection.Both, terminate=None):
+        self._direction = direction
+        self._terminate = terminate
+
+    def direction(self, d):
+        self._direction = d
+        return self
+
+    def terminate_after(self, count):
+        self._terminate = int(count)
+        return self
+
+    def terminal(self):
+        return self.terminate_after(1)
+
+
+class Event:
+    """An event function g(t, y) built into the device kernels (the Event
+    trait, src/solout/event.rs:52-60), with its EventConfig.
+
+    Event.component(c, value): y[c] - value
+    Event.time(value):         t - value
+    Event.plane(point, normal): y.minus(&point).dot(&normal)
+    `value` may be one number or one per trajectory."""
+
+    def __init__(self, fn, component=0, value=0.0, point=None, normal=None, config=None):
+        self.fn = fn
+        self.comp = int(component)
+        self.value = value
+        self.point = point
+        self.normal = normal
+        self.cfg = config if config is not None else EventConfig()
+
+    @classmethod
+    def component(cls, c, value, config=None):
+        return cls(DE_EVENT_COMPONENT, component=c, value=value, config=config)
+
+    @classmethod
+    def time(cls, value, config=None):
+        return cls(DE_EVENT_TIME, value=value, config=config)
+
+    @classmethod
+    def plane(cls, point, normal, config=None):
+        return cls(DE_EVENT_PLANE, point=point, normal=normal, config=config)
+
+    def config(self, cfg):
+        self.cfg = cfg
+        return self
+
+
 # ---------------------------------------------------------------- problems --
 class OdeEnsemble:
     """IVP::ode for an ensemble.  Arrays live in host memory (numpy)."""
@@ -231,6 +293,9 @@ class OdeEnsemble:
         # Per-component tolerances of every trajectory, in the layout of y0.
         self.rtol_comp = None
         self.atol_comp = None
+        # (DE_SOLOUT_*, Event) of an event / crossing solout, or None.
+        self._event = None
+        self.max_events = 16
 
     def method(self, m):
         self._method = m
@@ -243,9 +308,34 @@ class OdeEnsemble:
         if tf < t0:
             pts = pts[::-1].copy()
         self._t_eval = np.ascontiguousarray(pts)
+        self._event = None  # with_solout replaces the whole solout (src/ivp.rs:656-659)
         return self
 
-    def options(self, schedule=None, mapping=None, group_size=None, fast_math=None):
+    def event(self, ev):
+        """IVP::event (src/ivp.rs:662-678): wrap the current output (final state
+        or t_eval) with event detection."""
+        self._event = (DE_SOLOUT_EVENT, ev)
+        return self
+
+    def crossing(self, component_idx, threshold, direction=CrossingDirection.Both):
+        """IVP::crossing (src/ivp.rs:682-692): CrossingSolout replaces the output."""
+        self._t_eval = None
+        self._event = (DE_SOLOUT_CROSSING,
+                       Event.component(component_idx, threshold, EventConfig(direction)))
+        return self
+
+    def hyperplane_crossing(self, point, normal, direction=CrossingDirection.Both):
+        """IVP::hyperplane_crossing (src/ivp.rs:695-705) with the identity
+        extractor: the hyperplane lies in the state space."""
+        self._t_eval = None
+        self._event = (DE_SOLOUT_HYPERPLANE,
+                       Event.plane(point, normal, EventConfig(direction)))
+        return self
+
+    def options(self, schedule=None, mapping=None, group_size=None, fast_math=None,
+                max_events=None):
+        if max_events is not None:
+            self.max_events = int(max_events)
         if schedule is not None:
             self.schedule = schedule
         if mapping is not None:
@@ -325,7 +415,31 @@ class OdeEnsemble:
         b.mapping = self.mapping
         b.group_size = self.group_size
         b.flags = self.flags | DE_FLAG_HOST_MEMORY
-        res = Solution(self.n, state_len, n_eval)
+        cap = None
+        if self._event is not None:
+            kind, ev = self._event
+            cap = self.max_events
+            b.event_solout = kind
+            b.event_fn = ev.fn
+            b.event_component = ev.comp
+            b.event_direction = ev.cfg._direction
+            b.event_terminate = 0 if ev.cfg._terminate is None else ev.cfg._terminate
+            b.max_events = cap
+            if np.ndim(ev.value) > 0:
+                v = np.ascontiguousarray(ev.value, dtype=np.float64).reshape(-1)
+                if len(v) != self.n:
+                    raise ValueError("a per-trajectory event value needs %d entries" % self.n)
+                keep.append(v)
+                b.event_value_traj = _ptr(v)
+            else:
+                b.event_value = float(ev.value)
+            for name in ("point", "normal"):
+                v = getattr(ev, name)
+                if v is not None:
+                    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+                    keep.append(v)
+                    setattr(b, "event_" + name, _ptr(v))
+        res = Solution(self.n, state_len, n_eval, cap)
         return b, res, keep
 
     def solve(self, stream=None):
@@ -337,8 +451,9 @@ class OdeEnsemble:
 class Solution:
     """Per-trajectory results (Solution / Error of each reference solve)."""
 
-    def __init__(self, n, state_len, n_eval):
+    def __init__(self, n, state_len, n_eval, max_events=None):
         self.n = n
+        self.max_events = max_events or 0
         self.y = np.full(state_len, np.nan)
         self.t = np.full(n, np.nan)
         self.accepted = np.zeros(n, dtype=np.int64)
@@ -347,6 +462,20 @@ class Solution:
         self.status = np.full(n, -1, dtype=np.int32)
         self.dense = np.full(state_len * n_eval, np.nan) if n_eval else None
         self.n_dense = np.zeros(n, dtype=np.int32) if n_eval else None
+        # Event points: event_t[k * n + i], event_y[(k * dim + c) * n + i].
+        self.event_t = np.full(max_events * n, np.nan) if max_events else None
+        self.event_y = np.full(max_events * state_len, np.nan) if max_events else None
+        self.n_events = np.zeros(n, dtype=np.int32) if max_events is not None else None
+
+    def events(self, i):
+        """(t[k], y[k, dim]) of the event points kept for trajectory i."""
+        k = min(int(self.n_events[i]), self.max_events)
+        dim = len(self.y) // self.n
+        if k == 0:
+            return np.empty(0), np.empty((0, dim))
+        t = self.event_t.reshape(self.max_events, self.n)[:k, i]
+        y = self.event_y.reshape(self.max_events, dim, self.n)[:k, :, i]
+        return t.copy(), y.copy()
 
     def struct(self):
         r = OdeResult()
@@ -358,6 +487,9 @@ class Solution:
         r.status = _ptr(self.status)
         r.dense = _ptr(self.dense)
         r.n_dense = _ptr(self.n_dense)
+        r.event_t = _ptr(self.event_t)
+        r.event_y = _ptr(self.event_y)
+        r.n_events = _ptr(self.n_events)
         self._struct = r
         return r
 

@@ -18,6 +18,8 @@ DE_STATUS_MAX_STEPS = 1
 DE_STATUS_STEP_SIZE = 2
 DE_STATUS_STIFFNESS = 3
 DE_STATUS_BAD_INPUT = 4
+DE_STATUS_INTERRUPTED = 5
+DE_STATUS_OUT_OF_BOUNDS = 6
 
 DE_FAMILY_LORENZ = 0
 DE_FAMILY_VAN_DER_POL = 1
@@ -55,6 +57,19 @@ DE_MAP_AUTO = 0
 DE_MAP_LANE = 1
 DE_MAP_GROUP = 2
 
+DE_SOLOUT_DEFAULT = 0
+DE_SOLOUT_EVENT = 1
+DE_SOLOUT_CROSSING = 2
+DE_SOLOUT_HYPERPLANE = 3
+
+DE_EVENT_COMPONENT = 0
+DE_EVENT_TIME = 1
+DE_EVENT_PLANE = 2
+
+DE_CROSS_BOTH = 0
+DE_CROSS_POSITIVE = 1
+DE_CROSS_NEGATIVE = 2
+
 DE_FLAG_HOST_MEMORY = 1
 DE_FLAG_FAST_MATH = 2
 
@@ -81,6 +96,11 @@ class OdeBatch(C.Structure):
         ("schedule", C.c_int32), ("mapping", C.c_int32), ("group_size", C.c_int32),
         ("flags", C.c_uint32),
         ("rtol_comp", C.c_void_p), ("atol_comp", C.c_void_p),
+        ("event_solout", C.c_int32), ("event_fn", C.c_int32),
+        ("event_component", C.c_int32), ("event_direction", C.c_int32),
+        ("event_terminate", C.c_int32), ("max_events", C.c_int32),
+        ("event_value", C.c_double), ("event_value_traj", C.c_void_p),
+        ("event_point", C.c_void_p), ("event_normal", C.c_void_p),
     ]
 
 
@@ -89,6 +109,7 @@ class OdeResult(C.Structure):
         ("y_final", C.c_void_p), ("t_final", C.c_void_p),
         ("accepted", C.c_void_p), ("rejected", C.c_void_p), ("evals", C.c_void_p),
         ("status", C.c_void_p), ("dense", C.c_void_p), ("n_dense", C.c_void_p),
+        ("event_t", C.c_void_p), ("event_y", C.c_void_p), ("n_events", C.c_void_p),
     ]
 
 

@@ -48,6 +48,12 @@ extern "C" {
 #define DE_STATUS_STEP_SIZE 2  /* Error::StepSize  */
 #define DE_STATUS_STIFFNESS 3  /* Error::Stiffness */
 #define DE_STATUS_BAD_INPUT 4  /* Error::BadInput  */
+#define DE_STATUS_INTERRUPTED 5 /* Status::Interrupted: an event solout returned Terminate
+                                   (src/ode/solve_ivp.rs:254-258) */
+#define DE_STATUS_OUT_OF_BOUNDS 6 /* the reference solve panics here: a solout unwraps an
+                                     Interpolation::interpolate that returned
+                                     Error::OutOfBounds (t outside [t_prev, t] of the step, as
+                                     in every interpolation of a backward solve) */
 
 /* ODE families (device-compiled right-hand sides). */
 #define DE_FAMILY_LORENZ 0       /* dim 3, params sigma, rho, beta */
@@ -93,6 +99,25 @@ extern "C" {
 #define DE_MAP_LANE 1  /* one lane per system, state in HBM */
 #define DE_MAP_GROUP 2 /* `group_size` lanes per system, state in registers */
 
+/* Solouts beside DefaultSolout / TEvalSolout (src/solout, builders src/ivp.rs:662-705). */
+#define DE_SOLOUT_DEFAULT 0    /* DefaultSolout, or TEvalSolout when n_eval > 0 */
+#define DE_SOLOUT_EVENT 1      /* IVP::event: EventWrappedSolout around the above, root by
+                                  Brent-Dekker on the step interpolant (src/solout/event.rs) */
+#define DE_SOLOUT_CROSSING 2   /* IVP::crossing: CrossingSolout, Newton on the interpolant
+                                  (src/solout/crossing.rs); replaces the t_eval output */
+#define DE_SOLOUT_HYPERPLANE 3 /* IVP::hyperplane_crossing with the identity extractor
+                                  (src/solout/hyperplane.rs) */
+
+/* Event functions g(t, y) (Event::event, src/solout/event.rs) of DE_SOLOUT_EVENT. */
+#define DE_EVENT_COMPONENT 0 /* y[event_component] - value */
+#define DE_EVENT_TIME 1      /* t - value */
+#define DE_EVENT_PLANE 2     /* y.minus(&point).dot(&normal), normal as given */
+
+/* CrossingDirection (src/solout/mod.rs:38-45). */
+#define DE_CROSS_BOTH 0
+#define DE_CROSS_POSITIVE 1
+#define DE_CROSS_NEGATIVE 2
+
 /* Flags. */
 #define DE_FLAG_HOST_MEMORY 1u /* all pointers are host memory */
 #define DE_FLAG_FAST_MATH 2u   /* FMA-contracted kernels: faster, not bit-equal */
@@ -137,6 +162,20 @@ typedef struct de_ode_batch {
    * (src/traits.rs:394-411) and h_init atol[c] + rtol[c]*|y0| (src/methods/h_init.rs:80). */
   const double* rtol_comp;
   const double* atol_comp;
+  /* Event solouts (DE_SOLOUT_*; fixed-dimension families).  The event points of each
+   * trajectory -- the points the reference's event / crossing solout pushes into
+   * Solution -- come back in de_ode_result.event_t / event_y. */
+  int32_t event_solout;    /* DE_SOLOUT_* */
+  int32_t event_fn;        /* DE_EVENT_*: g(t, y) of DE_SOLOUT_EVENT */
+  int32_t event_component; /* component of DE_EVENT_COMPONENT and DE_SOLOUT_CROSSING */
+  int32_t event_direction; /* DE_CROSS_*: EventConfig::direction / CrossingDirection */
+  int32_t event_terminate; /* EventConfig::terminate: 0 = None, k > 0 = Some(k) */
+  int32_t max_events;      /* event points kept per trajectory */
+  double event_value;      /* value of DE_EVENT_COMPONENT / DE_EVENT_TIME, crossing threshold */
+  const double* event_value_traj; /* per-trajectory value, or NULL for event_value */
+  const double* event_point;      /* [dim] point of DE_EVENT_PLANE / DE_SOLOUT_HYPERPLANE */
+  const double* event_normal;     /* [dim] normal (normalised by the hyperplane solout,
+                                     src/solout/hyperplane.rs:126-132) */
 } de_ode_batch;
 
 typedef struct de_ode_result {
@@ -148,6 +187,9 @@ typedef struct de_ode_result {
   int32_t* status;   /* [n] DE_STATUS_* */
   double* dense;     /* t_eval output (NaN where a point was not produced), or NULL */
   int32_t* n_dense;  /* [n] number of t_eval points produced, or NULL */
+  double* event_t;   /* [max_events * n] event times, event_t[k * n + i]; NaN past the last */
+  double* event_y;   /* [max_events * dim * n] event states, event_y[(k * dim + c) * n + i] */
+  int32_t* n_events; /* [n] event points pushed (those past max_events are not kept), or NULL */
 } de_ode_result;
 
 typedef struct de_sde_batch {

@@ -23,6 +23,10 @@
 //   stochastic ERK (EM) src/methods/erk/fixed/stochastic.rs:18-146
 //   Milstein            src/methods/milstein.rs:76-180
 //   TEvalSolout         src/solout/t_eval.rs:73-172
+//   DefaultSolout       src/solout/default.rs
+//   EventWrappedSolout  src/solout/event.rs:320-523 (Brent-Dekker :395-490)
+//   CrossingSolout      src/solout/crossing.rs:181-320
+//   HyperplaneCrossing  src/solout/hyperplane.rs:126-340
 // The tableau's zero coefficients are applied like any other (the reference
 // loops over all j < i), so this restatement does too.
 //
@@ -242,12 +246,28 @@ struct TEval {
   int pushed = 0;
 };
 
+// Control flags of a solout call (ControlFlag, src/solout/solout.rs), plus the
+// panic of an unwrap on a failed interpolation.
+enum { kContinue = 0, kTerminate = 1, kPanic = 2 };
+
+// The time of the last point of the Solution (solution.t.last()), which the
+// event solout consults before pushing (src/solout/event.rs:379-383).
+struct SolTail {
+  bool has = false;
+  double t = 0.0;
+  void push(double tt) {
+    has = true;
+    t = tt;
+  }
+};
+
 // TEvalSolout::solout (src/solout/t_eval.rs:83-134); interp(te, out) is the
-// method's Interpolation::interpolate.
+// method's Interpolation::interpolate, returning false for Err(OutOfBounds),
+// on which the reference's unwrap (t_eval.rs:107) panics.
 template <class Interp>
-static void teval_solout(TEval& s, double t_curr, double t_prev, const double* y_curr, int n,
-                         Interp&& interp) {
-  if (!s.t_evals) return;
+static int teval_solout(TEval& s, double t_curr, double t_prev, const double* y_curr, int n,
+                        SolTail& tail, Interp&& interp) {
+  if (!s.t_evals) return kContinue;
   int idx = s.next;
   while (idx < s.n_eval) {
     double te = s.t_evals[idx];
@@ -258,9 +278,14 @@ static void teval_solout(TEval& s, double t_curr, double t_prev, const double* y
       if (te == t_curr) {
         copy(out, y_curr, n);
       } else {
-        interp(te, out);
+        if (!interp(te, out)) {
+          for (int c = 0; c < n; ++c) out[c] = NAN;
+          s.next = idx;
+          return kPanic;
+        }
       }
       s.pushed += 1;
+      tail.push(te);
       idx += 1;
     } else {
       if ((s.dir > 0.0 && te > t_curr) || (s.dir < 0.0 && te < t_curr)) break;
@@ -268,7 +293,292 @@ static void teval_solout(TEval& s, double t_curr, double t_prev, const double* y
     }
   }
   s.next = idx;
+  return kContinue;
 }
+
+// ------------------------------------------------------- event solouts -----
+// What the solve records of the event solout's pushes: the first `cap` points
+// (t, y) and the number pushed.
+struct EventOut {
+  double* t = nullptr;  // [cap]
+  double* y = nullptr;  // [cap][dim]
+  int cap = 0;
+  int pushed = 0;
+};
+
+struct EventSpec {
+  int solout = DE_SOLOUT_DEFAULT;
+  int fn = DE_EVENT_COMPONENT;
+  int comp = 0;
+  int direction = DE_CROSS_BOTH;
+  int terminate = 0;      // EventConfig::terminate (0 = None)
+  const double* point = nullptr;
+  const double* normal = nullptr;  // HYPERPLANE: already normalised
+};
+
+// Largest state dimension with event solouts (fixed-dimension families).
+constexpr int kEvMax = 8;
+
+// Rust f64::signum compared with != (a NaN signum differs from everything).
+static inline bool sign_differs(double a, double b) { return de::signum(a) != de::signum(b); }
+
+struct EventState {
+  const EventSpec* sp = nullptr;
+  int n = 0;
+  double value = 0.0;   // threshold / value of g for this trajectory
+  double direction = 1.0;  // (tf - t0).signum()
+  bool has_last = false;   // last_g / last_offset_value / last_distance is Some
+  double last = 0.0;
+  uint32_t count = 0;      // EventWrappedSolout::event_count
+  EventOut* out = nullptr;
+
+  // g of DE_SOLOUT_EVENT (Event::event), the offset of CrossingSolout
+  // (crossing.rs:198-199) or the signed distance of the hyperplane solout
+  // (hyperplane.rs:192-194: pos.minus(point).dot(normal), dot folding from 0).
+  double g(double t, const double* y) const {
+    int fn = sp->solout == DE_SOLOUT_CROSSING ? DE_EVENT_COMPONENT
+             : sp->solout == DE_SOLOUT_HYPERPLANE ? DE_EVENT_PLANE : sp->fn;
+    if (fn == DE_EVENT_COMPONENT) return y[sp->comp] - value;
+    if (fn == DE_EVENT_TIME) return t - value;
+    double sum = 0.0;
+    for (int c = 0; c < n; ++c) {
+      double d = y[c] + (-1.0) * sp->point[c];
+      sum = sum + d * sp->normal[c];
+    }
+    return sum;
+  }
+
+  void record(double t, const double* y) {
+    if (out->pushed < out->cap) {
+      out->t[out->pushed] = t;
+      copy(out->y + (int64_t)out->pushed * n, y, n);
+    }
+    out->pushed += 1;
+  }
+
+  // EventWrappedSolout::brent_dekker (src/solout/event.rs:395-490).
+  template <class Interp>
+  bool brent_dekker(double a, double b, double fa, double fb, Interp&& interp, double* root) {
+    if (fabs(fa) < fabs(fb)) {
+      std::swap(a, b);
+      std::swap(fa, fb);
+    }
+    double c = a, fc = fa;
+    double d = b - a;
+    double e = d;
+    const double one = 1.0, two = 2.0, half = one / two, three = 3.0;
+    const double abs_tol = 1e-14, rel_tol = 1e-12;
+    double yb[kEvMax];
+    for (uint32_t it = 0; it < 50u; ++it) {
+      if (fb == 0.0) {
+        *root = b;
+        return true;
+      }
+      if (de::signum(fa) == de::signum(fb)) {
+        a = c;
+        fa = fc;
+        c = b;
+        fc = fb;
+        d = b - a;
+        e = d;
+      }
+      if (fabs(fa) < fabs(fb)) {
+        c = b;
+        b = a;
+        a = c;
+        fc = fb;
+        fb = fa;
+        fa = fc;
+      }
+      double tol = fmax(abs_tol, rel_tol * fabs(b));
+      double m = half * (a - b);
+      if (fabs(m) <= tol || fb == 0.0) {
+        *root = b;
+        return true;
+      }
+      bool use_bis = true;
+      if (fabs(e) > tol && fabs(fa) > fabs(fb)) {
+        double s = fb / fa;
+        double p, q;
+        if (a == c) {
+          p = two * m * s;
+          q = one - s;
+        } else {
+          double q1 = fa / fc;
+          double r = fb / fc;
+          p = s * (two * m * q1 * (q1 - r) - (b - a) * (r - one));
+          q = (q1 - one) * (r - one) * (s - one);
+        }
+        double q_mod = q, p_mod = p;
+        if (q_mod > 0.0) p_mod = -p_mod;
+        else q_mod = -q_mod;
+        if (fabs(two * p_mod) < (three * m * q_mod - fabs(tol * q_mod)) &&
+            p_mod < fabs(e * half * q_mod)) {
+          e = d;
+          d = p_mod / q_mod;
+          use_bis = false;
+        }
+      }
+      if (use_bis) {
+        d = m;
+        e = m;
+      }
+      a = b;
+      fa = fb;
+      b = fabs(d) > tol ? b + d : b + (m > 0.0 ? tol : -tol);
+      if (!interp(b, yb)) return false;  // interpolate(b).ok()?
+      fb = g(b, yb);
+      c = a;
+      fc = fa;
+    }
+    return false;
+  }
+
+  // EventWrappedSolout::detect_event (src/solout/event.rs:335-393).
+  template <class Interp>
+  int detect(double t_curr, double t_prev, const double* y_curr, SolTail& tail, Interp&& interp) {
+    double g_curr = g(t_curr, y_curr);
+    if (!has_last) {
+      has_last = true;
+      last = g_curr;
+      return kContinue;
+    }
+    double g_prev = last;
+    bool sign_change = sign_differs(g_prev, g_curr);
+    bool ok = sp->direction == DE_CROSS_POSITIVE
+                  ? sign_change && g_prev < 0.0 && g_curr >= 0.0
+                  : sp->direction == DE_CROSS_NEGATIVE
+                        ? sign_change && g_prev > 0.0 && g_curr <= 0.0
+                        : sign_change;
+    if (ok) {
+      double a = t_prev, b = t_curr, fa = g_prev, fb = g_curr;
+      if ((direction > 0.0 && a > b) || (direction < 0.0 && a < b)) {
+        std::swap(a, b);
+        std::swap(fa, fb);
+      }
+      double t_event;
+      if (fa * fb <= 0.0 && brent_dekker(a, b, fa, fb, interp, &t_event)) {
+        double y_event[kEvMax];
+        if (!interp(t_event, y_event)) return kPanic;  // interpolate(t_event).unwrap()
+        bool push = !tail.has || fabs(t_event - tail.t) > 1e-14;
+        if (push) {
+          record(t_event, y_event);
+          tail.push(t_event);
+        }
+        count += 1;
+        if (sp->terminate > 0 && count >= (uint32_t)sp->terminate) {
+          last = g_curr;
+          return kTerminate;
+        }
+      }
+    }
+    last = g_curr;
+    return kContinue;
+  }
+
+  // find_crossing_newton of CrossingSolout (crossing.rs:253-320) or of the
+  // hyperplane solout (hyperplane.rs:277-340).  Returns kPanic when one of its
+  // interpolate(..).unwrap() calls fails, else kContinue with *found set.
+  template <class Interp>
+  int newton(double t_lower, double t_upper, double off_lower, double off_upper, Interp&& interp,
+             bool* found, double* t_out) {
+    const bool hyper = sp->solout == DE_SOLOUT_HYPERPLANE;
+    const double eps = 2.220446049250313080847e-16;
+    double t = t_lower - off_lower * (t_upper - t_lower) / (off_upper - off_lower);
+    const double tolerance = eps * 100.0;
+    double yt[kEvMax], yp[kEvMax];
+    double offset;
+    for (int it = 0; it < 10; ++it) {
+      if (!interp(t, yt)) return kPanic;
+      offset = g(t, yt);
+      if (fabs(offset) < tolerance) {
+        *found = true;
+        *t_out = t;
+        return kContinue;
+      }
+      double delta_t = (t_upper - t_lower) * 1e-6;
+      double t_plus = t + delta_t;
+      if (!interp(t_plus, yp)) return kPanic;
+      double off_plus = g(t_plus, yp);
+      double derivative = (off_plus - offset) / delta_t;
+      if (fabs(derivative) < (hyper ? eps : eps * 10.0)) break;
+      double t_new = t - offset / derivative;
+      if (t_new < t_lower || t_new > t_upper) {
+        t = (t_lower + t_upper) / 2.0;
+      } else {
+        if (!hyper) {
+          double change = fabs(t_new - t);
+          if (change < tolerance * 0.1) {
+            t = t_new;
+            break;
+          }
+        }
+        t = t_new;
+      }
+    }
+    if (!interp(t, yt)) return kPanic;
+    offset = g(t, yt);
+    *found = fabs(offset) < tolerance * 10.0;
+    *t_out = t;
+    return kContinue;
+  }
+
+  // CrossingSolout::solout (crossing.rs:186-248) / HyperplaneCrossingSolout::
+  // solout (hyperplane.rs:203-270).
+  template <class Interp>
+  int crossing(double t_curr, double t_prev, const double* y_curr, Interp&& interp) {
+    const bool hyper = sp->solout == DE_SOLOUT_HYPERPLANE;
+    double off = g(t_curr, y_curr);
+    if (has_last) {
+      double lo = last;
+      bool is_crossing = sign_differs(lo, off);
+      if (hyper) is_crossing = is_crossing || (lo == 0.0 && off != 0.0) || (lo != 0.0 && off == 0.0);
+      if (is_crossing) {
+        bool rec = sp->direction == DE_CROSS_POSITIVE ? lo < 0.0 && off >= 0.0
+                   : sp->direction == DE_CROSS_NEGATIVE ? lo > 0.0 && off <= 0.0
+                                                        : true;
+        if (rec) {
+          bool found = false;
+          double tc = 0.0;
+          if (newton(t_prev, t_curr, lo, off, interp, &found, &tc) == kPanic) return kPanic;
+          if (!found) {
+            double frac = -lo / (off - lo);
+            tc = t_prev + frac * (t_curr - t_prev);
+          }
+          double yc[kEvMax];
+          if (!interp(tc, yc)) return kPanic;
+          record(tc, yc);
+        }
+      }
+    }
+    last = off;
+    has_last = true;
+    return kContinue;
+  }
+};
+
+// The solout chain of one trajectory: DefaultSolout or TEvalSolout, wrapped by
+// the event solout (EventWrappedSolout::solout, event.rs:499-522: the base
+// first, then detection), or a crossing solout alone.
+struct Solouts {
+  TEval* te = nullptr;
+  EventState* ev = nullptr;
+  SolTail tail;
+  template <class Interp>
+  int call(double t_curr, double t_prev, const double* y_curr, int n, Interp&& interp) {
+    if (ev && (ev->sp->solout == DE_SOLOUT_CROSSING || ev->sp->solout == DE_SOLOUT_HYPERPLANE))
+      return ev->crossing(t_curr, t_prev, y_curr, interp);
+    int f;
+    if (te) {
+      f = teval_solout(*te, t_curr, t_prev, y_curr, n, tail, interp);
+    } else {
+      tail.push(t_curr);  // DefaultSolout pushes (t_curr, y_curr)
+      f = kContinue;
+    }
+    if (f != kContinue || !ev) return f;
+    return ev->detect(t_curr, t_prev, y_curr, tail, interp);
+  }
+};
 
 // ---------------------------------------------------------------- DOPRI5 ---
 template <class Sys>
@@ -404,8 +714,9 @@ struct Dopri5 {
     return -1;
   }
 
-  // ordinary.rs:301-336 (in-bounds callers only: TEvalSolout).
-  void interpolate(double t_interp, double* out) const {
+  // ordinary.rs:301-336; false for Err(OutOfBounds) (:303-309).
+  bool interpolate(double t_interp, double* out) const {
+    if (t_interp < t_prev || t_interp > t) return false;
     double s = (t_interp - t_prev) / h_prev;
     double s1 = 1.0 - s;
     double acc[M];
@@ -417,6 +728,7 @@ struct Dopri5 {
     }
     copy(out, cont[0], n);
     add_scaled(out, s, acc, n);
+    return true;
   }
 };
 
@@ -606,7 +918,8 @@ struct DormandPrince {
 
   // ordinary.rs:301-336: fold from cont[O-1] down to cont[1] with factors s1
   // or s (indices >= 4 alternate by distance to the last, 1..3 by parity).
-  void interpolate(double t_interp, double* out) const {
+  bool interpolate(double t_interp, double* out) const {
+    if (t_interp < t_prev || t_interp > t) return false;
     double s = (t_interp - t_prev) / h_prev;
     double s1 = 1.0 - s;
     const int ilast = O - 1;
@@ -619,6 +932,7 @@ struct DormandPrince {
     }
     copy(out, cont[0], n);
     add_scaled(out, s, acc, n);
+    return true;
   }
 };
 
@@ -687,8 +1001,10 @@ struct FixedErk {
   }
 
   // cubic_hermite_interpolate (src/interpolate.rs:40-60) on
-  // (t_prev, t, y_prev, y, dydt_prev, dydt).
-  void interpolate(double tt, double* out) const {
+  // (t_prev, t, y_prev, y, dydt_prev, dydt); false for Err(OutOfBounds)
+  // (fixed/ordinary.rs:170-177).
+  bool interpolate(double tt, double* out) const {
+    if (tt < t_prev || tt > t) return false;
     double hh = t - t_prev;
     double s = (tt - t_prev) / hh;
     double s2 = s * s;
@@ -702,14 +1018,17 @@ struct FixedErk {
     add_scaled(out, h10 * hh, dydt_prev, n);
     add_scaled(out, h01, y, n);
     add_scaled(out, h11 * hh, dydt, n);
+    return true;
   }
 };
 
 // ------------------------------------------------------------- solve_ode ---
-// src/ode/solve_ivp.rs:116-277 with TEvalSolout (or DefaultSolout when t_eval
-// is absent: only the final state is kept).
+// src/ode/solve_ivp.rs:116-277 with the trajectory's solout chain (DefaultSolout
+// keeps only the final state here).  ControlFlag::Terminate ends the solve with
+// Status::Interrupted (:178-183, :254-259); a solout's panic ends it with
+// DE_STATUS_OUT_OF_BOUNDS at the state the step reached.
 template <class Method>
-static TrajResult solve_ode(Method& m, double t0, double tf, const double* y0, TEval* te,
+static TrajResult solve_ode(Method& m, double t0, double tf, const double* y0, Solouts& sol,
                             double* y_out) {
   TrajResult r;
   const int n = m.n;
@@ -733,10 +1052,17 @@ static TrajResult solve_ode(Method& m, double t0, double tf, const double* y0, T
     return r;
   }
   r.evals += ev;
-  if (te) {
-    te->dir = dir;
-    teval_solout(*te, m.t, m.t_prev, m.y, n, [&](double tt, double* o) { m.interpolate(tt, o); });
-  }
+  if (sol.te) sol.te->dir = dir;
+  if (sol.ev) sol.ev->direction = dir;
+  auto interp = [&](double tt, double* o) { return m.interpolate(tt, o); };
+  auto end_with = [&](int flag) {
+    r.status = flag == kTerminate ? DE_STATUS_INTERRUPTED : DE_STATUS_OUT_OF_BOUNDS;
+    r.t = m.t;
+    copy(y_out, m.y, n);
+    return r;
+  };
+  int f = sol.call(m.t, m.t_prev, m.y, n, interp);
+  if (f != kContinue) return end_with(f);
   m.rejected_status = false;  // Status::Solving
   for (;;) {
     if ((m.t + m.h - tf) * dir > 0.0) {
@@ -758,9 +1084,8 @@ static TrajResult solve_ode(Method& m, double t0, double tf, const double* y0, T
       continue;
     }
     r.accepted += 1;
-    if (te) {
-      teval_solout(*te, m.t, m.t_prev, m.y, n, [&](double tt, double* o) { m.interpolate(tt, o); });
-    }
+    f = sol.call(m.t, m.t_prev, m.y, n, interp);
+    if (f != kContinue) return end_with(f);
     if (fabs(tf - m.t) <= de::kTenEps) break;
   }
   r.status = DE_STATUS_COMPLETE;
@@ -1045,38 +1370,47 @@ static void parallel_for(int64_t n, int threads, Fn&& fn) {
 template <class Sys>
 static void run_ode_traj(const Sys& sys, const de_ode_batch* b, const Settings& st, int64_t i,
                          const double* y0, double* y_out, double* dense, int32_t* n_dense,
-                         TrajResult* res) {
+                         const EventSpec* es, EventOut* eo, TrajResult* res) {
   double t0 = b->t0 ? b->t0[i] : b->t0_scalar;
   double tf = b->tf ? b->tf[i] : b->tf_scalar;
   TEval te;
-  TEval* tep = nullptr;
-  if (b->n_eval > 0 && dense) {
+  Solouts sol;
+  if (b->n_eval > 0 && dense && (!es || es->solout <= DE_SOLOUT_EVENT)) {
     te.t_evals = b->t_eval;
     te.n_eval = b->n_eval;
     te.dense = dense;
-    tep = &te;
+    sol.te = &te;
+  }
+  EventState evs;
+  if (es && es->solout != DE_SOLOUT_DEFAULT) {
+    evs.sp = es;
+    evs.n = sys.dim();
+    evs.value = b->event_value_traj ? b->event_value_traj[i] : b->event_value;
+    evs.out = eo;
+    sol.ev = &evs;
   }
   if (b->method == DE_METHOD_RK4 || b->method == DE_METHOD_EULER) {
     FixedErk<Sys> m(sys, st, b->method == DE_METHOD_RK4 ? kRK4 : kEuler);
-    *res = solve_ode(m, t0, tf, y0, tep, y_out);
+    *res = solve_ode(m, t0, tf, y0, sol, y_out);
   } else if (b->method == DE_METHOD_DOP853) {
     DormandPrince<Sys, Dop853Tableau> m(sys, kDOP853, st);
-    *res = solve_ode(m, t0, tf, y0, tep, y_out);
+    *res = solve_ode(m, t0, tf, y0, sol, y_out);
   } else if (g_generic_dopri5) {
     DormandPrince<Sys, Dopri5TableauG> m(sys, kDP5G, st);
-    *res = solve_ode(m, t0, tf, y0, tep, y_out);
+    *res = solve_ode(m, t0, tf, y0, sol, y_out);
   } else {
     Dopri5<Sys> m(sys, st);
-    *res = solve_ode(m, t0, tf, y0, tep, y_out);
+    *res = solve_ode(m, t0, tf, y0, sol, y_out);
   }
-  if (n_dense) *n_dense = tep ? tep->pushed : 0;
+  if (n_dense) *n_dense = sol.te ? te.pushed : 0;
 }
 
 template <class F>
 static void run_fixed_family(const de_ode_batch* b, de_ode_result* out, const Settings& base,
-                             int threads) {
+                             const EventSpec* es, int threads) {
   const int64_t n = b->n;
   constexpr int D = F::kDim;
+  static_assert(D <= kEvMax, "event solouts hold states of at most kEvMax components");
   parallel_for(n, threads, [&](int64_t i) {
     FixedSys<F> sys;
     sys.f.load(b->params, n, i);
@@ -1094,13 +1428,27 @@ static void run_fixed_family(const de_ode_batch* b, de_ode_result* out, const Se
     for (int c = 0; c < D; ++c) y0[c] = b->y0[c * n + i];
     std::vector<double> dense;
     if (b->n_eval > 0 && out->dense) dense.assign((size_t)b->n_eval * D, NAN);
+    const int cap = es ? b->max_events : 0;
+    std::vector<double> et((size_t)cap, NAN), ey((size_t)cap * D, NAN);
+    EventOut eo;
+    eo.t = et.data();
+    eo.y = ey.data();
+    eo.cap = cap;
     TrajResult r;
     run_ode_traj(sys, b, st, i, y0, yo, dense.empty() ? nullptr : dense.data(),
-                 out->n_dense ? &out->n_dense[i] : nullptr, &r);
+                 out->n_dense ? &out->n_dense[i] : nullptr, es, &eo, &r);
     for (int c = 0; c < D; ++c) out->y_final[c * n + i] = yo[c];
     if (!dense.empty())
       for (int e = 0; e < b->n_eval; ++e)
         for (int c = 0; c < D; ++c) out->dense[((int64_t)e * D + c) * n + i] = dense[(size_t)e * D + c];
+    if (es) {
+      for (int k = 0; k < cap; ++k) {
+        if (out->event_t) out->event_t[(int64_t)k * n + i] = et[k];
+        if (out->event_y)
+          for (int c = 0; c < D; ++c) out->event_y[((int64_t)k * D + c) * n + i] = ey[(size_t)k * D + c];
+      }
+      if (out->n_events) out->n_events[i] = eo.pushed;
+    }
     out->t_final[i] = r.t;
     out->accepted[i] = r.accepted;
     out->rejected[i] = r.rejected;
@@ -1134,7 +1482,7 @@ static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings
     }
     TrajResult r;
     run_ode_traj(sys, b, st, i, b->y0 + off, out->y_final + off, dense,
-                 out->n_dense ? &out->n_dense[i] : nullptr, &r);
+                 out->n_dense ? &out->n_dense[i] : nullptr, nullptr, nullptr, &r);
     out->t_final[i] = r.t;
     out->accepted[i] = r.accepted;
     out->rejected[i] = r.rejected;
@@ -1154,14 +1502,41 @@ int de_oracle_ode_solve(const de_ode_batch* b, de_ode_result* out, int pow_mode,
   g_pow_mode = pow_mode;
   Settings st{b->rtol, b->atol, b->h0, b->h_min, b->h_max, b->safety_factor,
               b->min_scale, b->max_scale, b->max_steps};
+  EventSpec es;
+  std::vector<double> normal;
+  const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
+  if (events) {
+    if (b->family == DE_FAMILY_SPRING_CHAIN || b->dim > kEvMax) return DE_ERR_UNSUPPORTED;
+    es.solout = b->event_solout;
+    es.fn = b->event_fn;
+    es.comp = b->event_component;
+    es.direction = b->event_direction;
+    es.terminate = b->event_terminate;
+    es.point = b->event_point;
+    es.normal = b->event_normal;
+    if (es.solout == DE_SOLOUT_HYPERPLANE) {
+      // HyperplaneCrossingSolout::new (hyperplane.rs:126-132): normal /= |normal|
+      // when |normal| > EPSILON, |normal| = sqrt(norm_squared()) folded from 0.
+      normal.assign(b->event_normal, b->event_normal + b->dim);
+      double sq = 0.0;
+      for (double x : normal) sq = sq + x * x;
+      double norm = sqrt(sq);
+      if (norm > 2.220446049250313080847e-16) {
+        double inv = 1.0 / norm;
+        for (double& x : normal) x *= inv;
+      }
+      es.normal = normal.data();
+    }
+  }
+  const EventSpec* esp = events ? &es : nullptr;
   switch (b->family) {
-    case DE_FAMILY_LORENZ: run_fixed_family<de::LorenzRhs>(b, out, st, threads); break;
-    case DE_FAMILY_VAN_DER_POL: run_fixed_family<de::VanDerPolRhs>(b, out, st, threads); break;
-    case DE_FAMILY_EXPONENTIAL: run_fixed_family<de::ExponentialRhs>(b, out, st, threads); break;
-    case DE_FAMILY_LINEAR: run_fixed_family<de::LinearRhs>(b, out, st, threads); break;
-    case DE_FAMILY_HARMONIC: run_fixed_family<de::HarmonicRhs>(b, out, st, threads); break;
-    case DE_FAMILY_LOGISTIC: run_fixed_family<de::LogisticRhs>(b, out, st, threads); break;
-    case DE_FAMILY_ROBERTSON: run_fixed_family<de::RobertsonRhs>(b, out, st, threads); break;
+    case DE_FAMILY_LORENZ: run_fixed_family<de::LorenzRhs>(b, out, st, esp, threads); break;
+    case DE_FAMILY_VAN_DER_POL: run_fixed_family<de::VanDerPolRhs>(b, out, st, esp, threads); break;
+    case DE_FAMILY_EXPONENTIAL: run_fixed_family<de::ExponentialRhs>(b, out, st, esp, threads); break;
+    case DE_FAMILY_LINEAR: run_fixed_family<de::LinearRhs>(b, out, st, esp, threads); break;
+    case DE_FAMILY_HARMONIC: run_fixed_family<de::HarmonicRhs>(b, out, st, esp, threads); break;
+    case DE_FAMILY_LOGISTIC: run_fixed_family<de::LogisticRhs>(b, out, st, esp, threads); break;
+    case DE_FAMILY_ROBERTSON: run_fixed_family<de::RobertsonRhs>(b, out, st, esp, threads); break;
     case DE_FAMILY_SPRING_CHAIN: run_spring(b, out, st, threads); break;
     default: return DE_ERR_UNSUPPORTED;
   }

@@ -243,13 +243,18 @@ def dp_interpolate(cont, t_prev, h_prev, t_interp):
 
 
 def solve_dp(f, t0, tf, y0, rtol, atol, tab, max_steps=100000, powf=math.pow,
-             t_eval=None):
+             t_eval=None, solout=None):
     """The reference's generic Dormand-Prince step (dormandprince/ordinary.rs:
     13-270, ExplicitRungeKutta<Ordinary, DormandPrince, T, Y, O, S, I>) with a
     tableau dict: S stages, I stages with the dense-output ones, order O,
     lists C, A, B, ER, BI and BH (None when the method has no second error
     estimate).  Same return value as solve(), plus the t_eval values (increasing
-    points, TEvalSolout of src/solout/t_eval.rs:83-134) when t_eval is given."""
+    points, TEvalSolout of src/solout/t_eval.rs:83-134) when t_eval is given.
+    `solout(t_curr, t_prev, y_curr, interp)` is called after init and after
+    every accepted step, where solve_ivp.rs:152-183,242-259 call Solout::solout;
+    interp(t) is Interpolation::interpolate (None for Err(OutOfBounds)).  It
+    returns "continue", "terminate" (status 5, Interrupted) or "panic"
+    (status 6: the reference unwraps a failed interpolation)."""
     S, I, O = tab["S"], tab["I"], tab["O"]
     Cc, Aa, Bb, ERr, BH, BI = tab["C"], tab["A"], tab["B"], tab["ER"], tab["BH"], tab["BI"]
     n = len(y0)
@@ -275,6 +280,10 @@ def solve_dp(f, t0, tf, y0, rtol, atol, tab, max_steps=100000, powf=math.pow,
     if t_eval is not None and len(t_eval) and t_eval[0] == t:
         dense.append(list(y))
         idx = 1
+    if solout is not None:
+        flag = solout(t, t, list(y), lambda tt: None)
+        if flag != "continue":
+            return y, t, acc, rej, evals, 5 if flag == "terminate" else 6
     while True:
         if (t + h - tf) * direction > 0.0:
             h_new = tf - t
@@ -378,6 +387,7 @@ def solve_dp(f, t0, tf, y0, rtol, atol, tab, max_steps=100000, powf=math.pow,
                     dense.append(list(y_new) if te == t_new else dp_interpolate(cont, t, h, te))
                     idx += 1
             h_prev = h
+            h_prev_t = t
             t = t_new
             y = y_new
             k[0] = dydt
@@ -386,6 +396,15 @@ def solve_dp(f, t0, tf, y0, rtol, atol, tab, max_steps=100000, powf=math.pow,
                 scale = fmin(scale, 1.0)
             evals += step_evals
             acc += 1
+            if solout is not None:
+                def interp(tt, t_lo=h_prev_t, h_lo=h_prev, cont_lo=cont, t_hi=t):
+                    if tt < t_lo or tt > t_hi:
+                        return None
+                    return dp_interpolate(cont_lo, t_lo, h_lo, tt)
+
+                flag = solout(t, h_prev_t, list(y), interp)
+                if flag != "continue":
+                    return y, t, acc, rej, evals, 5 if flag == "terminate" else 6
         else:
             rejected_status = True
             evals += step_evals

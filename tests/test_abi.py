@@ -76,6 +76,40 @@ def test_spring_chains_accept_dopri5_only():
             _abi.DE_ERR_UNSUPPORTED
 
 
+def _event_case(kind):
+    e = deamd.IVP.ode(deamd.Lorenz(10.0, 28.0, 8.0 / 3.0), 0.0, 1.0, [1.0, 1.0, 1.0], n=4)
+    return e.event(deamd.Event.component(2, 27.0)) if kind == "event" else e.crossing(0, 0.0)
+
+
+@pytest.mark.parametrize("kind,mutate,code", [
+    ("event", lambda b: setattr(b, "event_solout", 7), _abi.DE_ERR_INVALID),
+    ("event", lambda b: setattr(b, "event_fn", 5), _abi.DE_ERR_INVALID),
+    ("event", lambda b: setattr(b, "event_component", 3), _abi.DE_ERR_INVALID),
+    ("event", lambda b: setattr(b, "event_direction", 3), _abi.DE_ERR_INVALID),
+    ("event", lambda b: setattr(b, "event_terminate", -1), _abi.DE_ERR_INVALID),
+    ("event", lambda b: setattr(b, "max_events", -2), _abi.DE_ERR_INVALID),
+    ("event", lambda b: (setattr(b, "event_fn", _abi.DE_EVENT_PLANE),
+                         setattr(b, "event_normal", None)), _abi.DE_ERR_INVALID),
+    ("crossing", lambda b: setattr(b, "event_component", -1), _abi.DE_ERR_INVALID),
+    # a crossing solout replaces the t_eval output (src/ivp.rs:682-692)
+    ("crossing", lambda b: (setattr(b, "n_eval", 1), setattr(b, "t_eval", b.y0)),
+     _abi.DE_ERR_INVALID),
+])
+def test_invalid_event_descriptors_are_rejected(kind, mutate, code):
+    b, res, keep = _event_case(kind).batch()
+    mutate(b)
+    assert deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None) == code
+    assert deamd.lib().de_last_error()
+
+
+def test_event_solouts_are_refused_for_spring_chains():
+    sysm, y0 = deamd.gen_spring(8, seed=1)
+    ens = deamd.IVP.ode(sysm, 0.0, 1.0, y0).event(deamd.Event.time(0.5))
+    b, res, keep = ens.batch()
+    assert deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None) == \
+        _abi.DE_ERR_UNSUPPORTED
+
+
 def test_empty_batch_is_a_no_op():
     ens = deamd.IVP.ode(deamd.Lorenz(10.0, 28.0, 8.0 / 3.0), 0.0, 1.0, np.zeros((3, 0)))
     b, res, keep = ens.batch()
