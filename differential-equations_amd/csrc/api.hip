@@ -239,8 +239,10 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
   // n_eval rows, with NaN for the points it does not produce.
   // Event solouts run the parity kernels, with or without DE_FLAG_FAST_MATH.
   int r;
-  if (b->event_solout != DE_SOLOUT_DEFAULT)
+  if (b->event_solout == DE_SOLOUT_EVENT)
     r = de::parity::launch_ode_event(b->family, b->method, a, sched, s);
+  else if (b->event_solout != DE_SOLOUT_DEFAULT)
+    r = de::parity::launch_ode_crossing(b->family, b->method, a, sched, s);
   else if (b->flags & DE_FLAG_FAST_MATH)
     r = de::fast::launch_ode_lane(b->family, b->method, a, sched, s);
   else

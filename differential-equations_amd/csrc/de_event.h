@@ -3,16 +3,29 @@
 //                        brent_dekker :395-490, solout :499-522)
 //   CrossingSolout       src/solout/crossing.rs:181-320
 //   HyperplaneCrossing   src/solout/hyperplane.rs:126-340
-// A lane calls pre() after every accepted step with the step's end point; when
-// it returns true the step needs the interpolant (a bracketed root), and the
-// lane builds it before calling post().  post() also runs when pre() returned
-// false, to advance the solout's state.
+// A lane calls pre() after every attempt with the attempt's end point (g is
+// evaluated branch-free; only accepted steps count); when the accepted step
+// brackets a root, the lane builds the interpolant and calls post() (inline
+// root) or defer() (deferred root).  commit(take) then advances the solout's
+// state to the step's end.
 //
 // Interpolants are passed as callables interp(t, out[D]) -> bool that return
 // false for the reference's Err(OutOfBounds) (t outside [t_prev, t] of the
 // step).  The reference's brent_dekker treats that as "no root" (`.ok()?`);
 // the other interpolations unwrap it, i.e. the reference panics, which ends
 // the trajectory here with kOutOfBounds.
+//
+// Deferred roots.  A root search runs for a few dozen iterations on one lane
+// while the rest of its wave waits, and a Poincare-section event brackets
+// every few dozen steps, so found inline the searches dominate the solve.
+// Without a terminate count (EventConfig::terminate = None) nothing in the
+// integration depends on a root, so the lane stores each bracketed step (its
+// interpolant and bracket) in a per-trajectory region in HBM and moves on; a
+// root pass then searches every bracket with one thread each, and a
+// bookkeeping pass walks each trajectory's brackets in order, applying the
+// solout's push rules.  A trajectory that outgrows its region, or whose
+// search hits a failed interpolation (a reference panic, which must end the
+// solve at that step), is solved again with inline roots.
 #pragma once
 
 #include "de_common.h"
@@ -37,7 +50,26 @@ struct EventArgs {
   double* t_out;         // [max_events * n]: t_out[k * n + i]
   double* y_out;         // [(k * D + c) * n + i]
   int32_t* n_out;        // [n] points pushed, or null
+  // Deferred roots (null brk: inline).  Entry k of trajectory i is
+  // brk[(i * bmax + k) * width ...]: kEntryMeta fields, then the lane's
+  // interpolant; the root pass overwrites the interpolant with its result.
+  double* brk;
+  int32_t* bcount;  // [n] brackets stored, -1: solve again inline
+  int32_t bmax;
+  int32_t width;
 };
+
+// Fields of a deferred bracket: the step's start and its length (or end, for
+// the fixed-step lanes' Hermite interpolant), g at both ends, and the time and
+// number of the t_eval points pushed before the bracket was seen.
+enum : int { kEntryT0 = 0, kEntryH = 1, kEntryG0 = 2, kEntryG1 = 3, kEntryTail = 4,
+             kEntryTevals = 5, kEntryMeta = 6 };
+// Result written by the root pass over the interpolant's first fields.
+enum : int { kResFlag = kEntryMeta, kResT = kEntryMeta + 1, kResY = kEntryMeta + 2 };
+enum : int { kResNone = 0, kResPoint = 1, kResPanic = 2 };
+// Lane status of a trajectory whose deferred region is full: its results are
+// produced by the inline solve that follows.
+constexpr int32_t kDeferOverflow = 100;
 
 // g(t, y) of the event function: y[comp] - value, t - value, or
 // y.minus(&point).dot(&normal) (State::minus = plus_scaled(-1, ..), dot folds
@@ -173,15 +205,57 @@ struct LaneEvent {
   DE_HD void reset() {}
   DE_HD void start(const EventArgs&, int64_t, double, const double*) {}
   DE_HD void finish(const EventArgs&, int64_t, int64_t) const {}
+  DE_HD void commit(bool) {}
 };
+
+// Shared by the two solouts: the deferred entry of a bracketed step.
+// Returns the entry to fill with the interpolant, or null (no bracket, or the
+// region is full: *overflow).
+DE_HD double* event_defer(const EventArgs& e, int64_t i, int32_t& k, bool need, double t0,
+                          double h, double g0, double g1, double tail, int32_t tevals,
+                          bool* overflow) {
+  if (!need) return nullptr;
+  if (k >= e.bmax) {
+    *overflow = true;
+    return nullptr;
+  }
+  double* ent = e.brk + ((int64_t)i * e.bmax + k) * e.width;
+  ent[kEntryT0] = t0;
+  ent[kEntryH] = h;
+  ent[kEntryG0] = g0;
+  ent[kEntryG1] = g1;
+  ent[kEntryTail] = tail;
+  ent[kEntryTevals] = (double)tevals;
+  k += 1;
+  return ent;
+}
+
+// Result of a root search written over the entry's interpolant.
+template <int D>
+DE_HD void event_result(double* ent, int flag, double t, const double* y) {
+  ent[kResFlag] = (double)flag;
+  ent[kResT] = t;
+#pragma unroll
+  for (int c = 0; c < D; ++c) ent[kResY + c] = flag == kResPoint ? y[c] : 0.0;
+}
+
+// Deferred-mode end of a trajectory: the bracket count (the bookkeeping pass
+// writes the event outputs), or -1 after an overflow.
+DE_HD void event_defer_finish(const EventArgs& e, int64_t i, int32_t k, bool overflow) {
+  e.bcount[i] = overflow ? -1 : k;
+}
 
 // EventWrappedSolout around DefaultSolout or TEvalSolout.
 template <int D>
 struct LaneEvent<D, 1> {
   double value, last_g, g_new;
   int32_t count, pushed;
+  int32_t k;  // deferred brackets stored
 
-  DE_HD void reset() { pushed = 0; }
+  DE_HD void reset() {
+    pushed = 0;
+    k = 0;
+  }
   // The solout call after init: only records g(t0, y0) (event.rs:346-355).
   DE_HD void start(const EventArgs& e, int64_t i, double t0, const double* y0) {
     value = e.value_traj ? e.value_traj[i] : e.value;
@@ -234,11 +308,77 @@ struct LaneEvent<D, 1> {
         }
       }
     }
-    last_g = g_new;
     return st;
   }
+  DE_HD void commit(bool take) { last_g = take ? g_new : last_g; }
   DE_HD void finish(const EventArgs& e, int64_t n, int64_t i) const {
     event_finish<D>(e, n, i, pushed);
+  }
+  // Deferred mode after pre(): the bracket's entry (or null), and the state
+  // advanced as post() does.  `tail` / `tevals` are the last t_eval point pushed
+  // and the number pushed so far.
+  DE_HD double* defer(const EventArgs& e, int64_t i, bool need, double t0, double h,
+                      double tail, int32_t tevals, bool* overflow) {
+    return event_defer(e, i, k, need, t0, h, last_g, g_new, tail, tevals, overflow);
+  }
+  // Root pass on one entry: brent_dekker on [t_prev, t_curr] as post() runs it.
+  template <class Interp>
+  DE_HD static void root(const EventArgs& e, int64_t i, double* ent, double t_prev,
+                         double t_curr, double dir, Interp& interp) {
+    const double v = e.value_traj ? e.value_traj[i] : e.value;
+    double a = t_prev, b = t_curr, fa = ent[kEntryG0], fb = ent[kEntryG1];
+    if ((dir > 0.0 && a > b) || (dir < 0.0 && a < b)) {
+      double x = a;
+      a = b;
+      b = x;
+      x = fa;
+      fa = fb;
+      fb = x;
+    }
+    auto g = [&](double t, const double* y) { return event_g<D>(e, v, t, y); };
+    double t_event, ye[D];
+    int flag = kResNone;
+    if (brent_dekker<D>(a, b, fa, fb, interp, g, &t_event))
+      flag = interp(t_event, ye) ? kResPoint : kResPanic;
+    event_result<D>(ent, flag, t_event, ye);
+  }
+  // Bookkeeping of one trajectory's entries, in order: the push rule of
+  // detect_event (event.rs:379-385) against the Solution's last time, which is
+  // the step's end under DefaultSolout, else the last t_eval point pushed
+  // since the previous bracket or the previous event point.  Returns false when
+  // the trajectory must be solved again inline.
+  template <class Bounds>
+  DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool teval_base,
+                         Bounds& bounds) {
+    const int32_t cnt = e.bcount[i];
+    if (cnt < 0) return false;
+    const double* base = e.brk + (int64_t)i * e.bmax * e.width;
+    for (int32_t k = 0; k < cnt; ++k)
+      if (base[(int64_t)k * e.width + kResFlag] == (double)kResPanic) return false;
+    int32_t pushed = 0;
+    double tail = as_f64(kNaNBits), tevals = 0.0;
+#pragma unroll 4
+    for (int32_t k = 0; k < cnt; ++k) {
+      const double* ent = base + (int64_t)k * e.width;
+      if (!teval_base) {
+        double t0, t1;
+        bounds(ent, &t0, &t1);
+        tail = t1;
+      } else if (ent[kEntryTevals] != tevals) {
+        tail = ent[kEntryTail];
+        tevals = ent[kEntryTevals];
+      }
+      if (ent[kResFlag] == (double)kResPoint) {
+        const double t = ent[kResT];
+        if (tail != tail || fabs(t - tail) > 1e-14) {
+          event_store<D>(e, n, i, pushed, t, ent + kResY);
+          pushed += 1;
+          tail = t;
+        }
+      }
+    }
+    event_finish<D>(e, n, i, pushed);
+    return true;
   }
 };
 
@@ -249,8 +389,12 @@ template <int D>
 struct LaneEvent<D, 2> {
   double value, last, off_new;
   int32_t pushed;
+  int32_t k;  // deferred brackets stored
 
-  DE_HD void reset() { pushed = 0; }
+  DE_HD void reset() {
+    pushed = 0;
+    k = 0;
+  }
   DE_HD void start(const EventArgs& e, int64_t i, double t0, const double* y0) {
     value = e.value_traj ? e.value_traj[i] : e.value;
     last = event_g<D>(e, value, t0, y0);
@@ -330,11 +474,50 @@ struct LaneEvent<D, 2> {
         }
       }
     }
-    last = off_new;
     return st;
   }
+  DE_HD void commit(bool take) { last = take ? off_new : last; }
   DE_HD void finish(const EventArgs& e, int64_t n, int64_t i) const {
     event_finish<D>(e, n, i, pushed);
+  }
+  DE_HD double* defer(const EventArgs& e, int64_t i, bool need, double t0, double h,
+                      double tail, int32_t tevals, bool* overflow) {
+    return event_defer(e, i, k, need, t0, h, last, off_new, tail, tevals, overflow);
+  }
+  template <class Interp>
+  DE_HD static void root(const EventArgs& e, int64_t i, double* ent, double t_prev,
+                         double t_curr, double, Interp& interp) {
+    LaneEvent<D, 2> s;
+    s.value = e.value_traj ? e.value_traj[i] : e.value;
+    const double lo = ent[kEntryG0], off = ent[kEntryG1];
+    bool found = false;
+    double tc = 0.0, yc[D];
+    int flag = kResPanic;
+    if (s.newton(e, t_prev, t_curr, lo, off, interp, &found, &tc)) {
+      if (!found) {
+        const double frac = -lo / (off - lo);
+        tc = t_prev + frac * (t_curr - t_prev);
+      }
+      if (interp(tc, yc)) flag = kResPoint;
+    }
+    event_result<D>(ent, flag, tc, yc);
+  }
+  template <class Bounds>
+  DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool, Bounds&) {
+    const int32_t cnt = e.bcount[i];
+    if (cnt < 0) return false;
+    const double* base = e.brk + (int64_t)i * e.bmax * e.width;
+    for (int32_t k = 0; k < cnt; ++k)
+      if (base[(int64_t)k * e.width + kResFlag] == (double)kResPanic) return false;
+    int32_t pushed = 0;
+#pragma unroll 4
+    for (int32_t k = 0; k < cnt; ++k) {
+      const double* ent = base + (int64_t)k * e.width;
+      event_store<D>(e, n, i, pushed, ent[kResT], ent + kResY);
+      pushed += 1;
+    }
+    event_finish<D>(e, n, i, pushed);
+    return true;
   }
 };
 

@@ -62,6 +62,8 @@ struct OdeArgs {
   const double* rtol_comp;      // per-component tolerances [c*n + i] (or state layout), or null
   const double* atol_comp;
   EventArgs ev;                 // event solout (lanes built with EV != 0)
+  const uint32_t* n_dev;        // persistent schedule: positions to take, read on the
+                                // device (null: n)
 };
 
 // Tolerances of one trajectory (Tolerance, src/tolerance.rs:11-68): a scalar
@@ -221,6 +223,9 @@ struct Dopri5Lane {
   static constexpr int D = F::kDim;
   static constexpr int kEv = EV;
   static constexpr int kWavesPerEu = 3;  // occupancy hint of the persistent kernel
+  // Deferred event bracket: the meta fields, then y, y_new, f(t, y) and
+  // f(t_new, y_new); the root pass forms the interpolant from them.
+  static constexpr int kEntryWidth = kEntryMeta + 4 * D;
   F f;
   double y[D], k0[D];
   double t, h, h_prev, tf, dir;
@@ -234,6 +239,12 @@ struct Dopri5Lane {
   // and return the lane by value so that the step loop's copy of the lane
   // never has its address taken (it stays in registers).
   DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, Dopri5Lane L) {
+    if constexpr (EV != 0) {
+      if (st == kDeferOverflow) {
+        event_defer_finish(a.ev, i, 0, true);
+        return;
+      }
+    }
 #pragma unroll
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
     a.t_final[i] = L.t;
@@ -244,9 +255,57 @@ struct Dopri5Lane {
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
-    if constexpr (EV != 0) L.ev.finish(a.ev, a.n, i);
+    if constexpr (EV != 0) {
+      if (a.ev.brk)
+        event_defer_finish(a.ev, i, L.ev.k, false);
+      else
+        L.ev.finish(a.ev, a.n, i);
+    }
   }
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
+
+  // The interpolant of a deferred entry, as the step's (ordinary.rs:301-336):
+  // the root pass reads the entry once and forms cont = (y, ydiff, bspl,
+  // cont3) with the step's operations.
+  DE_HD static void entry_bounds(const double* ent, double* t0, double* t1) {
+    *t0 = ent[kEntryT0];
+    *t1 = ent[kEntryT0] + ent[kEntryH];
+  }
+  DE_HD static void entry_root(const EventArgs& e, int64_t i, double* ent) {
+    const double tp = ent[kEntryT0], hp = ent[kEntryH];
+    const double t_new = tp + hp;
+    const double* d = ent + kEntryMeta;
+    double y0p[D], cont1[D], cont2[D], cont3[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double yo = d[c], yn = d[D + c], ko = d[2 * D + c], kn = d[3 * D + c];
+      y0p[c] = yo;
+      double ydiff = yn + (-1.0) * yo;
+      cont1[c] = ydiff;
+      double bspl = hp * ko + (-1.0) * ydiff;
+      cont2[c] = bspl;
+      cont3[c] = (ydiff + (-hp) * kn) + (-1.0) * bspl;
+    }
+    const double cont4 = 0.0 * hp;
+    auto interp = [&](double te, double* out) -> bool {
+      if (te < tp || te > t_new) return false;
+      const double s = (te - tp) / hp;
+      const double s1 = 1.0 - s;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double acc_ = cont4 * s1 + cont3[c];
+        acc_ = acc_ * s + cont2[c];
+        acc_ = acc_ * s1 + cont1[c];
+        out[c] = y0p[c] + s * acc_;
+      }
+      return true;
+    };
+    LaneEvent<D, EV>::root(e, i, ent, tp, t_new, signum(hp), interp);
+  }
+  DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
+    auto bounds = [](const double* ent, double* t0, double* t1) { entry_bounds(ent, t0, t1); };
+    return LaneEvent<D, EV>::book(a.ev, a.n, i, EV == 1 && DENSE && a.n_eval > 0, bounds);
+  }
 
   DE_COLD static Dopri5Lane start(const OdeArgs& a, int64_t i, bool* ok) {
     Dopri5Lane L;
@@ -437,10 +496,18 @@ struct Dopri5Lane {
     const bool take = accept && !stop;
     int32_t sol = -1;  // solout outcome: -1 continue, else the final status
     if constexpr (DENSE || EV != 0) {
-      if (take) {
-        // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
-        // ordinary.rs:312-333, on [t, t_new] (ordinary.rs:303-309).
-        double cont1[D], cont2[D], cont3[D];
+      // The event function at the attempt's end is evaluated branch-free; the
+      // interpolant is built only when an accepted step has t_eval points or
+      // brackets an event.
+      bool need = false;
+      if constexpr (EV != 0) need = take && ev.pre(a.ev, t_new, ynew);
+      const bool tev = DENSE && EV != 2 && take && a.n_eval > 0;
+      // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
+      // ordinary.rs:312-333, on [t, t_new] (ordinary.rs:303-309).
+      double cont1[D], cont2[D], cont3[D];
+      const double hp = h, tp = t;
+      const double* y0p = y;
+      if (tev || (need && !a.ev.brk)) {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
           double ydiff = ynew[c] + (-1.0) * y[c];
@@ -449,23 +516,51 @@ struct Dopri5Lane {
           cont2[c] = bspl;
           cont3[c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
         }
-        const double hp = h, tp = t;
-        const double cont4 = 0.0 * h;
-        const double* y0p = y;
-        auto interp = [&](double te, double* out) -> bool {
-          if (te < tp || te > t_new) return false;
-          double s = (te - tp) / hp;
-          double s1 = 1.0 - s;
+      }
+      const double cont4 = 0.0 * h;
+      auto interp = [&](double te, double* out) -> bool {
+        if (te < tp || te > t_new) return false;
+        double s = (te - tp) / hp;
+        double s1 = 1.0 - s;
 #pragma unroll
-          for (int c = 0; c < D; ++c) {
-            double acc_ = cont4 * s1 + cont3[c];
-            acc_ = acc_ * s + cont2[c];
-            acc_ = acc_ * s1 + cont1[c];
-            out[c] = y0p[c] + s * acc_;
+        for (int c = 0; c < D; ++c) {
+          double acc_ = cont4 * s1 + cont3[c];
+          acc_ = acc_ * s + cont2[c];
+          acc_ = acc_ * s1 + cont1[c];
+          out[c] = y0p[c] + s * acc_;
+        }
+        return true;
+      };
+      // The solout chain of an accepted step (solve_ivp.rs:242-259): TEvalSolout
+      // or DefaultSolout, then EventWrappedSolout's detection (event.rs:499-522);
+      // or the crossing solout alone.
+      if (tev) {
+        if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
+          sol = kOutOfBounds;
+      } else {
+        tail = take ? t_new : tail;
+      }
+      if constexpr (EV != 0) {
+        if (need && sol < 0) {
+          if (a.ev.brk) {
+            bool overflow = false;
+            double* ent = ev.defer(a.ev, i, true, tp, hp, tail, pushed, &overflow);
+            if (ent) {
+              double* dst = ent + kEntryMeta;
+#pragma unroll
+              for (int c = 0; c < D; ++c) {
+                dst[c] = y0p[c];
+                dst[D + c] = ynew[c];
+                dst[2 * D + c] = k0[c];
+                dst[3 * D + c] = dn[c];
+              }
+            }
+            if (overflow) sol = kDeferOverflow;
+          } else {
+            sol = ev.post(a.ev, a.n, i, dir, true, t_new, tp, tail, interp);
           }
-          return true;
-        };
-        sol = solout_step(a, i, t_new, tp, ynew, interp);
+        }
+        ev.commit(take);
       }
     }
     h_prev = take ? h : h_prev;
@@ -485,33 +580,6 @@ struct Dopri5Lane {
              : (sol >= 0 ? sol : (take && fabs(tf - t) <= kTenEps ? kComplete : -1));
     h = constrain_step_size(h * scale, a.h_min, a.h_max);
     return result;
-  }
-
-  // The solout chain after an accepted step (solve_ivp.rs:242-259): TEvalSolout
-  // or DefaultSolout, then the event detection of EventWrappedSolout
-  // (event.rs:499-522); or the crossing solout alone.
-  template <class Interp>
-  DE_HD int32_t solout_step(const OdeArgs& a, int64_t i, double t_new, double tp,
-                            const double* ynew, Interp& interp) {
-    int32_t sol = -1;
-    if constexpr (EV == 2) {
-      const bool need = ev.pre(a.ev, t_new, ynew);
-      sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
-    } else {
-      if (DENSE && a.n_eval > 0) {
-        if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
-          sol = kOutOfBounds;
-      } else {
-        tail = t_new;
-      }
-      if constexpr (EV == 1) {
-        if (sol < 0) {
-          const bool need = ev.pre(a.ev, t_new, ynew);
-          sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
-        }
-      }
-    }
-    return sol;
   }
 
   // step() + finish(): returns true when the trajectory ended.
@@ -550,6 +618,8 @@ struct Dop853Lane {
   // Occupancy hint: with an event solout the 16-stage lane does not fit the
   // 168 registers of three waves per SIMD without spilling in the step.
   static constexpr int kWavesPerEu = EV != 0 ? 2 : 3;
+  // Deferred event bracket: the meta fields, then cont[0..7].
+  static constexpr int kEntryWidth = kEntryMeta + 8 * D;
   F f;
   double y[D], k0[D];
   double t, h, h_prev, tf, dir;
@@ -560,6 +630,12 @@ struct Dop853Lane {
   bool rejected_last;
 
   DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, Dop853Lane L) {
+    if constexpr (EV != 0) {
+      if (st == kDeferOverflow) {
+        event_defer_finish(a.ev, i, 0, true);
+        return;
+      }
+    }
 #pragma unroll
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
     a.t_final[i] = L.t;
@@ -570,9 +646,53 @@ struct Dop853Lane {
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
-    if constexpr (EV != 0) L.ev.finish(a.ev, a.n, i);
+    if constexpr (EV != 0) {
+      if (a.ev.brk)
+        event_defer_finish(a.ev, i, L.ev.k, false);
+      else
+        L.ev.finish(a.ev, a.n, i);
+    }
   }
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
+
+  DE_HD static void entry_bounds(const double* ent, double* t0, double* t1) {
+    *t0 = ent[kEntryT0];
+    *t1 = ent[kEntryT0] + ent[kEntryH];
+  }
+  DE_HD static bool entry_interp(const double* ent, double te, double* out) {
+    const double tp = ent[kEntryT0], hp = ent[kEntryH];
+    const double t_new = tp + hp;
+    if (te < tp || te > t_new) return false;
+    const double* d = ent + kEntryMeta;
+    const double s = (te - tp) / hp;
+    const double s1 = 1.0 - s;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double q = d[7 * D + c];
+      q = q * s1 + d[6 * D + c];
+      q = q * s + d[5 * D + c];
+      q = q * s1 + d[4 * D + c];
+      q = q * s1 + d[3 * D + c];
+      q = q * s + d[2 * D + c];
+      q = q * s1 + d[D + c];
+      out[c] = d[c] + s * q;
+    }
+    return true;
+  }
+  // The root pass reads the entry once into registers.
+  DE_HD static void entry_root(const EventArgs& e, int64_t i, double* ent) {
+    double loc[kEntryWidth];
+#pragma unroll
+    for (int f = 0; f < kEntryWidth; ++f) loc[f] = ent[f];
+    auto interp = [&](double te, double* out) { return entry_interp(loc, te, out); };
+    double t0, t1;
+    entry_bounds(loc, &t0, &t1);
+    LaneEvent<D, EV>::root(e, i, ent, t0, t1, signum(loc[kEntryH]), interp);
+  }
+  DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
+    auto bounds = [](const double* ent, double* t0, double* t1) { entry_bounds(ent, t0, t1); };
+    return LaneEvent<D, EV>::book(a.ev, a.n, i, EV == 1 && DENSE && a.n_eval > 0, bounds);
+  }
 
   DE_COLD static Dop853Lane start(const OdeArgs& a, int64_t i, bool* ok) {
     Dop853Lane L;
@@ -768,65 +888,74 @@ struct Dop853Lane {
     const bool take = accept && !stop;
     int32_t sol = -1;
     if constexpr (DENSE || EV != 0) {
-      if (take) {
-        // The continuous extension is built only when a t_eval point or an
-        // event bracket needs it: cont[0..3] as DOPRI5; k[12] = f(t_new,
-        // y_new); k[13..15] at c[13..15]; cont[i] = h sum_j bi[i][j] k[j] for
-        // i = 4..7.
-        bool need = false;
-        if constexpr (EV != 0) need = ev.pre(a.ev, t_new, ynew);
-        const bool tev = DENSE && EV != 2 && a.n_eval > 0;
-        double cont[8][D];
-        if (tev || need) {
+      // The continuous extension is built only when an accepted step has
+      // t_eval points or brackets an event: cont[0..3] as DOPRI5; k[12] =
+      // f(t_new, y_new); k[13..15] at c[13..15]; cont[i] = h sum_j bi[i][j] k[j]
+      // for i = 4..7.
+      bool need = false;
+      if constexpr (EV != 0) need = take && ev.pre(a.ev, t_new, ynew);
+      const bool tev = DENSE && EV != 2 && take && a.n_eval > 0;
+      double cont[8][D];
+      if (tev || need) {
 #pragma unroll
-          for (int c = 0; c < D; ++c) {
-            double ydiff = ynew[c] + (-1.0) * y[c];
-            cont[0][c] = y[c];
-            cont[1][c] = ydiff;
-            double bspl = h * k[0][c] + (-1.0) * ydiff;
-            cont[2][c] = bspl;
-            cont[3][c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
-            k[12][c] = dn[c];
-          }
-          dense_stages<13>(k, y, t, h);
-#pragma unroll
-          for (int r = 4; r < 8; ++r)
-#pragma unroll
-            for (int c = 0; c < D; ++c) cont[r][c] = comb<16>(d8::BI[r], k, c) * h;
+        for (int c = 0; c < D; ++c) {
+          double ydiff = ynew[c] + (-1.0) * y[c];
+          cont[0][c] = y[c];
+          cont[1][c] = ydiff;
+          double bspl = h * k[0][c] + (-1.0) * ydiff;
+          cont[2][c] = bspl;
+          cont[3][c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
+          k[12][c] = dn[c];
         }
-        const double hp = h, tp = t;
-        auto interp = [&](double te, double* out) -> bool {
-          // ordinary.rs:303-333 with 8 coefficients: factors s1, s, s1 for
-          // cont[6], cont[5], cont[4], then s1, s, s1 for cont[3..1].
-          if (te < tp || te > t_new) return false;
-          const double s = (te - tp) / hp;
-          const double s1 = 1.0 - s;
+        dense_stages<13>(k, y, t, h);
 #pragma unroll
-          for (int c = 0; c < D; ++c) {
-            double q = cont[7][c];
-            q = q * s1 + cont[6][c];
-            q = q * s + cont[5][c];
-            q = q * s1 + cont[4][c];
-            q = q * s1 + cont[3][c];
-            q = q * s + cont[2][c];
-            q = q * s1 + cont[1][c];
-            out[c] = cont[0][c] + s * q;
-          }
-          return true;
-        };
-        if constexpr (EV == 2) {
-          sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
-        } else {
-          if (tev) {
-            if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
-              sol = kOutOfBounds;
+        for (int r = 4; r < 8; ++r)
+#pragma unroll
+          for (int c = 0; c < D; ++c) cont[r][c] = comb<16>(d8::BI[r], k, c) * h;
+      }
+      const double hp = h, tp = t;
+      auto interp = [&](double te, double* out) -> bool {
+        // ordinary.rs:303-333 with 8 coefficients: factors s1, s, s1 for
+        // cont[6], cont[5], cont[4], then s1, s, s1 for cont[3..1].
+        if (te < tp || te > t_new) return false;
+        const double s = (te - tp) / hp;
+        const double s1 = 1.0 - s;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          double q = cont[7][c];
+          q = q * s1 + cont[6][c];
+          q = q * s + cont[5][c];
+          q = q * s1 + cont[4][c];
+          q = q * s1 + cont[3][c];
+          q = q * s + cont[2][c];
+          q = q * s1 + cont[1][c];
+          out[c] = cont[0][c] + s * q;
+        }
+        return true;
+      };
+      if (tev) {
+        if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
+          sol = kOutOfBounds;
+      } else {
+        tail = take ? t_new : tail;
+      }
+      if constexpr (EV != 0) {
+        if (need && sol < 0) {
+          if (a.ev.brk) {
+            bool overflow = false;
+            double* ent = ev.defer(a.ev, i, true, tp, hp, tail, pushed, &overflow);
+            if (ent) {
+#pragma unroll
+              for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int c = 0; c < D; ++c) ent[kEntryMeta + r * D + c] = cont[r][c];
+            }
+            if (overflow) sol = kDeferOverflow;
           } else {
-            tail = t_new;
-          }
-          if constexpr (EV == 1) {
-            if (sol < 0) sol = ev.post(a.ev, a.n, i, dir, need, t_new, tp, tail, interp);
+            sol = ev.post(a.ev, a.n, i, dir, true, t_new, tp, tail, interp);
           }
         }
+        ev.commit(take);
       }
     }
     h_prev = take ? h : h_prev;
@@ -863,6 +992,9 @@ struct FixedLane {
   static constexpr int D = F::kDim;
   static constexpr int kEv = EV;
   static constexpr int kWavesPerEu = 3;  // occupancy hint of the persistent kernel
+  // Deferred event bracket: the meta fields (kEntryH holds the step's end),
+  // then y_prev, dydt_prev, y, dydt of the Hermite interpolant.
+  static constexpr int kEntryWidth = kEntryMeta + 4 * D;
   F f;
   double y[D], k0[D];
   double t, h, tf, dir;
@@ -872,6 +1004,12 @@ struct FixedLane {
   int32_t next_eval, pushed;
 
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const {
+    if constexpr (EV != 0) {
+      if (st == kDeferOverflow) {
+        event_defer_finish(a.ev, i, 0, true);
+        return;
+      }
+    }
 #pragma unroll
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = y[c];
     a.t_final[i] = t;
@@ -881,7 +1019,49 @@ struct FixedLane {
     a.status[i] = st;
     if (a.n_dense) a.n_dense[i] = pushed;
     if (a.n_eval > 0) teval_tail_nan<D>(a, i, next_eval);
-    if constexpr (EV != 0) evs.finish(a.ev, a.n, i);
+    if constexpr (EV != 0) {
+      if (a.ev.brk)
+        event_defer_finish(a.ev, i, evs.k, false);
+      else
+        evs.finish(a.ev, a.n, i);
+    }
+  }
+
+  // cubic_hermite_interpolate of a deferred entry, as the step's.
+  DE_HD static void entry_bounds(const double* ent, double* t0, double* t1) {
+    *t0 = ent[kEntryT0];
+    *t1 = ent[kEntryH];
+  }
+  DE_HD static bool entry_interp(const double* ent, double te, double* out) {
+    const double tp = ent[kEntryT0], tc = ent[kEntryH];
+    if (te < tp || te > tc) return false;
+    const double* d = ent + kEntryMeta;
+    double hh = tc - tp;
+    double s = (te - tp) / hh;
+    double s2 = s * s;
+    double s3 = s2 * s;
+    double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+    double h10 = s3 - 2.0 * s2 + s;
+    double h01 = -2.0 * s3 + 3.0 * s2;
+    double h11 = s3 - s2;
+    double w10 = h10 * hh, w11 = h11 * hh;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      out[c] = ((h00 * d[c] + w10 * d[D + c]) + h01 * d[2 * D + c]) + w11 * d[3 * D + c];
+    return true;
+  }
+  DE_HD static void entry_root(const EventArgs& e, int64_t i, double* ent) {
+    double loc[kEntryWidth];
+#pragma unroll
+    for (int f = 0; f < kEntryWidth; ++f) loc[f] = ent[f];
+    auto interp = [&](double te, double* out) { return entry_interp(loc, te, out); };
+    double t0, t1;
+    entry_bounds(loc, &t0, &t1);
+    LaneEvent<D, EV>::root(e, i, ent, t0, t1, signum(t1 - t0), interp);
+  }
+  DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
+    auto bounds = [](const double* ent, double* t0, double* t1) { entry_bounds(ent, t0, t1); };
+    return LaneEvent<D, EV>::book(a.ev, a.n, i, EV == 1 && a.n_eval > 0, bounds);
   }
 
   DE_HD bool begin(const OdeArgs& a, int64_t i) { return init(a, i); }
@@ -991,22 +1171,34 @@ struct FixedLane {
           out[c] = ((h00 * yp[c] + w10 * kp[c]) + h01 * ynext[c]) + w11 * dn[c];
         return true;
       };
-      if constexpr (EV == 2) {
-        const bool need = evs.pre(a.ev, t, ynext);
-        sol = evs.post(a.ev, a.n, i, dir, need, t, tp, tail, interp);
+      if (EV != 2 && a.n_eval > 0) {
+        if (!teval_emit<D>(a, i, dir, next_eval, pushed, t, tp, ynext, interp, tail))
+          sol = kOutOfBounds;
       } else {
-        if (a.n_eval > 0) {
-          if (!teval_emit<D>(a, i, dir, next_eval, pushed, t, tp, ynext, interp, tail))
-            sol = kOutOfBounds;
-        } else {
-          tail = t;
-        }
-        if constexpr (EV == 1) {
-          if (sol < 0) {
-            const bool need = evs.pre(a.ev, t, ynext);
-            sol = evs.post(a.ev, a.n, i, dir, need, t, tp, tail, interp);
+        tail = t;
+      }
+      if constexpr (EV != 0) {
+        const bool need = evs.pre(a.ev, t, ynext);
+        if (need && sol < 0) {
+          if (a.ev.brk) {
+            bool overflow = false;
+            double* ent = evs.defer(a.ev, i, true, tp, tc, tail, pushed, &overflow);
+            if (ent) {
+              double* dst = ent + kEntryMeta;
+#pragma unroll
+              for (int c = 0; c < D; ++c) {
+                dst[c] = yp[c];
+                dst[D + c] = kp[c];
+                dst[2 * D + c] = ynext[c];
+                dst[3 * D + c] = dn[c];
+              }
+            }
+            if (overflow) sol = kDeferOverflow;
+          } else {
+            sol = evs.post(a.ev, a.n, i, dir, true, t, tp, tail, interp);
           }
         }
+        evs.commit(true);
       }
     }
     (void)hp;

@@ -37,6 +37,7 @@ struct SdeArgs {
 namespace parity {
 int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 int launch_ode_event(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+int launch_ode_crossing(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 int launch_sde(int family, int scheme, const SdeArgs& a, hipStream_t s);
 }  // namespace parity
 namespace fast {

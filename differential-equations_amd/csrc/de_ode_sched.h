@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "de_lane.h"
 #include "de_launch.h"
 
@@ -57,7 +59,7 @@ ode_persistent_kernel(const OdeArgs* __restrict__ ap) {
   bool drained = false;
   int64_t idx = 0;
   const int lane = __lane_id();
-  const unsigned long long n = (unsigned long long)a.n;
+  const unsigned long long n = a.n_dev ? (unsigned long long)*a.n_dev : (unsigned long long)a.n;
   for (;;) {
     if (!drained) {
       bool need = !have;
@@ -108,6 +110,83 @@ __global__ void normalise_normal_kernel(OdeArgs* args, double* out) {
   const double inv = 1.0 / norm;
   for (int c = 0; c < D; ++c) out[c] = norm > 2.220446049250313080847e-16 ? nrm[c] * inv : nrm[c];
   args->ev.normal = out;
+}
+
+// ---- Deferred event roots -------------------------------------------------
+// Root pass: each block takes the brackets of kBlock consecutive trajectories,
+// compacted by a scan of their counts in LDS, one bracket per thread at a
+// time, so waves stay full whatever the per-trajectory counts.
+template <class Lane>
+__global__ __launch_bounds__(kBlock) void event_root_kernel(const OdeArgs* __restrict__ ap) {
+  const OdeArgs& a = *ap;
+  __shared__ int32_t off[kBlock + 1];
+  const int t = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * kBlock;
+  int32_t cnt = 0;
+  if (i0 + t < a.n) cnt = max(a.ev.bcount[i0 + t], 0);
+  off[t + 1] = cnt;
+  if (t == 0) off[0] = 0;
+  __syncthreads();
+  for (int d = 1; d < kBlock; d <<= 1) {  // inclusive scan of off[1..kBlock]
+    const int32_t add = t >= d ? off[t + 1 - d] : 0;
+    __syncthreads();
+    off[t + 1] += add;
+    __syncthreads();
+  }
+  const int32_t total = off[kBlock];
+  for (int32_t j = t; j < total; j += kBlock) {
+    int lo = 0, hi = kBlock;  // the last lo with off[lo] <= j
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (off[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    const int64_t i = i0 + lo;
+    Lane::entry_root(a.ev, i, a.ev.brk + (i * a.ev.bmax + (j - off[lo])) * a.ev.width);
+  }
+}
+
+// Bookkeeping pass: one thread per trajectory; the trajectories to solve
+// again inline are appended to `rerun`.
+template <class Lane>
+__global__ __launch_bounds__(kBlock) void event_book_kernel(const OdeArgs* __restrict__ ap,
+                                                            uint32_t* rerun, uint32_t* nrerun) {
+  const OdeArgs& a = *ap;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  if (!Lane::entry_book(a, i)) rerun[atomicAdd(nrerun, 1u)] = (uint32_t)i;
+}
+
+static __global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
+// Arguments of the inline solve of the listed trajectories.
+static __global__ void rerun_args_kernel(const OdeArgs* src, OdeArgs* dst, const uint32_t* order,
+                                         const uint32_t* count) {
+  OdeArgs r = *src;
+  r.ev.brk = nullptr;
+  r.ev.bcount = nullptr;
+  r.order = order;
+  r.n_dev = count;
+  *dst = r;
+}
+
+// Deferred entries per trajectory: max_events + half again (at least 32 more),
+// halved while the region would exceed this budget; below 16 the roots are
+// found inline.
+constexpr size_t kDeferBudget = (size_t)32 << 30;
+
+template <class Lane>
+static int64_t defer_slots(const OdeArgs& a) {
+  if constexpr (Lane::kEv == 0) {
+    return 0;
+  } else {
+    if (a.ev.terminate != 0 || a.n > 0x7fffffffLL) return 0;
+    int64_t bmax = (int64_t)a.ev.max_events + std::max<int64_t>(a.ev.max_events / 2, 32);
+    while (bmax >= 16 && (size_t)a.n * (size_t)bmax * Lane::kEntryWidth * sizeof(double) >
+                             kDeferBudget)
+      bmax /= 2;
+    return bmax >= 16 ? bmax : 0;
+  }
 }
 
 // ---- Longest-first order for the persistent schedule ----------------------
@@ -208,38 +287,90 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
   int64_t want = (int64_t)num_cu * blocks_per_cu;
   int64_t need = (a.n + kBlock - 1) / kBlock;
   int64_t grid = want < need ? want : need;
-  // Stream-ordered workspace: the launch arguments, for event lanes 256 bytes
-  // for the normalised hyperplane normal, and, for the longest-first order
-  // (schedule PERSISTENT, batches of a few thousand trajectories or more), 256
-  // bucket counts, the order and one bucket byte per trajectory.
+  // Stream-ordered workspace: the launch arguments; for event lanes 256 bytes
+  // for the normalised hyperplane normal and, with deferred roots, a second
+  // argument block, the per-trajectory bracket counts, the rerun list and its
+  // count, and the bracket regions; for the longest-first order (schedule
+  // PERSISTENT, batches of a few thousand trajectories or more), 256 bucket
+  // counts, the order and one bucket byte per trajectory.
   const bool ordered = schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL;
   const size_t args_bytes = (sizeof(OdeArgs) + 255) & ~(size_t)255;
   const size_t ev_bytes = Lane::kEv != 0 ? 256 : 0;
-  const size_t bytes =
-      args_bytes + ev_bytes +
-      (ordered ? kBuckets * sizeof(uint32_t) + (size_t)a.n * (sizeof(uint32_t) + 1) : 0);
+  const size_t order_bytes =
+      ordered ? ((kBuckets * sizeof(uint32_t) + (size_t)a.n * (sizeof(uint32_t) + 1) + 255) &
+                 ~(size_t)255)
+              : 0;
+  const size_t list_bytes = ((size_t)a.n * sizeof(uint32_t) + 255) & ~(size_t)255;
+  int64_t bmax = defer_slots<Lane>(a);
   void* work = nullptr;
-  if (hipMallocAsync(&work, bytes, s) != hipSuccess) return -1;
-  OdeArgs* d_args = (OdeArgs*)work;
+  size_t defer_bytes = 0;
+  for (;;) {
+    defer_bytes = bmax > 0 ? args_bytes + 2 * list_bytes + 256 +
+                                 (size_t)a.n * (size_t)bmax * Lane::kEntryWidth * sizeof(double)
+                           : 0;
+    if (hipMallocAsync(&work, args_bytes + ev_bytes + order_bytes + defer_bytes, s) == hipSuccess)
+      break;
+    (void)hipGetLastError();
+    if (bmax == 0) return -1;
+    bmax = 0;  // no room for the bracket regions: roots inline
+  }
+  char* w = (char*)work;
+  OdeArgs* d_args = (OdeArgs*)w;
   OdeArgs pa = a;
   pa.order = nullptr;
+  pa.n_dev = nullptr;
   if (ordered) {
-    uint32_t* hist = (uint32_t*)((char*)work + args_bytes + ev_bytes);
+    uint32_t* hist = (uint32_t*)(w + args_bytes + ev_bytes);
     uint32_t* order = hist + kBuckets;
     uint8_t* bucket = (uint8_t*)(order + a.n);
     launch_order<Lane>(a, bucket, hist, order, s);
     pa.order = order;
+  }
+  OdeArgs* d_rerun_args = nullptr;
+  uint32_t *bcount = nullptr, *rerun = nullptr, *nrerun = nullptr;
+  if constexpr (Lane::kEv != 0) {
+    pa.ev.brk = nullptr;
+    pa.ev.bcount = nullptr;
+    if (bmax > 0) {
+      char* d = w + args_bytes + ev_bytes + order_bytes;
+      d_rerun_args = (OdeArgs*)d;
+      bcount = (uint32_t*)(d + args_bytes);
+      rerun = (uint32_t*)(d + args_bytes + list_bytes);
+      nrerun = (uint32_t*)(d + args_bytes + 2 * list_bytes);
+      pa.ev.brk = (double*)(d + args_bytes + 2 * list_bytes + 256);
+      pa.ev.bcount = (int32_t*)bcount;
+      pa.ev.bmax = (int32_t)bmax;
+      pa.ev.width = Lane::kEntryWidth;
+    }
   }
   hipLaunchKernelGGL(store_args_kernel, dim3(1), dim3(1), 0, s, pa, d_args);
   if constexpr (Lane::kEv != 0) {
     static_assert(Lane::D * sizeof(double) <= 256, "normal buffer");
     if (a.ev.solout == kSoloutHyperplane)
       hipLaunchKernelGGL(normalise_normal_kernel<Lane::D>, dim3(1), dim3(1), 0, s, d_args,
-                         (double*)((char*)work + args_bytes));
+                         (double*)(w + args_bytes));
   }
   hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
   hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s,
                      (const OdeArgs*)d_args);
+  if constexpr (Lane::kEv != 0) {
+    if (bmax > 0) {
+      // Roots of every stored bracket, the push rules per trajectory, and the
+      // inline solve of the trajectories listed for it (on the full grid; the
+      // count is known only on the device).
+      const unsigned gx = (unsigned)((a.n + kBlock - 1) / kBlock);
+      hipLaunchKernelGGL(event_root_kernel<Lane>, dim3(gx), dim3(kBlock), 0, s,
+                         (const OdeArgs*)d_args);
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, nrerun);
+      hipLaunchKernelGGL(event_book_kernel<Lane>, dim3(gx), dim3(kBlock), 0, s,
+                         (const OdeArgs*)d_args, rerun, nrerun);
+      hipLaunchKernelGGL(rerun_args_kernel, dim3(1), dim3(1), 0, s, (const OdeArgs*)d_args,
+                         d_rerun_args, (const uint32_t*)rerun, (const uint32_t*)nrerun);
+      hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
+      hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s,
+                         (const OdeArgs*)d_rerun_args);
+    }
+  }
   if (hipFreeAsync(work, s) != hipSuccess) return -1;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

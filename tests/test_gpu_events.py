@@ -169,3 +169,25 @@ def test_server_refuses_event_requests():
     with deamd.Server(8 << 30) as srv:
         with pytest.raises(RuntimeError):
             srv.submit(ens)
+
+
+@pytest.mark.parametrize("name", ["dopri5", "dop853", "rk4"])
+def test_deferred_regions_overflow_to_inline_solves(orc, name):
+    # Without a terminate count the roots are searched after the integration,
+    # from per-trajectory regions of max_events + max(max_events / 2, 32)
+    # brackets.  Oscillators of widely spread frequencies give 2 to ~250 zero
+    # crossings, so some trajectories outgrow their region and are solved
+    # again with inline roots; the results must not show which.
+    rng = np.random.default_rng(71)
+    n = 4096
+    k = np.exp(rng.uniform(np.log(0.5), np.log(400.0), n))
+    y0 = np.vstack([np.ones(n), rng.uniform(-1, 1, n)])
+    for cap in (0, 8):
+        ens = IVP.ode(Harmonic(k), 0.0, 40.0, y0).method(method(name, h=0.002)) \
+            .event(Event.component(0, 0.0)).options(max_events=cap)
+        want = orc.solve_ode(ens, threads=THREADS)
+        assert (want.n_events > 40 + cap).any() and (want.n_events < 32).any()
+        assert_events_equal(ens.solve(), want)
+        ens = IVP.ode(Harmonic(k), 0.0, 40.0, y0).method(method(name, h=0.002)) \
+            .crossing(1, 0.0).options(max_events=cap)
+        assert_events_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS))
