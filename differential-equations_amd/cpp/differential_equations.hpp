@@ -31,7 +31,9 @@
 namespace differential_equations {
 
 // ------------------------------------------------------------ results ----
-enum class ErrorKind { BadInput, MaxSteps, StepSize, Stiffness };
+// OutOfBounds: the reference solve panics here (a solout unwrapped an
+// Error::OutOfBounds interpolation); t, y are the state the step reached.
+enum class ErrorKind { BadInput, MaxSteps, StepSize, Stiffness, OutOfBounds };
 
 struct Error {  // Error::{BadInput, MaxSteps{t,y}, StepSize{t,y}, Stiffness{t,y}}
   ErrorKind kind;
@@ -48,17 +50,67 @@ struct Evals {  // src/stats.rs:16
   int64_t function = 0;
 };
 
+enum class Status { Complete, Interrupted };  // src/status.rs (successful outcomes)
+
 struct Solution {  // src/solution.rs:30-53 (points kept by the solout)
   std::vector<double> t;
   std::vector<std::vector<double>> y;
+  // The points an event solout pushed (IVP::event); the crossing solouts'
+  // points are t / y themselves, as they replace the output.
+  std::vector<double> event_t;
+  std::vector<std::vector<double>> event_y;
   Steps steps;
   Evals evals;
+  Status status = Status::Complete;
 };
 
 struct Result {  // Result<Solution, Error>
   bool ok;
   Solution value;
   Error error;
+};
+
+// ------------------------------------------------------------- events -----
+enum class CrossingDirection { Both, Positive, Negative };  // src/solout/mod.rs:38-45
+
+struct EventConfig {  // src/solout/event.rs:8-50
+  CrossingDirection dir = CrossingDirection::Both;
+  int32_t terminate = 0;  // 0: None
+  EventConfig direction(CrossingDirection d) const { EventConfig c = *this; c.dir = d; return c; }
+  EventConfig terminate_after(int32_t k) const { EventConfig c = *this; c.terminate = k; return c; }
+  EventConfig terminal() const { return terminate_after(1); }
+};
+
+// An event function built into the device kernels (the Event trait): y[c] - v,
+// t - v or y.minus(&point).dot(&normal); v is one value or one per trajectory.
+struct Event {
+  int32_t fn = DE_EVENT_COMPONENT;
+  int32_t component = 0;
+  std::vector<double> value{0.0};
+  std::vector<double> point, normal;
+  EventConfig config;
+  static Event component_at(int32_t c, std::vector<double> v, EventConfig cfg = {}) {
+    Event e;
+    e.component = c;
+    e.value = std::move(v);
+    e.config = cfg;
+    return e;
+  }
+  static Event time(std::vector<double> v, EventConfig cfg = {}) {
+    Event e;
+    e.fn = DE_EVENT_TIME;
+    e.value = std::move(v);
+    e.config = cfg;
+    return e;
+  }
+  static Event plane(std::vector<double> p, std::vector<double> nrm, EventConfig cfg = {}) {
+    Event e;
+    e.fn = DE_EVENT_PLANE;
+    e.point = std::move(p);
+    e.normal = std::move(nrm);
+    e.config = cfg;
+    return e;
+  }
 };
 
 // ------------------------------------------------------------ methods ----
@@ -149,26 +201,48 @@ class EnsembleSolution {
   std::vector<double> y, t, dense;
   std::vector<int64_t> accepted, rejected, evals;
   std::vector<int32_t> status, n_dense;
+  int32_t solout = DE_SOLOUT_DEFAULT;
+  int32_t max_events = 0;
+  std::vector<double> event_t, event_y;
+  std::vector<int32_t> n_events;
 
   // The reference's Result<Solution, Error> of trajectory i.
   Result at(int64_t i) const {
     Result r;
     std::vector<double> yi(dim);
     for (int c = 0; c < dim; ++c) yi[c] = y[(size_t)c * n + i];
-    if (status[i] != DE_STATUS_COMPLETE) {
+    if (status[i] != DE_STATUS_COMPLETE && status[i] != DE_STATUS_INTERRUPTED) {
       r.ok = false;
-      r.error.kind = status[i] == DE_STATUS_MAX_STEPS   ? ErrorKind::MaxSteps
-                     : status[i] == DE_STATUS_STEP_SIZE ? ErrorKind::StepSize
-                     : status[i] == DE_STATUS_STIFFNESS ? ErrorKind::Stiffness
-                                                        : ErrorKind::BadInput;
+      r.error.kind = status[i] == DE_STATUS_MAX_STEPS       ? ErrorKind::MaxSteps
+                     : status[i] == DE_STATUS_STEP_SIZE     ? ErrorKind::StepSize
+                     : status[i] == DE_STATUS_STIFFNESS     ? ErrorKind::Stiffness
+                     : status[i] == DE_STATUS_OUT_OF_BOUNDS ? ErrorKind::OutOfBounds
+                                                            : ErrorKind::BadInput;
       r.error.t = t[i];
       r.error.y = yi;
       return r;
     }
     r.ok = true;
+    r.value.status = status[i] == DE_STATUS_INTERRUPTED ? Status::Interrupted : Status::Complete;
     r.value.steps.accepted = accepted[i];
     r.value.steps.rejected = rejected[i];
     r.value.evals.function = evals[i];
+    const int32_t kept = n_events.empty() ? 0 : std::min(n_events[i], max_events);
+    std::vector<double> et;
+    std::vector<std::vector<double>> ey;
+    for (int32_t k = 0; k < kept; ++k) {
+      std::vector<double> yk(dim);
+      for (int c = 0; c < dim; ++c) yk[c] = event_y[((size_t)k * dim + c) * n + i];
+      et.push_back(event_t[(size_t)k * n + i]);
+      ey.push_back(yk);
+    }
+    if (solout == DE_SOLOUT_CROSSING || solout == DE_SOLOUT_HYPERPLANE) {
+      r.value.t = et;  // the crossing solouts replace the output
+      r.value.y = ey;
+      return r;
+    }
+    r.value.event_t = et;
+    r.value.event_y = ey;
     if (t_eval.empty()) {  // final state (the last point DefaultSolout keeps)
       r.value.t.push_back(t[i]);
       r.value.y.push_back(yi);
@@ -199,10 +273,38 @@ class IVP {
   IVP t_eval(std::vector<double> points) const {
     IVP p = *this;
     p.t_eval_ = points;
+    p.solout_ = DE_SOLOUT_DEFAULT;  // with_solout replaces the whole solout
     return p;
   }
   // Launch schedule (DE_SCHED_*): persistent lane refill by default.
   IVP schedule(int32_t s) const { IVP p = *this; p.schedule_ = s; return p; }
+  // IVP::event (src/ivp.rs:662-678): event detection around the current output.
+  IVP event(const Event& e) const {
+    IVP p = *this;
+    p.solout_ = DE_SOLOUT_EVENT;
+    p.event_ = e;
+    return p;
+  }
+  // IVP::crossing (src/ivp.rs:682-692): replaces the output (and any t_eval).
+  IVP crossing(int32_t component, double threshold,
+               CrossingDirection d = CrossingDirection::Both) const {
+    IVP p = *this;
+    p.solout_ = DE_SOLOUT_CROSSING;
+    p.event_ = Event::component_at(component, {threshold}, EventConfig{}.direction(d));
+    p.t_eval_.clear();
+    return p;
+  }
+  // IVP::hyperplane_crossing (src/ivp.rs:695-705) with the identity extractor.
+  IVP hyperplane_crossing(std::vector<double> point, std::vector<double> normal,
+                          CrossingDirection d = CrossingDirection::Both) const {
+    IVP p = *this;
+    p.solout_ = DE_SOLOUT_HYPERPLANE;
+    p.event_ = Event::plane(std::move(point), std::move(normal), EventConfig{}.direction(d));
+    p.t_eval_.clear();
+    return p;
+  }
+  // Event points kept per trajectory.
+  IVP max_events(int32_t k) const { IVP p = *this; p.max_events_ = k; return p; }
 
   EnsembleSolution solve() const {
     const int64_t n = (int64_t)y0_.size();
@@ -260,7 +362,36 @@ class IVP {
       s.dense.resize(te.size() * y0.size());
       s.n_dense.resize(n);
     }
-    de_ode_result r;
+    std::vector<double> ev_value;
+    if (solout_ != DE_SOLOUT_DEFAULT) {
+      const Event& e = event_;
+      if (e.value.size() != 1 && (int64_t)e.value.size() != n)
+        throw std::runtime_error("event value vector length must be 1 or n");
+      b.event_solout = solout_;
+      b.event_fn = e.fn;
+      b.event_component = e.component;
+      b.event_direction = e.config.dir == CrossingDirection::Positive   ? DE_CROSS_POSITIVE
+                          : e.config.dir == CrossingDirection::Negative ? DE_CROSS_NEGATIVE
+                                                                        : DE_CROSS_BOTH;
+      b.event_terminate = e.config.terminate;
+      b.max_events = max_events_;
+      b.event_value = e.value[0];
+      if (e.value.size() != 1) {
+        ev_value = e.value;
+        b.event_value_traj = ev_value.data();
+      }
+      b.event_point = e.point.empty() ? nullptr : e.point.data();
+      b.event_normal = e.normal.empty() ? nullptr : e.normal.data();
+      s.solout = solout_;
+      s.max_events = max_events_;
+      s.event_t.resize((size_t)max_events_ * n);
+      s.event_y.resize((size_t)max_events_ * y0.size());
+      s.n_events.resize(n);
+    }
+    de_ode_result r{};
+    r.event_t = s.event_t.empty() ? nullptr : s.event_t.data();
+    r.event_y = s.event_y.empty() ? nullptr : s.event_y.data();
+    r.n_events = s.n_events.empty() ? nullptr : s.n_events.data();
     r.y_final = s.y.data();
     r.t_final = s.t.data();
     r.accepted = s.accepted.data();
@@ -284,6 +415,9 @@ class IVP {
   ExplicitRungeKutta m_;
   std::vector<double> t_eval_;
   int32_t schedule_ = DE_SCHED_AUTO;
+  int32_t solout_ = DE_SOLOUT_DEFAULT;
+  Event event_;
+  int32_t max_events_ = 16;
 };
 
 }  // namespace differential_equations

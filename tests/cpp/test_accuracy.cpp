@@ -102,6 +102,53 @@ int main() {
   }
   std::printf("errors passed\n");
 
+  // tests/ode/errors.rs SimpleODE with its event g = t - 10,
+  // EventConfig::default().terminal(): the solve stops Interrupted after the
+  // step that crosses 10, with one event point at 10.
+  for (auto m : {ExplicitRungeKutta::dopri5(), ExplicitRungeKutta::dop853(),
+                 ExplicitRungeKutta::rk4(0.1), ExplicitRungeKutta::euler(0.1)}) {
+    std::vector<std::array<double, 1>> y0s(4, {1.0});
+    auto sol = IVP<ExponentialGrowth>::ode(ExponentialGrowth{{1.0}}, 0.0, 20.0, y0s)
+                   .method(m)
+                   .event(Event::time({10.0}, EventConfig{}.terminal()))
+                   .solve();
+    for (int64_t i = 0; i < sol.n; ++i) {
+      Result r = sol.at(i);
+      CHECK(r.ok && r.value.status == Status::Interrupted, "time event: not interrupted");
+      CHECK(r.value.event_t.size() == 1 && std::fabs(r.value.event_t[0] - 10.0) < 1e-9,
+            "time event: point");
+      CHECK(r.value.t.size() == 1 && r.value.t[0] >= 10.0, "time event: final time");
+    }
+  }
+  // examples/ode/03_logistic_growth: k = 1, m = 10, y0 = 1, g = y - 0.9 m,
+  // terminal, DOP853 rtol = atol = 1e-12; y(t) = 10 / (1 + 9 e^-t) = 9 at ln 81.
+  {
+    std::vector<std::array<double, 1>> y0s(4, {1.0});
+    auto sol = IVP<LogisticEquation>::ode(LogisticEquation{{1.0}, {10.0}}, 0.0, 10.0, y0s)
+                   .method(ExplicitRungeKutta::dop853().rtol(1e-12).atol(1e-12))
+                   .event(Event::component_at(0, {9.0}, EventConfig{}.terminal()))
+                   .solve();
+    Result r = sol.at(0);
+    CHECK(r.ok && r.value.status == Status::Interrupted && r.value.event_t.size() == 1,
+          "logistic event");
+    if (r.ok && r.value.event_t.size() == 1)
+      CHECK(std::fabs(r.value.event_t[0] - std::log(81.0)) < 1e-5, "logistic event time %.12g",
+            r.value.event_t[0]);
+  }
+  // IVP::crossing(0, 0.0) on y'' = -y, y(0) = (1, 0): the zeros of cos t.
+  {
+    std::vector<std::array<double, 2>> y0s(4, {1.0, 0.0});
+    auto sol = IVP<HarmonicOscillator>::ode(HarmonicOscillator{{1.0}}, 0.0, 20.0, y0s)
+                   .method(ExplicitRungeKutta::dopri5().rtol(1e-10).atol(1e-12))
+                   .crossing(0, 0.0)
+                   .solve();
+    Result r = sol.at(1);
+    CHECK(r.ok && r.value.t.size() == 6, "crossing: expected 6 zeros");
+    for (size_t k = 0; r.ok && k < r.value.t.size(); ++k)
+      CHECK(std::fabs(r.value.t[k] - (M_PI / 2 + M_PI * k)) < 1e-8, "crossing zero %zu", k);
+  }
+  std::printf("events passed\n");
+
   // Ensemble: per-trajectory parameters (the builder's vectors) on Lorenz.
   {
     const int n = 256;
