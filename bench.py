@@ -11,9 +11,10 @@ trajectory launch is reported beside it, with p50/p99 per-solve latency.
 
 Other workloads (--workload): events (the Lorenz ensemble with a Poincare
 section event z = rho - 1 per trajectory, and the crossing / hyperplane
-solouts), spring (65 536 uneven spring chains, dense output), sde (OU-8 + GBM
-paths, Euler-Maruyama and Milstein), serve (Poisson request stream through the
-serving loop).
+solouts), erk (the Lorenz ensemble with each tableau of the embedded family,
+rkf45 .. rkv989e), spring (65 536 uneven spring chains, dense output), sde
+(OU-8 + GBM paths, Euler-Maruyama and Milstein), serve (Poisson request stream
+through the serving loop).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -39,13 +40,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lorenz",
-                    choices=["lorenz", "events", "spring", "sde", "serve"])
+                    choices=["lorenz", "events", "erk", "spring", "sde", "serve"])
     ap.add_argument("--n", type=int, default=0, help="override ensemble size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--only", choices=["persistent", "fifo", "plain", "fast", "event",
-                                       "crossing"], default=None,
-                    help="run only one Lorenz variant (profiling)")
+    ap.add_argument("--only", default=None,
+                    help="run only one variant (profiling): persistent, fifo, plain, fast "
+                         "(lorenz); event, crossing (events); a tableau name (erk)")
     ap.add_argument("--cpu-sample-stride", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window")
@@ -269,6 +270,174 @@ def cpu_baseline_lorenz(args, p, y0, n):
                       "%d accepted steps, median %.2f s wall of 3 runs on %d threads); C++ "
                       "restatement of the reference solver with the C library pow"
                       % (k, len(idx), int(sol.accepted.sum()), dt, threads)}
+
+
+# ---------------------------------------------------- embedded ERK family ---
+ERK_FAMILY = ["rkf45", "cash_karp", "rkv655e", "rkv656e", "rkv766e", "rkv767e", "rkv877e",
+              "rkv878e", "rkv988e", "rkv989e"]
+
+
+def erk_tableau_counts():
+    """Stage counts and nonzero coefficients of each tableau, read from the
+    generated header the kernels are built from: {name: (S, I, fsal, nnz of
+    a[i][j] for i < S, nnz b, nnz bh)}."""
+    import re
+    text = open(os.path.join(REPO, "differential-equations_amd", "csrc",
+                             "de_erk_tables.h")).read()
+    out = {}
+    for name in ERK_FAMILY:
+        cls = "".join(w.capitalize() for w in name.split("_"))
+        body = text[text.index("struct %s {" % cls):]
+        body = body[:body.index("\n};\n")]
+        S, I = (int(x) for x in re.search(r"S = (\d+), I = (\d+)", body).groups())
+        fsal = "kFsal = true" in body
+
+        def nums(tag):
+            seg = body[body.index(tag):]
+            seg = seg[seg.index("{") + 1:]
+            depth, end = 1, 0
+            for end, ch in enumerate(seg):
+                depth += ch == "{"
+                depth -= ch == "}"
+                if depth == 0:
+                    break
+            return [float.fromhex(v) if "0x" in v else float(v)
+                    for v in re.findall(r"-?0x[0-9a-fA-F.p+-]+|-?\d+\.\d+", seg[:end])]
+        a = nums("A[%d][%d]" % (I, I))
+        nnz_a = sum(1 for i in range(S) for j in range(i) if a[i * I + j] != 0.0)
+        nnz_b = sum(1 for v in nums("B[%d]" % S) if v != 0.0)
+        nnz_bh = sum(1 for v in nums("BH[%d]" % S) if v != 0.0)
+        out[name] = (S, I, fsal, nnz_a, nnz_b, nnz_bh)
+    return out
+
+
+def erk_flops(counts, n=3, F=8):
+    """Flops the lane executes per attempt and per accepted step (Lorenz: F =
+    8 per right-hand side, n = 3): S - 1 right-hand sides and stage times; per
+    nonzero a[i][j], b[j], bh[j] one scalar product and n multiply-adds; the
+    error and its norm (9n); per accepted step f(t_new, y_new) unless FSAL."""
+    S, I, fsal, nnz_a, nnz_b, nnz_bh = counts
+    attempt = (S - 1) * (F + 2) + (nnz_a + nnz_b + nnz_bh) * (1 + 2 * n) + 9 * n
+    return attempt, (0 if fsal else F + 1)
+
+
+def bench_erk(args, dv):
+    """The 1M Lorenz ensemble of the headline workload (t in [0, 50], rtol
+    1e-8, atol 1e-10, persistent schedule, final state) solved with each
+    tableau of the embedded family.  `value` is the family's aggregate: the
+    accepted steps of the ten solves over their summed time."""
+    import ctypes as C
+    import numpy as np
+    import deamd
+    from deamd import _abi
+    torch = dv.torch
+    n = args.n or (1 << 20)
+    p, y0 = deamd.gen_lorenz(n, seed=SEED)
+    d_p = dv.put(p.reshape(-1))
+    d_y0 = dv.put(y0.reshape(-1))
+    out = dict(y=dv.empty(3 * n, torch.float64), t=dv.empty(n, torch.float64),
+               acc=dv.empty(n, torch.int64), rej=dv.empty(n, torch.int64),
+               ev=dv.empty(n, torch.int64), st=dv.empty(n, torch.int32))
+    lib = deamd.lib()
+    stream = C.c_void_p(dv.stream.cuda_stream)
+    counts = erk_tableau_counts()
+    names = [args.only] if args.only else ERK_FAMILY
+
+    def run(name):
+        b = _abi.ode_defaults()
+        b.family = _abi.DE_FAMILY_LORENZ
+        b.method = getattr(_abi, "DE_METHOD_" + name.upper())
+        b.n, b.dim = n, 3
+        b.params = C.c_void_p(d_p.data_ptr())
+        b.y0 = C.c_void_p(d_y0.data_ptr())
+        b.t0_scalar, b.tf_scalar = 0.0, 50.0
+        b.rtol, b.atol, b.max_steps = 1e-8, 1e-10, 1000000
+        b.schedule = _abi.DE_SCHED_PERSISTENT
+        r = _abi.OdeResult()
+        r.y_final = C.c_void_p(out["y"].data_ptr())
+        r.t_final = C.c_void_p(out["t"].data_ptr())
+        r.accepted = C.c_void_p(out["acc"].data_ptr())
+        r.rejected = C.c_void_p(out["rej"].data_ptr())
+        r.evals = C.c_void_p(out["ev"].data_ptr())
+        r.status = C.c_void_p(out["st"].data_ptr())
+        for _ in range(args.warmup):
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+        dv.sync()
+        evs = [(dv.event(), dv.event()) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(dv.stream)
+            _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
+            e1.record(dv.stream)
+        dv.sync()
+        wall = time.perf_counter() - t0
+        kernel_ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / len(evs)
+        A = int(out["acc"].sum().item())
+        R = int(out["rej"].sum().item())
+        ok = bool((out["st"] == 0).all().item())
+        ms = 1000.0 * wall / args.steps
+        per_att, per_acc = erk_flops(counts[name])
+        tflops = ((A + R) * per_att + A * per_acc) / (kernel_ms / 1000.0) / 1e12
+        print("erk %-9s %8.2f ms  %.3e acc*traj/s  %.2f TFLOP/s" % (name, ms, A / (ms / 1e3),
+                                                                  tflops),
+              file=sys.stderr, flush=True)
+        return dict(ms_per_step=ms, kernel_ms=kernel_ms, accepted=A, rejected=R,
+                    value=A / (ms / 1000.0), tflops=tflops, all_complete=ok,
+                    flops_per_attempt=per_att, flops_per_accepted_extra=per_acc)
+
+    res = {name: run(name) for name in names}
+    ms = sum(r["ms_per_step"] for r in res.values())
+    A = sum(r["accepted"] for r in res.values())
+    kern = sum(r["kernel_ms"] for r in res.values())
+    fl = sum(r["tflops"] * r["kernel_ms"] for r in res.values()) / kern
+    line = {
+        "metric": "accepted steps*traj/s FP64, embedded ERK family (rkf45 .. rkv989e), 1 GPU",
+        "value": A / (ms / 1000.0),
+        "unit": "accepted_steps*traj/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Philox4x32-10 inputs, seed 0x5EED2026)",
+        "config": {"workload": "lorenz_erk_family_1M", "n_traj": n, "t_span": [0.0, 50.0],
+                   "rtol": 1e-8, "atol": 1e-10, "schedule": "persistent",
+                   "tableaux": names, "parallelism": "none (1 GPU)",
+                   "step": "one solve of the ensemble with each tableau in turn"},
+        "methods": res,
+        "roofline": {"bound": "fp64_vector", "achieved": fl, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": fl / FP64_PEAK_TFLOPS,
+                     "traffic": load_traffic("lorenz_erk_family_1M"),
+                     "flop_model": "per attempt (S-1)(F+2) + nnz(a, b, bh)(1+2n) + 9n, per "
+                                   "accepted step F+1 unless FSAL; F = 8, n = 3; time-weighted "
+                                   "over the tableaux"},
+    }
+    if not args.no_cpu_baseline and not args.only:
+        line["cpu_baseline"] = cpu_baseline_erk(args, p, y0, n)
+    return line
+
+
+def cpu_baseline_erk(args, p, y0, n):
+    """The oracle (C library pow) on every k-th trajectory, each tableau."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import deamd
+    import oracle
+    k = max(1, args.cpu_sample_stride // 4)   # about 20 s of CPU work over the ten tableaux
+    idx = np.arange(0, n, k)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 16))
+    A, dt = 0, 0.0
+    for name in ERK_FAMILY:
+        ens = deamd.IVP.ode(deamd.Lorenz(*p[:, idx]), 0.0, 50.0, y0[:, idx]).method(
+            getattr(deamd.ExplicitRungeKutta, name)().rtol(1e-8).atol(1e-10).max_steps(1000000))
+        t0 = time.perf_counter()
+        sol = oracle.solve_ode(ens, pow_mode=oracle.POW_LIBM, threads=threads)
+        dt += time.perf_counter() - t0
+        A += int(sol.accepted.sum())
+    return {"value": A / dt, "unit": "accepted_steps*traj/s", "cores": threads, "kind": "port",
+            "sample": "every %d-th trajectory of the 1M Lorenz ensemble (%d trajectories) with "
+                      "each of the ten tableaux, %d accepted steps in %.2f s on %d threads; C++ "
+                      "restatement of the reference solver with the C library pow"
+                      % (k, len(idx), A, dt, threads)}
 
 
 # --------------------------------------------------------------- events ---
@@ -686,6 +855,8 @@ def main():
             line["cpu_baseline"] = cpu_baseline_lorenz(args, p, y0, n)
     elif args.workload == "events":
         line = bench_events(args, dv)
+    elif args.workload == "erk":
+        line = bench_erk(args, dv)
     elif args.workload == "spring":
         line = bench_spring(args, dv)
     elif args.workload == "sde":

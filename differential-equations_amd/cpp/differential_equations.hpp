@@ -120,6 +120,17 @@ class ExplicitRungeKutta {
   static ExplicitRungeKutta dop853() { return ExplicitRungeKutta(DE_METHOD_DOP853, 0.0); }
   static ExplicitRungeKutta rk4(double h0) { return ExplicitRungeKutta(DE_METHOD_RK4, h0); }
   static ExplicitRungeKutta euler(double h0) { return ExplicitRungeKutta(DE_METHOD_EULER, h0); }
+  // The embedded family (src/methods/erk/adaptive/mod.rs:43-122): h0 computed.
+  static ExplicitRungeKutta rkf45() { return ExplicitRungeKutta(DE_METHOD_RKF45, 0.0); }
+  static ExplicitRungeKutta cash_karp() { return ExplicitRungeKutta(DE_METHOD_CASH_KARP, 0.0); }
+  static ExplicitRungeKutta rkv655e() { return ExplicitRungeKutta(DE_METHOD_RKV655E, 0.0); }
+  static ExplicitRungeKutta rkv656e() { return ExplicitRungeKutta(DE_METHOD_RKV656E, 0.0); }
+  static ExplicitRungeKutta rkv766e() { return ExplicitRungeKutta(DE_METHOD_RKV766E, 0.0); }
+  static ExplicitRungeKutta rkv767e() { return ExplicitRungeKutta(DE_METHOD_RKV767E, 0.0); }
+  static ExplicitRungeKutta rkv877e() { return ExplicitRungeKutta(DE_METHOD_RKV877E, 0.0); }
+  static ExplicitRungeKutta rkv878e() { return ExplicitRungeKutta(DE_METHOD_RKV878E, 0.0); }
+  static ExplicitRungeKutta rkv988e() { return ExplicitRungeKutta(DE_METHOD_RKV988E, 0.0); }
+  static ExplicitRungeKutta rkv989e() { return ExplicitRungeKutta(DE_METHOD_RKV989E, 0.0); }
   ExplicitRungeKutta rtol(double v) const { auto m = *this; m.b_.rtol = v; m.rtol_v_.clear(); return m; }
   ExplicitRungeKutta atol(double v) const { auto m = *this; m.b_.atol = v; m.atol_v_.clear(); return m; }
   // Tolerance::Vector: one value per state component (src/tolerance.rs:11-68).
@@ -129,6 +140,8 @@ class ExplicitRungeKutta {
   ExplicitRungeKutta h_min(double v) const { auto m = *this; m.b_.h_min = v; return m; }
   ExplicitRungeKutta h_max(double v) const { auto m = *this; m.b_.h_max = v; return m; }
   ExplicitRungeKutta max_steps(int64_t v) const { auto m = *this; m.b_.max_steps = v; return m; }
+  // Consecutive rejections that end an embedded-family solve (erk/mod.rs:201).
+  ExplicitRungeKutta max_rejects(int64_t v) const { auto m = *this; m.b_.max_rejects = v; return m; }
   ExplicitRungeKutta safety_factor(double v) const { auto m = *this; m.b_.safety_factor = v; return m; }
   ExplicitRungeKutta min_scale(double v) const { auto m = *this; m.b_.min_scale = v; return m; }
   ExplicitRungeKutta max_scale(double v) const { auto m = *this; m.b_.max_scale = v; return m; }

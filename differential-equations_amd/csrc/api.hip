@@ -110,7 +110,7 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->n < 0) return fail(DE_ERR_INVALID, "n < 0");
   if (b->family < 0 || b->family > DE_FAMILY_ROBERTSON)
     return fail(DE_ERR_INVALID, "unknown family %d", b->family);
-  if (b->method < DE_METHOD_DOPRI5 || b->method > DE_METHOD_DOP853)
+  if (b->method < DE_METHOD_DOPRI5 || b->method > DE_METHOD_RKV989E)
     return fail(DE_ERR_INVALID, "unknown method %d", b->method);
   if (b->family != DE_FAMILY_SPRING_CHAIN && b->dim != kFamilyDim[b->family])
     return fail(DE_ERR_INVALID, "family %d has dimension %d, got dim %d", b->family,
@@ -132,11 +132,14 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->schedule < DE_SCHED_AUTO || b->schedule > DE_SCHED_PERSISTENT_FIFO)
     return fail(DE_ERR_INVALID, "unknown schedule %d", b->schedule);
   if (b->max_steps < 0) return fail(DE_ERR_INVALID, "max_steps < 0");
+  if (b->max_rejects < 0) return fail(DE_ERR_INVALID, "max_rejects < 0");
   if (b->event_solout < DE_SOLOUT_DEFAULT || b->event_solout > DE_SOLOUT_HYPERPLANE)
     return fail(DE_ERR_INVALID, "unknown event solout %d", b->event_solout);
   if (b->event_solout != DE_SOLOUT_DEFAULT) {
     if (b->family == DE_FAMILY_SPRING_CHAIN)
       return fail(DE_ERR_UNSUPPORTED, "event solouts are built for fixed-dimension families");
+    if (b->method >= DE_METHOD_RKF45)
+      return fail(DE_ERR_UNSUPPORTED, "event solouts are built for dopri5, dop853, rk4 and euler");
     if (b->event_solout != DE_SOLOUT_EVENT && b->n_eval > 0)
       return fail(DE_ERR_INVALID, "a crossing solout replaces the t_eval output (n_eval > 0)");
     if (b->event_direction < DE_CROSS_BOTH || b->event_direction > DE_CROSS_NEGATIVE)
@@ -182,6 +185,7 @@ de::OdeArgs make_args(const de_ode_batch* b, const de_ode_result* o) {
   a.min_scale = b->min_scale;
   a.max_scale = b->max_scale;
   a.max_steps = b->max_steps;
+  a.max_rejects = b->max_rejects;
   a.t_eval = b->t_eval;
   a.n_eval = b->n_eval;
   a.y_final = o->y_final;
@@ -209,6 +213,19 @@ de::OdeArgs make_args(const de_ode_batch* b, const de_ode_result* o) {
   a.ev.y_out = o->event_y;
   a.ev.n_out = o->n_events;
   return a;
+}
+
+// The embedded family (parity kernels, with or without DE_FLAG_FAST_MATH):
+// two tableaux per translation unit.
+int launch_ode_erk(int family, int method, const de::OdeArgs& a, int sched, hipStream_t s) {
+  switch ((method - DE_METHOD_RKF45) / 2) {
+    case 0: return de::parity::launch_ode_erk0(family, method, a, sched, s);
+    case 1: return de::parity::launch_ode_erk1(family, method, a, sched, s);
+    case 2: return de::parity::launch_ode_erk2(family, method, a, sched, s);
+    case 3: return de::parity::launch_ode_erk3(family, method, a, sched, s);
+    case 4: return de::parity::launch_ode_erk4(family, method, a, sched, s);
+    default: return 1;
+  }
 }
 
 // Launch on device-resident arguments.
@@ -239,7 +256,9 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
   // n_eval rows, with NaN for the points it does not produce.
   // Event solouts run the parity kernels, with or without DE_FLAG_FAST_MATH.
   int r;
-  if (b->event_solout == DE_SOLOUT_EVENT)
+  if (b->method >= DE_METHOD_RKF45 && b->event_solout == DE_SOLOUT_DEFAULT)
+    r = launch_ode_erk(b->family, b->method, a, sched, s);
+  else if (b->event_solout == DE_SOLOUT_EVENT)
     r = de::parity::launch_ode_event(b->family, b->method, a, sched, s);
   else if (b->event_solout != DE_SOLOUT_DEFAULT)
     r = de::parity::launch_ode_crossing(b->family, b->method, a, sched, s);
@@ -270,6 +289,7 @@ void de_ode_batch_init(de_ode_batch* b) {
   b->h_min = 0.0;
   b->h_max = INFINITY;
   b->max_steps = 10000;
+  b->max_rejects = 100;
   b->safety_factor = 0.9;
   b->min_scale = 0.2;
   b->max_scale = 10.0;

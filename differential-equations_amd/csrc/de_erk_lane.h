@@ -1,0 +1,335 @@
+// Lane-per-trajectory step of the embedded explicit Runge-Kutta family:
+// ExplicitRungeKutta<Ordinary, Adaptive, ..> with the rkf45, cash_karp and
+// Verner tableaux (src/methods/erk/adaptive/ordinary.rs:16-292), one
+// trajectory per lane, the tableau a compile-time constant (de_erk_tables.h).
+//
+// Zero coefficients.  The error estimate is an infinity norm folded with
+// max(), which ignores a NaN, so a stage that overflows is not necessarily
+// rejected and every NaN the reference produces must be reproduced.  A term
+// (0 h) k_j adds +-0 when k_j and h are finite (which changes nothing but,
+// possibly, the sign of an exact zero) and NaN otherwise.  The lanes skip the
+// zero terms and keep, per component, a bit mask of the stages whose value is
+// not finite: a combination whose skipped terms include such a stage (or that
+// has zero terms while h is not finite) is NaN, as in the reference.  Results
+// therefore equal the reference's as IEEE values; only the sign of a zero
+// result can differ.
+//
+// Registers: k[S][D] per attempt; with dense output and a tableau with
+// interpolation rows, k[I][D] (the extra stages are evaluated only on
+// accepted steps that reach a t_eval point; the evaluation count still adds
+// them on every accepted step, as the reference evaluates them always).
+#pragma once
+
+#include "de_erk_tables.h"
+#include "de_lane.h"
+
+namespace de {
+
+// Bit j set where a row's coefficient j is zero (j < n).
+DE_HD constexpr uint32_t erk_zero_mask(const double* row, int n) {
+  uint32_t m = 0;
+  for (int j = 0; j < n; ++j)
+    if (row[j] == 0.0) m |= 1u << j;
+  return m;
+}
+
+// Bit r set where interpolation row r is zero in every power.
+template <class Tab>
+DE_HD constexpr uint32_t erk_zero_bi_rows() {
+  uint32_t m = 0;
+  for (int r = 0; r < Tab::I; ++r) {
+    bool z = true;
+    for (int j = 0; j < Tab::O; ++j) z = z && Tab::BI[r][j] == 0.0;
+    if (z) m |= 1u << r;
+  }
+  return m;
+}
+
+template <class F, class Tab, bool DENSE, bool VTOL>
+struct ErkLane {
+  static constexpr int D = F::kDim;
+  static constexpr int S = Tab::S, I = Tab::I, O = Tab::O;
+  static constexpr bool kBi = Tab::kBi, kFsal = Tab::kFsal;
+  static constexpr int kEv = 0;
+  static constexpr int kEntryWidth = 0;
+  static constexpr int KN = (DENSE && kBi) ? I : S;  // stages held in registers
+  // Occupancy hint of the persistent kernel: three waves per SIMD (168 VGPRs)
+  // while the stage array is small, two (256 VGPRs) beyond 20 doubles of
+  // stages, where three waves spill.
+  static constexpr int kWavesPerEu = KN * D <= 20 ? 3 : 2;
+  // Evaluations per accepted step beyond the S - 1 of every attempt
+  // (ordinary.rs:151-181): f(t_new, y_new) unless FSAL, the dense stages.
+  static constexpr int kAccEvals = (kFsal ? 0 : 1) + (kBi ? I - S : 0);
+
+  F f;
+  double y[D], k0[D];
+  double t, h, h_prev, tf, dir;
+  double tail;
+  LaneTol<D, VTOL> tol;
+  int32_t acc, rej, base_ev, stiff, next_eval, pushed;
+  bool rejected_last;
+
+  DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, ErkLane L) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
+    a.t_final[i] = L.t;
+    a.accepted[i] = L.acc;
+    a.rejected[i] = L.rej;
+    a.evals[i] = st == kBadInput ? 0
+                                 : (int64_t)L.base_ev + (int64_t)(S - 1) * (L.acc + L.rej) +
+                                       (int64_t)kAccEvals * L.acc;
+    a.status[i] = st;
+    if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
+    if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
+  }
+  DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
+
+  DE_COLD static ErkLane start(const OdeArgs& a, int64_t i, bool* ok) {
+    ErkLane L;
+    *ok = L.init(a, i);
+    return L;
+  }
+  DE_HD bool begin(const OdeArgs& a, int64_t i) {
+    bool ok;
+    *this = start(a, i, &ok);
+    return ok;
+  }
+
+  DE_HD static double predicted_cost(const OdeArgs& a, int64_t i) {
+    F f;
+    f.load(a.params, a.n, i);
+    const double t0 = a.t0 ? a.t0[i] : a.t0s;
+    const double tf = a.tf ? a.tf[i] : a.tfs;
+    double h0 = a.h0;
+    if (h0 == 0.0 && tf != t0) {
+      double y[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
+      LaneTol<D, VTOL> tl;
+      tl.load(a, i);
+      h0 = lane_h_init<F, LaneTol<D, VTOL>, O>(f, t0, tf, y, tl, a.h_min, a.h_max);
+    }
+    return fabs(tf - t0) / fabs(h0);
+  }
+
+  // ordinary.rs:16-61: the computed h0 costs InitialStepSize's two
+  // evaluations plus two more, then f(t0, y0).
+  DE_HD bool init(const OdeArgs& a, int64_t i) {
+    f.load(a.params, a.n, i);
+    double t0 = a.t0 ? a.t0[i] : a.t0s;
+    tf = a.tf ? a.tf[i] : a.tfs;
+    tol.load(a, i);
+#pragma unroll
+    for (int c = 0; c < D; ++c) y[c] = a.y0[c * a.n + i];
+    t = t0;
+    acc = rej = 0;
+    pushed = 0;
+    next_eval = 0;
+    double s = signum(tf - t0);
+    dir = s;
+    bool ok = (s == 1.0 || s == -1.0);
+    double h0 = a.h0;
+    base_ev = 1;
+    if (ok && h0 == 0.0) {
+      h0 = lane_h_init<F, LaneTol<D, VTOL>, O>(f, t0, tf, y, tol, a.h_min, a.h_max);
+      base_ev = 5;
+    }
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      finish(a, i, kBadInput);
+      return false;
+    }
+    h = h0;
+    f.diff(t, y, k0);
+    h_prev = 0.0;
+    stiff = 0;
+    rejected_last = false;
+    tail = as_f64(kNaNBits);
+    if (DENSE && a.n_eval > 0)
+      teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
+                    [](double, double*) { return false; }, tail);
+    return true;
+  }
+
+  // y + sum_{j<R} (a[R][j] h) k[j] over the nonzero a[R][j], in j order
+  // (ordinary.rs:98-106); NaN where a skipped term would be (bad: the
+  // per-component masks of non-finite stages, all ones when h is not finite).
+  template <int R>
+  DE_HD static void stage_input(const double* yy, const double (*k)[D], double hh,
+                                const uint32_t* bad, double* ys) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) ys[c] = yy[c];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (Tab::A[R][j] != 0.0) {
+        const double ah = Tab::A[R][j] * hh;
+#pragma unroll
+        for (int c = 0; c < D; ++c) ys[c] = ys[c] + ah * k[j][c];
+      }
+    }
+    constexpr uint32_t z = erk_zero_mask(Tab::A[R], R);
+    if constexpr (z != 0) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) ys[c] = (bad[c] & z) ? as_f64(kNaNBits) : ys[c];
+    }
+  }
+
+  template <int R, int END>
+  DE_HD void stages(double (*k)[D], const double* yy, double tt, double hh, uint32_t* bad) const {
+    if constexpr (R < END) {
+      double ys[D];
+      stage_input<R>(yy, k, hh, bad, ys);
+      f.diff(tt + Tab::C[R] * hh, ys, k[R]);
+#pragma unroll
+      for (int c = 0; c < D; ++c) bad[c] |= isfinite(k[R][c]) ? 0u : (1u << R);
+      stages<R + 1, END>(k, yy, tt, hh, bad);
+    }
+  }
+
+  // y + sum_j (w[j] h) k[j] over the nonzero weights (y_high, y_low).
+  template <const double* W>
+  DE_HD static double weighted(double yc, const double (*k)[D], double hh, uint32_t bad, int c) {
+    double v = yc;
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      if (W[j] != 0.0) v = v + (W[j] * hh) * k[j][c];
+    constexpr uint32_t z = erk_zero_mask(W, S);
+    if constexpr (z != 0) v = (bad & z) ? as_f64(kNaNBits) : v;
+    return v;
+  }
+
+  // One pass of the driver loop (solve_ivp.rs:190-263 around ordinary.rs:63-210).
+  DE_HD int32_t step(const OdeArgs& a, int64_t i) {
+    {
+      const bool clip = (t + h - tf) * dir > 0.0;
+      const double hn = tf - t;
+      const bool done = clip && fabs(hn) < kTenEps;
+      h = clip ? hn : h;
+      const bool small = fabs(h) < fabs(h_prev) * 1e-14;
+      const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
+      if (done | small | maxed) return done ? kComplete : (small ? kStepSize : kMaxSteps);
+    }
+    double k[KN][D];
+    uint32_t bad[D];
+    const uint32_t hbad = isfinite(h) ? 0u : ~0u;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      k[0][c] = k0[c];
+      bad[c] = hbad | (isfinite(k0[c]) ? 0u : 1u);
+    }
+    stages<1, S>(k, y, t, h, bad);
+
+    double yh[D], m = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double hi = weighted<Tab::B>(y[c], k, h, bad[c], c);
+      const double lo = weighted<Tab::BH>(y[c], k, h, bad[c], c);
+      yh[c] = hi;
+      const double e = hi + (-1.0) * lo;
+      const double sk = tol.at(c) + tol.rt(c) * fmax(fabs(y[c]), fabs(hi));
+      m = fmax(m, fabs(e / sk));
+    }
+    double scale = a.safety * DE_CTRL_POW(m, -(1.0 / (double)O));
+    scale = fmin(fmax(scale, a.min_scale), a.max_scale);
+
+    const bool accept = m <= 1.0;
+    const int32_t stiff_next = accept ? (rejected_last ? 0 : stiff) : stiff + 1;
+    const bool stop = !accept && (int64_t)stiff_next >= a.max_rejects;
+    const double t_new = t + h;
+    double dn[D];
+    if constexpr (kFsal) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) dn[c] = k[S - 1][c];
+    } else {
+      f.diff(t_new, yh, dn);
+    }
+    int32_t sol = -1;
+    if constexpr (DENSE) {
+      // Interpolation is needed only when the next t_eval point is not past
+      // the new time (a NaN point is handed to teval_emit as well).
+      bool tev = false;
+      if (accept && next_eval < a.n_eval) {
+        const double te = a.t_eval[next_eval];
+        tev = !(dir > 0.0 ? te > t_new : te < t_new);
+      }
+      if (tev) {
+        if constexpr (kBi) stages<S, I>(k, y, t, h, bad);
+        const double tp = t, hp = h;
+        auto interp = [&](double te, double* out) -> bool {
+          if (te < tp || te > t_new) return false;
+          if constexpr (kBi) {
+            // ordinary.rs:252-277: Horner over bi[r][0..O) times s, then
+            // y_prev + sum_r (cont_r h_prev) k[r] in r order; rows that are zero
+            // in every power add +-0, or NaN where k[r] or the weight is not
+            // finite.
+            const double s = (te - tp) / hp;
+            constexpr uint32_t zr = erk_zero_bi_rows<Tab>();
+            const uint32_t wbad = isfinite(s) && isfinite(hp) ? 0u : ~0u;
+            double v[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) v[c] = y[c];
+#pragma unroll
+            for (int r = 0; r < I; ++r) {
+              if (!((zr >> r) & 1u)) {
+                double q = Tab::BI[r][O - 1];
+#pragma unroll
+                for (int j = O - 2; j >= 0; --j) q = q * s + Tab::BI[r][j];
+                q = q * s;
+                const double w = q * hp;
+#pragma unroll
+                for (int c = 0; c < D; ++c) v[c] = v[c] + w * k[r][c];
+              }
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c)
+              out[c] = (zr != 0 && ((bad[c] | wbad) & zr)) ? as_f64(kNaNBits) : v[c];
+          } else {
+            // cubic_hermite_interpolate(t_prev, t, y_prev, y, dydt_prev, dydt)
+            // (src/interpolate.rs:40-60), a linear combination from zero.
+            const double hh = t_new - tp;
+            const double s = (te - tp) / hh;
+            const double s2 = s * s;
+            const double s3 = s2 * s;
+            const double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+            const double h10 = s3 - 2.0 * s2 + s;
+            const double h01 = -2.0 * s3 + 3.0 * s2;
+            const double h11 = s3 - s2;
+            const double w10 = h10 * hh, w11 = h11 * hh;
+#pragma unroll
+            for (int c = 0; c < D; ++c)
+              out[c] = (((0.0 + h00 * y[c]) + w10 * k[0][c]) + h01 * yh[c]) + w11 * dn[c];
+          }
+          return true;
+        };
+        if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, yh, interp, tail))
+          sol = kOutOfBounds;
+      }
+    }
+    h_prev = accept ? h : h_prev;
+    t = accept ? t_new : t;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      y[c] = accept ? yh[c] : y[c];
+      k0[c] = accept ? dn[c] : k0[c];
+    }
+    scale = accept && rejected_last ? fmin(scale, 1.0) : scale;
+    rejected_last = !accept;
+    stiff = stiff_next;
+    acc += accept ? 1 : 0;
+    rej += (accept || stop) ? 0 : 1;
+    const int32_t result =
+        stop ? kStiffness
+             : (sol >= 0 ? sol : (accept && fabs(tf - t) <= kTenEps ? kComplete : -1));
+    h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    return result;
+  }
+
+  DE_HD bool advance(const OdeArgs& a, int64_t i) {
+    int32_t st = step(a, i);
+    if (st < 0) return false;
+    finish(a, i, st);
+    return true;
+  }
+};
+
+}  // namespace de

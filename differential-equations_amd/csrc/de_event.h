@@ -52,8 +52,9 @@ struct EventArgs {
   int32_t* n_out;        // [n] points pushed, or null
   // Deferred roots (null brk: inline).  Entry k of trajectory i is
   // brk[(i * bmax + k) * width ...]: kEntryMeta fields, then the lane's
-  // interpolant; the root pass overwrites the interpolant with its result.
+  // interpolant data; its result record is res[(i * bmax + k) * kResWidth ...].
   double* brk;
+  double* res;
   int32_t* bcount;  // [n] brackets stored, -1: solve again inline
   int32_t bmax;
   int32_t width;
@@ -64,9 +65,12 @@ struct EventArgs {
 // number of the t_eval points pushed before the bracket was seen.
 enum : int { kEntryT0 = 0, kEntryH = 1, kEntryG0 = 2, kEntryG1 = 3, kEntryTail = 4,
              kEntryTevals = 5, kEntryMeta = 6 };
-// Result written by the root pass over the interpolant's first fields.
-enum : int { kResFlag = kEntryMeta, kResT = kEntryMeta + 1, kResY = kEntryMeta + 2 };
+// Result record of a bracket, written by the root pass: flag, time, state,
+// then the t_eval fields the push rule needs; one 64-byte aligned record for
+// states of up to four components.
+enum : int { kResFlag = 0, kResT = 1, kResY = 2 };
 enum : int { kResNone = 0, kResPoint = 1, kResPanic = 2 };
+constexpr int kResWidth = 8;
 // Lane status of a trajectory whose deferred region is full: its results are
 // produced by the inline solve that follows.
 constexpr int32_t kDeferOverflow = 100;
@@ -230,13 +234,17 @@ DE_HD double* event_defer(const EventArgs& e, int64_t i, int32_t& k, bool need, 
   return ent;
 }
 
-// Result of a root search written over the entry's interpolant.
+// Result record of a root search.
 template <int D>
-DE_HD void event_result(double* ent, int flag, double t, const double* y) {
-  ent[kResFlag] = (double)flag;
-  ent[kResT] = t;
+DE_HD void event_result(double* r, int flag, double t, const double* y, double tail,
+                        double tevals) {
+  static_assert(4 + D <= kResWidth, "result record");
+  r[kResFlag] = (double)flag;
+  r[kResT] = t;
 #pragma unroll
-  for (int c = 0; c < D; ++c) ent[kResY + c] = flag == kResPoint ? y[c] : 0.0;
+  for (int c = 0; c < D; ++c) r[kResY + c] = flag == kResPoint ? y[c] : 0.0;
+  r[kResY + D] = tail;
+  r[kResY + D + 1] = tevals;
 }
 
 // Deferred-mode end of a trajectory: the bracket count (the bookkeeping pass
@@ -322,9 +330,13 @@ struct LaneEvent<D, 1> {
     return event_defer(e, i, k, need, t0, h, last_g, g_new, tail, tevals, overflow);
   }
   // Root pass on one entry: brent_dekker on [t_prev, t_curr] as post() runs it.
+  // Under DefaultSolout the Solution's last time is the step's end, so the
+  // push rule is decided here; under TEvalSolout it depends on the earlier
+  // brackets and is left to book().
   template <class Interp>
-  DE_HD static void root(const EventArgs& e, int64_t i, double* ent, double t_prev,
-                         double t_curr, double dir, Interp& interp) {
+  DE_HD static void root(const EventArgs& e, int64_t i, const double* ent, double* res,
+                         double t_prev, double t_curr, double dir, bool teval_base,
+                         Interp& interp) {
     const double v = e.value_traj ? e.value_traj[i] : e.value;
     double a = t_prev, b = t_curr, fa = ent[kEntryG0], fb = ent[kEntryG1];
     if ((dir > 0.0 && a > b) || (dir < 0.0 && a < b)) {
@@ -336,42 +348,39 @@ struct LaneEvent<D, 1> {
       fb = x;
     }
     auto g = [&](double t, const double* y) { return event_g<D>(e, v, t, y); };
-    double t_event, ye[D];
+    double t_event = 0.0, ye[D];
     int flag = kResNone;
     if (brent_dekker<D>(a, b, fa, fb, interp, g, &t_event))
       flag = interp(t_event, ye) ? kResPoint : kResPanic;
-    event_result<D>(ent, flag, t_event, ye);
+    if (!teval_base && flag == kResPoint && !(fabs(t_event - t_curr) > 1e-14)) flag = kResNone;
+    event_result<D>(res, flag, t_event, ye, ent[kEntryTail], ent[kEntryTevals]);
   }
   // Bookkeeping of one trajectory's entries, in order: the push rule of
   // detect_event (event.rs:379-385) against the Solution's last time, which is
   // the step's end under DefaultSolout, else the last t_eval point pushed
   // since the previous bracket or the previous event point.  Returns false when
   // the trajectory must be solved again inline.
-  template <class Bounds>
-  DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool teval_base,
-                         Bounds& bounds) {
+  // One pass over the trajectory's result records; a failed interpolation
+  // (a reference panic) sends the trajectory to the inline solve, which
+  // rewrites all of its outputs.
+  DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool teval_base) {
     const int32_t cnt = e.bcount[i];
     if (cnt < 0) return false;
-    const double* base = e.brk + (int64_t)i * e.bmax * e.width;
-    for (int32_t k = 0; k < cnt; ++k)
-      if (base[(int64_t)k * e.width + kResFlag] == (double)kResPanic) return false;
+    const double* base = e.res + (int64_t)i * e.bmax * kResWidth;
     int32_t pushed = 0;
     double tail = as_f64(kNaNBits), tevals = 0.0;
-#pragma unroll 4
     for (int32_t k = 0; k < cnt; ++k) {
-      const double* ent = base + (int64_t)k * e.width;
-      if (!teval_base) {
-        double t0, t1;
-        bounds(ent, &t0, &t1);
-        tail = t1;
-      } else if (ent[kEntryTevals] != tevals) {
-        tail = ent[kEntryTail];
-        tevals = ent[kEntryTevals];
+      const double* r = base + (int64_t)k * kResWidth;
+      const double flag = r[kResFlag];
+      if (flag == (double)kResPanic) return false;
+      if (teval_base && r[kResY + D + 1] != tevals) {
+        tail = r[kResY + D];
+        tevals = r[kResY + D + 1];
       }
-      if (ent[kResFlag] == (double)kResPoint) {
-        const double t = ent[kResT];
-        if (tail != tail || fabs(t - tail) > 1e-14) {
-          event_store<D>(e, n, i, pushed, t, ent + kResY);
+      if (flag == (double)kResPoint) {
+        const double t = r[kResT];
+        if (!teval_base || tail != tail || fabs(t - tail) > 1e-14) {
+          event_store<D>(e, n, i, pushed, t, r + kResY);
           pushed += 1;
           tail = t;
         }
@@ -485,8 +494,8 @@ struct LaneEvent<D, 2> {
     return event_defer(e, i, k, need, t0, h, last, off_new, tail, tevals, overflow);
   }
   template <class Interp>
-  DE_HD static void root(const EventArgs& e, int64_t i, double* ent, double t_prev,
-                         double t_curr, double, Interp& interp) {
+  DE_HD static void root(const EventArgs& e, int64_t i, const double* ent, double* res,
+                         double t_prev, double t_curr, double, bool, Interp& interp) {
     LaneEvent<D, 2> s;
     s.value = e.value_traj ? e.value_traj[i] : e.value;
     const double lo = ent[kEntryG0], off = ent[kEntryG1];
@@ -500,20 +509,17 @@ struct LaneEvent<D, 2> {
       }
       if (interp(tc, yc)) flag = kResPoint;
     }
-    event_result<D>(ent, flag, tc, yc);
+    event_result<D>(res, flag, tc, yc, 0.0, 0.0);
   }
-  template <class Bounds>
-  DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool, Bounds&) {
+  DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool) {
     const int32_t cnt = e.bcount[i];
     if (cnt < 0) return false;
-    const double* base = e.brk + (int64_t)i * e.bmax * e.width;
-    for (int32_t k = 0; k < cnt; ++k)
-      if (base[(int64_t)k * e.width + kResFlag] == (double)kResPanic) return false;
+    const double* base = e.res + (int64_t)i * e.bmax * kResWidth;
     int32_t pushed = 0;
-#pragma unroll 4
     for (int32_t k = 0; k < cnt; ++k) {
-      const double* ent = base + (int64_t)k * e.width;
-      event_store<D>(e, n, i, pushed, ent[kResT], ent + kResY);
+      const double* r = base + (int64_t)k * kResWidth;
+      if (r[kResFlag] == (double)kResPanic) return false;
+      event_store<D>(e, n, i, pushed, r[kResT], r + kResY);
       pushed += 1;
     }
     event_finish<D>(e, n, i, pushed);

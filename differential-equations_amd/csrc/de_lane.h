@@ -47,6 +47,7 @@ struct OdeArgs {
   double rtol, atol;
   double h0, h_min, h_max, safety, min_scale, max_scale;
   int64_t max_steps;
+  int64_t max_rejects;  // embedded family: consecutive rejections that end a solve
   const double* t_eval;
   int32_t n_eval;
   double* y_final;
@@ -271,7 +272,8 @@ struct Dopri5Lane {
     *t0 = ent[kEntryT0];
     *t1 = ent[kEntryT0] + ent[kEntryH];
   }
-  DE_HD static void entry_root(const EventArgs& e, int64_t i, double* ent) {
+  DE_HD static bool teval_base(const OdeArgs& a) { return EV == 1 && DENSE && a.n_eval > 0; }
+  DE_HD static void entry_root(const OdeArgs& a, int64_t i, const double* ent, double* res) {
     const double tp = ent[kEntryT0], hp = ent[kEntryH];
     const double t_new = tp + hp;
     const double* d = ent + kEntryMeta;
@@ -300,11 +302,10 @@ struct Dopri5Lane {
       }
       return true;
     };
-    LaneEvent<D, EV>::root(e, i, ent, tp, t_new, signum(hp), interp);
+    LaneEvent<D, EV>::root(a.ev, i, ent, res, tp, t_new, signum(hp), teval_base(a), interp);
   }
   DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
-    auto bounds = [](const double* ent, double* t0, double* t1) { entry_bounds(ent, t0, t1); };
-    return LaneEvent<D, EV>::book(a.ev, a.n, i, EV == 1 && DENSE && a.n_eval > 0, bounds);
+    return LaneEvent<D, EV>::book(a.ev, a.n, i, teval_base(a));
   }
 
   DE_COLD static Dopri5Lane start(const OdeArgs& a, int64_t i, bool* ok) {
@@ -680,18 +681,19 @@ struct Dop853Lane {
     return true;
   }
   // The root pass reads the entry once into registers.
-  DE_HD static void entry_root(const EventArgs& e, int64_t i, double* ent) {
+  DE_HD static bool teval_base(const OdeArgs& a) { return EV == 1 && DENSE && a.n_eval > 0; }
+  DE_HD static void entry_root(const OdeArgs& a, int64_t i, const double* ent, double* res) {
     double loc[kEntryWidth];
 #pragma unroll
     for (int f = 0; f < kEntryWidth; ++f) loc[f] = ent[f];
     auto interp = [&](double te, double* out) { return entry_interp(loc, te, out); };
     double t0, t1;
     entry_bounds(loc, &t0, &t1);
-    LaneEvent<D, EV>::root(e, i, ent, t0, t1, signum(loc[kEntryH]), interp);
+    LaneEvent<D, EV>::root(a.ev, i, loc, res, t0, t1, signum(loc[kEntryH]), teval_base(a),
+                           interp);
   }
   DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
-    auto bounds = [](const double* ent, double* t0, double* t1) { entry_bounds(ent, t0, t1); };
-    return LaneEvent<D, EV>::book(a.ev, a.n, i, EV == 1 && DENSE && a.n_eval > 0, bounds);
+    return LaneEvent<D, EV>::book(a.ev, a.n, i, teval_base(a));
   }
 
   DE_COLD static Dop853Lane start(const OdeArgs& a, int64_t i, bool* ok) {
@@ -1050,18 +1052,18 @@ struct FixedLane {
       out[c] = ((h00 * d[c] + w10 * d[D + c]) + h01 * d[2 * D + c]) + w11 * d[3 * D + c];
     return true;
   }
-  DE_HD static void entry_root(const EventArgs& e, int64_t i, double* ent) {
+  DE_HD static bool teval_base(const OdeArgs& a) { return EV == 1 && a.n_eval > 0; }
+  DE_HD static void entry_root(const OdeArgs& a, int64_t i, const double* ent, double* res) {
     double loc[kEntryWidth];
 #pragma unroll
     for (int f = 0; f < kEntryWidth; ++f) loc[f] = ent[f];
     auto interp = [&](double te, double* out) { return entry_interp(loc, te, out); };
     double t0, t1;
     entry_bounds(loc, &t0, &t1);
-    LaneEvent<D, EV>::root(e, i, ent, t0, t1, signum(t1 - t0), interp);
+    LaneEvent<D, EV>::root(a.ev, i, loc, res, t0, t1, signum(t1 - t0), teval_base(a), interp);
   }
   DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
-    auto bounds = [](const double* ent, double* t0, double* t1) { entry_bounds(ent, t0, t1); };
-    return LaneEvent<D, EV>::book(a.ev, a.n, i, EV == 1 && a.n_eval > 0, bounds);
+    return LaneEvent<D, EV>::book(a.ev, a.n, i, teval_base(a));
   }
 
   DE_HD bool begin(const OdeArgs& a, int64_t i) { return init(a, i); }

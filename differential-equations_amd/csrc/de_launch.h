@@ -9,6 +9,12 @@ namespace de {
 
 enum : int { kSchedAuto = 0, kSchedPlain = 1, kSchedPersistent = 2, kSchedPersistentFifo = 3 };
 enum : int { kMethodDopri5 = 0, kMethodRk4 = 1, kMethodEuler = 2, kMethodDop853 = 3 };
+// The embedded family (DE_METHOD_RKF45 .. DE_METHOD_RKV989E).
+enum : int {
+  kMethodRkf45 = 4, kMethodCashKarp = 5, kMethodRkv655e = 6, kMethodRkv656e = 7,
+  kMethodRkv766e = 8, kMethodRkv767e = 9, kMethodRkv877e = 10, kMethodRkv878e = 11,
+  kMethodRkv988e = 12, kMethodRkv989e = 13
+};
 enum : int { kSchemeEulerMaruyama = 0, kSchemeMilstein = 1 };
 enum : int { kNoisePhilox = 0, kNoiseTable = 1, kNoiseConstant = 2 };
 
@@ -39,6 +45,12 @@ int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipS
 int launch_ode_event(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 int launch_ode_crossing(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 int launch_sde(int family, int scheme, const SdeArgs& a, hipStream_t s);
+// The embedded family, two tableaux per translation unit.
+int launch_ode_erk0(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+int launch_ode_erk1(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+int launch_ode_erk2(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+int launch_ode_erk3(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+int launch_ode_erk4(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 }  // namespace parity
 namespace fast {
 int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);

@@ -142,7 +142,8 @@ __global__ __launch_bounds__(kBlock) void event_root_kernel(const OdeArgs* __res
       else hi = mid;
     }
     const int64_t i = i0 + lo;
-    Lane::entry_root(a.ev, i, a.ev.brk + (i * a.ev.bmax + (j - off[lo])) * a.ev.width);
+    const int64_t slot = i * a.ev.bmax + (j - off[lo]);
+    Lane::entry_root(a, i, a.ev.brk + slot * a.ev.width, a.ev.res + slot * kResWidth);
   }
 }
 
@@ -164,6 +165,7 @@ static __global__ void rerun_args_kernel(const OdeArgs* src, OdeArgs* dst, const
                                          const uint32_t* count) {
   OdeArgs r = *src;
   r.ev.brk = nullptr;
+  r.ev.res = nullptr;
   r.ev.bcount = nullptr;
   r.order = order;
   r.n_dev = count;
@@ -171,9 +173,9 @@ static __global__ void rerun_args_kernel(const OdeArgs* src, OdeArgs* dst, const
 }
 
 // Deferred entries per trajectory: max_events + half again (at least 32 more),
-// halved while the region would exceed this budget; below 16 the roots are
-// found inline.
-constexpr size_t kDeferBudget = (size_t)32 << 30;
+// halved while the regions (entries and result records) would exceed this
+// budget; below 16 the roots are found inline.
+constexpr size_t kDeferBudget = (size_t)48 << 30;
 
 template <class Lane>
 static int64_t defer_slots(const OdeArgs& a) {
@@ -182,7 +184,8 @@ static int64_t defer_slots(const OdeArgs& a) {
   } else {
     if (a.ev.terminate != 0 || a.n > 0x7fffffffLL) return 0;
     int64_t bmax = (int64_t)a.ev.max_events + std::max<int64_t>(a.ev.max_events / 2, 32);
-    while (bmax >= 16 && (size_t)a.n * (size_t)bmax * Lane::kEntryWidth * sizeof(double) >
+    while (bmax >= 16 && (size_t)a.n * (size_t)bmax * (Lane::kEntryWidth + kResWidth) *
+                                 sizeof(double) >
                              kDeferBudget)
       bmax /= 2;
     return bmax >= 16 ? bmax : 0;
@@ -306,7 +309,8 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
   size_t defer_bytes = 0;
   for (;;) {
     defer_bytes = bmax > 0 ? args_bytes + 2 * list_bytes + 256 +
-                                 (size_t)a.n * (size_t)bmax * Lane::kEntryWidth * sizeof(double)
+                                 (size_t)a.n * (size_t)bmax * (Lane::kEntryWidth + kResWidth) *
+                                     sizeof(double)
                            : 0;
     if (hipMallocAsync(&work, args_bytes + ev_bytes + order_bytes + defer_bytes, s) == hipSuccess)
       break;
@@ -337,7 +341,8 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
       bcount = (uint32_t*)(d + args_bytes);
       rerun = (uint32_t*)(d + args_bytes + list_bytes);
       nrerun = (uint32_t*)(d + args_bytes + 2 * list_bytes);
-      pa.ev.brk = (double*)(d + args_bytes + 2 * list_bytes + 256);
+      pa.ev.res = (double*)(d + args_bytes + 2 * list_bytes + 256);
+      pa.ev.brk = pa.ev.res + (size_t)a.n * (size_t)bmax * kResWidth;
       pa.ev.bcount = (int32_t*)bcount;
       pa.ev.bmax = (int32_t)bmax;
       pa.ev.width = Lane::kEntryWidth;

@@ -400,6 +400,7 @@ bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_l
   a.min_scale = b.min_scale;
   a.max_scale = b.max_scale;
   a.max_steps = b.max_steps;
+  a.max_rejects = b.max_rejects;
   a.t_eval = (const double*)put(r->t_eval.data(), r->t_eval.size() * 8);
   a.n_eval = b.n_eval;
   a.y_final = (double*)take((size_t)r->state_len * 8);

@@ -25,13 +25,15 @@ def lib():
 
 # ----------------------------------------------------------------- methods --
 class ExplicitRungeKutta:
-    """ExplicitRungeKutta::{dopri5, dop853, rk4, euler} with the builder setters of
+    """ExplicitRungeKutta::{dopri5, dop853, rk4, euler} and the embedded family
+    (rkf45, cash_karp, rkv655e .. rkv989e) with the builder setters of
     src/methods/erk/mod.rs:165-228 and the defaults of :115-158."""
 
     def __init__(self, method, h0=0.0):
         self.method_id = method
         self.settings = dict(rtol=1e-6, atol=1e-6, h0=h0, h_min=0.0, h_max=float("inf"),
-                             max_steps=10000, safety_factor=0.9, min_scale=0.2, max_scale=10.0)
+                             max_steps=10000, max_rejects=100, safety_factor=0.9,
+                             min_scale=0.2, max_scale=10.0)
 
     @classmethod
     def dopri5(cls):
@@ -41,6 +43,47 @@ class ExplicitRungeKutta:
     def dop853(cls):
         """ExplicitRungeKutta::dop853 (src/methods/erk/dormandprince/mod.rs:46-52)."""
         return cls(DE_METHOD_DOP853)
+
+    # The embedded family, src/methods/erk/adaptive/mod.rs:43-122.
+    @classmethod
+    def rkf45(cls):
+        return cls(DE_METHOD_RKF45)
+
+    @classmethod
+    def cash_karp(cls):
+        return cls(DE_METHOD_CASH_KARP)
+
+    @classmethod
+    def rkv655e(cls):
+        return cls(DE_METHOD_RKV655E)
+
+    @classmethod
+    def rkv656e(cls):
+        return cls(DE_METHOD_RKV656E)
+
+    @classmethod
+    def rkv766e(cls):
+        return cls(DE_METHOD_RKV766E)
+
+    @classmethod
+    def rkv767e(cls):
+        return cls(DE_METHOD_RKV767E)
+
+    @classmethod
+    def rkv877e(cls):
+        return cls(DE_METHOD_RKV877E)
+
+    @classmethod
+    def rkv878e(cls):
+        return cls(DE_METHOD_RKV878E)
+
+    @classmethod
+    def rkv988e(cls):
+        return cls(DE_METHOD_RKV988E)
+
+    @classmethod
+    def rkv989e(cls):
+        return cls(DE_METHOD_RKV989E)
 
     @classmethod
     def rk4(cls, h0):
@@ -74,6 +117,11 @@ class ExplicitRungeKutta:
 
     def max_steps(self, v):
         return self._set("max_steps", int(v))
+
+    def max_rejects(self, v):
+        """ExplicitRungeKutta::max_rejects (erk/mod.rs:201), the embedded family's
+        stiffness limit."""
+        return self._set("max_rejects", int(v))
 
     def safety_factor(self, v):
         return self._set("safety_factor", v)

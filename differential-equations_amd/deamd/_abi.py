@@ -40,6 +40,16 @@ DE_METHOD_DOPRI5 = 0
 DE_METHOD_RK4 = 1
 DE_METHOD_EULER = 2
 DE_METHOD_DOP853 = 3
+DE_METHOD_RKF45 = 4
+DE_METHOD_CASH_KARP = 5
+DE_METHOD_RKV655E = 6
+DE_METHOD_RKV656E = 7
+DE_METHOD_RKV766E = 8
+DE_METHOD_RKV767E = 9
+DE_METHOD_RKV877E = 10
+DE_METHOD_RKV878E = 11
+DE_METHOD_RKV988E = 12
+DE_METHOD_RKV989E = 13
 
 DE_SCHEME_EULER_MARUYAMA = 0
 DE_SCHEME_MILSTEIN = 1
@@ -101,6 +111,7 @@ class OdeBatch(C.Structure):
         ("event_terminate", C.c_int32), ("max_events", C.c_int32),
         ("event_value", C.c_double), ("event_value_traj", C.c_void_p),
         ("event_point", C.c_void_p), ("event_normal", C.c_void_p),
+        ("max_rejects", C.c_int64),
     ]
 
 
@@ -146,6 +157,7 @@ def ode_defaults():
     b.h_min = 0.0
     b.h_max = float("inf")
     b.max_steps = 10000
+    b.max_rejects = 100
     b.safety_factor = 0.9
     b.min_scale = 0.2
     b.max_scale = 10.0

@@ -75,6 +75,18 @@ extern "C" {
 #define DE_METHOD_RK4 1
 #define DE_METHOD_EULER 2
 #define DE_METHOD_DOP853 3 /* ExplicitRungeKutta::dop853 (dormandprince/mod.rs:46-52) */
+/* The embedded family ExplicitRungeKutta<_, Adaptive, ..> (erk/adaptive/mod.rs:43-122):
+ * inf-norm error, stiffness = max_rejects consecutive rejections. */
+#define DE_METHOD_RKF45 4     /* ::rkf45(), Fehlberg 4(5) */
+#define DE_METHOD_CASH_KARP 5 /* ::cash_karp() */
+#define DE_METHOD_RKV655E 6   /* ::rkv655e() .. ::rkv989e(): Verner pairs with interpolants */
+#define DE_METHOD_RKV656E 7
+#define DE_METHOD_RKV766E 8
+#define DE_METHOD_RKV767E 9
+#define DE_METHOD_RKV877E 10
+#define DE_METHOD_RKV878E 11
+#define DE_METHOD_RKV988E 12
+#define DE_METHOD_RKV989E 13
 
 /* SDE schemes: ExplicitRungeKutta::euler(dt) on an SDE (Euler-Maruyama,
  * src/methods/erk/fixed/stochastic.rs) and Milstein::new(dt)
@@ -176,6 +188,9 @@ typedef struct de_ode_batch {
   const double* event_point;      /* [dim] point of DE_EVENT_PLANE / DE_SOLOUT_HYPERPLANE */
   const double* event_normal;     /* [dim] normal (normalised by the hyperplane solout,
                                      src/solout/hyperplane.rs:126-132) */
+  /* ExplicitRungeKutta::max_rejects of the embedded family (erk/mod.rs:81,140,201):
+   * consecutive rejections that end the solve with Error::Stiffness. */
+  int64_t max_rejects;
 } de_ode_batch;
 
 typedef struct de_ode_result {
@@ -226,7 +241,7 @@ typedef struct de_sde_result {
 } de_sde_result;
 
 /* Fill a descriptor with the reference defaults: rtol = atol = 1e-6, h0 = 0,
- * h_min = 0, h_max = inf, max_steps = 10000, safety 0.9, scales [0.2, 10]
+ * h_min = 0, h_max = inf, max_steps = 10000, max_rejects = 100, safety 0.9, scales [0.2, 10]
  * (ExplicitRungeKutta::default, src/methods/erk/mod.rs:115-158). */
 void de_ode_batch_init(de_ode_batch* b);
 void de_sde_batch_init(de_sde_batch* b);

@@ -12,6 +12,9 @@
 //   solve_sde           src/sde/solve_ivp.rs:135-287
 //   DOPRI5 init/step    src/methods/erk/dormandprince/ordinary.rs:16-270
 //   DOPRI5 interpolate  src/methods/erk/dormandprince/ordinary.rs:298-336
+//   embedded ERK family src/methods/erk/adaptive/ordinary.rs:13-293 (rkf45,
+//                       cash_karp, rkv655e..rkv989e; tableaux
+//                       src/tableau/runge_kutta.rs, src/tableau/verner.rs)
 //   tableau dopri5      src/tableau/dorman_prince.rs:37-117
 //   tableau rk4         src/tableau/runge_kutta.rs:39-70
 //   defaults            src/methods/erk/mod.rs:115-158
@@ -46,6 +49,7 @@
 #include <vector>
 
 #include "../differential-equations_amd/csrc/de_dop853_table.h"
+#include "../differential-equations_amd/csrc/de_erk_tables.h"
 #include "../differential-equations_amd/csrc/de_families.h"
 #include "../differential-equations_amd/csrc/de_noise.h"
 #include "../include/de_api.h"
@@ -82,6 +86,7 @@ struct SpringSys {
 struct Settings {
   double rtol, atol, h0, h_min, h_max, safety, min_scale, max_scale;
   int64_t max_steps;
+  int64_t max_rejects = 100;
   // Tolerance indexed per component (src/tolerance.rs:33-51): this trajectory's
   // rtol[c], atol[c], the scalar replicated when no vector was given.
   const double* rtolv = nullptr;
@@ -936,6 +941,150 @@ struct DormandPrince {
   }
 };
 
+// -------------------------------------------------- embedded ERK family ----
+// ExplicitRungeKutta<Ordinary, Adaptive, T, Y, O, S, I>
+// (src/methods/erk/adaptive/ordinary.rs:13-293) with any tableau of the family:
+// stages S, dense-output stages up to I, order O, FSAL.  Every tableau term is
+// applied (zero coefficients included), the error is the inf-norm
+// (src/traits.rs:597-610, NaN-ignoring max), and max_rejects consecutive
+// rejections end the solve with Error::Stiffness.
+template <class Sys, class Tab>
+struct AdaptiveErk {
+  static constexpr int M = Sys::kMax;
+  static constexpr int S = Tab::S, I = Tab::I, O = Tab::O;
+  const Sys& sys;
+  Settings st;
+  int n;
+  double h0, h = 0.0, t = 0.0, h_prev = 0.0, t_prev = 0.0;
+  double y[M], dydt[M], y_prev[M], dydt_prev[M];
+  double k[I][M];
+  int64_t steps = 0, stiffness_counter = 0;
+  bool rejected_status = false;
+
+  AdaptiveErk(const Sys& s, const Settings& set) : sys(s), st(set), n(s.dim()), h0(set.h0) {}
+
+  // ordinary.rs:16-61.  InitialStepSize::compute counts its two evaluations
+  // and init adds two more (ordinary.rs:28): 5 evaluations before the first step.
+  bool init(double t0, double tf, const double* y0, int64_t* evals) {
+    if (h0 == 0.0) {
+      h0 = h_init(sys, t0, tf, y0, O, st.rtolv, st.atolv, st.h_min, st.h_max, evals);
+      *evals += 2;
+    }
+    if (!de::validate_step_size(h0, st.h_min, st.h_max, t0, tf)) return false;
+    h = h0;
+    stiffness_counter = 0;
+    t = t0;
+    copy(y, y0, n);
+    for (int i = 0; i < I; ++i) zero(k[i], n);
+    sys.diff(t, y, dydt);
+    *evals += 1;
+    t_prev = t;
+    copy(y_prev, y, n);
+    copy(dydt_prev, dydt, n);
+    return true;
+  }
+
+  // ordinary.rs:63-222; returns a status code, or -1 to continue.
+  int step(int64_t* evals) {
+    if (fabs(h) < fabs(h_prev) * 1e-14) return DE_STATUS_STEP_SIZE;
+    if (steps >= st.max_steps) return DE_STATUS_MAX_STEPS;
+    steps += 1;
+    copy(k[0], dydt, n);
+    for (int i = 1; i < S; ++i) {
+      double y_stage[M];
+      copy(y_stage, y, n);
+      for (int j = 0; j < i; ++j) add_scaled(y_stage, Tab::A[i][j] * h, k[j], n);
+      sys.diff(t + Tab::C[i] * h, y_stage, k[i]);
+    }
+    *evals += S - 1;
+    double y_high[M], y_low[M], err[M];
+    copy(y_high, y, n);
+    for (int i = 0; i < S; ++i) add_scaled(y_high, Tab::B[i] * h, k[i], n);
+    copy(y_low, y, n);
+    for (int i = 0; i < S; ++i) add_scaled(y_low, Tab::BH[i] * h, k[i], n);
+    // err = y_high.minus(&y_low) = y_high + (-1) y_low
+    copy(err, y_high, n);
+    add_scaled(err, -1.0, y_low, n);
+    // error_norm_inf (src/traits.rs:597-610)
+    double err_norm = 0.0;
+    for (int c = 0; c < n; ++c) {
+      double sk = st.atolv[c] + st.rtolv[c] * fmax(fabs(y[c]), fabs(y_high[c]));
+      err_norm = fmax(err_norm, fabs(err[c] / sk));
+    }
+    const double error_exponent = 1.0 / (double)O;
+    double scale = st.safety * ref_powf(err_norm, -error_exponent);
+    scale = fmin(fmax(scale, st.min_scale), st.max_scale);
+    if (err_norm <= 1.0) {
+      t_prev = t;
+      copy(y_prev, y, n);
+      copy(dydt_prev, k[0], n);
+      h_prev = h;
+      if (rejected_status) {
+        stiffness_counter = 0;
+        rejected_status = false;
+        scale = fmin(scale, 1.0);
+      }
+      if (Tab::kBi) {
+        for (int i = 0; i < I - S; ++i) {
+          double y_stage[M];
+          copy(y_stage, y, n);
+          for (int j = 0; j < S + i; ++j) add_scaled(y_stage, Tab::A[S + i][j] * h, k[j], n);
+          sys.diff(t + Tab::C[S + i] * h, y_stage, k[S + i]);
+        }
+        *evals += I - S;
+      }
+      t += h;
+      copy(y, y_high, n);
+      if (Tab::kFsal) {
+        copy(dydt, k[S - 1], n);
+      } else {
+        sys.diff(t, y, dydt);
+        *evals += 1;
+      }
+    } else {
+      rejected_status = true;
+      stiffness_counter += 1;
+      if (stiffness_counter >= st.max_rejects) return DE_STATUS_STIFFNESS;
+    }
+    h *= scale;
+    h = de::constrain_step_size(h, st.h_min, st.h_max);
+    return -1;
+  }
+
+  // ordinary.rs:244-292: Horner on the bi rows (all I stages), or the cubic
+  // Hermite of the step's ends (src/interpolate.rs:40-60); false for
+  // Err(OutOfBounds).
+  bool interpolate(double tt, double* out) const {
+    if (tt < t_prev || tt > t) return false;
+    if (Tab::kBi) {
+      double s = (tt - t_prev) / h_prev;
+      double cont[I];
+      for (int i = 0; i < I; ++i) {
+        cont[i] = Tab::BI[i][O - 1];
+        for (int j = O - 2; j >= 0; --j) cont[i] = cont[i] * s + Tab::BI[i][j];
+        cont[i] *= s;
+      }
+      copy(out, y_prev, n);
+      for (int i = 0; i < I; ++i) add_scaled(out, cont[i] * h_prev, k[i], n);
+      return true;
+    }
+    double hh = t - t_prev;
+    double s = (tt - t_prev) / hh;
+    double s2 = s * s;
+    double s3 = s2 * s;
+    double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+    double h10 = s3 - 2.0 * s2 + s;
+    double h01 = -2.0 * s3 + 3.0 * s2;
+    double h11 = s3 - s2;
+    zero(out, n);
+    add_scaled(out, h00, y_prev, n);
+    add_scaled(out, h10 * hh, dydt_prev, n);
+    add_scaled(out, h01, y, n);
+    add_scaled(out, h11 * hh, dydt, n);
+    return true;
+  }
+};
+
 // ------------------------------------------------------- fixed-step ERK ----
 // src/methods/erk/fixed/ordinary.rs (rk4, euler; fsal = false, bi = None)
 template <class Sys>
@@ -1389,7 +1538,28 @@ static void run_ode_traj(const Sys& sys, const de_ode_batch* b, const Settings& 
     evs.out = eo;
     sol.ev = &evs;
   }
-  if (b->method == DE_METHOD_RK4 || b->method == DE_METHOD_EULER) {
+  if (b->method >= DE_METHOD_RKF45) {
+    switch (b->method) {
+#define DE_ORC_ERK(ID, TAB)                              \
+  case ID: {                                             \
+    AdaptiveErk<Sys, de::erk_tab::TAB> m(sys, st);       \
+    *res = solve_ode(m, t0, tf, y0, sol, y_out);         \
+    break;                                               \
+  }
+      DE_ORC_ERK(DE_METHOD_RKF45, Rkf45)
+      DE_ORC_ERK(DE_METHOD_CASH_KARP, CashKarp)
+      DE_ORC_ERK(DE_METHOD_RKV655E, Rkv655e)
+      DE_ORC_ERK(DE_METHOD_RKV656E, Rkv656e)
+      DE_ORC_ERK(DE_METHOD_RKV766E, Rkv766e)
+      DE_ORC_ERK(DE_METHOD_RKV767E, Rkv767e)
+      DE_ORC_ERK(DE_METHOD_RKV877E, Rkv877e)
+      DE_ORC_ERK(DE_METHOD_RKV878E, Rkv878e)
+      DE_ORC_ERK(DE_METHOD_RKV988E, Rkv988e)
+      DE_ORC_ERK(DE_METHOD_RKV989E, Rkv989e)
+#undef DE_ORC_ERK
+      default: break;
+    }
+  } else if (b->method == DE_METHOD_RK4 || b->method == DE_METHOD_EULER) {
     FixedErk<Sys> m(sys, st, b->method == DE_METHOD_RK4 ? kRK4 : kEuler);
     *res = solve_ode(m, t0, tf, y0, sol, y_out);
   } else if (b->method == DE_METHOD_DOP853) {
@@ -1493,6 +1663,27 @@ static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings
 
 }  // namespace orc
 
+template <class Tab>
+static void erk_tableau_copy(double* c, double* a, double* b, double* bh, double* bi,
+                             int* dims) {
+  dims[0] = Tab::S;
+  dims[1] = Tab::I;
+  dims[2] = Tab::O;
+  dims[3] = Tab::kFsal;
+  dims[4] = Tab::kBi;
+  for (int i = 0; i < Tab::I; ++i) {
+    if (c) c[i] = Tab::C[i];
+    for (int j = 0; j < Tab::I; ++j)
+      if (a) a[i * Tab::I + j] = Tab::A[i][j];
+    for (int j = 0; j < Tab::O; ++j)
+      if (bi) bi[i * Tab::O + j] = Tab::BI[i][j];
+  }
+  for (int i = 0; i < Tab::S; ++i) {
+    if (b) b[i] = Tab::B[i];
+    if (bh) bh[i] = Tab::BH[i];
+  }
+}
+
 extern "C" {
 
 // Oracle batched ODE solve over host memory.  pow_mode: 0 project pow,
@@ -1501,7 +1692,7 @@ int de_oracle_ode_solve(const de_ode_batch* b, de_ode_result* out, int pow_mode,
   using namespace orc;
   g_pow_mode = pow_mode;
   Settings st{b->rtol, b->atol, b->h0, b->h_min, b->h_max, b->safety_factor,
-              b->min_scale, b->max_scale, b->max_steps};
+              b->min_scale, b->max_scale, b->max_steps, b->max_rejects};
   EventSpec es;
   std::vector<double> normal;
   const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
@@ -1612,6 +1803,26 @@ void de_oracle_dop853_tableau(double* c, double* a, double* b, double* bh, doubl
     b[i] = kDOP853.b[i];
     bh[i] = kDOP853.bh[i];
     er[i] = kDOP853.er[i];
+  }
+}
+
+// The embedded family's tableau as the oracle uses it: dims = {S, I, O, fsal,
+// has_bi}; c[I], a[I*I], b[S], bh[S], bi[I*O] (row-major).  Returns 0, or -1
+// for a method id outside the family.
+int de_oracle_erk_tableau(int method, double* c, double* a, double* b, double* bh, double* bi,
+                          int* dims) {
+  switch (method) {
+    case DE_METHOD_RKF45: erk_tableau_copy<de::erk_tab::Rkf45>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_CASH_KARP: erk_tableau_copy<de::erk_tab::CashKarp>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV655E: erk_tableau_copy<de::erk_tab::Rkv655e>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV656E: erk_tableau_copy<de::erk_tab::Rkv656e>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV766E: erk_tableau_copy<de::erk_tab::Rkv766e>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV767E: erk_tableau_copy<de::erk_tab::Rkv767e>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV877E: erk_tableau_copy<de::erk_tab::Rkv877e>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV878E: erk_tableau_copy<de::erk_tab::Rkv878e>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV988E: erk_tableau_copy<de::erk_tab::Rkv988e>(c, a, b, bh, bi, dims); return 0;
+    case DE_METHOD_RKV989E: erk_tableau_copy<de::erk_tab::Rkv989e>(c, a, b, bh, bi, dims); return 0;
+    default: return -1;
   }
 }
 

@@ -52,6 +52,8 @@ def load():
         lib.de_oracle_philox.restype = None
         lib.de_oracle_philox.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         lib.de_oracle_dopri5_tableau.restype = None
+        lib.de_oracle_erk_tableau.restype = C.c_int
+        lib.de_oracle_erk_tableau.argtypes = [C.c_int] + [C.c_void_p] * 6
         lib.de_oracle_dopri5_tableau.argtypes = [C.c_void_p] * 5
         _lib = lib
     return _lib
@@ -74,6 +76,21 @@ def dop853_tableau():
     load().de_oracle_dop853_tableau(c.ctypes.data, a.ctypes.data, b.ctypes.data,
                                     bh.ctypes.data, er.ctypes.data, bi.ctypes.data)
     return c, a.reshape(16, 16), b, bh, er, bi.reshape(16, 16)
+
+
+def erk_tableau(method):
+    """The embedded family's tableau as the oracle uses it: a dict with S, I, O,
+    fsal, has_bi and the arrays C[I], A[I, I], B[S], BH[S], BI[I, O]."""
+    import numpy as np
+    dims = np.zeros(5, dtype=np.int32)
+    if load().de_oracle_erk_tableau(method, None, None, None, None, None, dims.ctypes.data) != 0:
+        raise ValueError("not an embedded-family method id: %d" % method)
+    S, I, O = int(dims[0]), int(dims[1]), int(dims[2])
+    c, a, b, bh, bi = np.empty(I), np.empty(I * I), np.empty(S), np.empty(S), np.empty(I * O)
+    load().de_oracle_erk_tableau(method, c.ctypes.data, a.ctypes.data, b.ctypes.data,
+                                 bh.ctypes.data, bi.ctypes.data, dims.ctypes.data)
+    return dict(S=S, I=I, O=O, fsal=bool(dims[3]), has_bi=bool(dims[4]), C=c,
+                A=a.reshape(I, I), B=b, BH=bh, BI=bi.reshape(I, O))
 
 
 def solve_ode(ensemble, pow_mode=POW_PROJECT, threads=1):

@@ -1,4 +1,4 @@
-// The reference's DOPRI5 / RK4 / Euler accuracy and error tests
+// The reference's DOPRI5 / RK4 / Euler / embedded-family accuracy and error tests
 // (tests/ode/accuracy.rs, tests/ode/interpolation.rs, tests/ode/errors.rs,
 // tests/ode/from_fn.rs) written against the C++ builder mirror and run on the
 // GPU through the C-ABI.  Each case solves a small ensemble (the same problem
@@ -73,8 +73,50 @@ int main() {
   test_ode("Euler from_fn", ExponentialGrowth{{1.0}}, 0.0, 1.0, {1.0}, {2.5937},
            ExplicitRungeKutta::euler(0.1), 1e-3);
 
+  // tests/ode/accuracy.rs rows of the embedded family (rkv656e has no row and
+  // takes rkv655e's tolerances); each tableau, each system.
+  {
+    struct Named {
+      const char* name;
+      ExplicitRungeKutta m;
+      bool verner;
+    };
+    const Named family[] = {
+        {"RKF45", ExplicitRungeKutta::rkf45(), false},
+        {"CashKarp", ExplicitRungeKutta::cash_karp(), false},
+        {"RKV655e", ExplicitRungeKutta::rkv655e(), true},
+        {"RKV656e", ExplicitRungeKutta::rkv656e(), true},
+        {"RKV766e", ExplicitRungeKutta::rkv766e(), true},
+        {"RKV767e", ExplicitRungeKutta::rkv767e(), true},
+        {"RKV877e", ExplicitRungeKutta::rkv877e(), true},
+        {"RKV878e", ExplicitRungeKutta::rkv878e(), true},
+        {"RKV988e", ExplicitRungeKutta::rkv988e(), true},
+        {"RKV989e", ExplicitRungeKutta::rkv989e(), true},
+    };
+    char label[128];
+    for (const Named& f : family) {
+      std::snprintf(label, sizeof label, "%s exponential_growth_positive_direction", f.name);
+      test_ode(label, ExponentialGrowth{{1.0}}, 0.0, 10.0, {1.0}, {22026.46579479}, f.m,
+               f.verner ? 1e-1 : 1e3);  // :70-176
+      std::snprintf(label, sizeof label, "%s exponential_growth_negative_direction", f.name);
+      test_ode(label, ExponentialGrowth{{1.0}}, 0.0, -10.0, {22026.46579479}, {1.0}, f.m,
+               1e-2);  // :177-285
+      std::snprintf(label, sizeof label, "%s linear_equation", f.name);
+      test_ode(label, LinearEquation{{1.0}, {1.0}}, 0.0, 10.0, {1.0}, {44051.93158958}, f.m,
+               f.verner ? 1e1 : 1e3);  // :286-394
+      std::snprintf(label, sizeof label, "%s harmonic_oscillator", f.name);
+      test_ode(label, HarmonicOscillator{{1.0}}, 0.0, 10.0, {1.0, 0.0},
+               {-0.83907153, 0.54402111}, f.m, 1e-3);  // :395-503
+      std::snprintf(label, sizeof label, "%s logistic_equation", f.name);
+      test_ode(label, LogisticEquation{{1.0}, {10.0}}, 0.0, 10.0, {0.1}, {9.95525518}, f.m,
+               f.verner ? 1e-3 : 1e-2);  // :504-612
+    }
+  }
+
   // tests/ode/interpolation.rs:22-72
-  for (auto m : {ExplicitRungeKutta::dopri5(), ExplicitRungeKutta::rk4(0.01)}) {
+  for (auto m : {ExplicitRungeKutta::dopri5(), ExplicitRungeKutta::rk4(0.01),
+                 ExplicitRungeKutta::rkv655e(), ExplicitRungeKutta::rkv877e(),
+                 ExplicitRungeKutta::rkv988e(), ExplicitRungeKutta::rkf45()}) {
     std::vector<std::array<double, 1>> y0s(4, {1.0});
     auto sol = IVP<ExponentialGrowth>::ode(ExponentialGrowth{{1.0}}, 0.0, 2.0, y0s)
                    .t_eval({0.5, 1.0, 1.69})

@@ -50,7 +50,7 @@ def _solve(b):
 @pytest.mark.parametrize("mutate,code", [
     (lambda b: setattr(b, "family", 99), _abi.DE_ERR_INVALID),
     (lambda b: setattr(b, "dim", 4), _abi.DE_ERR_INVALID),
-    (lambda b: setattr(b, "method", 7), _abi.DE_ERR_INVALID),
+    (lambda b: setattr(b, "method", 14), _abi.DE_ERR_INVALID),
     (lambda b: setattr(b, "n", -1), _abi.DE_ERR_INVALID),
     (lambda b: setattr(b, "n_eval", 3), _abi.DE_ERR_INVALID),
     (lambda b: setattr(b, "schedule", 9), _abi.DE_ERR_INVALID),
