@@ -20,6 +20,8 @@
 // them on every accepted step, as the reference evaluates them always).
 #pragma once
 
+#include <utility>
+
 #include "de_erk_tables.h"
 #include "de_lane.h"
 
@@ -45,6 +47,17 @@ DE_HD constexpr uint32_t erk_zero_bi_rows() {
   return m;
 }
 
+// Stages that some combination takes with a zero coefficient: rows 1 .. rows-1
+// of a, then b and bh (and the interpolation rows that are zero throughout,
+// when the dense output is built).  Only these need a non-finite record.
+template <class Tab>
+DE_HD constexpr uint32_t erk_zero_stages(int rows, bool dense) {
+  uint32_t m = erk_zero_mask(Tab::B, Tab::S) | erk_zero_mask(Tab::BH, Tab::S);
+  for (int r = 1; r < rows; ++r) m |= erk_zero_mask(Tab::A[r], r);
+  if (dense) m |= erk_zero_bi_rows<Tab>();
+  return m;
+}
+
 template <class F, class Tab, bool DENSE, bool VTOL>
 struct ErkLane {
   static constexpr int D = F::kDim;
@@ -53,10 +66,11 @@ struct ErkLane {
   static constexpr int kEv = 0;
   static constexpr int kEntryWidth = 0;
   static constexpr int KN = (DENSE && kBi) ? I : S;  // stages held in registers
-  // Occupancy hint of the persistent kernel: three waves per SIMD (168 VGPRs)
-  // while the stage array is small, two (256 VGPRs) beyond 20 doubles of
-  // stages, where three waves spill.
-  static constexpr int kWavesPerEu = KN * D <= 20 ? 3 : 2;
+  static constexpr uint32_t kZeroStages = erk_zero_stages<Tab>(KN, DENSE && kBi);
+  // Occupancy hint of the persistent kernel: three waves per SIMD (168 VGPRs);
+  // two (256 VGPRs) for the dense lanes of more than 36 doubles of stages,
+  // which spill at three.
+  static constexpr int kWavesPerEu = (KN == S || KN * D <= 36) ? 3 : 2;
   // Evaluations per accepted step beyond the S - 1 of every attempt
   // (ordinary.rs:151-181): f(t_new, y_new) unless FSAL, the dense stages.
   static constexpr int kAccEvals = (kFsal ? 0 : 1) + (kBi ? I - S : 0);
@@ -151,6 +165,27 @@ struct ErkLane {
     return true;
   }
 
+  // Every coefficient is read into a constexpr local at a compile-time index
+  // (index-sequence folds): the tableau's static arrays are device globals,
+  // whose loads the compiler does not fold, and a coefficient loaded at run
+  // time would hold scalar registers and turn the zero tests into selects.
+
+  // ys += (a[R][J] h) k[J] for a nonzero a[R][J].
+  template <int R, int J>
+  DE_HD static void stage_term(const double (*k)[D], double hh, double* ys) {
+    constexpr double a = Tab::A[R][J];
+    if constexpr (a != 0.0) {
+      const double ah = a * hh;
+#pragma unroll
+      for (int c = 0; c < D; ++c) ys[c] = ys[c] + ah * k[J][c];
+    }
+  }
+  template <int R, int... J>
+  DE_HD static void stage_terms(std::integer_sequence<int, J...>, const double (*k)[D], double hh,
+                                double* ys) {
+    (stage_term<R, J>(k, hh, ys), ...);
+  }
+
   // y + sum_{j<R} (a[R][j] h) k[j] over the nonzero a[R][j], in j order
   // (ordinary.rs:98-106); NaN where a skipped term would be (bad: the
   // per-component masks of non-finite stages, all ones when h is not finite).
@@ -159,43 +194,84 @@ struct ErkLane {
                                 const uint32_t* bad, double* ys) {
 #pragma unroll
     for (int c = 0; c < D; ++c) ys[c] = yy[c];
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      if (Tab::A[R][j] != 0.0) {
-        const double ah = Tab::A[R][j] * hh;
-#pragma unroll
-        for (int c = 0; c < D; ++c) ys[c] = ys[c] + ah * k[j][c];
-      }
-    }
+    stage_terms<R>(std::make_integer_sequence<int, R>{}, k, hh, ys);
     constexpr uint32_t z = erk_zero_mask(Tab::A[R], R);
     if constexpr (z != 0) {
+      uint32_t any = 0;
 #pragma unroll
-      for (int c = 0; c < D; ++c) ys[c] = (bad[c] & z) ? as_f64(kNaNBits) : ys[c];
+      for (int c = 0; c < D; ++c) any |= bad[c];
+      if (__builtin_expect((any & z) != 0, 0)) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) ys[c] = (bad[c] & z) ? as_f64(kNaNBits) : ys[c];
+      }
     }
   }
 
   template <int R, int END>
   DE_HD void stages(double (*k)[D], const double* yy, double tt, double hh, uint32_t* bad) const {
     if constexpr (R < END) {
+      constexpr double cr = Tab::C[R];
       double ys[D];
       stage_input<R>(yy, k, hh, bad, ys);
-      f.diff(tt + Tab::C[R] * hh, ys, k[R]);
+      f.diff(tt + cr * hh, ys, k[R]);
+      if constexpr ((kZeroStages >> R) & 1u) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) bad[c] |= isfinite(k[R][c]) ? 0u : (1u << R);
+        for (int c = 0; c < D; ++c) bad[c] |= isfinite(k[R][c]) ? 0u : (1u << R);
+      }
       stages<R + 1, END>(k, yy, tt, hh, bad);
     }
   }
 
+  // v += (w[J] h) k[J][c] for a nonzero weight (SEL 0: b, 1: bh).
+  template <int SEL, int J>
+  DE_HD static void weight_term(const double (*k)[D], double hh, int c, double& v) {
+    constexpr double w = SEL ? Tab::BH[J] : Tab::B[J];
+    if constexpr (w != 0.0) v = v + (w * hh) * k[J][c];
+  }
+  template <int SEL, int... J>
+  DE_HD static void weight_terms(std::integer_sequence<int, J...>, const double (*k)[D], double hh,
+                                 int c, double& v) {
+    (weight_term<SEL, J>(k, hh, c, v), ...);
+  }
+
   // y + sum_j (w[j] h) k[j] over the nonzero weights (y_high, y_low).
-  template <const double* W>
+  template <int SEL>
   DE_HD static double weighted(double yc, const double (*k)[D], double hh, uint32_t bad, int c) {
     double v = yc;
-#pragma unroll
-    for (int j = 0; j < S; ++j)
-      if (W[j] != 0.0) v = v + (W[j] * hh) * k[j][c];
-    constexpr uint32_t z = erk_zero_mask(W, S);
+    weight_terms<SEL>(std::make_integer_sequence<int, S>{}, k, hh, c, v);
+    constexpr uint32_t z = erk_zero_mask(SEL ? Tab::BH : Tab::B, S);
     if constexpr (z != 0) v = (bad & z) ? as_f64(kNaNBits) : v;
     return v;
+  }
+
+  // Horner over bi[r][0 .. J], from the highest power down.
+  template <int Rr, int J>
+  DE_HD static double horner(double q, double s) {
+    if constexpr (J < 0) {
+      return q;
+    } else {
+      constexpr double b = Tab::BI[Rr][J];
+      return horner<Rr, J - 1>(q * s + b, s);
+    }
+  }
+
+  // v += (cont_r h_prev) k[r] for interpolation row r (ordinary.rs:256-275);
+  // rows zero in every power only when `all`.
+  template <int Rr>
+  DE_HD static void interp_row(const double (*k)[D], double s, double hp, bool all, double* v) {
+    constexpr bool zero = (erk_zero_bi_rows<Tab>() >> Rr) & 1u;
+    if (!zero || all) {
+      constexpr double top = Tab::BI[Rr][O - 1];
+      const double q = horner<Rr, O - 2>(top, s) * s;
+      const double w = q * hp;
+#pragma unroll
+      for (int c = 0; c < D; ++c) v[c] = v[c] + w * k[Rr][c];
+    }
+  }
+  template <int... Rr>
+  DE_HD static void interp_rows(std::integer_sequence<int, Rr...>, const double (*k)[D], double s,
+                                double hp, bool all, double* v) {
+    (interp_row<Rr>(k, s, hp, all, v), ...);
   }
 
   // One pass of the driver loop (solve_ivp.rs:190-263 around ordinary.rs:63-210).
@@ -215,15 +291,15 @@ struct ErkLane {
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       k[0][c] = k0[c];
-      bad[c] = hbad | (isfinite(k0[c]) ? 0u : 1u);
+      bad[c] = hbad | ((kZeroStages & 1u) && !isfinite(k0[c]) ? 1u : 0u);
     }
     stages<1, S>(k, y, t, h, bad);
 
     double yh[D], m = 0.0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const double hi = weighted<Tab::B>(y[c], k, h, bad[c], c);
-      const double lo = weighted<Tab::BH>(y[c], k, h, bad[c], c);
+      const double hi = weighted<0>(y[c], k, h, bad[c], c);
+      const double lo = weighted<1>(y[c], k, h, bad[c], c);
       yh[c] = hi;
       const double e = hi + (-1.0) * lo;
       const double sk = tol.at(c) + tol.rt(c) * fmax(fabs(y[c]), fabs(hi));
@@ -264,25 +340,16 @@ struct ErkLane {
             // finite.
             const double s = (te - tp) / hp;
             constexpr uint32_t zr = erk_zero_bi_rows<Tab>();
-            const uint32_t wbad = isfinite(s) && isfinite(hp) ? 0u : ~0u;
+            uint32_t anyb = 0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) anyb |= bad[c];
+            const bool all = (anyb & zr) != 0 || !isfinite(s) || !isfinite(hp);
             double v[D];
 #pragma unroll
             for (int c = 0; c < D; ++c) v[c] = y[c];
+            interp_rows(std::make_integer_sequence<int, I>{}, k, s, hp, all, v);
 #pragma unroll
-            for (int r = 0; r < I; ++r) {
-              if (!((zr >> r) & 1u)) {
-                double q = Tab::BI[r][O - 1];
-#pragma unroll
-                for (int j = O - 2; j >= 0; --j) q = q * s + Tab::BI[r][j];
-                q = q * s;
-                const double w = q * hp;
-#pragma unroll
-                for (int c = 0; c < D; ++c) v[c] = v[c] + w * k[r][c];
-              }
-            }
-#pragma unroll
-            for (int c = 0; c < D; ++c)
-              out[c] = (zr != 0 && ((bad[c] | wbad) & zr)) ? as_f64(kNaNBits) : v[c];
+            for (int c = 0; c < D; ++c) out[c] = v[c];
           } else {
             // cubic_hermite_interpolate(t_prev, t, y_prev, y, dydt_prev, dydt)
             // (src/interpolate.rs:40-60), a linear combination from zero.
