@@ -273,6 +273,72 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
   return DE_OK;
 }
 
+// ------------------------------------------------------------ to_csv ----
+// The device CSV writer (kernels_csv.hip) is switched off: its first run on
+// the GPU did not return, and no cause has been shown yet (its index
+// arithmetic, bounds and termination check out in a CPU emulation with the
+// same inputs).  The entry points validate their arguments and then refuse.
+constexpr bool kCsvDeviceEnabled = false;
+
+int csv_args(const de_ode_batch* b, const de_ode_result* r, int32_t source, de::CsvArgs* a) {
+  if (!b || !r) return fail(DE_ERR_INVALID, "null descriptor");
+  if (b->n < 0) return fail(DE_ERR_INVALID, "n < 0");
+  if (source < DE_CSV_FINAL || source > DE_CSV_EVENTS)
+    return fail(DE_ERR_INVALID, "unknown csv source %d", source);
+  if (b->family < 0 || b->family > DE_FAMILY_ROBERTSON || b->family == DE_FAMILY_SPRING_CHAIN)
+    return fail(DE_ERR_UNSUPPORTED, "csv output is built for the fixed-dimension families");
+  if (b->dim != kFamilyDim[b->family])
+    return fail(DE_ERR_INVALID, "dim %d does not match family %d", b->dim, b->family);
+  memset(a, 0, sizeof *a);
+  a->n = b->n;
+  a->dim = b->dim;
+  a->source = source;
+  a->t0 = b->t0;
+  a->tf = b->tf;
+  a->t0s = b->t0_scalar;
+  a->tfs = b->tf_scalar;
+  a->t_eval = b->t_eval;
+  a->n_eval = b->n_eval;
+  const bool any = b->n > 0;
+  if (source == DE_CSV_FINAL) {
+    if (any && (!r->t_final || !r->y_final)) return fail(DE_ERR_INVALID, "missing final state");
+    a->max_rows = 1;
+    a->t_final = r->t_final;
+    a->y_final = r->y_final;
+  } else if (source == DE_CSV_TEVAL) {
+    if (b->n_eval <= 0) return fail(DE_ERR_INVALID, "t_eval output needs n_eval > 0");
+    if (any && (!b->t_eval || !r->dense || !r->n_dense))
+      return fail(DE_ERR_INVALID, "missing t_eval, dense or n_dense");
+    a->max_rows = b->n_eval;
+    a->dense = r->dense;
+    a->n_dense = r->n_dense;
+  } else {
+    if (b->max_events < 0) return fail(DE_ERR_INVALID, "max_events < 0");
+    if (any && b->max_events > 0 && (!r->event_t || !r->event_y || !r->n_events))
+      return fail(DE_ERR_INVALID, "missing event_t, event_y or n_events");
+    a->max_rows = b->max_events;
+    a->event_t = r->event_t;
+    a->event_y = r->event_y;
+    a->n_events = r->n_events;
+  }
+  return DE_OK;
+}
+
+// Device copies of a host-memory request's inputs.
+void csv_stage(Staging& st, const de_ode_batch* b, de::CsvArgs* a) {
+  const size_t n = (size_t)a->n, d = (size_t)a->dim;
+  a->t0 = st.in(a->t0, n);
+  a->tf = st.in(a->tf, n);
+  a->t_eval = st.in(a->t_eval, (size_t)(b->n_eval > 0 ? b->n_eval : 0));
+  a->t_final = st.in(a->t_final, n);
+  a->y_final = st.in(a->y_final, d * n);
+  a->dense = st.in(a->dense, (size_t)a->max_rows * d * n);
+  a->n_dense = st.in(a->n_dense, n);
+  a->event_t = st.in(a->event_t, (size_t)a->max_rows * n);
+  a->event_y = st.in(a->event_y, (size_t)a->max_rows * d * n);
+  a->n_events = st.in(a->n_events, n);
+}
+
 }  // namespace
 
 extern "C" {
@@ -280,6 +346,79 @@ extern "C" {
 const char* de_last_error(void) { return g_err.c_str(); }
 
 const char* de_version(void) { return "de_amd 0.1.0 (gfx950, HIP)"; }
+
+int de_ode_csv_offsets(const de_ode_batch* b, const de_ode_result* r, int32_t source,
+                       int64_t* offsets, void* stream) {
+  de::CsvArgs a;
+  int rc = csv_args(b, r, source, &a);
+  if (rc != DE_OK) return rc;
+  if (!offsets) return fail(DE_ERR_INVALID, "null offsets");
+  if (!kCsvDeviceEnabled)
+    return fail(DE_ERR_UNSUPPORTED, "the device CSV writer is switched off in this build");
+  hipStream_t s = (hipStream_t)stream;
+  const bool host = (b->flags & DE_FLAG_HOST_MEMORY) != 0;
+  Staging st;
+  int64_t* d_off = offsets;
+  if (host) {
+    csv_stage(st, b, &a);
+    d_off = st.out(offsets, (size_t)a.n + 1);
+    if (st.err != hipSuccess) return hip_fail(st.err, "csv staging");
+  }
+  if (de::launch_csv(a, d_off, nullptr, 0, s) != 0) return hip_fail(hipGetLastError(), "csv launch");
+  if (host) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "csv sync");
+    st.back(offsets, d_off, (size_t)a.n + 1);
+    if (st.err != hipSuccess) return hip_fail(st.err, "csv copy-back");
+  }
+  return DE_OK;
+}
+
+int de_ode_csv_write(const de_ode_batch* b, const de_ode_result* r, int32_t source,
+                     const int64_t* offsets, char* out, void* stream) {
+  de::CsvArgs a;
+  int rc = csv_args(b, r, source, &a);
+  if (rc != DE_OK) return rc;
+  if (!offsets) return fail(DE_ERR_INVALID, "null offsets");
+  if (!kCsvDeviceEnabled)
+    return fail(DE_ERR_UNSUPPORTED, "the device CSV writer is switched off in this build");
+  hipStream_t s = (hipStream_t)stream;
+  const bool host = (b->flags & DE_FLAG_HOST_MEMORY) != 0;
+  // The caller's buffer size, offsets[n]; the documents are laid out again and
+  // must come to the same total.
+  int64_t cap = 0;
+  if (host) {
+    cap = offsets[a.n];
+  } else {
+    hipError_t e = hipMemcpyAsync(&cap, offsets + a.n, sizeof cap, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "reading offsets[n]");
+  }
+  if (cap < 0) return fail(DE_ERR_INVALID, "offsets[n] < 0");
+  if (cap > 0 && !out) return fail(DE_ERR_INVALID, "null out");
+  Staging st;
+  char* d_out = out;
+  if (host) {
+    csv_stage(st, b, &a);
+    d_out = st.out(out, (size_t)cap);
+  }
+  int64_t* d_off = st.out((int64_t*)offsets, (size_t)a.n + 1);  // the layout, recomputed
+  if (st.err != hipSuccess) return hip_fail(st.err, "csv staging");
+  if (de::launch_csv(a, d_off, cap > 0 ? d_out : nullptr, cap, s) != 0)
+    return hip_fail(hipGetLastError(), "csv launch");
+  int64_t total = 0;
+  hipError_t e = hipMemcpyAsync(&total, d_off + a.n, sizeof total, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "csv sync");
+  if (total != cap)
+    return fail(DE_ERR_INVALID, "offsets[n] = %lld but the documents take %lld bytes",
+                (long long)cap, (long long)total);
+  if (host) {
+    st.back(out, (const char*)d_out, (size_t)cap);
+    if (st.err != hipSuccess) return hip_fail(st.err, "csv copy-back");
+  }
+  return DE_OK;
+}
 
 void de_ode_batch_init(de_ode_batch* b) {
   memset(b, 0, sizeof *b);

@@ -40,6 +40,36 @@ struct SdeArgs {
   double* path_t;
 };
 
+// Solution::to_csv of every trajectory (kernels_csv.hip): the solve's
+// geometry and result arrays, device pointers.
+enum : int { kCsvFinal = 0, kCsvTeval = 1, kCsvEvents = 2 };
+constexpr int kCsvMaxDim = 8;
+struct CsvArgs {
+  int64_t n;
+  int32_t dim;
+  int32_t source;    // kCsv*
+  int32_t max_rows;  // points per trajectory at most: 1, n_eval or max_events
+  const double* t0;  // per trajectory, or null: t0s
+  const double* tf;
+  double t0s, tfs;
+  const double* t_eval;
+  int32_t n_eval;
+  const double* t_final;
+  const double* y_final;
+  const double* dense;
+  const int32_t* n_dense;
+  const double* event_t;
+  const double* event_y;
+  const int32_t* n_events;
+};
+// Bytes of the longest line: dim + 1 numbers of at most 24 characters,
+// their separators and the newline (the header is shorter).
+constexpr int csv_max_slot_bytes(int dim) { return (dim + 1) * 24 + dim + 1; }
+// offsets[0 .. n] and, when out is not null, the documents into out[0 .. cap)
+// (a block of lines that would pass cap is not written).  0, or -1 on a HIP
+// error (hipGetLastError has it).
+int launch_csv(const CsvArgs& a, int64_t* offsets, char* out, int64_t cap, hipStream_t s);
+
 namespace parity {
 int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 int launch_ode_event(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);

@@ -12,6 +12,7 @@ trajectories.  `solve()` runs the HIP library (include/de_api.h); there is no
 CPU fallback: a missing library raises.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -493,7 +494,16 @@ class OdeEnsemble:
     def solve(self, stream=None):
         b, res, keep = self.batch()
         _abi.check(lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), stream))
+        res._solved_by = (b, keep)  # the descriptor, for the CSV writer
         return res
+
+
+def _write_file(filename, data):
+    parent = os.path.dirname(filename)
+    if parent:
+        os.makedirs(parent, exist_ok=True)
+    with open(filename, "wb") as f:
+        f.write(data)
 
 
 class Solution:
@@ -524,6 +534,42 @@ class Solution:
         t = self.event_t.reshape(self.max_events, self.n)[:k, i]
         y = self.event_y.reshape(self.max_events, dim, self.n)[:k, :, i]
         return t.copy(), y.copy()
+
+    _CSV_SOURCES = {"final": DE_CSV_FINAL, "t_eval": DE_CSV_TEVAL, "events": DE_CSV_EVENTS}
+
+    def csv(self, source=None):
+        """Solution::to_csv of every trajectory (src/solution.rs:190-230),
+        formatted on the GPU: (bytes, offsets), trajectory i's document being
+        bytes[offsets[i]:offsets[i + 1]].  `source`: "t_eval" (the t_eval
+        points; the default when the solve had a t_eval output), "final" (the
+        final state; the default otherwise) or "events" (the event points)."""
+        if not hasattr(self, "_solved_by"):
+            raise RuntimeError("csv() needs a Solution returned by OdeEnsemble.solve()")
+        if source is None:
+            source = "t_eval" if self.n_dense is not None else "final"
+        src = self._CSV_SOURCES[source]
+        b, keep = self._solved_by
+        r = self.struct()
+        offsets = np.zeros(self.n + 1, dtype=np.int64)
+        _abi.check(lib().de_ode_csv_offsets(C.byref(b), C.byref(r), src, offsets.ctypes.data,
+                                            None))
+        out = np.zeros(max(1, int(offsets[-1])), dtype=np.uint8)
+        _abi.check(lib().de_ode_csv_write(C.byref(b), C.byref(r), src, offsets.ctypes.data,
+                                          out.ctypes.data, None))
+        return out[:int(offsets[-1])].tobytes(), offsets
+
+    def to_csv(self, filename, i=0, source=None):
+        """Solution::to_csv(filename) of trajectory i: creates the parent
+        directories, then writes the document (an empty file for a trajectory
+        without points, where the reference panics after creating it)."""
+        data, offsets = self.csv(source)
+        _write_file(filename, data[offsets[i]:offsets[i + 1]])
+
+    def to_csv_all(self, pattern, source=None):
+        """One to_csv file per trajectory: pattern.format(i=i)."""
+        data, offsets = self.csv(source)
+        for i in range(self.n):
+            _write_file(pattern.format(i=i), data[offsets[i]:offsets[i + 1]])
 
     def struct(self):
         r = OdeResult()

@@ -67,6 +67,9 @@ DE_MAP_AUTO = 0
 DE_MAP_LANE = 1
 DE_MAP_GROUP = 2
 
+DE_CSV_FINAL = 0
+DE_CSV_TEVAL = 1
+DE_CSV_EVENTS = 2
 DE_SOLOUT_DEFAULT = 0
 DE_SOLOUT_EVENT = 1
 DE_SOLOUT_CROSSING = 2
@@ -179,6 +182,10 @@ SIGNATURES = {
     "de_ode_batch_solve": (C.c_int, [C.POINTER(OdeBatch), C.POINTER(OdeResult), C.c_void_p]),
     "de_sde_batch_solve": (C.c_int, [C.POINTER(SdeBatch), C.POINTER(SdeResult), C.c_void_p]),
     "de_ode_workspace_bytes": (C.c_int64, [C.POINTER(OdeBatch)]),
+    "de_ode_csv_offsets": (C.c_int, [C.POINTER(OdeBatch), C.POINTER(OdeResult), C.c_int32,
+                                     C.c_void_p, C.c_void_p]),
+    "de_ode_csv_write": (C.c_int, [C.POINTER(OdeBatch), C.POINTER(OdeResult), C.c_int32,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "de_last_error": (C.c_char_p, []),
     "de_version": (C.c_char_p, []),
     "de_server_create": (C.c_int, [C.c_size_t, C.c_int32, C.POINTER(C.c_void_p)]),

@@ -256,6 +256,34 @@ int de_ode_batch_solve(const de_ode_batch* batch, de_ode_result* out, void* stre
  * (src/methods/milstein.rs:76-224). */
 int de_sde_batch_solve(const de_sde_batch* batch, de_sde_result* out, void* stream);
 
+/* ------------------------------------------------------------ CSV writer --
+ * Solution::to_csv (src/solution.rs:190-230) for every trajectory of an ODE
+ * solve: one CSV document per trajectory, the header "t,y0,...,y<dim-1>\n"
+ * then one "<t>,<y0>,...,<y dim-1>\n" line per point of the trajectory's
+ * Solution, every number as Rust's `{:?}` of an f64 (the shortest digits
+ * that read back as the same double, the closest of those; "NaN", "inf",
+ * "-inf", "-0.0"; d[.ddd]e<exp> outside 1e-4 <= |x| < 1e16, else positional
+ * with at least one fractional digit).  The documents are concatenated:
+ * trajectory i's is out[offsets[i] .. offsets[i+1]), offsets[0] = 0.  A
+ * trajectory without points gets an empty document (the reference's to_csv
+ * creates the file, then panics indexing y[0]).
+ *
+ * `batch` and `result` are the solve's (n, dim, t0 / tf and t_eval, and the
+ * result arrays of the chosen source); their pointers and offsets / out are
+ * device memory unless batch->flags has DE_FLAG_HOST_MEMORY.  Fixed-dimension
+ * families only (dim <= 8). */
+#define DE_CSV_FINAL 0  /* the final state: the point (t_final, y_final) */
+#define DE_CSV_TEVAL 1  /* the t_eval points produced: n_dense[i] of them, with the
+                           t_eval values as times (TEvalSolout, src/solout/t_eval.rs) */
+#define DE_CSV_EVENTS 2 /* the event points: n_events[i] of them (event_t / event_y) */
+
+/* Document sizes: offsets[0 .. n] (n + 1 int64 values). */
+int de_ode_csv_offsets(const de_ode_batch* batch, const de_ode_result* result, int32_t source,
+                       int64_t* offsets, void* stream);
+/* The documents, out[0 .. offsets[n]), from the offsets of de_ode_csv_offsets. */
+int de_ode_csv_write(const de_ode_batch* batch, const de_ode_result* result, int32_t source,
+                     const int64_t* offsets, char* out, void* stream);
+
 /* Bytes of device workspace a de_ode_batch_solve call with this descriptor
  * allocates beyond its inputs and outputs (HBM admission budget). */
 int64_t de_ode_workspace_bytes(const de_ode_batch* batch);

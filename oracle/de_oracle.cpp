@@ -41,10 +41,13 @@
 
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <atomic>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -1684,6 +1687,177 @@ static void erk_tableau_copy(double* c, double* a, double* b, double* bh, double
   }
 }
 
+// ------------------------------------------------------------ to_csv ----
+// Solution::to_csv (src/solution.rs:190-230): "t,y0,..." then one line per
+// point, each number `format!("{:?}", x)` of an f64.
+//
+// The digits are found by search, not by the product's algorithm: for
+// p = 1, 2, ... 17 significant digits, glibc's printf("%.*e") gives the
+// correctly rounded p-digit decimal and strtod (correctly rounded, ties to
+// even) tells whether it reads back as x; at a power of two the rounding
+// interval is narrower below x than above, so when the nearest p-digit
+// decimal lies below x and misses, the decimal one unit above it in the last
+// place is tried as well.  The first p that succeeds gives the shortest
+// digits, and the nearest of them.  The layout rules then follow core's
+// float_to_general_debug: exponential form outside 1e-4 <= |x| < 1e16.
+static int orc_fmt_debug(double x, char* out) {
+  if (x != x) {
+    memcpy(out, "NaN", 3);
+    return 3;
+  }
+  int n = 0;
+  if (signbit(x)) out[n++] = '-';
+  const double a = fabs(x);
+  if (isinf(a)) {
+    memcpy(out + n, "inf", 3);
+    return n + 3;
+  }
+  if (a == 0.0) {
+    memcpy(out + n, "0.0", 3);
+    return n + 3;
+  }
+  char dig[40];
+  int nd = 0, e10 = 0;
+  for (int p = 1; p <= 17 && nd == 0; ++p) {
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.*e", p - 1, a);
+    // buf = d[.ddd]e(+|-)XX
+    char d[32];
+    int k = 0;
+    const char* c = buf;
+    for (; *c != 'e'; ++c)
+      if (*c != '.') d[k++] = *c;
+    const int e = atoi(c + 1);
+    const double back = strtod(buf, nullptr);
+    if (back == a) {
+      memcpy(dig, d, k);
+      nd = k;
+      e10 = e;
+    } else if (back < a) {
+      // d + one unit in the last place, with the carry
+      int j = k - 1;
+      while (j >= 0 && d[j] == '9') d[j--] = '0';
+      int ee = e;
+      if (j < 0) {
+        d[0] = '1';
+        for (int q = 1; q < k; ++q) d[q] = '0';
+        ee += 1;
+      } else {
+        d[j] += 1;
+      }
+      char b2[64];
+      int m = 0;
+      b2[m++] = d[0];
+      if (k > 1) {
+        b2[m++] = '.';
+        for (int q = 1; q < k; ++q) b2[m++] = d[q];
+      }
+      snprintf(b2 + m, sizeof b2 - m, "e%d", ee);
+      if (strtod(b2, nullptr) == a) {
+        memcpy(dig, d, k);
+        nd = k;
+        e10 = ee;
+      }
+    }
+  }
+  while (nd > 1 && dig[nd - 1] == '0') --nd;  // a carry can leave zeros
+  // value = d0.d1d2... * 10^e10
+  if (e10 < -4 || e10 >= 16) {
+    out[n++] = dig[0];
+    if (nd > 1) {
+      out[n++] = '.';
+      for (int q = 1; q < nd; ++q) out[n++] = dig[q];
+    }
+    n += snprintf(out + n, 8, "e%d", e10);
+  } else if (e10 < 0) {
+    out[n++] = '0';
+    out[n++] = '.';
+    for (int q = 0; q < -e10 - 1; ++q) out[n++] = '0';
+    for (int q = 0; q < nd; ++q) out[n++] = dig[q];
+  } else {
+    for (int q = 0; q <= e10; ++q) out[n++] = q < nd ? dig[q] : '0';
+    out[n++] = '.';
+    if (nd > e10 + 1) {
+      for (int q = e10 + 1; q < nd; ++q) out[n++] = dig[q];
+    } else {
+      out[n++] = '0';
+    }
+  }
+  return n;
+}
+
+// The points of trajectory i's Solution for a CSV source: their number, and
+// for t_eval output where they sit in t_eval.  TEvalSolout's first call, at
+// t_prev = t_curr = t0, produces index 0 if it equals t0 and passes over
+// every other point not beyond t0 (those before t0, and repeats of t0 after
+// index 0); every later point is produced in order
+// (src/solout/t_eval.rs:100-135).  So the points are index 0 when `head`,
+// then the run from index `run`.
+static void csv_points(const de_ode_batch* b, const de_ode_result* r, int source, int64_t i,
+                       int64_t* count, int* head, int64_t* run) {
+  *head = 0;
+  *run = 0;
+  if (source == DE_CSV_FINAL) {
+    *count = 1;
+  } else if (source == DE_CSV_EVENTS) {
+    // the points stored: at most max_events of them
+    *count = r->n_events ? std::min<int64_t>(r->n_events[i], b->max_events) : 0;
+  } else {
+    const double t0 = b->t0 ? b->t0[i] : b->t0_scalar;
+    const double tf = b->tf ? b->tf[i] : b->tf_scalar;
+    const bool fwd = tf > t0;
+    *head = b->n_eval > 0 && b->t_eval[0] == t0;
+    int64_t idx = 0;
+    while (idx < b->n_eval) {
+      const double te = b->t_eval[idx];
+      if ((fwd && te > t0) || (!fwd && te < t0)) break;
+      ++idx;
+    }
+    *run = idx;
+    *count = r->n_dense ? std::min<int64_t>(r->n_dense[i], *head + (b->n_eval - idx)) : 0;
+  }
+}
+
+static std::string csv_document(const de_ode_batch* b, const de_ode_result* r, int source,
+                                int64_t i) {
+  const int64_t n = b->n;
+  const int dim = b->dim;
+  int64_t count, run;
+  int head;
+  csv_points(b, r, source, i, &count, &head, &run);
+  std::string doc;
+  if (count <= 0) return doc;
+  doc += "t";
+  char buf[64];
+  for (int c = 0; c < dim; ++c) {
+    snprintf(buf, sizeof buf, ",y%d", c);
+    doc += buf;
+  }
+  doc += "\n";
+  for (int64_t p = 0; p < count; ++p) {
+    const int64_t k = source == DE_CSV_TEVAL ? (p < head ? 0 : run + p - head) : p;
+    double t;
+    double y[8];
+    if (source == DE_CSV_FINAL) {
+      t = r->t_final[i];
+      for (int c = 0; c < dim; ++c) y[c] = r->y_final[c * n + i];
+    } else if (source == DE_CSV_EVENTS) {
+      t = r->event_t[k * n + i];
+      for (int c = 0; c < dim; ++c) y[c] = r->event_y[(k * dim + c) * n + i];
+    } else {
+      t = b->t_eval[k];
+      for (int c = 0; c < dim; ++c) y[c] = r->dense[(k * dim + c) * n + i];
+    }
+    doc.append(buf, orc_fmt_debug(t, buf));
+    for (int c = 0; c < dim; ++c) {
+      doc += ",";
+      doc.append(buf, orc_fmt_debug(y[c], buf));
+    }
+    doc += "\n";
+  }
+  return doc;
+}
+
 extern "C" {
 
 // Oracle batched ODE solve over host memory.  pow_mode: 0 project pow,
@@ -1809,6 +1983,23 @@ void de_oracle_dop853_tableau(double* c, double* a, double* b, double* bh, doubl
 // The embedded family's tableau as the oracle uses it: dims = {S, I, O, fsal,
 // has_bi}; c[I], a[I*I], b[S], bh[S], bi[I*O] (row-major).  Returns 0, or -1
 // for a method id outside the family.
+// Rust's {:?} text of x (at most 24 bytes, no terminator); returns its length.
+int de_oracle_fmt_debug(double x, char* out) { return orc_fmt_debug(x, out); }
+
+// Solution::to_csv of every trajectory (host arrays): offsets[0 .. n]; the
+// documents into out when it is not null.  Returns 0, or -1 for a bad source.
+int de_oracle_ode_csv(const de_ode_batch* b, const de_ode_result* r, int32_t source,
+                      int64_t* offsets, char* out) {
+  if (source < DE_CSV_FINAL || source > DE_CSV_EVENTS) return -1;
+  offsets[0] = 0;
+  for (int64_t i = 0; i < b->n; ++i) {
+    const std::string doc = csv_document(b, r, source, i);
+    if (out) memcpy(out + offsets[i], doc.data(), doc.size());
+    offsets[i + 1] = offsets[i] + (int64_t)doc.size();
+  }
+  return 0;
+}
+
 int de_oracle_erk_tableau(int method, double* c, double* a, double* b, double* bh, double* bi,
                           int* dims) {
   switch (method) {

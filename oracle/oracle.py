@@ -53,6 +53,11 @@ def load():
         lib.de_oracle_philox.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         lib.de_oracle_dopri5_tableau.restype = None
         lib.de_oracle_erk_tableau.restype = C.c_int
+        lib.de_oracle_fmt_debug.restype = C.c_int
+        lib.de_oracle_fmt_debug.argtypes = [C.c_double, C.c_char_p]
+        lib.de_oracle_ode_csv.restype = C.c_int
+        lib.de_oracle_ode_csv.argtypes = [C.POINTER(_abi.OdeBatch), C.POINTER(_abi.OdeResult),
+                                          C.c_int32, C.c_void_p, C.c_void_p]
         lib.de_oracle_erk_tableau.argtypes = [C.c_int] + [C.c_void_p] * 6
         lib.de_oracle_dopri5_tableau.argtypes = [C.c_void_p] * 5
         _lib = lib
@@ -91,6 +96,26 @@ def erk_tableau(method):
                                  bh.ctypes.data, bi.ctypes.data, dims.ctypes.data)
     return dict(S=S, I=I, O=O, fsal=bool(dims[3]), has_bi=bool(dims[4]), C=c,
                 A=a.reshape(I, I), B=b, BH=bh, BI=bi.reshape(I, O))
+
+
+def fmt_debug(x):
+    """Rust's `{:?}` text of a double (str)."""
+    buf = C.create_string_buffer(32)
+    n = load().de_oracle_fmt_debug(float(x), buf)
+    return buf.raw[:n].decode()
+
+
+def ode_csv(ensemble, solution, source):
+    """Solution::to_csv of every trajectory of a solved ensemble: (bytes, offsets)."""
+    import numpy as np
+    b, res, keep = ensemble.batch()
+    r = solution.struct()
+    offsets = np.zeros(ensemble.n + 1, dtype=np.int64)
+    if load().de_oracle_ode_csv(C.byref(b), C.byref(r), source, offsets.ctypes.data, None) != 0:
+        raise ValueError("bad csv source %d" % source)
+    out = C.create_string_buffer(max(1, int(offsets[-1])))
+    load().de_oracle_ode_csv(C.byref(b), C.byref(r), source, offsets.ctypes.data, out)
+    return out.raw[:int(offsets[-1])], offsets
 
 
 def solve_ode(ensemble, pow_mode=POW_PROJECT, threads=1):

@@ -168,3 +168,24 @@ def test_noise_normal_matches_oracle(orc):
         for s in range(20):
             for c in range(8):
                 assert lib.de_noise_normal(9, p, s, c) == olib.de_oracle_noise_normal(9, p, s, c)
+
+
+def test_csv_entry_points_validate_then_refuse():
+    # The device CSV writer validates its descriptor (an unknown source is
+    # DE_ERR_INVALID) and is otherwise switched off (DE_ERR_UNSUPPORTED), with
+    # no device work in either case.
+    import ctypes as C
+    import numpy as np
+    import deamd
+    from deamd import _abi
+    ens = deamd.IVP.ode(deamd.Exponential(1.0), 0.0, 1.0, [1.0], n=1)
+    b, res, keep = ens.batch()
+    r = res.struct()
+    off = np.zeros(2, dtype=np.int64)
+    lib = deamd.lib()
+    assert lib.de_ode_csv_offsets(C.byref(b), C.byref(r), 7, off.ctypes.data, None) == \
+        _abi.DE_ERR_INVALID
+    assert lib.de_ode_csv_offsets(C.byref(b), C.byref(r), _abi.DE_CSV_FINAL, off.ctypes.data,
+                                  None) == _abi.DE_ERR_UNSUPPORTED
+    assert lib.de_ode_csv_write(C.byref(b), C.byref(r), _abi.DE_CSV_FINAL, off.ctypes.data,
+                                None, None) == _abi.DE_ERR_UNSUPPORTED
