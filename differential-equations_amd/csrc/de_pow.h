@@ -192,77 +192,129 @@ DE_HD DE_FORCE_INLINE double pow_dd(double x, double y) {
 
 DE_TABLE double kPow5InvTable[128 * 2] = DE_POW5_INV_TABLE;
 DE_TABLE double kPow5ATable[640 * 2] = DE_POW5_A_TABLE;
+DE_TABLE double kPowR6InvTable[128 * 2] = DE_POWR6_INV_TABLE;
+DE_TABLE double kPowR6ATable[6 * 128 * 2] = DE_POWR6_A_TABLE;
+DE_TABLE double kPowR7InvTable[128 * 2] = DE_POWR7_INV_TABLE;
+DE_TABLE double kPowR7ATable[7 * 128 * 2] = DE_POWR7_A_TABLE;
+DE_TABLE double kPowR8InvTable[128 * 2] = DE_POWR8_INV_TABLE;
+DE_TABLE double kPowR8ATable[8 * 128 * 2] = DE_POWR8_A_TABLE;
+DE_TABLE double kPowR9InvTable[128 * 2] = DE_POWR9_INV_TABLE;
+DE_TABLE double kPowR9ATable[9 * 128 * 2] = DE_POWR9_A_TABLE;
 
-// x^(-0.2) for positive normal x.  Returns false when the result could not be
-// certified (then *out is meaningless); for other x the return is false too.
+// The constants of the k-th-root path, as compile-time values (a static
+// member array would be a device global whose loads are not folded).
+#define DE_POW_ROOT(K, P, INV, A)                                                \
+  template <>                                                                    \
+  struct PowRoot<K> {                                                            \
+    static constexpr double c1_hi = P##_C1_HI, c1_lo = P##_C1_LO;                \
+    static constexpr double neg_delta = P##_NEG_DELTA;                           \
+    static constexpr double neg_delta_ln2 = P##_NEG_DELTA_LN2;                   \
+    DE_HD static constexpr double poly(int j) {                                  \
+      constexpr double c[8] = P##_POLY;                                          \
+      return c[j];                                                               \
+    }                                                                            \
+    DE_HD static const double* inv() { return INV; }                             \
+    DE_HD static const double* a() { return A; }                                 \
+  };
+template <int K>
+struct PowRoot;
+DE_POW_ROOT(5, DE_POW5, kPow5InvTable, kPow5ATable)
+DE_POW_ROOT(6, DE_POWR6, kPowR6InvTable, kPowR6ATable)
+DE_POW_ROOT(7, DE_POWR7, kPowR7InvTable, kPowR7ATable)
+DE_POW_ROOT(8, DE_POWR8, kPowR8InvTable, kPowR8ATable)
+DE_POW_ROOT(9, DE_POWR9, kPowR9InvTable, kPowR9ATable)
+#undef DE_POW_ROOT
+
+// x^D with D = -fl(1/K), K = 5 .. 9, for positive normal x.  Returns false
+// when the result could not be certified (then *out is meaningless); for
+// other x the return is false too.
 //
-// Let D = -0.2 = -(1/5 + delta), delta = 2^-54/5.  With x = 2^E m, m in [1,2),
-// E = 5q + s (0 <= s < 5), r = m*invc_i - 1 (exact, |r| < 2^-7.48):
-//   x^D = 2^-q * [2^(-s/5) invc_i^(1/5)] * (1+r)^(-1/5) * exp(-delta log x)
-// The bracket is the table entry A = ah + al.  (1+r)^(-1/5) is its Taylor
-// series to r^9 (truncation < 2^-79), kept as 1 + p1 + ulo with p1 + e1 = -r/5
-// exact to 2^-118.  corr = -delta log x = -delta (E ln2 - log invc_i + r) is at
-// most 2^-46.8 and is needed to 2^-70 only (the dropped -delta r^2/2 is below
-// 2^-71.3).  Every rounding after the exact steps is at most 2^-70 relative to
-// the result and there are fewer than six of them, so s + w below is within
-// eps < 2^-67.4 (relative) of x^D.  Rounding test: res = RN(s + w) with
-// residual d = s + w - res (exact, Fast2Sum); if RN(res + d*(1 + 2^-12)) ==
-// res then |x^D - res| < ulp(res)/2 (ulp/4 below a power of two), because
-// 2^54 eps < 1 - 1/(1 + 2^-12).  The test fails for about 2^-12 of all x.
-DE_HD bool pow_m02_fast(double x, double* out) {
+// Let D = -(1/K + delta), delta = fl(1/K) - 1/K (|delta| <= 2^-55; 0 for
+// K = 8).  With x = 2^E m, m in [1,2), E = Kq + s (0 <= s < K),
+// r = m*invc_i - 1 (exact, |r| < 2^-7.48):
+//   x^D = 2^-q * [2^(-s/K) invc_i^(1/K)] * (1+r)^(-1/K) * exp(-delta log x)
+// The bracket is the table entry A = ah + al.  (1+r)^(-1/K) is its Taylor
+// series to r^9 (truncation < 2^-78), kept as 1 + p1 + ulo with p1 + e1 =
+// -r/K exact to 2^-118.  corr = -delta log x = -delta (E ln2 - log invc_i + r)
+// is at most 2^-46.5 and is needed to 2^-70 only (the dropped -delta r^2/2 is
+// below 2^-71).  Every rounding after the exact steps is at most 2^-70
+// relative to the result and there are fewer than six of them, so s + w
+// below is within eps < 2^-67.4 (relative) of x^D.  Rounding test:
+// res = RN(s + w) with residual d = s + w - res (exact, Fast2Sum); if
+// RN(res + d*(1 + 2^-12)) == res then |x^D - res| < ulp(res)/2 (ulp/4 below a
+// power of two), because 2^54 eps < 1 - 1/(1 + 2^-12).  The test fails for
+// about 2^-12 of all x.
+template <int K>
+DE_HD bool pow_root_fast(double x, double* out) {
   DE_NO_CONTRACT
+  using T = PowRoot<K>;
+  // E + K*Q0 = u with u >= 0 for every normal x; the result's scale 2^-q is
+  // 2^(Q0 - floor(u/K)).
+  constexpr uint32_t kQ0 = (1025u + K - 1u) / K;
+  constexpr uint32_t kOff = K * kQ0 - 1023u;
   const uint64_t ix = as_u64(x);
   const uint32_t hx = (uint32_t)(ix >> 32);
   const uint32_t i = (hx >> 13) & 127u;
-  const uint32_t u = ((hx >> 20) & 0x7ffu) + 2u;  // E + 1025 = 5 (q + 205) + s
-  const uint32_t fq = (u * 52429u) >> 18;         // floor(u / 5), exact for u < 81920
-  const uint32_t s = u - 5u * fq;
-  const uint32_t row = s * 128u + i;              // < 640 for every x
+  const uint32_t u = ((hx >> 20) & 0x7ffu) + kOff;
+  const uint32_t fq = u / (uint32_t)K;
+  const uint32_t s = u - (uint32_t)K * fq;
+  const uint32_t row = s * 128u + i;  // < 128 K for every x
   const double m = as_f64((ix & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
-  const double invc = kPow5InvTable[2 * i];
-  const double dlinv = kPow5InvTable[2 * i + 1];  // delta log(invc_i)
-  const double ah = kPow5ATable[2 * row];
-  const double al = kPow5ATable[2 * row + 1];
+  const double invc = T::inv()[2 * i];
+  const double dlinv = T::inv()[2 * i + 1];  // delta log(invc_i)
+  const double ah = T::a()[2 * row];
+  const double al = T::a()[2 * row + 1];
 
   const double r = fma(m, invc, -1.0);
-  const double e = (double)((int32_t)u - 1025);
-  const double corr = fma(DE_POW5_NEG_DELTA, r, fma(e, DE_POW5_NEG_DELTA_LN2, dlinv));
+  const double e = (double)((int32_t)u - (int32_t)(K * kQ0));
+  const double corr = fma(T::neg_delta, r, fma(e, T::neg_delta_ln2, dlinv));
 
-  const double c[8] = DE_POW5_POLY;  // binom(-1/5, k), k = 9 .. 2
-  double t = c[0];
-  t = fma(r, t, c[1]);
-  t = fma(r, t, c[2]);
-  t = fma(r, t, c[3]);
-  t = fma(r, t, c[4]);
-  t = fma(r, t, c[5]);
-  t = fma(r, t, c[6]);
-  t = fma(r, t, c[7]);
-  const double p1 = r * DE_POW5_C1_HI;
-  const double e1 = fma(r, DE_POW5_C1_HI, -p1);
-  const double v = fma(r, t, DE_POW5_C1_LO);
-  const double ulo = fma(r, v, e1);               // (1+r)^(-1/5) = 1 + p1 + ulo
+  double t = T::poly(0);  // binom(-1/K, j), j = 9 .. 2
+  t = fma(r, t, T::poly(1));
+  t = fma(r, t, T::poly(2));
+  t = fma(r, t, T::poly(3));
+  t = fma(r, t, T::poly(4));
+  t = fma(r, t, T::poly(5));
+  t = fma(r, t, T::poly(6));
+  t = fma(r, t, T::poly(7));
+  const double p1 = r * T::c1_hi;
+  const double e1 = fma(r, T::c1_hi, -p1);
+  const double v = fma(r, t, T::c1_lo);
+  const double ulo = fma(r, v, e1);                    // (1+r)^(-1/K) = 1 + p1 + ulo
   const double lt = ulo + fma(corr, p1 + ulo, corr);  // (..)(1+corr) = 1 + p1 + lt
   const double alt = fma(al, p1 + lt, al);
   const double m1 = ah * p1;
   const double m1e = fma(ah, p1, -m1);
   const double lo = fma(ah, lt, m1e + alt);
-  const double sh = ah + m1;                      // |ah| > |m1|
+  const double sh = ah + m1;  // |ah| > |m1|
   const double st = m1 - (sh - ah);
   const double w = st + lo;
   const double res = sh + w;
   const double d = w - (res - sh);
   const bool normal = (hx - 0x00100000u) < 0x7fe00000u;  // 0 < x < inf, not subnormal
-  *out = res * as_f64((uint64_t)(1228u - fq) << 52);     // 2^-q, q in [-205, 204]
+  *out = res * as_f64((uint64_t)(1023u + kQ0 - fq) << 52);
   return normal && fma(d, 0x1.001p+0, res) == res;
 }
 
-// The double-double path is forced inline (above): as a call inside the step
-// loop it would save and restore live registers through scratch memory.
+// x^(-0.2) (the fifth-root path).
+DE_HD bool pow_m02_fast(double x, double* out) { return pow_root_fast<5>(x, out); }
+
+// The controllers' exponents -fl(1/k), k = 5 .. 9, try the k-th-root path
+// first.  The double-double path is forced inline (above): as a call inside
+// the step loop it would save and restore live registers through scratch
+// memory.
 DE_HD double pow(double x, double y) {
+  double r;
   if (y == -0.2) {
-    double r;
-    if (__builtin_expect(pow_m02_fast(x, &r), 1)) return r;
-    return pow_dd(x, y);
+    if (__builtin_expect(pow_root_fast<5>(x, &r), 1)) return r;
+  } else if (y == -(1.0 / 6.0)) {
+    if (__builtin_expect(pow_root_fast<6>(x, &r), 1)) return r;
+  } else if (y == -(1.0 / 7.0)) {
+    if (__builtin_expect(pow_root_fast<7>(x, &r), 1)) return r;
+  } else if (y == -0.125) {
+    if (__builtin_expect(pow_root_fast<8>(x, &r), 1)) return r;
+  } else if (y == -(1.0 / 9.0)) {
+    if (__builtin_expect(pow_root_fast<9>(x, &r), 1)) return r;
   }
   return pow_dd(x, y);
 }

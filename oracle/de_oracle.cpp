@@ -2029,6 +2029,22 @@ void de_oracle_set_generic_dopri5(int on) { orc::g_generic_dopri5 = on; }
 
 // The first try of pow(x, -0.2) alone: ok[i] = 1 when its rounding test
 // certified out[i], else 0 (and pow falls back to the double-double path).
+// The k-th-root path of pow(x, -fl(1/k)), k = 5 .. 9: value and certification.
+void de_oracle_pow_root_fast(int k, const double* x, double* out, uint8_t* ok, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    bool c = false;
+    switch (k) {
+      case 5: c = de::pow_root_fast<5>(x[i], &out[i]); break;
+      case 6: c = de::pow_root_fast<6>(x[i], &out[i]); break;
+      case 7: c = de::pow_root_fast<7>(x[i], &out[i]); break;
+      case 8: c = de::pow_root_fast<8>(x[i], &out[i]); break;
+      case 9: c = de::pow_root_fast<9>(x[i], &out[i]); break;
+      default: out[i] = 0.0; break;
+    }
+    ok[i] = c ? 1 : 0;
+  }
+}
+
 void de_oracle_pow_m02_fast(const double* x, double* out, uint8_t* ok, int64_t n) {
   for (int64_t i = 0; i < n; ++i) ok[i] = de::pow_m02_fast(x[i], &out[i]) ? 1 : 0;
 }

@@ -45,6 +45,9 @@ def load():
         lib.de_oracle_dop853_tableau.argtypes = [C.c_void_p] * 6
         lib.de_oracle_set_generic_dopri5.restype = None
         lib.de_oracle_set_generic_dopri5.argtypes = [C.c_int]
+        lib.de_oracle_pow_root_fast.restype = None
+        lib.de_oracle_pow_root_fast.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_int64]
         lib.de_oracle_pow_m02_fast.restype = None
         lib.de_oracle_pow_m02_fast.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         lib.de_oracle_noise_normal.restype = C.c_double
@@ -147,6 +150,16 @@ def pow_many(x, y):
 def set_generic_dopri5(on):
     """Run DOPRI5 through the generic Dormand-Prince restatement (test switch)."""
     load().de_oracle_set_generic_dopri5(1 if on else 0)
+
+
+def pow_root_fast(k, x):
+    """The k-th-root path of pow(x, -fl(1/k)): (values, certified) arrays."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    ok = np.zeros(x.size, dtype=np.uint8)
+    load().de_oracle_pow_root_fast(k, x.ctypes.data, out.ctypes.data, ok.ctypes.data, x.size)
+    return out, ok.astype(bool)
 
 
 def pow_m02_fast(x):

@@ -47,7 +47,7 @@ def test_device_pow_matches_host(orc):
     n = 1 << 20
     x = np.concatenate([np.exp(rng.uniform(np.log(1e-8), np.log(1e4), n)),
                         np.exp(rng.uniform(np.log(1e-300), np.log(1e300), n // 4))])
-    for y in (-0.2, 0.2):
+    for y in (-0.2, 0.2, -1.0 / 6.0, -1.0 / 7.0, -0.125, -1.0 / 9.0):
         yy = np.full_like(x, y)
         out = np.empty_like(x)
         deamd._abi.check(lib.de_diag_pow(deamd._ptr(x), deamd._ptr(yy), deamd._ptr(out), len(x)))
