@@ -62,7 +62,7 @@ struct Staging {
   std::vector<void*> bufs;
   hipError_t err = hipSuccess;
   ~Staging() {
-    for (void* p : bufs) hipFree(p);
+    for (void* p : bufs) (void)hipFree(p);  // teardown: a failure here has no one to report to
   }
   template <class T>
   T* in(const T* host, size_t count) {
@@ -138,8 +138,6 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->event_solout != DE_SOLOUT_DEFAULT) {
     if (b->family == DE_FAMILY_SPRING_CHAIN)
       return fail(DE_ERR_UNSUPPORTED, "event solouts are built for fixed-dimension families");
-    if (b->method >= DE_METHOD_RKF45)
-      return fail(DE_ERR_UNSUPPORTED, "event solouts are built for dopri5, dop853, rk4 and euler");
     if (b->event_solout != DE_SOLOUT_EVENT && b->n_eval > 0)
       return fail(DE_ERR_INVALID, "a crossing solout replaces the t_eval output (n_eval > 0)");
     if (b->event_direction < DE_CROSS_BOTH || b->event_direction > DE_CROSS_NEGATIVE)
@@ -217,8 +215,31 @@ de::OdeArgs make_args(const de_ode_batch* b, const de_ode_result* o) {
 
 // The embedded family (parity kernels, with or without DE_FLAG_FAST_MATH):
 // two tableaux per translation unit.
-int launch_ode_erk(int family, int method, const de::OdeArgs& a, int sched, hipStream_t s) {
-  switch ((method - DE_METHOD_RKF45) / 2) {
+int launch_ode_erk(int family, int method, int solout, const de::OdeArgs& a, int sched,
+                   hipStream_t s) {
+  using namespace de::parity;
+  const int g = (method - DE_METHOD_RKF45) / 2;
+  if (solout == DE_SOLOUT_EVENT) {
+    switch (g) {
+      case 0: return launch_ode_erk0_ev1(family, method, a, sched, s);
+      case 1: return launch_ode_erk1_ev1(family, method, a, sched, s);
+      case 2: return launch_ode_erk2_ev1(family, method, a, sched, s);
+      case 3: return launch_ode_erk3_ev1(family, method, a, sched, s);
+      case 4: return launch_ode_erk4_ev1(family, method, a, sched, s);
+      default: return 1;
+    }
+  }
+  if (solout != DE_SOLOUT_DEFAULT) {
+    switch (g) {
+      case 0: return launch_ode_erk0_ev2(family, method, a, sched, s);
+      case 1: return launch_ode_erk1_ev2(family, method, a, sched, s);
+      case 2: return launch_ode_erk2_ev2(family, method, a, sched, s);
+      case 3: return launch_ode_erk3_ev2(family, method, a, sched, s);
+      case 4: return launch_ode_erk4_ev2(family, method, a, sched, s);
+      default: return 1;
+    }
+  }
+  switch (g) {
     case 0: return de::parity::launch_ode_erk0(family, method, a, sched, s);
     case 1: return de::parity::launch_ode_erk1(family, method, a, sched, s);
     case 2: return de::parity::launch_ode_erk2(family, method, a, sched, s);
@@ -256,8 +277,8 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
   // n_eval rows, with NaN for the points it does not produce.
   // Event solouts run the parity kernels, with or without DE_FLAG_FAST_MATH.
   int r;
-  if (b->method >= DE_METHOD_RKF45 && b->event_solout == DE_SOLOUT_DEFAULT)
-    r = launch_ode_erk(b->family, b->method, a, sched, s);
+  if (b->method >= DE_METHOD_RKF45)
+    r = launch_ode_erk(b->family, b->method, b->event_solout, a, sched, s);
   else if (b->event_solout == DE_SOLOUT_EVENT)
     r = de::parity::launch_ode_event(b->family, b->method, a, sched, s);
   else if (b->event_solout != DE_SOLOUT_DEFAULT)

@@ -58,18 +58,24 @@ DE_HD constexpr uint32_t erk_zero_stages(int rows, bool dense) {
   return m;
 }
 
-template <class F, class Tab, bool DENSE, bool VTOL>
+// EV: 0 none, 1 EventWrappedSolout, 2 crossing / hyperplane solouts (de_event.h).
+template <class F, class Tab, bool DENSE, bool VTOL, int EV = 0>
 struct ErkLane {
   static constexpr int D = F::kDim;
   static constexpr int S = Tab::S, I = Tab::I, O = Tab::O;
   static constexpr bool kBi = Tab::kBi, kFsal = Tab::kFsal;
-  static constexpr int kEv = 0;
-  static constexpr int kEntryWidth = 0;
-  static constexpr int KN = (DENSE && kBi) ? I : S;  // stages held in registers
-  static constexpr uint32_t kZeroStages = erk_zero_stages<Tab>(KN, DENSE && kBi);
+  static constexpr int kEv = EV;
+  // The interpolant is built when a t_eval point or an event bracket needs it.
+  static constexpr bool kInterp = DENSE || EV != 0;
+  static constexpr int KN = (kInterp && kBi) ? I : S;  // stages held in registers
+  static constexpr uint32_t kZeroStages = erk_zero_stages<Tab>(KN, kInterp && kBi);
+  // Deferred event bracket: the meta fields (kEntryH holds h), then y_prev and
+  // the I stages (Horner interpolant), or y_prev, dydt_prev, y, dydt (cubic
+  // Hermite).
+  static constexpr int kEntryWidth = EV == 0 ? 0 : kEntryMeta + (kBi ? (1 + I) * D : 4 * D);
   // Occupancy hint of the persistent kernel: three waves per SIMD (168 VGPRs);
-  // two (256 VGPRs) for the dense lanes of more than 36 doubles of stages,
-  // which spill at three.
+  // two (256 VGPRs) for the lanes of more than 36 doubles of interpolation
+  // stages, which spill at three.
   static constexpr int kWavesPerEu = (KN == S || KN * D <= 36) ? 3 : 2;
   // Evaluations per accepted step beyond the S - 1 of every attempt
   // (ordinary.rs:151-181): f(t_new, y_new) unless FSAL, the dense stages.
@@ -80,10 +86,17 @@ struct ErkLane {
   double t, h, h_prev, tf, dir;
   double tail;
   LaneTol<D, VTOL> tol;
+  LaneEvent<D, EV> ev;
   int32_t acc, rej, base_ev, stiff, next_eval, pushed;
   bool rejected_last;
 
   DE_COLD static void write_back(const OdeArgs& a, int64_t i, int32_t st, ErkLane L) {
+    if constexpr (EV != 0) {
+      if (st == kDeferOverflow) {
+        event_defer_finish(a.ev, i, 0, true);
+        return;
+      }
+    }
 #pragma unroll
     for (int c = 0; c < D; ++c) a.y_final[c * a.n + i] = L.y[c];
     a.t_final[i] = L.t;
@@ -95,8 +108,47 @@ struct ErkLane {
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
+    if constexpr (EV != 0) {
+      if (a.ev.brk)
+        event_defer_finish(a.ev, i, L.ev.k, false);
+      else
+        L.ev.finish(a.ev, a.n, i);
+    }
   }
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
+
+  // The interpolant of a deferred entry, as the step's.
+  DE_HD static void entry_bounds(const double* ent, double* t0, double* t1) {
+    *t0 = ent[kEntryT0];
+    *t1 = ent[kEntryT0] + ent[kEntryH];
+  }
+  DE_HD static bool entry_interp(const double* ent, double te, double* out) {
+    const double tp = ent[kEntryT0], hp = ent[kEntryH];
+    const double t_new = tp + hp;
+    if (te < tp || te > t_new) return false;
+    const double* d = ent + kEntryMeta;
+    if constexpr (kBi) {
+      const double(*k)[D] = reinterpret_cast<const double(*)[D]>(d + D);
+      horner_interp(d, k, tp, hp, te, stages_finite(k), out);
+    } else {
+      hermite_interp(d, d + D, d + 2 * D, d + 3 * D, tp, t_new, te, out);
+    }
+    return true;
+  }
+  DE_HD static bool teval_base(const OdeArgs& a) { return EV == 1 && DENSE && a.n_eval > 0; }
+  DE_HD static void entry_root(const OdeArgs& a, int64_t i, const double* ent, double* res) {
+    double loc[kEntryWidth];
+#pragma unroll
+    for (int f = 0; f < kEntryWidth; ++f) loc[f] = ent[f];
+    auto interp = [&](double te, double* out) { return entry_interp(loc, te, out); };
+    double t0, t1;
+    entry_bounds(loc, &t0, &t1);
+    LaneEvent<D, EV>::root(a.ev, i, loc, res, t0, t1, signum(loc[kEntryH]), teval_base(a),
+                           interp);
+  }
+  DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
+    return LaneEvent<D, EV>::book(a.ev, a.n, i, teval_base(a));
+  }
 
   DE_COLD static ErkLane start(const OdeArgs& a, int64_t i, bool* ok) {
     ErkLane L;
@@ -159,9 +211,15 @@ struct ErkLane {
     stiff = 0;
     rejected_last = false;
     tail = as_f64(kNaNBits);
-    if (DENSE && a.n_eval > 0)
+    if (DENSE && EV != 2 && a.n_eval > 0)
       teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
                     [](double, double*) { return false; }, tail);
+    else
+      tail = t;
+    if constexpr (EV != 0) {
+      ev.reset();
+      ev.start(a.ev, i, t, y);
+    }
     return true;
   }
 
@@ -274,6 +332,55 @@ struct ErkLane {
     (interp_row<Rr>(k, s, hp, all, v), ...);
   }
 
+  // Whether every stage that sits in an all-zero interpolation row is finite
+  // (then those rows add +-0 and are skipped).
+  DE_HD static bool stages_finite(const double (*k)[D]) {
+    constexpr uint32_t zr = erk_zero_bi_rows<Tab>();
+    bool fin = true;
+#pragma unroll
+    for (int r = 0; r < I; ++r)
+      if ((zr >> r) & 1u) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) fin = fin && isfinite(k[r][c]);
+      }
+    return fin;
+  }
+
+  // ordinary.rs:252-277: Horner over bi[r][0..O) times s, then
+  // y_prev + sum_r (cont_r h_prev) k[r] in r order.  Rows that are zero in
+  // every power add +-0 when their stage, s and h_prev are finite and are
+  // skipped then (`fin`: those stages are finite).
+  DE_HD static void horner_interp(const double* yp, const double (*k)[D], double tp, double hp,
+                                  double te, bool fin, double* out) {
+    const double s = (te - tp) / hp;
+    const bool all = !fin || !isfinite(s) || !isfinite(hp);
+    double v[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) v[c] = yp[c];
+    interp_rows(std::make_integer_sequence<int, I>{}, k, s, hp, all, v);
+#pragma unroll
+    for (int c = 0; c < D; ++c) out[c] = v[c];
+  }
+
+  // cubic_hermite_interpolate(t_prev, t, y_prev, y, dydt_prev, dydt)
+  // (src/interpolate.rs:40-60), a linear combination from zero.
+  DE_HD static void hermite_interp(const double* yp, const double* dp, const double* yc,
+                                   const double* dc, double tp, double tc, double te,
+                                   double* out) {
+    const double hh = tc - tp;
+    const double s = (te - tp) / hh;
+    const double s2 = s * s;
+    const double s3 = s2 * s;
+    const double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+    const double h10 = s3 - 2.0 * s2 + s;
+    const double h01 = -2.0 * s3 + 3.0 * s2;
+    const double h11 = s3 - s2;
+    const double w10 = h10 * hh, w11 = h11 * hh;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      out[c] = (((0.0 + h00 * yp[c]) + w10 * dp[c]) + h01 * yc[c]) + w11 * dc[c];
+  }
+
   // One pass of the driver loop (solve_ivp.rs:190-263 around ordinary.rs:63-210).
   DE_HD int32_t step(const OdeArgs& a, int64_t i) {
     {
@@ -320,57 +427,73 @@ struct ErkLane {
       f.diff(t_new, yh, dn);
     }
     int32_t sol = -1;
-    if constexpr (DENSE) {
-      // Interpolation is needed only when the next t_eval point is not past
-      // the new time (a NaN point is handed to teval_emit as well).
+    if constexpr (kInterp) {
+      // The interpolant is built when the next t_eval point is not past the
+      // new time (a NaN point is handed to teval_emit as well) or an event
+      // is bracketed.
+      const bool teval_out = DENSE && EV != 2 && a.n_eval > 0;
       bool tev = false;
-      if (accept && next_eval < a.n_eval) {
+      if (teval_out && accept && next_eval < a.n_eval) {
         const double te = a.t_eval[next_eval];
         tev = !(dir > 0.0 ? te > t_new : te < t_new);
       }
-      if (tev) {
+      // Without a t_eval output the Solution's last point is the step's end
+      // (DefaultSolout pushes it before an event solout looks at the step).
+      if (!teval_out) tail = accept ? t_new : tail;
+      bool need = false;
+      if constexpr (EV != 0) need = accept && ev.pre(a.ev, t_new, yh);
+      if (tev || need) {
         if constexpr (kBi) stages<S, I>(k, y, t, h, bad);
         const double tp = t, hp = h;
+        bool fin = true;
+        if constexpr (kBi) {
+          uint32_t anyb = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) anyb |= bad[c];
+          fin = (anyb & erk_zero_bi_rows<Tab>()) == 0;
+        }
         auto interp = [&](double te, double* out) -> bool {
           if (te < tp || te > t_new) return false;
-          if constexpr (kBi) {
-            // ordinary.rs:252-277: Horner over bi[r][0..O) times s, then
-            // y_prev + sum_r (cont_r h_prev) k[r] in r order; rows that are zero
-            // in every power add +-0, or NaN where k[r] or the weight is not
-            // finite.
-            const double s = (te - tp) / hp;
-            constexpr uint32_t zr = erk_zero_bi_rows<Tab>();
-            uint32_t anyb = 0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) anyb |= bad[c];
-            const bool all = (anyb & zr) != 0 || !isfinite(s) || !isfinite(hp);
-            double v[D];
-#pragma unroll
-            for (int c = 0; c < D; ++c) v[c] = y[c];
-            interp_rows(std::make_integer_sequence<int, I>{}, k, s, hp, all, v);
-#pragma unroll
-            for (int c = 0; c < D; ++c) out[c] = v[c];
-          } else {
-            // cubic_hermite_interpolate(t_prev, t, y_prev, y, dydt_prev, dydt)
-            // (src/interpolate.rs:40-60), a linear combination from zero.
-            const double hh = t_new - tp;
-            const double s = (te - tp) / hh;
-            const double s2 = s * s;
-            const double s3 = s2 * s;
-            const double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
-            const double h10 = s3 - 2.0 * s2 + s;
-            const double h01 = -2.0 * s3 + 3.0 * s2;
-            const double h11 = s3 - s2;
-            const double w10 = h10 * hh, w11 = h11 * hh;
-#pragma unroll
-            for (int c = 0; c < D; ++c)
-              out[c] = (((0.0 + h00 * y[c]) + w10 * k[0][c]) + h01 * yh[c]) + w11 * dn[c];
-          }
+          if constexpr (kBi)
+            horner_interp(y, k, tp, hp, te, fin, out);
+          else
+            hermite_interp(y, k[0], yh, dn, tp, t_new, te, out);
           return true;
         };
-        if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, yh, interp, tail))
+        if (tev && !teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, yh, interp, tail))
           sol = kOutOfBounds;
+        if constexpr (EV != 0) {
+          if (need && sol < 0) {
+            if (a.ev.brk) {
+              bool overflow = false;
+              double* ent = ev.defer(a.ev, i, true, tp, hp, tail, pushed, &overflow);
+              if (ent) {
+                double* dst = ent + kEntryMeta;
+                if constexpr (kBi) {
+#pragma unroll
+                  for (int c = 0; c < D; ++c) dst[c] = y[c];
+#pragma unroll
+                  for (int r = 0; r < I; ++r)
+#pragma unroll
+                    for (int c = 0; c < D; ++c) dst[D + r * D + c] = k[r][c];
+                } else {
+#pragma unroll
+                  for (int c = 0; c < D; ++c) {
+                    dst[c] = y[c];
+                    dst[D + c] = k[0][c];
+                    dst[2 * D + c] = yh[c];
+                    dst[3 * D + c] = dn[c];
+                  }
+                }
+              }
+              if (overflow) sol = kDeferOverflow;
+            } else {
+              sol = ev.post(a.ev, a.n, i, dir, true, t_new, tp, tail, interp);
+            }
+          }
+        }
       }
+      if constexpr (EV != 0) ev.commit(accept);
     }
     h_prev = accept ? h : h_prev;
     t = accept ? t_new : t;

@@ -81,6 +81,14 @@ int launch_ode_erk1(int family, int method, const OdeArgs& a, int schedule, hipS
 int launch_ode_erk2(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 int launch_ode_erk3(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
 int launch_ode_erk4(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);
+// The embedded family with an event solout: group g, kind 1 (event) / 2 (crossings).
+#define DE_ERK_EV_DECL(g, k)                                                          \
+  int launch_ode_erk##g##_ev##k(int family, int method, const OdeArgs& a, int schedule, \
+                                hipStream_t s);
+DE_ERK_EV_DECL(0, 1) DE_ERK_EV_DECL(1, 1) DE_ERK_EV_DECL(2, 1) DE_ERK_EV_DECL(3, 1)
+DE_ERK_EV_DECL(4, 1) DE_ERK_EV_DECL(0, 2) DE_ERK_EV_DECL(1, 2) DE_ERK_EV_DECL(2, 2)
+DE_ERK_EV_DECL(3, 2) DE_ERK_EV_DECL(4, 2)
+#undef DE_ERK_EV_DECL
 }  // namespace parity
 namespace fast {
 int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);

@@ -259,11 +259,10 @@ int launch_csv(const CsvArgs& a, int64_t* offsets, char* out, int64_t cap, hipSt
     hipLaunchKernelGGL(csv_base_kernel, dim3(1), dim3(1), 0, s, nslots, (const int64_t*)pos, base);
     e = hipGetLastError();
   }
-  hipFreeAsync(cnt_head_run, s);
-  hipFreeAsync(len, s);
-  hipFreeAsync(pos, s);
-  hipFreeAsync(base, s);
-  hipFreeAsync(scan_tmp, s);
+  for (void* p : {(void*)cnt_head_run, (void*)len, (void*)pos, (void*)base, scan_tmp}) {
+    const hipError_t f = hipFreeAsync(p, s);
+    if (e == hipSuccess) e = f;
+  }
   return e == hipSuccess ? 0 : -1;
 }
 

@@ -1,7 +1,8 @@
 """GPU parity of the embedded explicit Runge-Kutta family (rkf45, cash_karp,
 rkv655e .. rkv989e): the HIP lanes against the CPU oracle, equal as IEEE
 values (final states and times, dense output, step and evaluation counts,
-statuses) for every tableau, schedule, tolerance form and family.
+statuses, event points) for every tableau, schedule, tolerance form, family
+and event solout.
 """
 import os
 
@@ -11,7 +12,8 @@ import pytest
 import deamd
 from deamd import (DE_SCHED_PERSISTENT, DE_SCHED_PERSISTENT_FIFO, DE_SCHED_PLAIN,
                    DE_STATUS_BAD_INPUT, DE_STATUS_COMPLETE, DE_STATUS_OUT_OF_BOUNDS,
-                   DE_STATUS_STIFFNESS, Event, ExplicitRungeKutta, Exponential, Harmonic, IVP,
+                   DE_STATUS_INTERRUPTED, DE_STATUS_STIFFNESS, CrossingDirection, Event,
+                   EventConfig, ExplicitRungeKutta, Exponential, Harmonic, IVP,
                    Linear, Logistic, Lorenz, Robertson, VanDerPol)
 
 pytestmark = pytest.mark.gpu
@@ -161,9 +163,86 @@ def test_backward_and_bad_input(orc):
         assert_equal(got, orc.solve_ode(ens, threads=THREADS))
 
 
-def test_event_solouts_are_refused():
-    p, y0 = deamd.gen_lorenz(16, seed=99)
-    ens = IVP.ode(Lorenz(*p), 0.0, 1.0, y0).method(method("rkv655e")) \
-        .event(Event.component(2, 20.0))
-    with pytest.raises(RuntimeError):
-        ens.solve()
+def assert_events_equal(got, want, dense=False):
+    assert_equal(got, want, dense)
+    assert same(got.n_events, want.n_events)
+    assert same(got.event_t, want.event_t)
+    assert same(got.event_y, want.event_y)
+
+
+@pytest.mark.parametrize("name", FAMILY)
+def test_lorenz_section_events(orc, name):
+    # z = rho - 1 per trajectory, both directions, EventWrappedSolout around
+    # the final-state output; roots from the dense output of each tableau
+    # (Horner over the bi rows, or the cubic Hermite of rkf45 / cash_karp).
+    p, y0 = deamd.gen_lorenz(2048, seed=41)
+    ens = IVP.ode(Lorenz(*p), 0.0, 6.0, y0).method(method(name)) \
+        .event(Event.component(2, p[1] - 1.0)).options(max_events=24)
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.status == DE_STATUS_COMPLETE).mean() > 0.9 and want.n_events.max() > 0
+    assert_events_equal(ens.solve(), want)
+
+
+@pytest.mark.parametrize("name", ["rkf45", "rkv656e", "rkv878e", "rkv989e"])
+@pytest.mark.parametrize("kind", ["crossing", "hyperplane"])
+def test_crossing_and_hyperplane_solouts(orc, name, kind):
+    p, y0 = deamd.gen_lorenz(2048, seed=47)
+    ens = IVP.ode(Lorenz(*p), 0.0, 5.0, y0).method(method(name)).options(max_events=16)
+    if kind == "crossing":
+        ens = ens.crossing(0, 0.0, CrossingDirection.Both)
+    else:
+        ens = ens.hyperplane_crossing([0.0, 0.0, 25.0], [1.0, 1.0, 3.0],
+                                      CrossingDirection.Positive)
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert want.n_events.sum() > 0
+    assert_events_equal(ens.solve(), want)
+
+
+@pytest.mark.parametrize("name", ["cash_karp", "rkv655e", "rkv767e", "rkv988e"])
+def test_terminal_plane_event_with_t_eval(orc, name):
+    # terminate_after(3) around a t_eval output: dense rows, event points and
+    # the Interrupted statuses (the last dense row is the event's time).
+    p, y0 = deamd.gen_lorenz(4096, seed=43)
+    ens = IVP.ode(Lorenz(*p), 0.0, 8.0, y0).t_eval(np.linspace(0.0, 8.0, 17)) \
+        .method(method(name)).event(Event.plane([0.0, 0.0, 20.0], [0.2, -0.1, 1.0],
+                                                EventConfig().terminate_after(3))) \
+        .options(schedule=DE_SCHED_PERSISTENT, max_events=4)
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.status == DE_STATUS_INTERRUPTED).sum() > 100
+    assert_events_equal(ens.solve(), want, dense=True)
+
+
+@pytest.mark.parametrize("name", ["rkf45", "rkv655e", "rkv877e"])
+def test_deferred_regions_overflow_to_inline_solves(orc, name):
+    # Oscillators of widely spread frequencies: 2 to ~250 zero crossings, so
+    # some trajectories outgrow their deferred-root region and are solved
+    # again with inline roots; the results must not show which.
+    rng = np.random.default_rng(71)
+    n = 4096
+    k = np.exp(rng.uniform(np.log(0.5), np.log(400.0), n))
+    y0 = np.vstack([np.ones(n), rng.uniform(-1, 1, n)])
+    for cap in (0, 8):
+        ens = IVP.ode(Harmonic(k), 0.0, 40.0, y0).method(method(name, 1e-7, 1e-9)) \
+            .event(Event.component(0, 0.0)).options(max_events=cap)
+        want = orc.solve_ode(ens, threads=THREADS)
+        assert (want.n_events > 40 + cap).any() and (want.n_events < 32).any()
+        assert_events_equal(ens.solve(), want)
+
+
+def test_small_family_time_events_and_backward_solves(orc):
+    rng = np.random.default_rng(53)
+    n = 1024
+    cases = [
+        IVP.ode(Exponential(rng.uniform(0.1, 1.0, n)), 0.0, 10.0,
+                rng.uniform(0.5, 2.0, n).reshape(1, -1))
+        .event(Event.time(rng.uniform(1.0, 9.0, n), EventConfig().terminal())),
+        IVP.ode(Logistic(rng.uniform(0.5, 2.0, n), np.full(n, 10.0)), 0.0, 20.0,
+                np.full((1, n), 0.1)).event(Event.component(0, 9.0, EventConfig().terminal())),
+        IVP.ode(Harmonic(rng.uniform(0.5, 3.0, n)), 0.0, -12.0,
+                np.vstack([np.ones(n), np.zeros(n)])).crossing(0, 0.0, CrossingDirection.Negative)
+        .options(max_events=6),
+    ]
+    for name in ("cash_karp", "rkv766e", "rkv989e"):
+        for ens in cases:
+            ens = ens.method(method(name))
+            assert_events_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS))

@@ -21,8 +21,8 @@ import numpy as np
 import pytest
 
 import pyref_erk
-from deamd import (DE_STATUS_BAD_INPUT, DE_STATUS_COMPLETE, DE_STATUS_STIFFNESS,
-                   ExplicitRungeKutta, Exponential, Harmonic, IVP, Linear, Logistic, Lorenz,
+from deamd import (DE_STATUS_BAD_INPUT, DE_STATUS_COMPLETE, DE_STATUS_INTERRUPTED,
+                   DE_STATUS_STIFFNESS, CrossingDirection, Event, EventConfig, ExplicitRungeKutta, Exponential, Harmonic, IVP, Linear, Logistic, Lorenz,
                    VanDerPol)
 from deamd import _abi
 
@@ -277,3 +277,73 @@ def test_nan_error_is_accepted(orc):
     assert got.status[0] == want[5] == DE_STATUS_COMPLETE
     assert np.isnan(got.y).all() and all(v != v for v in want[0])
     assert (got.accepted[0], got.rejected[0], got.evals[0]) == tuple(want[2:5]) == (3, 0, 64)
+
+
+# --- event solouts over the family's dense output ---------------------------
+
+COS_ZEROS = math.pi / 2 + math.pi * np.arange(6)
+
+
+# Tableaux whose interpolant carries rounding noise above the crossing
+# solouts' Newton acceptance (|offset| < 100 eps, crossing.rs:268-318) near
+# some zeros of cos t: large bi coefficients cancel, and samples 5e-5 apart
+# scatter by ~1e-12 (rkv656e) and more for the order 8/9 pairs on steps of
+# ~0.33.  There find_crossing_newton returns None and the solout pushes the
+# chord's zero (crossing.rs:225-231).
+NEWTON_FALLS_BACK = {"rkv656e", "rkv877e", "rkv878e", "rkv988e"}
+
+
+def _cosine(name):
+    return IVP.ode(Harmonic(1.0), 0.0, 20.0, np.array([[1.0], [0.0]])).method(
+        getattr(ExplicitRungeKutta, name)().rtol(1e-10).atol(1e-12).max_steps(100000))
+
+
+@pytest.mark.parametrize("name", FAMILY)
+def test_events_find_the_zeros_of_cosine(orc, name):
+    # y'' = -y, y(0) = (1, 0): y0 = cos t, six zeros in [0, 20].  The event
+    # solout (Brent on the interpolant) finds each one through the tableau's
+    # interpolant (Horner over the bi rows, or the cubic Hermite of rkf45 /
+    # cash_karp); Negative keeps the falling zeros (pi/2, 5pi/2, ...),
+    # Positive the rising ones.  The crossing and hyperplane solouts push
+    # Newton's root where it converges, else the chord's zero.
+    fell_back = False
+    for d in (CrossingDirection.Both, CrossingDirection.Negative, CrossingDirection.Positive):
+        want = COS_ZEROS if d == CrossingDirection.Both else (
+            COS_ZEROS[0::2] if d == CrossingDirection.Negative else COS_ZEROS[1::2])
+        sol = orc.solve_ode(_cosine(name).event(Event.component(0, 0.0, EventConfig(d))))
+        assert sol.status[0] == DE_STATUS_COMPLETE
+        t, y = sol.events(0)
+        assert len(t) == len(want)
+        assert np.abs(t - want).max() < 1e-9
+        assert np.abs(y[:, 0]).max() < 1e-9
+        assert np.abs(np.abs(y[:, 1]) - 1.0).max() < 1e-9
+        for ens in (_cosine(name).crossing(0, 0.0, d),
+                    _cosine(name).hyperplane_crossing([0.0, 5.0], [2.0, 0.0], d)):
+            sol = orc.solve_ode(ens)
+            assert sol.status[0] == DE_STATUS_COMPLETE
+            t2, y2 = sol.events(0)
+            assert len(t2) == len(want)
+            newton = np.abs(y2[:, 0]) < 100 * 2.0 ** -52 * 10
+            assert np.abs(t2 - want)[newton].max(initial=0.0) < 1e-9
+            assert np.abs(t2 - want)[~newton].max(initial=0.0) < 1e-2
+            fell_back |= bool((~newton).any())
+    assert fell_back == (name in NEWTON_FALLS_BACK)
+
+
+@pytest.mark.parametrize("name", FAMILY)
+def test_terminal_time_event_like_the_reference_test(orc, name):
+    # tests/ode/errors.rs: y' = y with g = t - 10 and a terminal event: g is
+    # linear in t, so the root is 10 up to rounding, and the solve stops with
+    # Interrupted at the end of the step that crossed it.
+    m = getattr(ExplicitRungeKutta, name)().rtol(1e-10).atol(1e-10).max_steps(100000)
+    ens = IVP.ode(Exponential(1.0), 0.0, 20.0, [1.0], n=1).method(m) \
+        .event(Event.time(10.0, EventConfig().terminal()))
+    sol = orc.solve_ode(ens)
+    assert sol.status[0] == DE_STATUS_INTERRUPTED and sol.n_events[0] == 1
+    t, y = sol.events(0)
+    assert abs(t[0] - 10.0) <= 1e-11 and sol.t[0] >= 10.0
+    assert abs(y[0, 0] / math.exp(10.0) - 1.0) < 1e-6
+    free = orc.solve_ode(IVP.ode(Exponential(1.0), 0.0, 20.0, [1.0], n=1).method(m))
+    assert free.status[0] == DE_STATUS_COMPLETE and free.accepted[0] > sol.accepted[0]
+    # the steps before the event are the free solve's steps
+    assert sol.evals[0] < free.evals[0]
