@@ -468,10 +468,10 @@ def bench_events(args, dv):
     lib = deamd.lib()
     stream = C.c_void_p(dv.stream.cuda_stream)
 
-    def descriptor(kind):
+    def descriptor(kind, method=_abi.DE_METHOD_DOPRI5):
         b = _abi.ode_defaults()
         b.family = _abi.DE_FAMILY_LORENZ
-        b.method = _abi.DE_METHOD_DOPRI5
+        b.method = method
         b.n, b.dim = n, 3
         b.params = C.c_void_p(d_p.data_ptr())
         b.y0 = C.c_void_p(d_y0.data_ptr())
@@ -504,8 +504,8 @@ def bench_events(args, dv):
             r.n_events = C.c_void_p(out["ne"].data_ptr())
         return b, r
 
-    def run(kind, steps, warmup):
-        b, r = descriptor(kind)
+    def run(kind, steps, warmup, method=_abi.DE_METHOD_DOPRI5):
+        b, r = descriptor(kind, method)
         for _ in range(warmup):
             _abi.check(lib.de_ode_batch_solve(C.byref(b), C.byref(r), stream))
         dv.sync()
@@ -530,12 +530,27 @@ def bench_events(args, dv):
         r = run(args.only, args.steps, args.warmup)
         return {"variant": args.only, "accepted": r["A"], "events": r["events"],
                 "kernel_ms": r["kernel_ms"]}
+    if args.only in ("rkf45", "rkv656e", "rkv878e"):  # the event solve with a family tableau
+        r = run("event", args.steps, args.warmup, getattr(_abi, "DE_METHOD_" + args.only.upper()))
+        return {"variant": "event/" + args.only, "accepted": r["A"], "events": r["events"],
+                "kernel_ms": r["kernel_ms"]}
     ev = run("event", args.steps, args.warmup)
     base = run("none", max(3, args.steps // 2), 1)
     cr = run("crossing", max(3, args.steps // 2), 1)
     hp = run("hyperplane", max(3, args.steps // 2), 1)
     flops = LORENZ_FLOP_ACC * ev["A"] + LORENZ_FLOP_REJ * ev["R"]
     achieved = flops / (ev["kernel_ms"] / 1000.0) / 1e12
+    # The embedded family with the same event: the cubic Hermite interpolant
+    # (rkf45) and the bi interpolants with extra stages (rkv656e, rkv878e).
+    family = {}
+    for name in ("rkf45", "rkv656e", "rkv878e"):
+        m = getattr(_abi, "DE_METHOD_" + name.upper())
+        fe = run("event", max(3, args.steps // 2), 1, m)
+        fb = run("none", max(3, args.steps // 2), 1, m)
+        family[name] = {"value": fe["value"], "ms_per_step": fe["ms_per_step"],
+                        "kernel_ms": fe["kernel_ms"], "events_per_solve": fe["events"],
+                        "all_complete": fe["ok"],
+                        "event_overhead": fe["ms_per_step"] / fb["ms_per_step"] - 1.0}
     return {
         "metric": "accepted steps*traj/s FP64 DOPRI5 with an event solout, 1 GPU",
         "value": ev["value"],
@@ -557,6 +572,7 @@ def bench_events(args, dv):
                         "events_per_solve": cr["events"], "all_complete": cr["ok"]},
         "hyperplane": {"value": hp["value"], "ms_per_step": hp["ms_per_step"],
                        "events_per_solve": hp["events"], "all_complete": hp["ok"]},
+        "embedded_family": family,
         "roofline": {"bound": "fp64_vector", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": load_traffic("lorenz_event_dopri5_1M"),

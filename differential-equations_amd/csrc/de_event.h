@@ -50,14 +50,16 @@ struct EventArgs {
   double* t_out;         // [max_events * n]: t_out[k * n + i]
   double* y_out;         // [(k * D + c) * n + i]
   int32_t* n_out;        // [n] points pushed, or null
-  // Deferred roots (null brk: inline).  Entry k of trajectory i is
-  // brk[(i * bmax + k) * width ...]: kEntryMeta fields, then the lane's
-  // interpolant data; its result record is res[(i * bmax + k) * kResWidth ...].
+  // Deferred roots (null brk: inline), for the slice of trajectories that
+  // starts at `base`.  Entry k of trajectory i is brk[((i - base) * bmax + k)
+  // * width ...]: kEntryMeta fields, then the lane's interpolant data; its
+  // result record is res[((i - base) * bmax + k) * kResWidth ...].
   double* brk;
   double* res;
-  int32_t* bcount;  // [n] brackets stored, -1: solve again inline
+  int32_t* bcount;  // [slice] brackets stored, -1: solve again inline
   int32_t bmax;
   int32_t width;
+  int64_t base;
 };
 
 // Fields of a deferred bracket: the step's start and its length (or end, for
@@ -223,7 +225,7 @@ DE_HD double* event_defer(const EventArgs& e, int64_t i, int32_t& k, bool need, 
     *overflow = true;
     return nullptr;
   }
-  double* ent = e.brk + ((int64_t)i * e.bmax + k) * e.width;
+  double* ent = e.brk + ((i - e.base) * e.bmax + k) * e.width;
   ent[kEntryT0] = t0;
   ent[kEntryH] = h;
   ent[kEntryG0] = g0;
@@ -250,7 +252,7 @@ DE_HD void event_result(double* r, int flag, double t, const double* y, double t
 // Deferred-mode end of a trajectory: the bracket count (the bookkeeping pass
 // writes the event outputs), or -1 after an overflow.
 DE_HD void event_defer_finish(const EventArgs& e, int64_t i, int32_t k, bool overflow) {
-  e.bcount[i] = overflow ? -1 : k;
+  e.bcount[i - e.base] = overflow ? -1 : k;
 }
 
 // EventWrappedSolout around DefaultSolout or TEvalSolout.
@@ -364,9 +366,9 @@ struct LaneEvent<D, 1> {
   // (a reference panic) sends the trajectory to the inline solve, which
   // rewrites all of its outputs.
   DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool teval_base) {
-    const int32_t cnt = e.bcount[i];
+    const int32_t cnt = e.bcount[i - e.base];
     if (cnt < 0) return false;
-    const double* base = e.res + (int64_t)i * e.bmax * kResWidth;
+    const double* base = e.res + (i - e.base) * e.bmax * kResWidth;
     int32_t pushed = 0;
     double tail = as_f64(kNaNBits), tevals = 0.0;
     for (int32_t k = 0; k < cnt; ++k) {
@@ -512,9 +514,9 @@ struct LaneEvent<D, 2> {
     event_result<D>(res, flag, tc, yc, 0.0, 0.0);
   }
   DE_HD static bool book(const EventArgs& e, int64_t n, int64_t i, bool) {
-    const int32_t cnt = e.bcount[i];
+    const int32_t cnt = e.bcount[i - e.base];
     if (cnt < 0) return false;
-    const double* base = e.res + (int64_t)i * e.bmax * kResWidth;
+    const double* base = e.res + (i - e.base) * e.bmax * kResWidth;
     int32_t pushed = 0;
     for (int32_t k = 0; k < cnt; ++k) {
       const double* r = base + (int64_t)k * kResWidth;

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "de_lane.h"
 #include "de_launch.h"
@@ -117,13 +118,14 @@ __global__ void normalise_normal_kernel(OdeArgs* args, double* out) {
 // compacted by a scan of their counts in LDS, one bracket per thread at a
 // time, so waves stay full whatever the per-trajectory counts.
 template <class Lane>
-__global__ __launch_bounds__(kBlock) void event_root_kernel(const OdeArgs* __restrict__ ap) {
+__global__ __launch_bounds__(kBlock) void event_root_kernel(const OdeArgs* __restrict__ ap,
+                                                            int64_t count) {
   const OdeArgs& a = *ap;
   __shared__ int32_t off[kBlock + 1];
   const int t = threadIdx.x;
-  const int64_t i0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t j0 = (int64_t)blockIdx.x * kBlock;  // position in the slice
   int32_t cnt = 0;
-  if (i0 + t < a.n) cnt = max(a.ev.bcount[i0 + t], 0);
+  if (j0 + t < count) cnt = max(a.ev.bcount[j0 + t], 0);
   off[t + 1] = cnt;
   if (t == 0) off[0] = 0;
   __syncthreads();
@@ -141,9 +143,9 @@ __global__ __launch_bounds__(kBlock) void event_root_kernel(const OdeArgs* __res
       if (off[mid] <= j) lo = mid;
       else hi = mid;
     }
-    const int64_t i = i0 + lo;
-    const int64_t slot = i * a.ev.bmax + (j - off[lo]);
-    Lane::entry_root(a, i, a.ev.brk + slot * a.ev.width, a.ev.res + slot * kResWidth);
+    const int64_t slot = (j0 + lo) * a.ev.bmax + (j - off[lo]);
+    Lane::entry_root(a, a.ev.base + j0 + lo, a.ev.brk + slot * a.ev.width,
+                     a.ev.res + slot * kResWidth);
   }
 }
 
@@ -151,14 +153,24 @@ __global__ __launch_bounds__(kBlock) void event_root_kernel(const OdeArgs* __res
 // again inline are appended to `rerun`.
 template <class Lane>
 __global__ __launch_bounds__(kBlock) void event_book_kernel(const OdeArgs* __restrict__ ap,
-                                                            uint32_t* rerun, uint32_t* nrerun) {
+                                                            int64_t count, uint32_t* rerun,
+                                                            uint32_t* nrerun) {
   const OdeArgs& a = *ap;
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= count) return;
+  const int64_t i = a.ev.base + j;
   if (!Lane::entry_book(a, i)) rerun[atomicAdd(nrerun, 1u)] = (uint32_t)i;
 }
 
 static __global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+static __global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
+// order[j] = first + j: a slice's trajectories in index order.
+static __global__ __launch_bounds__(kBlock) void iota_kernel(uint32_t* order, int64_t first,
+                                                             int64_t count) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j < count) order[j] = (uint32_t)(first + j);
+}
 
 // Arguments of the inline solve of the listed trajectories.
 static __global__ void rerun_args_kernel(const OdeArgs* src, OdeArgs* dst, const uint32_t* order,
@@ -172,23 +184,40 @@ static __global__ void rerun_args_kernel(const OdeArgs* src, OdeArgs* dst, const
   *dst = r;
 }
 
-// Deferred entries per trajectory: max_events + half again (at least 32 more),
-// halved while the regions (entries and result records) would exceed this
-// budget; below 16 the roots are found inline.
+// Deferred entries per trajectory: max_events + half again (at least 32 more).
+// The regions (entries and result records) of one pass stay within
+// kDeferBudget bytes: when those of the whole ensemble would not, the
+// ensemble is solved in slices of consecutive trajectories, each a pass of
+// its own (integration, roots, bookkeeping, inline reruns), of at least
+// kMinDeferSlice trajectories (enough to fill the GPU); only if even that
+// does not fit is the number of entries halved, and below 16 the roots are
+// found inline.  The environment variable DE_EVENT_DEFER_SLICE, when set to
+// a positive count, fixes the slice size instead (for tests and tuning).
 constexpr size_t kDeferBudget = (size_t)48 << 30;
+constexpr int64_t kMinDeferSlice = int64_t(1) << 18;
 
 template <class Lane>
-static int64_t defer_slots(const OdeArgs& a) {
+static int64_t defer_slots(const OdeArgs& a, int64_t* slice) {
+  *slice = a.n;
   if constexpr (Lane::kEv == 0) {
     return 0;
   } else {
     if (a.ev.terminate != 0 || a.n > 0x7fffffffLL) return 0;
-    int64_t bmax = (int64_t)a.ev.max_events + std::max<int64_t>(a.ev.max_events / 2, 32);
-    while (bmax >= 16 && (size_t)a.n * (size_t)bmax * (Lane::kEntryWidth + kResWidth) *
-                                 sizeof(double) >
-                             kDeferBudget)
-      bmax /= 2;
-    return bmax >= 16 ? bmax : 0;
+    int64_t want = std::min<int64_t>(a.n, kMinDeferSlice);
+    int64_t forced = 0;
+    if (const char* f = getenv("DE_EVENT_DEFER_SLICE")) forced = atoll(f);
+    if (forced > 0) want = std::min<int64_t>(a.n, forced);
+    for (int64_t bmax = (int64_t)a.ev.max_events + std::max<int64_t>(a.ev.max_events / 2, 32);
+         bmax >= 16; bmax /= 2) {
+      const size_t per = (size_t)bmax * (Lane::kEntryWidth + kResWidth) * sizeof(double);
+      int64_t fit = (int64_t)std::min<size_t>(kDeferBudget / per, (size_t)a.n);
+      if (fit < want) continue;
+      if (forced > 0) fit = want;
+      const int64_t parts = (a.n + fit - 1) / fit;
+      *slice = (a.n + parts - 1) / parts;
+      return bmax;
+    }
+    return 0;
   }
 }
 
@@ -206,15 +235,15 @@ static __global__ __launch_bounds__(kBuckets) void zero_hist_kernel(uint32_t* hi
 }
 
 template <class Lane>
-__global__ __launch_bounds__(kBlock) void cost_kernel(OdeArgs a, uint8_t* bucket,
-                                                      uint32_t* hist) {
+__global__ __launch_bounds__(kBlock) void cost_kernel(OdeArgs a, int64_t first, int64_t count,
+                                                      uint8_t* bucket, uint32_t* hist) {
   __shared__ uint32_t h[kBuckets];
   h[threadIdx.x] = 0u;  // kBlock == kBuckets
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < a.n) {
-    const uint32_t b = cost_bucket(Lane::predicted_cost(a, i));
-    bucket[i] = (uint8_t)b;
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j < count) {
+    const uint32_t b = cost_bucket(Lane::predicted_cost(a, first + j));
+    bucket[j] = (uint8_t)b;
     atomicAdd(&h[b], 1u);
   }
   __syncthreads();
@@ -236,32 +265,37 @@ static __global__ __launch_bounds__(kBuckets) void scan_hist_kernel(uint32_t* hi
   hist[t] = v[t] - hist[t];
 }
 
-static __global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t n, const uint8_t* bucket,
-                                                         uint32_t* offset, uint32_t* order) {
+static __global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t first, int64_t count,
+                                                                const uint8_t* bucket,
+                                                                uint32_t* offset,
+                                                                uint32_t* order) {
   __shared__ uint32_t cnt[kBuckets], base[kBuckets];
   cnt[threadIdx.x] = 0u;
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   uint32_t b = 0u, local = 0u;
-  if (i < n) {
-    b = bucket[i];
+  if (j < count) {
+    b = bucket[j];
     local = atomicAdd(&cnt[b], 1u);
   }
   __syncthreads();
   if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&offset[threadIdx.x], cnt[threadIdx.x]);
   __syncthreads();
-  if (i < n) order[base[b] + local] = (uint32_t)i;
+  if (j < count) order[base[b] + local] = (uint32_t)(first + j);
 }
 
-// Fills a.order (n entries) using `bucket` (n bytes) and `hist` (256 words).
+// Fills order[0 .. count) with trajectories first .. first + count - 1,
+// longest predicted first, using `bucket` (count bytes) and `hist` (256 words).
 template <class Lane>
-static void launch_order(const OdeArgs& a, uint8_t* bucket, uint32_t* hist, uint32_t* order,
-                         hipStream_t s) {
-  const unsigned grid = (unsigned)((a.n + kBlock - 1) / kBlock);
+static void launch_order(const OdeArgs& a, int64_t first, int64_t count, uint8_t* bucket,
+                         uint32_t* hist, uint32_t* order, hipStream_t s) {
+  const unsigned grid = (unsigned)((count + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(zero_hist_kernel, dim3(1), dim3(kBuckets), 0, s, hist);
-  hipLaunchKernelGGL(cost_kernel<Lane>, dim3(grid), dim3(kBlock), 0, s, a, bucket, hist);
+  hipLaunchKernelGGL(cost_kernel<Lane>, dim3(grid), dim3(kBlock), 0, s, a, first, count, bucket,
+                     hist);
   hipLaunchKernelGGL(scan_hist_kernel, dim3(1), dim3(kBuckets), 0, s, hist);
-  hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kBlock), 0, s, a.n, bucket, hist, order);
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kBlock), 0, s, first, count,
+                     (const uint8_t*)bucket, hist, order);
 }
 
 template <class Lane>
@@ -293,87 +327,110 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
   // Stream-ordered workspace: the launch arguments; for event lanes 256 bytes
   // for the normalised hyperplane normal and, with deferred roots, a second
   // argument block, the per-trajectory bracket counts, the rerun list and its
-  // count, and the bracket regions; for the longest-first order (schedule
-  // PERSISTENT, batches of a few thousand trajectories or more), 256 bucket
-  // counts, the order and one bucket byte per trajectory.
+  // count, the slice's trajectory count and the bracket regions; for the
+  // longest-first order (schedule PERSISTENT, batches of a few thousand
+  // trajectories or more) or a sliced solve, 256 bucket counts, the order and
+  // one bucket byte per trajectory of a slice.
   const bool ordered = schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL;
   const size_t args_bytes = (sizeof(OdeArgs) + 255) & ~(size_t)255;
   const size_t ev_bytes = Lane::kEv != 0 ? 256 : 0;
-  const size_t order_bytes =
-      ordered ? ((kBuckets * sizeof(uint32_t) + (size_t)a.n * (sizeof(uint32_t) + 1) + 255) &
-                 ~(size_t)255)
-              : 0;
-  const size_t list_bytes = ((size_t)a.n * sizeof(uint32_t) + 255) & ~(size_t)255;
-  int64_t bmax = defer_slots<Lane>(a);
+  int64_t slice = a.n;
+  int64_t bmax = defer_slots<Lane>(a, &slice);
   void* work = nullptr;
-  size_t defer_bytes = 0;
+  size_t order_bytes = 0, list_bytes = 0, defer_bytes = 0;
   for (;;) {
+    const bool listed = ordered || slice < a.n;
+    order_bytes = listed ? ((kBuckets * sizeof(uint32_t) + (size_t)slice * (sizeof(uint32_t) + 1) +
+                             255) & ~(size_t)255)
+                         : 0;
+    list_bytes = ((size_t)slice * sizeof(uint32_t) + 255) & ~(size_t)255;
     defer_bytes = bmax > 0 ? args_bytes + 2 * list_bytes + 256 +
-                                 (size_t)a.n * (size_t)bmax * (Lane::kEntryWidth + kResWidth) *
+                                 (size_t)slice * (size_t)bmax * (Lane::kEntryWidth + kResWidth) *
                                      sizeof(double)
                            : 0;
     if (hipMallocAsync(&work, args_bytes + ev_bytes + order_bytes + defer_bytes, s) == hipSuccess)
       break;
     (void)hipGetLastError();
     if (bmax == 0) return -1;
-    bmax = 0;  // no room for the bracket regions: roots inline
+    bmax = 0;  // no room for the bracket regions: roots inline, one pass
+    slice = a.n;
   }
+  const bool sliced = slice < a.n;
   char* w = (char*)work;
   OdeArgs* d_args = (OdeArgs*)w;
-  OdeArgs pa = a;
-  pa.order = nullptr;
-  pa.n_dev = nullptr;
-  if (ordered) {
-    uint32_t* hist = (uint32_t*)(w + args_bytes + ev_bytes);
-    uint32_t* order = hist + kBuckets;
-    uint8_t* bucket = (uint8_t*)(order + a.n);
-    launch_order<Lane>(a, bucket, hist, order, s);
-    pa.order = order;
-  }
+  uint32_t* hist = order_bytes ? (uint32_t*)(w + args_bytes + ev_bytes) : nullptr;
+  uint32_t* order = hist ? hist + kBuckets : nullptr;
+  uint8_t* bucket = order ? (uint8_t*)(order + slice) : nullptr;
   OdeArgs* d_rerun_args = nullptr;
-  uint32_t *bcount = nullptr, *rerun = nullptr, *nrerun = nullptr;
-  if constexpr (Lane::kEv != 0) {
-    pa.ev.brk = nullptr;
-    pa.ev.bcount = nullptr;
-    if (bmax > 0) {
-      char* d = w + args_bytes + ev_bytes + order_bytes;
-      d_rerun_args = (OdeArgs*)d;
-      bcount = (uint32_t*)(d + args_bytes);
-      rerun = (uint32_t*)(d + args_bytes + list_bytes);
-      nrerun = (uint32_t*)(d + args_bytes + 2 * list_bytes);
-      pa.ev.res = (double*)(d + args_bytes + 2 * list_bytes + 256);
-      pa.ev.brk = pa.ev.res + (size_t)a.n * (size_t)bmax * kResWidth;
-      pa.ev.bcount = (int32_t*)bcount;
-      pa.ev.bmax = (int32_t)bmax;
-      pa.ev.width = Lane::kEntryWidth;
+  uint32_t *bcount = nullptr, *rerun = nullptr, *nrerun = nullptr, *nslice = nullptr;
+  double* res = nullptr;
+  if (bmax > 0) {
+    char* d = w + args_bytes + ev_bytes + order_bytes;
+    d_rerun_args = (OdeArgs*)d;
+    bcount = (uint32_t*)(d + args_bytes);
+    rerun = (uint32_t*)(d + args_bytes + list_bytes);
+    nrerun = (uint32_t*)(d + args_bytes + 2 * list_bytes);
+    nslice = nrerun + 1;
+    res = (double*)(d + args_bytes + 2 * list_bytes + 256);
+  }
+  for (int64_t first = 0; first < a.n; first += slice) {
+    const int64_t count = std::min<int64_t>(slice, a.n - first);
+    OdeArgs pa = a;
+    pa.order = nullptr;
+    pa.n_dev = nullptr;
+    if (order) {
+      if (ordered && count >= kOrderMinN)
+        launch_order<Lane>(a, first, count, bucket, hist, order, s);
+      else
+        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((count + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, order, first, count);
+      pa.order = order;
     }
-  }
-  hipLaunchKernelGGL(store_args_kernel, dim3(1), dim3(1), 0, s, pa, d_args);
-  if constexpr (Lane::kEv != 0) {
-    static_assert(Lane::D * sizeof(double) <= 256, "normal buffer");
-    if (a.ev.solout == kSoloutHyperplane)
-      hipLaunchKernelGGL(normalise_normal_kernel<Lane::D>, dim3(1), dim3(1), 0, s, d_args,
-                         (double*)(w + args_bytes));
-  }
-  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-  hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s,
-                     (const OdeArgs*)d_args);
-  if constexpr (Lane::kEv != 0) {
-    if (bmax > 0) {
-      // Roots of every stored bracket, the push rules per trajectory, and the
-      // inline solve of the trajectories listed for it (on the full grid; the
-      // count is known only on the device).
-      const unsigned gx = (unsigned)((a.n + kBlock - 1) / kBlock);
-      hipLaunchKernelGGL(event_root_kernel<Lane>, dim3(gx), dim3(kBlock), 0, s,
-                         (const OdeArgs*)d_args);
-      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, nrerun);
-      hipLaunchKernelGGL(event_book_kernel<Lane>, dim3(gx), dim3(kBlock), 0, s,
-                         (const OdeArgs*)d_args, rerun, nrerun);
-      hipLaunchKernelGGL(rerun_args_kernel, dim3(1), dim3(1), 0, s, (const OdeArgs*)d_args,
-                         d_rerun_args, (const uint32_t*)rerun, (const uint32_t*)nrerun);
-      hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-      hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)grid), dim3(kBlock), 0, s,
-                         (const OdeArgs*)d_rerun_args);
+    if (sliced) {
+      hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(1), 0, s, nslice, (uint32_t)count);
+      pa.n_dev = nslice;
+    }
+    if constexpr (Lane::kEv != 0) {
+      pa.ev.brk = nullptr;
+      pa.ev.res = nullptr;
+      pa.ev.bcount = nullptr;
+      pa.ev.base = first;
+      if (bmax > 0) {
+        pa.ev.res = res;
+        pa.ev.brk = res + (size_t)slice * (size_t)bmax * kResWidth;
+        pa.ev.bcount = (int32_t*)bcount;
+        pa.ev.bmax = (int32_t)bmax;
+        pa.ev.width = Lane::kEntryWidth;
+      }
+    }
+    hipLaunchKernelGGL(store_args_kernel, dim3(1), dim3(1), 0, s, pa, d_args);
+    if constexpr (Lane::kEv != 0) {
+      static_assert(Lane::D * sizeof(double) <= 256, "normal buffer");
+      if (a.ev.solout == kSoloutHyperplane)
+        hipLaunchKernelGGL(normalise_normal_kernel<Lane::D>, dim3(1), dim3(1), 0, s, d_args,
+                           (double*)(w + args_bytes));
+    }
+    const int64_t blocks = (count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
+    hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)std::min(grid, blocks)),
+                       dim3(kBlock), 0, s, (const OdeArgs*)d_args);
+    if constexpr (Lane::kEv != 0) {
+      if (bmax > 0) {
+        // Roots of every stored bracket, the push rules per trajectory, and the
+        // inline solve of the trajectories listed for it (on the slice's grid;
+        // the count is known only on the device).
+        const unsigned gx = (unsigned)blocks;
+        hipLaunchKernelGGL(event_root_kernel<Lane>, dim3(gx), dim3(kBlock), 0, s,
+                           (const OdeArgs*)d_args, count);
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, nrerun);
+        hipLaunchKernelGGL(event_book_kernel<Lane>, dim3(gx), dim3(kBlock), 0, s,
+                           (const OdeArgs*)d_args, count, rerun, nrerun);
+        hipLaunchKernelGGL(rerun_args_kernel, dim3(1), dim3(1), 0, s, (const OdeArgs*)d_args,
+                           d_rerun_args, (const uint32_t*)rerun, (const uint32_t*)nrerun);
+        hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
+        hipLaunchKernelGGL(ode_persistent_kernel<Lane>, dim3((unsigned)std::min(grid, blocks)),
+                           dim3(kBlock), 0, s, (const OdeArgs*)d_rerun_args);
+      }
     }
   }
   if (hipFreeAsync(work, s) != hipSuccess) return -1;

@@ -191,3 +191,23 @@ def test_deferred_regions_overflow_to_inline_solves(orc, name):
         ens = IVP.ode(Harmonic(k), 0.0, 40.0, y0).method(method(name, h=0.002)) \
             .crossing(1, 0.0).options(max_events=cap)
         assert_events_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS))
+
+
+@pytest.mark.parametrize("name,n,slice_", [("dopri5", 2048, 700), ("dop853", 9000, 4500),
+                                           ("rkv878e", 2048, 1000), ("rkf45", 9000, 4500)])
+def test_deferred_roots_in_slices(orc, monkeypatch, name, n, slice_):
+    # Ensembles whose bracket regions exceed the deferral budget are solved in
+    # slices of consecutive trajectories, each with its own region, root and
+    # bookkeeping passes and inline reruns; DE_EVENT_DEFER_SLICE forces small
+    # slices (in index order below 4096 trajectories, longest-first above).
+    monkeypatch.setenv("DE_EVENT_DEFER_SLICE", str(slice_))
+    m = method(name) if name in ("dopri5", "dop853") else \
+        getattr(ExplicitRungeKutta, name)().rtol(1e-8).atol(1e-10).max_steps(1000000)
+    # regions of 6 + 32 brackets: about a third of the trajectories outgrow them
+    p, ens = lorenz(n, 12.0, seed=61)
+    ens = ens.method(m).event(Event.component(2, p[1] - 1.0)).options(max_events=6)
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.n_events > 38).any() and (want.n_events < 38).any()
+    assert_events_equal(ens.solve(), want)
+    ens = ens.t_eval(np.linspace(0.0, 12.0, 9)).options(schedule=DE_SCHED_PERSISTENT_FIFO)
+    assert_events_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS), dense=True)
