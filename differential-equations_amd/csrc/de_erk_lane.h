@@ -70,9 +70,10 @@ struct ErkLane {
   static constexpr int KN = (kInterp && kBi) ? I : S;  // stages held in registers
   static constexpr uint32_t kZeroStages = erk_zero_stages<Tab>(KN, kInterp && kBi);
   // Deferred event bracket: the meta fields (kEntryH holds h), then y_prev and
-  // the I stages (Horner interpolant), or y_prev, dydt_prev, y, dydt (cubic
-  // Hermite).
-  static constexpr int kEntryWidth = EV == 0 ? 0 : kEntryMeta + (kBi ? (1 + I) * D : 4 * D);
+  // the S stages of the step (the root pass builds the I - S dense stages from
+  // them, with the step's own operations), or y_prev, dydt_prev, y, dydt
+  // (cubic Hermite).
+  static constexpr int kEntryWidth = EV == 0 ? 0 : kEntryMeta + (kBi ? (1 + S) * D : 4 * D);
   // Occupancy hint of the persistent kernel: three waves per SIMD (168 VGPRs);
   // two (256 VGPRs) for the lanes of more than 36 doubles of interpolation
   // stages, which spill at three.
@@ -117,34 +118,55 @@ struct ErkLane {
   }
   DE_HD void finish(const OdeArgs& a, int64_t i, int32_t st) const { write_back(a, i, st, *this); }
 
-  // The interpolant of a deferred entry, as the step's.
-  DE_HD static void entry_bounds(const double* ent, double* t0, double* t1) {
-    *t0 = ent[kEntryT0];
-    *t1 = ent[kEntryT0] + ent[kEntryH];
-  }
-  DE_HD static bool entry_interp(const double* ent, double te, double* out) {
-    const double tp = ent[kEntryT0], hp = ent[kEntryH];
-    const double t_new = tp + hp;
-    if (te < tp || te > t_new) return false;
-    const double* d = ent + kEntryMeta;
-    if constexpr (kBi) {
-      const double(*k)[D] = reinterpret_cast<const double(*)[D]>(d + D);
-      horner_interp(d, k, tp, hp, te, stages_finite(k), out);
-    } else {
-      hermite_interp(d, d + D, d + 2 * D, d + 3 * D, tp, t_new, te, out);
-    }
-    return true;
-  }
+  // The interpolant of a deferred entry, as the step's: the dense stages are
+  // built here once per bracket (same inputs, same operations, same bits as
+  // the integrator's), so the integrator does not build them on every step
+  // where some lane of its wave brackets.
   DE_HD static bool teval_base(const OdeArgs& a) { return EV == 1 && DENSE && a.n_eval > 0; }
   DE_HD static void entry_root(const OdeArgs& a, int64_t i, const double* ent, double* res) {
-    double loc[kEntryWidth];
+    double meta[kEntryMeta];
 #pragma unroll
-    for (int f = 0; f < kEntryWidth; ++f) loc[f] = ent[f];
-    auto interp = [&](double te, double* out) { return entry_interp(loc, te, out); };
-    double t0, t1;
-    entry_bounds(loc, &t0, &t1);
-    LaneEvent<D, EV>::root(a.ev, i, loc, res, t0, t1, signum(loc[kEntryH]), teval_base(a),
-                           interp);
+    for (int m = 0; m < kEntryMeta; ++m) meta[m] = ent[m];
+    const double tp = meta[kEntryT0], hp = meta[kEntryH];
+    const double t_new = tp + hp;
+    const double* d = ent + kEntryMeta;
+    if constexpr (kBi) {
+      double yp[D], k[I][D];
+      uint32_t bad[D];
+      const uint32_t hbad = isfinite(hp) ? 0u : ~0u;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        yp[c] = d[c];
+        bad[c] = hbad;
+      }
+#pragma unroll
+      for (int r = 0; r < S; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          k[r][c] = d[D + r * D + c];
+          if ((kZeroStages >> r) & 1u) bad[c] |= isfinite(k[r][c]) ? 0u : (1u << r);
+        }
+      ErkLane L;
+      L.f.load(a.params, a.n, i);
+      L.template stages<S, I>(k, yp, tp, hp, bad);
+      const bool fin = stages_finite(k);
+      auto interp = [&](double te, double* out) -> bool {
+        if (te < tp || te > t_new) return false;
+        horner_interp(yp, k, tp, hp, te, fin, out);
+        return true;
+      };
+      LaneEvent<D, EV>::root(a.ev, i, meta, res, tp, t_new, signum(hp), teval_base(a), interp);
+    } else {
+      double loc[4 * D];
+#pragma unroll
+      for (int f = 0; f < 4 * D; ++f) loc[f] = d[f];
+      auto interp = [&](double te, double* out) -> bool {
+        if (te < tp || te > t_new) return false;
+        hermite_interp(loc, loc + D, loc + 2 * D, loc + 3 * D, tp, t_new, te, out);
+        return true;
+      };
+      LaneEvent<D, EV>::root(a.ev, i, meta, res, tp, t_new, signum(hp), teval_base(a), interp);
+    }
   }
   DE_HD static bool entry_book(const OdeArgs& a, int64_t i) {
     return LaneEvent<D, EV>::book(a.ev, a.n, i, teval_base(a));
@@ -443,7 +465,13 @@ struct ErkLane {
       bool need = false;
       if constexpr (EV != 0) need = accept && ev.pre(a.ev, t_new, yh);
       if (tev || need) {
-        if constexpr (kBi) stages<S, I>(k, y, t, h, bad);
+        // The dense stages: for a t_eval point or an inline root search (a
+        // deferred bracket stores the S stages and the root pass builds them).
+        if constexpr (kBi) {
+          bool build = tev;
+          if constexpr (EV != 0) build = build || !a.ev.brk;
+          if (build) stages<S, I>(k, y, t, h, bad);
+        }
         const double tp = t, hp = h;
         bool fin = true;
         if constexpr (kBi) {
@@ -473,7 +501,7 @@ struct ErkLane {
 #pragma unroll
                   for (int c = 0; c < D; ++c) dst[c] = y[c];
 #pragma unroll
-                  for (int r = 0; r < I; ++r)
+                  for (int r = 0; r < S; ++r)
 #pragma unroll
                     for (int c = 0; c < D; ++c) dst[D + r * D + c] = k[r][c];
                 } else {
