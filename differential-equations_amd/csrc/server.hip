@@ -84,14 +84,13 @@ DE_DEV int find_range(const Range* r, int64_t n, int64_t ticket) {
   return (int)lo;
 }
 
-template <class F>
+template <class L>
 __global__ __launch_bounds__(kBlock) void chunk_lane(Queue* q, const Range* ranges, ReqDev* reqs,
-                                                    LaneSlot<Dopri5Lane<F, true>>* slots, int nslots,
-                                                    int k_iters) {
+                                                    LaneSlot<L>* slots, int nslots, int k_iters) {
   const int slot = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = slot < nslots;
   const int lane = __lane_id();
-  LaneSlot<Dopri5Lane<F, true>> S;
+  LaneSlot<L> S;
   if (valid) S = slots[slot];
   else S.state = -1;
   const int64_t published = *(volatile int64_t*)&q->published;
@@ -221,11 +220,11 @@ struct Family {
   void (*launch)(Family&, ReqDev*, int, hipStream_t);
 };
 
-template <class F>
+template <class L>
 void launch_lane(Family& f, ReqDev* reqs, int k, hipStream_t s) {
   int grid = (f.nslots + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(chunk_lane<F>, dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
-                     (LaneSlot<Dopri5Lane<F, true>>*)f.slots, f.nslots, k);
+  hipLaunchKernelGGL(chunk_lane<L>, dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
+                     (LaneSlot<L>*)f.slots, f.nslots, k);
 }
 
 template <int G, int C>
@@ -334,12 +333,18 @@ namespace {
 
 enum : int32_t { kServerGraph = 1 };
 
-int fam_index_for(de_server* s, int family, int nm) {
-  if (family == DE_FAMILY_LORENZ) return 0;
-  if (family == DE_FAMILY_VAN_DER_POL) return 1;
+// Pools: 0 / 1 Lorenz / Van der Pol with DOPRI5, 2 .. kSpringEnd - 1 the
+// spring classes, then Lorenz / Van der Pol with DOP853.
+constexpr int kSpringEnd = 9;
+constexpr int kLorenzDop853 = 9, kVdpDop853 = 10;
+
+int fam_index_for(de_server* s, int family, int nm, int method) {
+  const bool d853 = method == DE_METHOD_DOP853;
+  if (family == DE_FAMILY_LORENZ) return d853 ? kLorenzDop853 : 0;
+  if (family == DE_FAMILY_VAN_DER_POL) return d853 ? kVdpDop853 : 1;
   if (family == DE_FAMILY_SPRING_CHAIN) {
-    for (size_t k = 2; k < s->fams.size(); ++k)
-      if (nm <= s->fams[k].cap) return (int)k;
+    for (int k = 2; k < kSpringEnd; ++k)
+      if (nm <= s->fams[k].cap) return k;
   }
   return -1;
 }
@@ -511,7 +516,7 @@ void scheduler(de_server* s) {
       const de_ode_batch& b = r->b;
       if (b.family == DE_FAMILY_SPRING_CHAIN) {
         for (int64_t i = 0; i < b.n; ++i)
-          lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i])].push_back((int32_t)i);
+          lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i], b.method)].push_back((int32_t)i);
       }
       r->slot = s->free_slots.back();
       std::vector<int32_t*> d_lists(s->fams.size(), nullptr);
@@ -531,11 +536,12 @@ void scheduler(de_server* s) {
       s->pending.pop_front();
       s->active[r->slot] = r;
       if (b.family == DE_FAMILY_SPRING_CHAIN) {
-        for (size_t k = 2; k < s->fams.size(); ++k)
+        for (int k = 2; k < kSpringEnd; ++k)
           if (!lists[k].empty())
             publish(s, s->fams[k], r->slot, 0, (int32_t)lists[k].size(), d_lists[k]);
       } else {
-        publish(s, s->fams[fam_index_for(s, b.family, 0)], r->slot, 0, (int32_t)b.n, nullptr);
+        publish(s, s->fams[fam_index_for(s, b.family, 0, b.method)], r->slot, 0, (int32_t)b.n,
+                nullptr);
       }
     }
     if (s->active.empty()) {
@@ -608,14 +614,17 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  // Slot pools: Lorenz and Van der Pol lanes, and one pool per spring class.
+  // Slot pools: Lorenz and Van der Pol lanes, one pool per spring class, and
+  // the DOP853 lanes of Lorenz and Van der Pol.
   struct Spec {
     const char* name;
     int cap, g;
   };
   const Spec specs[] = {{"lorenz", 0, 1},     {"vdp", 0, 1},       {"spring4", 4, 4},
                         {"spring8", 8, 8},    {"spring16", 16, 16}, {"spring32", 32, 32},
-                        {"spring64", 64, 64}, {"spring128", 128, 64}, {"spring256", 256, 64}};
+                        {"spring64", 64, 64}, {"spring128", 128, 64}, {"spring256", 256, 64},
+                        {"lorenz_dop853", 0, 1}, {"vdp_dop853", 0, 1}};
+  static_assert(sizeof specs / sizeof specs[0] == kVdpDop853 + 1, "pool indices");
   s->fams.resize(sizeof specs / sizeof specs[0]);
   const int lane_slots = ncu * 4 * 256;  // 4 blocks of 256 lanes per CU
   const int group_lanes = ncu * 2 * 256;
@@ -627,9 +636,13 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
     f.g = specs[k].g;
     switch (k) {
       case 0: ok = make_pool<LaneSlot<de::Dopri5Lane<de::LorenzRhs, true>>>(f, lane_slots);
-        f.launch = launch_lane<de::LorenzRhs>; break;
+        f.launch = launch_lane<de::Dopri5Lane<de::LorenzRhs, true>>; break;
       case 1: ok = make_pool<LaneSlot<de::Dopri5Lane<de::VanDerPolRhs, true>>>(f, lane_slots);
-        f.launch = launch_lane<de::VanDerPolRhs>; break;
+        f.launch = launch_lane<de::Dopri5Lane<de::VanDerPolRhs, true>>; break;
+      case kLorenzDop853: ok = make_pool<LaneSlot<de::Dop853Lane<de::LorenzRhs, true>>>(f, lane_slots);
+        f.launch = launch_lane<de::Dop853Lane<de::LorenzRhs, true>>; break;
+      case kVdpDop853: ok = make_pool<LaneSlot<de::Dop853Lane<de::VanDerPolRhs, true>>>(f, lane_slots);
+        f.launch = launch_lane<de::Dop853Lane<de::VanDerPolRhs, true>>; break;
       case 2: ok = make_pool<LaneSlot<de::GroupDopri5<4, 1>>>(f, group_lanes);
         f.launch = launch_group<4, 1>; break;
       case 3: ok = make_pool<LaneSlot<de::GroupDopri5<8, 1>>>(f, group_lanes);
@@ -642,8 +655,9 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
         f.launch = launch_group<64, 1>; break;
       case 7: ok = make_pool<LaneSlot<de::GroupDopri5<64, 2>>>(f, group_lanes);
         f.launch = launch_group<64, 2>; break;
-      default: ok = make_pool<LaneSlot<de::GroupDopri5<64, 4>>>(f, group_lanes);
+      case 8: ok = make_pool<LaneSlot<de::GroupDopri5<64, 4>>>(f, group_lanes);
         f.launch = launch_group<64, 4>; break;
+      default: ok = false; break;
     }
   }
   size_t pools = 0;
@@ -676,9 +690,12 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
 
 int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
   if (!s || !b || !ticket) return DE_ERR_INVALID;
-  if (b->method != DE_METHOD_DOPRI5) return DE_ERR_UNSUPPORTED;
   if (b->family != DE_FAMILY_LORENZ && b->family != DE_FAMILY_VAN_DER_POL &&
       b->family != DE_FAMILY_SPRING_CHAIN)
+    return DE_ERR_UNSUPPORTED;
+  // DOPRI5 for every served family; DOP853 for the lane-mapped ones.
+  if (b->method != DE_METHOD_DOPRI5 &&
+      !(b->method == DE_METHOD_DOP853 && b->family != DE_FAMILY_SPRING_CHAIN))
     return DE_ERR_UNSUPPORTED;
   if (b->n <= 0 || !b->y0 || (b->n_eval > 0 && !b->t_eval)) return DE_ERR_INVALID;
   // Served requests carry scalar or per-trajectory tolerances and the final

@@ -118,3 +118,35 @@ def test_admission_respects_budget():
         ts = [srv.submit(lorenz_req(20000, seed=40 + k, tf=1.0)) for k in range(12)]
         for t in ts:
             assert (srv.poll(t).status == 0).all()
+
+
+def test_served_dop853_matches_direct_solve():
+    # DOP853 requests of the lane-mapped families run in their own slot pools
+    # beside the DOPRI5 ones: each gives the bits of a direct solve, alone and
+    # in a mixed stream, with and without t_eval output.
+    p, y0 = deamd.gen_lorenz(2000, seed=21)
+    d853 = IVP.ode(Lorenz(*p), 0.0, 5.0, y0).method(
+        ExplicitRungeKutta.dop853().rtol(1e-9).atol(1e-11).max_steps(1000000))
+    mu, v0 = deamd.gen_vdp(1500, seed=22)
+    vdp853 = IVP.ode(VanDerPol(*mu), 0.0, 8.0, v0).method(
+        ExplicitRungeKutta.dop853().rtol(1e-7).atol(1e-9).max_steps(1000000)) \
+        .t_eval(np.linspace(0.0, 8.0, 11))
+    want_l, want_v = d853.solve(), vdp853.solve()
+    with Server(8 * GIB) as srv:
+        assert_same(srv.poll(srv.submit(d853)), want_l)
+        ts = [srv.submit(lorenz_req(1000, seed=23)), srv.submit(d853), srv.submit(vdp853),
+              srv.submit(vdp_req(800, seed=24))]
+        got = [srv.poll(t) for t in ts]
+    assert_same(got[1], want_l)
+    assert_same(got[2], want_v)
+    assert np.array_equal(got[2].n_dense, want_v.n_dense)
+    assert np.array_equal(np.nan_to_num(got[2].dense), np.nan_to_num(want_v.dense))
+    assert_same(got[0], lorenz_req(1000, seed=23).solve())
+
+
+def test_server_refuses_dop853_spring_chains():
+    sysm, y0 = deamd.gen_spring(8, seed=1)
+    req = IVP.ode(sysm, 0.0, 1.0, y0).method(ExplicitRungeKutta.dop853())
+    with Server(8 * GIB) as srv:
+        with pytest.raises(RuntimeError):
+            srv.submit(req)
