@@ -62,7 +62,7 @@ def interpolate(tab, st, tt):
 
 
 def solve_erk(f, t0, tf, y0, rtol, atol, tab, h0=0.0, max_steps=10000, max_rejects=100,
-              powf=c_pow, t_eval=None):
+              powf=c_pow, t_eval=None, solout=None):
     """Returns (y, t, accepted, rejected, evals, status, dense); status 0
     complete, 1 max steps, 2 step size, 3 stiffness (max_rejects consecutive
     rejections), 4 bad input.  `t_eval` takes increasing points."""
@@ -96,6 +96,13 @@ def solve_erk(f, t0, tf, y0, rtol, atol, tab, h0=0.0, max_steps=10000, max_rejec
     if t_eval is not None and len(t_eval) and t_eval[0] == t0:
         dense.append(list(y0))
         idx = 1
+    # `solout(t_curr, t_prev, y_curr, interp)` after init and after every
+    # accepted step (solve_ivp.rs:152-183,242-259); "terminate" ends the solve
+    # Interrupted (5), anything else but "continue" with status 6.
+    if solout is not None:
+        flag = solout(t0, t0, list(st["y"]), lambda tt: None)
+        if flag != "continue":
+            return st["y"], t0, acc, rej, evals, 5 if flag == "terminate" else 6, dense
     while True:
         t, y, k = st["t"], st["y"], st["k"]
         if (t + h - tf) * direction > 0.0:
@@ -161,6 +168,14 @@ def solve_erk(f, t0, tf, y0, rtol, atol, tab, h0=0.0, max_steps=10000, max_rejec
                     te = t_eval[idx]
                     dense.append(list(st["y"]) if te == st["t"] else interpolate(tab, st, te))
                     idx += 1
+            if solout is not None:
+                snap = dict(st, k=[list(r) for r in st["k"]], y=list(st["y"]),
+                            dydt=list(st["dydt"]))
+                flag = solout(st["t"], st["t_prev"], list(st["y"]),
+                              lambda tt, sn=snap: interpolate(tab, sn, tt))
+                if flag != "continue":
+                    return (st["y"], st["t"], acc, rej, evals, 5 if flag == "terminate" else 6,
+                            dense)
         else:
             rejected_status = True
             stiff += 1

@@ -22,8 +22,8 @@ import pytest
 
 import pyref_erk
 from deamd import (DE_STATUS_BAD_INPUT, DE_STATUS_COMPLETE, DE_STATUS_INTERRUPTED,
-                   DE_STATUS_STIFFNESS, CrossingDirection, Event, EventConfig, ExplicitRungeKutta, Exponential, Harmonic, IVP, Linear, Logistic, Lorenz,
-                   VanDerPol)
+                   DE_STATUS_STIFFNESS, CrossingDirection, Event, EventConfig, ExplicitRungeKutta,
+                   Exponential, Harmonic, IVP, Linear, Logistic, Lorenz, VanDerPol)
 from deamd import _abi
 
 FAMILY = ["rkf45", "cash_karp", "rkv655e", "rkv656e", "rkv766e", "rkv767e", "rkv877e",
@@ -347,3 +347,68 @@ def test_terminal_time_event_like_the_reference_test(orc, name):
     assert free.status[0] == DE_STATUS_COMPLETE and free.accepted[0] > sol.accepted[0]
     # the steps before the event are the free solve's steps
     assert sol.evals[0] < free.evals[0]
+
+
+# Bit equality of the oracle's family event paths with the Python restatements
+# (pyref_erk's step and interpolant, pyref_events' solouts): every tableau,
+# every solout kind, the three directions taken in turn.
+_EV_KINDS = ("event", "event_teval", "event_plane_term", "crossing", "hyperplane")
+_EV_DIRS = [CrossingDirection.Both, CrossingDirection.Positive, CrossingDirection.Negative]
+_EV_DNAME = {CrossingDirection.Both: "both", CrossingDirection.Positive: "positive",
+             CrossingDirection.Negative: "negative"}
+EV_CASES = [(name, kind, _EV_DIRS[(a + b) % 3]) for a, name in enumerate(FAMILY)
+            for b, kind in enumerate(_EV_KINDS)]
+
+
+@pytest.mark.parametrize("name,kind,d", EV_CASES)
+def test_oracle_events_bit_equal_python_restatement(orc, name, kind, d):
+    import pyref_events as PE
+    tab = orc.erk_tableau(METHOD_ID[name])
+    rng = np.random.default_rng(700 + 10 * METHOD_ID[name] + _EV_KINDS.index(kind))
+    lorenz = kind in ("event", "event_plane_term", "hyperplane")
+    if lorenz:
+        rho = float(rng.uniform(20, 40))
+        y0 = list(1.0 + 0.1 * rng.standard_normal(3))
+        f, sysm, tf = _lorenz_rhs(rho), Lorenz(10.0, rho, 8.0 / 3.0), 3.0
+    else:
+        k = float(rng.uniform(0.5, 2.0))
+        y0 = [1.0, float(rng.uniform(-0.5, 0.5))]
+        f = lambda t, y, dy, k=k: dy.__setitem__(slice(None), [y[1], -k * y[0]])
+        sysm, tf = Harmonic(k), 12.0
+    rtol, atol = 1e-9, 1e-11
+    ens = IVP.ode(sysm, 0.0, tf, np.array(y0).reshape(-1, 1)) \
+        .method(build(name).rtol(rtol).atol(atol).max_steps(100000)).options(max_events=64)
+    sol = PE.Solution()
+    dname = _EV_DNAME[d]
+    if kind == "event":
+        ens = ens.event(Event.component(2, rho - 1.0, EventConfig(d)))
+        so = PE.EventWrapped(PE.DefaultSolout(sol), sol, lambda t, y: y[2] - (rho - 1.0),
+                             0.0, tf, dname, None)
+    elif kind == "event_teval":
+        pts = list(np.linspace(0.0, tf, 29))
+        ens = ens.t_eval(pts).event(Event.component(0, 0.25, EventConfig(d)))
+        so = PE.EventWrapped(PE.TEvalSolout(sol, pts, 0.0, tf), sol, lambda t, y: y[0] - 0.25,
+                             0.0, tf, dname, None)
+    elif kind == "event_plane_term":
+        p, nrm = [0.0, 0.0, rho - 1.0], [0.3, -0.2, 1.0]
+        ens = ens.event(Event.plane(p, nrm, EventConfig(d).terminate_after(2)))
+        g = PE.signed_distance(p, nrm)
+        so = PE.EventWrapped(PE.DefaultSolout(sol), sol, lambda t, y: g(y), 0.0, tf, dname, 2)
+    elif kind == "crossing":
+        ens = ens.crossing(1, 0.1, d)
+        so = PE.NewtonCrossing(sol, lambda y: y[1] - 0.1, dname)
+    else:
+        p, nrm = [1.0, 1.0, rho - 1.0], [0.5, 0.5, 2.0]
+        ens = ens.hyperplane_crossing(p, nrm, d)
+        so = PE.NewtonCrossing(sol, PE.signed_distance(p, PE.normalise(nrm)), dname,
+                               hyperplane=True)
+    want = pyref_erk.solve_erk(f, 0.0, tf, y0, rtol, atol, tab, max_steps=100000, solout=so)
+    got = orc.solve_ode(ens, pow_mode=1)
+    pts_want = so.root_points if kind.startswith("event") else sol.points
+    t, y = got.events(0)
+    assert len(pts_want) > 0
+    assert got.n_events[0] == len(pts_want)
+    assert t.tolist() == [p_[0] for p_ in pts_want]
+    assert y.tolist() == [p_[1] for p_ in pts_want]
+    assert list(got.y) == want[0] and got.t[0] == want[1]
+    assert (got.accepted[0], got.rejected[0], got.evals[0], got.status[0]) == tuple(want[2:6])
