@@ -216,6 +216,7 @@ struct Family {
   std::vector<Range> host_ranges;
   int64_t published = 0;
   int64_t active_tickets = 0;  // published but possibly unfinished
+  int active_reqs = 0;         // active requests with work in this pool
   hipStream_t stream;
   void (*launch)(Family&, ReqDev*, int, hipStream_t);
 };
@@ -279,16 +280,18 @@ struct Request {
   uint64_t ticket;
   de_ode_batch b;            // host copy; pointers below
   std::vector<double> params, y0, t0, tf, rtol, atol, t_eval;
+  std::vector<double> ev_value, ev_point, ev_normal;  // event solout inputs (normal normalised)
   std::vector<int32_t> sizes;
   std::vector<int64_t> so, po;
   int64_t state_len = 0;
   // results (host)
-  std::vector<double> y, t, dense;
+  std::vector<double> y, t, dense, event_t, event_y;
   std::vector<int64_t> acc, rej, ev;
-  std::vector<int32_t> status, n_dense;
+  std::vector<int32_t> status, n_dense, n_events;
   // device
   void* dmem = nullptr;
   size_t dbytes = 0;
+  uint32_t pools = 0;  // bit k: work published to pool k
   int slot = -1;
   bool done = false;
   int error = 0;
@@ -334,12 +337,21 @@ namespace {
 enum : int32_t { kServerGraph = 1 };
 
 // Pools: 0 / 1 Lorenz / Van der Pol with DOPRI5, 2 .. kSpringEnd - 1 the
-// spring classes, then Lorenz / Van der Pol with DOP853.
+// spring classes, then Lorenz / Van der Pol with DOP853, then DOPRI5 with an
+// event solout (EventWrappedSolout, or the crossing / hyperplane solouts),
+// whose roots are found inline.
 constexpr int kSpringEnd = 9;
 constexpr int kLorenzDop853 = 9, kVdpDop853 = 10;
+constexpr int kLorenzEvent = 11, kLorenzCross = 12, kVdpEvent = 13, kVdpCross = 14;
 
-int fam_index_for(de_server* s, int family, int nm, int method) {
+int fam_index_for(de_server* s, int family, int nm, int method, int solout) {
   const bool d853 = method == DE_METHOD_DOP853;
+  if (solout != DE_SOLOUT_DEFAULT) {
+    const bool cross = solout != DE_SOLOUT_EVENT;
+    if (family == DE_FAMILY_LORENZ) return cross ? kLorenzCross : kLorenzEvent;
+    if (family == DE_FAMILY_VAN_DER_POL) return cross ? kVdpCross : kVdpEvent;
+    return -1;
+  }
   if (family == DE_FAMILY_LORENZ) return d853 ? kLorenzDop853 : 0;
   if (family == DE_FAMILY_VAN_DER_POL) return d853 ? kVdpDop853 : 1;
   if (family == DE_FAMILY_SPRING_CHAIN) {
@@ -358,6 +370,9 @@ size_t footprint(const Request* r) {
   bytes += r->sizes.size() * 4 + (r->so.size() + r->po.size()) * 8 + n * 4;  // + class lists
   bytes += (size_t)r->state_len * 8 + n * (8 * 4 + 4 + 4);
   bytes += (size_t)b.n_eval * r->state_len * 8;
+  bytes += (r->ev_value.size() + r->ev_point.size() + r->ev_normal.size()) * 8;
+  if (b.event_solout != DE_SOLOUT_DEFAULT)
+    bytes += (size_t)b.max_events * n * 8 + (size_t)b.max_events * r->state_len * 8 + n * 4;
   return bytes + 64 * 256;
 }
 
@@ -416,6 +431,24 @@ bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_l
   a.status = (int32_t*)take(n * 4);
   a.n_dense = b.n_eval > 0 ? (int32_t*)take(n * 4) : nullptr;
   a.dense = b.n_eval > 0 ? (double*)take((size_t)b.n_eval * r->state_len * 8) : nullptr;
+  if (b.event_solout != DE_SOLOUT_DEFAULT) {
+    // The batch path's event arguments (roots inline: no bracket regions).
+    a.ev.solout = b.event_solout;
+    a.ev.fn = b.event_solout == DE_SOLOUT_CROSSING     ? de::kEventComponent
+              : b.event_solout == DE_SOLOUT_HYPERPLANE ? de::kEventPlane
+                                                        : b.event_fn;
+    a.ev.comp = b.event_component;
+    a.ev.direction = b.event_direction;
+    a.ev.terminate = b.event_terminate;
+    a.ev.max_events = b.max_events;
+    a.ev.value = b.event_value;
+    a.ev.value_traj = (const double*)put(r->ev_value.data(), r->ev_value.size() * 8);
+    a.ev.point = (const double*)put(r->ev_point.data(), r->ev_point.size() * 8);
+    a.ev.normal = (const double*)put(r->ev_normal.data(), r->ev_normal.size() * 8);
+    a.ev.t_out = (double*)take((size_t)b.max_events * n * 8);
+    a.ev.y_out = (double*)take((size_t)b.max_events * r->state_len * 8);
+    a.ev.n_out = (int32_t*)take(n * 4);
+  }
   if (b.family == DE_FAMILY_SPRING_CHAIN) {
     rd.a.sizes = (const int32_t*)put(r->sizes.data(), r->sizes.size() * 4);
     rd.a.state_offset = (const int64_t*)put(r->so.data(), r->so.size() * 8);
@@ -475,15 +508,32 @@ void retire(de_server* s, Request* r) {
                          s->ctl);
     (void)hipMemcpyAsync(r->n_dense.data(), a.n_dense, n * 4, hipMemcpyDeviceToHost, s->ctl);
   }
+  if (a.ev.n_out) {
+    const size_t m = (size_t)a.ev.max_events;
+    r->event_t.resize(m * n);
+    r->event_y.resize(m * r->state_len);
+    r->n_events.resize(n);
+    if (m) {
+      (void)hipMemcpyAsync(r->event_t.data(), a.ev.t_out, m * n * 8, hipMemcpyDeviceToHost,
+                           s->ctl);
+      (void)hipMemcpyAsync(r->event_y.data(), a.ev.y_out, m * r->state_len * 8,
+                           hipMemcpyDeviceToHost, s->ctl);
+    }
+    (void)hipMemcpyAsync(r->n_events.data(), a.ev.n_out, n * 4, hipMemcpyDeviceToHost, s->ctl);
+  }
   if (hipStreamSynchronize(s->ctl) != hipSuccess) r->error = DE_ERR_HIP;
   s->arena.release(r->dmem, r->dbytes);
   r->dmem = nullptr;
 }
 
-void launch_all(de_server* s) {
+// One chunk of every pool (graph capture), or of the pools that hold work of
+// an active request: an idle pool's launch would only read and write back its
+// slots.
+void launch_all(de_server* s, bool busy_only = false) {
   (void)hipEventRecord(s->fork, s->ctl);
   for (size_t k = 0; k < s->fams.size(); ++k) {
     Family& f = s->fams[k];
+    if (busy_only && f.active_reqs == 0) continue;
     (void)hipStreamWaitEvent(f.stream, s->fork, 0);
     f.launch(f, s->d_reqs, kChunk, f.stream);
     (void)hipEventRecord(s->fam_done[k], f.stream);
@@ -516,7 +566,8 @@ void scheduler(de_server* s) {
       const de_ode_batch& b = r->b;
       if (b.family == DE_FAMILY_SPRING_CHAIN) {
         for (int64_t i = 0; i < b.n; ++i)
-          lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i], b.method)].push_back((int32_t)i);
+          lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i], b.method, DE_SOLOUT_DEFAULT)]
+              .push_back((int32_t)i);
       }
       r->slot = s->free_slots.back();
       std::vector<int32_t*> d_lists(s->fams.size(), nullptr);
@@ -537,12 +588,17 @@ void scheduler(de_server* s) {
       s->active[r->slot] = r;
       if (b.family == DE_FAMILY_SPRING_CHAIN) {
         for (int k = 2; k < kSpringEnd; ++k)
-          if (!lists[k].empty())
+          if (!lists[k].empty()) {
             publish(s, s->fams[k], r->slot, 0, (int32_t)lists[k].size(), d_lists[k]);
+            r->pools |= 1u << k;
+          }
       } else {
-        publish(s, s->fams[fam_index_for(s, b.family, 0, b.method)], r->slot, 0, (int32_t)b.n,
-                nullptr);
+        const int k = fam_index_for(s, b.family, 0, b.method, b.event_solout);
+        publish(s, s->fams[k], r->slot, 0, (int32_t)b.n, nullptr);
+        r->pools |= 1u << k;
       }
+      for (size_t k = 0; k < s->fams.size(); ++k)
+        if ((r->pools >> k) & 1u) s->fams[k].active_reqs += 1;
     }
     if (s->active.empty()) {
       if (s->stop) break;
@@ -553,7 +609,7 @@ void scheduler(de_server* s) {
     if (s->graph) {
       (void)hipGraphLaunch(s->graph, s->ctl);
     } else {
-      launch_all(s);
+      launch_all(s, true);
     }
     // Remaining counts of the active requests.
     for (auto& kv : s->active)
@@ -577,6 +633,8 @@ void scheduler(de_server* s) {
       if (s->h_remaining[kv.first] <= 0) finished.push_back(kv.first);
     for (int slot : finished) {
       Request* r = s->active[slot];
+      for (size_t k = 0; k < s->fams.size(); ++k)
+        if ((r->pools >> k) & 1u) s->fams[k].active_reqs -= 1;
       retire(s, r);
       r->done = true;
       r->t_done = now_s();
@@ -623,10 +681,13 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
   const Spec specs[] = {{"lorenz", 0, 1},     {"vdp", 0, 1},       {"spring4", 4, 4},
                         {"spring8", 8, 8},    {"spring16", 16, 16}, {"spring32", 32, 32},
                         {"spring64", 64, 64}, {"spring128", 128, 64}, {"spring256", 256, 64},
-                        {"lorenz_dop853", 0, 1}, {"vdp_dop853", 0, 1}};
-  static_assert(sizeof specs / sizeof specs[0] == kVdpDop853 + 1, "pool indices");
+                        {"lorenz_dop853", 0, 1}, {"vdp_dop853", 0, 1},
+                        {"lorenz_event", 0, 1},  {"lorenz_cross", 0, 1},
+                        {"vdp_event", 0, 1},     {"vdp_cross", 0, 1}};
+  static_assert(sizeof specs / sizeof specs[0] == kVdpCross + 1, "pool indices");
   s->fams.resize(sizeof specs / sizeof specs[0]);
   const int lane_slots = ncu * 4 * 256;  // 4 blocks of 256 lanes per CU
+  const int event_slots = ncu * 256;     // event requests: one block per CU
   const int group_lanes = ncu * 2 * 256;
   bool ok = true;
   for (size_t k = 0; k < s->fams.size() && ok; ++k) {
@@ -643,6 +704,18 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
         f.launch = launch_lane<de::Dop853Lane<de::LorenzRhs, true>>; break;
       case kVdpDop853: ok = make_pool<LaneSlot<de::Dop853Lane<de::VanDerPolRhs, true>>>(f, lane_slots);
         f.launch = launch_lane<de::Dop853Lane<de::VanDerPolRhs, true>>; break;
+      case kLorenzEvent:
+        ok = make_pool<LaneSlot<de::Dopri5Lane<de::LorenzRhs, true, false, 1>>>(f, event_slots);
+        f.launch = launch_lane<de::Dopri5Lane<de::LorenzRhs, true, false, 1>>; break;
+      case kLorenzCross:
+        ok = make_pool<LaneSlot<de::Dopri5Lane<de::LorenzRhs, false, false, 2>>>(f, event_slots);
+        f.launch = launch_lane<de::Dopri5Lane<de::LorenzRhs, false, false, 2>>; break;
+      case kVdpEvent:
+        ok = make_pool<LaneSlot<de::Dopri5Lane<de::VanDerPolRhs, true, false, 1>>>(f, event_slots);
+        f.launch = launch_lane<de::Dopri5Lane<de::VanDerPolRhs, true, false, 1>>; break;
+      case kVdpCross:
+        ok = make_pool<LaneSlot<de::Dopri5Lane<de::VanDerPolRhs, false, false, 2>>>(f, event_slots);
+        f.launch = launch_lane<de::Dopri5Lane<de::VanDerPolRhs, false, false, 2>>; break;
       case 2: ok = make_pool<LaneSlot<de::GroupDopri5<4, 1>>>(f, group_lanes);
         f.launch = launch_group<4, 1>; break;
       case 3: ok = make_pool<LaneSlot<de::GroupDopri5<8, 1>>>(f, group_lanes);
@@ -699,10 +772,27 @@ int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
     return DE_ERR_UNSUPPORTED;
   if (b->n <= 0 || !b->y0 || (b->n_eval > 0 && !b->t_eval)) return DE_ERR_INVALID;
   // Served requests carry scalar or per-trajectory tolerances and the final
-  // state / t_eval output; per-component vectors and event solouts go through
-  // de_ode_batch_solve.
+  // state / t_eval output, or (DOPRI5 on Lorenz and Van der Pol) an event
+  // solout, validated as de_ode_batch_solve validates it; per-component
+  // vectors go through de_ode_batch_solve.
   if (b->rtol_comp || b->atol_comp) return DE_ERR_UNSUPPORTED;
-  if (b->event_solout != DE_SOLOUT_DEFAULT) return DE_ERR_UNSUPPORTED;
+  const int ev_fn = b->event_solout == DE_SOLOUT_CROSSING     ? DE_EVENT_COMPONENT
+                    : b->event_solout == DE_SOLOUT_HYPERPLANE ? DE_EVENT_PLANE
+                                                               : b->event_fn;
+  if (b->event_solout != DE_SOLOUT_DEFAULT) {
+    if (b->family == DE_FAMILY_SPRING_CHAIN || b->method != DE_METHOD_DOPRI5)
+      return DE_ERR_UNSUPPORTED;
+    if (b->event_solout < DE_SOLOUT_EVENT || b->event_solout > DE_SOLOUT_HYPERPLANE)
+      return DE_ERR_INVALID;
+    if (b->event_solout != DE_SOLOUT_EVENT && b->n_eval > 0) return DE_ERR_INVALID;
+    if (b->event_direction < DE_CROSS_BOTH || b->event_direction > DE_CROSS_NEGATIVE)
+      return DE_ERR_INVALID;
+    if (b->event_terminate < 0 || b->max_events < 0) return DE_ERR_INVALID;
+    if (ev_fn < DE_EVENT_COMPONENT || ev_fn > DE_EVENT_PLANE) return DE_ERR_INVALID;
+    if (ev_fn == DE_EVENT_COMPONENT && (b->event_component < 0 || b->event_component >= b->dim))
+      return DE_ERR_INVALID;
+    if (ev_fn == DE_EVENT_PLANE && (!b->event_point || !b->event_normal)) return DE_ERR_INVALID;
+  }
   const bool ragged = b->family == DE_FAMILY_SPRING_CHAIN;
   if (ragged && (!b->sizes || !b->state_offset || !b->param_offset)) return DE_ERR_INVALID;
   if (!ragged && b->dim != (b->family == DE_FAMILY_LORENZ ? 3 : 2)) return DE_ERR_INVALID;
@@ -731,6 +821,23 @@ int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
   if (b->rtol_traj) r->rtol.assign(b->rtol_traj, b->rtol_traj + n);
   if (b->atol_traj) r->atol.assign(b->atol_traj, b->atol_traj + n);
   if (b->n_eval > 0) r->t_eval.assign(b->t_eval, b->t_eval + b->n_eval);
+  if (b->event_solout != DE_SOLOUT_DEFAULT) {
+    if (b->event_value_traj) r->ev_value.assign(b->event_value_traj, b->event_value_traj + n);
+    if (ev_fn == DE_EVENT_PLANE) {
+      r->ev_point.assign(b->event_point, b->event_point + b->dim);
+      r->ev_normal.assign(b->event_normal, b->event_normal + b->dim);
+      if (b->event_solout == DE_SOLOUT_HYPERPLANE) {
+        // HyperplaneCrossingSolout::new (src/solout/hyperplane.rs:126-132): the
+        // normal times 1/|normal| when |normal| > f64::EPSILON, |normal| = sqrt
+        // of the squares summed from zero (the batch path's operations).
+        double sq = 0.0;
+        for (double v : r->ev_normal) sq = sq + v * v;
+        const double norm = sqrt(sq), inv = 1.0 / norm;
+        if (norm > 2.220446049250313080847e-16)
+          for (double& v : r->ev_normal) v = v * inv;
+      }
+    }
+  }
   r->t_submit = now_s();
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->error) {
@@ -766,6 +873,11 @@ int de_server_poll(de_server* s, uint64_t ticket, de_ode_result* out, int32_t wa
     if (out->status) memcpy(out->status, r->status.data(), n * 4);
     if (out->dense && !r->dense.empty()) memcpy(out->dense, r->dense.data(), r->dense.size() * 8);
     if (out->n_dense && !r->n_dense.empty()) memcpy(out->n_dense, r->n_dense.data(), n * 4);
+    if (out->event_t && !r->event_t.empty())
+      memcpy(out->event_t, r->event_t.data(), r->event_t.size() * 8);
+    if (out->event_y && !r->event_y.empty())
+      memcpy(out->event_y, r->event_y.data(), r->event_y.size() * 8);
+    if (out->n_events && !r->n_events.empty()) memcpy(out->n_events, r->n_events.data(), n * 4);
   }
   return 1;
 }

@@ -162,13 +162,17 @@ def test_events_with_per_component_tolerances_and_fast_flag(orc):
     assert_events_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS))
 
 
-def test_server_refuses_event_requests():
-    # Served requests carry the final-state / t_eval output only.
+def test_server_refuses_event_requests_it_does_not_build():
+    # Served event solouts: DOPRI5 on Lorenz and Van der Pol; DOP853 with an
+    # event, and an out-of-range component, are refused.
     p, ens = lorenz(16, 1.0, seed=67)
-    ens = ens.method(method("dopri5")).event(Event.component(2, 20.0))
+    ens = ens.method(method("dop853")).event(Event.component(2, 20.0))
     with deamd.Server(8 << 30) as srv:
         with pytest.raises(RuntimeError):
             srv.submit(ens)
+        p, bad = lorenz(16, 1.0, seed=67)
+        with pytest.raises(RuntimeError):
+            srv.submit(bad.method(method("dopri5")).crossing(3, 0.0))
 
 
 @pytest.mark.parametrize("name", ["dopri5", "dop853", "rk4"])

@@ -150,3 +150,45 @@ def test_server_refuses_dop853_spring_chains():
     with Server(8 * GIB) as srv:
         with pytest.raises(RuntimeError):
             srv.submit(req)
+
+
+def _event_fields(a, b):
+    assert np.array_equal(a.n_events, b.n_events)
+    for f in ("event_t", "event_y"):
+        x, y = getattr(a, f), getattr(b, f)
+        assert np.array_equal(np.isnan(x), np.isnan(y)) and \
+            np.array_equal(np.nan_to_num(x), np.nan_to_num(y)), f
+
+
+def test_served_event_solouts_match_direct_solves():
+    # DOPRI5 with an event solout on the lane-mapped families: served with
+    # inline roots, the same bits as a direct solve (deferred roots), alone
+    # and inside a mixed stream.
+    from deamd import CrossingDirection, Event, EventConfig
+    p, y0 = deamd.gen_lorenz(1500, seed=31)
+    base = lambda: IVP.ode(Lorenz(*p), 0.0, 6.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(1000000))
+    mu, v0 = deamd.gen_vdp(1200, seed=32)
+    reqs = [
+        base().event(Event.component(2, p[1] - 1.0)).options(max_events=16),
+        base().t_eval(np.linspace(0.0, 6.0, 13)).event(
+            Event.plane([0.0, 0.0, 20.0], [0.2, -0.1, 1.0], EventConfig().terminate_after(3)))
+        .options(max_events=4),
+        base().crossing(0, 0.0, CrossingDirection.Both).options(max_events=12),
+        base().hyperplane_crossing([0.0, 0.0, 25.0], [1.0, 1.0, 3.0],
+                                   CrossingDirection.Positive).options(max_events=12),
+        IVP.ode(VanDerPol(*mu), 0.0, 10.0, v0).method(
+            ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9).max_steps(1000000))
+        .crossing(0, 0.0, CrossingDirection.Negative).options(max_events=8),
+    ]
+    wants = [r.solve() for r in reqs]
+    assert all(w.n_events.sum() > 0 for w in wants)
+    with Server(8 * GIB) as srv:
+        alone = srv.poll(srv.submit(reqs[0]))
+        ts = [srv.submit(lorenz_req(900, seed=33))] + [srv.submit(r) for r in reqs]
+        got = [srv.poll(t) for t in ts]
+    for g, w in zip([alone] + got[1:], [wants[0]] + wants):
+        assert_same(g, w)
+        _event_fields(g, w)
+    assert np.array_equal(got[2].n_dense, wants[1].n_dense)
+    assert np.array_equal(np.nan_to_num(got[2].dense), np.nan_to_num(wants[1].dense))
