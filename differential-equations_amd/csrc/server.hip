@@ -190,6 +190,11 @@ chunk_group(Queue* q, const Range* ranges,
   if (valid) slots[slot] = S;
 }
 
+__global__ void gather_remaining(const ReqDev* reqs, int32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = reqs[i].remaining;
+}
+
 template <class Slot>
 __global__ void init_slots(Slot* s, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -315,6 +320,7 @@ struct de_server {
   de::serve::ReqDev* d_reqs = nullptr;
   std::vector<de::serve::ReqDev> h_reqs;
   int32_t* h_remaining = nullptr;  // pinned mirror
+  int32_t* d_remaining = nullptr;  // gathered from d_reqs for one copy per iteration
   std::vector<int> free_slots;
   std::map<int, Request*> active;  // request slot -> request
   std::deque<Request*> pending;
@@ -611,10 +617,13 @@ void scheduler(de_server* s) {
     } else {
       launch_all(s, true);
     }
-    // Remaining counts of the active requests.
-    for (auto& kv : s->active)
-      (void)hipMemcpyAsync(&s->h_remaining[kv.first], &s->d_reqs[kv.first].remaining, 4,
-                           hipMemcpyDeviceToHost, s->ctl);
+    // Remaining counts of the request slots up to the highest active one:
+    // gathered on the device, then one copy (not one per request).
+    const int span = s->active.rbegin()->first + 1;
+    hipLaunchKernelGGL(gather_remaining, dim3((span + 255) / 256), dim3(256), 0, s->ctl,
+                       (const de::serve::ReqDev*)s->d_reqs, s->d_remaining, span);
+    (void)hipMemcpyAsync(s->h_remaining, s->d_remaining, sizeof(int32_t) * span,
+                         hipMemcpyDeviceToHost, s->ctl);
     hipError_t e = hipStreamSynchronize(s->ctl);
     lk.lock();
     s->iterations += 1;
@@ -737,6 +746,7 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
   for (auto& f : s->fams) pools += f.slot_bytes * f.nslots + sizeof(Range) * kMaxRanges;
   ok = ok && hipMalloc(&s->d_reqs, sizeof(de::serve::ReqDev) * kMaxReq) == hipSuccess;
   ok = ok && hipHostMalloc(&s->h_remaining, sizeof(int32_t) * kMaxReq, 0) == hipSuccess;
+  ok = ok && hipMalloc(&s->d_remaining, sizeof(int32_t) * kMaxReq) == hipSuccess;
   ok = ok && hipStreamCreateWithFlags(&s->ctl, hipStreamNonBlocking) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) == hipSuccess;
   s->fam_done.resize(s->fams.size());
@@ -910,6 +920,7 @@ void de_server_destroy(de_server* s) {
   if (s->arena.base) (void)hipFree(s->arena.base);
   if (s->d_reqs) (void)hipFree(s->d_reqs);
   if (s->h_remaining) (void)hipHostFree(s->h_remaining);
+  if (s->d_remaining) (void)hipFree(s->d_remaining);
   for (auto& e : s->fam_done)
     if (e) (void)hipEventDestroy(e);
   if (s->fork) (void)hipEventDestroy(s->fork);
