@@ -348,7 +348,7 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
                                  (size_t)slice * (size_t)bmax * (Lane::kEntryWidth + kResWidth) *
                                      sizeof(double)
                            : 0;
-    if (hipMallocAsync(&work, args_bytes + ev_bytes + order_bytes + defer_bytes, s) == hipSuccess)
+    if (workspace_alloc(&work, args_bytes + ev_bytes + order_bytes + defer_bytes, s) == hipSuccess)
       break;
     (void)hipGetLastError();
     if (bmax == 0) return -1;
