@@ -261,7 +261,7 @@ def cpu_baseline_lorenz(args, p, y0, n):
     walls = []
     for _ in range(3):   # median of 3 repeats
         t0 = time.perf_counter()
-        sol = oracle.solve_ode(ens, pow_mode=oracle.POW_LIBM, threads=threads)
+        sol = oracle.solve_ode(ens, pow_mode=oracle.POW_HOST, threads=threads)
         walls.append(time.perf_counter() - t0)
     dt = sorted(walls)[1]
     return {"value": float(sol.accepted.sum()) / dt, "unit": "accepted_steps*traj/s",
@@ -430,7 +430,7 @@ def cpu_baseline_erk(args, p, y0, n):
         ens = deamd.IVP.ode(deamd.Lorenz(*p[:, idx]), 0.0, 50.0, y0[:, idx]).method(
             getattr(deamd.ExplicitRungeKutta, name)().rtol(1e-8).atol(1e-10).max_steps(1000000))
         t0 = time.perf_counter()
-        sol = oracle.solve_ode(ens, pow_mode=oracle.POW_LIBM, threads=threads)
+        sol = oracle.solve_ode(ens, pow_mode=oracle.POW_HOST, threads=threads)
         dt += time.perf_counter() - t0
         A += int(sol.accepted.sum())
     return {"value": A / dt, "unit": "accepted_steps*traj/s", "cores": threads, "kind": "port",

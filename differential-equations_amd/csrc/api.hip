@@ -751,7 +751,7 @@ namespace {
 
 __global__ void diag_pow_kernel(const double* x, const double* y, double* out, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = de::pow(x[i], y[i]);
+  if (i < n) out[i] = de::libm_pow(x[i], y[i]);
 }
 
 __global__ void diag_arith_kernel(const double* a, const double* b, const double* c,

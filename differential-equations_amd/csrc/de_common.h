@@ -36,6 +36,17 @@
 #define DE_TABLE static const
 #endif
 
+// Code whose every product and sum must be rounded on its own (error-free
+// transformations, operation sequences fixed for bit parity), also in
+// translation units compiled with FMA contraction.
+#if defined(__clang__)
+#define DE_NO_CONTRACT _Pragma("clang fp contract(off)")
+#define DE_FORCE_INLINE __attribute__((always_inline))
+#else
+#define DE_NO_CONTRACT
+#define DE_FORCE_INLINE
+#endif
+
 namespace de {
 
 // f64::EPSILON * 10, the end-of-interval threshold of the solve drivers

@@ -27,11 +27,11 @@
 #include "de_dop853_table.h"
 #include "de_event.h"
 #include "de_families.h"
-#include "de_pow.h"
+#include "de_pow_libm.h"
 
-// Controller pow: the project pow (correctly rounded but for rare cases, and
-// cheaper than the device libm pow for the -0.2 exponent) in both builds.
-#define DE_CTRL_POW(x, y) ::de::pow((x), (y))
+// Controller and initial-step pow: the C library's pow that f64::powf calls,
+// restated with the same roundings (de_pow_libm.h), in both builds.
+#define DE_CTRL_POW(x, y) ::de::libm_pow((x), (y))
 
 namespace de {
 

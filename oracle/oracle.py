@@ -15,8 +15,8 @@ sys.path.insert(0, os.path.join(REPO, "differential-equations_amd"))
 from deamd import _abi  # noqa: E402
 
 LIB_PATH = os.path.join(HERE, "_build", "libde_oracle.so")
-POW_PROJECT = 0
-POW_LIBM = 1
+POW_RESTATED = 0   # de::libm_pow, the pow the GPU kernels run
+POW_HOST = 1       # this host's C library pow (what the reference binary calls)
 
 _lib = None
 
@@ -45,11 +45,8 @@ def load():
         lib.de_oracle_dop853_tableau.argtypes = [C.c_void_p] * 6
         lib.de_oracle_set_generic_dopri5.restype = None
         lib.de_oracle_set_generic_dopri5.argtypes = [C.c_int]
-        lib.de_oracle_pow_root_fast.restype = None
-        lib.de_oracle_pow_root_fast.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
-                                                C.c_int64]
-        lib.de_oracle_pow_m02_fast.restype = None
-        lib.de_oracle_pow_m02_fast.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        lib.de_oracle_host_pow_many.restype = None
+        lib.de_oracle_host_pow_many.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         lib.de_oracle_noise_normal.restype = C.c_double
         lib.de_oracle_noise_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
         lib.de_oracle_philox.restype = None
@@ -121,7 +118,7 @@ def ode_csv(ensemble, solution, source):
     return out.raw[:int(offsets[-1])], offsets
 
 
-def solve_ode(ensemble, pow_mode=POW_PROJECT, threads=1):
+def solve_ode(ensemble, pow_mode=POW_RESTATED, threads=1):
     """Run an deamd.OdeEnsemble through the oracle; returns a deamd.Solution."""
     b, res, keep = ensemble.batch()
     rc = load().de_oracle_ode_solve(C.byref(b), C.byref(res.struct()), pow_mode, threads)
@@ -138,38 +135,20 @@ def solve_sde(ensemble, threads=1):
     return res
 
 
-def pow_many(x, y):
+def pow_many(x, y, host=False):
+    """de::libm_pow (host=False) or this host's C library pow (host=True)."""
     import numpy as np
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
     out = np.empty_like(x)
-    load().de_oracle_pow_many(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    fn = load().de_oracle_host_pow_many if host else load().de_oracle_pow_many
+    fn(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
     return out
 
 
 def set_generic_dopri5(on):
     """Run DOPRI5 through the generic Dormand-Prince restatement (test switch)."""
     load().de_oracle_set_generic_dopri5(1 if on else 0)
-
-
-def pow_root_fast(k, x):
-    """The k-th-root path of pow(x, -fl(1/k)): (values, certified) arrays."""
-    import numpy as np
-    x = np.ascontiguousarray(x, dtype=np.float64)
-    out = np.empty_like(x)
-    ok = np.zeros(x.size, dtype=np.uint8)
-    load().de_oracle_pow_root_fast(k, x.ctypes.data, out.ctypes.data, ok.ctypes.data, x.size)
-    return out, ok.astype(bool)
-
-
-def pow_m02_fast(x):
-    """First try of pow(x, -0.2): (values, certified flags)."""
-    import numpy as np
-    x = np.ascontiguousarray(x, dtype=np.float64)
-    out = np.empty_like(x)
-    ok = np.empty(x.shape, dtype=np.uint8)
-    load().de_oracle_pow_m02_fast(x.ctypes.data, out.ctypes.data, ok.ctypes.data, x.size)
-    return out, ok.astype(bool)
 
 
 def philox(ctr, k0, k1):

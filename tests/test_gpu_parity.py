@@ -42,16 +42,17 @@ def assert_ode_equal(got, want, dense=False):
 
 # ------------------------------------------------------- device arithmetic --
 def test_device_pow_matches_host(orc):
+    # The device pow against its host build and against this host's C library
+    # pow (the reference binary's powf) over every regime of arguments.
+    from test_oracle_math import _host_pow_is_glibc_fma, _pow_arguments
     lib = deamd.lib()
     rng = np.random.default_rng(1)
-    n = 1 << 20
-    x = np.concatenate([np.exp(rng.uniform(np.log(1e-8), np.log(1e4), n)),
-                        np.exp(rng.uniform(np.log(1e-300), np.log(1e300), n // 4))])
-    for y in (-0.2, 0.2, -1.0 / 6.0, -1.0 / 7.0, -0.125, -1.0 / 9.0):
-        yy = np.full_like(x, y)
-        out = np.empty_like(x)
-        deamd._abi.check(lib.de_diag_pow(deamd._ptr(x), deamd._ptr(yy), deamd._ptr(out), len(x)))
-        assert np.array_equal(out, orc.pow_many(x, y))
+    x, y = _pow_arguments(rng, 1 << 18)
+    out = np.empty_like(x)
+    deamd._abi.check(lib.de_diag_pow(deamd._ptr(x), deamd._ptr(y), deamd._ptr(out), len(x)))
+    assert same(out, orc.pow_many(x, y))
+    if _host_pow_is_glibc_fma():
+        assert same(out, orc.pow_many(x, y, host=True))
 
 
 def test_device_division_sqrt_fma_are_ieee():
@@ -109,6 +110,42 @@ def test_lorenz_c1_bit_exact(c1_oracle, schedule):
     got = lorenz_c1().options(schedule=schedule).solve()
     assert (got.status == DE_STATUS_COMPLETE).all()
     assert_ode_equal(got, c1_oracle)
+
+
+def test_lorenz_c1_equals_reference_arithmetic(orc):
+    # The oracle with this host's C library pow is the reference binary's
+    # arithmetic; the GPU matches it bit for bit on every trajectory.
+    from test_oracle_math import _host_pow_is_glibc_fma
+    if not _host_pow_is_glibc_fma():
+        pytest.skip("host pow is not glibc's fused x86-64 build")
+    want = orc.solve_ode(lorenz_c1(), pow_mode=orc.POW_HOST, threads=THREADS)
+    assert_ode_equal(lorenz_c1().solve(), want)
+
+
+def test_lorenz_c2_full_ensemble(orc):
+    # The 1 048 576-trajectory Lorenz ensemble over t in [0, 50] (rho ~ U[0, 50],
+    # rtol 1e-8, atol 1e-10), solved whole on the GPU: every 64th trajectory
+    # bit for bit against the oracle with this host's C library pow (the
+    # reference binary's arithmetic), and on every trajectory Complete,
+    # |t - tf| <= 10 eps and evals = 3 + 6 (A + R) + A.
+    from test_oracle_math import _host_pow_is_glibc_fma
+    n = 1 << 20
+    p, y0 = deamd.gen_lorenz(n)
+    mk = lambda p, y0: IVP.ode(Lorenz(*p), 0.0, 50.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(1000000))
+    got = mk(p, y0).solve()
+    assert (got.status == DE_STATUS_COMPLETE).all()
+    assert np.all(np.abs(got.t - 50.0) <= 10 * np.finfo(float).eps)
+    assert np.array_equal(got.evals, 3 + 6 * (got.accepted + got.rejected) + got.accepted)
+    idx = np.arange(0, n, 64)
+    mode = orc.POW_HOST if _host_pow_is_glibc_fma() else orc.POW_RESTATED
+    want = orc.solve_ode(mk(p[:, idx], y0[:, idx]), pow_mode=mode, threads=THREADS)
+    assert np.array_equal(got.status[idx], want.status)
+    assert np.array_equal(got.accepted[idx], want.accepted)
+    assert np.array_equal(got.rejected[idx], want.rejected)
+    assert np.array_equal(got.evals[idx], want.evals)
+    assert same(got.t[idx], want.t)
+    assert same(got.y.reshape(3, n)[:, idx].ravel(), want.y)
 
 
 def test_lorenz_persistent_large_batch_matches_plain():

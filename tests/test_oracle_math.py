@@ -15,8 +15,9 @@ fails it:
     on Lorenz trajectories, using the C library pow as the reference does;
   * SDE: closed-form recurrences for linear SDEs, Milstein == Euler-Maruyama
     for additive noise, the clipped last step of [0, 1] at dt = 1e-3;
-  * Philox4x32-10 known-answer vectors (Random123), pow correctly rounded
-    against mpmath, normal-transform moments.
+  * Philox4x32-10 known-answer vectors (Random123), normal-transform moments;
+  * pow: equal to the host C library's pow (glibc) bit for bit, under 0.52 ulp
+    against mpmath, and the IEEE special values.
 """
 import math
 from fractions import Fraction as Fr
@@ -296,111 +297,102 @@ def test_philox_known_answers(orc, ctr, k0, k1, want):
     assert orc.philox(ctr, k0, k1).tolist() == want
 
 
-def _near_midpoint(x, y, got, want, mpmath):
-    """|got - want| is one ulp and the exact x^y lies within 2^-74 (relative)
-    of the midpoint between them: a hard case the double-double path (about
-    2^-75 accurate before its rounding) may round either way."""
-    if abs(got - want) != abs(np.nextafter(want, got) - want):
+def _host_pow_is_glibc_fma():
+    """glibc >= 2.28 on x86-64 with FMA runs the fused build of the table-driven
+    pow that de::libm_pow restates; other C libraries round differently."""
+    import platform
+    lib, ver = platform.libc_ver()
+    if platform.machine() != "x86_64" or lib != "glibc":
         return False
-    exact = mpmath.power(mpmath.mpf(float(x)), mpmath.mpf(y))
-    mid = (mpmath.mpf(float(got)) + mpmath.mpf(float(want))) / 2
-    return abs(exact - mid) / exact < mpmath.mpf(2) ** -74
+    if tuple(int(v) for v in ver.split(".")[:2]) < (2, 28):
+        return False
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = f.read()
+    except OSError:
+        return False
+    return " fma" in flags and " avx2" in flags
 
 
-@pytest.mark.parametrize("y", [-0.2, 0.2, -1.0 / 6.0, -1.0 / 7.0, -0.125, -1.0 / 9.0])
-def test_pow_is_correctly_rounded(orc, y):
-    # Correctly rounded but for hard cases: x^y within 2^-74 of a rounding
-    # midpoint, which happen systematically just below 1 (x = 1 - m 2^-53:
-    # x^(-1/k) = 1 + m 2^-53 / k + O(2^-106 m^2) lands on a midpoint to first
-    # order for m in one residue class mod 2k) and otherwise about once in
-    # 2^20 arguments; there the double-double path may miss by one ulp.
+def _pow_arguments(rng, n):
+    """Arguments over every regime of pow: the controllers' exponents on any
+    positive x, subnormal x, x near 1, results near overflow and underflow,
+    negative x with integer y, and raw bit patterns."""
+    bits = lambda: rng.integers(0, 2 ** 63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2) \
+        + rng.integers(0, 2, n, dtype=np.int64).astype(np.uint64)
+    xs, ys = [], []
+    ctrl = np.array([-0.2, 0.2, -1.0 / 6.0, -1.0 / 7.0, -0.125, -1.0 / 9.0, 0.125])
+    xs.append(np.exp(rng.uniform(np.log(1e-300), np.log(1e300), n)))
+    ys.append(rng.choice(ctrl, n))
+    xs.append(1.0 + rng.integers(-2 ** 40, 2 ** 40, n) * 2.0 ** -60)
+    ys.append(rng.choice(ctrl, n))
+    xs.append(rng.integers(1, 2 ** 52, n).astype(np.float64) * 2.0 ** -1074)
+    ys.append(rng.uniform(-3, 3, n))
+    xs.append(np.exp(rng.uniform(-700, 700, n)))
+    ys.append(rng.uniform(-4, 4, n) * np.where(rng.integers(0, 2, n) == 1, 1.0, 200.0))
+    xs.append(-np.exp(rng.uniform(-20, 20, n)))
+    ys.append(rng.integers(-60, 60, n).astype(np.float64))
+    xs.append(bits().view(np.float64))
+    ys.append(bits().view(np.float64))
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+@pytest.mark.skipif(not _host_pow_is_glibc_fma(), reason="host pow is not glibc's fused x86-64 build")
+def test_pow_equals_host_c_library(orc):
+    # de::libm_pow (run by the GPU kernels and oracle mode 0) against the
+    # C library pow that a Rust build of the reference calls on this host.
+    rng = np.random.default_rng(2026)
+    x, y = _pow_arguments(rng, 400000)
+    special = np.array([0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, 3.0, math.inf, -math.inf,
+                        math.nan, 5e-324, -5e-324, 2.2250738585072014e-308, 1.7976931348623157e308,
+                        1e-300, 1e300, 0.2, -0.2, 2.0 ** -66, 2.0 ** 63, -(2.0 ** 63),
+                        1.0000000000000002, 0.9999999999999999])
+    sx, sy = np.meshgrid(special, special)
+    x = np.concatenate([x, sx.ravel()])
+    y = np.concatenate([y, sy.ravel()])
+    got = orc.pow_many(x, y)
+    want = orc.pow_many(x, y, host=True)
+    same = (got.view(np.uint64) == want.view(np.uint64)) | (np.isnan(got) & np.isnan(want))
+    bad = np.flatnonzero(~same)
+    assert bad.size == 0, [(x[i].hex(), y[i].hex(), got[i].hex(), want[i].hex()) for i in bad[:5]]
+
+
+@pytest.mark.parametrize("y", [-0.2, 0.2, -1.0 / 6.0, -0.125, -1.0 / 9.0, 2.5])
+def test_pow_error_below_0_52_ulp(orc, y):
+    # The method's documented accuracy: under 0.52 ulp everywhere (not
+    # correctly rounded).  A wrong table entry, ln2 split or polynomial
+    # coefficient costs at least the subinterval's share of accuracy and
+    # shows up here independently of any C library.
     mpmath = pytest.importorskip("mpmath")
     mpmath.mp.prec = 160
-    rng = np.random.default_rng(42 if y < 0 else 43)
-    xs = np.concatenate([np.exp(rng.uniform(np.log(1e-7), np.log(5e3), 6000)),
-                         np.exp(rng.uniform(np.log(1e-300), np.log(1e300), 2000)),
-                         1.0 + (np.arange(-500, 500) * 2.0 ** -52),
-                         1.0 - (np.arange(1, 400) * 2.0 ** -53),
-                         [5e-324, 1e-310, 2.2250738585072014e-308, 1.7976931348623157e308]])
+    rng = np.random.default_rng(7 + int(100 * abs(y)))
+    lim = min(460.0, 700.0 / abs(y))  # keep x^y finite and normal
+    xs = np.concatenate([np.exp(rng.uniform(-lim, lim, 3000)),
+                         1.0 + rng.integers(-2 ** 45, 2 ** 45, 1000) * 2.0 ** -52,
+                         # every log subinterval (top 7 bits of bits(x) - 0x3fe6955500000000)
+                         (0x3fe6955500000000 + (np.arange(128, dtype=np.uint64) << np.uint64(45))
+                          + np.uint64(12345)).view(np.float64)])
     got = orc.pow_many(xs, y)
-    libm_mis = hard_random = 0
-    for n, (x, g) in enumerate(zip(xs, got)):
-        want = float(mpmath.power(mpmath.mpf(float(x)), mpmath.mpf(y)))
-        if g != want:
-            assert _near_midpoint(x, y, g, want, mpmath), (x, g, want)
-            hard_random += n < 8000
-        libm_mis += math.pow(float(x), y) != want
-    assert hard_random <= 2  # away from 1, hard cases are rare
-    # The C library's pow is not correctly rounded; record that it differs rarely.
-    assert libm_mis < 0.01 * len(xs)
-
-
-def test_pow_fifth_root_path_certifies_only_correct_roundings(orc):
-    # pow(x, -0.2) first tries a table-driven fifth root and keeps it only when
-    # its rounding test certifies the correctly rounded value; the test is
-    # designed to fail for about 2^-12 of arguments, which then take the
-    # double-double path.  Check every failure and a sample of successes
-    # against a 160-bit evaluation, and the rate of failures.
-    mpmath = pytest.importorskip("mpmath")
-    mpmath.mp.prec = 160
-    rng = np.random.default_rng(44)
-    xs = np.concatenate([np.exp(rng.uniform(np.log(1e-12), np.log(1e6), 150000)),
-                         2.0 ** rng.uniform(-1022, 1023, 50000)])
-    fast, ok = orc.pow_m02_fast(xs)
-    full = orc.pow_many(xs, -0.2)
-    assert 0 < (~ok).sum() < 1e-3 * len(xs)
-    assert np.array_equal(full[ok], fast[ok])
-    cr = lambda x: float(mpmath.power(mpmath.mpf(float(x)), mpmath.mpf(-0.2)))
-    for x, g in zip(xs[~ok], full[~ok]):
-        assert g == cr(x), x
-    idx = np.flatnonzero(ok)[:: max(1, ok.sum() // 3000)]
-    for x, g in zip(xs[idx], fast[idx]):
-        assert g == cr(x), x
-    # Zero, subnormal, infinite, NaN and negative arguments are never certified.
-    _, ok_special = orc.pow_m02_fast(np.array([0.0, 5e-324, 1e-310, math.inf, math.nan, -1.0, -0.0]))
-    assert not ok_special.any()
-    _, ok_edges = orc.pow_m02_fast(np.array([2.2250738585072014e-308, 1.7976931348623157e308, 1.0]))
-    assert ok_edges.all()
-
-
-@pytest.mark.parametrize("k", [6, 7, 8, 9])
-def test_pow_kth_root_paths_certify_only_correct_roundings(orc, k):
-    # The exponents -fl(1/k) of the order-6 .. 9 controllers take the same
-    # table-driven path as -0.2, with their own tables and delta: every
-    # uncertified argument and a sample of certified ones against a 160-bit
-    # evaluation, the failure rate, and agreement of the certified values
-    # with pow itself.
-    mpmath = pytest.importorskip("mpmath")
-    mpmath.mp.prec = 160
-    y = -1.0 / k
-    rng = np.random.default_rng(50 + k)
-    xs = np.concatenate([np.exp(rng.uniform(np.log(1e-12), np.log(1e6), 60000)),
-                         2.0 ** rng.uniform(-1022, 1023, 20000)])
-    fast, ok = orc.pow_root_fast(k, xs)
-    full = orc.pow_many(xs, y)
-    assert 0 < (~ok).sum() < 1e-3 * len(xs)
-    assert np.array_equal(full[ok], fast[ok])
-    cr = lambda x: float(mpmath.power(mpmath.mpf(float(x)), mpmath.mpf(y)))
-    for x, g in zip(xs[~ok], full[~ok]):
-        assert g == cr(x), x
-    idx = np.flatnonzero(ok)[:: max(1, ok.sum() // 2000)]
-    for x, g in zip(xs[idx], fast[idx]):
-        assert g == cr(x), (k, x)
-    _, ok_special = orc.pow_root_fast(k, np.array([0.0, 5e-324, 1e-310, math.inf, math.nan,
-                                                   -1.0, -0.0]))
-    assert not ok_special.any()
-    edges = np.array([2.2250738585072014e-308, 1.7976931348623157e308, 1.0, 2.0 ** -1022])
-    v, ok_edges = orc.pow_root_fast(k, edges)
-    assert ok_edges.all() and all(g == cr(x) for x, g in zip(edges, v))
+    worst = 0.0
+    for x, g in zip(xs, got):
+        exact = mpmath.power(mpmath.mpf(float(x)), mpmath.mpf(y))
+        ulp = mpmath.mpf(float(np.spacing(abs(g))))
+        worst = max(worst, float(abs(mpmath.mpf(float(g)) - exact) / ulp))
+    assert worst < 0.52, worst
 
 
 def test_pow_special_cases(orc):
-    assert orc.pow_many(np.array([0.0]), -0.2)[0] == math.inf
-    assert orc.pow_many(np.array([0.0]), 0.2)[0] == 0.0
-    assert orc.pow_many(np.array([math.inf]), -0.2)[0] == 0.0
-    assert orc.pow_many(np.array([1.0]), -0.2)[0] == 1.0
-    assert math.isnan(orc.pow_many(np.array([math.nan]), -0.2)[0])
-    assert math.isnan(orc.pow_many(np.array([-1.0]), -0.2)[0])
+    # C99 Annex F / IEEE 754 pow special values.
+    cases = [(0.0, -0.2, math.inf), (0.0, 0.2, 0.0), (math.inf, -0.2, 0.0), (math.inf, 0.2, math.inf),
+             (1.0, -0.2, 1.0), (1.0, math.nan, 1.0), (math.nan, 0.0, 1.0), (-8.0, 3.0, -512.0),
+             (-0.0, -3.0, -math.inf), (-0.0, 3.0, -0.0), (-2.0, -1.0, -0.5), (0.5, math.inf, 0.0),
+             (2.0, -math.inf, 0.0), (-1.0, math.inf, 1.0), (2.0, 1024.0, math.inf),
+             (2.0, -1075.0, 0.0), (2.0, -1074.0, 5e-324), (4.0, 0.5, 2.0)]
+    for x, y, want in cases:
+        got = orc.pow_many(np.array([x]), y)[0]
+        assert got == want and math.copysign(1, got) == math.copysign(1, want), (x, y, got)
+    for x, y in [(math.nan, -0.2), (-1.0, -0.2), (-2.0, 0.5), (2.0, math.nan)]:
+        assert math.isnan(orc.pow_many(np.array([x]), y)[0])
 
 
 def test_noise_normals_are_standard_normal(orc):
