@@ -95,21 +95,30 @@ enum Status : int32_t {
   kRunning = -1,
 };
 
-// validate_step_size_parameters (src/utils.rs:60-157).  Returns true when the
-// parameters are valid; false corresponds to Err(Error::BadInput).
+// validate_step_size_parameters (src/utils.rs:60-157): 0 when the parameters
+// are valid, else the number of the first failed check in the reference's
+// order (each an Err(Error::BadInput) with its own message):
+//   1 tf == t0              2 sign(h0) != sign(tf - t0)   3 h_min < 0
+//   4 h_max < 0             5 h_min > h_max                6 |h0| < h_min
+//   7 |h0| > h_max          8 |h0| > |tf - t0|             9 h0 == 0
+DE_HD int validate_step_size_check(double h0, double h_min, double h_max, double t0,
+                                   double tf) {
+  if (tf == t0) return 1;
+  double sign = signum(tf - t0);
+  if (signum(h0) != sign) return 2;
+  if (h_min < 0.0) return 3;
+  if (h_max < 0.0) return 4;
+  if (h_min > h_max) return 5;
+  if (fabs(h0) < h_min) return 6;
+  if (fabs(h0) > h_max) return 7;
+  if (fabs(h0) > fabs(tf - t0)) return 8;
+  if (h0 == 0.0) return 9;
+  return 0;
+}
+
 DE_HD bool validate_step_size(double h0, double h_min, double h_max, double t0,
                               double tf) {
-  if (tf == t0) return false;
-  double sign = signum(tf - t0);
-  if (signum(h0) != sign) return false;
-  if (h_min < 0.0) return false;
-  if (h_max < 0.0) return false;
-  if (h_min > h_max) return false;
-  if (fabs(h0) < h_min) return false;
-  if (fabs(h0) > h_max) return false;
-  if (fabs(h0) > fabs(tf - t0)) return false;
-  if (h0 == 0.0) return false;
-  return true;
+  return validate_step_size_check(h0, h_min, h_max, t0, tf) == 0;
 }
 
 }  // namespace de

@@ -2019,6 +2019,15 @@ int de_oracle_erk_tableau(int method, double* c, double* a, double* b, double* b
   }
 }
 
+// constrain_step_size and validate_step_size_parameters (src/utils.rs:22-157)
+// as the solvers use them; validate returns the number of the failed check.
+double de_oracle_constrain_step_size(double h, double h_min, double h_max) {
+  return de::constrain_step_size(h, h_min, h_max);
+}
+int de_oracle_validate_step_size(double h0, double h_min, double h_max, double t0, double tf) {
+  return de::validate_step_size_check(h0, h_min, h_max, t0, tf);
+}
+
 double de_oracle_pow(double x, double y) { return de::libm_pow(x, y); }
 
 void de_oracle_pow_many(const double* x, const double* y, double* out, int64_t n) {

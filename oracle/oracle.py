@@ -45,6 +45,10 @@ def load():
         lib.de_oracle_dop853_tableau.argtypes = [C.c_void_p] * 6
         lib.de_oracle_set_generic_dopri5.restype = None
         lib.de_oracle_set_generic_dopri5.argtypes = [C.c_int]
+        lib.de_oracle_constrain_step_size.restype = C.c_double
+        lib.de_oracle_constrain_step_size.argtypes = [C.c_double] * 3
+        lib.de_oracle_validate_step_size.restype = C.c_int
+        lib.de_oracle_validate_step_size.argtypes = [C.c_double] * 5
         lib.de_oracle_host_pow_many.restype = None
         lib.de_oracle_host_pow_many.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         lib.de_oracle_noise_normal.restype = C.c_double
@@ -144,6 +148,15 @@ def pow_many(x, y, host=False):
     fn = load().de_oracle_host_pow_many if host else load().de_oracle_pow_many
     fn(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
     return out
+
+
+def constrain_step_size(h, h_min, h_max):
+    return load().de_oracle_constrain_step_size(h, h_min, h_max)
+
+
+def validate_step_size(h0, h_min, h_max, t0, tf):
+    """0 if valid, else the number of the first failed check (1 .. 9)."""
+    return load().de_oracle_validate_step_size(h0, h_min, h_max, t0, tf)
 
 
 def set_generic_dopri5(on):
