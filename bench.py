@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--n", type=int, default=0, help="override ensemble size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="lorenz: skip the spring-chain, SDE and serving legs")
     ap.add_argument("--only", default=None,
                     help="run only one variant (profiling): persistent, fifo, plain, fast "
                          "(lorenz); event, crossing (events); a tableau name (erk)")
@@ -177,8 +179,9 @@ def bench_lorenz(args, dv):
                  "fifo": _abi.DE_SCHED_PERSISTENT_FIFO}.get(args.only, _abi.DE_SCHED_PERSISTENT)
         r = run(sched, args.only == "fast", args.steps, args.warmup)
         return {"variant": args.only, "accepted": r["A"], "rejected": r["R"],
-                "kernel_ms": sum(r["lat"]) / len(r["lat"])}, (p, y0, n)
+                "kernel_ms": sum(r["lat"]) / len(r["lat"])}, (p, y0, n, None)
     res = run(_abi.DE_SCHED_PERSISTENT, False, args.steps, args.warmup)
+    gpu_out = {k: out[k].cpu().numpy() for k in ("y", "t", "acc", "rej", "st")}
     plain = None if args.no_plain else run(_abi.DE_SCHED_PLAIN, False, max(3, args.steps // 2), 1)
     fifo = run(_abi.DE_SCHED_PERSISTENT_FIFO, False, max(3, args.steps // 2), 1)
     d853 = run(_abi.DE_SCHED_PERSISTENT, False, max(3, args.steps // 2), 1, _abi.DE_METHOD_DOP853)
@@ -243,11 +246,13 @@ def bench_lorenz(args, dv):
                          "tflops": fast_flops / (fast_ms / 1000.0) / 1e12,
                          "frac": fast_flops / (fast_ms / 1000.0) / 1e12 / FP64_PEAK_TFLOPS,
                          "note": "FMA-contracted build: same algorithm, not bit-equal"}
-    return line, (p, y0, n)
+    return line, (p, y0, n, gpu_out)
 
 
-def cpu_baseline_lorenz(args, p, y0, n):
-    """The oracle (pow = C library pow, as the reference) on every k-th trajectory."""
+def cpu_baseline_lorenz(args, p, y0, n, gpu_out):
+    """The oracle with this host's C library pow (the reference binary's
+    arithmetic) on every k-th trajectory, timed on all host cores, and the
+    GPU's results on the same trajectories compared with it."""
     import numpy as np
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import deamd
@@ -264,12 +269,32 @@ def cpu_baseline_lorenz(args, p, y0, n):
         sol = oracle.solve_ode(ens, pow_mode=oracle.POW_HOST, threads=threads)
         walls.append(time.perf_counter() - t0)
     dt = sorted(walls)[1]
-    return {"value": float(sol.accepted.sum()) / dt, "unit": "accepted_steps*traj/s",
+    base = {"value": float(sol.accepted.sum()) / dt, "unit": "accepted_steps*traj/s",
             "cores": threads, "kind": "port",
             "sample": "every %d-th trajectory of the 1M Lorenz ensemble (%d trajectories, "
                       "%d accepted steps, median %.2f s wall of 3 runs on %d threads); C++ "
                       "restatement of the reference solver with the C library pow"
                       % (k, len(idx), int(sol.accepted.sum()), dt, threads)}
+    # Parity with the reference's arithmetic on the sampled trajectories.
+    gy = gpu_out["y"].reshape(3, n)[:, idx]
+    wy = sol.y.reshape(3, len(idx))
+    counts = (gpu_out["acc"][idx] == sol.accepted) & (gpu_out["rej"][idx] == sol.rejected) \
+        & (gpu_out["st"][idx] == sol.status)
+    rel = np.max(np.abs(gy - wy) / np.maximum(np.abs(wy), 1e-300), axis=0)
+    bits = np.all(gy.view(np.uint64) == wy.view(np.uint64), axis=0) \
+        & (gpu_out["t"][idx].view(np.uint64) == sol.t.view(np.uint64))
+    chaotic = p[1, idx] >= 24.74
+
+    def share(mask, sel):
+        return float(mask[sel].mean()) if sel.any() else None
+
+    everything = np.ones(len(idx), dtype=bool)
+    parity = {"trajectories": int(len(idx)), "host_pow": "C library pow of the host"}
+    for name, sel in (("all", everything), ("rho_lt_24_74", ~chaotic), ("rho_ge_24_74", chaotic)):
+        parity[name] = {"step_counts_equal": share(counts, sel),
+                        "final_within_1e-10": share(rel <= 1e-10, sel),
+                        "bit_identical": share(bits & counts, sel)}
+    return base, parity
 
 
 # ---------------------------------------------------- embedded ERK family ---
@@ -857,6 +882,35 @@ def bench_serve(args, dv):
     }
 
 
+def other_configs(args, dv):
+    """The other configurations, each in a short run of its own workload: the
+    uneven spring-chain ensemble (batched vs plain), the SDE paths and a
+    one-second window of the Poisson serving stream."""
+    def sub(**kw):
+        a = argparse.Namespace(**vars(args))
+        a.n = 0
+        for k, v in kw.items():
+            setattr(a, k, v)
+        return a
+
+    keep = ("value", "ms_per_step", "latency_ms", "roofline", "plain", "batched_over_plain")
+    out = {}
+    spring = bench_spring(sub(steps=5, warmup=1), dv)
+    out["spring_chain_dopri5_64k"] = dict({k: spring[k] for k in keep if k in spring},
+                                          all_complete=spring["config"]["all_complete"])
+    sde = bench_sde(sub(steps=2, warmup=1), dv)
+    out["sde_ou8_gbm_4M"] = dict({k: sde[k] for k in keep if k in sde},
+                                 per_run=sde["config"]["per_run"])
+    serve = bench_serve(sub(serve_seconds=1.0), dv)
+    out["serve_poisson"] = dict({k: serve[k] for k in keep if k in serve},
+                                rate_req_per_s=serve["config"]["rate_req_per_s"],
+                                arrival_window_s=serve["config"]["arrival_window_s"],
+                                requests=serve["config"]["requests"],
+                                trajectories=serve["config"]["trajectories"],
+                                latency_ms_by_family=serve["config"]["latency_ms_by_family"])
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -866,9 +920,12 @@ def main():
         sys.exit(2)
     dv = Device()
     if args.workload == "lorenz":
-        line, (p, y0, n) = bench_lorenz(args, dv)
+        line, (p, y0, n, gpu_out) = bench_lorenz(args, dv)
         if not args.no_cpu_baseline and not args.only:
-            line["cpu_baseline"] = cpu_baseline_lorenz(args, p, y0, n)
+            line["cpu_baseline"], line["parity_vs_reference_arith"] = \
+                cpu_baseline_lorenz(args, p, y0, n, gpu_out)
+        if not args.only and not args.no_other_configs:
+            line["other_configs"] = other_configs(args, dv)
     elif args.workload == "events":
         line = bench_events(args, dv)
     elif args.workload == "erk":

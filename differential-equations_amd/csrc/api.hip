@@ -368,6 +368,19 @@ const char* de_last_error(void) { return g_err.c_str(); }
 
 const char* de_version(void) { return "de_amd 0.1.0 (gfx950, HIP)"; }
 
+int de_set_workspace_budget(size_t bytes) {
+  if (bytes < ((size_t)1 << 20)) return fail(DE_ERR_INVALID, "workspace budget below 1 MiB");
+  if (de::set_workspace_budget(bytes) != hipSuccess) return fail(DE_ERR_HIP, "hipMemPool attribute");
+  return DE_OK;
+}
+
+size_t de_workspace_budget(void) { return (size_t)de::workspace_budget(); }
+
+int de_workspace_trim(void) {
+  if (de::workspace_trim() != hipSuccess) return fail(DE_ERR_HIP, "hipMemPoolTrimTo");
+  return DE_OK;
+}
+
 int de_ode_csv_offsets(const de_ode_batch* b, const de_ode_result* r, int32_t source,
                        int64_t* offsets, void* stream) {
   de::CsvArgs a;

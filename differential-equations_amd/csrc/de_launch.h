@@ -11,24 +11,75 @@
 namespace de {
 
 // Stream-ordered workspace of the ODE launches, from a pool of the library's
-// own that keeps up to kWorkspaceKeep bytes of freed blocks instead of handing
-// them back to the driver at the next synchronisation: fresh device memory
-// costs ~18 ms per GB to obtain again (a 45 GB event workspace measured 809
-// ms), which a repeated solve would pay every time.  One pool per device,
-// created on first use.
-constexpr uint64_t kWorkspaceKeep = uint64_t(64) << 30;
+// own that keeps freed blocks instead of handing them back to the driver at
+// the next synchronisation: fresh device memory costs ~18 ms per GB to obtain
+// again (a 45 GB event workspace measured 809 ms), which a repeated solve
+// would pay every time.  One pool per device, created on first use.
+//
+// Device-memory budget of the workspace: one number (default 64 GiB of the
+// 288 GiB HBM, de_set_workspace_budget) bounds every workspace allocation,
+// the freed blocks the pool keeps, and (at 3/4 of it) the bracket regions of
+// one deferred-root event pass.  A failed allocation first trims the pool's
+// kept blocks and tries once more.  The serving arenas are sized by their
+// own creation argument; callers' buffers are theirs.
+constexpr uint64_t kWorkspaceBudgetDefault = uint64_t(64) << 30;
+constexpr int kMaxDev = 64;
+
+struct WorkspacePools {
+  std::mutex mu;
+  hipMemPool_t pools[kMaxDev] = {};
+  uint64_t budget = kWorkspaceBudgetDefault;
+};
+
+inline WorkspacePools& workspace_pools() {
+  static WorkspacePools w;
+  return w;
+}
+
+inline uint64_t workspace_budget() {
+  WorkspacePools& w = workspace_pools();
+  std::lock_guard<std::mutex> lk(w.mu);
+  return w.budget;
+}
+
+// New budget for every device's pool (its kept blocks are trimmed to it).
+inline hipError_t set_workspace_budget(uint64_t bytes) {
+  WorkspacePools& w = workspace_pools();
+  std::lock_guard<std::mutex> lk(w.mu);
+  w.budget = bytes;
+  for (int d = 0; d < kMaxDev; ++d) {
+    if (!w.pools[d]) continue;
+    uint64_t keep = bytes;
+    hipError_t e = hipMemPoolSetAttribute(w.pools[d], hipMemPoolAttrReleaseThreshold, &keep);
+    if (e == hipSuccess) e = hipMemPoolTrimTo(w.pools[d], bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Hand every kept (freed) block of every device's pool back to the driver.
+inline hipError_t workspace_trim() {
+  WorkspacePools& w = workspace_pools();
+  std::lock_guard<std::mutex> lk(w.mu);
+  for (int d = 0; d < kMaxDev; ++d) {
+    if (!w.pools[d]) continue;
+    hipError_t e = hipMemPoolTrimTo(w.pools[d], 0);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 inline hipError_t workspace_alloc(void** p, size_t bytes, hipStream_t s) {
-  constexpr int kMaxDev = 64;
-  static hipMemPool_t pools[kMaxDev] = {};
-  static std::mutex mu;
+  WorkspacePools& w = workspace_pools();
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  if (dev < 0 || dev >= kMaxDev) return hipMallocAsync(p, bytes, s);
   hipMemPool_t pool;
   {
-    std::lock_guard<std::mutex> lk(mu);
-    if (!pools[dev]) {
+    std::lock_guard<std::mutex> lk(w.mu);
+    if (bytes > w.budget) return hipErrorOutOfMemory;
+    if (dev < 0 || dev >= kMaxDev) return hipMallocAsync(p, bytes, s);
+    if (!w.pools[dev]) {
       hipMemPoolProps props = {};
       props.allocType = hipMemAllocationTypePinned;
       props.location.type = hipMemLocationTypeDevice;
@@ -36,16 +87,21 @@ inline hipError_t workspace_alloc(void** p, size_t bytes, hipStream_t s) {
       hipMemPool_t created = nullptr;
       e = hipMemPoolCreate(&created, &props);
       if (e != hipSuccess) return e;
-      uint64_t keep = kWorkspaceKeep;
+      uint64_t keep = w.budget;
       e = hipMemPoolSetAttribute(created, hipMemPoolAttrReleaseThreshold, &keep);
       if (e != hipSuccess) {
         (void)hipMemPoolDestroy(created);
         return e;
       }
-      pools[dev] = created;
+      w.pools[dev] = created;
     }
-    pool = pools[dev];
+    pool = w.pools[dev];
   }
+  e = hipMallocFromPoolAsync(p, bytes, pool, s);
+  if (e == hipSuccess) return e;
+  (void)hipGetLastError();
+  // The pool's kept blocks may be what is missing: release them and retry.
+  (void)hipMemPoolTrimTo(pool, 0);
   return hipMallocFromPoolAsync(p, bytes, pool, s);
 }
 

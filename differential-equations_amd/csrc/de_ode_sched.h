@@ -185,15 +185,15 @@ static __global__ void rerun_args_kernel(const OdeArgs* src, OdeArgs* dst, const
 }
 
 // Deferred entries per trajectory: max_events + half again (at least 32 more).
-// The regions (entries and result records) of one pass stay within
-// kDeferBudget bytes: when those of the whole ensemble would not, the
+// The regions (entries and result records) of one pass stay within 3/4 of
+// the workspace budget (48 GiB by default): when those of the whole ensemble
+// would not, the
 // ensemble is solved in slices of consecutive trajectories, each a pass of
 // its own (integration, roots, bookkeeping, inline reruns), of at least
 // kMinDeferSlice trajectories (enough to fill the GPU); only if even that
 // does not fit is the number of entries halved, and below 16 the roots are
 // found inline.  The environment variable DE_EVENT_DEFER_SLICE, when set to
 // a positive count, fixes the slice size instead (for tests and tuning).
-constexpr size_t kDeferBudget = (size_t)48 << 30;
 constexpr int64_t kMinDeferSlice = int64_t(1) << 18;
 
 template <class Lane>
@@ -207,10 +207,11 @@ static int64_t defer_slots(const OdeArgs& a, int64_t* slice) {
     int64_t forced = 0;
     if (const char* f = getenv("DE_EVENT_DEFER_SLICE")) forced = atoll(f);
     if (forced > 0) want = std::min<int64_t>(a.n, forced);
+    const size_t budget = (size_t)(workspace_budget() / 4 * 3);  // the regions of one pass
     for (int64_t bmax = (int64_t)a.ev.max_events + std::max<int64_t>(a.ev.max_events / 2, 32);
          bmax >= 16; bmax /= 2) {
       const size_t per = (size_t)bmax * (Lane::kEntryWidth + kResWidth) * sizeof(double);
-      int64_t fit = (int64_t)std::min<size_t>(kDeferBudget / per, (size_t)a.n);
+      int64_t fit = (int64_t)std::min<size_t>(budget / per, (size_t)a.n);
       if (fit < want) continue;
       if (forced > 0) fit = want;
       const int64_t parts = (a.n + fit - 1) / fit;
@@ -308,20 +309,26 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
       return hipGetLastError() == hipSuccess ? 0 : -1;
     }
   }
-  static int blocks_per_cu = -1;
-  static int num_cu = 0;
-  if (blocks_per_cu < 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return -1;
-    if (hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      return -1;
-    int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ode_persistent_kernel<Lane>, kBlock, 0) !=
-        hipSuccess)
-      return -1;
-    blocks_per_cu = b > 0 ? b : 1;
-  }
-  int64_t want = (int64_t)num_cu * blocks_per_cu;
+  // Resident blocks of the persistent kernel: queried once per lane type
+  // (a function-local static is initialised exactly once, also when several
+  // threads launch at the same time).
+  struct Occupancy {
+    int blocks_per_cu = 0, num_cu = 0;
+  };
+  static const Occupancy occ = [] {
+    Occupancy o;
+    int dev = 0, cu = 0, b = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ode_persistent_kernel<Lane>, kBlock, 0) !=
+            hipSuccess)
+      return o;
+    o.num_cu = cu;
+    o.blocks_per_cu = b > 0 ? b : 1;
+    return o;
+  }();
+  if (occ.num_cu <= 0) return -1;
+  int64_t want = (int64_t)occ.num_cu * occ.blocks_per_cu;
   int64_t need = (a.n + kBlock - 1) / kBlock;
   int64_t grid = want < need ? want : need;
   // Stream-ordered workspace: the launch arguments; for event lanes 256 bytes
@@ -331,7 +338,7 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
   // longest-first order (schedule PERSISTENT, batches of a few thousand
   // trajectories or more) or a sliced solve, 256 bucket counts, the order and
   // one bucket byte per trajectory of a slice.
-  const bool ordered = schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL;
+  bool ordered = schedule == kSchedPersistent && a.n >= kOrderMinN && a.n <= 0xffffffffLL;
   const size_t args_bytes = (sizeof(OdeArgs) + 255) & ~(size_t)255;
   const size_t ev_bytes = Lane::kEv != 0 ? 256 : 0;
   int64_t slice = a.n;
@@ -351,9 +358,14 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
     if (workspace_alloc(&work, args_bytes + ev_bytes + order_bytes + defer_bytes, s) == hipSuccess)
       break;
     (void)hipGetLastError();
-    if (bmax == 0) return -1;
-    bmax = 0;  // no room for the bracket regions: roots inline, one pass
-    slice = a.n;
+    if (bmax > 0) {
+      bmax = 0;  // no room for the bracket regions: roots inline, one pass
+      slice = a.n;
+    } else if (ordered) {
+      ordered = false;  // no room for the longest-first order: index order
+    } else {
+      return -1;
+    }
   }
   const bool sliced = slice < a.n;
   char* w = (char*)work;

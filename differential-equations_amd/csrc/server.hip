@@ -23,6 +23,7 @@
 // host memory.  Every trajectory's arithmetic depends only on its own inputs,
 // so a request's results do not depend on what else is being served.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -195,6 +196,23 @@ __global__ void gather_remaining(const ReqDev* reqs, int32_t* out, int n) {
   if (i < n) out[i] = reqs[i].remaining;
 }
 
+// Reset of a drained pool (no request with work in it): slots that hold an
+// unbacked ticket (state 1) become idle and the ticket and range counters
+// restart from zero, so the range table is reused.
+template <class Slot>
+__global__ void reset_pool_kernel(Slot* s, int n, Queue* q) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && s[i].state == 1) {
+    s[i].state = 0;
+    s[i].ticket = 0;
+  }
+  if (i == 0) {
+    q->next = 0;
+    q->published = 0;
+    q->nranges = 0;
+  }
+}
+
 template <class Slot>
 __global__ void init_slots(Slot* s, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -222,9 +240,17 @@ struct Family {
   int64_t published = 0;
   int64_t active_tickets = 0;  // published but possibly unfinished
   int active_reqs = 0;         // active requests with work in this pool
+  int64_t max_ranges = 0;      // capacity of the range table
   hipStream_t stream;
   void (*launch)(Family&, ReqDev*, int, hipStream_t);
+  void (*reset)(Family&, hipStream_t);
 };
+
+template <class Slot>
+void reset_pool(Family& f, hipStream_t s) {
+  hipLaunchKernelGGL(reset_pool_kernel<Slot>, dim3((f.nslots + 255) / 256), dim3(256), 0, s,
+                     (Slot*)f.slots, f.nslots, f.q);
+}
 
 template <class L>
 void launch_lane(Family& f, ReqDev* reqs, int k, hipStream_t s) {
@@ -335,6 +361,7 @@ struct de_server {
   std::vector<hipEvent_t> fam_done;
   hipGraphExec_t graph = nullptr;
   int iterations = 0;
+  int64_t range_resets = 0;
   int error = 0;
 };
 
@@ -563,18 +590,35 @@ void scheduler(de_server* s) {
   while (true) {
     // Admission: first come, first served, while the footprint fits.
     while (!s->pending.empty() && !s->free_slots.empty()) {
-      bool ranges_full = false;
-      for (auto& f : s->fams)
-        if ((int64_t)f.host_ranges.size() + 1 >= kMaxRanges) ranges_full = true;
-      if (ranges_full) break;
       Request* r = s->pending.front();
       std::vector<std::vector<int32_t>> lists(s->fams.size());
       const de_ode_batch& b = r->b;
+      std::vector<int> needs;
       if (b.family == DE_FAMILY_SPRING_CHAIN) {
         for (int64_t i = 0; i < b.n; ++i)
           lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i], b.method, DE_SOLOUT_DEFAULT)]
               .push_back((int32_t)i);
+        for (int k = 2; k < kSpringEnd; ++k)
+          if (!lists[k].empty()) needs.push_back(k);
+      } else {
+        needs.push_back(fam_index_for(s, b.family, 0, b.method, b.event_solout));
       }
+      // A pool whose range table is full takes no new work until it drains;
+      // then (no kernel is in flight between iterations) its table restarts.
+      bool held = false;
+      for (int k : needs) {
+        Family& f = s->fams[k];
+        if ((int64_t)f.host_ranges.size() + 1 < f.max_ranges) continue;
+        if (f.active_reqs > 0) {
+          held = true;
+          break;
+        }
+        f.reset(f, s->ctl);
+        f.host_ranges.clear();
+        f.published = 0;
+        s->range_resets += 1;
+      }
+      if (held) break;
       r->slot = s->free_slots.back();
       std::vector<int32_t*> d_lists(s->fams.size(), nullptr);
       if (!upload(s, r, lists, d_lists.data())) {
@@ -599,7 +643,7 @@ void scheduler(de_server* s) {
             r->pools |= 1u << k;
           }
       } else {
-        const int k = fam_index_for(s, b.family, 0, b.method, b.event_solout);
+        const int k = needs[0];
         publish(s, s->fams[k], r->slot, 0, (int32_t)b.n, nullptr);
         r->pools |= 1u << k;
       }
@@ -658,12 +702,13 @@ template <class Slot>
 bool make_pool(Family& f, int nslots) {
   f.nslots = nslots;
   f.slot_bytes = sizeof(Slot);
+  f.reset = reset_pool<Slot>;
   if (hipMalloc(&f.slots, sizeof(Slot) * (size_t)nslots) != hipSuccess) return false;
   hipLaunchKernelGGL(init_slots<Slot>, dim3((nslots + 255) / 256), dim3(256), 0, 0,
                      (Slot*)f.slots, nslots);
   if (hipMalloc(&f.q, sizeof(de::serve::Queue)) != hipSuccess) return false;
   if (hipMemset(f.q, 0, sizeof(de::serve::Queue)) != hipSuccess) return false;
-  if (hipMalloc(&f.ranges, sizeof(Range) * kMaxRanges) != hipSuccess) return false;
+  if (hipMalloc(&f.ranges, sizeof(Range) * (size_t)f.max_ranges) != hipSuccess) return false;
   if (hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking) != hipSuccess) return false;
   return true;
 }
@@ -678,6 +723,13 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
   de_server* s = new de_server();
   s->budget = hbm_budget_bytes;
   s->flags = flags;
+  // Range-table capacity per pool: kMaxRanges, or DE_SERVER_MAX_RANGES (at
+  // least 4) to exercise the table's reuse with few requests.
+  int64_t max_ranges = kMaxRanges;
+  if (const char* env = getenv("DE_SERVER_MAX_RANGES")) {
+    long long v = atoll(env);
+    if (v >= 4 && v <= kMaxRanges) max_ranges = v;
+  }
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -704,6 +756,7 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
     f.name = specs[k].name;
     f.cap = specs[k].cap;
     f.g = specs[k].g;
+    f.max_ranges = max_ranges;
     switch (k) {
       case 0: ok = make_pool<LaneSlot<de::Dopri5Lane<de::LorenzRhs, true>>>(f, lane_slots);
         f.launch = launch_lane<de::Dopri5Lane<de::LorenzRhs, true>>; break;
@@ -743,7 +796,7 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
     }
   }
   size_t pools = 0;
-  for (auto& f : s->fams) pools += f.slot_bytes * f.nslots + sizeof(Range) * kMaxRanges;
+  for (auto& f : s->fams) pools += f.slot_bytes * f.nslots + sizeof(Range) * (size_t)f.max_ranges;
   ok = ok && hipMalloc(&s->d_reqs, sizeof(de::serve::ReqDev) * kMaxReq) == hipSuccess;
   ok = ok && hipHostMalloc(&s->h_remaining, sizeof(int32_t) * kMaxReq, 0) == hipSuccess;
   ok = ok && hipMalloc(&s->d_remaining, sizeof(int32_t) * kMaxReq) == hipSuccess;
@@ -898,6 +951,14 @@ double de_server_latency(de_server* s, uint64_t ticket) {
   auto it = s->all.find(ticket);
   if (it == s->all.end() || !it->second->done) return -1.0;
   return it->second->t_done - it->second->t_submit;
+}
+
+int de_server_stats(de_server* s, int64_t* iterations, int64_t* range_resets) {
+  if (!s) return DE_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (iterations) *iterations = s->iterations;
+  if (range_resets) *range_resets = s->range_resets;
+  return DE_OK;
 }
 
 void de_server_destroy(de_server* s) {

@@ -765,6 +765,12 @@ class Server:
     def latency(self, ticket):
         return lib().de_server_latency(self.h, ticket)
 
+    def stats(self):
+        """(iterations, range-table restarts) of the scheduler."""
+        it, rr = C.c_int64(), C.c_int64()
+        _abi.check(lib().de_server_stats(self.h, C.byref(it), C.byref(rr)))
+        return it.value, rr.value
+
     def close(self):
         if self.h:
             lib().de_server_destroy(self.h)

@@ -188,10 +188,14 @@ SIGNATURES = {
                                    C.c_void_p, C.c_void_p, C.c_void_p]),
     "de_last_error": (C.c_char_p, []),
     "de_version": (C.c_char_p, []),
+    "de_set_workspace_budget": (C.c_int, [C.c_size_t]),
+    "de_workspace_budget": (C.c_size_t, []),
+    "de_workspace_trim": (C.c_int, []),
     "de_server_create": (C.c_int, [C.c_size_t, C.c_int32, C.POINTER(C.c_void_p)]),
     "de_server_submit": (C.c_int, [C.c_void_p, C.POINTER(OdeBatch), C.POINTER(C.c_uint64)]),
     "de_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(OdeResult), C.c_int32]),
     "de_server_latency": (C.c_double, [C.c_void_p, C.c_uint64]),
+    "de_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "de_server_destroy": (None, [C.c_void_p]),
     "de_gen_lorenz": (None, [C.c_uint64, C.c_int64, C.c_void_p, C.c_void_p]),
     "de_gen_vdp": (None, [C.c_uint64, C.c_int64, C.c_void_p, C.c_void_p]),

@@ -294,6 +294,20 @@ const char* de_last_error(void);
 /* Library version string. */
 const char* de_version(void);
 
+/* ---- Device-memory budget of the launch workspace ------------------------
+ * No counterpart in the reference (it allocates per trajectory on the host).
+ * Batch solves take their stream-ordered workspace (launch arguments, the
+ * longest-first order, the bracket regions of deferred event roots) from a
+ * library pool that keeps freed blocks for the next solve.  One budget, 64 GiB
+ * by default, bounds each workspace allocation and the blocks the pool keeps;
+ * the bracket regions of one event pass stay within 3/4 of it (an ensemble
+ * whose regions would not is solved in slices).  The serving arena is sized by
+ * de_server_create's own budget.  Total device memory of the library: the
+ * workspace budget plus the live servers' budgets. */
+int de_set_workspace_budget(size_t bytes);   /* >= 1 MiB; trims kept blocks to it */
+size_t de_workspace_budget(void);
+int de_workspace_trim(void);                 /* return every kept block to the driver */
+
 /* ---- Serving: continuous batching of ensemble solves on one GPU ----------
  * Requests (each a de_ode_batch over host memory) are admitted while their
  * device footprint fits the HBM budget and lanes are free; their state stays
@@ -308,6 +322,11 @@ int de_server_submit(de_server* s, const de_ode_batch* req, uint64_t* ticket);
 int de_server_poll(de_server* s, uint64_t ticket, de_ode_result* out, int32_t wait);
 /* Submit-to-completion latency of a finished request, in seconds. */
 double de_server_latency(de_server* s, uint64_t ticket);
+/* Scheduler counters: iterations (chunk launches of the pools with work) and
+ * restarts of a drained pool's range table (each pool holds up to 2^20
+ * published ranges; the environment variable DE_SERVER_MAX_RANGES, read at
+ * creation, lowers that capacity). */
+int de_server_stats(de_server* s, int64_t* iterations, int64_t* range_resets);
 void de_server_destroy(de_server* s);
 
 #ifdef __cplusplus

@@ -42,6 +42,17 @@ def test_version_and_defaults():
     assert b.h_max == float("inf")
 
 
+def test_workspace_budget_setters():
+    lib = deamd.lib()
+    assert lib.de_workspace_budget() == 64 << 30
+    assert lib.de_set_workspace_budget(1000) == _abi.DE_ERR_INVALID
+    assert lib.de_workspace_budget() == 64 << 30
+    assert lib.de_set_workspace_budget(8 << 30) == _abi.DE_OK
+    assert lib.de_workspace_budget() == 8 << 30
+    assert lib.de_set_workspace_budget(64 << 30) == _abi.DE_OK
+    assert lib.de_workspace_trim() == _abi.DE_OK
+
+
 def _solve(b):
     res = deamd.Solution(max(int(b.n), 1), max(int(b.n), 1) * max(b.dim, 1), 0)
     return deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None)

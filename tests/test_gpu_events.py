@@ -215,3 +215,22 @@ def test_deferred_roots_in_slices(orc, monkeypatch, name, n, slice_):
     assert_events_equal(ens.solve(), want)
     ens = ens.t_eval(np.linspace(0.0, 12.0, 9)).options(schedule=DE_SCHED_PERSISTENT_FIFO)
     assert_events_equal(ens.solve(), orc.solve_ode(ens, threads=THREADS), dense=True)
+
+
+@pytest.mark.parametrize("budget_mib,name", [(2, "dopri5"), (64, "dop853")])
+def test_workspace_budget_bounds_the_event_regions(orc, budget_mib, name):
+    # A small workspace budget leaves no room for the bracket regions (2 MiB:
+    # roots inline; 64 MiB: fewer entries per trajectory, the overflowing ones
+    # rerun inline); the bits do not change.
+    lib = deamd.lib()
+    old = lib.de_workspace_budget()
+    p, ens = lorenz(6000, 8.0, seed=71)
+    ens = ens.method(method(name)).event(Event.component(2, p[1] - 1.0)).options(max_events=40)
+    want = orc.solve_ode(ens, threads=THREADS)
+    try:
+        deamd._abi.check(lib.de_set_workspace_budget(budget_mib << 20))
+        got = ens.solve()
+    finally:
+        deamd._abi.check(lib.de_set_workspace_budget(old))
+        deamd._abi.check(lib.de_workspace_trim())
+    assert_events_equal(got, want)

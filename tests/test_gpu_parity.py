@@ -148,6 +148,24 @@ def test_lorenz_c2_full_ensemble(orc):
     assert same(got.y.reshape(3, n)[:, idx].ravel(), want.y)
 
 
+def test_workspace_budget_below_the_order_falls_back_to_index_order():
+    # 1 MiB of workspace cannot hold the longest-first order of 2^20
+    # trajectories (5 MiB): the persistent kernel runs them in index order,
+    # with the same bits.
+    lib = deamd.lib()
+    p, y0 = deamd.gen_lorenz(1 << 20, seed=12)
+    mk = lambda: IVP.ode(Lorenz(*p), 0.0, 0.5, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10).max_steps(100000))
+    want = mk().solve()
+    old = lib.de_workspace_budget()
+    try:
+        deamd._abi.check(lib.de_set_workspace_budget(1 << 20))
+        got = mk().solve()
+    finally:
+        deamd._abi.check(lib.de_set_workspace_budget(old))
+    assert_ode_equal(got, want)
+
+
 def test_lorenz_persistent_large_batch_matches_plain():
     # 65 536 trajectories (several refills per lane) over a short span.
     p, y0 = deamd.gen_lorenz(65536, seed=11)

@@ -192,3 +192,20 @@ def test_served_event_solouts_match_direct_solves():
         _event_fields(g, w)
     assert np.array_equal(got[2].n_dense, wants[1].n_dense)
     assert np.array_equal(np.nan_to_num(got[2].dense), np.nan_to_num(wants[1].dense))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_range_table_restarts_when_pool_drains(monkeypatch, graph):
+    # With room for 8 ranges per pool, 40 requests to one pool fill the table
+    # several times: admission holds until the pool drains, the table
+    # restarts, and every request still completes with the direct solve's bits.
+    monkeypatch.setenv("DE_SERVER_MAX_RANGES", "8")
+    reqs = [lorenz_req(64 + 7 * k, seed=100 + k, tf=1.0) for k in range(40)]
+    wants = [r.solve() for r in reqs]
+    with Server(4 * GIB, graph=graph) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        got = [srv.poll(t) for t in tickets]
+        _, resets = srv.stats()
+    assert resets >= 40 // 8
+    for g, w in zip(got, wants):
+        assert_same(g, w)
