@@ -268,8 +268,8 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
                               : hip_fail(hipGetLastError(), "spring launch");
     return DE_OK;
   }
-  if (sched == de::kSchedPersistent || sched == de::kSchedPersistentFifo ||
-      b->event_solout != DE_SOLOUT_DEFAULT) {
+  if ((sched == de::kSchedPersistent || sched == de::kSchedPersistentFifo ||
+       b->event_solout != DE_SOLOUT_DEFAULT) && !a.counter) {
     int rc = counter_slot(&a.counter);
     if (rc != DE_OK) return rc;
   }
@@ -552,6 +552,76 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   if (dr.n_events) st.back(o->n_events, dr.n_events, n);
   if (st.err != hipSuccess) return hip_fail(st.err, "copying results");
   return DE_OK;
+}
+
+// ------------------------------------------------- no-allocation solve ----
+// de_ode_batch_solve for callers that cannot allocate (code under graph
+// capture, serving loops, harnesses that forbid hipMalloc): the launch's
+// scratch memory -- the persistent schedule's work counter, the launch
+// arguments, the hyperplane normal and the longest-first order -- comes from
+// one file-scope device array per GPU instead of the library's pool.  Every
+// call on a GPU shares that GPU's array: calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr size_t kNoAllocCounterBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of this GPU's copy of `scratch` (host code never uses
+// the array's name as an address).
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kScratchDevices) return fail(DE_ERR_UNSUPPORTED, "device %d", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DE_OK;
+}
+}  // namespace
+
+int64_t de_ode_noalloc_scratch_bytes(const de_ode_batch* b) {
+  if (!b || b->n < 0) return -1;
+  const int sched = b->schedule == DE_SCHED_AUTO ? de::kSchedPersistent : b->schedule;
+  const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
+  const bool persistent = sched != de::kSchedPlain || events;
+  if (!persistent) return 0;
+  const size_t args = (sizeof(de::OdeArgs) + 255) & ~(size_t)255;
+  const bool ordered = sched == de::kSchedPersistent;
+  const size_t order = ordered ? (((size_t)256 * 4 + (size_t)b->n * 5 + 255) & ~(size_t)255) : 0;
+  return (int64_t)(kNoAllocCounterBytes + args + (events ? 256 : 0) + order);
+}
+
+int de_ode_batch_solve_noalloc(const de_ode_batch* b, de_ode_result* o, void* stream) {
+  int rc = validate_ode(b, o);
+  if (rc != DE_OK) return rc;
+  if (b->flags & DE_FLAG_HOST_MEMORY)
+    return fail(DE_ERR_UNSUPPORTED, "the no-allocation solve takes device memory only");
+  if (b->family == DE_FAMILY_SPRING_CHAIN)
+    return fail(DE_ERR_UNSUPPORTED, "the no-allocation solve is built for the lane families");
+  if (b->n == 0) return DE_OK;
+  const int64_t need = de_ode_noalloc_scratch_bytes(b);
+  if (need < 0 || (size_t)need > kNoAllocScratchBytes)
+    return fail(DE_ERR_INVALID,
+                "n = %lld needs %lld bytes of scratch, more than the %zu-byte maximum of the "
+                "no-allocation solve", (long long)b->n, (long long)need, kNoAllocScratchBytes);
+  char* base = nullptr;
+  rc = scratch_address((void**)&base);
+  if (rc != DE_OK) return rc;
+  de::OdeArgs a = make_args(b, o);
+  a.counter = (unsigned long long*)base;
+  a.scratch = base + kNoAllocCounterBytes;
+  a.scratch_bytes = (int64_t)(kNoAllocScratchBytes - kNoAllocCounterBytes);
+  return run_ode_device(b, nullptr, a, (hipStream_t)stream);
 }
 
 int de_sde_batch_solve(const de_sde_batch* b, de_sde_result* o, void* stream) {

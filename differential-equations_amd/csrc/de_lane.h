@@ -65,6 +65,8 @@ struct OdeArgs {
   EventArgs ev;                 // event solout (lanes built with EV != 0)
   const uint32_t* n_dev;        // persistent schedule: positions to take, read on the
                                 // device (null: n)
+  void* scratch;                // caller-provided launch workspace (no allocation), or null
+  int64_t scratch_bytes;
 };
 
 // Tolerances of one trajectory (Tolerance, src/tolerance.rs:11-68): a scalar

@@ -355,9 +355,17 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
                                  (size_t)slice * (size_t)bmax * (Lane::kEntryWidth + kResWidth) *
                                      sizeof(double)
                            : 0;
-    if (workspace_alloc(&work, args_bytes + ev_bytes + order_bytes + defer_bytes, s) == hipSuccess)
+    const size_t total = args_bytes + ev_bytes + order_bytes + defer_bytes;
+    if (a.scratch) {
+      if (total <= (size_t)a.scratch_bytes) {
+        work = a.scratch;
+        break;
+      }
+    } else if (workspace_alloc(&work, total, s) == hipSuccess) {
       break;
-    (void)hipGetLastError();
+    } else {
+      (void)hipGetLastError();
+    }
     if (bmax > 0) {
       bmax = 0;  // no room for the bracket regions: roots inline, one pass
       slice = a.n;
@@ -445,7 +453,7 @@ static int launch(const OdeArgs& a, int schedule, hipStream_t s) {
       }
     }
   }
-  if (hipFreeAsync(work, s) != hipSuccess) return -1;
+  if (!a.scratch && hipFreeAsync(work, s) != hipSuccess) return -1;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -191,6 +191,9 @@ SIGNATURES = {
     "de_set_workspace_budget": (C.c_int, [C.c_size_t]),
     "de_workspace_budget": (C.c_size_t, []),
     "de_workspace_trim": (C.c_int, []),
+    "de_ode_batch_solve_noalloc": (C.c_int, [C.POINTER(OdeBatch), C.POINTER(OdeResult),
+                                             C.c_void_p]),
+    "de_ode_noalloc_scratch_bytes": (C.c_int64, [C.POINTER(OdeBatch)]),
     "de_server_create": (C.c_int, [C.c_size_t, C.c_int32, C.POINTER(C.c_void_p)]),
     "de_server_submit": (C.c_int, [C.c_void_p, C.POINTER(OdeBatch), C.POINTER(C.c_uint64)]),
     "de_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(OdeResult), C.c_int32]),

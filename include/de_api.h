@@ -251,6 +251,18 @@ void de_sde_batch_init(de_sde_batch* b);
  * (src/methods/erk/fixed/ordinary.rs:16-220). */
 int de_ode_batch_solve(const de_ode_batch* batch, de_ode_result* out, void* stream);
 
+/* No counterpart in the reference: de_ode_batch_solve for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same results as de_ode_batch_solve; device
+ * memory only (no DE_FLAG_HOST_MEMORY), fixed-dimension families.  The launch's
+ * scratch memory (work counter, launch arguments, the longest-first order: about
+ * 2 KiB + 5 bytes per trajectory, de_ode_noalloc_scratch_bytes) comes from one
+ * 64 MiB device array per GPU that the library holds until the process ends;
+ * larger requests fail with DE_ERR_INVALID.  Event solouts find their roots
+ * inline.  Calls that overlap on one GPU share that array and race. */
+int de_ode_batch_solve_noalloc(const de_ode_batch* batch, de_ode_result* out, void* stream);
+int64_t de_ode_noalloc_scratch_bytes(const de_ode_batch* batch);
+
 /* Batched solve_sde (src/sde/solve_ivp.rs:135-287) with Euler-Maruyama
  * (src/methods/erk/fixed/stochastic.rs:18-193) or Milstein
  * (src/methods/milstein.rs:76-224). */
