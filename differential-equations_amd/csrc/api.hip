@@ -135,6 +135,22 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->max_rejects < 0) return fail(DE_ERR_INVALID, "max_rejects < 0");
   if (b->event_solout < DE_SOLOUT_DEFAULT || b->event_solout > DE_SOLOUT_HYPERPLANE)
     return fail(DE_ERR_INVALID, "unknown event solout %d", b->event_solout);
+  if (b->points < DE_POINTS_NONE || b->points > DE_POINTS_EVEN)
+    return fail(DE_ERR_INVALID, "unknown point output %d", b->points);
+  if (b->points != DE_POINTS_NONE) {
+    if (b->family == DE_FAMILY_SPRING_CHAIN)
+      return fail(DE_ERR_UNSUPPORTED, "point outputs are built for fixed-dimension families");
+    if (b->event_solout != DE_SOLOUT_DEFAULT)
+      return fail(DE_ERR_UNSUPPORTED, "a point output cannot be wrapped by an event solout");
+    if (b->n_eval > 0)
+      return fail(DE_ERR_INVALID, "a point output replaces the t_eval output (n_eval > 0)");
+    if (b->points == DE_POINTS_DENSE && b->points_n < 1)
+      return fail(DE_ERR_INVALID, "IVP::dense(n) needs n >= 1");
+    if (b->points == DE_POINTS_EVEN && !(b->points_dt > 0.0 && b->points_dt < INFINITY))
+      return fail(DE_ERR_INVALID, "IVP::even(dt) needs a finite dt > 0");
+    if (b->n > 0 && (!o->points_offset || !o->n_points))
+      return fail(DE_ERR_INVALID, "a point output needs points_offset and n_points");
+  }
   if (b->event_solout != DE_SOLOUT_DEFAULT) {
     if (b->family == DE_FAMILY_SPRING_CHAIN)
       return fail(DE_ERR_UNSUPPORTED, "event solouts are built for fixed-dimension families");
@@ -195,6 +211,13 @@ de::OdeArgs make_args(const de_ode_batch* b, const de_ode_result* o) {
   a.dense = b->n_eval > 0 ? o->dense : nullptr;
   a.n_dense = b->n_eval > 0 ? o->n_dense : nullptr;
   if (!a.dense) a.n_eval = 0;
+  a.pts_mode = b->points;
+  a.pts_n = b->points_n;
+  a.pts_dt = b->points_dt;
+  a.pts_offset = o->points_offset;
+  a.pts_t = o->points_t;
+  a.pts_y = o->points_y;
+  a.n_pts = o->n_points;
   a.ev.solout = b->event_solout;
   a.ev.fn = b->event_solout == DE_SOLOUT_CROSSING     ? de::kEventComponent
             : b->event_solout == DE_SOLOUT_HYPERPLANE ? de::kEventPlane
@@ -534,6 +557,13 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   dr.event_t = st.out(o->event_t, ev_cap);
   dr.event_y = st.out(o->event_y, ev_cap * b->dim);
   dr.n_events = st.out(o->n_events, events ? n : 0);
+  const bool pts = b->points != DE_POINTS_NONE;
+  const int64_t pts_cap = pts ? o->points_offset[n] : 0;
+  if (pts && pts_cap < 0) return fail(DE_ERR_INVALID, "points_offset[n] < 0");
+  dr.points_offset = st.in(o->points_offset, pts ? n + 1 : 0);
+  dr.points_t = st.out(o->points_t, pts_cap);
+  dr.points_y = st.out(o->points_y, pts_cap * b->dim);
+  dr.n_points = st.out(o->n_points, pts ? n : 0);
   if (st.err != hipSuccess) return hip_fail(st.err, "staging inputs");
   rc = run_ode_device(&db, ragged ? b->sizes : nullptr, make_args(&db, &dr), s);
   if (rc != DE_OK) return rc;
@@ -550,6 +580,9 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   if (dr.event_t) st.back(o->event_t, dr.event_t, ev_cap);
   if (dr.event_y) st.back(o->event_y, dr.event_y, ev_cap * b->dim);
   if (dr.n_events) st.back(o->n_events, dr.n_events, n);
+  if (dr.points_t) st.back(o->points_t, dr.points_t, pts_cap);
+  if (dr.points_y) st.back(o->points_y, dr.points_y, pts_cap * b->dim);
+  if (dr.n_points) st.back(o->n_points, dr.n_points, n);
   if (st.err != hipSuccess) return hip_fail(st.err, "copying results");
   return DE_OK;
 }

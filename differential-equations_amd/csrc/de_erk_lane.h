@@ -108,6 +108,7 @@ struct ErkLane {
                                        (int64_t)kAccEvals * L.acc;
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
+    if (DENSE && EV == 0 && a.n_pts) a.n_pts[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
     if constexpr (EV != 0) {
       if (a.ev.brk)
@@ -233,7 +234,9 @@ struct ErkLane {
     stiff = 0;
     rejected_last = false;
     tail = as_f64(kNaNBits);
-    if (DENSE && EV != 2 && a.n_eval > 0)
+    if (DENSE && EV == 0 && a.pts_mode != kPointsNone)
+      points_start<D>(a, i, pushed, tail, t, y);
+    else if (DENSE && EV != 2 && a.n_eval > 0)
       teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
                     [](double, double*) { return false; }, tail);
     else
@@ -461,14 +464,20 @@ struct ErkLane {
       }
       // Without a t_eval output the Solution's last point is the step's end
       // (DefaultSolout pushes it before an event solout looks at the step).
-      if (!teval_out) tail = accept ? t_new : tail;
+      const bool pts_out = DENSE && EV == 0 && a.pts_mode != kPointsNone;
+      if (!teval_out && !pts_out) tail = accept ? t_new : tail;
+      // A point output: DefaultSolout pushes the step's end; the dense and even
+      // outputs interpolate.
+      const bool pts_interp = pts_out && accept && a.pts_mode != kPointsSteps;
+      if (pts_out && accept && a.pts_mode == kPointsSteps) points_push<D>(a, i, pushed, t_new, yh);
       bool need = false;
       if constexpr (EV != 0) need = accept && ev.pre(a.ev, t_new, yh);
-      if (tev || need) {
-        // The dense stages: for a t_eval point or an inline root search (a
-        // deferred bracket stores the S stages and the root pass builds them).
+      if (tev || need || pts_interp) {
+        // The dense stages: for a t_eval point, a point output or an inline
+        // root search (a deferred bracket stores the S stages and the root
+        // pass builds them).
         if constexpr (kBi) {
-          bool build = tev;
+          bool build = tev || pts_interp;
           if constexpr (EV != 0) build = build || !a.ev.brk;
           if (build) stages<S, I>(k, y, t, h, bad);
         }
@@ -489,6 +498,8 @@ struct ErkLane {
           return true;
         };
         if (tev && !teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, yh, interp, tail))
+          sol = kOutOfBounds;
+        if (pts_interp && !points_emit<D>(a, i, dir, tf, pushed, tail, t_new, tp, yh, interp))
           sol = kOutOfBounds;
         if constexpr (EV != 0) {
           if (need && sol < 0) {

@@ -67,6 +67,15 @@ struct OdeArgs {
                                 // device (null: n)
   void* scratch;                // caller-provided launch workspace (no allocation), or null
   int64_t scratch_bytes;
+  // Variable-count point output (kPoints*): slots pts_offset[i] .. pts_offset[i+1]
+  // of pts_t / pts_y (state at [s * D + c]); n_pts[i] = points pushed.
+  int32_t pts_mode;
+  int32_t pts_n;     // DenseSolout n
+  double pts_dt;     // EvenSolout dt
+  const int64_t* pts_offset;
+  double* pts_t;
+  double* pts_y;
+  int64_t* n_pts;
 };
 
 // Tolerances of one trajectory (Tolerance, src/tolerance.rs:11-68): a scalar
@@ -177,6 +186,83 @@ DE_HD void teval_tail_nan(const OdeArgs& a, int64_t i, int32_t from) {
     for (int c = 0; c < D; ++c) a.dense[((int64_t)e * D + c) * a.n + i] = as_f64(kNaNBits);
 }
 
+// Variable-count point outputs: DefaultSolout (src/solout/default.rs:71-91),
+// DenseSolout (src/solout/dense.rs:81-111) and EvenSolout (src/solout/
+// even.rs:83-190).  Point k of trajectory i goes to slot pts_offset[i] + k
+// while the trajectory's slots last; `pushed` counts every point.  `last` is
+// EvenSolout's last_output_t, which the start call sets (that call has
+// t_prev = t0 exactly, so the solout's other first-call branch never runs).
+enum : int32_t { kPointsNone = 0, kPointsSteps = 1, kPointsDense = 2, kPointsEven = 3 };
+
+template <int D>
+DE_HD void points_push(const OdeArgs& a, int64_t i, int32_t& pushed, double t, const double* y) {
+  const int64_t s = a.pts_offset[i] + pushed;
+  if (s < a.pts_offset[i + 1]) {
+    a.pts_t[s] = t;
+#pragma unroll
+    for (int c = 0; c < D; ++c) a.pts_y[s * D + c] = y[c];
+  }
+  pushed += 1;
+}
+
+// The solout call after init (t_curr = t_prev = t0): every strategy pushes (t0, y0).
+template <int D>
+DE_HD void points_start(const OdeArgs& a, int64_t i, int32_t& pushed, double& last, double t0,
+                        const double* y0) {
+  points_push<D>(a, i, pushed, t0, y0);
+  last = t0;
+}
+
+// The solout call of an accepted step [t_prev, t_curr]; false when an
+// interpolation failed (the solouts unwrap it: the reference panics).
+template <int D, class Interp>
+DE_HD bool points_emit(const OdeArgs& a, int64_t i, double dir, double tf, int32_t& pushed,
+                       double& last, double t_curr, double t_prev, const double* y_curr,
+                       Interp&& interp) {
+  double yi[D];
+  if (a.pts_mode == kPointsSteps) {
+    points_push<D>(a, i, pushed, t_curr, y_curr);
+    return true;
+  }
+  if (a.pts_mode == kPointsDense) {
+    if (t_prev != t_curr) {
+      for (int32_t k = 1; k < a.pts_n; ++k) {
+        const double h_old = t_curr - t_prev;
+        const double ti = t_prev + (double)k * h_old / (double)a.pts_n;
+        if (!interp(ti, yi)) return false;
+        points_push<D>(a, i, pushed, ti, yi);
+      }
+    }
+    points_push<D>(a, i, pushed, t_curr, y_curr);
+    return true;
+  }
+  const double dt = a.pts_dt;
+  const double tol = fabs(dt) * 1e-12 + 2.220446049250313080847e-16 * 10.0;
+  double ti = last + dt * dir;
+  while ((dir > 0.0 && ti <= t_curr) || (dir < 0.0 && ti >= t_curr)) {
+    if ((dir > 0.0 && ti >= t_prev && ti <= t_curr) ||
+        (dir < 0.0 && ti <= t_prev && ti >= t_curr)) {
+      if (!(fabs(ti - last) <= tol)) {
+        if (!interp(ti, yi)) return false;
+        points_push<D>(a, i, pushed, ti, yi);
+        last = ti;
+      }
+    }
+    ti += dt * dir;
+  }
+  if (t_curr == tf) {
+    if (fabs(last - tf) <= tol) {
+      pushed -= 1;  // Solution::pop, then the exact final point
+      points_push<D>(a, i, pushed, tf, y_curr);
+      last = tf;
+    } else if (last != tf) {
+      points_push<D>(a, i, pushed, tf, y_curr);
+      last = tf;
+    }
+  }
+  return true;
+}
+
 // InitialStepSize::<Ordinary>::compute (src/methods/h_init.rs:45-135).
 template <class F, class Tol, int ORDER = 5>
 DE_HD double lane_h_init(const F& f, double t0, double tf, const double* y0, const Tol& tol,
@@ -257,6 +343,7 @@ struct Dopri5Lane {
         st == kBadInput ? 0 : (int64_t)L.base_ev + 6 * (int64_t)L.rej + 7 * (int64_t)L.acc;
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
+    if (DENSE && EV == 0 && a.n_pts) a.n_pts[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
     if constexpr (EV != 0) {
       if (a.ev.brk)
@@ -383,7 +470,9 @@ struct Dopri5Lane {
   // event detection; the DefaultSolout pushes (t0, y0).
   DE_HD void solout_start(const OdeArgs& a, int64_t i) {
     tail = as_f64(kNaNBits);
-    if (DENSE && a.n_eval > 0)
+    if (DENSE && EV == 0 && a.pts_mode != kPointsNone)
+      points_start<D>(a, i, pushed, tail, t, y);
+    else if (DENSE && a.n_eval > 0)
       teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
                     [](double, double*) { return false; }, tail);
     else
@@ -505,12 +594,13 @@ struct Dopri5Lane {
       bool need = false;
       if constexpr (EV != 0) need = take && ev.pre(a.ev, t_new, ynew);
       const bool tev = DENSE && EV != 2 && take && a.n_eval > 0;
+      const bool pts = DENSE && EV == 0 && take && a.pts_mode != kPointsNone;
       // cont = (y, ydiff, bspl, cont3, 0*h) and the Horner form of
       // ordinary.rs:312-333, on [t, t_new] (ordinary.rs:303-309).
       double cont1[D], cont2[D], cont3[D];
       const double hp = h, tp = t;
       const double* y0p = y;
-      if (tev || (need && !a.ev.brk)) {
+      if (tev || (pts && a.pts_mode != kPointsSteps) || (need && !a.ev.brk)) {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
           double ydiff = ynew[c] + (-1.0) * y[c];
@@ -539,6 +629,9 @@ struct Dopri5Lane {
       // or the crossing solout alone.
       if (tev) {
         if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
+          sol = kOutOfBounds;
+      } else if (pts) {
+        if (!points_emit<D>(a, i, dir, tf, pushed, tail, t_new, tp, ynew, interp))
           sol = kOutOfBounds;
       } else {
         tail = take ? t_new : tail;
@@ -648,6 +741,7 @@ struct Dop853Lane {
         st == kBadInput ? 0 : (int64_t)L.base_ev + 11 * (int64_t)(L.acc + L.rej) + 4 * (int64_t)L.acc;
     a.status[i] = st;
     if (DENSE && a.n_dense) a.n_dense[i] = L.pushed;
+    if (DENSE && EV == 0 && a.n_pts) a.n_pts[i] = L.pushed;
     if (DENSE && a.n_eval > 0) teval_tail_nan<D>(a, i, L.next_eval);
     if constexpr (EV != 0) {
       if (a.ev.brk)
@@ -760,7 +854,9 @@ struct Dop853Lane {
     countdown = 100;
     rejected_last = false;
     tail = as_f64(kNaNBits);
-    if (DENSE && a.n_eval > 0)
+    if (DENSE && EV == 0 && a.pts_mode != kPointsNone)
+      points_start<D>(a, i, pushed, tail, t, y);
+    else if (DENSE && a.n_eval > 0)
       teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
                     [](double, double*) { return false; }, tail);
     else
@@ -899,8 +995,9 @@ struct Dop853Lane {
       bool need = false;
       if constexpr (EV != 0) need = take && ev.pre(a.ev, t_new, ynew);
       const bool tev = DENSE && EV != 2 && take && a.n_eval > 0;
+      const bool pts = DENSE && EV == 0 && take && a.pts_mode != kPointsNone;
       double cont[8][D];
-      if (tev || need) {
+      if (tev || (pts && a.pts_mode != kPointsSteps) || need) {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
           double ydiff = ynew[c] + (-1.0) * y[c];
@@ -939,6 +1036,9 @@ struct Dop853Lane {
       };
       if (tev) {
         if (!teval_emit<D>(a, i, dir, next_eval, pushed, t_new, tp, ynew, interp, tail))
+          sol = kOutOfBounds;
+      } else if (pts) {
+        if (!points_emit<D>(a, i, dir, tf, pushed, tail, t_new, tp, ynew, interp))
           sol = kOutOfBounds;
       } else {
         tail = take ? t_new : tail;
@@ -1022,6 +1122,7 @@ struct FixedLane {
     a.evals[i] = ev;
     a.status[i] = st;
     if (a.n_dense) a.n_dense[i] = pushed;
+    if (EV == 0 && a.n_pts) a.n_pts[i] = pushed;
     if (a.n_eval > 0) teval_tail_nan<D>(a, i, next_eval);
     if constexpr (EV != 0) {
       if (a.ev.brk)
@@ -1104,7 +1205,9 @@ struct FixedLane {
     ev = 1;
     steps = 0;
     tail = as_f64(kNaNBits);
-    if (EV != 2 && a.n_eval > 0)
+    if (EV == 0 && a.pts_mode != kPointsNone)
+      points_start<D>(a, i, pushed, tail, t, y);
+    else if (EV != 2 && a.n_eval > 0)
       teval_emit<D>(a, i, dir, next_eval, pushed, t, t, y,
                     [](double, double*) { return false; }, tail);
     else
@@ -1153,7 +1256,7 @@ struct FixedLane {
     ev += 1;
     acc += 1;
     int32_t sol = -1;
-    if (EV != 0 || a.n_eval > 0) {
+    if (EV != 0 || a.n_eval > 0 || a.pts_mode != kPointsNone) {
       const double* yp = y;
       const double* kp = k0;
       const double tc = t;
@@ -1177,6 +1280,9 @@ struct FixedLane {
       };
       if (EV != 2 && a.n_eval > 0) {
         if (!teval_emit<D>(a, i, dir, next_eval, pushed, t, tp, ynext, interp, tail))
+          sol = kOutOfBounds;
+      } else if (EV == 0 && a.pts_mode != kPointsNone) {
+        if (!points_emit<D>(a, i, dir, tf, pushed, tail, t, tp, ynext, interp))
           sol = kOutOfBounds;
       } else {
         tail = t;

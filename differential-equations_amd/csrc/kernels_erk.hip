@@ -21,7 +21,7 @@ namespace parity {
 template <class F, class Tab>
 static int launch_erk_tab(const OdeArgs& a, int schedule, hipStream_t s) {
   const bool vtol = a.rtol_comp || a.atol_comp;
-  if (a.n_eval > 0)
+  if (a.n_eval > 0 || a.pts_mode != kPointsNone)
     return vtol ? launch<ErkLane<F, Tab, true, true>>(a, schedule, s)
                 : launch<ErkLane<F, Tab, true, false>>(a, schedule, s);
   return vtol ? launch<ErkLane<F, Tab, false, true>>(a, schedule, s)

@@ -24,7 +24,7 @@ static int launch_method(int method, const OdeArgs& a, int schedule, hipStream_t
   switch (method) {
     case kMethodDopri5: {
       const bool vtol = a.rtol_comp || a.atol_comp;
-      if (a.n_eval > 0)
+      if (a.n_eval > 0 || a.pts_mode != kPointsNone)
         return vtol ? launch<Dopri5Lane<F, true, true>>(a, schedule, s)
                     : launch<Dopri5Lane<F, true, false>>(a, schedule, s);
       return vtol ? launch<Dopri5Lane<F, false, true>>(a, schedule, s)
@@ -35,7 +35,7 @@ static int launch_method(int method, const OdeArgs& a, int schedule, hipStream_t
     case kMethodEuler: return launch<FixedLane<F, 1>>(a, schedule, s);
     case kMethodDop853: {
       const bool vtol = a.rtol_comp || a.atol_comp;
-      if (a.n_eval > 0)
+      if (a.n_eval > 0 || a.pts_mode != kPointsNone)
         return vtol ? launch<Dop853Lane<F, true, true>>(a, schedule, s)
                     : launch<Dop853Lane<F, true, false>>(a, schedule, s);
       return vtol ? launch<Dop853Lane<F, false, true>>(a, schedule, s)

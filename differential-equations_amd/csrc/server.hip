@@ -839,6 +839,7 @@ int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
   // solout, validated as de_ode_batch_solve validates it; per-component
   // vectors go through de_ode_batch_solve.
   if (b->rtol_comp || b->atol_comp) return DE_ERR_UNSUPPORTED;
+  if (b->points != DE_POINTS_NONE) return DE_ERR_UNSUPPORTED;  // final state, t_eval or events
   const int ev_fn = b->event_solout == DE_SOLOUT_CROSSING     ? DE_EVENT_COMPONENT
                     : b->event_solout == DE_SOLOUT_HYPERPLANE ? DE_EVENT_PLANE
                                                                : b->event_fn;

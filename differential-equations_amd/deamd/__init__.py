@@ -345,12 +345,37 @@ class OdeEnsemble:
         # (DE_SOLOUT_*, Event) of an event / crossing solout, or None.
         self._event = None
         self.max_events = 16
+        # Variable-count point output: (DE_POINTS_*, dense n, even dt).
+        self._points = (DE_POINTS_NONE, 0, 0.0)
 
     def method(self, m):
         self._method = m
         return self
 
+    def all_steps(self):
+        """DefaultSolout, the reference's default output: (t0, y0) and every
+        accepted step (src/solout/default.rs:71-91).  The points come back in
+        Solution.points(i)."""
+        return self._set_points(DE_POINTS_STEPS, 0, 0.0)
+
+    def dense(self, n):
+        """IVP::dense(n) (src/ivp.rs:652-655): DenseSolout, n - 1 interpolated
+        points inside every accepted step, then the step itself."""
+        return self._set_points(DE_POINTS_DENSE, int(n), 0.0)
+
+    def even(self, dt):
+        """IVP::even(dt) (src/ivp.rs:645-649): EvenSolout, the points t0, t0 + dt,
+        ... in the integration interval, and tf."""
+        return self._set_points(DE_POINTS_EVEN, 0, float(dt))
+
+    def _set_points(self, mode, n, dt):
+        self._points = (mode, n, dt)
+        self._t_eval = None   # with_solout replaces the whole solout
+        self._event = None
+        return self
+
     def t_eval(self, points):
+        self._points = (DE_POINTS_NONE, 0, 0.0)
         pts = np.sort(np.asarray(points, dtype=np.float64))
         t0 = float(np.asarray(self.t0).reshape(-1)[0])
         tf = float(np.asarray(self.tf).reshape(-1)[0])
@@ -369,6 +394,7 @@ class OdeEnsemble:
     def crossing(self, component_idx, threshold, direction=CrossingDirection.Both):
         """IVP::crossing (src/ivp.rs:682-692): CrossingSolout replaces the output."""
         self._t_eval = None
+        self._points = (DE_POINTS_NONE, 0, 0.0)
         self._event = (DE_SOLOUT_CROSSING,
                        Event.component(component_idx, threshold, EventConfig(direction)))
         return self
@@ -377,6 +403,7 @@ class OdeEnsemble:
         """IVP::hyperplane_crossing (src/ivp.rs:695-705) with the identity
         extractor: the hyperplane lies in the state space."""
         self._t_eval = None
+        self._points = (DE_POINTS_NONE, 0, 0.0)
         self._event = (DE_SOLOUT_HYPERPLANE,
                        Event.plane(point, normal, EventConfig(direction)))
         return self
@@ -396,8 +423,9 @@ class OdeEnsemble:
                 self.flags & ~DE_FLAG_FAST_MATH)
         return self
 
-    def batch(self):
-        """The de_ode_batch descriptor (host pointers) plus result buffers."""
+    def batch(self, points_offset=None):
+        """The de_ode_batch descriptor (host pointers) plus result buffers;
+        with a point output, `points_offset` (n + 1 slot offsets) sizes it."""
         s = self.system
         b = ode_defaults()
         b.family = s.family
@@ -489,10 +517,50 @@ class OdeEnsemble:
                     keep.append(v)
                     setattr(b, "event_" + name, _ptr(v))
         res = Solution(self.n, state_len, n_eval, cap)
+        mode, pn, pdt = self._points
+        if mode != DE_POINTS_NONE and points_offset is not None:
+            b.points, b.points_n, b.points_dt = mode, pn, pdt
+            res.set_points(np.ascontiguousarray(points_offset, dtype=np.int64),
+                           0 if isinstance(s, SpringChain) else s.dim)
         return b, res, keep
 
-    def solve(self, stream=None):
-        b, res, keep = self.batch()
+    def points_offsets(self, accepted=None):
+        """Slot offsets (n + 1) that hold every point of the point output:
+        accepted + 1 per trajectory (DefaultSolout), n * accepted + 1
+        (DenseSolout), floor(|tf - t0| / dt) + 3 (EvenSolout)."""
+        mode, pn, pdt = self._points
+        if mode == DE_POINTS_EVEN:
+            span = np.abs(np.broadcast_to(np.asarray(self.tf, dtype=np.float64), (self.n,)) -
+                          np.broadcast_to(np.asarray(self.t0, dtype=np.float64), (self.n,)))
+            cnt = np.floor(span / pdt).astype(np.int64) + 3
+        else:
+            acc = np.asarray(accepted, dtype=np.int64)
+            cnt = acc + 1 if mode == DE_POINTS_STEPS else pn * acc + 1
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        np.cumsum(cnt, out=off[1:])
+        return off
+
+    def solve(self, stream=None, solver=None):
+        """The ensemble's solve.  With a point output from all_steps() or
+        dense(n), a first solve counts each trajectory's accepted steps (the
+        results are deterministic) and a second one writes the points."""
+        solver = solver or (lambda e, off: e._solve_once(stream, off))
+        mode = self._points[0]
+        offsets = None
+        if mode in (DE_POINTS_STEPS, DE_POINTS_DENSE):
+            saved = self._points
+            self._points = (DE_POINTS_NONE, 0, 0.0)
+            try:
+                first = solver(self, None)
+            finally:
+                self._points = saved
+            offsets = self.points_offsets(first.accepted)
+        elif mode == DE_POINTS_EVEN:
+            offsets = self.points_offsets()
+        return solver(self, offsets)
+
+    def _solve_once(self, stream, points_offset):
+        b, res, keep = self.batch(points_offset)
         _abi.check(lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), stream))
         res._solved_by = (b, keep)  # the descriptor, for the CSV writer
         return res
@@ -524,6 +592,23 @@ class Solution:
         self.event_t = np.full(max_events * n, np.nan) if max_events else None
         self.event_y = np.full(max_events * state_len, np.nan) if max_events else None
         self.n_events = np.zeros(n, dtype=np.int32) if max_events is not None else None
+        self.points_offset = self.points_t = self.points_y = self.n_points = None
+
+    def set_points(self, offsets, dim):
+        """Buffers of a point output: slots offsets[i] .. offsets[i + 1] of trajectory i."""
+        self.points_offset = offsets
+        cap = int(offsets[-1])
+        self.points_t = np.full(max(cap, 1), np.nan)
+        self.points_y = np.full(max(cap * dim, 1), np.nan)
+        self.n_points = np.zeros(self.n, dtype=np.int64)
+
+    def points(self, i):
+        """(t[k], y[k, dim]) of trajectory i's Solution points (the kept ones)."""
+        dim = len(self.y) // self.n
+        lo, hi = int(self.points_offset[i]), int(self.points_offset[i + 1])
+        k = min(int(self.n_points[i]), hi - lo)
+        return (self.points_t[lo:lo + k].copy(),
+                self.points_y[lo * dim:(lo + k) * dim].reshape(k, dim).copy())
 
     def events(self, i):
         """(t[k], y[k, dim]) of the event points kept for trajectory i."""
@@ -584,6 +669,10 @@ class Solution:
         r.event_t = _ptr(self.event_t)
         r.event_y = _ptr(self.event_y)
         r.n_events = _ptr(self.n_events)
+        r.points_offset = _ptr(self.points_offset)
+        r.points_t = _ptr(self.points_t)
+        r.points_y = _ptr(self.points_y)
+        r.n_points = _ptr(self.n_points)
         self._struct = r
         return r
 

@@ -74,6 +74,10 @@ DE_SOLOUT_DEFAULT = 0
 DE_SOLOUT_EVENT = 1
 DE_SOLOUT_CROSSING = 2
 DE_SOLOUT_HYPERPLANE = 3
+DE_POINTS_NONE = 0
+DE_POINTS_STEPS = 1
+DE_POINTS_DENSE = 2
+DE_POINTS_EVEN = 3
 
 DE_EVENT_COMPONENT = 0
 DE_EVENT_TIME = 1
@@ -115,6 +119,7 @@ class OdeBatch(C.Structure):
         ("event_value", C.c_double), ("event_value_traj", C.c_void_p),
         ("event_point", C.c_void_p), ("event_normal", C.c_void_p),
         ("max_rejects", C.c_int64),
+        ("points", C.c_int32), ("points_n", C.c_int32), ("points_dt", C.c_double),
     ]
 
 
@@ -124,6 +129,8 @@ class OdeResult(C.Structure):
         ("accepted", C.c_void_p), ("rejected", C.c_void_p), ("evals", C.c_void_p),
         ("status", C.c_void_p), ("dense", C.c_void_p), ("n_dense", C.c_void_p),
         ("event_t", C.c_void_p), ("event_y", C.c_void_p), ("n_events", C.c_void_p),
+        ("points_offset", C.c_void_p), ("points_t", C.c_void_p), ("points_y", C.c_void_p),
+        ("n_points", C.c_void_p),
     ]
 
 

@@ -120,6 +120,17 @@ extern "C" {
 #define DE_SOLOUT_HYPERPLANE 3 /* IVP::hyperplane_crossing with the identity extractor
                                   (src/solout/hyperplane.rs) */
 
+/* Solution points of the output strategies that push a variable number of points
+ * (src/ivp.rs:643-655 and the default):
+ * de_ode_batch.points selects one; the points come back in de_ode_result.points_*. */
+#define DE_POINTS_NONE 0  /* final state only (and t_eval / events when asked) */
+#define DE_POINTS_STEPS 1 /* DefaultSolout: (t0, y0) and every accepted step
+                             (src/solout/default.rs:71-91) */
+#define DE_POINTS_DENSE 2 /* IVP::dense(n): DenseSolout, n - 1 interpolated points inside
+                             every accepted step, then the step (src/solout/dense.rs:81-111) */
+#define DE_POINTS_EVEN 3  /* IVP::even(dt): EvenSolout, t0, t0 + dt, ... and tf
+                             (src/solout/even.rs:83-190) */
+
 /* Event functions g(t, y) (Event::event, src/solout/event.rs) of DE_SOLOUT_EVENT. */
 #define DE_EVENT_COMPONENT 0 /* y[event_component] - value */
 #define DE_EVENT_TIME 1      /* t - value */
@@ -191,6 +202,11 @@ typedef struct de_ode_batch {
   /* ExplicitRungeKutta::max_rejects of the embedded family (erk/mod.rs:81,140,201):
    * consecutive rejections that end the solve with Error::Stiffness. */
   int64_t max_rejects;
+  /* Output strategy with a variable number of points (DE_POINTS_*; fixed-dimension
+   * families, without t_eval or an event solout). */
+  int32_t points;
+  int32_t points_n;  /* DE_POINTS_DENSE: n (points per step, including the step) */
+  double points_dt;  /* DE_POINTS_EVEN: dt */
 } de_ode_batch;
 
 typedef struct de_ode_result {
@@ -205,6 +221,18 @@ typedef struct de_ode_result {
   double* event_t;   /* [max_events * n] event times, event_t[k * n + i]; NaN past the last */
   double* event_y;   /* [max_events * dim * n] event states, event_y[(k * dim + c) * n + i] */
   int32_t* n_events; /* [n] event points pushed (those past max_events are not kept), or NULL */
+  /* DE_POINTS_*: trajectory i's Solution points go to slots points_offset[i] ..
+   * points_offset[i+1] (n + 1 values, given by the caller): time points_t[s], state
+   * points_y[s * dim + c].  n_points[i] is the number of points the solout pushed; those
+   * beyond the trajectory's slots are counted, not kept, and slots past n_points[i] are
+   * left unspecified.  Slot counts that always
+   * suffice: accepted + 1 (STEPS), n * accepted + 1 (DENSE), from the accepted counts
+   * of a solve of the same ensemble (results are deterministic), and
+   * floor(|tf - t0| / dt) + 3 (EVEN). */
+  const int64_t* points_offset;
+  double* points_t;
+  double* points_y;
+  int64_t* n_points;
 } de_ode_result;
 
 typedef struct de_sde_batch {

@@ -567,15 +567,117 @@ struct EventState {
   }
 };
 
+// ------------------------------------------- variable-count point outputs ----
+// DefaultSolout (src/solout/default.rs:71-91), DenseSolout (src/solout/
+// dense.rs:81-111) and EvenSolout (src/solout/even.rs:83-190): the points the
+// solout pushes into the Solution of one trajectory; the first `cap` are kept
+// (t[k], y[k * dim + c]) and all are counted.  interp(te, out) is the method's
+// Interpolation::interpolate (false: Err(OutOfBounds), on which the solouts'
+// unwrap panics).
+struct Points {
+  int mode = DE_POINTS_NONE;
+  int dense_n = 0;
+  double dt = 0.0, t0 = 0.0, tf = 0.0, direction = 0.0;
+  bool has_last = false;  // EvenSolout::last_output_t
+  double last = 0.0;
+  int n = 0;
+  double* t = nullptr;
+  double* y = nullptr;
+  int64_t cap = 0;
+  int64_t pushed = 0;
+
+  void push(double tt, const double* yy) {
+    if (pushed < cap) {
+      t[pushed] = tt;
+      copy(y + pushed * n, yy, n);
+    }
+    pushed += 1;
+  }
+
+  template <class Interp>
+  int call(double t_curr, double t_prev, const double* y_curr, const double* y_prev,
+           Interp&& interp) {
+    double yi[kEvMax > 16 ? kEvMax : 16];
+    if (mode == DE_POINTS_STEPS) {
+      push(t_curr, y_curr);
+      return kContinue;
+    }
+    if (mode == DE_POINTS_DENSE) {
+      if (t_prev != t_curr) {
+        for (int i = 1; i < dense_n; ++i) {
+          double h_old = t_curr - t_prev;
+          double ti = t_prev + (double)i * h_old / (double)dense_n;
+          if (!interp(ti, yi)) return kPanic;
+          push(ti, yi);
+        }
+      }
+      push(t_curr, y_curr);
+      return kContinue;
+    }
+    // EvenSolout
+    const double offset = fmod(t0, dt);
+    const double eps = 2.220446049250313080847e-16;
+    const double tol = fabs(dt) * 1e-12 + eps * 10.0;
+    double start_t;
+    if (has_last) {
+      start_t = last + dt * direction;
+    } else if (fabs(t_prev - t0) < eps) {
+      push(t0, y_prev);
+      has_last = true;
+      last = t0;
+      start_t = t0 + dt * direction;
+    } else {
+      double rem = fmod(t_prev - offset, dt);
+      if (direction > 0.0)
+        start_t = fabs(rem) < eps ? t_prev : t_prev + (dt - rem);
+      else
+        start_t = fabs(rem) < eps ? t_prev : t_prev - rem;
+    }
+    double ti = start_t;
+    while ((direction > 0.0 && ti <= t_curr) || (direction < 0.0 && ti >= t_curr)) {
+      if ((direction > 0.0 && ti >= t_prev && ti <= t_curr) ||
+          (direction < 0.0 && ti <= t_prev && ti >= t_curr)) {
+        if (!(has_last && fabs(ti - last) <= tol)) {
+          if (!interp(ti, yi)) return kPanic;
+          push(ti, yi);
+          has_last = true;
+          last = ti;
+        }
+      }
+      ti += dt * direction;
+    }
+    if (t_curr == tf) {
+      if (has_last) {
+        if (fabs(last - tf) <= tol) {
+          pushed -= 1;  // Solution::pop, then the exact final point
+          push(tf, y_curr);
+          last = tf;
+        } else if (last != tf) {
+          push(tf, y_curr);
+          last = tf;
+        }
+      } else {
+        push(tf, y_curr);
+        has_last = true;
+        last = tf;
+      }
+    }
+    return kContinue;
+  }
+};
+
 // The solout chain of one trajectory: DefaultSolout or TEvalSolout, wrapped by
 // the event solout (EventWrappedSolout::solout, event.rs:499-522: the base
 // first, then detection), or a crossing solout alone.
 struct Solouts {
   TEval* te = nullptr;
   EventState* ev = nullptr;
+  Points* pts = nullptr;  // a variable-count point output instead of the above
   SolTail tail;
   template <class Interp>
-  int call(double t_curr, double t_prev, const double* y_curr, int n, Interp&& interp) {
+  int call(double t_curr, double t_prev, const double* y_curr, const double* y_prev, int n,
+           Interp&& interp) {
+    if (pts) return pts->call(t_curr, t_prev, y_curr, y_prev, interp);
     if (ev && (ev->sp->solout == DE_SOLOUT_CROSSING || ev->sp->solout == DE_SOLOUT_HYPERPLANE))
       return ev->crossing(t_curr, t_prev, y_curr, interp);
     int f;
@@ -1215,7 +1317,7 @@ static TrajResult solve_ode(Method& m, double t0, double tf, const double* y0, S
     copy(y_out, m.y, n);
     return r;
   };
-  int f = sol.call(m.t, m.t_prev, m.y, n, interp);
+  int f = sol.call(m.t, m.t_prev, m.y, m.y_prev, n, interp);
   if (f != kContinue) return end_with(f);
   m.rejected_status = false;  // Status::Solving
   for (;;) {
@@ -1238,7 +1340,7 @@ static TrajResult solve_ode(Method& m, double t0, double tf, const double* y0, S
       continue;
     }
     r.accepted += 1;
-    f = sol.call(m.t, m.t_prev, m.y, n, interp);
+    f = sol.call(m.t, m.t_prev, m.y, m.y_prev, n, interp);
     if (f != kContinue) return end_with(f);
     if (fabs(tf - m.t) <= de::kTenEps) break;
   }
@@ -1524,11 +1626,18 @@ static void parallel_for(int64_t n, int threads, Fn&& fn) {
 template <class Sys>
 static void run_ode_traj(const Sys& sys, const de_ode_batch* b, const Settings& st, int64_t i,
                          const double* y0, double* y_out, double* dense, int32_t* n_dense,
-                         const EventSpec* es, EventOut* eo, TrajResult* res) {
+                         const EventSpec* es, EventOut* eo, TrajResult* res,
+                         Points* pts = nullptr) {
   double t0 = b->t0 ? b->t0[i] : b->t0_scalar;
   double tf = b->tf ? b->tf[i] : b->tf_scalar;
   TEval te;
   Solouts sol;
+  if (pts) {
+    pts->t0 = t0;
+    pts->tf = tf;
+    pts->direction = de::signum(tf - t0);
+    sol.pts = pts;
+  }
   if (b->n_eval > 0 && dense && (!es || es->solout <= DE_SOLOUT_EVENT)) {
     te.t_evals = b->t_eval;
     te.n_eval = b->n_eval;
@@ -1610,8 +1719,22 @@ static void run_fixed_family(const de_ode_batch* b, de_ode_result* out, const Se
     eo.y = ey.data();
     eo.cap = cap;
     TrajResult r;
+    Points pts;
+    const bool with_pts = b->points != DE_POINTS_NONE && out->points_offset;
+    if (with_pts) {
+      const int64_t off = out->points_offset[i];
+      pts.mode = b->points;
+      pts.dense_n = b->points_n;
+      pts.dt = b->points_dt;
+      pts.n = D;
+      pts.t = out->points_t + off;
+      pts.y = out->points_y + off * D;
+      pts.cap = out->points_offset[i + 1] - off;
+    }
     run_ode_traj(sys, b, st, i, y0, yo, dense.empty() ? nullptr : dense.data(),
-                 out->n_dense ? &out->n_dense[i] : nullptr, es, &eo, &r);
+                 out->n_dense ? &out->n_dense[i] : nullptr, es, &eo, &r,
+                 with_pts ? &pts : nullptr);
+    if (with_pts && out->n_points) out->n_points[i] = pts.pushed;
     for (int c = 0; c < D; ++c) out->y_final[c * n + i] = yo[c];
     if (!dense.empty())
       for (int e = 0; e < b->n_eval; ++e)

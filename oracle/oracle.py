@@ -123,12 +123,15 @@ def ode_csv(ensemble, solution, source):
 
 
 def solve_ode(ensemble, pow_mode=POW_RESTATED, threads=1):
-    """Run an deamd.OdeEnsemble through the oracle; returns a deamd.Solution."""
-    b, res, keep = ensemble.batch()
-    rc = load().de_oracle_ode_solve(C.byref(b), C.byref(res.struct()), pow_mode, threads)
-    if rc != 0:
-        raise RuntimeError("oracle ode solve failed: %d" % rc)
-    return res
+    """Run an deamd.OdeEnsemble through the oracle; returns a deamd.Solution
+    (with a point output, sized by a first counting solve as on the GPU)."""
+    def once(ens, points_offset):
+        b, res, keep = ens.batch(points_offset)
+        rc = load().de_oracle_ode_solve(C.byref(b), C.byref(res.struct()), pow_mode, threads)
+        if rc != 0:
+            raise RuntimeError("oracle ode solve failed: %d" % rc)
+        return res
+    return ensemble.solve(solver=once)
 
 
 def solve_sde(ensemble, threads=1):
