@@ -318,11 +318,15 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
 }
 
 // ------------------------------------------------------------ to_csv ----
-// The device CSV writer (kernels_csv.hip) is switched off: its first run on
-// the GPU did not return, and no cause has been shown yet (its index
-// arithmetic, bounds and termination check out in a CPU emulation with the
-// same inputs).  The entry points validate their arguments and then refuse.
-constexpr bool kCsvDeviceEnabled = false;
+// The device CSV writer (kernels_csv.hip): bounded loops only (the digit
+// search has fixed caps, the scan is three passes of the library's own with
+// no inter-block waits), workspace from the library pool.
+constexpr bool kCsvDeviceEnabled = true;
+
+int csv_launch_fail(int r) {
+  if (r == -2) return fail(DE_ERR_HIP, "csv formatter reached a loop cap (a bug: report it)");
+  return hip_fail(hipGetLastError(), "csv launch");
+}
 
 int csv_args(const de_ode_batch* b, const de_ode_result* r, int32_t source, de::CsvArgs* a) {
   if (!b || !r) return fail(DE_ERR_INVALID, "null descriptor");
@@ -421,7 +425,7 @@ int de_ode_csv_offsets(const de_ode_batch* b, const de_ode_result* r, int32_t so
     d_off = st.out(offsets, (size_t)a.n + 1);
     if (st.err != hipSuccess) return hip_fail(st.err, "csv staging");
   }
-  if (de::launch_csv(a, d_off, nullptr, 0, s) != 0) return hip_fail(hipGetLastError(), "csv launch");
+  if (int r = de::launch_csv(a, d_off, nullptr, 0, s)) return csv_launch_fail(r);
   if (host) {
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "csv sync");
@@ -461,8 +465,8 @@ int de_ode_csv_write(const de_ode_batch* b, const de_ode_result* r, int32_t sour
   }
   int64_t* d_off = st.out((int64_t*)offsets, (size_t)a.n + 1);  // the layout, recomputed
   if (st.err != hipSuccess) return hip_fail(st.err, "csv staging");
-  if (de::launch_csv(a, d_off, cap > 0 ? d_out : nullptr, cap, s) != 0)
-    return hip_fail(hipGetLastError(), "csv launch");
+  if (int r = de::launch_csv(a, d_off, cap > 0 ? d_out : nullptr, cap, s))
+    return csv_launch_fail(r);
   int64_t total = 0;
   hipError_t e = hipMemcpyAsync(&total, d_off + a.n, sizeof total, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -918,6 +922,21 @@ int de_diag_arith(const double* a, const double* b, const double* c, double* out
   hipError_t e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_fail(e, "diag_arith");
   st.back(out, dout, 3 * n);
+  return st.err == hipSuccess ? DE_OK : hip_fail(st.err, "copy back");
+}
+
+int de_diag_fmt(const double* x, char* out, int32_t* len, int64_t n) {
+  if (n <= 0) return DE_OK;
+  Staging st;
+  const double* dx = st.in(x, n);
+  char* dout = st.out(out, (size_t)n * 32);
+  int32_t* dlen = st.out(len, n);
+  if (st.err != hipSuccess) return hip_fail(st.err, "staging");
+  if (de::launch_fmt_diag(dx, dout, dlen, n, 0) != 0) return hip_fail(hipGetLastError(), "diag_fmt");
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "diag_fmt");
+  st.back(out, dout, (size_t)n * 32);
+  st.back(len, dlen, n);
   return st.err == hipSuccess ? DE_OK : hip_fail(st.err, "copy back");
 }
 

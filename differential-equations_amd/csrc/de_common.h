@@ -80,6 +80,29 @@ DE_HD double constrain_step_size(double h, double h_min, double h_max) {
   return a < h_min ? sign * h_min : (a > h_max ? sign * h_max : h);
 }
 
+// The stiffness test's decision h * sqrt(num / den) > 6.1, with den > 0, as
+// the reference evaluates it (dormandprince/ordinary.rs:172-174: a division,
+// a square root and a product, each rounded), but usually without them.  For
+// h, num and den in [2^-300, 2^300] the products h^2 num and RN(6.1)^2 den
+// are within a relative 3 * 2^-53 of their exact values, and the rounded
+// expression within a relative 2.5 * 2^-53 of h sqrt(num / den); so when the
+// two products differ by more than a relative 1e-12 their order is the
+// decision.  Otherwise, and outside that range (h <= 0 included), the
+// reference's own sequence decides.
+DE_HD bool stiff_exceeds(double h, double num, double den) {
+  constexpr double kLo = 0x1p-300, kHi = 0x1p300;
+  constexpr double c = 6.1;
+  constexpr double c2 = c * c;  // rounded once; inside the 3 * 2^-53 above
+  const bool in_range = h > kLo && h < kHi && num > kLo && num < kHi && den > kLo && den < kHi;
+  if (in_range) {
+    const double lhs = (h * h) * num;
+    const double rhs = c2 * den;
+    if (lhs < rhs * (1.0 - 1e-12)) return false;
+    if (lhs > rhs * (1.0 + 1e-12)) return true;
+  }
+  return h * sqrt(num / den) > c;
+}
+
 // Per-trajectory status codes.  0..4 mirror Status::Complete and the Error
 // variants a solve can end with (src/error.rs:13-41), 5 Status::Interrupted
 // (an event solout's Terminate), 6 a solout's unwrap of a failed

@@ -35,26 +35,35 @@ DE_HD int32_t pow5bits(int32_t e) { return (int32_t)(((uint32_t)e * 1217359u) >>
 DE_HD uint32_t log10_pow2(int32_t e) { return ((uint32_t)e * 78913u) >> 18; }
 DE_HD uint32_t log10_pow5(int32_t e) { return ((uint32_t)e * 732923u) >> 20; }
 
+// High 64 bits of a 64 x 64-bit product from four 32 x 32-bit products: the
+// same integer operations on the host and the device, so the host build of
+// this file exercises exactly the device's arithmetic.
 DE_HD uint64_t umulh(uint64_t a, uint64_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __umul64hi(a, b);
-#else
-  return (uint64_t)(((unsigned __int128)a * b) >> 64);
-#endif
+  const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+  const uint64_t p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0, p11 = a1 * b1;
+  const uint64_t mid = (p00 >> 32) + (uint32_t)p01 + (uint32_t)p10;
+  return p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
 }
 
+// Every loop below has a fixed iteration cap: a 64-bit value has at most 27
+// factors 5 and 20 decimal digits, so a cap is never reached by a valid
+// input; reaching one marks the result bad (the writer reports it) instead
+// of looping.
+constexpr int kLoopCap = 32;
+
 // Number of factors 5 of v > 0: v * 5^-1 mod 2^64 stays <= (2^64 - 1) / 5
-// exactly while 5 divides v.
-DE_HD uint32_t pow5_factor(uint64_t v) {
+// exactly while 5 divides v.  kLoopCap (and *bad set) for v = 0.
+DE_HD uint32_t pow5_factor(uint64_t v, bool* bad) {
   uint32_t count = 0;
-  for (;;) {
+  for (int it = 0; it < kLoopCap; ++it) {
     v *= 14757395258967641293ull;
-    if (v > 3689348814741910323ull) break;
+    if (v > 3689348814741910323ull) return count;
     ++count;
   }
+  *bad = true;
   return count;
 }
-DE_HD bool multiple_of_pow5(uint64_t v, uint32_t p) { return pow5_factor(v) >= p; }
+DE_HD bool multiple_of_pow5(uint64_t v, uint32_t p, bool* bad) { return pow5_factor(v, bad) >= p; }
 DE_HD bool multiple_of_pow2(uint64_t v, uint32_t p) { return (v & ((1ull << p) - 1)) == 0; }
 
 // floor(m * mul / 2^j) for a 128-bit mul, 64 < j < 128, m < 2^55.
@@ -71,10 +80,12 @@ DE_HD uint64_t mul_shift(uint64_t m, const uint64_t* mul, int32_t j) {
 struct Decimal {
   uint64_t digits;  // value = digits * 10^exp
   int32_t exp;
+  bool bad;         // a loop cap was reached (never for a valid input)
 };
 
 // Shortest closest decimal of a finite nonzero double's magnitude.
 DE_HD Decimal shortest(uint64_t ieee_mantissa, uint32_t ieee_exponent) {
+  bool bad = false;
   int32_t e2;
   uint64_t m2;
   if (ieee_exponent == 0) {
@@ -103,11 +114,11 @@ DE_HD Decimal shortest(uint64_t ieee_mantissa, uint32_t ieee_exponent) {
     if (q <= 21) {
       // Only one of mp, mv, mm can be a multiple of 5, if any.
       if (mv % 5 == 0)
-        vr_tz = multiple_of_pow5(mv, q);
+        vr_tz = multiple_of_pow5(mv, q, &bad);
       else if (accept_bounds)
-        vm_tz = multiple_of_pow5(mv - 1 - mm_shift, q);
+        vm_tz = multiple_of_pow5(mv - 1 - mm_shift, q, &bad);
       else
-        vp -= multiple_of_pow5(mv + 2, q);
+        vp -= multiple_of_pow5(mv + 2, q, &bad);
     }
   } else {
     const uint32_t q = log10_pow5(-e2) - (-e2 > 1);
@@ -135,9 +146,13 @@ DE_HD Decimal shortest(uint64_t ieee_mantissa, uint32_t ieee_exponent) {
   uint32_t last = 0;
   uint64_t out;
   if (vm_tz || vr_tz) {
-    for (;;) {
+    for (int it = 0;; ++it) {
       const uint64_t vp10 = vp / 10, vm10 = vm / 10;
       if (vp10 <= vm10) break;
+      if (it == kLoopCap) {
+        bad = true;
+        break;
+      }
       const uint64_t vr10 = vr / 10;
       vm_tz &= (uint32_t)(vm - 10 * vm10) == 0;
       vr_tz &= last == 0;
@@ -148,9 +163,13 @@ DE_HD Decimal shortest(uint64_t ieee_mantissa, uint32_t ieee_exponent) {
       ++removed;
     }
     if (vm_tz) {
-      for (;;) {
+      for (int it = 0;; ++it) {
         const uint64_t vm10 = vm / 10;
         if ((uint32_t)(vm - 10 * vm10) != 0) break;
+        if (it == kLoopCap) {  // vm = 0
+          bad = true;
+          break;
+        }
         const uint64_t vp10 = vp / 10, vr10 = vr / 10;
         vr_tz &= last == 0;
         last = (uint32_t)(vr - 10 * vr10);
@@ -165,9 +184,13 @@ DE_HD Decimal shortest(uint64_t ieee_mantissa, uint32_t ieee_exponent) {
     out = vr + (((vr == vm) && (!accept_bounds || !vm_tz)) || last >= 5);
   } else {
     bool round_up = false;
-    for (;;) {
+    for (int it = 0;; ++it) {
       const uint64_t vp10 = vp / 10, vm10 = vm / 10;
       if (vp10 <= vm10) break;
+      if (it == kLoopCap) {
+        bad = true;
+        break;
+      }
       const uint64_t vr10 = vr / 10;
       round_up = (uint32_t)(vr - 10 * vr10) >= 5;
       vr = vr10;
@@ -177,7 +200,7 @@ DE_HD Decimal shortest(uint64_t ieee_mantissa, uint32_t ieee_exponent) {
     }
     out = vr + (vr == vm || round_up);
   }
-  return {out, e10 + removed};
+  return {out, e10 + removed, bad};
 }
 
 }  // namespace ryu
@@ -190,6 +213,7 @@ struct FmtF64 {
   int16_t olen;     // number of digits
   int8_t kind;      // 0 finite nonzero, 1 zero, 2 infinite, 3 NaN
   int8_t neg;
+  int8_t bad;       // a loop cap of the digit search was reached
   int32_t len;      // bytes of the text
 };
 
@@ -208,6 +232,7 @@ DE_HD FmtF64 fmt_prepare(double x) {
   f.digits = 0;
   f.point = 0;
   f.olen = 0;
+  f.bad = 0;
   f.neg = (int8_t)(bits >> 63);
   if (ex == 0x7ffu && mant != 0) {
     f.kind = 3;
@@ -230,6 +255,16 @@ DE_HD FmtF64 fmt_prepare(double x) {
   f.digits = d.digits;
   f.olen = (int16_t)decimal_length(d.digits);
   f.point = d.exp + f.olen;
+  if (d.bad || f.olen > 17 || f.point < -330 || f.point > 330) {
+    // Never for a valid input (17 significant digits at most, exponents of a
+    // double): flagged, and written as the 3 bytes "NaN" so that every text
+    // keeps within its 24-byte bound.
+    f.bad = 1;
+    f.kind = 3;
+    f.neg = 0;
+    f.len = 3;
+    return f;
+  }
   const int olen = f.olen, point = f.point;
   if (point < -3 || point > 16) {
     const int e = point - 1 < 0 ? 1 - point : point - 1;

@@ -572,8 +572,7 @@ struct Dopri5Lane {
         stden = c == 0 ? v * v : stden + v * v;
       }
       if (stden > 0.0) {
-        double h_lamb = h * sqrt(stdnum / stden);
-        if (h_lamb > 6.1) {
+        if (stiff_exceeds(h, stdnum, stden)) {
           nonstiff = 0;
           stiff += 1;
         }
@@ -974,8 +973,7 @@ struct Dop853Lane {
         stden = c == 0 ? v * v : stden + v * v;
       }
       if (stden > 0.0) {
-        double h_lamb = h * sqrt(stdnum / stden);
-        if (h_lamb > 6.1) {
+        if (stiff_exceeds(h, stdnum, stden)) {
           nonstiff = 0;
           stiff += 1;
         }

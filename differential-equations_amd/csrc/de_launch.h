@@ -167,6 +167,9 @@ constexpr int csv_max_slot_bytes(int dim) { return (dim + 1) * 24 + dim + 1; }
 // (a block of lines that would pass cap is not written).  0, or -1 on a HIP
 // error (hipGetLastError has it).
 int launch_csv(const CsvArgs& a, int64_t* offsets, char* out, int64_t cap, hipStream_t s);
+// Device self-check of the number formatter: the text of x[i] in out[32 i ..],
+// its length in len[i] (negative: a loop cap was reached).
+int launch_fmt_diag(const double* x, char* out, int32_t* len, int64_t n, hipStream_t s);
 
 namespace parity {
 int launch_ode_lane(int family, int method, const OdeArgs& a, int schedule, hipStream_t s);

@@ -90,10 +90,13 @@ DE_HD double uflow(uint64_t sign) { return sign ? -0.0 : 0.0; }
 DE_HD DE_FORCE_INLINE double log_inline(uint64_t ix, double* tail) {
   DE_NO_CONTRACT
   const uint64_t tmp = ix - kOff;
-  const int i = (int)((tmp >> 45) % 128);
-  const int k = (int)((int64_t)tmp >> 52);
-  const uint64_t iz = ix - (tmp & 0xfffULL << 52);
-  const double z = as_f64(iz);
+  // Integer steps on the high word (the low one of tmp & 0xfff << 52 is 0):
+  // i = bits 45..51 of tmp, k = its top 12 bits signed, z = x / 2^k.
+  const uint32_t thi = (uint32_t)(tmp >> 32);
+  const int i = (int)((thi >> 13) & 127u);
+  const int32_t k = (int32_t)thi >> 20;
+  const uint32_t zhi = (uint32_t)(ix >> 32) - (thi & 0xfff00000u);
+  const double z = as_f64(((uint64_t)zhi << 32) | (uint32_t)ix);
   const double kd = (double)k;
   const double invc = kLibmPowLog[3 * i + 0];
   const double logc = kLibmPowLog[3 * i + 1];

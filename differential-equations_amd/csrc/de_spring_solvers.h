@@ -267,8 +267,7 @@ struct VecDopri5 {
           stden = c == 0 ? w2 * w2 : stden + w2 * w2;
         }
         if (stden > 0.0) {
-          double h_lamb = h * sqrt(stdnum / stden);
-          if (h_lamb > 6.1) {
+          if (stiff_exceeds(h, stdnum, stden)) {
             nonstiff = 0;
             stiff += 1;
             if (stiff == 15) {
@@ -646,8 +645,7 @@ struct GroupDopri5 {
         stdnum = gsum(stdnum);
         stden = gsum(stden);
         if (stden > 0.0) {
-          double h_lamb = h * sqrt(stdnum / stden);
-          if (h_lamb > 6.1) {
+          if (stiff_exceeds(h, stdnum, stden)) {
             nonstiff = 0;
             stiff += 1;
             if (stiff == 15) {

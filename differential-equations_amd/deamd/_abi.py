@@ -222,6 +222,7 @@ SIGNATURES = {
     "de_diag_arith": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "de_spring_tuned": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32)]),
+    "de_diag_fmt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "de_diag_noise": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int64]),
 }

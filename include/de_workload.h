@@ -58,6 +58,9 @@ int de_diag_pow(const double* x, const double* y, double* out, int64_t n);
 /* out[3*i + 0..2] = a/b, sqrt(a), fma(a, b, c) as computed on the device. */
 int de_diag_arith(const double* a, const double* b, const double* c, double* out,
                   int64_t n);
+/* Rust's `{:?}` text of x[i] by the device CSV formatter: bytes out[32 i ..],
+ * length len[i] (negative if its digit search reached a loop cap). */
+int de_diag_fmt(const double* x, char* out, int32_t* len, int64_t n);
 /* out[i] = de_noise_normal(seed, path[i], step[i], comp[i]) on the device. */
 int de_diag_noise(uint64_t seed, const uint64_t* path, const uint32_t* step,
                   const uint32_t* comp, double* out, int64_t n);

@@ -2151,6 +2151,16 @@ int de_oracle_validate_step_size(double h0, double h_min, double h_max, double t
   return de::validate_step_size_check(h0, h_min, h_max, t0, tf);
 }
 
+// The kernels' screened stiffness decision (de::stiff_exceeds) against the
+// reference's expression h * sqrt(num / den) > 6.1: out[i] = 1 when they agree.
+void de_oracle_stiff_check(const double* h, const double* num, const double* den,
+                           uint8_t* agree, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    const bool want = h[i] * sqrt(num[i] / den[i]) > 6.1;
+    agree[i] = de::stiff_exceeds(h[i], num[i], den[i]) == want ? 1 : 0;
+  }
+}
+
 double de_oracle_pow(double x, double y) { return de::libm_pow(x, y); }
 
 void de_oracle_pow_many(const double* x, const double* y, double* out, int64_t n) {

@@ -49,6 +49,8 @@ def load():
         lib.de_oracle_constrain_step_size.argtypes = [C.c_double] * 3
         lib.de_oracle_validate_step_size.restype = C.c_int
         lib.de_oracle_validate_step_size.argtypes = [C.c_double] * 5
+        lib.de_oracle_stiff_check.restype = None
+        lib.de_oracle_stiff_check.argtypes = [C.c_void_p] * 4 + [C.c_int64]
         lib.de_oracle_host_pow_many.restype = None
         lib.de_oracle_host_pow_many.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         lib.de_oracle_noise_normal.restype = C.c_double

@@ -181,10 +181,9 @@ def test_noise_normal_matches_oracle(orc):
                 assert lib.de_noise_normal(9, p, s, c) == olib.de_oracle_noise_normal(9, p, s, c)
 
 
-def test_csv_entry_points_validate_then_refuse():
-    # The device CSV writer validates its descriptor (an unknown source is
-    # DE_ERR_INVALID) and is otherwise switched off (DE_ERR_UNSUPPORTED), with
-    # no device work in either case.
+def test_csv_entry_points_validate_before_any_device_work():
+    # The device CSV writer validates its descriptor first: an unknown source
+    # or a spring-chain batch is refused without device work.
     import ctypes as C
     import numpy as np
     import deamd
@@ -196,7 +195,6 @@ def test_csv_entry_points_validate_then_refuse():
     lib = deamd.lib()
     assert lib.de_ode_csv_offsets(C.byref(b), C.byref(r), 7, off.ctypes.data, None) == \
         _abi.DE_ERR_INVALID
+    b.family = _abi.DE_FAMILY_SPRING_CHAIN
     assert lib.de_ode_csv_offsets(C.byref(b), C.byref(r), _abi.DE_CSV_FINAL, off.ctypes.data,
                                   None) == _abi.DE_ERR_UNSUPPORTED
-    assert lib.de_ode_csv_write(C.byref(b), C.byref(r), _abi.DE_CSV_FINAL, off.ctypes.data,
-                                None, None) == _abi.DE_ERR_UNSUPPORTED

@@ -304,3 +304,31 @@ def test_planted_mistakes_are_caught(orc):
                     _pins(orc, group)
     finally:
         orc._lib, orc.LIB_PATH = saved_lib, saved_path
+
+
+def test_screened_stiffness_decision_equals_the_reference_expression(orc):
+    # The kernels decide h sqrt(num / den) > 6.1 by comparing h^2 num with
+    # 6.1^2 den when the two differ by more than 1e-12 (relative): the
+    # decision must equal the rounded expression's everywhere, in particular
+    # within a few ulps of the threshold and at the edges of the screen's range.
+    rng = np.random.default_rng(11)
+    n = 400000
+    den = np.exp(rng.uniform(-300, 300, n))
+    h = np.exp(rng.uniform(-20, 5, n))
+    ratio = (6.1 / h) ** 2 * (1.0 + rng.normal(0, 1e-12, n) * rng.choice([1, 1e3, 1e-3], n))
+    num = ratio * den
+    num[:1000] = np.nextafter(num[:1000], np.inf)
+    edges = np.array([float.fromhex(v) for v in ("0x1p-300", "0x1p300", "0x1.0000000000001p-300",
+                                                 "0x1.fffffffffffffp299")])
+    h[1000:1004], num[1004:1008], den[1008:1012] = edges, edges, edges
+    h[1012:1020] = [-1.0, 0.0, np.inf, np.nan, 1e-310, 1.0, 1.0, 1.0]
+    num[1017:1020] = [0.0, np.inf, np.nan]
+    h, num, den = (np.ascontiguousarray(v, dtype=np.float64) for v in (h, num, den))
+    agree = np.zeros(len(h), dtype=np.uint8)
+    orc.load().de_oracle_stiff_check(h.ctypes.data, num.ctypes.data, den.ctypes.data,
+                                     agree.ctypes.data, len(h))
+    assert agree.all(), np.flatnonzero(agree == 0)[:10]
+    # the sample really straddles the threshold
+    lhs = h * np.sqrt(num / den)
+    assert ((lhs > 6.1) & (lhs < 6.1 * (1 + 1e-11))).sum() > 1000
+    assert ((lhs <= 6.1) & (lhs > 6.1 * (1 - 1e-11))).sum() > 1000
