@@ -331,7 +331,7 @@ int csv_launch_fail(int r) {
 int csv_args(const de_ode_batch* b, const de_ode_result* r, int32_t source, de::CsvArgs* a) {
   if (!b || !r) return fail(DE_ERR_INVALID, "null descriptor");
   if (b->n < 0) return fail(DE_ERR_INVALID, "n < 0");
-  if (source < DE_CSV_FINAL || source > DE_CSV_EVENTS)
+  if (source < DE_CSV_FINAL || source > DE_CSV_POINTS)
     return fail(DE_ERR_INVALID, "unknown csv source %d", source);
   if (b->family < 0 || b->family > DE_FAMILY_ROBERTSON || b->family == DE_FAMILY_SPRING_CHAIN)
     return fail(DE_ERR_UNSUPPORTED, "csv output is built for the fixed-dimension families");
@@ -360,6 +360,32 @@ int csv_args(const de_ode_batch* b, const de_ode_result* r, int32_t source, de::
     a->max_rows = b->n_eval;
     a->dense = r->dense;
     a->n_dense = r->n_dense;
+  } else if (source == DE_CSV_POINTS) {
+    if (any && (!r->points_offset || !r->n_points || !r->points_t || !r->points_y))
+      return fail(DE_ERR_INVALID, "missing points_offset, n_points, points_t or points_y");
+    // The stride of the slot grid: the most points any trajectory keeps.
+    int64_t widest = 0;
+    if (any) {
+      std::vector<int64_t> off((size_t)b->n + 1);
+      if (b->flags & DE_FLAG_HOST_MEMORY) {
+        memcpy(off.data(), r->points_offset, sizeof(int64_t) * off.size());
+      } else {
+        hipError_t e = hipMemcpy(off.data(), r->points_offset, sizeof(int64_t) * off.size(),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "reading points_offset");
+      }
+      for (int64_t i = 0; i < b->n; ++i) {
+        const int64_t w = off[i + 1] - off[i];
+        if (w < 0) return fail(DE_ERR_INVALID, "points_offset decreases at %lld", (long long)i);
+        widest = w > widest ? w : widest;
+      }
+    }
+    if (widest > 0x7ffffffe) return fail(DE_ERR_UNSUPPORTED, "more than 2^31 points");
+    a->max_rows = (int32_t)widest;
+    a->pts_offset = r->points_offset;
+    a->pts_t = r->points_t;
+    a->pts_y = r->points_y;
+    a->n_pts = r->n_points;
   } else {
     if (b->max_events < 0) return fail(DE_ERR_INVALID, "max_events < 0");
     if (any && b->max_events > 0 && (!r->event_t || !r->event_y || !r->n_events))
@@ -385,6 +411,14 @@ void csv_stage(Staging& st, const de_ode_batch* b, de::CsvArgs* a) {
   a->event_t = st.in(a->event_t, (size_t)a->max_rows * n);
   a->event_y = st.in(a->event_y, (size_t)a->max_rows * d * n);
   a->n_events = st.in(a->n_events, n);
+  if (a->source == de::kCsvPoints && n > 0) {
+    const int64_t* off = a->pts_offset;  // host memory here
+    const size_t slots = (size_t)off[n];
+    a->pts_t = st.in(a->pts_t, slots);
+    a->pts_y = st.in(a->pts_y, slots * d);
+    a->n_pts = st.in(a->n_pts, n);
+    a->pts_offset = st.in(off, n + 1);
+  }
 }
 
 }  // namespace

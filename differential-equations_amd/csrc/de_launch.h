@@ -140,7 +140,7 @@ struct SdeArgs {
 
 // Solution::to_csv of every trajectory (kernels_csv.hip): the solve's
 // geometry and result arrays, device pointers.
-enum : int { kCsvFinal = 0, kCsvTeval = 1, kCsvEvents = 2 };
+enum : int { kCsvFinal = 0, kCsvTeval = 1, kCsvEvents = 2, kCsvPoints = 3 };
 constexpr int kCsvMaxDim = 8;
 struct CsvArgs {
   int64_t n;
@@ -159,6 +159,10 @@ struct CsvArgs {
   const double* event_t;
   const double* event_y;
   const int32_t* n_events;
+  const int64_t* pts_offset;  // kCsvPoints
+  const double* pts_t;
+  const double* pts_y;
+  const int64_t* n_pts;
 };
 // Bytes of the longest line: dim + 1 numbers of at most 24 characters,
 // their separators and the newline (the header is shorter).

@@ -62,6 +62,10 @@ DE_DEV void csv_point(const CsvArgs& a, int64_t i, int32_t p, int32_t head, int3
   } else if (a.source == kCsvEvents) {
     *t = a.event_t[(int64_t)p * n + i];
     for (int c = 0; c < dim; ++c) y[c] = a.event_y[((int64_t)p * dim + c) * n + i];
+  } else if (a.source == kCsvPoints) {
+    const int64_t s = a.pts_offset[i] + p;
+    *t = a.pts_t[s];
+    for (int c = 0; c < dim; ++c) y[c] = a.pts_y[s * dim + c];
   } else {
     const int64_t k = p < head ? 0 : (int64_t)run + p - head;
     *t = a.t_eval[k];
@@ -84,6 +88,10 @@ __global__ void csv_points_kernel(CsvArgs a, int64_t i0, int64_t cnt, int32_t* c
     c = 1;
   } else if (a.source == kCsvEvents) {
     c = a.n_events ? a.n_events[i] : 0;
+  } else if (a.source == kCsvPoints) {
+    const int64_t kept = a.pts_offset[i + 1] - a.pts_offset[i];
+    const int64_t pushed = a.n_pts[i];
+    c = (int32_t)(pushed < kept ? pushed : kept);
   } else {
     const double t0 = a.t0 ? a.t0[i] : a.t0s;
     const double tf = a.tf ? a.tf[i] : a.tfs;

@@ -620,18 +620,22 @@ class Solution:
         y = self.event_y.reshape(self.max_events, dim, self.n)[:k, :, i]
         return t.copy(), y.copy()
 
-    _CSV_SOURCES = {"final": DE_CSV_FINAL, "t_eval": DE_CSV_TEVAL, "events": DE_CSV_EVENTS}
+    _CSV_SOURCES = {"final": DE_CSV_FINAL, "t_eval": DE_CSV_TEVAL, "events": DE_CSV_EVENTS,
+                    "points": DE_CSV_POINTS}
 
     def csv(self, source=None):
         """Solution::to_csv of every trajectory (src/solution.rs:190-230),
         formatted on the GPU: (bytes, offsets), trajectory i's document being
-        bytes[offsets[i]:offsets[i + 1]].  `source`: "t_eval" (the t_eval
-        points; the default when the solve had a t_eval output), "final" (the
-        final state; the default otherwise) or "events" (the event points)."""
+        bytes[offsets[i]:offsets[i + 1]].  `source`: "points" (the point output
+        of all_steps(), dense(n) or even(dt): the reference's whole Solution;
+        the default for such a solve), "t_eval" (the t_eval points; the
+        default when the solve had a t_eval output), "final" (the final state;
+        the default otherwise) or "events" (the event points)."""
         if not hasattr(self, "_solved_by"):
             raise RuntimeError("csv() needs a Solution returned by OdeEnsemble.solve()")
         if source is None:
-            source = "t_eval" if self.n_dense is not None else "final"
+            source = ("points" if self.n_points is not None else
+                      "t_eval" if self.n_dense is not None else "final")
         src = self._CSV_SOURCES[source]
         b, keep = self._solved_by
         r = self.struct()
@@ -649,6 +653,43 @@ class Solution:
         without points, where the reference panics after creating it)."""
         data, offsets = self.csv(source)
         _write_file(filename, data[offsets[i]:offsets[i + 1]])
+
+    def to_dataframe(self, i=0):
+        """Solution::to_polars of trajectory i (src/solution.rs:239-280): columns
+        t, y0 .. y<dim-1> over its Solution points (the point output's, else
+        the t_eval points produced, else the final state), as a pandas
+        DataFrame (polars is not available here)."""
+        import pandas as pd
+        dim = len(self.y) // self.n
+        if self.n_points is not None:
+            t, y = self.points(i)
+        elif self.n_dense is not None:
+            # TEvalSolout's points: index 0 when it equals t0, then the run of
+            # points beyond t0 (src/solout/t_eval.rs:100-135), n_dense[i] in all.
+            te = self._t_eval_values()
+            b, _ = self._solved_by
+            t0 = (np.ctypeslib.as_array(C.cast(b.t0, C.POINTER(C.c_double)), (self.n,))[i]
+                  if b.t0 else b.t0_scalar)
+            tf = (np.ctypeslib.as_array(C.cast(b.tf, C.POINTER(C.c_double)), (self.n,))[i]
+                  if b.tf else b.tf_scalar)
+            head = int(len(te) > 0 and te[0] == t0)
+            beyond = (te > t0) if tf > t0 else (te < t0)
+            run = int(np.argmax(beyond)) if beyond.any() else len(te)
+            idx = ([0] if head else []) + list(range(run, len(te)))
+            idx = idx[:int(self.n_dense[i])]
+            rows = self.dense.reshape(-1, dim, self.n)[:, :, i]
+            t, y = te[idx], rows[idx]
+        else:
+            t = self.t[i:i + 1]
+            y = self.y.reshape(dim, self.n)[:, i][None, :]
+        cols = {"t": np.asarray(t, dtype=np.float64)}
+        for c in range(dim):
+            cols["y%d" % c] = np.asarray(y[:, c], dtype=np.float64)
+        return pd.DataFrame(cols)
+
+    def _t_eval_values(self):
+        b, keep = self._solved_by
+        return np.ctypeslib.as_array(C.cast(b.t_eval, C.POINTER(C.c_double)), (b.n_eval,))
 
     def to_csv_all(self, pattern, source=None):
         """One to_csv file per trajectory: pattern.format(i=i)."""

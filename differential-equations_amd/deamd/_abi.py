@@ -70,6 +70,7 @@ DE_MAP_GROUP = 2
 DE_CSV_FINAL = 0
 DE_CSV_TEVAL = 1
 DE_CSV_EVENTS = 2
+DE_CSV_POINTS = 3
 DE_SOLOUT_DEFAULT = 0
 DE_SOLOUT_EVENT = 1
 DE_SOLOUT_CROSSING = 2

@@ -316,6 +316,9 @@ int de_sde_batch_solve(const de_sde_batch* batch, de_sde_result* out, void* stre
 #define DE_CSV_TEVAL 1  /* the t_eval points produced: n_dense[i] of them, with the
                            t_eval values as times (TEvalSolout, src/solout/t_eval.rs) */
 #define DE_CSV_EVENTS 2 /* the event points: n_events[i] of them (event_t / event_y) */
+#define DE_CSV_POINTS 3 /* a point output (DE_POINTS_*): the kept points of trajectory i,
+                           slots points_offset[i] .. + n_points[i] -- the whole Solution of
+                           the reference's default solve with DE_POINTS_STEPS */
 
 /* Document sizes: offsets[0 .. n] (n + 1 int64 values). */
 int de_ode_csv_offsets(const de_ode_batch* batch, const de_ode_result* result, int32_t source,

@@ -1927,6 +1927,9 @@ static void csv_points(const de_ode_batch* b, const de_ode_result* r, int source
   } else if (source == DE_CSV_EVENTS) {
     // the points stored: at most max_events of them
     *count = r->n_events ? std::min<int64_t>(r->n_events[i], b->max_events) : 0;
+  } else if (source == DE_CSV_POINTS) {
+    // the kept points of the point output (Solution::iter, src/solution.rs:214)
+    *count = std::min<int64_t>(r->n_points[i], r->points_offset[i + 1] - r->points_offset[i]);
   } else {
     const double t0 = b->t0 ? b->t0[i] : b->t0_scalar;
     const double tf = b->tf ? b->tf[i] : b->tf_scalar;
@@ -1969,6 +1972,10 @@ static std::string csv_document(const de_ode_batch* b, const de_ode_result* r, i
     } else if (source == DE_CSV_EVENTS) {
       t = r->event_t[k * n + i];
       for (int c = 0; c < dim; ++c) y[c] = r->event_y[(k * dim + c) * n + i];
+    } else if (source == DE_CSV_POINTS) {
+      const int64_t slot = r->points_offset[i] + k;
+      t = r->points_t[slot];
+      for (int c = 0; c < dim; ++c) y[c] = r->points_y[slot * dim + c];
     } else {
       t = b->t_eval[k];
       for (int c = 0; c < dim; ++c) y[c] = r->dense[(k * dim + c) * n + i];
@@ -2115,7 +2122,7 @@ int de_oracle_fmt_debug(double x, char* out) { return orc_fmt_debug(x, out); }
 // documents into out when it is not null.  Returns 0, or -1 for a bad source.
 int de_oracle_ode_csv(const de_ode_batch* b, const de_ode_result* r, int32_t source,
                       int64_t* offsets, char* out) {
-  if (source < DE_CSV_FINAL || source > DE_CSV_EVENTS) return -1;
+  if (source < DE_CSV_FINAL || source > DE_CSV_POINTS) return -1;
   offsets[0] = 0;
   for (int64_t i = 0; i < b->n; ++i) {
     const std::string doc = csv_document(b, r, source, i);

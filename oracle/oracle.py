@@ -132,6 +132,7 @@ def solve_ode(ensemble, pow_mode=POW_RESTATED, threads=1):
         rc = load().de_oracle_ode_solve(C.byref(b), C.byref(res.struct()), pow_mode, threads)
         if rc != 0:
             raise RuntimeError("oracle ode solve failed: %d" % rc)
+        res._solved_by = (b, keep)
         return res
     return ensemble.solve(solver=once)
 

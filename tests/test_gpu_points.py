@@ -89,3 +89,24 @@ def test_point_slots_that_run_out(orc):
         results.append(res)
     assert_points_equal(*results)
     assert np.array_equal(results[0].n_points, full.n_points)
+
+
+def test_points_to_csv_and_dataframe(orc):
+    # Solution::to_csv of the reference's default solve: every step, formatted
+    # on the GPU, byte for byte the oracle's; and to_polars' columns.
+    from deamd import DE_CSV_POINTS
+    from test_gpu_csv import assert_bytes_equal
+    p, y0 = deamd.gen_lorenz(300, seed=96)
+    ens = IVP.ode(Lorenz(*p), 0.0, 1.0, y0).method(METHODS["dopri5"]().max_steps(100000)) \
+        .all_steps()
+    sol = ens.solve()
+    data, offsets = sol.csv()
+    want, woff = orc.ode_csv(ens, sol, DE_CSV_POINTS)
+    assert np.array_equal(offsets, woff)
+    assert_bytes_equal(data, want)
+    t, y = sol.points(7)
+    df = sol.to_dataframe(7)
+    assert list(df.columns) == ["t", "y0", "y1", "y2"]
+    assert np.array_equal(df["t"].to_numpy(), t) and np.array_equal(df[["y0", "y1", "y2"]].to_numpy(), y)
+    lines = data[offsets[7]:offsets[8]].decode().split("\n")
+    assert lines[0] == "t,y0,y1,y2" and len(lines) == len(t) + 2

@@ -10,7 +10,9 @@ power of two and ten, plus the fixed strings Rust prints for named values.
 
 Documents: the header, one line per point, and for t_eval output exactly the
 points TEvalSolout pushes (checked against which dense rows the oracle's own
-solve produced).
+solve produced); for a point output (every step, dense(n), even(dt)) the kept
+points in order.  to_dataframe (Solution::to_polars) holds the same rows as
+the document, parsed back.
 """
 import math
 import re
@@ -18,7 +20,7 @@ import re
 import numpy as np
 import pytest
 
-from deamd import (DE_CSV_EVENTS, DE_CSV_FINAL, DE_CSV_TEVAL, CrossingDirection, ExplicitRungeKutta,
+from deamd import (DE_CSV_EVENTS, DE_CSV_FINAL, DE_CSV_POINTS, DE_CSV_TEVAL, CrossingDirection, ExplicitRungeKutta,
                    Exponential, Harmonic, IVP, Lorenz)
 
 
@@ -139,3 +141,46 @@ def test_event_documents_and_empty_t_eval(orc):
     sol = orc.solve_ode(ens)
     data, offsets = orc.ode_csv(ens, sol, DE_CSV_TEVAL)
     assert sol.n_dense[0] == 0 and offsets.tolist() == [0, 0]
+
+
+@pytest.mark.parametrize("mk", [lambda e: e.all_steps(), lambda e: e.dense(3),
+                                lambda e: e.even(0.1)], ids=["steps", "dense3", "even"])
+def test_point_documents_and_dataframes(orc, mk):
+    p = np.array([[10.0, 9.0], [28.0, 27.0], [8.0 / 3.0, 2.5]])
+    y0 = np.array([[1.0, 0.5], [2.0, 1.0], [3.0, 2.0]])
+    ens = mk(IVP.ode(Lorenz(*p), 0.0, 0.7, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-8).atol(1e-10)))
+    sol = orc.solve_ode(ens)
+    data, offsets = orc.ode_csv(ens, sol, DE_CSV_POINTS)
+    for i in range(2):
+        t, y = sol.points(i)
+        lines = _doc(data, offsets, i).split("\n")
+        assert lines[0] == "t,y0,y1,y2" and lines[-1] == "" and len(lines) == len(t) + 2
+        for ln, tk, yk in zip(lines[1:-1], t, y):
+            assert ln.split(",") == [python_debug(v) for v in [tk, *yk]]
+        df = sol.to_dataframe(i)
+        assert list(df.columns) == ["t", "y0", "y1", "y2"]
+        parsed = np.array([[float(v) for v in ln.split(",")] for ln in lines[1:-1]])
+        assert np.array_equal(df.to_numpy(), parsed)
+
+
+def test_dataframes_of_t_eval_and_final_outputs(orc):
+    n = 3
+    k = np.array([1.0, 2.0, 3.0])
+    y0 = np.vstack([np.ones(n), np.zeros(n)])
+    for pts in ([0.0, 0.25, 0.5, 1.0], [-1.0, 0.0, 0.0, 0.5, 0.9]):
+        ens = IVP.ode(Harmonic(k), 0.0, 1.0, y0).t_eval(pts).method(ExplicitRungeKutta.dopri5())
+        sol = orc.solve_ode(ens)
+        data, offsets = orc.ode_csv(ens, sol, DE_CSV_TEVAL)
+        for i in range(n):
+            lines = _doc(data, offsets, i).split("\n")[1:-1]
+            parsed = np.array([[float(v) for v in ln.split(",")] for ln in lines])
+            df = sol.to_dataframe(i)
+            assert list(df.columns) == ["t", "y0", "y1"] and len(df) == sol.n_dense[i]
+            assert np.array_equal(df.to_numpy(), parsed)
+    ens = IVP.ode(Harmonic(k), 0.0, 1.0, y0).method(ExplicitRungeKutta.dopri5())
+    sol = orc.solve_ode(ens)
+    data, offsets = orc.ode_csv(ens, sol, DE_CSV_FINAL)
+    for i in range(n):
+        line = _doc(data, offsets, i).split("\n")[1]
+        assert np.array_equal(sol.to_dataframe(i).to_numpy()[0], [float(v) for v in line.split(",")])
