@@ -809,31 +809,10 @@ def bench_serve(args, dv):
     import threading
     import numpy as np
     import deamd
-    rng = np.random.default_rng(SEED)
-    big = 65536
-    lp, ly0 = deamd.gen_lorenz(big, seed=SEED)
-    vp, vy0 = deamd.gen_vdp(big, seed=SEED)
-    sp, sy0 = deamd.gen_spring(big, seed=SEED)
-    n_req = int(args.serve_rate * args.serve_seconds)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n_req))
-    reqs = []
-    for k in range(n_req):
-        fam = int(rng.integers(0, 3))
-        n = int(np.exp(rng.uniform(0, np.log(big))))
-        a = int(rng.integers(0, big - n + 1))
-        rtol = float(rng.choice([1e-6, 1e-8]))
-        tf = float(rng.uniform(1.0, 50.0))
-        m = deamd.ExplicitRungeKutta.dopri5().rtol(rtol).atol(rtol * 1e-2).max_steps(1000000)
-        if fam == 0:
-            e = deamd.IVP.ode(deamd.Lorenz(*lp[:, a:a + n]), 0.0, tf, ly0[:, a:a + n]).method(m)
-        elif fam == 1:
-            e = deamd.IVP.ode(deamd.VanDerPol(*vp[:, a:a + n]), 0.0, tf, vy0[:, a:a + n]).method(m)
-        else:
-            sizes = sp.sizes[a:a + n]
-            params = sp.params_packed[sp.param_offset[a]:sp.param_offset[a + n]]
-            y0 = sy0[sp.state_offset[a]:sp.state_offset[a + n]]
-            e = deamd.IVP.ode(deamd.SpringChain(sizes, params), 0.0, tf, y0).method(m)
-        reqs.append((fam, n, e))
+    stream = deamd.poisson_requests(args.serve_rate, args.serve_seconds, seed=SEED)
+    n_req = len(stream)
+    arrivals = [r[0] for r in stream]
+    reqs = [r[1:] for r in stream]
     srv = deamd.Server(128 << 30, graph=args.serve_graph)
     tickets = [None] * n_req
     t_start = time.perf_counter()

@@ -861,6 +861,40 @@ def gen_gbm(n, mu=0.05, sigma=0.2, s0=100.0):
     return LinearDiagonal(1, p), y0
 
 
+def poisson_requests(rate, seconds, seed=0x5EED2026, big=65536):
+    """A seeded Poisson request stream: (arrival time s, family, size,
+    OdeEnsemble) per request, arrivals at `rate` per second over `seconds`.
+    Each request: one family (0 Lorenz, 1 Van der Pol mu ~ LogUniform[0.1, 10],
+    2 spring chains N in [4, 256]), ensemble size log-uniform in [1, big], a
+    slice of one pool of `big` generated systems, rtol in {1e-6, 1e-8} with
+    atol = rtol / 100, t_final ~ U[1, 50], DOPRI5."""
+    rng = np.random.default_rng(seed)
+    lp, ly0 = gen_lorenz(big, seed=seed)
+    vp, vy0 = gen_vdp(big, seed=seed)
+    sp, sy0 = gen_spring(big, seed=seed)
+    n_req = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    out = []
+    for k in range(n_req):
+        fam = int(rng.integers(0, 3))
+        n = int(np.exp(rng.uniform(0, np.log(big))))
+        a = int(rng.integers(0, big - n + 1))
+        rtol = float(rng.choice([1e-6, 1e-8]))
+        tf = float(rng.uniform(1.0, 50.0))
+        m = ExplicitRungeKutta.dopri5().rtol(rtol).atol(rtol * 1e-2).max_steps(1000000)
+        if fam == 0:
+            e = IVP.ode(Lorenz(*lp[:, a:a + n]), 0.0, tf, ly0[:, a:a + n]).method(m)
+        elif fam == 1:
+            e = IVP.ode(VanDerPol(*vp[:, a:a + n]), 0.0, tf, vy0[:, a:a + n]).method(m)
+        else:
+            sizes = sp.sizes[a:a + n]
+            params = sp.params_packed[sp.param_offset[a]:sp.param_offset[a + n]]
+            y0 = sy0[sp.state_offset[a]:sp.state_offset[a + n]]
+            e = IVP.ode(SpringChain(sizes, params), 0.0, tf, y0).method(m)
+        out.append((float(arrivals[k]), fam, n, e))
+    return out
+
+
 # ----------------------------------------------------------------- serving --
 DE_SERVER_GRAPH = 1
 

@@ -209,3 +209,45 @@ def test_range_table_restarts_when_pool_drains(monkeypatch, graph):
     assert resets >= 40 // 8
     for g, w in zip(got, wants):
         assert_same(g, w)
+
+
+def test_poisson_stream_results_equal_direct_solves():
+    # The serving workload's request stream (families, sizes up to 65 536,
+    # rtol, t_final as the serving benchmark draws them) at 2 000 requests/s,
+    # submitted in real time over a 40 ms window: every Lorenz and Van der Pol
+    # request returns its direct solve's bits; spring-chain requests (their
+    # slot pools group chains by size class, so the reductions' order can differ
+    # from a direct solve's) agree to 1e-10 of the largest state, with equal
+    # step counts on at least 99.9% of the chains.
+    import threading
+    import time
+    stream = deamd.poisson_requests(2000.0, 0.04, seed=2026)
+    assert len(stream) == 80 and {f for _, f, _, _ in stream} == {0, 1, 2}
+    tickets = [None] * len(stream)
+    with Server(32 * GIB) as srv:
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k, (at, _, _, e) in enumerate(stream):
+                delay = t0 + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[k] = srv.submit(e)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        got = [srv.poll(t) for t in tickets]
+    same_counts = total = 0
+    for (_, fam, n, e), g in zip(stream, got):
+        assert (g.status == 0).all()
+        want = e.solve()
+        if fam < 2:
+            assert_same(g, want)
+        else:
+            err = np.abs(g.y - want.y).max() / np.abs(want.y).max()
+            assert err <= 1e-10
+            same_counts += int(((g.accepted == want.accepted) &
+                                (g.rejected == want.rejected)).sum())
+            total += n
+    assert total > 0 and same_counts >= 0.999 * total
