@@ -716,6 +716,16 @@ def bench_spring(args, dv):
 
 
 # ------------------------------------------------------------------ sde ---
+# FP64 operations (an fma counts 2) of one OU-8 Euler-Maruyama path-step, read
+# off the source: the step y + h f(y) + sigma dW and the drift theta (mu - y) at
+# the new state, 6 per component; and one standard normal of the noise, half
+# of a Box-Muller pair (two uniforms 3 each, log 27, -2 log and sqrt 2, angle
+# reduction 5, sin/cos polynomials 34, the two products 2: 76) plus its
+# scaling by sqrt(h).
+SDE_STEP_FLOPS = 48
+SDE_NORMAL_FLOPS = 39
+
+
 def bench_sde(args, dv):
     """4 194 304 paths each of OU-8 and GBM, Euler-Maruyama and Milstein, dt 1e-3,
     1000 steps, Philox noise; every 100th state recorded."""
@@ -784,6 +794,7 @@ def bench_sde(args, dv):
                                      "all_complete": bool((o["st"] == 0).all().item())}
     ms = 1000.0 * wall / args.steps
     ou_em = per["ou8_em"]
+    ou_tf = ou_em["steps_paths_per_s"] * (SDE_STEP_FLOPS + 8 * SDE_NORMAL_FLOPS) / 1e12
     return {
         "metric": "SDE path-steps*paths/s FP64 (Euler-Maruyama, Milstein), 1 GPU",
         "value": total_steps / (ms / 1000.0), "unit": "steps*paths/s", "n_gpus": 1,
@@ -792,11 +803,19 @@ def bench_sde(args, dv):
         "data": "synthetic (Philox4x32-10 noise keyed by seed, path, step, component)",
         "config": {"workload": "sde_ou8_gbm_4M", "n_paths": n, "dt": 1e-3, "steps": 1000,
                    "record_every": stride, "per_run": per},
-        "roofline": {"bound": "hbm", "achieved": ou_em["path_write_GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ou_em["path_write_GBps"] / HBM_PEAK_GBS,
+        "roofline": {"bound": "fp64_vector", "achieved": ou_tf, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": ou_tf / FP64_PEAK_TFLOPS,
                      "traffic": load_traffic("sde_ou8_em_4M"),
-                     "note": "OU-8 Euler-Maruyama path writes; the kernel is bound by noise "
-                             "generation (Philox rounds and the Box-Muller log/sincos)"},
+                     "flop_model": "OU-8 Euler-Maruyama, per path-step: %d for the step (drift, "
+                                   "increment, diffusion term) + 8 x %d for the normals of the "
+                                   "noise (Box-Muller: uniforms, log, sqrt, sin/cos polynomials, "
+                                   "scaling by sqrt(h)); fma = 2; the Philox rounds' integer "
+                                   "products and xors are not counted" % (SDE_STEP_FLOPS,
+                                                                           SDE_NORMAL_FLOPS),
+                     "update_only_frac": ou_em["steps_paths_per_s"] * SDE_STEP_FLOPS / 1e12
+                     / FP64_PEAK_TFLOPS,
+                     "path_write_GBps": ou_em["path_write_GBps"],
+                     "path_write_frac": ou_em["path_write_GBps"] / HBM_PEAK_GBS},
     }
 
 

@@ -26,11 +26,12 @@ struct SdeLane {
   double t, h, tf, dir;
   int64_t steps, ev;
   uint64_t path;
+  double sqrt_h;  // sqrt(h), formed where h is set (the start and the clipped last step)
 
-  DE_DEV void noise_fill(const SdeArgs& a, double hh, double* dw) const {
+  DE_DEV void noise_fill(const SdeArgs& a, double* dw) const {
     uint32_t call = (uint32_t)(steps - 1);
     if (a.noise == kNoisePhilox) {
-      double sq = sqrt(hh);
+      const double sq = sqrt_h;
 #pragma unroll
       for (int c = 0; c < D; c += 2) {
         u32x4 ctr = {{(uint32_t)(c >> 1), call, (uint32_t)path, (uint32_t)(path >> 32)}};
@@ -76,6 +77,7 @@ struct SdeLane {
       return;
     }
     h = h0;
+    sqrt_h = sqrt(h);
     sys.drift(t, y, dydt);
     ev = SCHEME == kSchemeMilstein ? 1 : 2;  // EM init also evaluates the diffusion
     for (;;) {
@@ -83,6 +85,7 @@ struct SdeLane {
         double hn = tf - t;
         if (fabs(hn) < kTenEps) break;
         h = hn;
+        sqrt_h = sqrt(h);
       }
       if (steps >= a.max_steps) {
         finish(a, i, acc, kMaxSteps);
@@ -95,13 +98,12 @@ struct SdeLane {
         double inc[D];
         for (int c = 0; c < D; ++c) inc[c] = b0h * dydt[c];
         sys.diffusion(t, y, b);
-        noise_fill(a, h, dw);
+        noise_fill(a, dw);
         for (int c = 0; c < D; ++c) ynext[c] = (y[c] + inc[c]) + b[c] * dw[c];
         ev += 2;
       } else {
         sys.diffusion(t, y, b);
-        noise_fill(a, h, dw);
-        double sqrt_h = sqrt(h);
+        noise_fill(a, dw);
         double yaux[D], baux[D];
         for (int c = 0; c < D; ++c) yaux[c] = y[c] + sqrt_h * b[c];
         sys.diffusion(t, yaux, baux);

@@ -408,6 +408,27 @@ def test_noise_normals_are_standard_normal(orc):
     assert lib.de_oracle_noise_normal(1, 2, 3, 4) != lib.de_oracle_noise_normal(1, 2, 3, 5)
 
 
+# Normals of the noise stream at fixed keys (seed, path, step, component), as
+# the stream has been defined since the SDE kernels' first bit-exact runs: a
+# change to Philox, the uniform conversion or the Box-Muller arithmetic that
+# alters any bit fails here, on the CPU (the device draws the same normals).
+NOISE_GOLDEN = [
+    ((0x5EED2026, 0, 0, 0), "-0x1.6bde92f837ce2p-2"),
+    ((0x5EED2026, 0, 0, 1), "0x1.0b9d12597893fp-2"),
+    ((0x5EED2026, 1, 0, 0), "-0x1.030d167799bb5p+0"),
+    ((0x5EED2026, 0, 999, 7), "0x1.3e7c07e4aa601p-5"),
+    ((0x5EED2026, 4194303, 999, 6), "0x1.fa4cc38dad201p-1"),
+    ((1, 2, 3, 4), "-0x1.b2e24a544ce7ap+0"),
+    ((0xFFFFFFFFFFFFFFFF, 2 ** 40 + 5, 123456, 3), "-0x1.e38474b2845f5p-3"),
+    ((0, 0, 0, 0), "-0x1.f1bcd549837c0p-4"),
+]
+
+
+@pytest.mark.parametrize("key,want", NOISE_GOLDEN)
+def test_noise_normals_golden(orc, key, want):
+    assert orc.load().de_oracle_noise_normal(*key) == float.fromhex(want)
+
+
 # ------------------------------------------------------------------ DOP853 --
 def _scipy_dop853():
     """SciPy's independent copy of the published DOP853 coefficients, as the
