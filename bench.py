@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window")
     ap.add_argument("--serve-graph", action="store_true", help="replay chunks from a hipGraph")
+    ap.add_argument("--serve-offline", action="store_true",
+                    help="also solve the stream's work as one batch per family")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC traffic calibration)")
     return ap.parse_args()
@@ -858,7 +860,10 @@ def bench_serve(args, dv):
         fam_lat[reqs[k][0]].append(l)
     t_end = time.perf_counter()
     th.join()
+    iterations, _ = srv.stats()
+    t_admit, t_iter, t_retire = srv.timing()
     srv.close()
+    offline = serve_offline(stream) if args.serve_offline else None
     elapsed = t_end - t_start
     ms = lambda v: 1000.0 * v if v is not None else None
     traj = sum(r[1] for r in reqs)
@@ -871,13 +876,53 @@ def bench_serve(args, dv):
         "config": {"workload": "serve_poisson", "rate_req_per_s": args.serve_rate,
                    "arrival_window_s": args.serve_seconds, "requests": n_req,
                    "trajectories": traj, "graph": bool(args.serve_graph),
-                   "wall_s": elapsed, "accepted_steps": steps,
+                   "wall_s": elapsed, "accepted_steps": steps, "iterations": iterations,
+                   "offline": offline,
+                   "scheduler_s": {"admit": t_admit, "iterate": t_iter, "retire": t_retire},
                    "latency_ms_by_family": {
                        name: {"p50": ms(percentile(fam_lat[i], 50)),
                               "p99": ms(percentile(fam_lat[i], 99)), "n": len(fam_lat[i])}
                        for i, name in ((0, "lorenz"), (1, "vdp"), (2, "spring"))}},
         "latency_ms": {"p50": ms(percentile(lat, 50)), "p99": ms(percentile(lat, 99))},
     }
+
+
+def serve_offline(stream):
+    """The stream's work as three batch solves (one per family, every
+    trajectory with its request's t_final and tolerances), host memory in and
+    out: the throughput the same work gets without arrivals."""
+    import numpy as np
+    import deamd
+    by = {0: [], 1: [], 2: []}
+    for (_, fam, _, e) in stream:
+        by[fam].append(e)
+    total_s, steps, per = 0.0, 0, {}
+    for fam, es in by.items():
+        if not es:
+            continue
+        tf = np.concatenate([np.full(e.n, float(e.tf)) for e in es])
+        rt = np.concatenate([np.full(e.n, float(e._method.settings["rtol"])) for e in es])
+        at = np.concatenate([np.full(e.n, float(e._method.settings["atol"])) for e in es])
+        if fam < 2:
+            dim = es[0].system.dim
+            params = [np.concatenate([np.broadcast_to(e.system.params[j], (e.n,)) for e in es])
+                      for j in range(len(es[0].system.params))]
+            y0 = np.concatenate([e.y0.reshape(dim, e.n) for e in es], axis=1)
+            sysm = type(es[0].system)(*params)
+        else:
+            sysm = deamd.SpringChain(np.concatenate([e.system.sizes for e in es]),
+                                     np.concatenate([e.system.params_packed for e in es]))
+            y0 = np.concatenate([e.y0 for e in es])
+        big = deamd.IVP.ode(sysm, 0.0, tf, y0).method(
+            deamd.ExplicitRungeKutta.dopri5().max_steps(1000000))
+        big.rtol_traj, big.atol_traj = rt, at
+        t0 = time.perf_counter()
+        sol = big.solve()
+        dt = time.perf_counter() - t0
+        total_s += dt
+        steps += int(sol.accepted.sum())
+        per[("lorenz", "vdp", "spring")[fam]] = {"s": dt, "trajectories": big.n}
+    return {"s": total_s, "accepted_steps": steps, "value": steps / total_s, "per_family": per}
 
 
 def other_configs(args, dv):

@@ -20,8 +20,13 @@
 // publishes them, launches one chunk per family with work (concurrent streams,
 // optionally replayed from a captured hipGraph), reads back the per-request
 // remaining counts, and retires finished requests by copying their results to
-// host memory.  Every trajectory's arithmetic depends only on its own inputs,
-// so a request's results do not depend on what else is being served.
+// host memory.  The loop is pipelined: iteration k+1 is queued on the device
+// before the host reads iteration k's counts, and admissions (uploads on their
+// own stream) and retirements (copies on theirs) run while the device
+// iterates.  A request's work becomes visible to the chunk kernels only
+// through a commit kernel ordered between two iterations.  Every trajectory's
+// arithmetic depends only on its own inputs, so a request's results do not
+// depend on what else is being served.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
@@ -191,6 +196,15 @@ chunk_group(Queue* q, const Range* ranges,
   if (valid) slots[slot] = S;
 }
 
+// Publishes a pool's new work between two iterations: the chunk kernels read
+// the counters once at their start.
+__global__ void commit_pool(Queue* q, int64_t published, int64_t nranges) {
+  if (threadIdx.x == 0) {
+    q->published = published;
+    q->nranges = nranges;
+  }
+}
+
 __global__ void gather_remaining(const ReqDev* reqs, int32_t* out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = reqs[i].remaining;
@@ -241,6 +255,7 @@ struct Family {
   int64_t active_tickets = 0;  // published but possibly unfinished
   int active_reqs = 0;         // active requests with work in this pool
   int64_t max_ranges = 0;      // capacity of the range table
+  bool dirty = false;          // published work not yet committed to the device
   hipStream_t stream;
   void (*launch)(Family&, ReqDev*, int, hipStream_t);
   void (*reset)(Family&, hipStream_t);
@@ -266,15 +281,33 @@ void launch_group(Family& f, ReqDev* reqs, int k, hipStream_t s) {
                      (LaneSlot<GroupDopri5<G, C>>*)f.slots, f.nslots, k);
 }
 
+// First-fit arena over one allocation: device memory (inputs) or pinned,
+// device-mapped host memory that the kernels write results into.
 struct Arena {
   char* base = nullptr;
   size_t size = 0;
+  bool host = false;
   std::map<size_t, size_t> free_;  // offset -> length
-  bool init(size_t bytes) {
-    if (hipMalloc(&base, bytes) != hipSuccess) return false;
+  bool init(size_t bytes, bool in_host) {
+    host = in_host;
+    const hipError_t e = in_host ? hipHostMalloc(&base, bytes, hipHostMallocCoherent)
+                                 : hipMalloc(&base, bytes);
+    if (e != hipSuccess) {
+      base = nullptr;
+      return false;
+    }
     size = bytes;
     free_[0] = bytes;
     return true;
+  }
+  void destroy() {
+    if (base) (void)(host ? hipHostFree(base) : hipFree(base));
+    base = nullptr;
+  }
+  size_t largest_free() const {
+    size_t m = 0;
+    for (const auto& kv : free_) m = std::max(m, kv.second);
+    return m;
   }
   void* alloc(size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
@@ -315,18 +348,28 @@ struct Request {
   std::vector<int32_t> sizes;
   std::vector<int64_t> so, po;
   int64_t state_len = 0;
-  // results (host)
-  std::vector<double> y, t, dense, event_t, event_y;
-  std::vector<int64_t> acc, rej, ev;
-  std::vector<int32_t> status, n_dense, n_events;
-  // device
+  // results: written by the kernels into the host arena at these pointers
+  de::OdeArgs out;
+  // device inputs and host-arena results
   void* dmem = nullptr;
   size_t dbytes = 0;
+  void* hmem = nullptr;
+  size_t hbytes = 0;
   uint32_t pools = 0;  // bit k: work published to pool k
   int slot = -1;
+  int64_t first_iter = 0;  // first iteration whose gathered counts include this request
   bool done = false;
   int error = 0;
   double t_submit = 0, t_done = 0;
+
+  // The host copies of the inputs, once they are on the device.
+  void drop_inputs() {
+    for (auto* v : {&params, &y0, &t0, &tf, &rtol, &atol, &t_eval, &ev_value, &ev_point, &ev_normal})
+      std::vector<double>().swap(*v);
+    std::vector<int32_t>().swap(sizes);
+    std::vector<int64_t>().swap(so);
+    std::vector<int64_t>().swap(po);
+  }
 };
 
 static double now_s() {
@@ -341,27 +384,39 @@ using namespace de::serve;
 struct de_server {
   size_t budget = 0;
   int32_t flags = 0;
-  Arena arena;
+  Arena arena;   // device: request inputs
+  Arena harena;  // pinned host: request results
   std::vector<Family> fams;
   de::serve::ReqDev* d_reqs = nullptr;
   std::vector<de::serve::ReqDev> h_reqs;
-  int32_t* h_remaining = nullptr;  // pinned mirror
+  int32_t* h_counts[2] = {};       // pinned: gathered counts of iterations k (k & 1)
   int32_t* d_remaining = nullptr;  // gathered from d_reqs for one copy per iteration
+  hipStream_t upl = nullptr;  // admission uploads
+  hipEvent_t ev_counts[2] = {}, ev_upl = nullptr;
+  // Pinned staging of an admission phase's uploads (reused once the upload
+  // stream is synchronised).
+  char* stage = nullptr;
+  size_t stage_size = 0, stage_used = 0;
+  int64_t launched = 0;            // iterations launched
   std::vector<int> free_slots;
   std::map<int, Request*> active;  // request slot -> request
   std::deque<Request*> pending;
-  std::map<uint64_t, Request*> all;
+  std::map<uint64_t, Request*> all;      // submitted, not yet delivered
+  std::map<uint64_t, double> latency;    // delivered requests' latencies
   uint64_t next_ticket = 1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   bool stop = false;
   std::thread thread;
   hipStream_t ctl;
-  hipEvent_t fork, join_ev;
+  hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> fam_done;
   hipGraphExec_t graph = nullptr;
   int iterations = 0;
   int64_t range_resets = 0;
+  // Scheduler time split (seconds): admitting (uploads, publishing), the
+  // iterations from launch to their counts on the host, and retiring.
+  double t_admit = 0, t_iter = 0, t_retire = 0;
   int error = 0;
 };
 
@@ -394,40 +449,71 @@ int fam_index_for(de_server* s, int family, int nm, int method, int solout) {
   return -1;
 }
 
-size_t footprint(const Request* r) {
+// Device bytes of a request's inputs (and spring class lists) and host bytes
+// of its results.
+void footprint(const Request* r, size_t* dev, size_t* host) {
   const de_ode_batch& b = r->b;
   size_t n = (size_t)b.n;
-  size_t bytes = 0;
-  bytes += r->params.size() * 8 + r->y0.size() * 8 + r->t0.size() * 8 + r->tf.size() * 8;
-  bytes += r->rtol.size() * 8 + r->atol.size() * 8 + r->t_eval.size() * 8;
-  bytes += r->sizes.size() * 4 + (r->so.size() + r->po.size()) * 8 + n * 4;  // + class lists
-  bytes += (size_t)r->state_len * 8 + n * (8 * 4 + 4 + 4);
-  bytes += (size_t)b.n_eval * r->state_len * 8;
-  bytes += (r->ev_value.size() + r->ev_point.size() + r->ev_normal.size()) * 8;
+  size_t d = 0, h = 0;
+  d += r->params.size() * 8 + r->y0.size() * 8 + r->t0.size() * 8 + r->tf.size() * 8;
+  d += r->rtol.size() * 8 + r->atol.size() * 8 + r->t_eval.size() * 8;
+  d += r->sizes.size() * 4 + (r->so.size() + r->po.size()) * 8 + n * 4;  // + class lists
+  d += (r->ev_value.size() + r->ev_point.size() + r->ev_normal.size()) * 8;
+  h += (size_t)r->state_len * 8 + n * (8 * 4 + 4 + 4);
+  h += (size_t)b.n_eval * r->state_len * 8;
   if (b.event_solout != DE_SOLOUT_DEFAULT)
-    bytes += (size_t)b.max_events * n * 8 + (size_t)b.max_events * r->state_len * 8 + n * 4;
-  return bytes + 64 * 256;
+    h += (size_t)b.max_events * n * 8 + (size_t)b.max_events * r->state_len * 8 + n * 4;
+  *dev = d + 64 * 256;
+  *host = h + 16 * 256;
+}
+
+// Host-to-device copy on the upload stream through the pinned stage (one
+// asynchronous DMA, no staging inside the runtime); a full stage first waits
+// for the uploads in flight.
+void stage_copy(de_server* s, void* dst, const void* src, size_t nb) {
+  const size_t need = (nb + 255) & ~size_t(255);
+  if (need > s->stage_size) {  // larger than the stage: the runtime stages it
+    (void)hipMemcpyAsync(dst, src, nb, hipMemcpyHostToDevice, s->upl);
+    (void)hipStreamSynchronize(s->upl);
+    return;
+  }
+  if (s->stage_used + need > s->stage_size) {
+    (void)hipStreamSynchronize(s->upl);
+    s->stage_used = 0;
+  }
+  char* h = s->stage + s->stage_used;
+  memcpy(h, src, nb);
+  s->stage_used += need;
+  (void)hipMemcpyAsync(dst, h, nb, hipMemcpyHostToDevice, s->upl);
 }
 
 // Upload a request into the arena and build its device descriptor.
 bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_lists,
             int32_t** d_lists) {
-  size_t bytes = footprint(r);
+  size_t bytes = 0, hbytes = 0;
+  footprint(r, &bytes, &hbytes);
   char* p = (char*)s->arena.alloc(bytes);
   if (!p) return false;
+  char* hp = (char*)s->harena.alloc(hbytes);
+  if (!hp) {
+    s->arena.release(p, bytes);
+    return false;
+  }
   r->dmem = p;
   r->dbytes = bytes;
+  r->hmem = hp;
+  r->hbytes = hbytes;
   auto put = [&](const void* src, size_t nb) -> void* {
     if (!nb) return nullptr;
     void* d = p;
-    (void)hipMemcpyAsync(d, src, nb, hipMemcpyHostToDevice, s->ctl);
+    stage_copy(s, d, src, nb);
     p += (nb + 255) & ~size_t(255);
     return d;
   };
-  auto take = [&](size_t nb) -> void* {
+  auto take = [&](size_t nb) -> void* {  // results: the host arena
     if (!nb) return nullptr;
-    void* d = p;
-    p += (nb + 255) & ~size_t(255);
+    void* d = hp;
+    hp += (nb + 255) & ~size_t(255);
     return d;
   };
   const de_ode_batch& b = r->b;
@@ -491,9 +577,8 @@ bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_l
   }
   rd.remaining = (int32_t)n;
   s->h_reqs[r->slot] = rd;
-  (void)hipMemcpyAsync(s->d_reqs + r->slot, &s->h_reqs[r->slot], sizeof rd, hipMemcpyHostToDevice,
-                       s->ctl);
-  s->h_remaining[r->slot] = (int32_t)n;
+  r->out = rd.a.o;
+  stage_copy(s, s->d_reqs + r->slot, &s->h_reqs[r->slot], sizeof rd);
   return true;
 }
 
@@ -508,55 +593,9 @@ void publish(de_server* s, Family& f, int req_slot, int32_t first, int32_t count
   rg.list = list;
   int64_t idx = (int64_t)f.host_ranges.size();
   f.host_ranges.push_back(rg);
-  (void)hipMemcpyAsync(f.ranges + idx, &f.host_ranges[idx], sizeof rg, hipMemcpyHostToDevice,
-                       s->ctl);
+  stage_copy(s, f.ranges + idx, &rg, sizeof rg);
   f.published += count;
-  de::serve::Queue qh;
-  qh.published = f.published;
-  qh.nranges = idx + 1;
-  (void)hipMemcpyAsync(&f.q->published, &qh.published, 16, hipMemcpyHostToDevice, s->ctl);
-  (void)hipStreamSynchronize(s->ctl);
-}
-
-void retire(de_server* s, Request* r) {
-  const ReqDev& rd = s->h_reqs[r->slot];
-  const de::OdeArgs& a = rd.a.o;
-  const int64_t n = r->b.n;
-  r->y.resize(r->state_len);
-  r->t.resize(n);
-  r->acc.resize(n);
-  r->rej.resize(n);
-  r->ev.resize(n);
-  r->status.resize(n);
-  (void)hipMemcpyAsync(r->y.data(), a.y_final, r->state_len * 8, hipMemcpyDeviceToHost, s->ctl);
-  (void)hipMemcpyAsync(r->t.data(), a.t_final, n * 8, hipMemcpyDeviceToHost, s->ctl);
-  (void)hipMemcpyAsync(r->acc.data(), a.accepted, n * 8, hipMemcpyDeviceToHost, s->ctl);
-  (void)hipMemcpyAsync(r->rej.data(), a.rejected, n * 8, hipMemcpyDeviceToHost, s->ctl);
-  (void)hipMemcpyAsync(r->ev.data(), a.evals, n * 8, hipMemcpyDeviceToHost, s->ctl);
-  (void)hipMemcpyAsync(r->status.data(), a.status, n * 4, hipMemcpyDeviceToHost, s->ctl);
-  if (a.dense) {
-    r->dense.resize((size_t)r->b.n_eval * r->state_len);
-    r->n_dense.resize(n);
-    (void)hipMemcpyAsync(r->dense.data(), a.dense, r->dense.size() * 8, hipMemcpyDeviceToHost,
-                         s->ctl);
-    (void)hipMemcpyAsync(r->n_dense.data(), a.n_dense, n * 4, hipMemcpyDeviceToHost, s->ctl);
-  }
-  if (a.ev.n_out) {
-    const size_t m = (size_t)a.ev.max_events;
-    r->event_t.resize(m * n);
-    r->event_y.resize(m * r->state_len);
-    r->n_events.resize(n);
-    if (m) {
-      (void)hipMemcpyAsync(r->event_t.data(), a.ev.t_out, m * n * 8, hipMemcpyDeviceToHost,
-                           s->ctl);
-      (void)hipMemcpyAsync(r->event_y.data(), a.ev.y_out, m * r->state_len * 8,
-                           hipMemcpyDeviceToHost, s->ctl);
-    }
-    (void)hipMemcpyAsync(r->n_events.data(), a.ev.n_out, n * 4, hipMemcpyDeviceToHost, s->ctl);
-  }
-  if (hipStreamSynchronize(s->ctl) != hipSuccess) r->error = DE_ERR_HIP;
-  s->arena.release(r->dmem, r->dbytes);
-  r->dmem = nullptr;
+  f.dirty = true;
 }
 
 // One chunk of every pool (graph capture), or of the pools that hold work of
@@ -587,8 +626,11 @@ bool capture(de_server* s) {
 
 void scheduler(de_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
+  int64_t inflight = -1;  // last queued iteration whose counts are not read yet
+  std::vector<Request*> admitted;  // inputs in flight on the upload stream
   while (true) {
     // Admission: first come, first served, while the footprint fits.
+    const double t_a0 = now_s();
     while (!s->pending.empty() && !s->free_slots.empty()) {
       Request* r = s->pending.front();
       std::vector<std::vector<int32_t>> lists(s->fams.size());
@@ -623,7 +665,10 @@ void scheduler(de_server* s) {
       std::vector<int32_t*> d_lists(s->fams.size(), nullptr);
       if (!upload(s, r, lists, d_lists.data())) {
         r->slot = -1;
-        if (s->active.empty()) {  // can never fit
+        size_t dev = 0, host = 0;
+        footprint(r, &dev, &host);
+        const bool never = dev + 255 > s->arena.size || host + 255 > s->harena.size;
+        if (never || (s->active.empty() && dev + 255 > s->arena.largest_free())) {
           s->pending.pop_front();
           r->error = DE_ERR_NOMEM;
           r->done = true;
@@ -636,6 +681,8 @@ void scheduler(de_server* s) {
       s->free_slots.pop_back();
       s->pending.pop_front();
       s->active[r->slot] = r;
+      r->first_iter = s->launched;
+      admitted.push_back(r);
       if (b.family == DE_FAMILY_SPRING_CHAIN) {
         for (int k = 2; k < kSpringEnd; ++k)
           if (!lists[k].empty()) {
@@ -650,12 +697,41 @@ void scheduler(de_server* s) {
       for (size_t k = 0; k < s->fams.size(); ++k)
         if ((r->pools >> k) & 1u) s->fams[k].active_reqs += 1;
     }
+    // The uploads and range entries are on the device before the commit
+    // kernels, which run after the iteration in flight (stream order on ctl).
+    bool dirty = false;
+    for (auto& f : s->fams) dirty = dirty || f.dirty;
+    if (dirty) {
+      (void)hipEventRecord(s->ev_upl, s->upl);
+      (void)hipStreamSynchronize(s->upl);
+      s->stage_used = 0;
+      for (Request* r : admitted) r->drop_inputs();
+      admitted.clear();
+      (void)hipStreamWaitEvent(s->ctl, s->ev_upl, 0);
+      for (auto& f : s->fams) {
+        if (!f.dirty) continue;
+        hipLaunchKernelGGL(commit_pool, dim3(1), dim3(64), 0, s->ctl, f.q, (int64_t)f.published,
+                           (int64_t)f.host_ranges.size());
+        f.dirty = false;
+      }
+    }
+    const double t_a1 = now_s();
+    s->t_admit += t_a1 - t_a0;
     if (s->active.empty()) {
+      // Nothing left to run: let the iteration in flight (if any) finish.
+      if (inflight >= 0) {
+        lk.unlock();
+        (void)hipEventSynchronize(s->ev_counts[inflight & 1]);
+        lk.lock();
+        inflight = -1;
+        continue;
+      }
       if (s->stop) break;
       s->cv_work.wait(lk);
       continue;
     }
-    lk.unlock();
+    // Queue iteration k; its counts land in h_counts[k & 1].
+    const int64_t k = s->launched++;
     if (s->graph) {
       (void)hipGraphLaunch(s->graph, s->ctl);
     } else {
@@ -666,10 +742,17 @@ void scheduler(de_server* s) {
     const int span = s->active.rbegin()->first + 1;
     hipLaunchKernelGGL(gather_remaining, dim3((span + 255) / 256), dim3(256), 0, s->ctl,
                        (const de::serve::ReqDev*)s->d_reqs, s->d_remaining, span);
-    (void)hipMemcpyAsync(s->h_remaining, s->d_remaining, sizeof(int32_t) * span,
+    (void)hipMemcpyAsync(s->h_counts[k & 1], s->d_remaining, sizeof(int32_t) * span,
                          hipMemcpyDeviceToHost, s->ctl);
-    hipError_t e = hipStreamSynchronize(s->ctl);
+    (void)hipEventRecord(s->ev_counts[k & 1], s->ctl);
+    const int64_t prev = inflight;
+    inflight = k;
+    lk.unlock();
+    // The previous iteration's counts (the device has iteration k queued).
+    hipError_t e = prev >= 0 ? hipEventSynchronize(s->ev_counts[prev & 1]) : hipSuccess;
     lk.lock();
+    const double t_i1 = now_s();
+    s->t_iter += t_i1 - t_a1;
     s->iterations += 1;
     if (e != hipSuccess) {
       s->error = DE_ERR_HIP;
@@ -681,21 +764,31 @@ void scheduler(de_server* s) {
       s->cv_done.notify_all();
       break;
     }
+    if (prev < 0) continue;
+    // Requests whose count reached zero by iteration prev (counts of requests
+    // admitted later are not theirs yet).
+    const int32_t* counts = s->h_counts[prev & 1];
     std::vector<int> finished;
     for (auto& kv : s->active)
-      if (s->h_remaining[kv.first] <= 0) finished.push_back(kv.first);
-    for (int slot : finished) {
-      Request* r = s->active[slot];
-      for (size_t k = 0; k < s->fams.size(); ++k)
-        if ((r->pools >> k) & 1u) s->fams[k].active_reqs -= 1;
-      retire(s, r);
-      r->done = true;
-      r->t_done = now_s();
-      s->active.erase(slot);
-      s->free_slots.push_back(slot);
+      if (kv.second->first_iter <= prev && counts[kv.first] <= 0) finished.push_back(kv.first);
+    if (!finished.empty()) {
+      const double t_done = now_s();
+      for (int slot : finished) {
+        Request* r = s->active[slot];
+        for (size_t q = 0; q < s->fams.size(); ++q)
+          if ((r->pools >> q) & 1u) s->fams[q].active_reqs -= 1;
+        s->arena.release(r->dmem, r->dbytes);  // the results stay until delivered
+        r->dmem = nullptr;
+        r->done = true;
+        r->t_done = t_done;
+        s->active.erase(slot);
+        s->free_slots.push_back(slot);
+      }
+      s->cv_done.notify_all();
     }
-    if (!finished.empty()) s->cv_done.notify_all();
+    s->t_retire += now_s() - t_i1;
   }
+  if (inflight >= 0) (void)hipEventSynchronize(s->ev_counts[inflight & 1]);
 }
 
 template <class Slot>
@@ -798,14 +891,30 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
   size_t pools = 0;
   for (auto& f : s->fams) pools += f.slot_bytes * f.nslots + sizeof(Range) * (size_t)f.max_ranges;
   ok = ok && hipMalloc(&s->d_reqs, sizeof(de::serve::ReqDev) * kMaxReq) == hipSuccess;
-  ok = ok && hipHostMalloc(&s->h_remaining, sizeof(int32_t) * kMaxReq, 0) == hipSuccess;
+  for (auto& h : s->h_counts)
+    ok = ok && hipHostMalloc(&h, sizeof(int32_t) * kMaxReq, 0) == hipSuccess;
   ok = ok && hipMalloc(&s->d_remaining, sizeof(int32_t) * kMaxReq) == hipSuccess;
   ok = ok && hipStreamCreateWithFlags(&s->ctl, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&s->upl, hipStreamNonBlocking) == hipSuccess;
+  for (auto& e : s->ev_counts)
+    ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&s->ev_upl, hipEventDisableTiming) == hipSuccess;
+  s->stage_size = (size_t)64 << 20;
+  ok = ok && hipHostMalloc(&s->stage, s->stage_size, 0) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) == hipSuccess;
   s->fam_done.resize(s->fams.size());
   for (auto& e : s->fam_done) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
   if (ok && hbm_budget_bytes <= pools) ok = false;
-  ok = ok && s->arena.init(hbm_budget_bytes - pools);
+  ok = ok && s->arena.init(hbm_budget_bytes - pools, false);
+  // Results go to pinned host memory that the kernels write directly, so a
+  // finished request is retired without a device copy while the device keeps
+  // iterating: min(budget / 8, 4 GiB), or DE_SERVER_HOST_BYTES.
+  size_t host_bytes = std::min(hbm_budget_bytes / 8, (size_t)4 << 30);
+  if (const char* env = getenv("DE_SERVER_HOST_BYTES")) {
+    long long v = atoll(env);
+    if (v >= (1 << 20)) host_bytes = (size_t)v;
+  }
+  ok = ok && s->harena.init(host_bytes, true);
   ok = ok && hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     de_server_destroy(s);
@@ -926,29 +1035,46 @@ int de_server_poll(de_server* s, uint64_t ticket, de_ode_result* out, int32_t wa
     if (!wait) return 0;
     s->cv_done.wait(lk);
   }
-  if (r->error) return r->error;
-  if (out) {
+  // Delivered once: the results leave the host arena and the request is
+  // forgotten (its latency stays queryable).
+  const int err = r->error;
+  if (!err && out) {
+    const de::OdeArgs& a = r->out;
     const int64_t n = r->b.n;
-    if (out->y_final) memcpy(out->y_final, r->y.data(), r->y.size() * 8);
-    if (out->t_final) memcpy(out->t_final, r->t.data(), n * 8);
-    if (out->accepted) memcpy(out->accepted, r->acc.data(), n * 8);
-    if (out->rejected) memcpy(out->rejected, r->rej.data(), n * 8);
-    if (out->evals) memcpy(out->evals, r->ev.data(), n * 8);
-    if (out->status) memcpy(out->status, r->status.data(), n * 4);
-    if (out->dense && !r->dense.empty()) memcpy(out->dense, r->dense.data(), r->dense.size() * 8);
-    if (out->n_dense && !r->n_dense.empty()) memcpy(out->n_dense, r->n_dense.data(), n * 4);
-    if (out->event_t && !r->event_t.empty())
-      memcpy(out->event_t, r->event_t.data(), r->event_t.size() * 8);
-    if (out->event_y && !r->event_y.empty())
-      memcpy(out->event_y, r->event_y.data(), r->event_y.size() * 8);
-    if (out->n_events && !r->n_events.empty()) memcpy(out->n_events, r->n_events.data(), n * 4);
+    const size_t sl = (size_t)r->state_len;
+    auto put = [](void* dst, const void* src, size_t bytes) {
+      if (dst && src && bytes) memcpy(dst, src, bytes);
+    };
+    put(out->y_final, a.y_final, sl * 8);
+    put(out->t_final, a.t_final, n * 8);
+    put(out->accepted, a.accepted, n * 8);
+    put(out->rejected, a.rejected, n * 8);
+    put(out->evals, a.evals, n * 8);
+    put(out->status, a.status, n * 4);
+    if (a.dense) {
+      put(out->dense, a.dense, (size_t)r->b.n_eval * sl * 8);
+      put(out->n_dense, a.n_dense, n * 4);
+    }
+    if (a.ev.n_out) {
+      const size_t m = (size_t)a.ev.max_events;
+      put(out->event_t, a.ev.t_out, m * n * 8);
+      put(out->event_y, a.ev.y_out, m * sl * 8);
+      put(out->n_events, a.ev.n_out, n * 4);
+    }
   }
-  return 1;
+  s->latency[ticket] = r->t_done - r->t_submit;
+  if (r->hmem) s->harena.release(r->hmem, r->hbytes);
+  s->all.erase(it);
+  delete r;
+  s->cv_work.notify_all();  // the freed results may let a waiting request in
+  return err ? err : 1;
 }
 
 double de_server_latency(de_server* s, uint64_t ticket) {
   if (!s) return -1.0;
   std::lock_guard<std::mutex> lk(s->mu);
+  auto d = s->latency.find(ticket);
+  if (d != s->latency.end()) return d->second;
   auto it = s->all.find(ticket);
   if (it == s->all.end() || !it->second->done) return -1.0;
   return it->second->t_done - it->second->t_submit;
@@ -959,6 +1085,15 @@ int de_server_stats(de_server* s, int64_t* iterations, int64_t* range_resets) {
   std::lock_guard<std::mutex> lk(s->mu);
   if (iterations) *iterations = s->iterations;
   if (range_resets) *range_resets = s->range_resets;
+  return DE_OK;
+}
+
+int de_server_timing(de_server* s, double* admit_s, double* iterate_s, double* retire_s) {
+  if (!s) return DE_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (admit_s) *admit_s = s->t_admit;
+  if (iterate_s) *iterate_s = s->t_iter;
+  if (retire_s) *retire_s = s->t_retire;
   return DE_OK;
 }
 
@@ -979,14 +1114,21 @@ void de_server_destroy(de_server* s) {
     if (f.stream) (void)hipStreamDestroy(f.stream);
   }
   for (auto& kv : s->all) delete kv.second;
-  if (s->arena.base) (void)hipFree(s->arena.base);
+  s->arena.destroy();
+  s->harena.destroy();
   if (s->d_reqs) (void)hipFree(s->d_reqs);
-  if (s->h_remaining) (void)hipHostFree(s->h_remaining);
+  for (auto h : s->h_counts)
+    if (h) (void)hipHostFree(h);
   if (s->d_remaining) (void)hipFree(s->d_remaining);
   for (auto& e : s->fam_done)
     if (e) (void)hipEventDestroy(e);
   if (s->fork) (void)hipEventDestroy(s->fork);
+  for (auto e : s->ev_counts)
+    if (e) (void)hipEventDestroy(e);
+  if (s->ev_upl) (void)hipEventDestroy(s->ev_upl);
+  if (s->stage) (void)hipHostFree(s->stage);
   if (s->ctl) (void)hipStreamDestroy(s->ctl);
+  if (s->upl) (void)hipStreamDestroy(s->upl);
   delete s;
 }
 

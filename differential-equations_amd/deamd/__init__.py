@@ -935,6 +935,12 @@ class Server:
         _abi.check(lib().de_server_stats(self.h, C.byref(it), C.byref(rr)))
         return it.value, rr.value
 
+    def timing(self):
+        """Scheduler seconds spent admitting, iterating and retiring."""
+        a, i, r = C.c_double(), C.c_double(), C.c_double()
+        _abi.check(lib().de_server_timing(self.h, C.byref(a), C.byref(i), C.byref(r)))
+        return a.value, i.value, r.value
+
     def close(self):
         if self.h:
             lib().de_server_destroy(self.h)

@@ -353,9 +353,13 @@ int de_workspace_trim(void);                 /* return every kept block to the d
 
 /* ---- Serving: continuous batching of ensemble solves on one GPU ----------
  * Requests (each a de_ode_batch over host memory) are admitted while their
- * device footprint fits the HBM budget and lanes are free; their state stays
- * resident on the device until the request completes.  Requests admitted
- * while others run share the persistent lanes of the running kernels. */
+ * inputs fit the HBM budget (slot pools included), their results fit a pinned
+ * host arena of min(budget / 8, 4 GiB) (the environment variable
+ * DE_SERVER_HOST_BYTES, read at creation, sets another size) and request slots
+ * are free; their state stays resident on the device until the request
+ * completes, and the kernels write each trajectory's results straight into the
+ * host arena.  Requests admitted while others run share the persistent lanes
+ * of the running kernels. */
 typedef struct de_server de_server;
 int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out);
 /* Copies the request's inputs; returns a ticket. */
@@ -370,6 +374,10 @@ double de_server_latency(de_server* s, uint64_t ticket);
  * published ranges; the environment variable DE_SERVER_MAX_RANGES, read at
  * creation, lowers that capacity). */
 int de_server_stats(de_server* s, int64_t* iterations, int64_t* range_resets);
+/* Scheduler wall time so far, in seconds: admitting requests (uploads and
+ * publishing their work), iterating (launch to the remaining counts on the
+ * host) and retiring finished requests (result copies). */
+int de_server_timing(de_server* s, double* admit_s, double* iterate_s, double* retire_s);
 void de_server_destroy(de_server* s);
 
 #ifdef __cplusplus

@@ -107,9 +107,10 @@ def test_spring_requests_within_bound(orc):
 
 
 def test_admission_respects_budget():
-    # Budget a little above the slot pools (about 0.6 GB): a 4M-trajectory
-    # request cannot fit and is rejected; small ones are admitted
-    # progressively as earlier ones retire, and all complete.
+    # Budget a little above the slot pools (about 0.6 GB), so a results arena
+    # of 128 MiB: a 4M-trajectory request (250 MB of results) can never fit
+    # and is rejected; small ones are admitted progressively as earlier ones
+    # retire, and all complete.
     with Server(1 * GIB) as srv:
         big = lorenz_req(1 << 22, seed=3, tf=0.1)
         t_big = srv.submit(big)
