@@ -312,6 +312,7 @@ struct Dopri5Lane {
   static constexpr int D = F::kDim;
   static constexpr int kEv = EV;
   static constexpr int kWavesPerEu = 3;  // occupancy hint of the persistent kernel
+  static constexpr bool kLiveStep = true;  // step(a, i, live_in): lanes without work ride along
   // Deferred event bracket: the meta fields, then y, y_new, f(t, y) and
   // f(t_new, y_new); the root pass forms the interpolant from them.
   static constexpr int kEntryWidth = kEntryMeta + 4 * D;
@@ -482,23 +483,24 @@ struct Dopri5Lane {
 
   // One pass of the driver loop: overshoot clip, one DOPRI5 attempt, solout,
   // end test.  Returns -1 while running, else the final status (the caller
-  // writes the results with finish()).
-  DE_HD int32_t step(const OdeArgs& a, int64_t i) {
-    {
-      // Overshoot clip and the two pre-step error checks of the reference,
-      // evaluated as flags so the loop has a single early exit.
-      const bool clip = (t + h - tf) * dir > 0.0;
-      const double hn = tf - t;
-      const bool done = clip && fabs(hn) < kTenEps;
-      h = clip ? hn : h;
-      const bool small = fabs(h) < fabs(h_prev) * 1e-14;
-      const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
-      if (done | small | maxed) return done ? kComplete : (small ? kStepSize : kMaxSteps);
-    }
+  // writes the results with finish()).  A lane with live_in false (no
+  // trajectory) or that ends at the pre-step checks runs the attempt's
+  // arithmetic with its results discarded: the body has no branch around it,
+  // so the state updates stay selects on the lane's own registers.
+  DE_HD int32_t step(const OdeArgs& a, int64_t i, bool live_in = true) {
+    // Overshoot clip and the two pre-step error checks of the reference.
+    const bool clip = live_in && (t + h - tf) * dir > 0.0;
+    const double hn = tf - t;
+    const bool done = clip && fabs(hn) < kTenEps;
+    h = clip ? hn : h;
+    const bool small = fabs(h) < fabs(h_prev) * 1e-14;
+    const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
+    const int32_t early = done ? kComplete : (small ? kStepSize : kMaxSteps);
+    const bool live = live_in && !(done | small | maxed);
     // The stiffness test runs on accepted attempts whose number is a multiple
     // of 100 (ordinary.rs:167); the countdown restarts on every such attempt.
-    countdown -= 1;
-    const bool stiff_check = countdown == 0;
+    countdown -= live ? 1 : 0;
+    const bool stiff_check = live && countdown == 0;
     if (stiff_check) countdown = 100;
 
     double k1[D], k2[D], k3[D], k4[D], k5[D], k6[D], ys[D];
@@ -583,8 +585,8 @@ struct Dopri5Lane {
     }
     // Error::Stiffness (stiff reaches 15 only on an accepted attempt) ends the
     // solve with (t, y) at the start of the step: the attempt is not taken.
-    const bool stop = stiff == 15;
-    const bool take = accept && !stop;
+    const bool stop = live && stiff == 15;
+    const bool take = live && accept && !stop;
     int32_t sol = -1;  // solout outcome: -1 continue, else the final status
     if constexpr (DENSE || EV != 0) {
       // The event function at the attempt's end is evaluated branch-free; the
@@ -667,13 +669,14 @@ struct Dopri5Lane {
     }
     // After a rejection the next accepted step may not grow h.
     scale = take && rejected_last ? fmin(scale, 1.0) : scale;
-    rejected_last = !accept;
+    rejected_last = live ? !accept : rejected_last;
     acc += take ? 1 : 0;
-    rej += accept ? 0 : 1;
+    rej += live && !accept ? 1 : 0;
     const int32_t result =
-        stop ? kStiffness
-             : (sol >= 0 ? sol : (take && fabs(tf - t) <= kTenEps ? kComplete : -1));
-    h = constrain_step_size(h * scale, a.h_min, a.h_max);
+        !live ? (live_in ? early : -1)
+              : (stop ? kStiffness
+                      : (sol >= 0 ? sol : (take && fabs(tf - t) <= kTenEps ? kComplete : -1)));
+    h = live ? constrain_step_size(h * scale, a.h_min, a.h_max) : h;
     return result;
   }
 

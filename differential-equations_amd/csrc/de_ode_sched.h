@@ -21,6 +21,8 @@
 #include "de_lane.h"
 #include "de_launch.h"
 
+#include <type_traits>
+
 #ifndef DE_KNS
 #error "define DE_KNS before including de_ode_sched.h"
 #endif
@@ -43,6 +45,12 @@ __global__ __launch_bounds__(kBlock) void ode_plain_kernel(OdeArgs a) {
   } while (st < 0);
   L.finish(a, i, st);
 }
+
+// Lanes whose step takes a live flag (Dopri5Lane::kLiveStep).
+template <class L, class = void>
+struct LiveStep : std::false_type {};
+template <class L>
+struct LiveStep<L, std::void_t<decltype(L::kLiveStep)>> : std::bool_constant<L::kLiveStep> {};
 
 // Waves per SIMD from the lane (Lane::kWavesPerEu): three for most lanes (the
 // DOPRI5 lane needs about 165 VGPRs and the hint keeps the allocation under
@@ -85,7 +93,15 @@ ode_persistent_kernel(const OdeArgs* __restrict__ ap) {
       if (drained) break;
       continue;
     }
-    if (have) {
+    if constexpr (LiveStep<Lane>::value) {
+      // Every lane of the wave runs the attempt; a lane without work discards
+      // it, so the lane state is never carried around a divergent branch.
+      const int32_t st = L.step(a, idx, have);
+      if (have && st >= 0) {
+        L.finish(a, idx, st);
+        have = false;
+      }
+    } else if (have) {
       int32_t st = L.step(a, idx);
       if (st >= 0) {
         L.finish(a, idx, st);
