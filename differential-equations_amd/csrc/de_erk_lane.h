@@ -78,6 +78,7 @@ struct ErkLane {
   // two (256 VGPRs) for the lanes of more than 36 doubles of interpolation
   // stages, which spill at three.
   static constexpr int kWavesPerEu = (KN == S || KN * D <= 36) ? 3 : 2;
+  static constexpr bool kLiveStep = true;  // step(a, i, live_in)
   // Evaluations per accepted step beyond the S - 1 of every attempt
   // (ordinary.rs:151-181): f(t_new, y_new) unless FSAL, the dense stages.
   static constexpr int kAccEvals = (kFsal ? 0 : 1) + (kBi ? I - S : 0);
@@ -407,16 +408,18 @@ struct ErkLane {
   }
 
   // One pass of the driver loop (solve_ivp.rs:190-263 around ordinary.rs:63-210).
-  DE_HD int32_t step(const OdeArgs& a, int64_t i) {
-    {
-      const bool clip = (t + h - tf) * dir > 0.0;
-      const double hn = tf - t;
-      const bool done = clip && fabs(hn) < kTenEps;
-      h = clip ? hn : h;
-      const bool small = fabs(h) < fabs(h_prev) * 1e-14;
-      const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
-      if (done | small | maxed) return done ? kComplete : (small ? kStepSize : kMaxSteps);
-    }
+  // Lanes with live_in false (no trajectory) or ending at the pre-step
+  // checks run the attempt with its results discarded, so the state updates
+  // stay selects on the lane's own registers.
+  DE_HD int32_t step(const OdeArgs& a, int64_t i, bool live_in = true) {
+    const bool clip = live_in && (t + h - tf) * dir > 0.0;
+    const double hn = tf - t;
+    const bool done = clip && fabs(hn) < kTenEps;
+    h = clip ? hn : h;
+    const bool small = fabs(h) < fabs(h_prev) * 1e-14;
+    const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
+    const int32_t early = done ? kComplete : (small ? kStepSize : kMaxSteps);
+    const bool live = live_in && !(done | small | maxed);
     double k[KN][D];
     uint32_t bad[D];
     const uint32_t hbad = isfinite(h) ? 0u : ~0u;
@@ -440,9 +443,9 @@ struct ErkLane {
     double scale = a.safety * DE_CTRL_POW(m, -(1.0 / (double)O));
     scale = fmin(fmax(scale, a.min_scale), a.max_scale);
 
-    const bool accept = m <= 1.0;
+    const bool accept = live && m <= 1.0;
     const int32_t stiff_next = accept ? (rejected_last ? 0 : stiff) : stiff + 1;
-    const bool stop = !accept && (int64_t)stiff_next >= a.max_rejects;
+    const bool stop = live && !accept && (int64_t)stiff_next >= a.max_rejects;
     const double t_new = t + h;
     double dn[D];
     if constexpr (kFsal) {
@@ -542,14 +545,15 @@ struct ErkLane {
       k0[c] = accept ? dn[c] : k0[c];
     }
     scale = accept && rejected_last ? fmin(scale, 1.0) : scale;
-    rejected_last = !accept;
-    stiff = stiff_next;
+    rejected_last = live ? !accept : rejected_last;
+    stiff = live ? stiff_next : stiff;
     acc += accept ? 1 : 0;
-    rej += (accept || stop) ? 0 : 1;
+    rej += live && !accept && !stop ? 1 : 0;
     const int32_t result =
-        stop ? kStiffness
-             : (sol >= 0 ? sol : (accept && fabs(tf - t) <= kTenEps ? kComplete : -1));
-    h = constrain_step_size(h * scale, a.h_min, a.h_max);
+        !live ? (live_in ? early : -1)
+              : (stop ? kStiffness
+                      : (sol >= 0 ? sol : (accept && fabs(tf - t) <= kTenEps ? kComplete : -1)));
+    h = live ? constrain_step_size(h * scale, a.h_min, a.h_max) : h;
     return result;
   }
 

@@ -716,6 +716,7 @@ struct Dop853Lane {
   // Occupancy hint: with an event solout the 16-stage lane does not fit the
   // 168 registers of three waves per SIMD without spilling in the step.
   static constexpr int kWavesPerEu = EV != 0 ? 2 : 3;
+  static constexpr bool kLiveStep = true;  // step(a, i, live_in)
   // Deferred event bracket: the meta fields, then cont[0..7].
   static constexpr int kEntryWidth = kEntryMeta + 8 * D;
   F f;
@@ -916,18 +917,19 @@ struct Dop853Lane {
     return s;
   }
 
-  DE_HD int32_t step(const OdeArgs& a, int64_t i) {
-    {
-      const bool clip = (t + h - tf) * dir > 0.0;
-      const double hn = tf - t;
-      const bool done = clip && fabs(hn) < kTenEps;
-      h = clip ? hn : h;
-      const bool small = fabs(h) < fabs(h_prev) * 1e-14;
-      const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
-      if (done | small | maxed) return done ? kComplete : (small ? kStepSize : kMaxSteps);
-    }
-    countdown -= 1;
-    const bool stiff_check = countdown == 0;
+  // As Dopri5Lane::step: lanes with live_in false or ending at the pre-step
+  // checks run the attempt with its results discarded.
+  DE_HD int32_t step(const OdeArgs& a, int64_t i, bool live_in = true) {
+    const bool clip = live_in && (t + h - tf) * dir > 0.0;
+    const double hn = tf - t;
+    const bool done = clip && fabs(hn) < kTenEps;
+    h = clip ? hn : h;
+    const bool small = fabs(h) < fabs(h_prev) * 1e-14;
+    const bool maxed = (int64_t)(acc + rej) >= a.max_steps;
+    const int32_t early = done ? kComplete : (small ? kStepSize : kMaxSteps);
+    const bool live = live_in && !(done | small | maxed);
+    countdown -= live ? 1 : 0;
+    const bool stiff_check = live && countdown == 0;
     if (stiff_check) countdown = 100;
 
     double k[(DENSE || EV != 0) ? 16 : 12][D];
@@ -985,8 +987,8 @@ struct Dop853Lane {
         if (nonstiff == 6) stiff = 0;
       }
     }
-    const bool stop = stiff == 15;
-    const bool take = accept && !stop;
+    const bool stop = live && stiff == 15;
+    const bool take = live && accept && !stop;
     int32_t sol = -1;
     if constexpr (DENSE || EV != 0) {
       // The continuous extension is built only when an accepted step has
@@ -1071,13 +1073,14 @@ struct Dop853Lane {
       k0[c] = take ? dn[c] : k0[c];
     }
     scale = take && rejected_last ? fmin(scale, 1.0) : scale;
-    rejected_last = !accept;
+    rejected_last = live ? !accept : rejected_last;
     acc += take ? 1 : 0;
-    rej += accept ? 0 : 1;
+    rej += live && !accept ? 1 : 0;
     const int32_t result =
-        stop ? kStiffness
-             : (sol >= 0 ? sol : (take && fabs(tf - t) <= kTenEps ? kComplete : -1));
-    h = constrain_step_size(h * scale, a.h_min, a.h_max);
+        !live ? (live_in ? early : -1)
+              : (stop ? kStiffness
+                      : (sol >= 0 ? sol : (take && fabs(tf - t) <= kTenEps ? kComplete : -1)));
+    h = live ? constrain_step_size(h * scale, a.h_min, a.h_max) : h;
     return result;
   }
 
