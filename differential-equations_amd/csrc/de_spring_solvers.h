@@ -319,6 +319,442 @@ struct VecDopri5 {
   }
 };
 
+// Shared pieces of the one-lane-per-system solvers below: TEvalSolout over a
+// system's dense rows (t_eval.rs:83-134) with an interpolation callback, the
+// write-back, and the automatic initial step (h_init.rs:45-135) of order ORDER.
+struct VecCommon {
+  const SpringArgs* A;
+  int64_t i;
+  int nm, n;  // masses, state length
+  const double* p;
+  double* y;  // y_final slot doubles as the working state
+  double t, h, tf, dir;
+  int32_t next_eval, pushed;
+
+  DE_HD void diff(double, const double* yy, double* dy) const { spring_chain_diff(nm, p, yy, dy); }
+
+  DE_HD void write(int32_t st, int64_t acc, int64_t rej, int64_t ev) {
+    const OdeArgs& a = A->o;
+    a.t_final[i] = t;
+    a.accepted[i] = acc;
+    a.rejected[i] = rej;
+    a.evals[i] = ev;
+    a.status[i] = st;
+    if (a.n_dense) a.n_dense[i] = pushed;
+    if (a.n_eval > 0) {  // points not reached (all of them on BadInput)
+      double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
+      for (int64_t e = (int64_t)next_eval * n; e < (int64_t)a.n_eval * n; ++e)
+        dbase[e] = as_f64(kNaNBits);
+    }
+  }
+
+  // interp(te, out) writes the state at te in [t_prev, t_curr]; with
+  // have_cont false only the step's end is written (the start of a solve).
+  template <class Interp>
+  DE_HD void emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
+                  Interp interp) {
+    const OdeArgs& a = A->o;
+    int32_t idx = next_eval;
+    double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
+    while (idx < a.n_eval) {
+      double te = a.t_eval[idx];
+      bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
+                                : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
+      if (in_range) {
+        double* out = dbase + (int64_t)idx * n;
+        if (te == t_curr || !have_cont) {
+          for (int c = 0; c < n; ++c) out[c] = ycur[c];
+        } else {
+          interp(te, out);
+        }
+        pushed += 1;
+        idx += 1;
+      } else {
+        if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
+        double* out = dbase + (int64_t)idx * n;  // passed over: not produced
+        for (int c = 0; c < n; ++c) out[c] = as_f64(kNaNBits);
+        idx += 1;
+      }
+    }
+    next_eval = idx;
+  }
+
+  // Common prologue: the system, its state and time span.  w: this system's
+  // workspace.
+  DE_HD double setup(const SpringArgs* args, int64_t sys) {
+    A = args;
+    i = sys;
+    const OdeArgs& a = A->o;
+    nm = A->sizes[i];
+    n = 2 * nm;
+    p = a.params + A->param_offset[i];
+    const int64_t so = A->state_offset[i];
+    y = a.y_final + so;
+    const double* y0 = a.y0 + so;
+    for (int c = 0; c < n; ++c) y[c] = y0[c];
+    const double t0 = a.t0 ? a.t0[i] : a.t0s;
+    tf = a.tf ? a.tf[i] : a.tfs;
+    t = t0;
+    pushed = next_eval = 0;
+    dir = signum(tf - t0);
+    return t0;
+  }
+
+  // h_init with f0, f1, sk, y1 in workspace vectors.
+  template <int ORDER>
+  DE_HD double h_init(double t0, double* f0, double* f1, double* sk, double* y1) const {
+    const OdeArgs& a = A->o;
+    const int64_t so = A->state_offset[i];
+    const double rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
+    const double atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    const double* rtv = a.rtol_comp ? a.rtol_comp + so : nullptr;
+    const double* atv = a.atol_comp ? a.atol_comp + so : nullptr;
+    diff(t0, y, f0);
+    double dnf = 0.0, dny = 0.0;
+    for (int c = 0; c < n; ++c) {
+      sk[c] = (atv ? atv[c] : atol) + (rtv ? rtv[c] : rtol) * fabs(y[c]);
+      double u = f0[c] / sk[c];
+      dnf += u * u;
+      double w2 = y[c] / sk[c];
+      dny += w2 * w2;
+    }
+    double hh = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+    hh = fmin(hh, a.h_max);
+    hh *= dir;
+    for (int c = 0; c < n; ++c) y1[c] = y[c] + hh * f0[c];
+    diff(t0 + hh, y1, f1);
+    double der2 = 0.0;
+    for (int c = 0; c < n; ++c) {
+      double d = (f1[c] - f0[c]) / sk[c];
+      der2 += d * d;
+    }
+    der2 = sqrt(der2) / fabs(hh);
+    double der12 = fmax(sqrt(dnf), der2);
+    double h1 = der12 <= 1.0e-15 ? fabs(hh) * 1.0e-3
+                                 : DE_CTRL_POW(0.01 / der12, 1.0 / (double)ORDER);
+    double interval = fabs(tf - t0);
+    hh = fmin(fmax(fmin(fmin(fabs(hh) * 100.0, h1), a.h_max), a.h_min), interval);
+    return hh * dir;
+  }
+};
+
+// ---------------------------------------------------------------- VecFixed --
+// RK4 / Euler (fixed/ordinary.rs, the same steps as FixedLane) on one lane per
+// system, vectors in the workspace: k0..k3, ys, ynext, dn.
+template <int STAGES>
+struct VecFixed : VecCommon {
+  static constexpr int kVectors = 7;
+  double *k0, *k1, *k2, *k3, *ys, *ynext, *dn;
+  int64_t steps, acc, ev;
+
+  DE_HD void finish(int32_t st) { write(st, acc, 0, ev); }
+
+  DE_HD bool init(const SpringArgs* args, int64_t sys) {
+    const double t0 = setup(args, sys);
+    const OdeArgs& a = A->o;
+    double* w = A->work + kVectors * A->state_offset[i];
+    k0 = w;
+    k1 = w + n;
+    k2 = w + 2 * (int64_t)n;
+    k3 = w + 3 * (int64_t)n;
+    ys = w + 4 * (int64_t)n;
+    ynext = w + 5 * (int64_t)n;
+    dn = w + 6 * (int64_t)n;
+    acc = ev = steps = 0;
+    bool ok = dir == 1.0 || dir == -1.0;
+    double h0 = a.h0;
+    if (h0 == 0.0) h0 = fabs(tf - t0) / 100.0;
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      finish(kBadInput);
+      return false;
+    }
+    h = h0;
+    diff(t, y, k0);
+    ev = 1;
+    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) {});
+    return true;
+  }
+
+  DE_HD bool advance() {
+    const OdeArgs& a = A->o;
+    if ((t + h - tf) * dir > 0.0) {
+      double hn = tf - t;
+      if (fabs(hn) < kTenEps) {
+        finish(kComplete);
+        return true;
+      }
+      h = hn;
+    }
+    if (steps >= a.max_steps) {
+      finish(kMaxSteps);
+      return true;
+    }
+    steps += 1;
+    if (STAGES == 4) {
+      const double hh = 0.5 * h;
+      for (int c = 0; c < n; ++c) ys[c] = y[c] + hh * k0[c];
+      diff(t + 0.5 * h, ys, k1);
+      for (int c = 0; c < n; ++c) ys[c] = y[c] + hh * k1[c];
+      diff(t + 0.5 * h, ys, k2);
+      const double h1 = 1.0 * h;
+      for (int c = 0; c < n; ++c) ys[c] = y[c] + h1 * k2[c];
+      diff(t + 1.0 * h, ys, k3);
+      const double b0h = (1.0 / 6.0) * h, b1h = (1.0 / 3.0) * h;
+      for (int c = 0; c < n; ++c)
+        ynext[c] = (((y[c] + b0h * k0[c]) + b1h * k1[c]) + b1h * k2[c]) + b0h * k3[c];
+      ev += 3;
+    } else {
+      const double b0h = 1.0 * h;
+      for (int c = 0; c < n; ++c) ynext[c] = y[c] + b0h * k0[c];
+    }
+    const double tp = t;
+    t += h;
+    diff(t, ynext, dn);
+    ev += 1;
+    acc += 1;
+    if (a.n_eval > 0) {
+      const double tc = t;
+      // cubic_hermite_interpolate on [t_prev, t] (fixed/ordinary.rs:170-177).
+      emit(t, tp, ynext, true, [&](double te, double* out) {
+        double hh = tc - tp;
+        double s = (te - tp) / hh;
+        double s2 = s * s;
+        double s3 = s2 * s;
+        double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+        double h10 = s3 - 2.0 * s2 + s;
+        double h01 = -2.0 * s3 + 3.0 * s2;
+        double h11 = s3 - s2;
+        double w10 = h10 * hh, w11 = h11 * hh;
+        for (int c = 0; c < n; ++c)
+          out[c] = ((h00 * y[c] + w10 * k0[c]) + h01 * ynext[c]) + w11 * dn[c];
+      });
+    }
+    for (int c = 0; c < n; ++c) {
+      y[c] = ynext[c];
+      k0[c] = dn[c];
+    }
+    if (fabs(tf - t) <= kTenEps) {
+      finish(kComplete);
+      return true;
+    }
+    return false;
+  }
+};
+
+// --------------------------------------------------------------- VecDop853 --
+// DOP853 (the generic Dormand-Prince step of dormandprince/ordinary.rs:63-270
+// with the tableau of src/tableau/dorman_prince.rs:155-380; the same operations
+// as Dop853Lane) on one lane per system.  Workspace: k[0..15], ys, ynew, yseg
+// and the dense-output coefficients cont[1..7] (cont[0] is y).
+struct VecDop853 : VecCommon {
+  static constexpr int kVectors = 26;
+  double *k[16], *ys, *ynew, *yseg, *cont[8];
+  double h_prev, rtol, atol;
+  const double *rtv, *atv;
+  int64_t acc, rej, ev, steps;
+  int32_t stiff, nonstiff, countdown;
+  bool rejected_last;
+
+  DE_HD double rt(int c) const { return rtv ? rtv[c] : rtol; }
+  DE_HD double at(int c) const { return atv ? atv[c] : atol; }
+  DE_HD void finish(int32_t st) { write(st, acc, rej, ev); }
+
+  // ys = y + sum_{j<I} (a[I][j] h) k[j], zero coefficients skipped.
+  DE_HD void stage_input(int I, const double* yy, double hh, double* out) const {
+    for (int c = 0; c < n; ++c) out[c] = yy[c];
+    for (int j = 0; j < I; ++j) {
+      if (d8::A[I][j] != 0.0) {
+        const double ah = d8::A[I][j] * hh;
+        for (int c = 0; c < n; ++c) out[c] = out[c] + ah * k[j][c];
+      }
+    }
+  }
+
+  // sum_j w[j] k[j][c] from the first nonzero weight.
+  DE_HD double comb(const double* wts, int nw, int c) const {
+    double s = 0.0;
+    bool first = true;
+    for (int j = 0; j < nw; ++j) {
+      if (wts[j] != 0.0) {
+        s = first ? wts[j] * k[j][c] : s + wts[j] * k[j][c];
+        first = false;
+      }
+    }
+    return s;
+  }
+
+  DE_HD bool init(const SpringArgs* args, int64_t sys) {
+    const double t0 = setup(args, sys);
+    const OdeArgs& a = A->o;
+    const int64_t so = A->state_offset[i];
+    double* w = A->work + kVectors * so;
+    for (int v = 0; v < 16; ++v) k[v] = w + (int64_t)v * n;
+    ys = w + 16 * (int64_t)n;
+    ynew = w + 17 * (int64_t)n;
+    yseg = w + 18 * (int64_t)n;
+    cont[0] = y;
+    for (int r = 1; r < 8; ++r) cont[r] = w + (int64_t)(18 + r) * n;
+    rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
+    atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    rtv = a.rtol_comp ? a.rtol_comp + so : nullptr;
+    atv = a.atol_comp ? a.atol_comp + so : nullptr;
+    acc = rej = ev = steps = 0;
+    bool ok = dir == 1.0 || dir == -1.0;
+    double h0 = a.h0;
+    ev = 1;
+    if (ok && h0 == 0.0) {
+      h0 = h_init<8>(t0, k[0], k[1], ys, ynew);
+      ev = 3;
+    }
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      ev = 0;
+      finish(kBadInput);
+      return false;
+    }
+    h = h0;
+    diff(t, y, k[0]);
+    h_prev = 0.0;
+    stiff = nonstiff = 0;
+    countdown = 100;
+    rejected_last = false;
+    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) {});
+    return true;
+  }
+
+  DE_HD bool advance() {
+    const OdeArgs& a = A->o;
+    if ((t + h - tf) * dir > 0.0) {
+      double hn = tf - t;
+      if (fabs(hn) < kTenEps) {
+        finish(kComplete);
+        return true;
+      }
+      h = hn;
+    }
+    if (fabs(h) < fabs(h_prev) * 1e-14) {
+      finish(kStepSize);
+      return true;
+    }
+    if (steps >= a.max_steps) {
+      finish(kMaxSteps);
+      return true;
+    }
+    steps += 1;
+    countdown -= 1;
+    const bool stiff_check = countdown == 0;
+    if (stiff_check) countdown = 100;
+    for (int I = 1; I < 12; ++I) {  // ys ends as the stage-11 input
+      stage_input(I, y, h, ys);
+      diff(t + d8::C[I] * h, ys, k[I]);
+    }
+    double err = 0.0, err2 = 0.0;
+    for (int c = 0; c < n; ++c) {
+      yseg[c] = comb(d8::B, 12, c);
+      ynew[c] = y[c] + h * yseg[c];
+    }
+    for (int c = 0; c < n; ++c) {
+      const double es = comb(d8::ER, 12, c);
+      double e2s = yseg[c];
+      for (int j = 0; j < 12; ++j)
+        if (d8::BH[j] != 0.0) e2s = e2s + (-d8::BH[j]) * k[j][c];
+      const double sk = at(c) + rt(c) * fmax(fabs(y[c]), fabs(ynew[c]));
+      const double e = es / sk;
+      const double e2 = e2s / sk;
+      err = c == 0 ? e * e : err + e * e;
+      err2 = c == 0 ? e2 * e2 : err2 + e2 * e2;
+    }
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * (double)n));
+    double scale = a.safety * DE_CTRL_POW(err, -0.125);
+    scale = fmin(fmax(scale, a.min_scale), a.max_scale);
+    if (err <= 1.0) {
+      const double t_new = t + h;
+      double* dn = k[12];  // f(t_new, y_new), the 13th stage of the dense output
+      diff(t_new, ynew, dn);
+      if (stiff_check) {
+        double stdnum = 0.0, stden = 0.0;
+        for (int c = 0; c < n; ++c) {
+          double u = yseg[c] - k[11][c];
+          stdnum = c == 0 ? u * u : stdnum + u * u;
+          double v = dn[c] - ys[c];
+          stden = c == 0 ? v * v : stden + v * v;
+        }
+        if (stden > 0.0) {
+          if (stiff_exceeds(h, stdnum, stden)) {
+            nonstiff = 0;
+            stiff += 1;
+            if (stiff == 15) {
+              finish(kStiffness);
+              return true;
+            }
+          }
+        } else {
+          nonstiff += 1;
+          if (nonstiff == 6) stiff = 0;
+        }
+      }
+      if (a.n_eval > 0) {
+        // cont[1..3] as DOPRI5's, k[13..15] at c[13..15], cont[r] = h sum_j
+        // bi[r][j] k[j] for r = 4..7.
+        for (int c = 0; c < n; ++c) {
+          double ydiff = ynew[c] + (-1.0) * y[c];
+          cont[1][c] = ydiff;
+          double bspl = h * k[0][c] + (-1.0) * ydiff;
+          cont[2][c] = bspl;
+          cont[3][c] = (ydiff + (-h) * dn[c]) + (-1.0) * bspl;
+        }
+        for (int I = 13; I < 16; ++I) {
+          stage_input(I, y, h, ys);
+          diff(t + d8::C[I] * h, ys, k[I]);
+        }
+        for (int r = 4; r < 8; ++r)
+          for (int c = 0; c < n; ++c) cont[r][c] = comb(d8::BI[r], 16, c) * h;
+        const double tp = t, hp = h;
+        emit(t_new, tp, ynew, true, [&](double te, double* out) {
+          const double s = (te - tp) / hp;
+          const double s1 = 1.0 - s;
+          for (int c = 0; c < n; ++c) {
+            double q = cont[7][c];
+            q = q * s1 + cont[6][c];
+            q = q * s + cont[5][c];
+            q = q * s1 + cont[4][c];
+            q = q * s1 + cont[3][c];
+            q = q * s + cont[2][c];
+            q = q * s1 + cont[1][c];
+            out[c] = y[c] + s * q;
+          }
+        });
+      }
+      h_prev = h;
+      t = t_new;
+      for (int c = 0; c < n; ++c) {
+        y[c] = ynew[c];
+        k[0][c] = dn[c];
+      }
+      if (rejected_last) {
+        rejected_last = false;
+        scale = fmin(scale, 1.0);
+      }
+      acc += 1;
+      ev += 15;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+      if (fabs(tf - t) <= kTenEps) {
+        finish(kComplete);
+        return true;
+      }
+    } else {
+      rejected_last = true;
+      rej += 1;
+      ev += 11;
+      h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    }
+    return false;
+  }
+};
+
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
 // ----------------------------------------------------------- GroupDopri5 --
 // G lanes per system (G a power of two <= 64), C masses per lane:

@@ -2,7 +2,8 @@
 //
 // Mappings (selected by de_ode_batch.mapping):
 //   DE_MAP_LANE   one thread per system, vectors in an HBM workspace (plain
-//                 launch; bit-identical to the CPU restatement).
+//                 launch; bit-identical to the CPU restatement).  DOPRI5, and
+//                 the only mapping of DOP853, RK4 and Euler.
 //   DE_MAP_GROUP  G lanes per system with C masses per lane in registers.
 //   DE_MAP_AUTO   systems are bucketed into size classes (N <= 4, 8, ..., 256);
 //                 for each class an autotuner times every (G, C, block) with
@@ -25,12 +26,23 @@ namespace parity {
 
 constexpr int kSpringBlock = 256;
 
+template <class Solver>
 __global__ __launch_bounds__(kSpringBlock) void spring_lane_kernel(SpringArgs A) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.o.n) return;
-  VecDopri5 s;
+  Solver s;
   if (!s.init(&A, i)) return;
   while (!s.advance()) {
+  }
+}
+
+// Workspace doubles per state component of the one-lane solver of a method.
+static int lane_vectors(int method) {
+  switch (method) {
+    case DE_METHOD_DOP853: return VecDop853::kVectors;
+    case DE_METHOD_RK4:
+    case DE_METHOD_EULER: return VecFixed<4>::kVectors;
+    default: return 9;  // VecDopri5
   }
 }
 
@@ -193,10 +205,10 @@ static Choice autotune(const SpringArgs& base, int k, const int32_t* d_list, int
 
 int64_t spring_workspace_bytes(const de_ode_batch* b) {
   if (!b) return 0;
-  if (b->mapping == DE_MAP_LANE) {
-    // 9 state-length vectors per system; offsets are device memory here, so
-    // the caller's bound uses 512 doubles per system.
-    return (int64_t)9 * 512 * 8 * b->n;
+  if (b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5) {
+    // The one-lane solver's state-length vectors per system; offsets are
+    // device memory here, so the caller's bound uses 512 doubles per system.
+    return (int64_t)lane_vectors(b->method) * 512 * 8 * b->n;
   }
   return (int64_t)sizeof(int32_t) * b->n + (int64_t)sizeof(unsigned long long) * (kNumCaps + 1);
 }
@@ -225,17 +237,24 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     masses[k] += host_sizes[i];
     state_len += 2 * (int64_t)host_sizes[i];
   }
-  if (b->mapping == DE_MAP_LANE) {
-    if (g_work_len < 9 * state_len) {
+  // One lane per system: DE_MAP_LANE, and every method but DOPRI5 (the group
+  // mapping is built for DOPRI5).
+  if (b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5) {
+    const int64_t need = (int64_t)lane_vectors(b->method) * state_len;
+    if (g_work_len < need) {
       if (g_work) (void)hipFree(g_work);
       g_work = nullptr;
       g_work_len = 0;
-      if (hipMalloc(&g_work, sizeof(double) * 9 * state_len) != hipSuccess) return -4;
-      g_work_len = 9 * state_len;
+      if (hipMalloc(&g_work, sizeof(double) * need) != hipSuccess) return -4;
+      g_work_len = need;
     }
     base.work = g_work;
     int64_t grid = (n + kSpringBlock - 1) / kSpringBlock;
-    hipLaunchKernelGGL(spring_lane_kernel, dim3((unsigned)grid), dim3(kSpringBlock), 0, s, base);
+    void (*k)(SpringArgs) = spring_lane_kernel<VecDopri5>;
+    if (b->method == DE_METHOD_DOP853) k = spring_lane_kernel<VecDop853>;
+    else if (b->method == DE_METHOD_RK4) k = spring_lane_kernel<VecFixed<4>>;
+    else if (b->method == DE_METHOD_EULER) k = spring_lane_kernel<VecFixed<1>>;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(kSpringBlock), 0, s, base);
     return hipGetLastError() == hipSuccess ? 0 : -1;
   }
   // Group mapping: upload class lists (one buffer, classes back to back).

@@ -76,11 +76,11 @@ def test_invalid_descriptors_are_rejected_before_any_device_work(mutate, code):
     assert deamd.lib().de_last_error()
 
 
-def test_spring_chains_accept_dopri5_only():
-    # The spring-chain solvers implement DOPRI5; other methods are refused
-    # rather than silently solved with DOPRI5.
+def test_spring_chains_refuse_the_embedded_family():
+    # The spring-chain solvers implement DOPRI5, DOP853, RK4 and Euler; the
+    # embedded family is refused rather than silently solved with another method.
     sysm, y0 = deamd.gen_spring(8, seed=1)
-    for m in (deamd.ExplicitRungeKutta.rk4(0.01), deamd.ExplicitRungeKutta.dop853()):
+    for m in (deamd.ExplicitRungeKutta.rkf45(), deamd.ExplicitRungeKutta.rkv878e()):
         ens = deamd.IVP.ode(sysm, 0.0, 1.0, y0).method(m)
         b, res, keep = ens.batch()
         assert deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None) == \

@@ -142,3 +142,45 @@ def test_dense_rows_of_unfinished_systems_are_nan(orc):
     got = ens.options(mapping=DE_MAP_GROUP, group_size=8).solve()
     assert np.array_equal(got.n_dense, want.n_dense)
     assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
+
+
+@pytest.mark.parametrize("method", ["dop853", "rk4", "euler"])
+@pytest.mark.parametrize("dense", [False, True])
+def test_other_methods_bit_exact(orc, method, dense):
+    # DOP853, RK4 and Euler on spring chains run one lane per system (whatever
+    # the mapping asked for) and equal the oracle bit for bit, with and
+    # without t_eval output, under automatic and given initial steps.
+    sysm, y0 = deamd.gen_spring(600, seed=31)
+    m = {"dop853": lambda: ExplicitRungeKutta.dop853().rtol(1e-7).atol(1e-9).max_steps(200000),
+         "rk4": lambda: ExplicitRungeKutta.rk4(0.01),
+         "euler": lambda: ExplicitRungeKutta.euler(0.002).max_steps(100000)}[method]()
+    ens = IVP.ode(sysm, 0.0, 3.0, y0).method(m)
+    if dense:
+        ens = ens.t_eval(np.linspace(-0.5, 3.5, 41))   # points before t0 and past tf
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.status == DE_STATUS_COMPLETE).all()
+    for mapping in (DE_MAP_AUTO, DE_MAP_LANE):
+        got = ens.options(mapping=mapping).solve()
+        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+            assert np.array_equal(getattr(got, f), getattr(want, f)), (f, mapping)
+        if dense:
+            assert np.array_equal(got.n_dense, want.n_dense)
+            assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
+            assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
+
+
+def test_dop853_spring_errors_and_stiffness(orc):
+    # Status paths of the one-lane DOP853 solver: max_steps, a step size that
+    # underflows h_min's bound (BadInput), per-trajectory tolerances.
+    sysm, y0 = deamd.gen_spring(64, seed=32)
+    n = len(sysm.sizes)
+    for mk in (lambda e: e.method(ExplicitRungeKutta.dop853().max_steps(3)),
+               lambda e: e.method(ExplicitRungeKutta.dop853().h0(-0.1)),
+               lambda e: e.method(ExplicitRungeKutta.dop853().h_max(0.05))):
+        ens = mk(IVP.ode(sysm, 0.0, 2.0, y0))
+        ens.rtol_traj = np.geomspace(1e-9, 1e-5, n)
+        ens.atol_traj = np.geomspace(1e-11, 1e-7, n)
+        want = orc.solve_ode(ens, threads=THREADS)
+        got = ens.solve()
+        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+            assert np.array_equal(getattr(got, f), getattr(want, f)), f
