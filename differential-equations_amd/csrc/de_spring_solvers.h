@@ -64,18 +64,26 @@ struct VecDopri5 {
     }
   }
 
-  DE_HD void emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
+  // Returns false when a point needs the interpolant outside [t_prev, t_curr]
+  // (every interior point of a backward step, ordinary.rs:303-309), where the
+  // reference's TEvalSolout unwrap panics: the solve then ends OutOfBounds.
+  DE_HD bool emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
                   double hp, const double* y_old, const double* c1, const double* c2,
                   const double* c3) {
     const OdeArgs& a = A->o;
     int32_t idx = next_eval;
     double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
+    bool ok = true;
     while (idx < a.n_eval) {
       double te = a.t_eval[idx];
       bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
                                 : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
       if (in_range) {
         double* out = dbase + (int64_t)idx * n;
+        if (have_cont && te != t_curr && (te < t_prev || te > t_curr)) {
+          ok = false;
+          break;
+        }
         if (te == t_curr || !have_cont) {
           for (int c = 0; c < n; ++c) out[c] = ycur[c];
         } else {
@@ -99,6 +107,7 @@ struct VecDopri5 {
       }
     }
     next_eval = idx;
+    return ok;
   }
 
   DE_HD bool init(const SpringArgs* args, int64_t sys) {
@@ -280,6 +289,7 @@ struct VecDopri5 {
           if (nonstiff == 6) stiff = 0;
         }
       }
+      bool emitted = true;
       if (a.n_eval > 0) {
         // cont1 = ydiff (k[4]), cont2 = bspl (k[5]), cont3 (k[6]).
         for (int c = 0; c < n; ++c) {
@@ -290,7 +300,7 @@ struct VecDopri5 {
           k[5][c] = bspl;
           k[6][c] = c3;
         }
-        emit(t_new, t, ynew, true, h, y, k[4], k[5], k[6]);
+        emitted = emit(t_new, t, ynew, true, h, y, k[4], k[5], k[6]);
       }
       h_prev = h;
       t = t_new;
@@ -305,6 +315,10 @@ struct VecDopri5 {
       acc += 1;
       ev += 7;
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
+      if (!emitted) {
+        finish(kOutOfBounds);
+        return true;
+      }
       if (fabs(tf - t) <= kTenEps) {
         finish(kComplete);
         return true;
@@ -348,14 +362,18 @@ struct VecCommon {
     }
   }
 
-  // interp(te, out) writes the state at te in [t_prev, t_curr]; with
-  // have_cont false only the step's end is written (the start of a solve).
+  // interp(te, out) writes the state at te and returns false outside the
+  // step (every interior point of a backward step), where the reference's
+  // TEvalSolout unwrap panics: emit then returns false and the solve ends
+  // OutOfBounds.  With have_cont false only the step's end is written (the
+  // start of a solve).
   template <class Interp>
-  DE_HD void emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
+  DE_HD bool emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
                   Interp interp) {
     const OdeArgs& a = A->o;
     int32_t idx = next_eval;
     double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
+    bool ok = true;
     while (idx < a.n_eval) {
       double te = a.t_eval[idx];
       bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
@@ -364,8 +382,9 @@ struct VecCommon {
         double* out = dbase + (int64_t)idx * n;
         if (te == t_curr || !have_cont) {
           for (int c = 0; c < n; ++c) out[c] = ycur[c];
-        } else {
-          interp(te, out);
+        } else if (!interp(te, out)) {
+          ok = false;
+          break;
         }
         pushed += 1;
         idx += 1;
@@ -377,6 +396,7 @@ struct VecCommon {
       }
     }
     next_eval = idx;
+    return ok;
   }
 
   // Common prologue: the system, its state and time span.  w: this system's
@@ -472,7 +492,7 @@ struct VecFixed : VecCommon {
     h = h0;
     diff(t, y, k0);
     ev = 1;
-    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) {});
+    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) { return true; });
     return true;
   }
 
@@ -513,10 +533,12 @@ struct VecFixed : VecCommon {
     diff(t, ynext, dn);
     ev += 1;
     acc += 1;
+    bool emitted = true;
     if (a.n_eval > 0) {
       const double tc = t;
       // cubic_hermite_interpolate on [t_prev, t] (fixed/ordinary.rs:170-177).
-      emit(t, tp, ynext, true, [&](double te, double* out) {
+      emitted = emit(t, tp, ynext, true, [&](double te, double* out) {
+        if (te < tp || te > tc) return false;
         double hh = tc - tp;
         double s = (te - tp) / hh;
         double s2 = s * s;
@@ -528,11 +550,16 @@ struct VecFixed : VecCommon {
         double w10 = h10 * hh, w11 = h11 * hh;
         for (int c = 0; c < n; ++c)
           out[c] = ((h00 * y[c] + w10 * k0[c]) + h01 * ynext[c]) + w11 * dn[c];
+        return true;
       });
     }
     for (int c = 0; c < n; ++c) {
       y[c] = ynext[c];
       k0[c] = dn[c];
+    }
+    if (!emitted) {
+      finish(kOutOfBounds);
+      return true;
     }
     if (fabs(tf - t) <= kTenEps) {
       finish(kComplete);
@@ -619,7 +646,7 @@ struct VecDop853 : VecCommon {
     stiff = nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) {});
+    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) { return true; });
     return true;
   }
 
@@ -696,6 +723,7 @@ struct VecDop853 : VecCommon {
           if (nonstiff == 6) stiff = 0;
         }
       }
+      bool emitted = true;
       if (a.n_eval > 0) {
         // cont[1..3] as DOPRI5's, k[13..15] at c[13..15], cont[r] = h sum_j
         // bi[r][j] k[j] for r = 4..7.
@@ -713,7 +741,8 @@ struct VecDop853 : VecCommon {
         for (int r = 4; r < 8; ++r)
           for (int c = 0; c < n; ++c) cont[r][c] = comb(d8::BI[r], 16, c) * h;
         const double tp = t, hp = h;
-        emit(t_new, tp, ynew, true, [&](double te, double* out) {
+        emitted = emit(t_new, tp, ynew, true, [&](double te, double* out) {
+          if (te < tp || te > t_new) return false;
           const double s = (te - tp) / hp;
           const double s1 = 1.0 - s;
           for (int c = 0; c < n; ++c) {
@@ -726,6 +755,7 @@ struct VecDop853 : VecCommon {
             q = q * s1 + cont[1][c];
             out[c] = y[c] + s * q;
           }
+          return true;
         });
       }
       h_prev = h;
@@ -741,6 +771,10 @@ struct VecDop853 : VecCommon {
       acc += 1;
       ev += 15;
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
+      if (!emitted) {
+        finish(kOutOfBounds);
+        return true;
+      }
       if (fabs(tf - t) <= kTenEps) {
         finish(kComplete);
         return true;
@@ -833,12 +867,15 @@ struct GroupDopri5 {
     }
   }
 
+  // As VecCommon::emit; interp returns false outside the step.  The time
+  // tests are the same on every lane of the group, so the group agrees.
   template <class Interp>
-  DE_DEV void emit(const SpringArgs& A, double t_curr, double t_prev, const double* xc,
+  DE_DEV bool emit(const SpringArgs& A, double t_curr, double t_prev, const double* xc,
                    const double* vc, Interp&& interp) {
     const OdeArgs& a = A.o;
     int32_t idx = next_eval;
     double* dbase = a.dense + (int64_t)a.n_eval * so;
+    bool ok = true;
     while (idx < a.n_eval) {
       double te = a.t_eval[idx];
       bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
@@ -849,7 +886,10 @@ struct GroupDopri5 {
           write_state(out, xc, vc);
         } else {
           double ox[C], ov[C];
-          interp(te, ox, ov);
+          if (!interp(te, ox, ov)) {
+            ok = false;
+            break;
+          }
           write_state(out, ox, ov);
         }
         pushed += 1;
@@ -864,6 +904,7 @@ struct GroupDopri5 {
       }
     }
     next_eval = idx;
+    return ok;
   }
 
   // Load system `s`; returns false (after writing results) on BadInput.
@@ -955,7 +996,7 @@ struct GroupDopri5 {
     stiff = nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (a.n_eval > 0) emit(A, t, t, x, v, [](double, double*, double*) {});
+    if (a.n_eval > 0) emit(A, t, t, x, v, [](double, double*, double*) { return true; });
     return true;
   }
 
@@ -1094,6 +1135,7 @@ struct GroupDopri5 {
           if (nonstiff == 6) stiff = 0;
         }
       }
+      bool emitted = true;
       if (a.n_eval > 0) {
         double c1x[C], c1v[C], c2x[C], c2v[C], c3x[C], c3v[C];
 #pragma unroll
@@ -1108,7 +1150,8 @@ struct GroupDopri5 {
           c3v[q] = (dv2 + (-h) * dnv[q]) + (-1.0) * bv;
         }
         const double hp = h, tp = t, cont4 = 0.0 * h;
-        emit(A, t_new, t, nx, nv, [&](double te, double* ox, double* ov) {
+        emitted = emit(A, t_new, t, nx, nv, [&](double te, double* ox, double* ov) {
+          if (te < tp || te > t_new) return false;
           double s = (te - tp) / hp;
           double s1 = 1.0 - s;
 #pragma unroll
@@ -1122,6 +1165,7 @@ struct GroupDopri5 {
             av = av * s1 + c1v[q];
             ov[q] = v[q] + s * av;
           }
+          return true;
         });
       }
       h_prev = h;
@@ -1140,6 +1184,10 @@ struct GroupDopri5 {
       acc += 1;
       ev += 7;
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
+      if (!emitted) {
+        finish(A, kOutOfBounds);
+        return true;
+      }
       if (fabs(tf - t) <= kTenEps) {
         finish(A, kComplete);
         return true;

@@ -184,3 +184,28 @@ def test_dop853_spring_errors_and_stiffness(orc):
         got = ens.solve()
         for f in ("status", "accepted", "rejected", "evals", "t", "y"):
             assert np.array_equal(getattr(got, f), getattr(want, f)), f
+
+
+def test_backward_t_eval_ends_out_of_bounds(orc):
+    # A backward solve with t_eval points ends OutOfBounds at the first point
+    # that needs the interpolant (the reference's interpolants reject t outside
+    # [t_prev, t] of a backward step and TEvalSolout unwraps): every mapping
+    # and method stops there with the oracle's status, counts and rows.
+    sysm, y0 = deamd.gen_spring(300, seed=33)
+    pts = np.linspace(0.0, -2.0, 9)
+    for m, mappings in ((ExplicitRungeKutta.dopri5, (DE_MAP_LANE, DE_MAP_GROUP, DE_MAP_AUTO)),
+                        (ExplicitRungeKutta.dop853, (DE_MAP_LANE,)),
+                        (lambda: ExplicitRungeKutta.rk4(-0.01), (DE_MAP_LANE,))):
+        ens = IVP.ode(sysm, 0.0, -2.0, y0).method(m()).t_eval(pts)
+        want = orc.solve_ode(ens, threads=THREADS)
+        assert (want.status == deamd.DE_STATUS_OUT_OF_BOUNDS).all()
+        for mapping in mappings:
+            got = ens.options(mapping=mapping, group_size=16 if mapping == DE_MAP_GROUP else 0).solve()
+            for f in ("status", "accepted", "rejected", "n_dense"):
+                assert np.array_equal(getattr(got, f), getattr(want, f)), (f, mapping)
+            if mapping == DE_MAP_LANE:
+                assert np.array_equal(got.y, want.y) and np.array_equal(got.t, want.t)
+                assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
+                assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
+            else:
+                assert (rel_state_error(sysm, got, want) <= 1e-10).all()
