@@ -126,10 +126,6 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->n > 0 && b->family != DE_FAMILY_SPRING_CHAIN && kFamilyParams[b->family] > 0 &&
       !b->params)
     return fail(DE_ERR_INVALID, "family %d needs params", b->family);
-  if (b->family == DE_FAMILY_SPRING_CHAIN && b->method >= DE_METHOD_RKF45)
-    return fail(DE_ERR_UNSUPPORTED,
-                "spring chains are built for DOPRI5, DOP853, RK4 and Euler (method %d)",
-                b->method);
   if (b->schedule < DE_SCHED_AUTO || b->schedule > DE_SCHED_PERSISTENT_FIFO)
     return fail(DE_ERR_INVALID, "unknown schedule %d", b->schedule);
   if (b->max_steps < 0) return fail(DE_ERR_INVALID, "max_steps < 0");

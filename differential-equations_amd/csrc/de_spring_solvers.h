@@ -16,6 +16,7 @@
 //                 results agree to rounding rather than bit for bit.
 #pragma once
 
+#include "de_erk_tables.h"
 #include "de_lane.h"
 
 namespace de {
@@ -784,6 +785,238 @@ struct VecDop853 : VecCommon {
       rej += 1;
       ev += 11;
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    }
+    return false;
+  }
+};
+
+// ------------------------------------------------------------------ VecErk --
+// The embedded family (adaptive/ordinary.rs:16-292, the same operations as
+// ErkLane in de_erk_lane.h) on one lane per system: stages k[0..I), ys, yh,
+// dn and one vector of per-component masks of the stages whose value is not
+// finite (a skipped zero-coefficient term of such a stage is NaN in the
+// reference; see de_erk_lane.h).  Every stage is tracked, which changes
+// nothing where no zero coefficient meets it.
+template <class Tab>
+struct VecErk : VecCommon {
+  static constexpr int S = Tab::S, I = Tab::I, O = Tab::O;
+  static constexpr bool kBi = Tab::kBi, kFsal = Tab::kFsal;
+  static constexpr int kVectors = I + 4;
+  static constexpr int kAccEvals = (kFsal ? 0 : 1) + (kBi ? I - S : 0);
+  double *k[I], *ys, *yh, *dn;
+  uint32_t* bad;
+  double h_prev, rtol, atol;
+  const double *rtv, *atv;
+  int64_t acc, rej, base_ev;
+  int32_t stiff;
+  bool rejected_last;
+
+  DE_HD double rt(int c) const { return rtv ? rtv[c] : rtol; }
+  DE_HD double at(int c) const { return atv ? atv[c] : atol; }
+  DE_HD void finish(int32_t st) {
+    write(st, acc, rej,
+          st == kBadInput ? 0 : base_ev + (int64_t)(S - 1) * (acc + rej) + (int64_t)kAccEvals * acc);
+  }
+
+  DE_HD static uint32_t zero_mask(const double* row, int nrow) {
+    uint32_t m = 0;
+    for (int j = 0; j < nrow; ++j)
+      if (row[j] == 0.0) m |= 1u << j;
+    return m;
+  }
+  DE_HD static uint32_t zero_bi_rows() {
+    uint32_t m = 0;
+    for (int r = 0; r < I; ++r) {
+      bool z = true;
+      for (int j = 0; j < O; ++j) z = z && Tab::BI[r][j] == 0.0;
+      if (z) m |= 1u << r;
+    }
+    return m;
+  }
+
+  // Stages R .. END-1 from y at (tt, hh): ys = y + sum_{j<R} (a[R][j] hh) k[j]
+  // over the nonzero a in j order, NaN where a skipped term meets a
+  // non-finite stage.
+  DE_HD void stages(int R0, int END, double tt, double hh) {
+    for (int R = R0; R < END; ++R) {
+      for (int c = 0; c < n; ++c) ys[c] = y[c];
+      for (int J = 0; J < R; ++J) {
+        const double aj = Tab::A[R][J];
+        if (aj != 0.0) {
+          const double ah = aj * hh;
+          for (int c = 0; c < n; ++c) ys[c] = ys[c] + ah * k[J][c];
+        }
+      }
+      const uint32_t z = zero_mask(Tab::A[R], R);
+      if (z != 0)
+        for (int c = 0; c < n; ++c) ys[c] = (bad[c] & z) ? as_f64(kNaNBits) : ys[c];
+      diff(tt + Tab::C[R] * hh, ys, k[R]);
+      for (int c = 0; c < n; ++c) bad[c] |= isfinite(k[R][c]) ? 0u : (1u << R);
+    }
+  }
+
+  // y[c] + sum_j (w[j] h) k[j][c] over the nonzero weights (SEL 0: b, 1: bh).
+  DE_HD double weighted(const double* w, int c) const {
+    double v = y[c];
+    for (int J = 0; J < S; ++J)
+      if (w[J] != 0.0) v = v + (w[J] * h) * k[J][c];
+    const uint32_t z = zero_mask(w, S);
+    return (z != 0 && (bad[c] & z)) ? as_f64(kNaNBits) : v;
+  }
+
+  DE_HD bool init(const SpringArgs* args, int64_t sys) {
+    const double t0 = setup(args, sys);
+    const OdeArgs& a = A->o;
+    const int64_t so = A->state_offset[i];
+    double* w = A->work + kVectors * so;
+    for (int v = 0; v < I; ++v) k[v] = w + (int64_t)v * n;
+    ys = w + (int64_t)I * n;
+    yh = w + (int64_t)(I + 1) * n;
+    dn = w + (int64_t)(I + 2) * n;
+    bad = (uint32_t*)(w + (int64_t)(I + 3) * n);
+    rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
+    atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    rtv = a.rtol_comp ? a.rtol_comp + so : nullptr;
+    atv = a.atol_comp ? a.atol_comp + so : nullptr;
+    acc = rej = 0;
+    base_ev = 1;
+    bool ok = dir == 1.0 || dir == -1.0;
+    double h0 = a.h0;
+    if (ok && h0 == 0.0) {
+      // ordinary.rs:16-61: InitialStepSize's two evaluations plus two more.
+      h0 = h_init<O>(t0, k[0], k[1], ys, yh);
+      base_ev = 5;
+    }
+    ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
+    if (!ok) {
+      finish(kBadInput);
+      return false;
+    }
+    h = h0;
+    diff(t, y, k[0]);
+    h_prev = 0.0;
+    stiff = 0;
+    rejected_last = false;
+    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) { return true; });
+    return true;
+  }
+
+  DE_HD bool advance() {
+    const OdeArgs& a = A->o;
+    if ((t + h - tf) * dir > 0.0) {
+      double hn = tf - t;
+      if (fabs(hn) < kTenEps) {
+        finish(kComplete);
+        return true;
+      }
+      h = hn;
+    }
+    if (fabs(h) < fabs(h_prev) * 1e-14) {
+      finish(kStepSize);
+      return true;
+    }
+    if (acc + rej >= a.max_steps) {
+      finish(kMaxSteps);
+      return true;
+    }
+    const uint32_t hbad = isfinite(h) ? 0u : ~0u;
+    for (int c = 0; c < n; ++c) bad[c] = hbad | (isfinite(k[0][c]) ? 0u : 1u);
+    stages(1, S, t, h);
+    double m = 0.0;
+    for (int c = 0; c < n; ++c) {
+      const double hi = weighted(Tab::B, c);
+      const double lo = weighted(Tab::BH, c);
+      yh[c] = hi;
+      const double e = hi + (-1.0) * lo;
+      const double sk = at(c) + rt(c) * fmax(fabs(y[c]), fabs(hi));
+      m = fmax(m, fabs(e / sk));
+    }
+    double scale = a.safety * DE_CTRL_POW(m, -(1.0 / (double)O));
+    scale = fmin(fmax(scale, a.min_scale), a.max_scale);
+    const bool accept = m <= 1.0;
+    const int32_t stiff_next = accept ? (rejected_last ? 0 : stiff) : stiff + 1;
+    const bool stop = !accept && (int64_t)stiff_next >= a.max_rejects;
+    const double t_new = t + h;
+    if (kFsal) {
+      for (int c = 0; c < n; ++c) dn[c] = k[S - 1][c];
+    } else {
+      diff(t_new, yh, dn);
+    }
+    bool emitted = true;
+    if (a.n_eval > 0 && accept && next_eval < a.n_eval) {
+      const double te0 = a.t_eval[next_eval];
+      if (!(dir > 0.0 ? te0 > t_new : te0 < t_new)) {
+        // The dense stages for the interpolant (ordinary.rs:240-277), or the
+        // cubic Hermite interpolant without them.
+        if (kBi) stages(S, I, t, h);
+        bool fin = true;
+        if (kBi) {
+          uint32_t anyb = 0;
+          for (int c = 0; c < n; ++c) anyb |= bad[c];
+          fin = (anyb & zero_bi_rows()) == 0;
+        }
+        const double tp = t, hp = h;
+        emitted = emit(t_new, tp, yh, true, [&](double te, double* out) {
+          if (te < tp || te > t_new) return false;
+          if (kBi) {
+            // Horner over bi[r][0..O) times s, then y_prev + sum_r
+            // (cont_r h_prev) k[r] in r order; all-zero rows add +-0 and are
+            // skipped when their stages, s and h_prev are finite.
+            const double s = (te - tp) / hp;
+            const bool all = !fin || !isfinite(s) || !isfinite(hp);
+            const uint32_t zr = zero_bi_rows();
+            for (int c = 0; c < n; ++c) out[c] = y[c];
+            for (int r = 0; r < I; ++r) {
+              if (((zr >> r) & 1u) && !all) continue;
+              double q = Tab::BI[r][O - 1];
+              for (int j = O - 2; j >= 0; --j) q = q * s + Tab::BI[r][j];
+              q = q * s;
+              const double wq = q * hp;
+              for (int c = 0; c < n; ++c) out[c] = out[c] + wq * k[r][c];
+            }
+          } else {
+            // cubic_hermite_interpolate (src/interpolate.rs:40-60).
+            const double hh = t_new - tp;
+            const double s = (te - tp) / hh;
+            const double s2 = s * s;
+            const double s3 = s2 * s;
+            const double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+            const double h10 = s3 - 2.0 * s2 + s;
+            const double h01 = -2.0 * s3 + 3.0 * s2;
+            const double h11 = s3 - s2;
+            const double w10 = h10 * hh, w11 = h11 * hh;
+            for (int c = 0; c < n; ++c)
+              out[c] = (((0.0 + h00 * y[c]) + w10 * k[0][c]) + h01 * yh[c]) + w11 * dn[c];
+          }
+          return true;
+        });
+      }
+    }
+    if (accept) {
+      h_prev = h;
+      t = t_new;
+      for (int c = 0; c < n; ++c) {
+        y[c] = yh[c];
+        k[0][c] = dn[c];
+      }
+    }
+    scale = accept && rejected_last ? fmin(scale, 1.0) : scale;
+    rejected_last = !accept;
+    stiff = stiff_next;
+    acc += accept ? 1 : 0;
+    rej += (accept || stop) ? 0 : 1;
+    h = constrain_step_size(h * scale, a.h_min, a.h_max);
+    if (stop) {
+      finish(kStiffness);
+      return true;
+    }
+    if (!emitted) {
+      finish(kOutOfBounds);
+      return true;
+    }
+    if (accept && fabs(tf - t) <= kTenEps) {
+      finish(kComplete);
+      return true;
     }
     return false;
   }

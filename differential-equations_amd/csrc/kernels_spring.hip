@@ -3,7 +3,7 @@
 // Mappings (selected by de_ode_batch.mapping):
 //   DE_MAP_LANE   one thread per system, vectors in an HBM workspace (plain
 //                 launch; bit-identical to the CPU restatement).  DOPRI5, and
-//                 the only mapping of DOP853, RK4 and Euler.
+//                 the only mapping of DOP853, RK4, Euler and the embedded family.
 //   DE_MAP_GROUP  G lanes per system with C masses per lane in registers.
 //   DE_MAP_AUTO   systems are bucketed into size classes (N <= 4, 8, ..., 256);
 //                 for each class an autotuner times every (G, C, block) with
@@ -36,13 +36,32 @@ __global__ __launch_bounds__(kSpringBlock) void spring_lane_kernel(SpringArgs A)
   }
 }
 
-// Workspace doubles per state component of the one-lane solver of a method.
-static int lane_vectors(int method) {
+// The one-lane solver of each method: its kernel and its workspace doubles
+// per state component.
+struct LaneSolver {
+  void (*kernel)(SpringArgs);
+  int vectors;
+};
+template <class Solver>
+static LaneSolver lane_solver_of() {
+  return {spring_lane_kernel<Solver>, Solver::kVectors};
+}
+static LaneSolver lane_solver(int method) {
   switch (method) {
-    case DE_METHOD_DOP853: return VecDop853::kVectors;
-    case DE_METHOD_RK4:
-    case DE_METHOD_EULER: return VecFixed<4>::kVectors;
-    default: return 9;  // VecDopri5
+    case DE_METHOD_DOP853: return lane_solver_of<VecDop853>();
+    case DE_METHOD_RK4: return lane_solver_of<VecFixed<4>>();
+    case DE_METHOD_EULER: return lane_solver_of<VecFixed<1>>();
+    case DE_METHOD_RKF45: return lane_solver_of<VecErk<erk_tab::Rkf45>>();
+    case DE_METHOD_CASH_KARP: return lane_solver_of<VecErk<erk_tab::CashKarp>>();
+    case DE_METHOD_RKV655E: return lane_solver_of<VecErk<erk_tab::Rkv655e>>();
+    case DE_METHOD_RKV656E: return lane_solver_of<VecErk<erk_tab::Rkv656e>>();
+    case DE_METHOD_RKV766E: return lane_solver_of<VecErk<erk_tab::Rkv766e>>();
+    case DE_METHOD_RKV767E: return lane_solver_of<VecErk<erk_tab::Rkv767e>>();
+    case DE_METHOD_RKV877E: return lane_solver_of<VecErk<erk_tab::Rkv877e>>();
+    case DE_METHOD_RKV878E: return lane_solver_of<VecErk<erk_tab::Rkv878e>>();
+    case DE_METHOD_RKV988E: return lane_solver_of<VecErk<erk_tab::Rkv988e>>();
+    case DE_METHOD_RKV989E: return lane_solver_of<VecErk<erk_tab::Rkv989e>>();
+    default: return {spring_lane_kernel<VecDopri5>, 9};
   }
 }
 
@@ -208,7 +227,7 @@ int64_t spring_workspace_bytes(const de_ode_batch* b) {
   if (b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5) {
     // The one-lane solver's state-length vectors per system; offsets are
     // device memory here, so the caller's bound uses 512 doubles per system.
-    return (int64_t)lane_vectors(b->method) * 512 * 8 * b->n;
+    return (int64_t)lane_solver(b->method).vectors * 512 * 8 * b->n;
   }
   return (int64_t)sizeof(int32_t) * b->n + (int64_t)sizeof(unsigned long long) * (kNumCaps + 1);
 }
@@ -240,7 +259,8 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
   // One lane per system: DE_MAP_LANE, and every method but DOPRI5 (the group
   // mapping is built for DOPRI5).
   if (b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5) {
-    const int64_t need = (int64_t)lane_vectors(b->method) * state_len;
+    const LaneSolver ls = lane_solver(b->method);
+    const int64_t need = (int64_t)ls.vectors * state_len;
     if (g_work_len < need) {
       if (g_work) (void)hipFree(g_work);
       g_work = nullptr;
@@ -250,11 +270,7 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     }
     base.work = g_work;
     int64_t grid = (n + kSpringBlock - 1) / kSpringBlock;
-    void (*k)(SpringArgs) = spring_lane_kernel<VecDopri5>;
-    if (b->method == DE_METHOD_DOP853) k = spring_lane_kernel<VecDop853>;
-    else if (b->method == DE_METHOD_RK4) k = spring_lane_kernel<VecFixed<4>>;
-    else if (b->method == DE_METHOD_EULER) k = spring_lane_kernel<VecFixed<1>>;
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(kSpringBlock), 0, s, base);
+    hipLaunchKernelGGL(ls.kernel, dim3((unsigned)grid), dim3(kSpringBlock), 0, s, base);
     return hipGetLastError() == hipSuccess ? 0 : -1;
   }
   // Group mapping: upload class lists (one buffer, classes back to back).

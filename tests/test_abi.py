@@ -76,17 +76,6 @@ def test_invalid_descriptors_are_rejected_before_any_device_work(mutate, code):
     assert deamd.lib().de_last_error()
 
 
-def test_spring_chains_refuse_the_embedded_family():
-    # The spring-chain solvers implement DOPRI5, DOP853, RK4 and Euler; the
-    # embedded family is refused rather than silently solved with another method.
-    sysm, y0 = deamd.gen_spring(8, seed=1)
-    for m in (deamd.ExplicitRungeKutta.rkf45(), deamd.ExplicitRungeKutta.rkv878e()):
-        ens = deamd.IVP.ode(sysm, 0.0, 1.0, y0).method(m)
-        b, res, keep = ens.batch()
-        assert deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None) == \
-            _abi.DE_ERR_UNSUPPORTED
-
-
 def _event_case(kind):
     e = deamd.IVP.ode(deamd.Lorenz(10.0, 28.0, 8.0 / 3.0), 0.0, 1.0, [1.0, 1.0, 1.0], n=4)
     return e.event(deamd.Event.component(2, 27.0)) if kind == "event" else e.crossing(0, 0.0)

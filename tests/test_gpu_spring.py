@@ -209,3 +209,29 @@ def test_backward_t_eval_ends_out_of_bounds(orc):
                 assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
             else:
                 assert (rel_state_error(sysm, got, want) <= 1e-10).all()
+
+
+FAMILY = ["rkf45", "cash_karp", "rkv655e", "rkv656e", "rkv766e", "rkv767e", "rkv877e", "rkv878e",
+          "rkv988e", "rkv989e"]
+
+
+@pytest.mark.parametrize("name", FAMILY)
+def test_embedded_family_equals_oracle(orc, name):
+    # The embedded family on spring chains (one lane per system) equals the
+    # oracle as IEEE values (a zero's sign may differ), with and without t_eval
+    # output, automatic initial step.
+    sysm, y0 = deamd.gen_spring(200, seed=34)
+    for dense in (False, True):
+        ens = IVP.ode(sysm, 0.0, 2.0, y0).method(
+            getattr(ExplicitRungeKutta, name)().rtol(1e-7).atol(1e-9).max_steps(200000))
+        if dense:
+            ens = ens.t_eval(np.linspace(-0.25, 2.25, 21))
+        want = orc.solve_ode(ens, threads=THREADS)
+        assert (want.status == DE_STATUS_COMPLETE).all()
+        got = ens.solve()
+        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+            assert np.array_equal(getattr(got, f), getattr(want, f)), (f, dense)
+        if dense:
+            assert np.array_equal(got.n_dense, want.n_dense)
+            assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
+            assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
