@@ -9,9 +9,10 @@
 //                 error norm is the reference's sequential sum, so results are
 //                 bit-identical to the CPU restatement.
 //   GroupDopri5   G lanes per system, C masses per lane held in registers.
-//                 Neighbour displacements cross lanes with __shfl_up/down and
-//                 the norms are fixed butterfly reductions, identical on every
-//                 lane of the group.  The butterfly sums in a different order
+//                 Neighbour displacements cross lanes by DPP wavefront shifts
+//                 and the norms are fixed butterfly reductions (DPP within a
+//                 row, permutes across rows), identical on every lane of the
+//                 group.  The butterfly sums in a different order
 //                 than the reference's sequential loop (traits.rs:398-409), so
 //                 results agree to rounding rather than bit for bit.
 #pragma once
@@ -1027,6 +1028,24 @@ struct VecErk : VecCommon {
 // G lanes per system (G a power of two <= 64), C masses per lane:
 // lane l of the group owns masses j = l*C + q, q < C, and keeps x_j, v_j of
 // every vector in registers.
+// A double moved across lanes by one DPP operation on each half (dpp_ctrl
+// CTRL; a lane whose source is out of range or inactive reads 0).  Lane
+// exchanges within a row and wavefront shifts are VALU operations, without
+// the LDS crossbar round trip of a permute.
+template <int CTRL>
+DE_DEV double dpp_double(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+constexpr int kDppQuadSwap1 = 0xB1;   // quad_perm [1,0,3,2]: lane i reads i ^ 1
+constexpr int kDppQuadSwap2 = 0x4E;   // quad_perm [2,3,0,1]: lane i reads i ^ 2
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror: lane i reads 7 - i in its 8
+constexpr int kDppMirror = 0x140;     // row_mirror: lane i reads 15 - i in its 16
+constexpr int kDppWaveShr1 = 0x138;   // wave_shr:1: lane i reads lane i - 1
+constexpr int kDppWaveShl1 = 0x130;   // wave_shl:1: lane i reads lane i + 1
+
 template <int G, int C>
 struct GroupDopri5 {
   double x[C], v[C];      // y
@@ -1044,16 +1063,26 @@ struct GroupDopri5 {
   DE_DEV double rt(const OdeArgs& a, int c) const { return a.rtol_comp ? a.rtol_comp[so + c] : rtol; }
   DE_DEV double at(const OdeArgs& a, int c) const { return a.atol_comp ? a.atol_comp[so + c] : atol; }
 
+  // Sum over the group's G lanes, identical on every lane: each level adds
+  // the value of a partner lane under an involution (xor 1, xor 2, the mirror
+  // within 8 and within 16, then xor 16 and xor 32), so a lane and its partner
+  // add the same two operands.
   DE_DEV double gsum(double s) const {
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) s += __shfl_xor(s, m, G);
+    if constexpr (G >= 2) s += dpp_double<kDppQuadSwap1>(s);
+    if constexpr (G >= 4) s += dpp_double<kDppQuadSwap2>(s);
+    if constexpr (G >= 8) s += dpp_double<kDppHalfMirror>(s);
+    if constexpr (G >= 16) s += dpp_double<kDppMirror>(s);
+    if constexpr (G >= 32) s += __shfl_xor(s, 16, G);
+    if constexpr (G >= 64) s += __shfl_xor(s, 32, G);
     return s;
   }
 
   // (dx, dv) = f(x, v): dx_j = v_j, dv_j = spring_accel(...), zero beyond N.
+  // The neighbours' displacements come by a wavefront shift; a group's first
+  // and last lanes take the chain's fixed ends instead.
   DE_DEV void rhs(const double* xs, const double* vs, double* dx, double* dv) const {
-    double left = __shfl_up(xs[C - 1], 1, G);
-    double right = __shfl_down(xs[0], 1, G);
+    double left = dpp_double<kDppWaveShr1>(xs[C - 1]);
+    double right = dpp_double<kDppWaveShl1>(xs[0]);
     if (l == 0) left = 0.0;
     if (l == G - 1) right = 0.0;
 #pragma unroll
