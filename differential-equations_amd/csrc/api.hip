@@ -135,8 +135,6 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
   if (b->points < DE_POINTS_NONE || b->points > DE_POINTS_EVEN)
     return fail(DE_ERR_INVALID, "unknown point output %d", b->points);
   if (b->points != DE_POINTS_NONE) {
-    if (b->family == DE_FAMILY_SPRING_CHAIN)
-      return fail(DE_ERR_UNSUPPORTED, "point outputs are built for fixed-dimension families");
     if (b->event_solout != DE_SOLOUT_DEFAULT)
       return fail(DE_ERR_UNSUPPORTED, "a point output cannot be wrapped by an event solout");
     if (b->n_eval > 0)
@@ -284,6 +282,7 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
     }
     int r = de::parity::launch_spring(b, host_sizes, a, sched, s);
     if (r == 1) return fail(DE_ERR_UNSUPPORTED, "spring-chain mapping not supported");
+    if (r == 2) return fail(DE_ERR_INVALID, "points_offset is negative or decreases");
     if (r < 0) return r == -4 ? fail(DE_ERR_NOMEM, "spring workspace allocation failed")
                               : hip_fail(hipGetLastError(), "spring launch");
     return DE_OK;
@@ -595,9 +594,20 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   const bool pts = b->points != DE_POINTS_NONE;
   const int64_t pts_cap = pts ? o->points_offset[n] : 0;
   if (pts && pts_cap < 0) return fail(DE_ERR_INVALID, "points_offset[n] < 0");
+  // points_y: the slots times the state length (per system for spring chains).
+  int64_t pts_ylen = pts_cap * b->dim;
+  if (pts && ragged) {
+    pts_ylen = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t slots = o->points_offset[i + 1] - o->points_offset[i];
+      if (o->points_offset[i] < 0 || slots < 0)
+        return fail(DE_ERR_INVALID, "points_offset is negative or decreases");
+      pts_ylen += slots * 2 * (int64_t)b->sizes[i];
+    }
+  }
   dr.points_offset = st.in(o->points_offset, pts ? n + 1 : 0);
   dr.points_t = st.out(o->points_t, pts_cap);
-  dr.points_y = st.out(o->points_y, pts_cap * b->dim);
+  dr.points_y = st.out(o->points_y, pts_ylen);
   dr.n_points = st.out(o->n_points, pts ? n : 0);
   if (st.err != hipSuccess) return hip_fail(st.err, "staging inputs");
   rc = run_ode_device(&db, ragged ? b->sizes : nullptr, make_args(&db, &dr), s);
@@ -616,7 +626,7 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   if (dr.event_y) st.back(o->event_y, dr.event_y, ev_cap * b->dim);
   if (dr.n_events) st.back(o->n_events, dr.n_events, n);
   if (dr.points_t) st.back(o->points_t, dr.points_t, pts_cap);
-  if (dr.points_y) st.back(o->points_y, dr.points_y, pts_cap * b->dim);
+  if (dr.points_y) st.back(o->points_y, dr.points_y, pts_ylen);
   if (dr.n_points) st.back(o->n_points, dr.n_points, n);
   if (st.err != hipSuccess) return hip_fail(st.err, "copying results");
   return DE_OK;

@@ -1,20 +1,23 @@
-// DOPRI5 solvers for ragged damped spring-mass chains (device code).
+// Solvers for ragged damped spring-mass chains (device code).
 //
-// Same algorithm and operation order as the lane solvers of de_lane.h
-// (solve_ode src/ode/solve_ivp.rs:116-277, DOPRI5 ordinary.rs:16-336, h_init
-// h_init.rs:45-135, TEvalSolout t_eval.rs:83-134), for a state of runtime
-// length n = 2N.  Two mappings:
+// Same algorithm and operation order as the lane solvers of de_lane.h and
+// de_erk_lane.h (solve_ode src/ode/solve_ivp.rs:116-277, DOPRI5
+// ordinary.rs:16-336, h_init h_init.rs:45-135, the solouts of src/solout/),
+// for a state of runtime length n = 2N.  Two mappings:
 //
-//   VecDopri5     one lane per system; vectors live in an HBM workspace.  The
-//                 error norm is the reference's sequential sum, so results are
-//                 bit-identical to the CPU restatement.
+//   Vec*          one lane per system (VecDopri5, VecDop853, VecFixed for RK4
+//                 and Euler, VecErk for the embedded family); vectors live in
+//                 an HBM workspace.  The error norm is the reference's
+//                 sequential sum, so results are bit-identical to the CPU
+//                 restatement.  The t_eval output and the point outputs.
 //   GroupDopri5   G lanes per system, C masses per lane held in registers.
 //                 Neighbour displacements cross lanes by DPP wavefront shifts
 //                 and the norms are fixed butterfly reductions (DPP within a
 //                 row, permutes across rows), identical on every lane of the
 //                 group.  The butterfly sums in a different order
 //                 than the reference's sequential loop (traits.rs:398-409), so
-//                 results agree to rounding rather than bit for bit.
+//                 results agree to rounding rather than bit for bit.  DOPRI5
+//                 with or without the t_eval output.
 #pragma once
 
 #include "de_erk_tables.h"
@@ -27,31 +30,38 @@ struct SpringArgs {
   const int32_t* sizes;        // N per system
   const int64_t* state_offset; // n + 1
   const int64_t* param_offset; // n + 1
-  double* work;                // VecDopri5: 9 vectors per system at 9*state_offset
+  double* work;                // one-lane solvers: kVectors vectors per system at
+                               // kVectors * state_offset
+  const int64_t* pts_ypos;     // point output: start of system i's states in pts_y
   const int32_t* list;         // GroupDopri5: system indices of this size class
   int64_t list_len;
 };
 
-// ------------------------------------------------------------- VecDopri5 --
-struct VecDopri5 {
+// Shared pieces of the one-lane-per-system solvers: TEvalSolout over a
+// system's dense rows (t_eval.rs:83-134) and the variable-count point outputs
+// (DefaultSolout default.rs:71-91, DenseSolout dense.rs:81-111, EvenSolout
+// even.rs:83-190, the same steps as points_emit in de_lane.h) with an
+// interpolation callback, the write-back, and the automatic initial step
+// (h_init.rs:45-135) of order ORDER.
+//
+// interp(te, out) returns false when te is outside the step (every interior
+// point of a backward step), where the reference's solouts unwrap the
+// interpolation and panic: the solve then ends OutOfBounds.  Otherwise it
+// writes the state at te to out, unless out is null (a point past the
+// system's slots, counted and not kept).
+struct VecCommon {
   const SpringArgs* A;
   int64_t i;
   int nm, n;  // masses, state length
   const double* p;
-  double* y;   // y_final slot doubles as the working state
-  double *k[7], *ys, *ynew;
-  double t, h, h_prev, tf, dir, rtol, atol;
-  const double *rtv, *atv;  // per-component tolerances of this system, or null
-  int64_t acc, rej, ev, steps;
-
-  DE_HD double rt(int c) const { return rtv ? rtv[c] : rtol; }
-  DE_HD double at(int c) const { return atv ? atv[c] : atol; }
-  int32_t stiff, nonstiff, countdown, next_eval, pushed;
-  bool rejected_last;
+  double* y;  // y_final slot doubles as the working state
+  double t, h, tf, dir;
+  double last;  // EvenSolout::last_output_t
+  int32_t next_eval, pushed;
 
   DE_HD void diff(double, const double* yy, double* dy) const { spring_chain_diff(nm, p, yy, dy); }
 
-  DE_HD void finish(int32_t st) {
+  DE_HD void write(int32_t st, int64_t acc, int64_t rej, int64_t ev) {
     const OdeArgs& a = A->o;
     a.t_final[i] = t;
     a.accepted[i] = acc;
@@ -59,6 +69,7 @@ struct VecDopri5 {
     a.evals[i] = ev;
     a.status[i] = st;
     if (a.n_dense) a.n_dense[i] = pushed;
+    if (a.pts_mode != kPointsNone && a.n_pts) a.n_pts[i] = pushed;
     if (a.n_eval > 0) {  // points not reached (all of them on BadInput)
       double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
       for (int64_t e = (int64_t)next_eval * n; e < (int64_t)a.n_eval * n; ++e)
@@ -66,12 +77,105 @@ struct VecDopri5 {
     }
   }
 
-  // Returns false when a point needs the interpolant outside [t_prev, t_curr]
-  // (every interior point of a backward step, ordinary.rs:303-309), where the
-  // reference's TEvalSolout unwrap panics: the solve then ends OutOfBounds.
+  DE_HD bool points_on() const { return A->o.pts_mode != kPointsNone; }
+  // An accepted step writes output: t_eval rows or a point output.
+  DE_HD bool output_on() const { return A->o.n_eval > 0 || points_on(); }
+
+  // The state slot of the next point, or null past this system's slots.
+  DE_HD double* pts_dst() const {
+    const OdeArgs& a = A->o;
+    return a.pts_offset[i] + pushed < a.pts_offset[i + 1]
+               ? a.pts_y + A->pts_ypos[i] + (int64_t)pushed * n
+               : nullptr;
+  }
+  // Counts the next point and keeps its time while the slots last.
+  DE_HD void pts_commit(double tt) {
+    const OdeArgs& a = A->o;
+    const int64_t s = a.pts_offset[i] + pushed;
+    if (s < a.pts_offset[i + 1]) a.pts_t[s] = tt;
+    pushed += 1;
+  }
+  DE_HD void pts_push(double tt, const double* yy) {
+    double* d = pts_dst();
+    if (d)
+      for (int c = 0; c < n; ++c) d[c] = yy[c];
+    pts_commit(tt);
+  }
+  template <class Interp>
+  DE_HD bool pts_interp(double ti, Interp& interp) {
+    if (!interp(ti, pts_dst())) return false;
+    pts_commit(ti);
+    return true;
+  }
+
+  // The solout call after init (t_curr = t_prev = t0): TEvalSolout writes the
+  // points equal to t0, every point output pushes (t0, y0).
+  DE_HD void output_start() {
+    const OdeArgs& a = A->o;
+    if (points_on()) {
+      pts_push(t, y);
+      last = t;
+    } else if (a.n_eval > 0) {
+      emit(t, t, y, false, [](double, double*) { return true; });
+    }
+  }
+
+  // The solout call of an accepted step [t_prev, t_curr] ending in y_curr.
+  template <class Interp>
+  DE_HD bool output(double t_curr, double t_prev, const double* y_curr, Interp interp) {
+    return points_on() ? points_step(t_curr, t_prev, y_curr, interp)
+                       : emit(t_curr, t_prev, y_curr, true, interp);
+  }
+
+  template <class Interp>
+  DE_HD bool points_step(double t_curr, double t_prev, const double* y_curr, Interp& interp) {
+    const OdeArgs& a = A->o;
+    if (a.pts_mode == kPointsSteps) {
+      pts_push(t_curr, y_curr);
+      return true;
+    }
+    if (a.pts_mode == kPointsDense) {
+      if (t_prev != t_curr) {
+        for (int32_t k = 1; k < a.pts_n; ++k) {
+          const double h_old = t_curr - t_prev;
+          const double ti = t_prev + (double)k * h_old / (double)a.pts_n;
+          if (!pts_interp(ti, interp)) return false;
+        }
+      }
+      pts_push(t_curr, y_curr);
+      return true;
+    }
+    const double dt = a.pts_dt;
+    const double tol = fabs(dt) * 1e-12 + 2.220446049250313080847e-16 * 10.0;
+    double ti = last + dt * dir;
+    while ((dir > 0.0 && ti <= t_curr) || (dir < 0.0 && ti >= t_curr)) {
+      if ((dir > 0.0 && ti >= t_prev && ti <= t_curr) ||
+          (dir < 0.0 && ti <= t_prev && ti >= t_curr)) {
+        if (!(fabs(ti - last) <= tol)) {
+          if (!pts_interp(ti, interp)) return false;
+          last = ti;
+        }
+      }
+      ti += dt * dir;
+    }
+    if (t_curr == tf) {
+      if (fabs(last - tf) <= tol) {
+        pushed -= 1;  // Solution::pop, then the exact final point
+        pts_push(tf, y_curr);
+        last = tf;
+      } else if (last != tf) {
+        pts_push(tf, y_curr);
+        last = tf;
+      }
+    }
+    return true;
+  }
+
+  // TEvalSolout: with have_cont false only the step's end is written (the
+  // start of a solve).
+  template <class Interp>
   DE_HD bool emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
-                  double hp, const double* y_old, const double* c1, const double* c2,
-                  const double* c3) {
+                  Interp interp) {
     const OdeArgs& a = A->o;
     int32_t idx = next_eval;
     double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
@@ -82,22 +186,11 @@ struct VecDopri5 {
                                 : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
       if (in_range) {
         double* out = dbase + (int64_t)idx * n;
-        if (have_cont && te != t_curr && (te < t_prev || te > t_curr)) {
-          ok = false;
-          break;
-        }
         if (te == t_curr || !have_cont) {
           for (int c = 0; c < n; ++c) out[c] = ycur[c];
-        } else {
-          double s = (te - t_prev) / hp;
-          double s1 = 1.0 - s;
-          double cont4 = 0.0 * hp;
-          for (int c = 0; c < n; ++c) {
-            double acc_ = cont4 * s1 + c3[c];
-            acc_ = acc_ * s + c2[c];
-            acc_ = acc_ * s1 + c1[c];
-            out[c] = y_old[c] + s * acc_;
-          }
+        } else if (!interp(te, out)) {
+          ok = false;
+          break;
         }
         pushed += 1;
         idx += 1;
@@ -112,7 +205,9 @@ struct VecDopri5 {
     return ok;
   }
 
-  DE_HD bool init(const SpringArgs* args, int64_t sys) {
+  // Common prologue: the system, its state and time span.  w: this system's
+  // workspace.
+  DE_HD double setup(const SpringArgs* args, int64_t sys) {
     A = args;
     i = sys;
     const OdeArgs& a = A->o;
@@ -121,55 +216,86 @@ struct VecDopri5 {
     p = a.params + A->param_offset[i];
     const int64_t so = A->state_offset[i];
     y = a.y_final + so;
-    double* w = A->work + 9 * so;
+    const double* y0 = a.y0 + so;
+    for (int c = 0; c < n; ++c) y[c] = y0[c];
+    const double t0 = a.t0 ? a.t0[i] : a.t0s;
+    tf = a.tf ? a.tf[i] : a.tfs;
+    t = t0;
+    pushed = next_eval = 0;
+    dir = signum(tf - t0);
+    return t0;
+  }
+
+  // h_init with f0, f1, sk, y1 in workspace vectors.
+  template <int ORDER>
+  DE_HD double h_init(double t0, double* f0, double* f1, double* sk, double* y1) const {
+    const OdeArgs& a = A->o;
+    const int64_t so = A->state_offset[i];
+    const double rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
+    const double atol = a.atol_traj ? a.atol_traj[i] : a.atol;
+    const double* rtv = a.rtol_comp ? a.rtol_comp + so : nullptr;
+    const double* atv = a.atol_comp ? a.atol_comp + so : nullptr;
+    diff(t0, y, f0);
+    double dnf = 0.0, dny = 0.0;
+    for (int c = 0; c < n; ++c) {
+      sk[c] = (atv ? atv[c] : atol) + (rtv ? rtv[c] : rtol) * fabs(y[c]);
+      double u = f0[c] / sk[c];
+      dnf += u * u;
+      double w2 = y[c] / sk[c];
+      dny += w2 * w2;
+    }
+    double hh = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+    hh = fmin(hh, a.h_max);
+    hh *= dir;
+    for (int c = 0; c < n; ++c) y1[c] = y[c] + hh * f0[c];
+    diff(t0 + hh, y1, f1);
+    double der2 = 0.0;
+    for (int c = 0; c < n; ++c) {
+      double d = (f1[c] - f0[c]) / sk[c];
+      der2 += d * d;
+    }
+    der2 = sqrt(der2) / fabs(hh);
+    double der12 = fmax(sqrt(dnf), der2);
+    double h1 = der12 <= 1.0e-15 ? fabs(hh) * 1.0e-3
+                                 : DE_CTRL_POW(0.01 / der12, 1.0 / (double)ORDER);
+    double interval = fabs(tf - t0);
+    hh = fmin(fmax(fmin(fmin(fabs(hh) * 100.0, h1), a.h_max), a.h_min), interval);
+    return hh * dir;
+  }
+};
+
+// ------------------------------------------------------------- VecDopri5 --
+// Workspace: k[0..7), ys, ynew.
+struct VecDopri5 : VecCommon {
+  static constexpr int kVectors = 9;
+  double *k[7], *ys, *ynew;
+  double h_prev, rtol, atol;
+  const double *rtv, *atv;  // per-component tolerances of this system, or null
+  int64_t acc, rej, ev, steps;
+  int32_t stiff, nonstiff, countdown;
+  bool rejected_last;
+
+  DE_HD double rt(int c) const { return rtv ? rtv[c] : rtol; }
+  DE_HD double at(int c) const { return atv ? atv[c] : atol; }
+  DE_HD void finish(int32_t st) { write(st, acc, rej, ev); }
+
+  DE_HD bool init(const SpringArgs* args, int64_t sys) {
+    const double t0 = setup(args, sys);
+    const OdeArgs& a = A->o;
+    const int64_t so = A->state_offset[i];
+    double* w = A->work + kVectors * so;
     for (int v = 0; v < 7; ++v) k[v] = w + (int64_t)v * n;
     ys = w + 7 * (int64_t)n;
     ynew = w + 8 * (int64_t)n;
-    const double* y0 = a.y0 + so;
-    for (int c = 0; c < n; ++c) y[c] = y0[c];
-    double t0 = a.t0 ? a.t0[i] : a.t0s;
-    tf = a.tf ? a.tf[i] : a.tfs;
     rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
     atol = a.atol_traj ? a.atol_traj[i] : a.atol;
     rtv = a.rtol_comp ? a.rtol_comp + so : nullptr;
     atv = a.atol_comp ? a.atol_comp + so : nullptr;
-    t = t0;
     acc = rej = ev = steps = 0;
-    pushed = next_eval = 0;
-    dir = signum(tf - t0);
     bool ok = dir == 1.0 || dir == -1.0;
     double h0 = a.h0;
     if (ok && h0 == 0.0) {
-      // h_init with f0 in k[0], f1 in k[1], sk in ys, y1 in ynew.
-      double* f0 = k[0];
-      double* f1 = k[1];
-      double* sk = ys;
-      double* y1 = ynew;
-      diff(t0, y, f0);
-      double dnf = 0.0, dny = 0.0;
-      for (int c = 0; c < n; ++c) {
-        sk[c] = at(c) + rt(c) * fabs(y[c]);
-        double u = f0[c] / sk[c];
-        dnf += u * u;
-        double w2 = y[c] / sk[c];
-        dny += w2 * w2;
-      }
-      double hh = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
-      hh = fmin(hh, a.h_max);
-      hh *= dir;
-      for (int c = 0; c < n; ++c) y1[c] = y[c] + hh * f0[c];
-      diff(t0 + hh, y1, f1);
-      double der2 = 0.0;
-      for (int c = 0; c < n; ++c) {
-        double d = (f1[c] - f0[c]) / sk[c];
-        der2 += d * d;
-      }
-      der2 = sqrt(der2) / fabs(hh);
-      double der12 = fmax(sqrt(dnf), der2);
-      double h1 = der12 <= 1.0e-15 ? fabs(hh) * 1.0e-3 : DE_CTRL_POW(0.01 / der12, 1.0 / 5.0);
-      double interval = fabs(tf - t0);
-      hh = fmin(fmax(fmin(fmin(fabs(hh) * 100.0, h1), a.h_max), a.h_min), interval);
-      h0 = hh * dir;
+      h0 = h_init<5>(t0, k[0], k[1], ys, ynew);
       ev += 2;
     }
     ok = ok && validate_step_size(h0, a.h_min, a.h_max, t0, tf);
@@ -185,7 +311,7 @@ struct VecDopri5 {
     stiff = nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (a.n_eval > 0) emit(t, t, y, false, 0.0, nullptr, nullptr, nullptr, nullptr);
+    output_start();
     return true;
   }
 
@@ -292,7 +418,9 @@ struct VecDopri5 {
         }
       }
       bool emitted = true;
-      if (a.n_eval > 0) {
+      if (a.pts_mode == kPointsSteps) {
+        emitted = output(t_new, t, ynew, [](double, double*) { return true; });
+      } else if (output_on()) {
         // cont1 = ydiff (k[4]), cont2 = bspl (k[5]), cont3 (k[6]).
         for (int c = 0; c < n; ++c) {
           double ydiff = ynew[c] + (-1.0) * y[c];
@@ -302,7 +430,23 @@ struct VecDopri5 {
           k[5][c] = bspl;
           k[6][c] = c3;
         }
-        emitted = emit(t_new, t, ynew, true, h, y, k[4], k[5], k[6]);
+        const double tp = t, hp = h;
+        const double *c1 = k[4], *c2 = k[5], *c3 = k[6];
+        emitted = output(t_new, tp, ynew, [&](double te, double* out) {
+          // ordinary.rs:303-333: factors s1, s, s1 over cont[4] = 0 .. cont[1].
+          if (te < tp || te > t_new) return false;
+          if (!out) return true;
+          const double s = (te - tp) / hp;
+          const double s1 = 1.0 - s;
+          const double cont4 = 0.0 * hp;
+          for (int c = 0; c < n; ++c) {
+            double acc_ = cont4 * s1 + c3[c];
+            acc_ = acc_ * s + c2[c];
+            acc_ = acc_ * s1 + c1[c];
+            out[c] = y[c] + s * acc_;
+          }
+          return true;
+        });
       }
       h_prev = h;
       t = t_new;
@@ -332,131 +476,6 @@ struct VecDopri5 {
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
     }
     return false;
-  }
-};
-
-// Shared pieces of the one-lane-per-system solvers below: TEvalSolout over a
-// system's dense rows (t_eval.rs:83-134) with an interpolation callback, the
-// write-back, and the automatic initial step (h_init.rs:45-135) of order ORDER.
-struct VecCommon {
-  const SpringArgs* A;
-  int64_t i;
-  int nm, n;  // masses, state length
-  const double* p;
-  double* y;  // y_final slot doubles as the working state
-  double t, h, tf, dir;
-  int32_t next_eval, pushed;
-
-  DE_HD void diff(double, const double* yy, double* dy) const { spring_chain_diff(nm, p, yy, dy); }
-
-  DE_HD void write(int32_t st, int64_t acc, int64_t rej, int64_t ev) {
-    const OdeArgs& a = A->o;
-    a.t_final[i] = t;
-    a.accepted[i] = acc;
-    a.rejected[i] = rej;
-    a.evals[i] = ev;
-    a.status[i] = st;
-    if (a.n_dense) a.n_dense[i] = pushed;
-    if (a.n_eval > 0) {  // points not reached (all of them on BadInput)
-      double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
-      for (int64_t e = (int64_t)next_eval * n; e < (int64_t)a.n_eval * n; ++e)
-        dbase[e] = as_f64(kNaNBits);
-    }
-  }
-
-  // interp(te, out) writes the state at te and returns false outside the
-  // step (every interior point of a backward step), where the reference's
-  // TEvalSolout unwrap panics: emit then returns false and the solve ends
-  // OutOfBounds.  With have_cont false only the step's end is written (the
-  // start of a solve).
-  template <class Interp>
-  DE_HD bool emit(double t_curr, double t_prev, const double* ycur, bool have_cont,
-                  Interp interp) {
-    const OdeArgs& a = A->o;
-    int32_t idx = next_eval;
-    double* dbase = a.dense + (int64_t)a.n_eval * A->state_offset[i];
-    bool ok = true;
-    while (idx < a.n_eval) {
-      double te = a.t_eval[idx];
-      bool in_range = dir > 0.0 ? ((te == t_prev && idx == 0) || (te > t_prev && te <= t_curr))
-                                : ((te == t_prev && idx == 0) || (te < t_prev && te >= t_curr));
-      if (in_range) {
-        double* out = dbase + (int64_t)idx * n;
-        if (te == t_curr || !have_cont) {
-          for (int c = 0; c < n; ++c) out[c] = ycur[c];
-        } else if (!interp(te, out)) {
-          ok = false;
-          break;
-        }
-        pushed += 1;
-        idx += 1;
-      } else {
-        if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
-        double* out = dbase + (int64_t)idx * n;  // passed over: not produced
-        for (int c = 0; c < n; ++c) out[c] = as_f64(kNaNBits);
-        idx += 1;
-      }
-    }
-    next_eval = idx;
-    return ok;
-  }
-
-  // Common prologue: the system, its state and time span.  w: this system's
-  // workspace.
-  DE_HD double setup(const SpringArgs* args, int64_t sys) {
-    A = args;
-    i = sys;
-    const OdeArgs& a = A->o;
-    nm = A->sizes[i];
-    n = 2 * nm;
-    p = a.params + A->param_offset[i];
-    const int64_t so = A->state_offset[i];
-    y = a.y_final + so;
-    const double* y0 = a.y0 + so;
-    for (int c = 0; c < n; ++c) y[c] = y0[c];
-    const double t0 = a.t0 ? a.t0[i] : a.t0s;
-    tf = a.tf ? a.tf[i] : a.tfs;
-    t = t0;
-    pushed = next_eval = 0;
-    dir = signum(tf - t0);
-    return t0;
-  }
-
-  // h_init with f0, f1, sk, y1 in workspace vectors.
-  template <int ORDER>
-  DE_HD double h_init(double t0, double* f0, double* f1, double* sk, double* y1) const {
-    const OdeArgs& a = A->o;
-    const int64_t so = A->state_offset[i];
-    const double rtol = a.rtol_traj ? a.rtol_traj[i] : a.rtol;
-    const double atol = a.atol_traj ? a.atol_traj[i] : a.atol;
-    const double* rtv = a.rtol_comp ? a.rtol_comp + so : nullptr;
-    const double* atv = a.atol_comp ? a.atol_comp + so : nullptr;
-    diff(t0, y, f0);
-    double dnf = 0.0, dny = 0.0;
-    for (int c = 0; c < n; ++c) {
-      sk[c] = (atv ? atv[c] : atol) + (rtv ? rtv[c] : rtol) * fabs(y[c]);
-      double u = f0[c] / sk[c];
-      dnf += u * u;
-      double w2 = y[c] / sk[c];
-      dny += w2 * w2;
-    }
-    double hh = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
-    hh = fmin(hh, a.h_max);
-    hh *= dir;
-    for (int c = 0; c < n; ++c) y1[c] = y[c] + hh * f0[c];
-    diff(t0 + hh, y1, f1);
-    double der2 = 0.0;
-    for (int c = 0; c < n; ++c) {
-      double d = (f1[c] - f0[c]) / sk[c];
-      der2 += d * d;
-    }
-    der2 = sqrt(der2) / fabs(hh);
-    double der12 = fmax(sqrt(dnf), der2);
-    double h1 = der12 <= 1.0e-15 ? fabs(hh) * 1.0e-3
-                                 : DE_CTRL_POW(0.01 / der12, 1.0 / (double)ORDER);
-    double interval = fabs(tf - t0);
-    hh = fmin(fmax(fmin(fmin(fabs(hh) * 100.0, h1), a.h_max), a.h_min), interval);
-    return hh * dir;
   }
 };
 
@@ -494,7 +513,7 @@ struct VecFixed : VecCommon {
     h = h0;
     diff(t, y, k0);
     ev = 1;
-    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) { return true; });
+    output_start();
     return true;
   }
 
@@ -536,11 +555,12 @@ struct VecFixed : VecCommon {
     ev += 1;
     acc += 1;
     bool emitted = true;
-    if (a.n_eval > 0) {
+    if (output_on()) {
       const double tc = t;
       // cubic_hermite_interpolate on [t_prev, t] (fixed/ordinary.rs:170-177).
-      emitted = emit(t, tp, ynext, true, [&](double te, double* out) {
+      emitted = output(t, tp, ynext, [&](double te, double* out) {
         if (te < tp || te > tc) return false;
+        if (!out) return true;
         double hh = tc - tp;
         double s = (te - tp) / hh;
         double s2 = s * s;
@@ -648,7 +668,7 @@ struct VecDop853 : VecCommon {
     stiff = nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) { return true; });
+    output_start();
     return true;
   }
 
@@ -726,7 +746,9 @@ struct VecDop853 : VecCommon {
         }
       }
       bool emitted = true;
-      if (a.n_eval > 0) {
+      if (a.pts_mode == kPointsSteps) {
+        emitted = output(t_new, t, ynew, [](double, double*) { return true; });
+      } else if (output_on()) {
         // cont[1..3] as DOPRI5's, k[13..15] at c[13..15], cont[r] = h sum_j
         // bi[r][j] k[j] for r = 4..7.
         for (int c = 0; c < n; ++c) {
@@ -743,8 +765,9 @@ struct VecDop853 : VecCommon {
         for (int r = 4; r < 8; ++r)
           for (int c = 0; c < n; ++c) cont[r][c] = comb(d8::BI[r], 16, c) * h;
         const double tp = t, hp = h;
-        emitted = emit(t_new, tp, ynew, true, [&](double te, double* out) {
+        emitted = output(t_new, tp, ynew, [&](double te, double* out) {
           if (te < tp || te > t_new) return false;
+          if (!out) return true;
           const double s = (te - tp) / hp;
           const double s1 = 1.0 - s;
           for (int c = 0; c < n; ++c) {
@@ -898,7 +921,7 @@ struct VecErk : VecCommon {
     h_prev = 0.0;
     stiff = 0;
     rejected_last = false;
-    if (a.n_eval > 0) emit(t, t, y, false, [](double, double*) { return true; });
+    output_start();
     return true;
   }
 
@@ -944,54 +967,65 @@ struct VecErk : VecCommon {
       diff(t_new, yh, dn);
     }
     bool emitted = true;
-    if (a.n_eval > 0 && accept && next_eval < a.n_eval) {
+    // The interpolant is built for a dense or even point output, or when the
+    // next t_eval point is not past the new time; DefaultSolout pushes the
+    // step's end.
+    bool interp_out = false;
+    if (accept && points_on()) {
+      if (a.pts_mode == kPointsSteps)
+        pts_push(t_new, yh);
+      else
+        interp_out = true;
+    } else if (a.n_eval > 0 && accept && next_eval < a.n_eval) {
       const double te0 = a.t_eval[next_eval];
-      if (!(dir > 0.0 ? te0 > t_new : te0 < t_new)) {
-        // The dense stages for the interpolant (ordinary.rs:240-277), or the
-        // cubic Hermite interpolant without them.
-        if (kBi) stages(S, I, t, h);
-        bool fin = true;
-        if (kBi) {
-          uint32_t anyb = 0;
-          for (int c = 0; c < n; ++c) anyb |= bad[c];
-          fin = (anyb & zero_bi_rows()) == 0;
-        }
-        const double tp = t, hp = h;
-        emitted = emit(t_new, tp, yh, true, [&](double te, double* out) {
-          if (te < tp || te > t_new) return false;
-          if (kBi) {
-            // Horner over bi[r][0..O) times s, then y_prev + sum_r
-            // (cont_r h_prev) k[r] in r order; all-zero rows add +-0 and are
-            // skipped when their stages, s and h_prev are finite.
-            const double s = (te - tp) / hp;
-            const bool all = !fin || !isfinite(s) || !isfinite(hp);
-            const uint32_t zr = zero_bi_rows();
-            for (int c = 0; c < n; ++c) out[c] = y[c];
-            for (int r = 0; r < I; ++r) {
-              if (((zr >> r) & 1u) && !all) continue;
-              double q = Tab::BI[r][O - 1];
-              for (int j = O - 2; j >= 0; --j) q = q * s + Tab::BI[r][j];
-              q = q * s;
-              const double wq = q * hp;
-              for (int c = 0; c < n; ++c) out[c] = out[c] + wq * k[r][c];
-            }
-          } else {
-            // cubic_hermite_interpolate (src/interpolate.rs:40-60).
-            const double hh = t_new - tp;
-            const double s = (te - tp) / hh;
-            const double s2 = s * s;
-            const double s3 = s2 * s;
-            const double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
-            const double h10 = s3 - 2.0 * s2 + s;
-            const double h01 = -2.0 * s3 + 3.0 * s2;
-            const double h11 = s3 - s2;
-            const double w10 = h10 * hh, w11 = h11 * hh;
-            for (int c = 0; c < n; ++c)
-              out[c] = (((0.0 + h00 * y[c]) + w10 * k[0][c]) + h01 * yh[c]) + w11 * dn[c];
-          }
-          return true;
-        });
+      interp_out = !(dir > 0.0 ? te0 > t_new : te0 < t_new);
+    }
+    if (interp_out) {
+      // The dense stages for the interpolant (ordinary.rs:240-277), or the
+      // cubic Hermite interpolant without them.
+      if (kBi) stages(S, I, t, h);
+      bool fin = true;
+      if (kBi) {
+        uint32_t anyb = 0;
+        for (int c = 0; c < n; ++c) anyb |= bad[c];
+        fin = (anyb & zero_bi_rows()) == 0;
       }
+      const double tp = t, hp = h;
+      emitted = output(t_new, tp, yh, [&](double te, double* out) {
+        if (te < tp || te > t_new) return false;
+        if (!out) return true;
+        if (kBi) {
+          // Horner over bi[r][0..O) times s, then y_prev + sum_r
+          // (cont_r h_prev) k[r] in r order; all-zero rows add +-0 and are
+          // skipped when their stages, s and h_prev are finite.
+          const double s = (te - tp) / hp;
+          const bool all = !fin || !isfinite(s) || !isfinite(hp);
+          const uint32_t zr = zero_bi_rows();
+          for (int c = 0; c < n; ++c) out[c] = y[c];
+          for (int r = 0; r < I; ++r) {
+            if (((zr >> r) & 1u) && !all) continue;
+            double q = Tab::BI[r][O - 1];
+            for (int j = O - 2; j >= 0; --j) q = q * s + Tab::BI[r][j];
+            q = q * s;
+            const double wq = q * hp;
+            for (int c = 0; c < n; ++c) out[c] = out[c] + wq * k[r][c];
+          }
+        } else {
+          // cubic_hermite_interpolate (src/interpolate.rs:40-60).
+          const double hh = t_new - tp;
+          const double s = (te - tp) / hh;
+          const double s2 = s * s;
+          const double s3 = s2 * s;
+          const double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+          const double h10 = s3 - 2.0 * s2 + s;
+          const double h01 = -2.0 * s3 + 3.0 * s2;
+          const double h11 = s3 - s2;
+          const double w10 = h10 * hh, w11 = h11 * hh;
+          for (int c = 0; c < n; ++c)
+            out[c] = (((0.0 + h00 * y[c]) + w10 * k[0][c]) + h01 * yh[c]) + w11 * dn[c];
+        }
+        return true;
+      });
     }
     if (accept) {
       h_prev = h;

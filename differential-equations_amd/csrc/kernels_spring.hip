@@ -3,7 +3,8 @@
 // Mappings (selected by de_ode_batch.mapping):
 //   DE_MAP_LANE   one thread per system, vectors in an HBM workspace (plain
 //                 launch; bit-identical to the CPU restatement).  DOPRI5, and
-//                 the only mapping of DOP853, RK4, Euler and the embedded family.
+//                 the only mapping of DOP853, RK4, Euler, the embedded family
+//                 and the point outputs.
 //   DE_MAP_GROUP  G lanes per system with C masses per lane in registers.
 //   DE_MAP_AUTO   systems are bucketed into size classes (N <= 4, 8, ..., 256);
 //                 for each class an autotuner times every (G, C, block) with
@@ -61,7 +62,7 @@ static LaneSolver lane_solver(int method) {
     case DE_METHOD_RKV878E: return lane_solver_of<VecErk<erk_tab::Rkv878e>>();
     case DE_METHOD_RKV988E: return lane_solver_of<VecErk<erk_tab::Rkv988e>>();
     case DE_METHOD_RKV989E: return lane_solver_of<VecErk<erk_tab::Rkv989e>>();
-    default: return {spring_lane_kernel<VecDopri5>, 9};
+    default: return lane_solver_of<VecDopri5>();
   }
 }
 
@@ -222,12 +223,21 @@ static Choice autotune(const SpringArgs& base, int k, const int32_t* d_list, int
   return best;
 }
 
+// One thread per system: DE_MAP_LANE, every method but DOPRI5 (the group
+// mapping is built for DOPRI5) and the point outputs.
+static bool lane_mapping(const de_ode_batch* b) {
+  return b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5 ||
+         b->points != DE_POINTS_NONE;
+}
+
 int64_t spring_workspace_bytes(const de_ode_batch* b) {
   if (!b) return 0;
-  if (b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5) {
-    // The one-lane solver's state-length vectors per system; offsets are
-    // device memory here, so the caller's bound uses 512 doubles per system.
-    return (int64_t)lane_solver(b->method).vectors * 512 * 8 * b->n;
+  if (lane_mapping(b)) {
+    // The one-lane solver's state-length vectors per system (offsets are
+    // device memory here, so the bound uses 512 doubles per system) and, with
+    // a point output, n + 1 state offsets into points_y.
+    return (int64_t)lane_solver(b->method).vectors * 512 * 8 * b->n +
+           (b->points != DE_POINTS_NONE ? 8 * (b->n + 1) : 0);
   }
   return (int64_t)sizeof(int32_t) * b->n + (int64_t)sizeof(unsigned long long) * (kNumCaps + 1);
 }
@@ -243,6 +253,7 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
   base.state_offset = b->state_offset;
   base.param_offset = b->param_offset;
   base.work = nullptr;
+  base.pts_ypos = nullptr;
   base.list = nullptr;
   base.list_len = 0;
   // Host-side size scan: validity, total state length, classes.
@@ -256,11 +267,11 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     masses[k] += host_sizes[i];
     state_len += 2 * (int64_t)host_sizes[i];
   }
-  // One lane per system: DE_MAP_LANE, and every method but DOPRI5 (the group
-  // mapping is built for DOPRI5).
-  if (b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5) {
+  if (lane_mapping(b)) {
     const LaneSolver ls = lane_solver(b->method);
-    const int64_t need = (int64_t)ls.vectors * state_len;
+    const bool pts = b->points != DE_POINTS_NONE;
+    const int64_t vec_len = (int64_t)ls.vectors * state_len;
+    const int64_t need = vec_len + (pts ? n + 1 : 0);
     if (g_work_len < need) {
       if (g_work) (void)hipFree(g_work);
       g_work = nullptr;
@@ -269,6 +280,27 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
       g_work_len = need;
     }
     base.work = g_work;
+    if (pts) {
+      // Start of each system's point states in points_y: the slot counts
+      // times the state lengths, summed over the systems before it.
+      std::vector<int64_t> off((size_t)n + 1), ypos((size_t)n + 1);
+      if (hipMemcpyAsync(off.data(), a.pts_offset, sizeof(int64_t) * (n + 1),
+                         hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return -1;
+      ypos[0] = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        const int64_t slots = off[i + 1] - off[i];
+        if (off[i] < 0 || slots < 0) return 2;
+        ypos[i + 1] = ypos[i] + slots * 2 * (int64_t)host_sizes[i];
+      }
+      int64_t* d_ypos = (int64_t*)(g_work + vec_len);
+      if (hipMemcpyAsync(d_ypos, ypos.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
+                         s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return -1;
+      base.pts_ypos = d_ypos;
+    }
     int64_t grid = (n + kSpringBlock - 1) / kSpringBlock;
     hipLaunchKernelGGL(ls.kernel, dim3((unsigned)grid), dim3(kSpringBlock), 0, s, base);
     return hipGetLastError() == hipSuccess ? 0 : -1;

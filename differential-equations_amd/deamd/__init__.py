@@ -517,11 +517,14 @@ class OdeEnsemble:
                     keep.append(v)
                     setattr(b, "event_" + name, _ptr(v))
         res = Solution(self.n, state_len, n_eval, cap)
+        if isinstance(s, SpringChain):
+            res.state_offset = s.state_offset
         mode, pn, pdt = self._points
         if mode != DE_POINTS_NONE and points_offset is not None:
             b.points, b.points_n, b.points_dt = mode, pn, pdt
             res.set_points(np.ascontiguousarray(points_offset, dtype=np.int64),
-                           0 if isinstance(s, SpringChain) else s.dim)
+                           2 * s.sizes.astype(np.int64) if isinstance(s, SpringChain)
+                           else s.dim)
         return b, res, keep
 
     def points_offsets(self, accepted=None):
@@ -593,22 +596,47 @@ class Solution:
         self.event_y = np.full(max_events * state_len, np.nan) if max_events else None
         self.n_events = np.zeros(n, dtype=np.int32) if max_events is not None else None
         self.points_offset = self.points_t = self.points_y = self.n_points = None
+        self.points_ypos = None
+        # Spring chains: trajectory i's state is y[state_offset[i]:state_offset[i + 1]].
+        self.state_offset = None
 
     def set_points(self, offsets, dim):
-        """Buffers of a point output: slots offsets[i] .. offsets[i + 1] of trajectory i."""
+        """Buffers of a point output: slots offsets[i] .. offsets[i + 1] of
+        trajectory i, states of `dim` components.  With per-trajectory state
+        lengths (an array: spring chains) each trajectory's slot states are
+        kept back to back from points_ypos[i]."""
         self.points_offset = offsets
         cap = int(offsets[-1])
         self.points_t = np.full(max(cap, 1), np.nan)
-        self.points_y = np.full(max(cap * dim, 1), np.nan)
+        if np.ndim(dim):
+            self.points_ypos = np.zeros(self.n + 1, dtype=np.int64)
+            np.cumsum(np.diff(offsets) * np.asarray(dim, dtype=np.int64),
+                      out=self.points_ypos[1:])
+            ylen = int(self.points_ypos[-1])
+        else:
+            ylen = cap * dim
+        self.points_y = np.full(max(ylen, 1), np.nan)
         self.n_points = np.zeros(self.n, dtype=np.int64)
+
+    def _dim(self, i):
+        if self.state_offset is not None:
+            return int(self.state_offset[i + 1] - self.state_offset[i])
+        return len(self.y) // self.n
+
+    def final_state(self, i):
+        """The final state of trajectory i (the state at failure for an error status)."""
+        if self.state_offset is not None:
+            return self.y[self.state_offset[i]:self.state_offset[i + 1]].copy()
+        return self.y.reshape(self._dim(i), self.n)[:, i].copy()
 
     def points(self, i):
         """(t[k], y[k, dim]) of trajectory i's Solution points (the kept ones)."""
-        dim = len(self.y) // self.n
+        dim = self._dim(i)
         lo, hi = int(self.points_offset[i]), int(self.points_offset[i + 1])
         k = min(int(self.n_points[i]), hi - lo)
+        y0 = int(self.points_ypos[i]) if self.points_ypos is not None else lo * dim
         return (self.points_t[lo:lo + k].copy(),
-                self.points_y[lo * dim:(lo + k) * dim].reshape(k, dim).copy())
+                self.points_y[y0:y0 + k * dim].reshape(k, dim).copy())
 
     def events(self, i):
         """(t[k], y[k, dim]) of the event points kept for trajectory i."""
@@ -660,7 +688,7 @@ class Solution:
         the t_eval points produced, else the final state), as a pandas
         DataFrame (polars is not available here)."""
         import pandas as pd
-        dim = len(self.y) // self.n
+        dim = self._dim(i)
         if self.n_points is not None:
             t, y = self.points(i)
         elif self.n_dense is not None:
@@ -677,11 +705,16 @@ class Solution:
             run = int(np.argmax(beyond)) if beyond.any() else len(te)
             idx = ([0] if head else []) + list(range(run, len(te)))
             idx = idx[:int(self.n_dense[i])]
-            rows = self.dense.reshape(-1, dim, self.n)[:, :, i]
+            if self.state_offset is not None:
+                ne = len(te)
+                so = self.state_offset
+                rows = self.dense[ne * so[i]:ne * so[i + 1]].reshape(ne, dim)
+            else:
+                rows = self.dense.reshape(-1, dim, self.n)[:, :, i]
             t, y = te[idx], rows[idx]
         else:
             t = self.t[i:i + 1]
-            y = self.y.reshape(dim, self.n)[:, i][None, :]
+            y = self.final_state(i)[None, :]
         cols = {"t": np.asarray(t, dtype=np.float64)}
         for c in range(dim):
             cols["y%d" % c] = np.asarray(y[:, c], dtype=np.float64)

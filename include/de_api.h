@@ -202,8 +202,8 @@ typedef struct de_ode_batch {
   /* ExplicitRungeKutta::max_rejects of the embedded family (erk/mod.rs:81,140,201):
    * consecutive rejections that end the solve with Error::Stiffness. */
   int64_t max_rejects;
-  /* Output strategy with a variable number of points (DE_POINTS_*; fixed-dimension
-   * families, without t_eval or an event solout). */
+  /* Output strategy with a variable number of points (DE_POINTS_*; without t_eval
+   * or an event solout; spring chains run it on the one-lane-per-system mapping). */
   int32_t points;
   int32_t points_n;  /* DE_POINTS_DENSE: n (points per step, including the step) */
   double points_dt;  /* DE_POINTS_EVEN: dt */
@@ -223,7 +223,11 @@ typedef struct de_ode_result {
   int32_t* n_events; /* [n] event points pushed (those past max_events are not kept), or NULL */
   /* DE_POINTS_*: trajectory i's Solution points go to slots points_offset[i] ..
    * points_offset[i+1] (n + 1 values, given by the caller): time points_t[s], state
-   * points_y[s * dim + c].  n_points[i] is the number of points the solout pushed; those
+   * points_y[s * dim + c].  Spring chains (state length 2 sizes[i]) keep the states of
+   * trajectory i's slots back to back from Y_i = sum_{j<i} (points_offset[j+1] -
+   * points_offset[j]) * 2 sizes[j]: slot points_offset[i] + k at points_y[Y_i + k * 2
+   * sizes[i] + c], points_y holding Y_n values.  n_points[i] is the number of points the
+   * solout pushed; those
    * beyond the trajectory's slots are counted, not kept, and slots past n_points[i] are
    * left unspecified.  Slot counts that always
    * suffice: accepted + 1 (STEPS), n * accepted + 1 (DENSE), from the accepted counts

@@ -597,7 +597,7 @@ struct Points {
   template <class Interp>
   int call(double t_curr, double t_prev, const double* y_curr, const double* y_prev,
            Interp&& interp) {
-    double yi[kEvMax > 16 ? kEvMax : 16];
+    double yi[SpringSys::kMax];  // the longest state of any system
     if (mode == DE_POINTS_STEPS) {
       push(t_curr, y_curr);
       return kContinue;
@@ -1758,6 +1758,16 @@ static void run_fixed_family(const de_ode_batch* b, de_ode_result* out, const Se
 static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings& base,
                        int threads) {
   const int64_t n = b->n;
+  // A point output keeps the states of system i's slots back to back from
+  // ypos[i] = sum_{j<i} slots_j * 2 sizes[j].
+  const bool with_pts = b->points != DE_POINTS_NONE && out->points_offset;
+  std::vector<int64_t> ypos(with_pts ? (size_t)n + 1 : 0);
+  if (with_pts) {
+    ypos[0] = 0;
+    for (int64_t i = 0; i < n; ++i)
+      ypos[i + 1] = ypos[i] + (out->points_offset[i + 1] - out->points_offset[i]) * 2 *
+                                  (int64_t)b->sizes[i];
+  }
   parallel_for(n, threads, [&](int64_t i) {
     SpringSys sys;
     sys.nmass = b->sizes[i];
@@ -1778,9 +1788,21 @@ static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings
       dense = out->dense + (int64_t)b->n_eval * off;
       for (int64_t e = 0; e < (int64_t)b->n_eval * sys.dim(); ++e) dense[e] = NAN;
     }
+    Points pts;
+    if (with_pts) {
+      pts.mode = b->points;
+      pts.dense_n = b->points_n;
+      pts.dt = b->points_dt;
+      pts.n = sys.dim();
+      pts.t = out->points_t + out->points_offset[i];
+      pts.y = out->points_y + ypos[i];
+      pts.cap = out->points_offset[i + 1] - out->points_offset[i];
+    }
     TrajResult r;
     run_ode_traj(sys, b, st, i, b->y0 + off, out->y_final + off, dense,
-                 out->n_dense ? &out->n_dense[i] : nullptr, nullptr, nullptr, &r);
+                 out->n_dense ? &out->n_dense[i] : nullptr, nullptr, nullptr, &r,
+                 with_pts ? &pts : nullptr);
+    if (with_pts && out->n_points) out->n_points[i] = pts.pushed;
     out->t_final[i] = r.t;
     out->accepted[i] = r.accepted;
     out->rejected[i] = r.rejected;

@@ -235,3 +235,66 @@ def test_embedded_family_equals_oracle(orc, name):
             assert np.array_equal(got.n_dense, want.n_dense)
             assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
             assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
+
+
+def same_points(got, want):
+    """The kept points of every trajectory, as bits (NaN where the oracle has NaN)."""
+    assert np.array_equal(got.n_points, want.n_points)
+    assert np.array_equal(got.points_offset, want.points_offset)
+    for i in range(got.n):
+        gt, gy = got.points(i)
+        wt, wy = want.points(i)
+        assert np.array_equal(gt, wt), i
+        assert np.array_equal(np.isnan(gy), np.isnan(wy)), i
+        assert np.array_equal(np.nan_to_num(gy), np.nan_to_num(wy)), i
+
+
+@pytest.mark.parametrize("out", ["steps", "dense", "even"])
+@pytest.mark.parametrize("method", ["dopri5", "dop853", "rk4", "rkv878e"])
+def test_point_outputs_bit_exact(orc, method, out):
+    # DefaultSolout, IVP::dense(3) and IVP::even(dt) on spring chains (one lane
+    # per system whatever the mapping asked for; each system's point states
+    # back to back) equal the oracle bit for bit.
+    sysm, y0 = deamd.gen_spring(400, seed=35)
+    m = {"dopri5": lambda: ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9),
+         "dop853": lambda: ExplicitRungeKutta.dop853().rtol(1e-8).atol(1e-10),
+         "rk4": lambda: ExplicitRungeKutta.rk4(0.01),
+         "rkv878e": lambda: ExplicitRungeKutta.rkv878e().rtol(1e-8).atol(1e-10)}[method]
+    pick = {"steps": lambda e: e.all_steps(), "dense": lambda e: e.dense(3),
+            "even": lambda e: e.even(0.07)}[out]
+    ens = pick(IVP.ode(sysm, 0.0, 2.0, y0).method(m().max_steps(200000)))
+    want = orc.solve_ode(ens, threads=THREADS)
+    assert (want.status == DE_STATUS_COMPLETE).all()
+    for mapping in (DE_MAP_AUTO, DE_MAP_LANE):
+        got = ens.options(mapping=mapping).solve()
+        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+            assert np.array_equal(getattr(got, f), getattr(want, f)), (f, mapping)
+        same_points(got, want)
+
+
+def test_point_outputs_backward_and_short_slots(orc):
+    # A backward solve: the step output completes, the dense and even outputs
+    # end OutOfBounds at their first interpolation; slots short of the points
+    # keep the first ones and count the rest.
+    sysm, y0 = deamd.gen_spring(300, seed=36)
+    mk = lambda: IVP.ode(sysm, 0.0, -1.5, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9))
+    for ens in (mk().all_steps(), mk().dense(2), mk().even(0.2)):
+        want = orc.solve_ode(ens, threads=THREADS)
+        got = ens.solve()
+        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+            assert np.array_equal(getattr(got, f), getattr(want, f)), f
+        same_points(got, want)
+    ens = IVP.ode(sysm, 0.0, 2.0, y0).method(
+        ExplicitRungeKutta.dop853().rtol(1e-8).atol(1e-10)).all_steps()
+    full = orc.solve_ode(ens, threads=THREADS)
+    cap = np.minimum(full.accepted + 1, np.arange(len(sysm.sizes)) % 7)
+    off = np.zeros(len(cap) + 1, dtype=np.int64)
+    np.cumsum(cap, out=off[1:])
+    got = ens._solve_once(None, off)
+    assert np.array_equal(got.n_points, full.n_points)
+    for i in range(len(cap)):
+        t, y = got.points(i)
+        ft, fy = full.points(i)
+        assert len(t) == cap[i]
+        assert np.array_equal(t, ft[:cap[i]]) and np.array_equal(y, fy[:cap[i]])
