@@ -147,8 +147,6 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
       return fail(DE_ERR_INVALID, "a point output needs points_offset and n_points");
   }
   if (b->event_solout != DE_SOLOUT_DEFAULT) {
-    if (b->family == DE_FAMILY_SPRING_CHAIN)
-      return fail(DE_ERR_UNSUPPORTED, "event solouts are built for fixed-dimension families");
     if (b->event_solout != DE_SOLOUT_EVENT && b->n_eval > 0)
       return fail(DE_ERR_INVALID, "a crossing solout replaces the t_eval output (n_eval > 0)");
     if (b->event_direction < DE_CROSS_BOTH || b->event_direction > DE_CROSS_NEGATIVE)
@@ -162,7 +160,14 @@ int validate_ode(const de_ode_batch* b, const de_ode_result* o) {
                                                               : b->event_fn;
     if (fn < DE_EVENT_COMPONENT || fn > DE_EVENT_PLANE)
       return fail(DE_ERR_INVALID, "unknown event function %d", b->event_fn);
-    if (fn == DE_EVENT_COMPONENT && (b->event_component < 0 || b->event_component >= b->dim))
+    if (b->family == DE_FAMILY_SPRING_CHAIN && fn == DE_EVENT_PLANE)
+      return fail(DE_ERR_UNSUPPORTED,
+                  "plane events and the hyperplane solout need one state length: not for "
+                  "spring chains");
+    if (fn == DE_EVENT_COMPONENT && b->event_component < 0)
+      return fail(DE_ERR_INVALID, "event component %d < 0", b->event_component);
+    if (fn == DE_EVENT_COMPONENT && b->family != DE_FAMILY_SPRING_CHAIN &&
+        b->event_component >= b->dim)
       return fail(DE_ERR_INVALID, "event component %d outside [0, %d)", b->event_component,
                   b->dim);
     if (fn == DE_EVENT_PLANE && (!b->event_point || !b->event_normal))
@@ -283,6 +288,9 @@ int run_ode_device(const de_ode_batch* b, const int32_t* host_sizes, de::OdeArgs
     int r = de::parity::launch_spring(b, host_sizes, a, sched, s);
     if (r == 1) return fail(DE_ERR_UNSUPPORTED, "spring-chain mapping not supported");
     if (r == 2) return fail(DE_ERR_INVALID, "points_offset is negative or decreases");
+    if (r == 3)
+      return fail(DE_ERR_INVALID, "event component %d outside a spring chain's state",
+                  b->event_component);
     if (r < 0) return r == -4 ? fail(DE_ERR_NOMEM, "spring workspace allocation failed")
                               : hip_fail(hipGetLastError(), "spring launch");
     return DE_OK;
@@ -589,7 +597,8 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   dr.n_dense = b->n_eval > 0 ? st.out(o->n_dense, n) : nullptr;
   const int64_t ev_cap = events ? (int64_t)b->max_events * n : 0;
   dr.event_t = st.out(o->event_t, ev_cap);
-  dr.event_y = st.out(o->event_y, ev_cap * b->dim);
+  const int64_t ev_ylen = events ? (int64_t)b->max_events * (ragged ? state_len : b->dim * n) : 0;
+  dr.event_y = st.out(o->event_y, ev_ylen);
   dr.n_events = st.out(o->n_events, events ? n : 0);
   const bool pts = b->points != DE_POINTS_NONE;
   const int64_t pts_cap = pts ? o->points_offset[n] : 0;
@@ -623,7 +632,7 @@ int de_ode_batch_solve(const de_ode_batch* b, de_ode_result* o, void* stream) {
   if (dr.dense) st.back(o->dense, dr.dense, dense_len);
   if (dr.n_dense) st.back(o->n_dense, dr.n_dense, n);
   if (dr.event_t) st.back(o->event_t, dr.event_t, ev_cap);
-  if (dr.event_y) st.back(o->event_y, dr.event_y, ev_cap * b->dim);
+  if (dr.event_y) st.back(o->event_y, dr.event_y, ev_ylen);
   if (dr.n_events) st.back(o->n_events, dr.n_events, n);
   if (dr.points_t) st.back(o->points_t, dr.points_t, pts_cap);
   if (dr.points_y) st.back(o->points_y, dr.points_y, pts_ylen);

@@ -33,6 +33,7 @@ struct SpringArgs {
   double* work;                // one-lane solvers: kVectors vectors per system at
                                // kVectors * state_offset
   const int64_t* pts_ypos;     // point output: start of system i's states in pts_y
+  double* ev_work;             // event solouts: one state per system at state_offset
   const int32_t* list;         // GroupDopri5: system indices of this size class
   int64_t list_len;
 };
@@ -47,8 +48,16 @@ struct SpringArgs {
 // interp(te, out) returns false when te is outside the step (every interior
 // point of a backward step), where the reference's solouts unwrap the
 // interpolation and panic: the solve then ends OutOfBounds.  Otherwise it
-// writes the state at te to out, unless out is null (a point past the
-// system's slots, counted and not kept).
+// writes components ic0 .. ic1 - 1 of the state at te to out (all of them but
+// in an event's root search), unless out is null (a point past the system's
+// slots, counted and not kept, or a time event's search).
+//
+// Event solouts (EventWrappedSolout around the t_eval or default output, or
+// CrossingSolout; event.rs:320-523, crossing.rs:181-320) search their roots
+// inline with the lane code of de_event.h, on the event component projected
+// to a one-component state.  Event point k of system i: time
+// t_out[k * n_systems + i], state at y_out[max_events * state_offset[i] +
+// k * n].
 struct VecCommon {
   const SpringArgs* A;
   int64_t i;
@@ -58,6 +67,10 @@ struct VecCommon {
   double t, h, tf, dir;
   double last;  // EvenSolout::last_output_t
   int32_t next_eval, pushed;
+  int ic0, ic1;                  // components the interpolation callbacks write
+  double ev_value, ev_last, ev_new;
+  double tail;                   // time of the Solution's last point (NaN: none)
+  int32_t ev_count, ev_pushed;
 
   DE_HD void diff(double, const double* yy, double* dy) const { spring_chain_diff(nm, p, yy, dy); }
 
@@ -75,7 +88,135 @@ struct VecCommon {
       for (int64_t e = (int64_t)next_eval * n; e < (int64_t)a.n_eval * n; ++e)
         dbase[e] = as_f64(kNaNBits);
     }
+    if (a.ev.solout != kSoloutDefault) {  // event count, NaN past the last point
+      const EventArgs& e = a.ev;
+      if (e.n_out) e.n_out[i] = ev_pushed;
+      double* yb = e.y_out + (int64_t)e.max_events * A->state_offset[i];
+      for (int32_t k = ev_pushed; k < e.max_events; ++k) {
+        e.t_out[(int64_t)k * a.n + i] = as_f64(kNaNBits);
+        for (int c = 0; c < n; ++c) yb[(int64_t)k * n + c] = as_f64(kNaNBits);
+      }
+    }
   }
+
+  // ---- event solouts
+  DE_HD bool events_on() const { return A->o.ev.solout != kSoloutDefault; }
+  DE_HD double ev_g(double tt, const double* yy) const {
+    const EventArgs& e = A->o.ev;
+    return e.fn == kEventTime ? tt - ev_value : yy[e.comp] - ev_value;
+  }
+  // The solout call after init records g(t0, y0) (event.rs:346-355,
+  // crossing.rs:240-246).
+  DE_HD void ev_start() {
+    if (!events_on()) return;
+    const EventArgs& e = A->o.ev;
+    ev_value = e.value_traj ? e.value_traj[i] : e.value;
+    ev_last = ev_g(t, y);
+  }
+  // g at an accepted step's end; true when the step brackets a root to search
+  // in the configured direction (LaneEvent::pre of both solouts).
+  DE_HD bool ev_pre(double t_curr, const double* y_curr) {
+    if (!events_on()) return false;
+    const EventArgs& e = A->o.ev;
+    ev_new = ev_g(t_curr, y_curr);
+    const double lo = ev_last, g = ev_new;
+    const bool change = sign_differs(lo, g);
+    if (e.solout == kSoloutEvent) {
+      const bool ok = e.direction == kCrossPositive   ? change && lo < 0.0 && g >= 0.0
+                      : e.direction == kCrossNegative ? change && lo > 0.0 && g <= 0.0
+                                                      : change;
+      return ok && lo * g <= 0.0;
+    }
+    const bool rec = e.direction == kCrossPositive   ? lo < 0.0 && g >= 0.0
+                     : e.direction == kCrossNegative ? lo > 0.0 && g <= 0.0
+                                                     : true;
+    return change && rec;
+  }
+  // The state slot of the next event point, or null past max_events.
+  DE_HD double* ev_dst() const {
+    const EventArgs& e = A->o.ev;
+    return ev_pushed < e.max_events
+               ? e.y_out + (int64_t)e.max_events * A->state_offset[i] + (int64_t)ev_pushed * n
+               : nullptr;
+  }
+  DE_HD void ev_push(double tt) {
+    const EventArgs& e = A->o.ev;
+    if (ev_pushed < e.max_events) e.t_out[(int64_t)ev_pushed * A->o.n + i] = tt;
+    ev_pushed += 1;
+  }
+  // The root search of a bracketed step [t_prev, t_curr] and its push
+  // (detect_event event.rs:335-393, CrossingSolout::solout crossing.rs:
+  // 186-248); -1 to continue, else the solve's final status.  The Solution's
+  // last time is the step's end under the default output, else `tail`.
+  template <class Interp>
+  DE_HD int32_t ev_post(bool need, double t_curr, double t_prev, Interp& interp) {
+    if (!need) return -1;
+    const EventArgs& e = A->o.ev;
+    EventArgs e1 = e;
+    e1.comp = 0;  // g of the projected one-component state
+    double* ew = A->ev_work + A->state_offset[i];
+    auto interp1 = [&](double tt, double* o) -> bool {
+      if (e.fn == kEventTime) {
+        o[0] = 0.0;
+        return interp(tt, nullptr);
+      }
+      ic0 = e.comp;
+      ic1 = e.comp + 1;
+      const bool ok = interp(tt, ew);
+      ic0 = 0;
+      ic1 = n;
+      o[0] = ok ? ew[e.comp] : 0.0;
+      return ok;
+    };
+    int32_t st = -1;
+    if (e.solout == kSoloutEvent) {
+      double a0 = t_prev, b0 = t_curr, fa = ev_last, fb = ev_new;
+      if ((dir > 0.0 && a0 > b0) || (dir < 0.0 && a0 < b0)) {
+        double x = a0;
+        a0 = b0;
+        b0 = x;
+        x = fa;
+        fa = fb;
+        fb = x;
+      }
+      const double v = ev_value;
+      auto g = [&](double tt, const double* yy) { return event_g<1>(e1, v, tt, yy); };
+      double t_event;
+      if (brent_dekker<1>(a0, b0, fa, fb, interp1, g, &t_event)) {
+        if (!interp(t_event, ev_dst())) {
+          st = kOutOfBounds;
+        } else {
+          const double tl = A->o.n_eval > 0 ? tail : t_curr;
+          if (tl != tl || fabs(t_event - tl) > 1e-14) {
+            ev_push(t_event);
+            tail = t_event;
+          }
+          ev_count += 1;
+          if (e.terminate > 0 && ev_count >= e.terminate) st = kInterrupted;
+        }
+      }
+    } else {
+      LaneEvent<1, 2> cs;
+      cs.value = ev_value;
+      bool found = false;
+      double tc = 0.0;
+      if (!cs.newton(e1, t_prev, t_curr, ev_last, ev_new, interp1, &found, &tc)) {
+        st = kOutOfBounds;
+      } else {
+        if (!found) {
+          const double frac = -ev_last / (ev_new - ev_last);
+          tc = t_prev + frac * (t_curr - t_prev);
+        }
+        if (!interp(tc, ev_dst()))
+          st = kOutOfBounds;
+        else
+          ev_push(tc);
+      }
+    }
+    return st;
+  }
+  // After an accepted step's solout call: g at its end is the next last value.
+  DE_HD void ev_commit() { ev_last = ev_new; }
 
   DE_HD bool points_on() const { return A->o.pts_mode != kPointsNone; }
   // An accepted step writes output: t_eval rows or a point output.
@@ -193,6 +334,7 @@ struct VecCommon {
           break;
         }
         pushed += 1;
+        tail = te;
         idx += 1;
       } else {
         if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
@@ -222,6 +364,10 @@ struct VecCommon {
     tf = a.tf ? a.tf[i] : a.tfs;
     t = t0;
     pushed = next_eval = 0;
+    ic0 = 0;
+    ic1 = n;
+    ev_count = ev_pushed = 0;
+    tail = as_f64(kNaNBits);
     dir = signum(tf - t0);
     return t0;
   }
@@ -312,6 +458,7 @@ struct VecDopri5 : VecCommon {
     countdown = 100;
     rejected_last = false;
     output_start();
+    ev_start();
     return true;
   }
 
@@ -417,10 +564,11 @@ struct VecDopri5 : VecCommon {
           if (nonstiff == 6) stiff = 0;
         }
       }
-      bool emitted = true;
+      const bool need = ev_pre(t_new, ynew);
+      int32_t sol = -1;
       if (a.pts_mode == kPointsSteps) {
-        emitted = output(t_new, t, ynew, [](double, double*) { return true; });
-      } else if (output_on()) {
+        if (!output(t_new, t, ynew, [](double, double*) { return true; })) sol = kOutOfBounds;
+      } else if (output_on() || need) {
         // cont1 = ydiff (k[4]), cont2 = bspl (k[5]), cont3 (k[6]).
         for (int c = 0; c < n; ++c) {
           double ydiff = ynew[c] + (-1.0) * y[c];
@@ -432,22 +580,25 @@ struct VecDopri5 : VecCommon {
         }
         const double tp = t, hp = h;
         const double *c1 = k[4], *c2 = k[5], *c3 = k[6];
-        emitted = output(t_new, tp, ynew, [&](double te, double* out) {
+        auto interp = [&](double te, double* out) {
           // ordinary.rs:303-333: factors s1, s, s1 over cont[4] = 0 .. cont[1].
           if (te < tp || te > t_new) return false;
           if (!out) return true;
           const double s = (te - tp) / hp;
           const double s1 = 1.0 - s;
           const double cont4 = 0.0 * hp;
-          for (int c = 0; c < n; ++c) {
+          for (int c = ic0; c < ic1; ++c) {
             double acc_ = cont4 * s1 + c3[c];
             acc_ = acc_ * s + c2[c];
             acc_ = acc_ * s1 + c1[c];
             out[c] = y[c] + s * acc_;
           }
           return true;
-        });
+        };
+        if (output_on() && !output(t_new, tp, ynew, interp)) sol = kOutOfBounds;
+        if (sol < 0) sol = ev_post(need, t_new, tp, interp);
       }
+      if (events_on()) ev_commit();
       h_prev = h;
       t = t_new;
       for (int c = 0; c < n; ++c) {
@@ -461,8 +612,8 @@ struct VecDopri5 : VecCommon {
       acc += 1;
       ev += 7;
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
-      if (!emitted) {
-        finish(kOutOfBounds);
+      if (sol >= 0) {
+        finish(sol);
         return true;
       }
       if (fabs(tf - t) <= kTenEps) {
@@ -514,6 +665,7 @@ struct VecFixed : VecCommon {
     diff(t, y, k0);
     ev = 1;
     output_start();
+    ev_start();
     return true;
   }
 
@@ -554,11 +706,12 @@ struct VecFixed : VecCommon {
     diff(t, ynext, dn);
     ev += 1;
     acc += 1;
-    bool emitted = true;
-    if (output_on()) {
+    const bool need = ev_pre(t, ynext);
+    int32_t sol = -1;
+    if (output_on() || need) {
       const double tc = t;
       // cubic_hermite_interpolate on [t_prev, t] (fixed/ordinary.rs:170-177).
-      emitted = output(t, tp, ynext, [&](double te, double* out) {
+      auto interp = [&](double te, double* out) {
         if (te < tp || te > tc) return false;
         if (!out) return true;
         double hh = tc - tp;
@@ -570,17 +723,20 @@ struct VecFixed : VecCommon {
         double h01 = -2.0 * s3 + 3.0 * s2;
         double h11 = s3 - s2;
         double w10 = h10 * hh, w11 = h11 * hh;
-        for (int c = 0; c < n; ++c)
+        for (int c = ic0; c < ic1; ++c)
           out[c] = ((h00 * y[c] + w10 * k0[c]) + h01 * ynext[c]) + w11 * dn[c];
         return true;
-      });
+      };
+      if (output_on() && !output(t, tp, ynext, interp)) sol = kOutOfBounds;
+      if (sol < 0) sol = ev_post(need, t, tp, interp);
     }
+    if (events_on()) ev_commit();
     for (int c = 0; c < n; ++c) {
       y[c] = ynext[c];
       k0[c] = dn[c];
     }
-    if (!emitted) {
-      finish(kOutOfBounds);
+    if (sol >= 0) {
+      finish(sol);
       return true;
     }
     if (fabs(tf - t) <= kTenEps) {
@@ -669,6 +825,7 @@ struct VecDop853 : VecCommon {
     countdown = 100;
     rejected_last = false;
     output_start();
+    ev_start();
     return true;
   }
 
@@ -745,10 +902,11 @@ struct VecDop853 : VecCommon {
           if (nonstiff == 6) stiff = 0;
         }
       }
-      bool emitted = true;
+      const bool need = ev_pre(t_new, ynew);
+      int32_t sol = -1;
       if (a.pts_mode == kPointsSteps) {
-        emitted = output(t_new, t, ynew, [](double, double*) { return true; });
-      } else if (output_on()) {
+        if (!output(t_new, t, ynew, [](double, double*) { return true; })) sol = kOutOfBounds;
+      } else if (output_on() || need) {
         // cont[1..3] as DOPRI5's, k[13..15] at c[13..15], cont[r] = h sum_j
         // bi[r][j] k[j] for r = 4..7.
         for (int c = 0; c < n; ++c) {
@@ -765,12 +923,12 @@ struct VecDop853 : VecCommon {
         for (int r = 4; r < 8; ++r)
           for (int c = 0; c < n; ++c) cont[r][c] = comb(d8::BI[r], 16, c) * h;
         const double tp = t, hp = h;
-        emitted = output(t_new, tp, ynew, [&](double te, double* out) {
+        auto interp = [&](double te, double* out) {
           if (te < tp || te > t_new) return false;
           if (!out) return true;
           const double s = (te - tp) / hp;
           const double s1 = 1.0 - s;
-          for (int c = 0; c < n; ++c) {
+          for (int c = ic0; c < ic1; ++c) {
             double q = cont[7][c];
             q = q * s1 + cont[6][c];
             q = q * s + cont[5][c];
@@ -781,8 +939,11 @@ struct VecDop853 : VecCommon {
             out[c] = y[c] + s * q;
           }
           return true;
-        });
+        };
+        if (output_on() && !output(t_new, tp, ynew, interp)) sol = kOutOfBounds;
+        if (sol < 0) sol = ev_post(need, t_new, tp, interp);
       }
+      if (events_on()) ev_commit();
       h_prev = h;
       t = t_new;
       for (int c = 0; c < n; ++c) {
@@ -796,8 +957,8 @@ struct VecDop853 : VecCommon {
       acc += 1;
       ev += 15;
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
-      if (!emitted) {
-        finish(kOutOfBounds);
+      if (sol >= 0) {
+        finish(sol);
         return true;
       }
       if (fabs(tf - t) <= kTenEps) {
@@ -922,6 +1083,7 @@ struct VecErk : VecCommon {
     stiff = 0;
     rejected_last = false;
     output_start();
+    ev_start();
     return true;
   }
 
@@ -966,10 +1128,11 @@ struct VecErk : VecCommon {
     } else {
       diff(t_new, yh, dn);
     }
-    bool emitted = true;
-    // The interpolant is built for a dense or even point output, or when the
-    // next t_eval point is not past the new time; DefaultSolout pushes the
-    // step's end.
+    int32_t sol = -1;
+    // The interpolant is built for a dense or even point output, when the
+    // next t_eval point is not past the new time or for an event's root
+    // search; DefaultSolout pushes the step's end.
+    const bool need = accept && ev_pre(t_new, yh);
     bool interp_out = false;
     if (accept && points_on()) {
       if (a.pts_mode == kPointsSteps)
@@ -980,7 +1143,7 @@ struct VecErk : VecCommon {
       const double te0 = a.t_eval[next_eval];
       interp_out = !(dir > 0.0 ? te0 > t_new : te0 < t_new);
     }
-    if (interp_out) {
+    if (interp_out || need) {
       // The dense stages for the interpolant (ordinary.rs:240-277), or the
       // cubic Hermite interpolant without them.
       if (kBi) stages(S, I, t, h);
@@ -991,7 +1154,7 @@ struct VecErk : VecCommon {
         fin = (anyb & zero_bi_rows()) == 0;
       }
       const double tp = t, hp = h;
-      emitted = output(t_new, tp, yh, [&](double te, double* out) {
+      auto interp = [&](double te, double* out) {
         if (te < tp || te > t_new) return false;
         if (!out) return true;
         if (kBi) {
@@ -1001,14 +1164,14 @@ struct VecErk : VecCommon {
           const double s = (te - tp) / hp;
           const bool all = !fin || !isfinite(s) || !isfinite(hp);
           const uint32_t zr = zero_bi_rows();
-          for (int c = 0; c < n; ++c) out[c] = y[c];
+          for (int c = ic0; c < ic1; ++c) out[c] = y[c];
           for (int r = 0; r < I; ++r) {
             if (((zr >> r) & 1u) && !all) continue;
             double q = Tab::BI[r][O - 1];
             for (int j = O - 2; j >= 0; --j) q = q * s + Tab::BI[r][j];
             q = q * s;
             const double wq = q * hp;
-            for (int c = 0; c < n; ++c) out[c] = out[c] + wq * k[r][c];
+            for (int c = ic0; c < ic1; ++c) out[c] = out[c] + wq * k[r][c];
           }
         } else {
           // cubic_hermite_interpolate (src/interpolate.rs:40-60).
@@ -1021,12 +1184,15 @@ struct VecErk : VecCommon {
           const double h01 = -2.0 * s3 + 3.0 * s2;
           const double h11 = s3 - s2;
           const double w10 = h10 * hh, w11 = h11 * hh;
-          for (int c = 0; c < n; ++c)
+          for (int c = ic0; c < ic1; ++c)
             out[c] = (((0.0 + h00 * y[c]) + w10 * k[0][c]) + h01 * yh[c]) + w11 * dn[c];
         }
         return true;
-      });
+      };
+      if (interp_out && !output(t_new, tp, yh, interp)) sol = kOutOfBounds;
+      if (sol < 0) sol = ev_post(need, t_new, tp, interp);
     }
+    if (accept && events_on()) ev_commit();
     if (accept) {
       h_prev = h;
       t = t_new;
@@ -1045,8 +1211,8 @@ struct VecErk : VecCommon {
       finish(kStiffness);
       return true;
     }
-    if (!emitted) {
-      finish(kOutOfBounds);
+    if (sol >= 0) {
+      finish(sol);
       return true;
     }
     if (accept && fabs(tf - t) <= kTenEps) {

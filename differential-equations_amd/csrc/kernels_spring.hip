@@ -224,19 +224,22 @@ static Choice autotune(const SpringArgs& base, int k, const int32_t* d_list, int
 }
 
 // One thread per system: DE_MAP_LANE, every method but DOPRI5 (the group
-// mapping is built for DOPRI5) and the point outputs.
+// mapping is built for DOPRI5), the point outputs and the event solouts.
 static bool lane_mapping(const de_ode_batch* b) {
   return b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5 ||
-         b->points != DE_POINTS_NONE;
+         b->points != DE_POINTS_NONE || b->event_solout != DE_SOLOUT_DEFAULT;
 }
 
 int64_t spring_workspace_bytes(const de_ode_batch* b) {
   if (!b) return 0;
   if (lane_mapping(b)) {
     // The one-lane solver's state-length vectors per system (offsets are
-    // device memory here, so the bound uses 512 doubles per system) and, with
-    // a point output, n + 1 state offsets into points_y.
-    return (int64_t)lane_solver(b->method).vectors * 512 * 8 * b->n +
+    // device memory here, so the bound uses 512 doubles per system), one more
+    // for an event solout and, with a point output, n + 1 state offsets into
+    // points_y.
+    const int vectors =
+        lane_solver(b->method).vectors + (b->event_solout != DE_SOLOUT_DEFAULT ? 1 : 0);
+    return (int64_t)vectors * 512 * 8 * b->n +
            (b->points != DE_POINTS_NONE ? 8 * (b->n + 1) : 0);
   }
   return (int64_t)sizeof(int32_t) * b->n + (int64_t)sizeof(unsigned long long) * (kNumCaps + 1);
@@ -254,15 +257,19 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
   base.param_offset = b->param_offset;
   base.work = nullptr;
   base.pts_ypos = nullptr;
+  base.ev_work = nullptr;
   base.list = nullptr;
   base.list_len = 0;
   // Host-side size scan: validity, total state length, classes.
   int64_t state_len = 0;
   std::vector<std::vector<int32_t>> lists(kNumCaps);
   std::vector<int64_t> masses(kNumCaps, 0);
+  const bool comp_event =
+      b->event_solout != DE_SOLOUT_DEFAULT && a.ev.fn == kEventComponent;
   for (int64_t i = 0; i < n; ++i) {
     int k = cap_index(host_sizes[i]);
     if (host_sizes[i] < 1 || k < 0) return 1;
+    if (comp_event && b->event_component >= 2 * host_sizes[i]) return 3;
     lists[k].push_back((int32_t)i);
     masses[k] += host_sizes[i];
     state_len += 2 * (int64_t)host_sizes[i];
@@ -270,7 +277,8 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
   if (lane_mapping(b)) {
     const LaneSolver ls = lane_solver(b->method);
     const bool pts = b->points != DE_POINTS_NONE;
-    const int64_t vec_len = (int64_t)ls.vectors * state_len;
+    const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
+    const int64_t vec_len = ((int64_t)ls.vectors + (events ? 1 : 0)) * state_len;
     const int64_t need = vec_len + (pts ? n + 1 : 0);
     if (g_work_len < need) {
       if (g_work) (void)hipFree(g_work);
@@ -280,6 +288,7 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
       g_work_len = need;
     }
     base.work = g_work;
+    if (events) base.ev_work = g_work + (int64_t)ls.vectors * state_len;
     if (pts) {
       // Start of each system's point states in points_y: the slot counts
       // times the state lengths, summed over the systems before it.

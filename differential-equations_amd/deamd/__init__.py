@@ -641,11 +641,16 @@ class Solution:
     def events(self, i):
         """(t[k], y[k, dim]) of the event points kept for trajectory i."""
         k = min(int(self.n_events[i]), self.max_events)
-        dim = len(self.y) // self.n
+        dim = self._dim(i)
         if k == 0:
             return np.empty(0), np.empty((0, dim))
         t = self.event_t.reshape(self.max_events, self.n)[:k, i]
-        y = self.event_y.reshape(self.max_events, dim, self.n)[:k, :, i]
+        if self.state_offset is not None:
+            # spring chains: point k at event_y[max_events * state_offset[i] + k * dim + c]
+            lo = self.max_events * int(self.state_offset[i])
+            y = self.event_y[lo:lo + k * dim].reshape(k, dim)
+        else:
+            y = self.event_y.reshape(self.max_events, dim, self.n)[:k, :, i]
         return t.copy(), y.copy()
 
     _CSV_SOURCES = {"final": DE_CSV_FINAL, "t_eval": DE_CSV_TEVAL, "events": DE_CSV_EVENTS,

@@ -185,7 +185,8 @@ typedef struct de_ode_batch {
    * (src/traits.rs:394-411) and h_init atol[c] + rtol[c]*|y0| (src/methods/h_init.rs:80). */
   const double* rtol_comp;
   const double* atol_comp;
-  /* Event solouts (DE_SOLOUT_*; fixed-dimension families).  The event points of each
+  /* Event solouts (DE_SOLOUT_*; spring chains: component and time events and the
+   * crossing solout, on the one-lane-per-system mapping).  The event points of each
    * trajectory -- the points the reference's event / crossing solout pushes into
    * Solution -- come back in de_ode_result.event_t / event_y. */
   int32_t event_solout;    /* DE_SOLOUT_* */
@@ -219,7 +220,9 @@ typedef struct de_ode_result {
   double* dense;     /* t_eval output (NaN where a point was not produced), or NULL */
   int32_t* n_dense;  /* [n] number of t_eval points produced, or NULL */
   double* event_t;   /* [max_events * n] event times, event_t[k * n + i]; NaN past the last */
-  double* event_y;   /* [max_events * dim * n] event states, event_y[(k * dim + c) * n + i] */
+  double* event_y;   /* [max_events * dim * n] event states, event_y[(k * dim + c) * n + i];
+                        spring chains: event_y[max_events * state_offset[i] + k * 2 sizes[i]
+                        + c], max_events * state_offset[n] values */
   int32_t* n_events; /* [n] event points pushed (those past max_events are not kept), or NULL */
   /* DE_POINTS_*: trajectory i's Solution points go to slots points_offset[i] ..
    * points_offset[i+1] (n + 1 values, given by the caller): time points_t[s], state

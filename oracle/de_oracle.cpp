@@ -326,8 +326,10 @@ struct EventSpec {
   const double* normal = nullptr;  // HYPERPLANE: already normalised
 };
 
-// Largest state dimension with event solouts (fixed-dimension families).
+// Largest state dimension of the fixed-dimension families with event solouts,
+// and the states the solouts interpolate into (spring chains: up to 512).
 constexpr int kEvMax = 8;
+constexpr int kEvBuf = 512;
 
 // Rust f64::signum compared with != (a NaN signum differs from everything).
 static inline bool sign_differs(double a, double b) { return de::signum(a) != de::signum(b); }
@@ -378,7 +380,7 @@ struct EventState {
     double e = d;
     const double one = 1.0, two = 2.0, half = one / two, three = 3.0;
     const double abs_tol = 1e-14, rel_tol = 1e-12;
-    double yb[kEvMax];
+    double yb[kEvBuf];
     for (uint32_t it = 0; it < 50u; ++it) {
       if (fb == 0.0) {
         *root = b;
@@ -468,7 +470,7 @@ struct EventState {
       }
       double t_event;
       if (fa * fb <= 0.0 && brent_dekker(a, b, fa, fb, interp, &t_event)) {
-        double y_event[kEvMax];
+        double y_event[kEvBuf];
         if (!interp(t_event, y_event)) return kPanic;  // interpolate(t_event).unwrap()
         bool push = !tail.has || fabs(t_event - tail.t) > 1e-14;
         if (push) {
@@ -496,7 +498,7 @@ struct EventState {
     const double eps = 2.220446049250313080847e-16;
     double t = t_lower - off_lower * (t_upper - t_lower) / (off_upper - off_lower);
     const double tolerance = eps * 100.0;
-    double yt[kEvMax], yp[kEvMax];
+    double yt[kEvBuf], yp[kEvBuf];
     double offset;
     for (int it = 0; it < 10; ++it) {
       if (!interp(t, yt)) return kPanic;
@@ -555,7 +557,7 @@ struct EventState {
             double frac = -lo / (off - lo);
             tc = t_prev + frac * (t_curr - t_prev);
           }
-          double yc[kEvMax];
+          double yc[kEvBuf];
           if (!interp(tc, yc)) return kPanic;
           record(tc, yc);
         }
@@ -1756,7 +1758,7 @@ static void run_fixed_family(const de_ode_batch* b, de_ode_result* out, const Se
 }
 
 static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings& base,
-                       int threads) {
+                       const EventSpec* es, int threads) {
   const int64_t n = b->n;
   // A point output keeps the states of system i's slots back to back from
   // ypos[i] = sum_{j<i} slots_j * 2 sizes[j].
@@ -1798,11 +1800,28 @@ static void run_spring(const de_ode_batch* b, de_ode_result* out, const Settings
       pts.y = out->points_y + ypos[i];
       pts.cap = out->points_offset[i + 1] - out->points_offset[i];
     }
+    // Event points: time event_t[k * n + i], state at event_y[max_events *
+    // state_offset[i] + k * dim + c]; NaN past the last one kept.
+    const int cap = es ? b->max_events : 0;
+    std::vector<double> et((size_t)cap, NAN), ey((size_t)cap * sys.dim(), NAN);
+    EventOut eo;
+    eo.t = et.data();
+    eo.y = ey.data();
+    eo.cap = cap;
     TrajResult r;
     run_ode_traj(sys, b, st, i, b->y0 + off, out->y_final + off, dense,
-                 out->n_dense ? &out->n_dense[i] : nullptr, nullptr, nullptr, &r,
+                 out->n_dense ? &out->n_dense[i] : nullptr, es, &eo, &r,
                  with_pts ? &pts : nullptr);
     if (with_pts && out->n_points) out->n_points[i] = pts.pushed;
+    if (es) {
+      for (int k = 0; k < cap; ++k) {
+        if (out->event_t) out->event_t[(int64_t)k * n + i] = et[k];
+        if (out->event_y)
+          copy(out->event_y + (int64_t)cap * off + (int64_t)k * sys.dim(),
+               ey.data() + (size_t)k * sys.dim(), sys.dim());
+      }
+      if (out->n_events) out->n_events[i] = eo.pushed;
+    }
     out->t_final[i] = r.t;
     out->accepted[i] = r.accepted;
     out->rejected[i] = r.rejected;
@@ -2025,7 +2044,11 @@ int de_oracle_ode_solve(const de_ode_batch* b, de_ode_result* out, int pow_mode,
   std::vector<double> normal;
   const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
   if (events) {
-    if (b->family == DE_FAMILY_SPRING_CHAIN || b->dim > kEvMax) return DE_ERR_UNSUPPORTED;
+    if (b->family == DE_FAMILY_SPRING_CHAIN
+            ? b->event_solout == DE_SOLOUT_HYPERPLANE ||
+                  (b->event_solout == DE_SOLOUT_EVENT && b->event_fn == DE_EVENT_PLANE)
+            : b->dim > kEvMax)
+      return DE_ERR_UNSUPPORTED;
     es.solout = b->event_solout;
     es.fn = b->event_fn;
     es.comp = b->event_component;
@@ -2056,7 +2079,7 @@ int de_oracle_ode_solve(const de_ode_batch* b, de_ode_result* out, int pow_mode,
     case DE_FAMILY_HARMONIC: run_fixed_family<de::HarmonicRhs>(b, out, st, esp, threads); break;
     case DE_FAMILY_LOGISTIC: run_fixed_family<de::LogisticRhs>(b, out, st, esp, threads); break;
     case DE_FAMILY_ROBERTSON: run_fixed_family<de::RobertsonRhs>(b, out, st, esp, threads); break;
-    case DE_FAMILY_SPRING_CHAIN: run_spring(b, out, st, threads); break;
+    case DE_FAMILY_SPRING_CHAIN: run_spring(b, out, st, esp, threads); break;
     default: return DE_ERR_UNSUPPORTED;
   }
   return DE_OK;

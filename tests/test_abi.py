@@ -102,12 +102,16 @@ def test_invalid_event_descriptors_are_rejected(kind, mutate, code):
     assert deamd.lib().de_last_error()
 
 
-def test_event_solouts_are_refused_for_spring_chains():
+def test_plane_events_are_refused_for_spring_chains():
+    # Spring chains take component and time events and the crossing solout;
+    # a plane needs one state length for the whole ensemble.
     sysm, y0 = deamd.gen_spring(8, seed=1)
-    ens = deamd.IVP.ode(sysm, 0.0, 1.0, y0).event(deamd.Event.time(0.5))
-    b, res, keep = ens.batch()
-    assert deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None) == \
-        _abi.DE_ERR_UNSUPPORTED
+    mk = lambda: deamd.IVP.ode(sysm, 0.0, 1.0, y0)
+    for ens in (mk().event(deamd.Event.plane(np.zeros(8), np.ones(8))),
+                mk().hyperplane_crossing(np.zeros(8), np.ones(8))):
+        b, res, keep = ens.batch()
+        assert deamd.lib().de_ode_batch_solve(C.byref(b), C.byref(res.struct()), None) == \
+            _abi.DE_ERR_UNSUPPORTED
 
 
 def test_empty_batch_is_a_no_op():

@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 import deamd
-from deamd import DE_MAP_AUTO, DE_MAP_GROUP, DE_MAP_LANE, DE_STATUS_COMPLETE, ExplicitRungeKutta, IVP
+from deamd import (DE_MAP_AUTO, DE_MAP_GROUP, DE_MAP_LANE, DE_STATUS_COMPLETE, CrossingDirection,
+                   Event, EventConfig, ExplicitRungeKutta, IVP)
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
@@ -298,3 +299,76 @@ def test_point_outputs_backward_and_short_slots(orc):
         ft, fy = full.points(i)
         assert len(t) == cap[i]
         assert np.array_equal(t, ft[:cap[i]]) and np.array_equal(y, fy[:cap[i]])
+
+
+def same_events(got, want):
+    assert np.array_equal(got.n_events, want.n_events)
+    assert np.array_equal(np.isnan(got.event_t), np.isnan(want.event_t))
+    assert np.array_equal(np.nan_to_num(got.event_t), np.nan_to_num(want.event_t))
+    assert np.array_equal(np.isnan(got.event_y), np.isnan(want.event_y))
+    assert np.array_equal(np.nan_to_num(got.event_y), np.nan_to_num(want.event_y))
+
+
+EVENT_KINDS = {
+    "component": lambda e: e.event(Event.component(1, 0.0)).options(max_events=16),
+    "terminate": lambda e: e.event(Event.component(
+        0, 0.0, EventConfig(CrossingDirection.Both).terminate_after(2))).options(max_events=4),
+    "positive": lambda e: e.event(Event.component(
+        1, 0.0, EventConfig(CrossingDirection.Positive))).options(max_events=4),
+    "time_terminal_teval": lambda e: e.t_eval(np.linspace(0.0, 2.0, 9)).event(
+        Event.time(1.25, EventConfig().terminal())).options(max_events=2),
+    "teval_component": lambda e: e.t_eval(np.linspace(-0.5, 2.5, 31)).event(
+        Event.component(0, 0.0, EventConfig(CrossingDirection.Negative))).options(max_events=8),
+    "crossing": lambda e: e.crossing(0, 0.0, CrossingDirection.Both).options(max_events=1),
+}
+
+
+@pytest.mark.parametrize("kind", list(EVENT_KINDS))
+@pytest.mark.parametrize("method", ["dopri5", "dop853", "rk4", "rkv767e"])
+def test_event_solouts_bit_exact(orc, method, kind):
+    # Event and crossing solouts on spring chains (one lane per system, roots
+    # searched inline, event states back to back per system) equal the oracle
+    # bit for bit; a few slots per system, so some points are counted and not
+    # kept.
+    sysm, y0 = deamd.gen_spring(300, seed=37)
+    m = {"dopri5": lambda: ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9),
+         "dop853": lambda: ExplicitRungeKutta.dop853().rtol(1e-8).atol(1e-10),
+         "rk4": lambda: ExplicitRungeKutta.rk4(0.01),
+         "rkv767e": lambda: ExplicitRungeKutta.rkv767e().rtol(1e-8).atol(1e-10)}[method]
+    ens = EVENT_KINDS[kind](IVP.ode(sysm, 0.0, 2.0, y0).method(m().max_steps(200000)))
+    want = orc.solve_ode(ens, threads=THREADS)
+    if kind == "time_terminal_teval":
+        # the root lands on the t_eval point 1.25 just pushed: counted, mostly
+        # not pushed, and the solve stops there
+        assert (want.status == deamd.DE_STATUS_INTERRUPTED).all()
+    else:
+        assert (want.n_events > 0).mean() > 0.25
+    for mapping in (DE_MAP_AUTO, DE_MAP_LANE):
+        got = ens.options(mapping=mapping).solve()
+        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+            assert np.array_equal(getattr(got, f), getattr(want, f)), (f, mapping)
+        same_events(got, want)
+        if kind in ("time_terminal_teval", "teval_component"):
+            assert np.array_equal(got.n_dense, want.n_dense)
+            assert np.array_equal(np.isnan(got.dense), np.isnan(want.dense))
+            assert np.array_equal(np.nan_to_num(got.dense), np.nan_to_num(want.dense))
+
+
+def test_event_solouts_backward_and_refusals(orc):
+    # Backward: brent's interpolations fail (no root) and the crossing solout's
+    # unwrap ends the solve OutOfBounds, as the oracle; plane events and an
+    # event component past a system's state are refused.
+    sysm, y0 = deamd.gen_spring(200, seed=38)
+    mk = lambda: IVP.ode(sysm, 0.0, -1.5, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9))
+    for ens in (mk().event(Event.component(1, 0.0)).options(max_events=8),
+                mk().crossing(1, 0.0).options(max_events=8)):
+        want = orc.solve_ode(ens, threads=THREADS)
+        got = ens.solve()
+        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+            assert np.array_equal(getattr(got, f), getattr(want, f)), f
+        same_events(got, want)
+    with pytest.raises(RuntimeError):
+        mk().event(Event.plane(np.zeros(8), np.ones(8))).solve()
+    with pytest.raises(RuntimeError):
+        mk().event(Event.component(2 * int(sysm.sizes.min()), 0.0)).solve()
