@@ -34,6 +34,8 @@ struct SpringArgs {
                                // kVectors * state_offset
   const int64_t* pts_ypos;     // point output: start of system i's states in pts_y
   double* ev_work;             // event solouts: one state per system at state_offset
+  double* ystate;              // one-lane solvers' working state at state_offset, or
+                               // null: y_final (which the result then needs no copy to)
   const int32_t* list;         // GroupDopri5: system indices of this size class
   int64_t list_len;
 };
@@ -63,7 +65,7 @@ struct VecCommon {
   int64_t i;
   int nm, n;  // masses, state length
   const double* p;
-  double* y;  // y_final slot doubles as the working state
+  double* y;  // working state: y_final's slot, or ystate's
   double t, h, tf, dir;
   double last;  // EvenSolout::last_output_t
   int32_t next_eval, pushed;
@@ -76,6 +78,10 @@ struct VecCommon {
 
   DE_HD void write(int32_t st, int64_t acc, int64_t rej, int64_t ev) {
     const OdeArgs& a = A->o;
+    if (A->ystate) {
+      double* yf = a.y_final + A->state_offset[i];
+      for (int c = 0; c < n; ++c) yf[c] = y[c];
+    }
     a.t_final[i] = t;
     a.accepted[i] = acc;
     a.rejected[i] = rej;
@@ -357,7 +363,7 @@ struct VecCommon {
     n = 2 * nm;
     p = a.params + A->param_offset[i];
     const int64_t so = A->state_offset[i];
-    y = a.y_final + so;
+    y = (A->ystate ? A->ystate : a.y_final) + so;
     const double* y0 = a.y0 + so;
     for (int c = 0; c < n; ++c) y[c] = y0[c];
     const double t0 = a.t0 ? a.t0[i] : a.t0s;

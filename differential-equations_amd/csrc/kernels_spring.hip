@@ -258,6 +258,7 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
   base.work = nullptr;
   base.pts_ypos = nullptr;
   base.ev_work = nullptr;
+  base.ystate = nullptr;
   base.list = nullptr;
   base.list_len = 0;
   // Host-side size scan: validity, total state length, classes.

@@ -42,159 +42,10 @@
 
 #include "../../include/de_api.h"
 #include "de_launch.h"
-#include "de_spring_solvers.h"
+#include "de_server.h"
 
 namespace de {
 namespace serve {
-
-constexpr int kMaxReq = 8192;     // requests resident at once
-constexpr int kMaxRanges = 1 << 20;
-constexpr int kChunk = 256;       // attempts per slot per launch
-constexpr int kBlock = 256;
-
-struct Range {
-  int64_t start;       // first ticket
-  int32_t req;         // request slot
-  int32_t first;       // first trajectory (index into list when list != null)
-  int32_t count;
-  int32_t pad;
-  const int32_t* list; // spring: system indices of this size class
-};
-
-struct Queue {
-  unsigned long long next;  // next ticket to hand out
-  int64_t published;        // tickets below this are backed by a range
-  int64_t nranges;
-};
-
-struct ReqDev {
-  SpringArgs a;  // lane families use a.o only
-  int32_t remaining;
-  int32_t pad;
-};
-
-template <class L>
-struct LaneSlot {
-  L s;
-  int64_t ticket;
-  int32_t req, traj, state;  // state 0 idle, 1 holding ticket, 2 running
-};
-
-DE_DEV int find_range(const Range* r, int64_t n, int64_t ticket) {
-  int64_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int64_t mid = (lo + hi + 1) >> 1;
-    if (r[mid].start <= ticket) lo = mid;
-    else hi = mid - 1;
-  }
-  return (int)lo;
-}
-
-template <class L>
-__global__ __launch_bounds__(kBlock) void chunk_lane(Queue* q, const Range* ranges, ReqDev* reqs,
-                                                    LaneSlot<L>* slots, int nslots, int k_iters) {
-  const int slot = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = slot < nslots;
-  const int lane = __lane_id();
-  LaneSlot<L> S;
-  if (valid) S = slots[slot];
-  else S.state = -1;
-  const int64_t published = *(volatile int64_t*)&q->published;
-  const int64_t nranges = *(volatile int64_t*)&q->nranges;
-  for (int it = 0; it < k_iters; ++it) {
-    bool need = S.state == 0;
-    unsigned long long mask = __ballot(need);
-    if (mask) {
-      int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(&q->next, (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader);
-      if (need) {
-        S.ticket = (int64_t)(base + __popcll(mask & __lanemask_lt()));
-        S.state = 1;
-      }
-    }
-    if (S.state == 1 && S.ticket < published) {
-      int r = find_range(ranges, nranges, S.ticket);
-      S.req = ranges[r].req;
-      S.traj = ranges[r].first + (int32_t)(S.ticket - ranges[r].start);
-      if (S.s.init(reqs[S.req].a.o, S.traj)) {
-        S.state = 2;
-      } else {
-        atomicSub(&reqs[S.req].remaining, 1);
-        S.state = 0;
-      }
-    }
-    bool act = S.state == 2;
-    if (!__any(act)) {
-      if (!__any(S.state == 0 || (S.state == 1 && S.ticket < published))) break;
-      continue;
-    }
-    if (act && S.s.advance(reqs[S.req].a.o, S.traj)) {
-      atomicSub(&reqs[S.req].remaining, 1);
-      S.state = 0;
-    }
-  }
-  if (valid) slots[slot] = S;
-}
-
-// Occupancy hints: the rarely taken double-double pow fallback raises the
-// peak register count past a wave-slot boundary; the hints trade that for a
-// few spills outside the step block.
-template <int G, int C>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(C == 1 ? 3 : 2))) void
-chunk_group(Queue* q, const Range* ranges,
-                                                     ReqDev* reqs,
-                                                     LaneSlot<GroupDopri5<G, C>>* slots,
-                                                     int nslots, int k_iters) {
-  const int slot = blockIdx.x * kBlock + threadIdx.x;  // one per lane; G lanes per system
-  const bool valid = slot < nslots;
-  const int lane = __lane_id();
-  const int l = lane & (G - 1);
-  const int leader_lane = lane - l;
-  LaneSlot<GroupDopri5<G, C>> S;
-  if (valid) S = slots[slot];
-  else S.state = -1;
-  const int64_t published = *(volatile int64_t*)&q->published;
-  const int64_t nranges = *(volatile int64_t*)&q->nranges;
-  for (int it = 0; it < k_iters; ++it) {
-    bool need = S.state == 0;
-    unsigned long long mask = __ballot(need && l == 0);
-    if (mask) {
-      int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(&q->next, (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader);
-      if (need) {
-        unsigned long long below =
-            leader_lane == 0 ? 0ull : (mask & ((1ull << leader_lane) - 1ull));
-        S.ticket = (int64_t)(base + __popcll(below));
-        S.state = 1;
-      }
-    }
-    if (S.state == 1 && S.ticket < published) {
-      int r = find_range(ranges, nranges, S.ticket);
-      S.req = ranges[r].req;
-      S.traj = ranges[r].list[ranges[r].first + (int32_t)(S.ticket - ranges[r].start)];
-      if (S.s.init(reqs[S.req].a, S.traj, l)) {
-        S.state = 2;
-      } else {
-        if (l == 0) atomicSub(&reqs[S.req].remaining, 1);
-        S.state = 0;
-      }
-    }
-    bool act = S.state == 2;
-    if (!__any(act)) {
-      if (!__any(S.state == 0 || (S.state == 1 && S.ticket < published))) break;
-      continue;
-    }
-    if (act && S.s.advance(reqs[S.req].a)) {
-      if (l == 0) atomicSub(&reqs[S.req].remaining, 1);
-      S.state = 0;
-    }
-  }
-  if (valid) slots[slot] = S;
-}
 
 // Publishes a pool's new work between two iterations: the chunk kernels read
 // the counters once at their start.
@@ -208,77 +59,6 @@ __global__ void commit_pool(Queue* q, int64_t published, int64_t nranges) {
 __global__ void gather_remaining(const ReqDev* reqs, int32_t* out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = reqs[i].remaining;
-}
-
-// Reset of a drained pool (no request with work in it): slots that hold an
-// unbacked ticket (state 1) become idle and the ticket and range counters
-// restart from zero, so the range table is reused.
-template <class Slot>
-__global__ void reset_pool_kernel(Slot* s, int n, Queue* q) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && s[i].state == 1) {
-    s[i].state = 0;
-    s[i].ticket = 0;
-  }
-  if (i == 0) {
-    q->next = 0;
-    q->published = 0;
-    q->nranges = 0;
-  }
-}
-
-template <class Slot>
-__global__ void init_slots(Slot* s, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    s[i].state = 0;
-    s[i].ticket = 0;
-    s[i].req = 0;
-    s[i].traj = 0;
-  }
-}
-
-// ---------------------------------------------------------------- host -----
-typedef void (*ChunkFn)(Queue*, const Range*, ReqDev*, void*, int, int);
-
-struct Family {
-  const char* name;
-  int cap;         // spring: class cap; 0 for lane families
-  int g;           // lanes per system (1 for lane families)
-  size_t slot_bytes;
-  int nslots;      // lanes
-  void* slots;
-  Queue* q;        // device
-  Range* ranges;   // device
-  std::vector<Range> host_ranges;
-  int64_t published = 0;
-  int64_t active_tickets = 0;  // published but possibly unfinished
-  int active_reqs = 0;         // active requests with work in this pool
-  int64_t max_ranges = 0;      // capacity of the range table
-  bool dirty = false;          // published work not yet committed to the device
-  hipStream_t stream;
-  void (*launch)(Family&, ReqDev*, int, hipStream_t);
-  void (*reset)(Family&, hipStream_t);
-};
-
-template <class Slot>
-void reset_pool(Family& f, hipStream_t s) {
-  hipLaunchKernelGGL(reset_pool_kernel<Slot>, dim3((f.nslots + 255) / 256), dim3(256), 0, s,
-                     (Slot*)f.slots, f.nslots, f.q);
-}
-
-template <class L>
-void launch_lane(Family& f, ReqDev* reqs, int k, hipStream_t s) {
-  int grid = (f.nslots + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(chunk_lane<L>, dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
-                     (LaneSlot<L>*)f.slots, f.nslots, k);
-}
-
-template <int G, int C>
-void launch_group(Family& f, ReqDev* reqs, int k, hipStream_t s) {
-  int grid = (f.nslots + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((chunk_group<G, C>), dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
-                     (LaneSlot<GroupDopri5<G, C>>*)f.slots, f.nslots, k);
 }
 
 // First-fit arena over one allocation: device memory (inputs) or pinned,
@@ -348,6 +128,7 @@ struct Request {
   std::vector<int32_t> sizes;
   std::vector<int64_t> so, po;
   int64_t state_len = 0;
+  int vectors = 0;  // spring chains on one lane per system: workspace vectors
   // results: written by the kernels into the host arena at these pointers
   de::OdeArgs out;
   // device inputs and host-arena results
@@ -355,7 +136,7 @@ struct Request {
   size_t dbytes = 0;
   void* hmem = nullptr;
   size_t hbytes = 0;
-  uint32_t pools = 0;  // bit k: work published to pool k
+  uint64_t pools = 0;  // bit k: work published to pool k
   int slot = -1;
   int64_t first_iter = 0;  // first iteration whose gathered counts include this request
   bool done = false;
@@ -384,6 +165,8 @@ using namespace de::serve;
 struct de_server {
   size_t budget = 0;
   int32_t flags = 0;
+  int ncu = 256;
+  int64_t max_ranges = 0;
   Arena arena;   // device: request inputs
   Arena harena;  // pinned host: request results
   std::vector<Family> fams;
@@ -424,29 +207,66 @@ namespace {
 
 enum : int32_t { kServerGraph = 1 };
 
-// Pools: 0 / 1 Lorenz / Van der Pol with DOPRI5, 2 .. kSpringEnd - 1 the
-// spring classes, then Lorenz / Van der Pol with DOP853, then DOPRI5 with an
-// event solout (EventWrappedSolout, or the crossing / hyperplane solouts),
-// whose roots are found inline.
+// Pools made at creation: 0 / 1 Lorenz / Van der Pol with DOPRI5, 2 ..
+// kSpringEnd - 1 the spring classes (DOPRI5, group mappings), then Lorenz /
+// Van der Pol with DOP853, then DOPRI5 with an event solout
+// (EventWrappedSolout, or the crossing / hyperplane solouts), whose roots are
+// found inline.  Pools made on first use (server_pools.hip), their memory
+// carved from the arena: Lorenz / Van der Pol with RK4, Euler and the
+// embedded family at kLaneLazy + 2 method + (Van der Pol), and spring chains
+// one lane per system at kSpringVec + method (every method but DOPRI5, and
+// DOPRI5 with an event solout).
 constexpr int kSpringEnd = 9;
 constexpr int kLorenzDop853 = 9, kVdpDop853 = 10;
 constexpr int kLorenzEvent = 11, kLorenzCross = 12, kVdpEvent = 13, kVdpCross = 14;
+constexpr int kEagerPools = 15;
+constexpr int kMethods = DE_METHOD_RKV989E + 1;
+constexpr int kLaneLazy = kEagerPools;
+constexpr int kSpringVec = kLaneLazy + 2 * kMethods;
+constexpr int kPools = kSpringVec + kMethods;
+static_assert(kPools <= 64, "a request's pools are a 64-bit mask");
+constexpr int kLazyRanges = 1 << 16;  // range table of a lazy pool
 
+bool lane_lazy_method(int method) {
+  return method == DE_METHOD_RK4 || method == DE_METHOD_EULER || method >= DE_METHOD_RKF45;
+}
+
+// The pool of a request (spring chains: of its systems with nm masses), or -1.
 int fam_index_for(de_server* s, int family, int nm, int method, int solout) {
+  if (method < 0 || method >= kMethods) return -1;
   const bool d853 = method == DE_METHOD_DOP853;
-  if (solout != DE_SOLOUT_DEFAULT) {
-    const bool cross = solout != DE_SOLOUT_EVENT;
-    if (family == DE_FAMILY_LORENZ) return cross ? kLorenzCross : kLorenzEvent;
-    if (family == DE_FAMILY_VAN_DER_POL) return cross ? kVdpCross : kVdpEvent;
-    return -1;
-  }
-  if (family == DE_FAMILY_LORENZ) return d853 ? kLorenzDop853 : 0;
-  if (family == DE_FAMILY_VAN_DER_POL) return d853 ? kVdpDop853 : 1;
   if (family == DE_FAMILY_SPRING_CHAIN) {
+    if (method != DE_METHOD_DOPRI5 || solout != DE_SOLOUT_DEFAULT) return kSpringVec + method;
     for (int k = 2; k < kSpringEnd; ++k)
       if (nm <= s->fams[k].cap) return k;
+    return -1;
   }
-  return -1;
+  if (family != DE_FAMILY_LORENZ && family != DE_FAMILY_VAN_DER_POL) return -1;
+  const int v = family == DE_FAMILY_VAN_DER_POL;
+  if (solout != DE_SOLOUT_DEFAULT) {
+    if (method != DE_METHOD_DOPRI5) return -1;
+    const bool cross = solout != DE_SOLOUT_EVENT;
+    return v ? (cross ? kVdpCross : kVdpEvent) : (cross ? kLorenzCross : kLorenzEvent);
+  }
+  if (method == DE_METHOD_DOPRI5) return v;
+  if (d853) return v ? kVdpDop853 : kLorenzDop853;
+  return kLaneLazy + 2 * method + v;
+}
+
+// The slot kind and chunk kernel of lazy pool k (family and method from its index).
+bool lazy_kind(int k, Family& f) {
+  int family, method;
+  if (k >= kSpringVec) {
+    family = DE_FAMILY_SPRING_CHAIN;
+    method = k - kSpringVec;
+  } else {
+    family = (k - kLaneLazy) & 1 ? DE_FAMILY_VAN_DER_POL : DE_FAMILY_LORENZ;
+    method = (k - kLaneLazy) >> 1;
+    if (!lane_lazy_method(method)) return false;
+  }
+  return lazy_pool_kind_0(family, method, f) || lazy_pool_kind_1(family, method, f) ||
+         lazy_pool_kind_2(family, method, f) || lazy_pool_kind_3(family, method, f) ||
+         lazy_pool_kind_4(family, method, f) || lazy_pool_kind_5(family, method, f);
 }
 
 // Device bytes of a request's inputs (and spring class lists) and host bytes
@@ -459,6 +279,9 @@ void footprint(const Request* r, size_t* dev, size_t* host) {
   d += r->rtol.size() * 8 + r->atol.size() * 8 + r->t_eval.size() * 8;
   d += r->sizes.size() * 4 + (r->so.size() + r->po.size()) * 8 + n * 4;  // + class lists
   d += (r->ev_value.size() + r->ev_point.size() + r->ev_normal.size()) * 8;
+  // one-lane spring solvers: their vectors, the working state and, with an
+  // event solout, the event search's state
+  d += (size_t)r->vectors * r->state_len * 8;
   h += (size_t)r->state_len * 8 + n * (8 * 4 + 4 + 4);
   h += (size_t)b.n_eval * r->state_len * 8;
   if (b.event_solout != DE_SOLOUT_DEFAULT)
@@ -509,6 +332,12 @@ bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_l
     stage_copy(s, d, src, nb);
     p += (nb + 255) & ~size_t(255);
     return d;
+  };
+  auto reserve = [&](size_t nb) -> double* {  // device workspace, not initialised
+    if (!nb) return nullptr;
+    void* d = p;
+    p += (nb + 255) & ~size_t(255);
+    return (double*)d;
   };
   auto take = [&](size_t nb) -> void* {  // results: the host arena
     if (!nb) return nullptr;
@@ -568,6 +397,13 @@ bool upload(de_server* s, Request* r, std::vector<std::vector<int32_t>>& class_l
     a.ev.y_out = (double*)take((size_t)b.max_events * r->state_len * 8);
     a.ev.n_out = (int32_t*)take(n * 4);
   }
+  if (r->vectors > 0) {  // one lane per system: working state, event state, vectors
+    const size_t sl = (size_t)r->state_len * 8;
+    const int ev = b.event_solout != DE_SOLOUT_DEFAULT;
+    rd.a.ystate = reserve(sl);
+    rd.a.ev_work = ev ? reserve(sl) : nullptr;
+    rd.a.work = reserve((size_t)(r->vectors - 1 - ev) * sl);
+  }
   if (b.family == DE_FAMILY_SPRING_CHAIN) {
     rd.a.sizes = (const int32_t*)put(r->sizes.data(), r->sizes.size() * 4);
     rd.a.state_offset = (const int64_t*)put(r->so.data(), r->so.size() * 8);
@@ -601,10 +437,13 @@ void publish(de_server* s, Family& f, int req_slot, int32_t first, int32_t count
 // One chunk of every pool (graph capture), or of the pools that hold work of
 // an active request: an idle pool's launch would only read and write back its
 // slots.
-void launch_all(de_server* s, bool busy_only = false) {
-  (void)hipEventRecord(s->fork, s->ctl);
+// With lazy_only, the pools made on first use (outside a captured graph),
+// forked from the event recorded before the graph's launch.
+void launch_all(de_server* s, bool busy_only = false, bool lazy_only = false) {
+  if (!lazy_only) (void)hipEventRecord(s->fork, s->ctl);
   for (size_t k = 0; k < s->fams.size(); ++k) {
     Family& f = s->fams[k];
+    if (!f.slots || (lazy_only && !f.lazy)) continue;
     if (busy_only && f.active_reqs == 0) continue;
     (void)hipStreamWaitEvent(f.stream, s->fork, 0);
     f.launch(f, s->d_reqs, kChunk, f.stream);
@@ -624,6 +463,32 @@ bool capture(de_server* s) {
   return ok;
 }
 
+// Makes lazy pool k: slots, queue and range table from the arena, slots
+// initialised and the queue cleared on the control stream (before any of the
+// pool's launches, which fork from it).
+bool make_lazy(de_server* s, int k) {
+  Family& f = s->fams[k];
+  if (!lazy_kind(k, f)) return false;
+  const size_t sb = f.slot_bytes * (size_t)f.nslots, rb = sizeof(Range) * (size_t)f.max_ranges;
+  void* slots = s->arena.alloc(sb);
+  void* q = slots ? s->arena.alloc(sizeof(Queue)) : nullptr;
+  void* ranges = q ? s->arena.alloc(rb) : nullptr;
+  bool ok = ranges != nullptr;
+  if (ok && !f.stream) ok = hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking) == hipSuccess;
+  if (!ok) {
+    if (ranges) s->arena.release(ranges, rb);
+    if (q) s->arena.release(q, sizeof(Queue));
+    if (slots) s->arena.release(slots, sb);
+    return false;
+  }
+  f.slots = slots;
+  f.q = (Queue*)q;
+  f.ranges = (Range*)ranges;
+  f.init(f, s->ctl);
+  f.reset(f, s->ctl);  // the queue's counters from zero
+  return true;
+}
+
 void scheduler(de_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   int64_t inflight = -1;  // last queued iteration whose counts are not read yet
@@ -636,15 +501,32 @@ void scheduler(de_server* s) {
       std::vector<std::vector<int32_t>> lists(s->fams.size());
       const de_ode_batch& b = r->b;
       std::vector<int> needs;
-      if (b.family == DE_FAMILY_SPRING_CHAIN) {
+      const int k0 = fam_index_for(s, b.family, 0, b.method, b.event_solout);
+      const bool classes = b.family == DE_FAMILY_SPRING_CHAIN && k0 < kSpringVec;
+      if (classes) {
         for (int64_t i = 0; i < b.n; ++i)
           lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i], b.method, DE_SOLOUT_DEFAULT)]
               .push_back((int32_t)i);
         for (int k = 2; k < kSpringEnd; ++k)
           if (!lists[k].empty()) needs.push_back(k);
       } else {
-        needs.push_back(fam_index_for(s, b.family, 0, b.method, b.event_solout));
+        needs.push_back(k0);
       }
+      // A pool made on first use takes its memory from the arena.
+      bool unmade = false;
+      for (int k : needs)
+        if (!s->fams[k].slots && !make_lazy(s, k)) unmade = true;
+      if (unmade) {
+        if (!s->active.empty()) break;  // after the running requests release memory
+        s->pending.pop_front();
+        r->error = DE_ERR_NOMEM;
+        r->done = true;
+        r->t_done = now_s();
+        s->cv_done.notify_all();
+        continue;
+      }
+      if (k0 >= kSpringVec)
+        r->vectors = s->fams[k0].vectors + 1 + (b.event_solout != DE_SOLOUT_DEFAULT ? 1 : 0);
       // A pool whose range table is full takes no new work until it drains;
       // then (no kernel is in flight between iterations) its table restarts.
       bool held = false;
@@ -683,19 +565,19 @@ void scheduler(de_server* s) {
       s->active[r->slot] = r;
       r->first_iter = s->launched;
       admitted.push_back(r);
-      if (b.family == DE_FAMILY_SPRING_CHAIN) {
+      if (classes) {
         for (int k = 2; k < kSpringEnd; ++k)
           if (!lists[k].empty()) {
             publish(s, s->fams[k], r->slot, 0, (int32_t)lists[k].size(), d_lists[k]);
-            r->pools |= 1u << k;
+            r->pools |= 1ull << k;
           }
       } else {
         const int k = needs[0];
         publish(s, s->fams[k], r->slot, 0, (int32_t)b.n, nullptr);
-        r->pools |= 1u << k;
+        r->pools |= 1ull << k;
       }
       for (size_t k = 0; k < s->fams.size(); ++k)
-        if ((r->pools >> k) & 1u) s->fams[k].active_reqs += 1;
+        if ((r->pools >> k) & 1ull) s->fams[k].active_reqs += 1;
     }
     // The uploads and range entries are on the device before the commit
     // kernels, which run after the iteration in flight (stream order on ctl).
@@ -733,7 +615,9 @@ void scheduler(de_server* s) {
     // Queue iteration k; its counts land in h_counts[k & 1].
     const int64_t k = s->launched++;
     if (s->graph) {
+      (void)hipEventRecord(s->fork, s->ctl);
       (void)hipGraphLaunch(s->graph, s->ctl);
+      launch_all(s, true, true);
     } else {
       launch_all(s, true);
     }
@@ -776,7 +660,7 @@ void scheduler(de_server* s) {
       for (int slot : finished) {
         Request* r = s->active[slot];
         for (size_t q = 0; q < s->fams.size(); ++q)
-          if ((r->pools >> q) & 1u) s->fams[q].active_reqs -= 1;
+          if ((r->pools >> q) & 1ull) s->fams[q].active_reqs -= 1;
         s->arena.release(r->dmem, r->dbytes);  // the results stay until delivered
         r->dmem = nullptr;
         r->done = true;
@@ -794,11 +678,9 @@ void scheduler(de_server* s) {
 template <class Slot>
 bool make_pool(Family& f, int nslots) {
   f.nslots = nslots;
-  f.slot_bytes = sizeof(Slot);
-  f.reset = reset_pool<Slot>;
+  set_pool_kind<Slot>(f);
   if (hipMalloc(&f.slots, sizeof(Slot) * (size_t)nslots) != hipSuccess) return false;
-  hipLaunchKernelGGL(init_slots<Slot>, dim3((nslots + 255) / 256), dim3(256), 0, 0,
-                     (Slot*)f.slots, nslots);
+  f.init(f, 0);
   if (hipMalloc(&f.q, sizeof(de::serve::Queue)) != hipSuccess) return false;
   if (hipMemset(f.q, 0, sizeof(de::serve::Queue)) != hipSuccess) return false;
   if (hipMalloc(&f.ranges, sizeof(Range) * (size_t)f.max_ranges) != hipSuccess) return false;
@@ -826,6 +708,8 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  s->ncu = ncu;
+  s->max_ranges = max_ranges;
   // Slot pools: Lorenz and Van der Pol lanes, one pool per spring class, and
   // the DOP853 lanes of Lorenz and Van der Pol.
   struct Spec {
@@ -838,13 +722,20 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
                         {"lorenz_dop853", 0, 1}, {"vdp_dop853", 0, 1},
                         {"lorenz_event", 0, 1},  {"lorenz_cross", 0, 1},
                         {"vdp_event", 0, 1},     {"vdp_cross", 0, 1}};
-  static_assert(sizeof specs / sizeof specs[0] == kVdpCross + 1, "pool indices");
-  s->fams.resize(sizeof specs / sizeof specs[0]);
+  static_assert(sizeof specs / sizeof specs[0] == kEagerPools, "pool indices");
+  s->fams.resize(kPools);
   const int lane_slots = ncu * 4 * 256;  // 4 blocks of 256 lanes per CU
   const int event_slots = ncu * 256;     // event requests: one block per CU
   const int group_lanes = ncu * 2 * 256;
   bool ok = true;
-  for (size_t k = 0; k < s->fams.size() && ok; ++k) {
+  for (int k = kEagerPools; k < kPools; ++k) {  // made on first use
+    Family& f = s->fams[k];
+    f.name = k >= kSpringVec ? "spring_lane" : "lane";
+    f.lazy = true;
+    f.nslots = event_slots;
+    f.max_ranges = std::min<int64_t>(max_ranges, kLazyRanges);
+  }
+  for (size_t k = 0; k < (size_t)kEagerPools && ok; ++k) {
     Family& f = s->fams[k];
     f.name = specs[k].name;
     f.cap = specs[k].cap;
@@ -938,31 +829,34 @@ int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
   if (b->family != DE_FAMILY_LORENZ && b->family != DE_FAMILY_VAN_DER_POL &&
       b->family != DE_FAMILY_SPRING_CHAIN)
     return DE_ERR_UNSUPPORTED;
-  // DOPRI5 for every served family; DOP853 for the lane-mapped ones.
-  if (b->method != DE_METHOD_DOPRI5 &&
-      !(b->method == DE_METHOD_DOP853 && b->family != DE_FAMILY_SPRING_CHAIN))
-    return DE_ERR_UNSUPPORTED;
+  // Every method for every served family.
+  if (b->method < DE_METHOD_DOPRI5 || b->method > DE_METHOD_RKV989E) return DE_ERR_INVALID;
   if (b->n <= 0 || !b->y0 || (b->n_eval > 0 && !b->t_eval)) return DE_ERR_INVALID;
   // Served requests carry scalar or per-trajectory tolerances and the final
-  // state / t_eval output, or (DOPRI5 on Lorenz and Van der Pol) an event
-  // solout, validated as de_ode_batch_solve validates it; per-component
-  // vectors go through de_ode_batch_solve.
+  // state / t_eval output, or an event solout (DOPRI5 on Lorenz and Van der
+  // Pol; every method on spring chains, without planes), validated as
+  // de_ode_batch_solve validates it; per-component vectors go through
+  // de_ode_batch_solve.
   if (b->rtol_comp || b->atol_comp) return DE_ERR_UNSUPPORTED;
   if (b->points != DE_POINTS_NONE) return DE_ERR_UNSUPPORTED;  // final state, t_eval or events
   const int ev_fn = b->event_solout == DE_SOLOUT_CROSSING     ? DE_EVENT_COMPONENT
                     : b->event_solout == DE_SOLOUT_HYPERPLANE ? DE_EVENT_PLANE
                                                                : b->event_fn;
   if (b->event_solout != DE_SOLOUT_DEFAULT) {
-    if (b->family == DE_FAMILY_SPRING_CHAIN || b->method != DE_METHOD_DOPRI5)
+    if (b->family != DE_FAMILY_SPRING_CHAIN && b->method != DE_METHOD_DOPRI5)
       return DE_ERR_UNSUPPORTED;
     if (b->event_solout < DE_SOLOUT_EVENT || b->event_solout > DE_SOLOUT_HYPERPLANE)
       return DE_ERR_INVALID;
+    if (b->family == DE_FAMILY_SPRING_CHAIN && ev_fn == DE_EVENT_PLANE)
+      return DE_ERR_UNSUPPORTED;
     if (b->event_solout != DE_SOLOUT_EVENT && b->n_eval > 0) return DE_ERR_INVALID;
     if (b->event_direction < DE_CROSS_BOTH || b->event_direction > DE_CROSS_NEGATIVE)
       return DE_ERR_INVALID;
     if (b->event_terminate < 0 || b->max_events < 0) return DE_ERR_INVALID;
     if (ev_fn < DE_EVENT_COMPONENT || ev_fn > DE_EVENT_PLANE) return DE_ERR_INVALID;
-    if (ev_fn == DE_EVENT_COMPONENT && (b->event_component < 0 || b->event_component >= b->dim))
+    if (ev_fn == DE_EVENT_COMPONENT &&
+        (b->event_component < 0 ||
+         (b->family != DE_FAMILY_SPRING_CHAIN && b->event_component >= b->dim)))
       return DE_ERR_INVALID;
     if (ev_fn == DE_EVENT_PLANE && (!b->event_point || !b->event_normal)) return DE_ERR_INVALID;
   }
@@ -979,6 +873,12 @@ int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
         delete r;
         return DE_ERR_UNSUPPORTED;
       }
+    if (b->event_solout != DE_SOLOUT_DEFAULT && ev_fn == DE_EVENT_COMPONENT)
+      for (int64_t i = 0; i < n; ++i)
+        if (b->event_component >= 2 * r->sizes[i]) {  // past this system's state
+          delete r;
+          return DE_ERR_INVALID;
+        }
     r->so.assign(b->state_offset, b->state_offset + n + 1);
     r->po.assign(b->param_offset, b->param_offset + n + 1);
     r->state_len = r->so[n];
@@ -1108,9 +1008,11 @@ void de_server_destroy(de_server* s) {
   (void)hipDeviceSynchronize();
   if (s->graph) (void)hipGraphExecDestroy(s->graph);
   for (auto& f : s->fams) {
-    if (f.slots) (void)hipFree(f.slots);
-    if (f.q) (void)hipFree(f.q);
-    if (f.ranges) (void)hipFree(f.ranges);
+    if (!f.lazy) {  // lazy pools live in the arena
+      if (f.slots) (void)hipFree(f.slots);
+      if (f.q) (void)hipFree(f.q);
+      if (f.ranges) (void)hipFree(f.ranges);
+    }
     if (f.stream) (void)hipStreamDestroy(f.stream);
   }
   for (auto& kv : s->all) delete kv.second;

@@ -145,12 +145,77 @@ def test_served_dop853_matches_direct_solve():
     assert_same(got[0], lorenz_req(1000, seed=23).solve())
 
 
-def test_server_refuses_dop853_spring_chains():
+def _same_outputs(got, want):
+    assert_same(got, want)
+    for f in ("dense", "event_t", "event_y"):
+        x, y = getattr(got, f), getattr(want, f)
+        assert (x is None) == (y is None), f
+        if x is not None:
+            assert np.array_equal(np.isnan(x), np.isnan(y)) and \
+                np.array_equal(np.nan_to_num(x), np.nan_to_num(y)), f
+    for f in ("n_dense", "n_events"):
+        x, y = getattr(got, f), getattr(want, f)
+        assert (x is None) == (y is None) and (x is None or np.array_equal(x, y)), f
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_served_methods_match_direct_solves(graph):
+    # Every method is served: RK4, Euler and the embedded family on Lorenz and
+    # Van der Pol lanes, and every method (and DOPRI5 with an event solout) on
+    # spring chains with one lane per system, in pools made on first use.
+    # Each request gives the bits of its direct solve, in a mixed stream.
+    from deamd import CrossingDirection, Event
+    p, y0 = deamd.gen_lorenz(1200, seed=31)
+    mu, v0 = deamd.gen_vdp(900, seed=32)
+    sysm, sy0 = deamd.gen_spring(150, seed=33)
+    lor = lambda m: IVP.ode(Lorenz(*p), 0.0, 3.0, y0).method(m.max_steps(1000000))
+    vdp = lambda m: IVP.ode(VanDerPol(*mu), 0.0, 6.0, v0).method(m.max_steps(1000000))
+    spr = lambda m: IVP.ode(sysm, 0.0, 2.0, sy0).method(m.max_steps(1000000))
+    E = ExplicitRungeKutta
+    reqs = [
+        lor(E.rkv878e().rtol(1e-9).atol(1e-11)).t_eval(np.linspace(0.0, 3.0, 7)),
+        vdp(E.rkf45().rtol(1e-7).atol(1e-9)),
+        lor(E.rk4(0.002)),
+        vdp(E.euler(0.001)).t_eval(np.linspace(0.0, 6.0, 5)),
+        spr(E.dop853().rtol(1e-8).atol(1e-10)),
+        spr(E.rk4(0.01)).t_eval(np.linspace(0.0, 2.0, 9)),
+        spr(E.rkv767e().rtol(1e-8).atol(1e-10)),
+        spr(E.dopri5().rtol(1e-7).atol(1e-9)).event(Event.component(1, 0.0))
+        .options(max_events=4),
+        spr(E.dop853().rtol(1e-8).atol(1e-10)).crossing(0, 0.0, CrossingDirection.Positive)
+        .options(max_events=2),
+    ]
+    want = [r.solve() for r in reqs]
+    with Server(8 * GIB, graph=graph) as srv:
+        tickets = []
+        for k, r in enumerate(reqs):
+            tickets.append(srv.submit(r))
+            if k % 3 == 0:
+                tickets.append(srv.submit(lorenz_req(500, seed=40 + k)))
+        got = [srv.poll(t) for t in tickets]
+        again = srv.poll(srv.submit(reqs[4]))   # a second request to a made pool
+    idx = 0
+    for k in range(len(reqs)):
+        _same_outputs(got[idx], want[k])
+        idx += 2 if k % 3 == 0 else 1
+    _same_outputs(again, want[4])
+
+
+def test_server_refusals():
+    # Event solouts are served for DOPRI5 on Lorenz and Van der Pol and for
+    # every method on spring chains, whose states have no single length for
+    # a plane; an event component must lie in every system's state.
+    from deamd import Event
     sysm, y0 = deamd.gen_spring(8, seed=1)
-    req = IVP.ode(sysm, 0.0, 1.0, y0).method(ExplicitRungeKutta.dop853())
+    p, ly0 = deamd.gen_lorenz(8, seed=1)
     with Server(8 * GIB) as srv:
-        with pytest.raises(RuntimeError):
-            srv.submit(req)
+        for req in (IVP.ode(sysm, 0.0, 1.0, y0).event(Event.plane(np.zeros(8), np.ones(8))),
+                    IVP.ode(Lorenz(*p), 0.0, 1.0, ly0).method(ExplicitRungeKutta.rkf45())
+                    .event(Event.component(0, 0.0)),
+                    IVP.ode(sysm, 0.0, 1.0, y0).event(
+                        Event.component(2 * int(sysm.sizes.min()), 0.0))):
+            with pytest.raises(RuntimeError):
+                srv.submit(req)
 
 
 def _event_fields(a, b):
