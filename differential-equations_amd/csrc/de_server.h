@@ -262,15 +262,17 @@ void set_pool_kind(Family& f) {
 
 // Pools made on first use, by method: the embedded family on Lorenz and Van
 // der Pol lanes (group g of two tableaux, server_pools.hip with
-// DE_SERVE_GROUP = g) and the one-lane spring-chain solvers (group 5).  Sets
-// f's slot kind and launcher; false when (family, method, solout) is not in
-// the group.
-bool lazy_pool_kind_0(int family, int method, Family& f);
-bool lazy_pool_kind_1(int family, int method, Family& f);
-bool lazy_pool_kind_2(int family, int method, Family& f);
-bool lazy_pool_kind_3(int family, int method, Family& f);
-bool lazy_pool_kind_4(int family, int method, Family& f);
-bool lazy_pool_kind_5(int family, int method, Family& f);
+// DE_SERVE_GROUP = g), and DOP853, RK4 and Euler lanes and the one-lane
+// spring-chain solvers (group 5).  ev: the lane's event solout (0 none, 1
+// EventWrappedSolout, 2 crossing / hyperplane; spring chains choose at run
+// time).  Sets f's slot kind and launcher; false when (family, method, ev) is
+// not in the group.
+bool lazy_pool_kind_0(int family, int method, int ev, Family& f);
+bool lazy_pool_kind_1(int family, int method, int ev, Family& f);
+bool lazy_pool_kind_2(int family, int method, int ev, Family& f);
+bool lazy_pool_kind_3(int family, int method, int ev, Family& f);
+bool lazy_pool_kind_4(int family, int method, int ev, Family& f);
+bool lazy_pool_kind_5(int family, int method, int ev, Family& f);
 
 }  // namespace serve
 }  // namespace de

@@ -136,7 +136,7 @@ struct Request {
   size_t dbytes = 0;
   void* hmem = nullptr;
   size_t hbytes = 0;
-  uint64_t pools = 0;  // bit k: work published to pool k
+  std::vector<int> pools;  // the pools its work is published to
   int slot = -1;
   int64_t first_iter = 0;  // first iteration whose gathered counts include this request
   bool done = false;
@@ -213,9 +213,11 @@ enum : int32_t { kServerGraph = 1 };
 // (EventWrappedSolout, or the crossing / hyperplane solouts), whose roots are
 // found inline.  Pools made on first use (server_pools.hip), their memory
 // carved from the arena: Lorenz / Van der Pol with RK4, Euler and the
-// embedded family at kLaneLazy + 2 method + (Van der Pol), and spring chains
-// one lane per system at kSpringVec + method (every method but DOPRI5, and
-// DOPRI5 with an event solout).
+// embedded family at kLaneLazy + 2 method + (Van der Pol), spring chains one
+// lane per system at kSpringVec + method (every method but DOPRI5, and DOPRI5
+// with an event solout), and Lorenz / Van der Pol with an event solout and a
+// method other than DOPRI5 at kLaneEvent + 4 method + 2 (crossing) + (Van der
+// Pol).
 constexpr int kSpringEnd = 9;
 constexpr int kLorenzDop853 = 9, kVdpDop853 = 10;
 constexpr int kLorenzEvent = 11, kLorenzCross = 12, kVdpEvent = 13, kVdpCross = 14;
@@ -223,8 +225,8 @@ constexpr int kEagerPools = 15;
 constexpr int kMethods = DE_METHOD_RKV989E + 1;
 constexpr int kLaneLazy = kEagerPools;
 constexpr int kSpringVec = kLaneLazy + 2 * kMethods;
-constexpr int kPools = kSpringVec + kMethods;
-static_assert(kPools <= 64, "a request's pools are a 64-bit mask");
+constexpr int kLaneEvent = kSpringVec + kMethods;
+constexpr int kPools = kLaneEvent + 4 * kMethods;
 constexpr int kLazyRanges = 1 << 16;  // range table of a lazy pool
 
 bool lane_lazy_method(int method) {
@@ -244,8 +246,8 @@ int fam_index_for(de_server* s, int family, int nm, int method, int solout) {
   if (family != DE_FAMILY_LORENZ && family != DE_FAMILY_VAN_DER_POL) return -1;
   const int v = family == DE_FAMILY_VAN_DER_POL;
   if (solout != DE_SOLOUT_DEFAULT) {
-    if (method != DE_METHOD_DOPRI5) return -1;
     const bool cross = solout != DE_SOLOUT_EVENT;
+    if (method != DE_METHOD_DOPRI5) return kLaneEvent + 4 * method + 2 * cross + v;
     return v ? (cross ? kVdpCross : kVdpEvent) : (cross ? kLorenzCross : kLorenzEvent);
   }
   if (method == DE_METHOD_DOPRI5) return v;
@@ -255,8 +257,14 @@ int fam_index_for(de_server* s, int family, int nm, int method, int solout) {
 
 // The slot kind and chunk kernel of lazy pool k (family and method from its index).
 bool lazy_kind(int k, Family& f) {
-  int family, method;
-  if (k >= kSpringVec) {
+  int family, method, ev = 0;
+  if (k >= kLaneEvent) {
+    const int rel = k - kLaneEvent;
+    family = rel & 1 ? DE_FAMILY_VAN_DER_POL : DE_FAMILY_LORENZ;
+    ev = (rel >> 1) & 1 ? 2 : 1;
+    method = rel >> 2;
+    if (method == DE_METHOD_DOPRI5) return false;
+  } else if (k >= kSpringVec) {
     family = DE_FAMILY_SPRING_CHAIN;
     method = k - kSpringVec;
   } else {
@@ -264,9 +272,9 @@ bool lazy_kind(int k, Family& f) {
     method = (k - kLaneLazy) >> 1;
     if (!lane_lazy_method(method)) return false;
   }
-  return lazy_pool_kind_0(family, method, f) || lazy_pool_kind_1(family, method, f) ||
-         lazy_pool_kind_2(family, method, f) || lazy_pool_kind_3(family, method, f) ||
-         lazy_pool_kind_4(family, method, f) || lazy_pool_kind_5(family, method, f);
+  return lazy_pool_kind_0(family, method, ev, f) || lazy_pool_kind_1(family, method, ev, f) ||
+         lazy_pool_kind_2(family, method, ev, f) || lazy_pool_kind_3(family, method, ev, f) ||
+         lazy_pool_kind_4(family, method, ev, f) || lazy_pool_kind_5(family, method, ev, f);
 }
 
 // Device bytes of a request's inputs (and spring class lists) and host bytes
@@ -569,15 +577,14 @@ void scheduler(de_server* s) {
         for (int k = 2; k < kSpringEnd; ++k)
           if (!lists[k].empty()) {
             publish(s, s->fams[k], r->slot, 0, (int32_t)lists[k].size(), d_lists[k]);
-            r->pools |= 1ull << k;
+            r->pools.push_back(k);
           }
       } else {
         const int k = needs[0];
         publish(s, s->fams[k], r->slot, 0, (int32_t)b.n, nullptr);
-        r->pools |= 1ull << k;
+        r->pools.push_back(k);
       }
-      for (size_t k = 0; k < s->fams.size(); ++k)
-        if ((r->pools >> k) & 1ull) s->fams[k].active_reqs += 1;
+      for (int k : r->pools) s->fams[k].active_reqs += 1;
     }
     // The uploads and range entries are on the device before the commit
     // kernels, which run after the iteration in flight (stream order on ctl).
@@ -659,8 +666,7 @@ void scheduler(de_server* s) {
       const double t_done = now_s();
       for (int slot : finished) {
         Request* r = s->active[slot];
-        for (size_t q = 0; q < s->fams.size(); ++q)
-          if ((r->pools >> q) & 1ull) s->fams[q].active_reqs -= 1;
+        for (int q : r->pools) s->fams[q].active_reqs -= 1;
         s->arena.release(r->dmem, r->dbytes);  // the results stay until delivered
         r->dmem = nullptr;
         r->done = true;
@@ -780,7 +786,8 @@ int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out) {
     }
   }
   size_t pools = 0;
-  for (auto& f : s->fams) pools += f.slot_bytes * f.nslots + sizeof(Range) * (size_t)f.max_ranges;
+  for (auto& f : s->fams)  // lazy pools take theirs from the arena when made
+    if (!f.lazy) pools += f.slot_bytes * f.nslots + sizeof(Range) * (size_t)f.max_ranges;
   ok = ok && hipMalloc(&s->d_reqs, sizeof(de::serve::ReqDev) * kMaxReq) == hipSuccess;
   for (auto& h : s->h_counts)
     ok = ok && hipHostMalloc(&h, sizeof(int32_t) * kMaxReq, 0) == hipSuccess;
@@ -829,22 +836,19 @@ int de_server_submit(de_server* s, const de_ode_batch* b, uint64_t* ticket) {
   if (b->family != DE_FAMILY_LORENZ && b->family != DE_FAMILY_VAN_DER_POL &&
       b->family != DE_FAMILY_SPRING_CHAIN)
     return DE_ERR_UNSUPPORTED;
-  // Every method for every served family.
+  // Every method for every served family, with or without an event solout.
   if (b->method < DE_METHOD_DOPRI5 || b->method > DE_METHOD_RKV989E) return DE_ERR_INVALID;
   if (b->n <= 0 || !b->y0 || (b->n_eval > 0 && !b->t_eval)) return DE_ERR_INVALID;
   // Served requests carry scalar or per-trajectory tolerances and the final
-  // state / t_eval output, or an event solout (DOPRI5 on Lorenz and Van der
-  // Pol; every method on spring chains, without planes), validated as
-  // de_ode_batch_solve validates it; per-component vectors go through
-  // de_ode_batch_solve.
+  // state / t_eval output, or an event solout (no planes on spring chains),
+  // validated as de_ode_batch_solve validates it; per-component vectors and
+  // point outputs go through de_ode_batch_solve.
   if (b->rtol_comp || b->atol_comp) return DE_ERR_UNSUPPORTED;
   if (b->points != DE_POINTS_NONE) return DE_ERR_UNSUPPORTED;  // final state, t_eval or events
   const int ev_fn = b->event_solout == DE_SOLOUT_CROSSING     ? DE_EVENT_COMPONENT
                     : b->event_solout == DE_SOLOUT_HYPERPLANE ? DE_EVENT_PLANE
                                                                : b->event_fn;
   if (b->event_solout != DE_SOLOUT_DEFAULT) {
-    if (b->family != DE_FAMILY_SPRING_CHAIN && b->method != DE_METHOD_DOPRI5)
-      return DE_ERR_UNSUPPORTED;
     if (b->event_solout < DE_SOLOUT_EVENT || b->event_solout > DE_SOLOUT_HYPERPLANE)
       return DE_ERR_INVALID;
     if (b->family == DE_FAMILY_SPRING_CHAIN && ev_fn == DE_EVENT_PLANE)

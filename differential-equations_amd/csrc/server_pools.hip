@@ -1,10 +1,15 @@
 // Serving pools made on first use (their chunk kernels, by translation unit):
 //   DE_SERVE_GROUP 0..4   two embedded-family tableaux each, on Lorenz and Van
-//                         der Pol lanes (ErkLane, adaptive/ordinary.rs:16-292)
+//                         der Pol lanes (ErkLane, adaptive/ordinary.rs:16-292),
+//                         without and with an event solout
 //   DE_SERVE_GROUP 5      RK4 and Euler on Lorenz and Van der Pol lanes
-//                         (FixedLane), and every method on spring chains with
-//                         one lane per system (the Vec* solvers, their vectors
-//                         in the request's device workspace)
+//                         (FixedLane) and DOP853 lanes with an event solout,
+//                         and every method on spring chains with one lane per
+//                         system (the Vec* solvers, their vectors in the
+//                         request's device workspace)
+// A lane with an event solout finds its roots inline (no deferred brackets):
+// EventWrappedSolout (EV 1) keeps the t_eval output, the crossing solouts
+// (EV 2) replace it.
 #include "../../include/de_api.h"
 #include "de_server.h"
 
@@ -23,19 +28,24 @@ static bool lane_kind(Family& f) {
 }
 
 #if DE_SERVE_GROUP < 5
+template <class F, class Tab>
+static bool erk_kind_of(int ev, Family& f) {
+  if (ev == 1) return lane_kind<ErkLane<F, Tab, true, false, 1>>(f);
+  if (ev == 2) return lane_kind<ErkLane<F, Tab, false, false, 2>>(f);
+  return lane_kind<ErkLane<F, Tab, true, false>>(f);
+}
 template <class Tab>
-static bool erk_kind(int family, Family& f) {
-  if (family == DE_FAMILY_LORENZ) return lane_kind<ErkLane<LorenzRhs, Tab, true, false>>(f);
-  if (family == DE_FAMILY_VAN_DER_POL)
-    return lane_kind<ErkLane<VanDerPolRhs, Tab, true, false>>(f);
+static bool erk_kind(int family, int ev, Family& f) {
+  if (family == DE_FAMILY_LORENZ) return erk_kind_of<LorenzRhs, Tab>(ev, f);
+  if (family == DE_FAMILY_VAN_DER_POL) return erk_kind_of<VanDerPolRhs, Tab>(ev, f);
   return false;
 }
 
-#define DE_SERVE_PAIR(G, M0, T0, M1, T1)                                  \
-  bool lazy_pool_kind_##G(int family, int method, Family& f) {           \
-    if (method == M0) return erk_kind<erk_tab::T0>(family, f);           \
-    if (method == M1) return erk_kind<erk_tab::T1>(family, f);           \
-    return false;                                                         \
+#define DE_SERVE_PAIR(G, M0, T0, M1, T1)                                          \
+  bool lazy_pool_kind_##G(int family, int method, int ev, Family& f) {           \
+    if (method == M0) return erk_kind<erk_tab::T0>(family, ev, f);               \
+    if (method == M1) return erk_kind<erk_tab::T1>(family, ev, f);               \
+    return false;                                                                 \
   }
 
 #if DE_SERVE_GROUP == 0
@@ -58,16 +68,24 @@ static bool vec_kind(Family& f) {
   return lane_kind<V>(f);
 }
 
+template <class F, int STAGES>
+static bool fixed_kind_of(int ev, Family& f) {
+  if (ev == 1) return lane_kind<FixedLane<F, STAGES, 1>>(f);
+  if (ev == 2) return lane_kind<FixedLane<F, STAGES, 2>>(f);
+  return lane_kind<FixedLane<F, STAGES>>(f);
+}
 template <class F>
-static bool fixed_kind(int method, Family& f) {
-  if (method == DE_METHOD_RK4) return lane_kind<FixedLane<F, 4>>(f);
-  if (method == DE_METHOD_EULER) return lane_kind<FixedLane<F, 1>>(f);
+static bool lane_kind_5(int method, int ev, Family& f) {
+  if (method == DE_METHOD_RK4) return fixed_kind_of<F, 4>(ev, f);
+  if (method == DE_METHOD_EULER) return fixed_kind_of<F, 1>(ev, f);
+  if (method == DE_METHOD_DOP853 && ev == 1) return lane_kind<Dop853Lane<F, true, false, 1>>(f);
+  if (method == DE_METHOD_DOP853 && ev == 2) return lane_kind<Dop853Lane<F, false, false, 2>>(f);
   return false;
 }
 
-bool lazy_pool_kind_5(int family, int method, Family& f) {
-  if (family == DE_FAMILY_LORENZ) return fixed_kind<LorenzRhs>(method, f);
-  if (family == DE_FAMILY_VAN_DER_POL) return fixed_kind<VanDerPolRhs>(method, f);
+bool lazy_pool_kind_5(int family, int method, int ev, Family& f) {
+  if (family == DE_FAMILY_LORENZ) return lane_kind_5<LorenzRhs>(method, ev, f);
+  if (family == DE_FAMILY_VAN_DER_POL) return lane_kind_5<VanDerPolRhs>(method, ev, f);
   if (family != DE_FAMILY_SPRING_CHAIN) return false;
   switch (method) {
     case DE_METHOD_DOPRI5: return vec_kind<VecDopri5>(f);

@@ -370,9 +370,9 @@ int de_workspace_trim(void);                 /* return every kept block to the d
 typedef struct de_server de_server;
 int de_server_create(size_t hbm_budget_bytes, int32_t flags, de_server** out);
 /* Copies the request's inputs; returns a ticket.  Lorenz, Van der Pol and spring
- * chains with every method; event solouts for DOPRI5 on Lorenz and Van der Pol and
- * (no planes) for every method on spring chains; final state, t_eval or events (no
- * point outputs, no per-component tolerances). */
+ * chains with every method, with or without an event solout (no planes on spring
+ * chains); final state, t_eval or events (no point outputs, no per-component
+ * tolerances). */
 int de_server_submit(de_server* s, const de_ode_batch* req, uint64_t* ticket);
 /* Returns 1 and fills host-memory `out` when the request is done, 0 if it is
  * still running (wait = 0) or blocks until done (wait = 1). */

@@ -202,16 +202,16 @@ def test_served_methods_match_direct_solves(graph):
 
 
 def test_server_refusals():
-    # Event solouts are served for DOPRI5 on Lorenz and Van der Pol and for
-    # every method on spring chains, whose states have no single length for
-    # a plane; an event component must lie in every system's state.
+    # Event solouts are served for every method; spring chains' states have
+    # no single length for a plane, and an event component must lie in every
+    # trajectory's state.
     from deamd import Event
     sysm, y0 = deamd.gen_spring(8, seed=1)
     p, ly0 = deamd.gen_lorenz(8, seed=1)
     with Server(8 * GIB) as srv:
         for req in (IVP.ode(sysm, 0.0, 1.0, y0).event(Event.plane(np.zeros(8), np.ones(8))),
                     IVP.ode(Lorenz(*p), 0.0, 1.0, ly0).method(ExplicitRungeKutta.rkf45())
-                    .event(Event.component(0, 0.0)),
+                    .event(Event.component(3, 0.0)),
                     IVP.ode(sysm, 0.0, 1.0, y0).event(
                         Event.component(2 * int(sysm.sizes.min()), 0.0))):
             with pytest.raises(RuntimeError):
@@ -258,6 +258,38 @@ def test_served_event_solouts_match_direct_solves():
         _event_fields(g, w)
     assert np.array_equal(got[2].n_dense, wants[1].n_dense)
     assert np.array_equal(np.nan_to_num(got[2].dense), np.nan_to_num(wants[1].dense))
+
+
+@pytest.mark.parametrize("method", ["dop853", "rk4", "euler", "rkf45", "rkv878e"])
+def test_served_event_solouts_of_other_methods(method):
+    # Event solouts with DOP853, RK4, Euler and the embedded family on Lorenz
+    # and Van der Pol lanes (pools made on first use, roots inline): the bits
+    # of a direct solve, t_eval and terminate counts included.
+    from deamd import CrossingDirection, Event, EventConfig
+    E = ExplicitRungeKutta
+    m = {"dop853": lambda: E.dop853().rtol(1e-9).atol(1e-11),
+         "rk4": lambda: E.rk4(0.002), "euler": lambda: E.euler(0.0005),
+         "rkf45": lambda: E.rkf45().rtol(1e-8).atol(1e-10),
+         "rkv878e": lambda: E.rkv878e().rtol(1e-9).atol(1e-11)}[method]
+    p, y0 = deamd.gen_lorenz(800, seed=41)
+    mu, v0 = deamd.gen_vdp(600, seed=42)
+    lor = lambda: IVP.ode(Lorenz(*p), 0.0, 4.0, y0).method(m().max_steps(1000000))
+    reqs = [
+        lor().event(Event.component(2, p[1] - 1.0)).options(max_events=8),
+        lor().t_eval(np.linspace(0.0, 4.0, 9)).event(
+            Event.component(0, 0.0, EventConfig().terminate_after(2))).options(max_events=4),
+        lor().hyperplane_crossing([0.0, 0.0, 25.0], [1.0, 1.0, 3.0],
+                                  CrossingDirection.Positive).options(max_events=8),
+        IVP.ode(VanDerPol(*mu), 0.0, 8.0, v0).method(m().max_steps(1000000))
+        .crossing(0, 0.0, CrossingDirection.Negative).options(max_events=6),
+    ]
+    wants = [r.solve() for r in reqs]
+    assert all(w.n_events.sum() > 0 for w in wants)
+    with Server(8 * GIB) as srv:
+        ts = [srv.submit(r) for r in reqs] + [srv.submit(lorenz_req(600, seed=43))]
+        got = [srv.poll(t) for t in ts]
+    for g, w in zip(got, wants):
+        _same_outputs(g, w)
 
 
 @pytest.mark.parametrize("graph", [False, True])
