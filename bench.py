@@ -957,8 +957,9 @@ def other_configs(args, dv):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(trajectories shard by index ranges across GPUs; see DESIGN.md)" % args.gpus,
+        print("bench.py: runs on one GPU only; --gpus %d is not supported "
+              "(an ensemble would shard by trajectory index ranges, one range per GPU)"
+              % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     dv = Device()

@@ -239,7 +239,7 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
-                "libde_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+                "libde_amd.so is not built (%s); run make -C differential-equations_amd" % LIB_PATH)
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
