@@ -16,7 +16,8 @@ namespace serve {
 
 constexpr int kMaxReq = 8192;     // requests resident at once
 constexpr int kMaxRanges = 1 << 20;
-constexpr int kChunk = 256;       // attempts per slot per launch
+constexpr int kChunk = 256;       // attempts per slot per launch (no backlog)
+constexpr int kMaxChunk = 4096;   // attempts per slot per launch under a backlog
 constexpr int kBlock = 256;
 
 struct Range {

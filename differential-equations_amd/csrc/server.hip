@@ -165,6 +165,11 @@ using namespace de::serve;
 struct de_server {
   size_t budget = 0;
   int32_t flags = 0;
+  // Attempts per slot of the next iteration: kChunk while the active work
+  // fits the pools' slots, up to kMaxChunk under a backlog (fewer iteration
+  // boundaries, where every pool's kernel must end before the next starts).
+  // A captured graph keeps kChunk.
+  int chunk = de::serve::kChunk;
   int ncu = 256;
   int64_t max_ranges = 0;
   Arena arena;   // device: request inputs
@@ -454,7 +459,7 @@ void launch_all(de_server* s, bool busy_only = false, bool lazy_only = false) {
     if (!f.slots || (lazy_only && !f.lazy)) continue;
     if (busy_only && f.active_reqs == 0) continue;
     (void)hipStreamWaitEvent(f.stream, s->fork, 0);
-    f.launch(f, s->d_reqs, kChunk, f.stream);
+    f.launch(f, s->d_reqs, s->chunk, f.stream);
     (void)hipEventRecord(s->fam_done[k], f.stream);
     (void)hipStreamWaitEvent(s->ctl, s->fam_done[k], 0);
   }
@@ -662,6 +667,17 @@ void scheduler(de_server* s) {
     std::vector<int> finished;
     for (auto& kv : s->active)
       if (kv.second->first_iter <= prev && counts[kv.first] <= 0) finished.push_back(kv.first);
+    if (!s->graph) {
+      // Backlog: remaining trajectories over the slots of the pools in use.
+      double remaining = 0.0, slots = 0.0;
+      for (auto& kv : s->active)
+        remaining += kv.second->first_iter <= prev ? std::max(counts[kv.first], 0)
+                                                   : (double)kv.second->b.n;
+      for (auto& f : s->fams)
+        if (f.active_reqs > 0) slots += (double)f.nslots / f.g;
+      if (remaining > slots) s->chunk = kMaxChunk;
+      else if (remaining < 0.25 * slots) s->chunk = kChunk;
+    }
     if (!finished.empty()) {
       const double t_done = now_s();
       for (int slot : finished) {
