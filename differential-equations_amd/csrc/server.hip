@@ -139,6 +139,7 @@ struct Request {
   std::vector<int> pools;  // the pools its work is published to
   int slot = -1;
   int64_t first_iter = 0;  // first iteration whose gathered counts include this request
+  bool trimmed = false;    // idle lazy pools were released to make room for it
   bool done = false;
   int error = 0;
   double t_submit = 0, t_done = 0;
@@ -202,6 +203,7 @@ struct de_server {
   hipGraphExec_t graph = nullptr;
   int iterations = 0;
   int64_t range_resets = 0;
+  int64_t pools_released = 0;  // idle lazy pools released to the arena
   // Scheduler time split (seconds): admitting (uploads, publishing), the
   // iterations from launch to their counts on the host, and retiring.
   double t_admit = 0, t_iter = 0, t_retire = 0;
@@ -502,6 +504,30 @@ bool make_lazy(de_server* s, int k) {
   return true;
 }
 
+// Releases the lazy pools that hold no active request's work back to the
+// arena, after the iterations in flight (which may still run their chunks)
+// have ended.  Returns whether any was released.
+bool trim_lazy(de_server* s) {
+  bool any = false;
+  for (auto& f : s->fams)
+    any = any || (f.lazy && f.slots && f.active_reqs == 0 && !f.dirty);
+  if (!any) return false;
+  (void)hipStreamSynchronize(s->ctl);
+  for (auto& f : s->fams) {
+    if (!(f.lazy && f.slots && f.active_reqs == 0 && !f.dirty)) continue;
+    s->arena.release(f.ranges, sizeof(Range) * (size_t)f.max_ranges);
+    s->arena.release(f.q, sizeof(Queue));
+    s->arena.release(f.slots, f.slot_bytes * (size_t)f.nslots);
+    f.slots = nullptr;
+    f.q = nullptr;
+    f.ranges = nullptr;
+    f.host_ranges.clear();
+    f.published = 0;
+    s->pools_released += 1;
+  }
+  return true;
+}
+
 void scheduler(de_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   int64_t inflight = -1;  // last queued iteration whose counts are not read yet
@@ -531,6 +557,10 @@ void scheduler(de_server* s) {
         if (!s->fams[k].slots && !make_lazy(s, k)) unmade = true;
       if (unmade) {
         if (!s->active.empty()) break;  // after the running requests release memory
+        if (!r->trimmed) {
+          r->trimmed = true;
+          if (trim_lazy(s)) continue;  // try again with the idle pools released
+        }
         s->pending.pop_front();
         r->error = DE_ERR_NOMEM;
         r->done = true;
@@ -563,6 +593,10 @@ void scheduler(de_server* s) {
         size_t dev = 0, host = 0;
         footprint(r, &dev, &host);
         const bool never = dev + 255 > s->arena.size || host + 255 > s->harena.size;
+        if (!never && s->active.empty() && dev + 255 > s->arena.largest_free() && !r->trimmed) {
+          r->trimmed = true;
+          if (trim_lazy(s)) continue;  // try again with the idle pools released
+        }
         if (never || (s->active.empty() && dev + 255 > s->arena.largest_free())) {
           s->pending.pop_front();
           r->error = DE_ERR_NOMEM;
@@ -1005,6 +1039,16 @@ int de_server_stats(de_server* s, int64_t* iterations, int64_t* range_resets) {
   std::lock_guard<std::mutex> lk(s->mu);
   if (iterations) *iterations = s->iterations;
   if (range_resets) *range_resets = s->range_resets;
+  return DE_OK;
+}
+
+int de_server_pools(de_server* s, int32_t* made, int64_t* released) {
+  if (!s) return DE_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int32_t m = 0;
+  for (auto& f : s->fams) m += f.lazy && f.slots ? 1 : 0;
+  if (made) *made = m;
+  if (released) *released = s->pools_released;
   return DE_OK;
 }
 

@@ -973,6 +973,12 @@ class Server:
         _abi.check(lib().de_server_stats(self.h, C.byref(it), C.byref(rr)))
         return it.value, rr.value
 
+    def pools(self):
+        """(pools made on first use that are made now, releases of idle ones)."""
+        m, r = C.c_int32(), C.c_int64()
+        _abi.check(lib().de_server_pools(self.h, C.byref(m), C.byref(r)))
+        return m.value, r.value
+
     def timing(self):
         """Scheduler seconds spent admitting, iterating and retiring."""
         a, i, r = C.c_double(), C.c_double(), C.c_double()

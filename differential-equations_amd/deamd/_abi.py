@@ -207,6 +207,7 @@ SIGNATURES = {
     "de_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(OdeResult), C.c_int32]),
     "de_server_latency": (C.c_double, [C.c_void_p, C.c_uint64]),
     "de_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "de_server_pools": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "de_server_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]),
     "de_server_destroy": (None, [C.c_void_p]),

@@ -349,3 +349,27 @@ def test_poisson_stream_results_equal_direct_solves():
                                 (g.rejected == want.rejected)).sum())
             total += n
     assert total > 0 and same_counts >= 0.999 * total
+
+
+def test_idle_lazy_pools_are_released_under_a_small_budget():
+    # A budget a little above the pools made at creation leaves an arena of a
+    # few hundred MB: the pools made on first use for 27 (family, method)
+    # kinds do not all fit, so idle ones are released for the next, and a
+    # released kind made again still gives the bits of a direct solve.
+    E = ExplicitRungeKutta
+    names = ["rk4", "euler", "rkf45", "cash_karp", "rkv655e", "rkv656e", "rkv766e",
+             "rkv767e", "rkv877e", "rkv878e", "rkv988e", "rkv989e"]
+    mk = {"rk4": lambda: E.rk4(0.01), "euler": lambda: E.euler(0.002)}
+    method = lambda nm: mk[nm]() if nm in mk else getattr(E, nm)().rtol(1e-7).atol(1e-9)
+    p, y0 = deamd.gen_lorenz(64, seed=51)
+    sysm, sy0 = deamd.gen_spring(16, seed=52)
+    reqs = [IVP.ode(Lorenz(*p), 0.0, 1.0, y0).method(method(nm).max_steps(100000))
+            for nm in names]
+    reqs += [IVP.ode(sysm, 0.0, 0.5, sy0).method(method(nm).max_steps(100000))
+             for nm in names + ["dop853"]]
+    reqs.append(reqs[0])                     # a released kind, made again
+    with Server(1 * GIB) as srv:
+        for r in reqs:
+            _same_outputs(srv.poll(srv.submit(r)), r.solve())
+        made, released = srv.pools()
+        assert released > 0 and 0 < made < 25
