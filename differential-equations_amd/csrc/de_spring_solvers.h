@@ -17,7 +17,7 @@
 //                 group.  The butterfly sums in a different order
 //                 than the reference's sequential loop (traits.rs:398-409), so
 //                 results agree to rounding rather than bit for bit.  DOPRI5
-//                 with or without the t_eval output.
+//                 with the t_eval output or a point output.
 #pragma once
 
 #include "de_erk_tables.h"
@@ -1261,6 +1261,7 @@ struct GroupDopri5 {
   int nm, l;
   int64_t sys, so;
   double t, h, h_prev, tf, dir, rtol, atol;
+  double last;  // EvenSolout::last_output_t
   int32_t acc, rej, ev, steps, stiff, nonstiff, countdown, next_eval, pushed;
   bool rejected_last;
 
@@ -1324,6 +1325,7 @@ struct GroupDopri5 {
       a.evals[sys] = ev;
       a.status[sys] = st;
       if (a.n_dense) a.n_dense[sys] = pushed;
+      if (a.pts_mode != kPointsNone && a.n_pts) a.n_pts[sys] = pushed;
     }
     if (a.n_eval > 0) {  // points not reached (all of them on BadInput)
       double nan_[C];
@@ -1373,6 +1375,70 @@ struct GroupDopri5 {
     }
     next_eval = idx;
     return ok;
+  }
+
+  // Point outputs, as VecCommon's: point k of the system has its time at
+  // pts_t[pts_offset[sys] + k] (written by the group's first lane) and its
+  // state at pts_y[pts_ypos[sys] + k * 2N] while the system's slots last.
+  DE_DEV void pts_push(const SpringArgs& A, double tt, const double* xs, const double* vs) {
+    const OdeArgs& a = A.o;
+    const int64_t lo = a.pts_offset[sys];
+    if (lo + pushed < a.pts_offset[sys + 1]) {
+      if (l == 0) a.pts_t[lo + pushed] = tt;
+      write_state(a.pts_y + A.pts_ypos[sys] + (int64_t)pushed * 2 * nm, xs, vs);
+    }
+    pushed += 1;
+  }
+  template <class Interp>
+  DE_DEV bool pts_interp(const SpringArgs& A, double ti, Interp& interp) {
+    double ox[C], ov[C];
+    if (!interp(ti, ox, ov)) return false;
+    pts_push(A, ti, ox, ov);
+    return true;
+  }
+  template <class Interp>
+  DE_DEV bool points_step(const SpringArgs& A, double t_curr, double t_prev, const double* xc,
+                          const double* vc, Interp& interp) {
+    const OdeArgs& a = A.o;
+    if (a.pts_mode == kPointsSteps) {
+      pts_push(A, t_curr, xc, vc);
+      return true;
+    }
+    if (a.pts_mode == kPointsDense) {
+      if (t_prev != t_curr) {
+        for (int32_t k = 1; k < a.pts_n; ++k) {
+          const double h_old = t_curr - t_prev;
+          const double ti = t_prev + (double)k * h_old / (double)a.pts_n;
+          if (!pts_interp(A, ti, interp)) return false;
+        }
+      }
+      pts_push(A, t_curr, xc, vc);
+      return true;
+    }
+    const double dt = a.pts_dt;
+    const double tol = fabs(dt) * 1e-12 + 2.220446049250313080847e-16 * 10.0;
+    double ti = last + dt * dir;
+    while ((dir > 0.0 && ti <= t_curr) || (dir < 0.0 && ti >= t_curr)) {
+      if ((dir > 0.0 && ti >= t_prev && ti <= t_curr) ||
+          (dir < 0.0 && ti <= t_prev && ti >= t_curr)) {
+        if (!(fabs(ti - last) <= tol)) {
+          if (!pts_interp(A, ti, interp)) return false;
+          last = ti;
+        }
+      }
+      ti += dt * dir;
+    }
+    if (t_curr == tf) {
+      if (fabs(last - tf) <= tol) {
+        pushed -= 1;  // Solution::pop, then the exact final point
+        pts_push(A, tf, xc, vc);
+        last = tf;
+      } else if (last != tf) {
+        pts_push(A, tf, xc, vc);
+        last = tf;
+      }
+    }
+    return true;
   }
 
   // Load system `s`; returns false (after writing results) on BadInput.
@@ -1464,7 +1530,12 @@ struct GroupDopri5 {
     stiff = nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (a.n_eval > 0) emit(A, t, t, x, v, [](double, double*, double*) { return true; });
+    if (a.pts_mode != kPointsNone) {
+      pts_push(A, t, x, v);
+      last = t;
+    } else if (a.n_eval > 0) {
+      emit(A, t, t, x, v, [](double, double*, double*) { return true; });
+    }
     return true;
   }
 
@@ -1604,7 +1675,10 @@ struct GroupDopri5 {
         }
       }
       bool emitted = true;
-      if (a.n_eval > 0) {
+      const bool pts = a.pts_mode != kPointsNone;
+      if (pts && a.pts_mode == kPointsSteps) {
+        pts_push(A, t_new, nx, nv);
+      } else if (a.n_eval > 0 || pts) {
         double c1x[C], c1v[C], c2x[C], c2v[C], c3x[C], c3v[C];
 #pragma unroll
         for (int q = 0; q < C; ++q) {
@@ -1618,7 +1692,7 @@ struct GroupDopri5 {
           c3v[q] = (dv2 + (-h) * dnv[q]) + (-1.0) * bv;
         }
         const double hp = h, tp = t, cont4 = 0.0 * h;
-        emitted = emit(A, t_new, t, nx, nv, [&](double te, double* ox, double* ov) {
+        auto interp = [&](double te, double* ox, double* ov) {
           if (te < tp || te > t_new) return false;
           double s = (te - tp) / hp;
           double s1 = 1.0 - s;
@@ -1634,7 +1708,9 @@ struct GroupDopri5 {
             ov[q] = v[q] + s * av;
           }
           return true;
-        });
+        };
+        emitted = pts ? points_step(A, t_new, t, nx, nv, interp)
+                      : emit(A, t_new, t, nx, nv, interp);
       }
       h_prev = h;
       t = t_new;

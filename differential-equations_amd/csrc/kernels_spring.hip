@@ -224,10 +224,30 @@ static Choice autotune(const SpringArgs& base, int k, const int32_t* d_list, int
 }
 
 // One thread per system: DE_MAP_LANE, every method but DOPRI5 (the group
-// mapping is built for DOPRI5), the point outputs and the event solouts.
+// mapping is built for DOPRI5 with the t_eval or a point output) and the
+// event solouts.
 static bool lane_mapping(const de_ode_batch* b) {
   return b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5 ||
-         b->points != DE_POINTS_NONE || b->event_solout != DE_SOLOUT_DEFAULT;
+         b->event_solout != DE_SOLOUT_DEFAULT;
+}
+
+// Start of each system's point states in points_y (n + 1 values): the slot
+// counts times the state lengths, summed over the systems before it.
+// Returns 0, 2 for offsets that decrease or are negative, -1 on a HIP error.
+static int point_state_offsets(const OdeArgs& a, const int32_t* host_sizes, int64_t n,
+                               std::vector<int64_t>& ypos, hipStream_t s) {
+  std::vector<int64_t> off((size_t)n + 1);
+  if (hipMemcpyAsync(off.data(), a.pts_offset, sizeof(int64_t) * (n + 1),
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -1;
+  ypos.assign((size_t)n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t slots = off[i + 1] - off[i];
+    if (off[i] < 0 || slots < 0) return 2;
+    ypos[i + 1] = ypos[i] + slots * 2 * (int64_t)host_sizes[i];
+  }
+  return 0;
 }
 
 int64_t spring_workspace_bytes(const de_ode_batch* b) {
@@ -242,7 +262,8 @@ int64_t spring_workspace_bytes(const de_ode_batch* b) {
     return (int64_t)vectors * 512 * 8 * b->n +
            (b->points != DE_POINTS_NONE ? 8 * (b->n + 1) : 0);
   }
-  return (int64_t)sizeof(int32_t) * b->n + (int64_t)sizeof(unsigned long long) * (kNumCaps + 1);
+  return (int64_t)sizeof(int32_t) * b->n + (int64_t)sizeof(unsigned long long) * (kNumCaps + 1) +
+         (b->points != DE_POINTS_NONE ? 8 * (b->n + 2) : 0);
 }
 
 int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArgs& a,
@@ -275,9 +296,14 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     masses[k] += host_sizes[i];
     state_len += 2 * (int64_t)host_sizes[i];
   }
+  const bool pts = b->points != DE_POINTS_NONE;
+  std::vector<int64_t> ypos;
+  if (pts) {
+    const int r = point_state_offsets(a, host_sizes, n, ypos, s);
+    if (r != 0) return r;
+  }
   if (lane_mapping(b)) {
     const LaneSolver ls = lane_solver(b->method);
-    const bool pts = b->points != DE_POINTS_NONE;
     const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
     const int64_t vec_len = ((int64_t)ls.vectors + (events ? 1 : 0)) * state_len;
     const int64_t need = vec_len + (pts ? n + 1 : 0);
@@ -291,19 +317,6 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     base.work = g_work;
     if (events) base.ev_work = g_work + (int64_t)ls.vectors * state_len;
     if (pts) {
-      // Start of each system's point states in points_y: the slot counts
-      // times the state lengths, summed over the systems before it.
-      std::vector<int64_t> off((size_t)n + 1), ypos((size_t)n + 1);
-      if (hipMemcpyAsync(off.data(), a.pts_offset, sizeof(int64_t) * (n + 1),
-                         hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
-        return -1;
-      ypos[0] = 0;
-      for (int64_t i = 0; i < n; ++i) {
-        const int64_t slots = off[i + 1] - off[i];
-        if (off[i] < 0 || slots < 0) return 2;
-        ypos[i + 1] = ypos[i] + slots * 2 * (int64_t)host_sizes[i];
-      }
       int64_t* d_ypos = (int64_t*)(g_work + vec_len);
       if (hipMemcpyAsync(d_ypos, ypos.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
                          s) != hipSuccess ||
@@ -323,13 +336,19 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     all.insert(all.end(), lists[k].begin(), lists[k].end());
   }
   start[kNumCaps] = (int64_t)all.size();
+  // The point states' offsets follow the lists, 8-byte aligned.
+  const size_t list_bytes = (sizeof(int32_t) * (all.size() + 1) + 7) & ~(size_t)7;
   int32_t* d_list = nullptr;
-  if (hipMalloc(&d_list, sizeof(int32_t) * (all.size() + 1)) != hipSuccess) return -4;
+  if (hipMalloc(&d_list, list_bytes + (pts ? sizeof(int64_t) * (n + 1) : 0)) != hipSuccess)
+    return -4;
   if (hipMemcpy(d_list, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice) !=
-      hipSuccess) {
+          hipSuccess ||
+      (pts && hipMemcpy((char*)d_list + list_bytes, ypos.data(), sizeof(int64_t) * (n + 1),
+                        hipMemcpyHostToDevice) != hipSuccess)) {
     (void)hipFree(d_list);
     return -1;
   }
+  if (pts) base.pts_ypos = (const int64_t*)((const char*)d_list + list_bytes);
   // Choose a configuration per class.
   std::vector<Choice> choice(kNumCaps);
   for (int k = 0; k < kNumCaps; ++k) {

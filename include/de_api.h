@@ -204,7 +204,8 @@ typedef struct de_ode_batch {
    * consecutive rejections that end the solve with Error::Stiffness. */
   int64_t max_rejects;
   /* Output strategy with a variable number of points (DE_POINTS_*; without t_eval
-   * or an event solout; spring chains run it on the one-lane-per-system mapping). */
+   * or an event solout; spring chains: DOPRI5 on the group mappings, the other
+   * methods on one lane per system). */
   int32_t points;
   int32_t points_n;  /* DE_POINTS_DENSE: n (points per step, including the step) */
   double points_dt;  /* DE_POINTS_EVEN: dt */

