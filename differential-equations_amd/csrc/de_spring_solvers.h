@@ -1262,7 +1262,9 @@ struct GroupDopri5 {
   int64_t sys, so;
   double t, h, h_prev, tf, dir, rtol, atol;
   double last;  // EvenSolout::last_output_t
+  double ev_value, ev_last, ev_new, tail;  // event solouts (as VecCommon's)
   int32_t acc, rej, ev, steps, stiff, nonstiff, countdown, next_eval, pushed;
+  int32_t ev_count, ev_pushed;
   bool rejected_last;
 
   // Tolerances of state component c (x_j: c = j, v_j: c = nm + j); the
@@ -1335,6 +1337,122 @@ struct GroupDopri5 {
       for (int32_t e = next_eval; e < a.n_eval; ++e)
         write_state(dbase + (int64_t)e * 2 * nm, nan_, nan_);
     }
+    if (a.ev.solout != kSoloutDefault) {  // event count, NaN past the last point
+      const EventArgs& e = a.ev;
+      if (l == 0 && e.n_out) e.n_out[sys] = ev_pushed;
+      double nan_[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) nan_[q] = as_f64(kNaNBits);
+      for (int32_t k = ev_pushed; k < e.max_events; ++k) {
+        if (l == 0) e.t_out[(int64_t)k * a.n + sys] = as_f64(kNaNBits);
+        write_state(e.y_out + (int64_t)e.max_events * so + (int64_t)k * 2 * nm, nan_, nan_);
+      }
+    }
+  }
+
+  // ---- event solouts (inline roots, as VecCommon's): the event component
+  // lives on one lane of the group and reaches the others by a permute, so
+  // every lane runs the same root search on the same values.
+  DE_DEV double component(const double* xs, const double* vs, int c) const {
+    const int j = c < nm ? c : c - nm;
+    const int q = j % C;
+    double mine = c < nm ? xs[0] : vs[0];
+#pragma unroll
+    for (int r = 1; r < C; ++r)
+      if (q == r) mine = c < nm ? xs[r] : vs[r];
+    return __shfl(mine, j / C, G);
+  }
+  DE_DEV double ev_g(const EventArgs& e, double tt, const double* xs, const double* vs) const {
+    return e.fn == kEventTime ? tt - ev_value : component(xs, vs, e.comp) - ev_value;
+  }
+  DE_DEV void ev_start(const OdeArgs& a) {
+    if (a.ev.solout == kSoloutDefault) return;
+    ev_value = a.ev.value_traj ? a.ev.value_traj[sys] : a.ev.value;
+    ev_last = ev_g(a.ev, t, x, v);
+  }
+  DE_DEV bool ev_pre(const OdeArgs& a, double t_curr, const double* xs, const double* vs) {
+    const EventArgs& e = a.ev;
+    if (e.solout == kSoloutDefault) return false;
+    ev_new = ev_g(e, t_curr, xs, vs);
+    const double lo = ev_last, g = ev_new;
+    const bool change = sign_differs(lo, g);
+    if (e.solout == kSoloutEvent) {
+      const bool ok = e.direction == kCrossPositive   ? change && lo < 0.0 && g >= 0.0
+                      : e.direction == kCrossNegative ? change && lo > 0.0 && g <= 0.0
+                                                      : change;
+      return ok && lo * g <= 0.0;
+    }
+    const bool rec = e.direction == kCrossPositive   ? lo < 0.0 && g >= 0.0
+                     : e.direction == kCrossNegative ? lo > 0.0 && g <= 0.0
+                                                     : true;
+    return change && rec;
+  }
+  // Interpolates the state at tt into the next event slot (while the slots
+  // last) and counts the point.
+  template <class Interp>
+  DE_DEV bool ev_record(const OdeArgs& a, double tt, Interp& interp) {
+    const EventArgs& e = a.ev;
+    double ox[C], oy[C];
+    if (!interp(tt, ox, oy)) return false;
+    if (ev_pushed < e.max_events) {
+      if (l == 0) e.t_out[(int64_t)ev_pushed * a.n + sys] = tt;
+      write_state(e.y_out + (int64_t)e.max_events * so + (int64_t)ev_pushed * 2 * nm, ox, oy);
+    }
+    ev_pushed += 1;
+    return true;
+  }
+  template <class Interp>
+  DE_DEV int32_t ev_post(const OdeArgs& a, bool need, double t_curr, double t_prev,
+                         Interp& interp) {
+    if (!need) return -1;
+    const EventArgs& e = a.ev;
+    EventArgs e1 = e;
+    e1.comp = 0;  // g of the projected one-component state
+    auto interp1 = [&](double tt, double* o) -> bool {
+      double ox[C], oy[C];
+      if (!interp(tt, ox, oy)) return false;
+      o[0] = e.fn == kEventTime ? 0.0 : component(ox, oy, e.comp);
+      return true;
+    };
+    int32_t st = -1;
+    if (e.solout == kSoloutEvent) {
+      double a0 = t_prev, b0 = t_curr, fa = ev_last, fb = ev_new;
+      if ((dir > 0.0 && a0 > b0) || (dir < 0.0 && a0 < b0)) {
+        double w = a0;
+        a0 = b0;
+        b0 = w;
+        w = fa;
+        fa = fb;
+        fb = w;
+      }
+      const double val = ev_value;
+      auto g = [&](double tt, const double* yy) { return event_g<1>(e1, val, tt, yy); };
+      double t_event;
+      if (brent_dekker<1>(a0, b0, fa, fb, interp1, g, &t_event)) {
+        const double tl = a.n_eval > 0 ? tail : t_curr;
+        if (tl != tl || fabs(t_event - tl) > 1e-14) {
+          if (!ev_record(a, t_event, interp)) return kOutOfBounds;
+          tail = t_event;
+        } else {
+          double ox[C], oy[C];  // the reference interpolates before the push test
+          if (!interp(t_event, ox, oy)) return kOutOfBounds;
+        }
+        ev_count += 1;
+        if (e.terminate > 0 && ev_count >= e.terminate) st = kInterrupted;
+      }
+    } else {
+      LaneEvent<1, 2> cs;
+      cs.value = ev_value;
+      bool found = false;
+      double tc = 0.0;
+      if (!cs.newton(e1, t_prev, t_curr, ev_last, ev_new, interp1, &found, &tc)) return kOutOfBounds;
+      if (!found) {
+        const double frac = -ev_last / (ev_new - ev_last);
+        tc = t_prev + frac * (t_curr - t_prev);
+      }
+      if (!ev_record(a, tc, interp)) st = kOutOfBounds;
+    }
+    return st;
   }
 
   // As VecCommon::emit; interp returns false outside the step.  The time
@@ -1363,6 +1481,7 @@ struct GroupDopri5 {
           write_state(out, ox, ov);
         }
         pushed += 1;
+        tail = te;
         idx += 1;
       } else {
         if ((dir > 0.0 && te > t_curr) || (dir < 0.0 && te < t_curr)) break;
@@ -1469,6 +1588,8 @@ struct GroupDopri5 {
     t = t0;
     acc = rej = ev = steps = 0;
     pushed = next_eval = 0;
+    ev_count = ev_pushed = 0;
+    tail = as_f64(kNaNBits);
     dir = signum(tf - t0);
     bool ok = dir == 1.0 || dir == -1.0;
     double h0 = a.h0;
@@ -1536,6 +1657,7 @@ struct GroupDopri5 {
     } else if (a.n_eval > 0) {
       emit(A, t, t, x, v, [](double, double*, double*) { return true; });
     }
+    ev_start(a);
     return true;
   }
 
@@ -1675,10 +1797,12 @@ struct GroupDopri5 {
         }
       }
       bool emitted = true;
+      int32_t sol = -1;
       const bool pts = a.pts_mode != kPointsNone;
+      const bool need = ev_pre(a, t_new, nx, nv);
       if (pts && a.pts_mode == kPointsSteps) {
         pts_push(A, t_new, nx, nv);
-      } else if (a.n_eval > 0 || pts) {
+      } else if (a.n_eval > 0 || pts || need) {
         double c1x[C], c1v[C], c2x[C], c2v[C], c3x[C], c3v[C];
 #pragma unroll
         for (int q = 0; q < C; ++q) {
@@ -1709,9 +1833,13 @@ struct GroupDopri5 {
           }
           return true;
         };
-        emitted = pts ? points_step(A, t_new, t, nx, nv, interp)
-                      : emit(A, t_new, t, nx, nv, interp);
+        if (pts)
+          emitted = points_step(A, t_new, t, nx, nv, interp);
+        else if (a.n_eval > 0)
+          emitted = emit(A, t_new, t, nx, nv, interp);
+        sol = emitted ? ev_post(a, need, t_new, t, interp) : kOutOfBounds;
       }
+      if (a.ev.solout != kSoloutDefault) ev_last = ev_new;
       h_prev = h;
       t = t_new;
 #pragma unroll
@@ -1728,8 +1856,8 @@ struct GroupDopri5 {
       acc += 1;
       ev += 7;
       h = constrain_step_size(h * scale, a.h_min, a.h_max);
-      if (!emitted) {
-        finish(A, kOutOfBounds);
+      if (sol >= 0) {
+        finish(A, sol);
         return true;
       }
       if (fabs(tf - t) <= kTenEps) {

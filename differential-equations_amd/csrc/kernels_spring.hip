@@ -223,12 +223,10 @@ static Choice autotune(const SpringArgs& base, int k, const int32_t* d_list, int
   return best;
 }
 
-// One thread per system: DE_MAP_LANE, every method but DOPRI5 (the group
-// mapping is built for DOPRI5 with the t_eval or a point output) and the
-// event solouts.
+// One thread per system: DE_MAP_LANE and every method but DOPRI5 (the group
+// mapping is built for DOPRI5, with every output and event solout).
 static bool lane_mapping(const de_ode_batch* b) {
-  return b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5 ||
-         b->event_solout != DE_SOLOUT_DEFAULT;
+  return b->mapping == DE_MAP_LANE || b->method != DE_METHOD_DOPRI5;
 }
 
 // Start of each system's point states in points_y (n + 1 values): the slot

@@ -186,7 +186,7 @@ typedef struct de_ode_batch {
   const double* rtol_comp;
   const double* atol_comp;
   /* Event solouts (DE_SOLOUT_*; spring chains: component and time events and the
-   * crossing solout, on the one-lane-per-system mapping).  The event points of each
+   * crossing solout).  The event points of each
    * trajectory -- the points the reference's event / crossing solout pushes into
    * Solution -- come back in de_ode_result.event_t / event_y. */
   int32_t event_solout;    /* DE_SOLOUT_* */

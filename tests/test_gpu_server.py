@@ -180,8 +180,9 @@ def test_served_methods_match_direct_solves(graph):
         spr(E.dop853().rtol(1e-8).atol(1e-10)),
         spr(E.rk4(0.01)).t_eval(np.linspace(0.0, 2.0, 9)),
         spr(E.rkv767e().rtol(1e-8).atol(1e-10)),
+        # served on one lane per system: its direct solve on the same mapping
         spr(E.dopri5().rtol(1e-7).atol(1e-9)).event(Event.component(1, 0.0))
-        .options(max_events=4),
+        .options(max_events=4, mapping=deamd.DE_MAP_LANE),
         spr(E.dop853().rtol(1e-8).atol(1e-10)).crossing(0, 0.0, CrossingDirection.Positive)
         .options(max_events=2),
     ]

@@ -355,6 +355,28 @@ def same_events(got, want):
     assert np.array_equal(np.nan_to_num(got.event_y), np.nan_to_num(want.event_y))
 
 
+def events_within_bound(sysm, got, want):
+    # DOPRI5 on the group mappings (butterfly norm: step sizes differ in their
+    # last bits): equal statuses, step counts and event counts equal on at
+    # least 99.9% of systems, and where both are and the solve completed, the
+    # final state within 1e-10 and every kept event point's time within 1e-8
+    # (relative to |tf|), its state within 1e-7.
+    assert (got.status == want.status).all()
+    same = (got.accepted == want.accepted) & (got.rejected == want.rejected)
+    assert same.mean() >= 0.999
+    same_ev = same & (got.n_events == want.n_events)
+    assert same_ev.mean() >= 0.999
+    done = same_ev & (got.status == DE_STATUS_COMPLETE)
+    assert (rel_state_error(sysm, got, want)[done] <= 1e-10).all()
+    for i in np.nonzero(same_ev)[0]:
+        gt, gy = got.events(i)
+        wt, wy = want.events(i)
+        assert gt.shape == wt.shape
+        assert np.abs(gt - wt).max(initial=0.0) <= 1e-8 * max(1.0, np.abs(wt).max(initial=0.0))
+        assert np.abs(gy - wy).max(initial=0.0) <= 1e-7 * max(np.abs(wy).max(initial=0.0),
+                                                                1e-300)
+
+
 EVENT_KINDS = {
     "component": lambda e: e.event(Event.component(1, 0.0)).options(max_events=16),
     "terminate": lambda e: e.event(Event.component(
@@ -372,9 +394,10 @@ EVENT_KINDS = {
 @pytest.mark.parametrize("kind", list(EVENT_KINDS))
 @pytest.mark.parametrize("method", ["dopri5", "dop853", "rk4", "rkv767e"])
 def test_event_solouts_bit_exact(orc, method, kind):
-    # Event and crossing solouts on spring chains (one lane per system, roots
-    # searched inline, event states back to back per system) equal the oracle
-    # bit for bit; a few slots per system, so some points are counted and not
+    # Event and crossing solouts on spring chains (roots searched inline,
+    # event states back to back per system) equal the oracle bit for bit on
+    # one lane per system, and within the group mappings' bound there
+    # (DOPRI5); a few slots per system, so some points are counted and not
     # kept.
     sysm, y0 = deamd.gen_spring(300, seed=37)
     m = {"dopri5": lambda: ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9),
@@ -391,6 +414,9 @@ def test_event_solouts_bit_exact(orc, method, kind):
         assert (want.n_events > 0).mean() > 0.25
     for mapping in (DE_MAP_AUTO, DE_MAP_LANE):
         got = ens.options(mapping=mapping).solve()
+        if method == "dopri5" and mapping == DE_MAP_AUTO:
+            events_within_bound(sysm, got, want)
+            continue
         for f in ("status", "accepted", "rejected", "evals", "t", "y"):
             assert np.array_equal(getattr(got, f), getattr(want, f)), (f, mapping)
         same_events(got, want)
@@ -410,7 +436,8 @@ def test_event_solouts_backward_and_refusals(orc):
     for ens in (mk().event(Event.component(1, 0.0)).options(max_events=8),
                 mk().crossing(1, 0.0).options(max_events=8)):
         want = orc.solve_ode(ens, threads=THREADS)
-        got = ens.solve()
+        events_within_bound(sysm, ens.options(mapping=DE_MAP_AUTO).solve(), want)
+        got = ens.options(mapping=DE_MAP_LANE).solve()
         for f in ("status", "accepted", "rejected", "evals", "t", "y"):
             assert np.array_equal(getattr(got, f), getattr(want, f)), f
         same_events(got, want)
