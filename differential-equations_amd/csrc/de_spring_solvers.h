@@ -17,7 +17,7 @@
 //                 group.  The butterfly sums in a different order
 //                 than the reference's sequential loop (traits.rs:398-409), so
 //                 results agree to rounding rather than bit for bit.  DOPRI5
-//                 with the t_eval output or a point output.
+//                 with the t_eval output or a point output, and event solouts.
 #pragma once
 
 #include "de_erk_tables.h"

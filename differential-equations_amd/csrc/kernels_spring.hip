@@ -3,8 +3,8 @@
 // Mappings (selected by de_ode_batch.mapping):
 //   DE_MAP_LANE   one thread per system, vectors in an HBM workspace (plain
 //                 launch; bit-identical to the CPU restatement).  DOPRI5, and
-//                 the only mapping of DOP853, RK4, Euler, the embedded family
-//                 and the point outputs.
+//                 the only mapping of DOP853, RK4, Euler and the embedded
+//                 family.
 //   DE_MAP_GROUP  G lanes per system with C masses per lane in registers.
 //   DE_MAP_AUTO   systems are bucketed into size classes (N <= 4, 8, ..., 256);
 //                 for each class an autotuner times every (G, C, block) with
