@@ -114,3 +114,29 @@ def test_points_beyond_the_slots_are_counted_not_written(orc):
         assert len(t) == k and y.shape == (k, 2 * SIZES[i])
         ft, fy = full.points(i)
         assert np.array_equal(t, ft[:k]) and np.array_equal(y, fy[:k])
+
+
+def test_dataframes_of_ragged_systems(orc):
+    # Solution::to_polars of each spring chain (a pandas DataFrame here):
+    # columns t, y0 .. y{2N-1} of that system's own length, rows its points,
+    # its t_eval points produced, or its final state.
+    sysm, y0 = chains()
+    base = lambda: IVP.ode(sysm, 0.0, 2.0, y0).method(METHODS["dopri5"]())
+    pts = orc.solve_ode(base().all_steps())
+    te = np.linspace(0.0, 2.0, 5)
+    tev = orc.solve_ode(base().t_eval(te))
+    fin = orc.solve_ode(base())
+    for i in range(len(SIZES)):
+        n = 2 * int(SIZES[i])
+        a, b = sysm.state_offset[i], sysm.state_offset[i + 1]
+        df = pts.to_dataframe(i)
+        t, y = pts.points(i)
+        assert list(df.columns) == ["t"] + ["y%d" % c for c in range(n)]
+        assert np.array_equal(df.to_numpy(), np.column_stack([t, y]))
+        df = tev.to_dataframe(i)
+        rows = tev.dense[len(te) * a:len(te) * b].reshape(len(te), n)
+        assert np.array_equal(df["t"].to_numpy(), te)
+        assert np.array_equal(df.to_numpy()[:, 1:], rows)
+        df = fin.to_dataframe(i)
+        assert df.shape == (1, n + 1) and df["t"][0] == fin.t[i]
+        assert np.array_equal(df.to_numpy()[0, 1:], fin.y[a:b])
