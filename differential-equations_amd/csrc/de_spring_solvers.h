@@ -1252,7 +1252,9 @@ constexpr int kDppMirror = 0x140;     // row_mirror: lane i reads 15 - i in its 
 constexpr int kDppWaveShr1 = 0x138;   // wave_shr:1: lane i reads lane i - 1
 constexpr int kDppWaveShl1 = 0x130;   // wave_shl:1: lane i reads lane i + 1
 
-template <int G, int C>
+// OUT: the point outputs and event solouts are compiled in (their paths
+// would otherwise raise the register count of every group kernel).
+template <int G, int C, bool OUT = false>
 struct GroupDopri5 {
   double x[C], v[C];      // y
   double k0x[C], k0v[C];  // f(t, y)
@@ -1327,7 +1329,7 @@ struct GroupDopri5 {
       a.evals[sys] = ev;
       a.status[sys] = st;
       if (a.n_dense) a.n_dense[sys] = pushed;
-      if (a.pts_mode != kPointsNone && a.n_pts) a.n_pts[sys] = pushed;
+      if (OUT && a.pts_mode != kPointsNone && a.n_pts) a.n_pts[sys] = pushed;
     }
     if (a.n_eval > 0) {  // points not reached (all of them on BadInput)
       double nan_[C];
@@ -1337,7 +1339,7 @@ struct GroupDopri5 {
       for (int32_t e = next_eval; e < a.n_eval; ++e)
         write_state(dbase + (int64_t)e * 2 * nm, nan_, nan_);
     }
-    if (a.ev.solout != kSoloutDefault) {  // event count, NaN past the last point
+    if (OUT && a.ev.solout != kSoloutDefault) {  // event count, NaN past the last point
       const EventArgs& e = a.ev;
       if (l == 0 && e.n_out) e.n_out[sys] = ev_pushed;
       double nan_[C];
@@ -1651,13 +1653,13 @@ struct GroupDopri5 {
     stiff = nonstiff = 0;
     countdown = 100;
     rejected_last = false;
-    if (a.pts_mode != kPointsNone) {
+    if (OUT && a.pts_mode != kPointsNone) {
       pts_push(A, t, x, v);
       last = t;
     } else if (a.n_eval > 0) {
       emit(A, t, t, x, v, [](double, double*, double*) { return true; });
     }
-    ev_start(a);
+    if constexpr (OUT) ev_start(a);
     return true;
   }
 
@@ -1798,8 +1800,9 @@ struct GroupDopri5 {
       }
       bool emitted = true;
       int32_t sol = -1;
-      const bool pts = a.pts_mode != kPointsNone;
-      const bool need = ev_pre(a, t_new, nx, nv);
+      const bool pts = OUT && a.pts_mode != kPointsNone;
+      bool need = false;
+      if constexpr (OUT) need = ev_pre(a, t_new, nx, nv);
       if (pts && a.pts_mode == kPointsSteps) {
         pts_push(A, t_new, nx, nv);
       } else if (a.n_eval > 0 || pts || need) {
@@ -1833,13 +1836,20 @@ struct GroupDopri5 {
           }
           return true;
         };
-        if (pts)
-          emitted = points_step(A, t_new, t, nx, nv, interp);
-        else if (a.n_eval > 0)
+        if constexpr (OUT) {
+          if (pts)
+            emitted = points_step(A, t_new, t, nx, nv, interp);
+          else if (a.n_eval > 0)
+            emitted = emit(A, t_new, t, nx, nv, interp);
+          sol = emitted ? ev_post(a, need, t_new, t, interp) : kOutOfBounds;
+        } else {
           emitted = emit(A, t_new, t, nx, nv, interp);
-        sol = emitted ? ev_post(a, need, t_new, t, interp) : kOutOfBounds;
+          sol = emitted ? -1 : kOutOfBounds;
+        }
       }
-      if (a.ev.solout != kSoloutDefault) ev_last = ev_new;
+      if constexpr (OUT) {
+        if (a.ev.solout != kSoloutDefault) ev_last = ev_new;
+      }
       h_prev = h;
       t = t_new;
 #pragma unroll

@@ -69,13 +69,13 @@ static LaneSolver lane_solver(int method) {
 // Occupancy hints: the rarely taken double-double pow fallback raises the
 // peak register count past a wave-slot boundary; the hints trade that for a
 // few spills outside the step block.
-template <int G, int C>
+template <int G, int C, bool OUT>
 __global__ __launch_bounds__(kSpringBlock) __attribute__((amdgpu_waves_per_eu(C == 1 ? 3 : 2))) void
 spring_group_kernel(SpringArgs A) {
   const int lane = __lane_id();
   const int l = lane & (G - 1);
   const int leader_lane = lane - l;
-  GroupDopri5<G, C> S;
+  GroupDopri5<G, C, OUT> S;
   bool have = false;
   bool drained = false;
   const unsigned long long len = (unsigned long long)A.list_len;
@@ -114,10 +114,11 @@ typedef void (*GroupKernel)(SpringArgs);
 
 struct Config {
   int g, c;
-  GroupKernel k;
+  GroupKernel k;   // final state and t_eval output
+  GroupKernel kx;  // with the point outputs and event solouts
 };
 
-#define DE_CFG(G, C) {G, C, spring_group_kernel<G, C>}
+#define DE_CFG(G, C) {G, C, spring_group_kernel<G, C, false>, spring_group_kernel<G, C, true>}
 static const Config kConfigs[] = {
     DE_CFG(4, 1),  DE_CFG(8, 1),  DE_CFG(16, 1), DE_CFG(32, 1), DE_CFG(64, 1),
     DE_CFG(2, 2),  DE_CFG(4, 2),  DE_CFG(8, 2),  DE_CFG(16, 2), DE_CFG(32, 2), DE_CFG(64, 2),
@@ -173,6 +174,12 @@ static int blocks_per_cu(GroupKernel k, int block) {
   return b;
 }
 
+// The group kernel of a launch: with the point outputs and event solouts
+// compiled in only when the launch asks for one of them.
+static GroupKernel class_kernel(const SpringArgs& base, const Config& cf) {
+  return base.o.pts_mode != kPointsNone || base.o.ev.solout != kSoloutDefault ? cf.kx : cf.k;
+}
+
 static int launch_class(const SpringArgs& base, const int32_t* d_list, int64_t len,
                         int64_t work_share_blocks, const Choice& ch, unsigned long long* counter,
                         hipStream_t s) {
@@ -186,7 +193,7 @@ static int launch_class(const SpringArgs& base, const int32_t* d_list, int64_t l
   int64_t grid = work_share_blocks < need ? work_share_blocks : need;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(spring_zero_counter, dim3(1), dim3(1), 0, s, counter);
-  hipLaunchKernelGGL(cf.k, dim3((unsigned)grid), dim3(ch.block), 0, s, A);
+  hipLaunchKernelGGL(class_kernel(base, cf), dim3((unsigned)grid), dim3(ch.block), 0, s, A);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -205,7 +212,7 @@ static Choice autotune(const SpringArgs& base, int k, const int32_t* d_list, int
       Choice ch;
       ch.cfg = ci;
       ch.block = block;
-      int64_t full = (int64_t)g_num_cu * blocks_per_cu(kConfigs[ci].k, block);
+      int64_t full = (int64_t)g_num_cu * blocks_per_cu(class_kernel(base, kConfigs[ci]), block);
       (void)hipEventRecord(e0, s);
       if (launch_class(base, d_list, sample, full, ch, g_counters + kNumCaps, s) != 0) continue;
       (void)hipEventRecord(e1, s);
@@ -397,7 +404,7 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     const int k = kLaunchOrder[q];
     if (lists[k].empty()) continue;
     const Choice& ch = choice[k];
-    int64_t share = (int64_t)g_num_cu * blocks_per_cu(kConfigs[ch.cfg].k, ch.block);
+    int64_t share = (int64_t)g_num_cu * blocks_per_cu(class_kernel(base, kConfigs[ch.cfg]), ch.block);
     if (hipStreamWaitEvent(g_streams[k], g_fork, 0) != hipSuccess) return -1;
     if (launch_class(base, d_list + start[k], (int64_t)lists[k].size(), share, ch,
                      g_counters + k, g_streams[k]) != 0)
