@@ -142,8 +142,6 @@ static std::map<int, Choice> g_tuned;
 static hipStream_t g_streams[kNumCaps] = {};
 static hipEvent_t g_fork = nullptr, g_join[kNumCaps] = {};
 static unsigned long long* g_counters = nullptr;
-static double* g_work = nullptr;
-static int64_t g_work_len = 0;
 static int g_num_cu = 0;
 
 static int cap_index(int nm) {
@@ -312,26 +310,26 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
     const bool events = b->event_solout != DE_SOLOUT_DEFAULT;
     const int64_t vec_len = ((int64_t)ls.vectors + (events ? 1 : 0)) * state_len;
     const int64_t need = vec_len + (pts ? n + 1 : 0);
-    if (g_work_len < need) {
-      if (g_work) (void)hipFree(g_work);
-      g_work = nullptr;
-      g_work_len = 0;
-      if (hipMalloc(&g_work, sizeof(double) * need) != hipSuccess) return -4;
-      g_work_len = need;
-    }
-    base.work = g_work;
-    if (events) base.ev_work = g_work + (int64_t)ls.vectors * state_len;
+    // The vectors come from the library's workspace pool (within its budget)
+    // and return to it, stream-ordered, after the launch.
+    double* work = nullptr;
+    if (workspace_alloc((void**)&work, sizeof(double) * need, s) != hipSuccess) return -4;
+    base.work = work;
+    if (events) base.ev_work = work + (int64_t)ls.vectors * state_len;
     if (pts) {
-      int64_t* d_ypos = (int64_t*)(g_work + vec_len);
+      int64_t* d_ypos = (int64_t*)(work + vec_len);
       if (hipMemcpyAsync(d_ypos, ypos.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
                          s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
+          hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipFreeAsync(work, s);
         return -1;
+      }
       base.pts_ypos = d_ypos;
     }
     int64_t grid = (n + kSpringBlock - 1) / kSpringBlock;
     hipLaunchKernelGGL(ls.kernel, dim3((unsigned)grid), dim3(kSpringBlock), 0, s, base);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    const hipError_t e = hipGetLastError();
+    return hipFreeAsync(work, s) == hipSuccess && e == hipSuccess ? 0 : -1;
   }
   // Group mapping: upload class lists (one buffer, classes back to back).
   std::vector<int32_t> all;
@@ -344,13 +342,15 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
   // The point states' offsets follow the lists, 8-byte aligned.
   const size_t list_bytes = (sizeof(int32_t) * (all.size() + 1) + 7) & ~(size_t)7;
   int32_t* d_list = nullptr;
-  if (hipMalloc(&d_list, list_bytes + (pts ? sizeof(int64_t) * (n + 1) : 0)) != hipSuccess)
+  if (workspace_alloc((void**)&d_list, list_bytes + (pts ? sizeof(int64_t) * (n + 1) : 0), s) !=
+      hipSuccess)
     return -4;
-  if (hipMemcpy(d_list, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice) !=
-          hipSuccess ||
-      (pts && hipMemcpy((char*)d_list + list_bytes, ypos.data(), sizeof(int64_t) * (n + 1),
-                        hipMemcpyHostToDevice) != hipSuccess)) {
-    (void)hipFree(d_list);
+  if (hipMemcpyAsync(d_list, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice,
+                     s) != hipSuccess ||
+      (pts && hipMemcpyAsync((char*)d_list + list_bytes, ypos.data(), sizeof(int64_t) * (n + 1),
+                             hipMemcpyHostToDevice, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipFreeAsync(d_list, s);
     return -1;
   }
   if (pts) base.pts_ypos = (const int64_t*)((const char*)d_list + list_bytes);
@@ -377,7 +377,7 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
       } else {
         ch = autotune(base, k, d_list + start[k], (int64_t)lists[k].size(), s);
         if (ch.cfg < 0) {
-          (void)hipFree(d_list);
+          (void)hipFreeAsync(d_list, s);
           return -1;
         }
         g_tuned[k] = ch;
@@ -398,24 +398,24 @@ int launch_spring(const de_ode_batch* b, const int32_t* host_sizes, const OdeArg
   // GPU idle whenever classes waited behind others for one of the few
   // hardware queues a process gets; ordering each class's systems by size was
   // slower, 14.9 ms: it scatters the group's memory accesses.)
-  if (hipEventRecord(g_fork, s) != hipSuccess) return -1;
   static const int kLaunchOrder[kNumCaps] = {0, 1, 2, 6, 5, 4, 3};
-  for (int q = 0; q < kNumCaps; ++q) {
+  bool ok = hipEventRecord(g_fork, s) == hipSuccess;
+  for (int q = 0; q < kNumCaps && ok; ++q) {
     const int k = kLaunchOrder[q];
     if (lists[k].empty()) continue;
     const Choice& ch = choice[k];
     int64_t share = (int64_t)g_num_cu * blocks_per_cu(class_kernel(base, kConfigs[ch.cfg]), ch.block);
-    if (hipStreamWaitEvent(g_streams[k], g_fork, 0) != hipSuccess) return -1;
-    if (launch_class(base, d_list + start[k], (int64_t)lists[k].size(), share, ch,
-                     g_counters + k, g_streams[k]) != 0)
-      return -1;
-    if (hipEventRecord(g_join[k], g_streams[k]) != hipSuccess) return -1;
-    if (hipStreamWaitEvent(s, g_join[k], 0) != hipSuccess) return -1;
+    ok = hipStreamWaitEvent(g_streams[k], g_fork, 0) == hipSuccess &&
+         launch_class(base, d_list + start[k], (int64_t)lists[k].size(), share, ch,
+                      g_counters + k, g_streams[k]) == 0 &&
+         hipEventRecord(g_join[k], g_streams[k]) == hipSuccess &&
+         hipStreamWaitEvent(s, g_join[k], 0) == hipSuccess;
   }
-  // The list buffer is released once the classes are done.
-  if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  (void)hipFree(d_list);
-  return 0;
+  // The list buffer goes back to the pool after the classes (stream order);
+  // the call returns once they are done (the class counters are shared).
+  ok = hipFreeAsync(d_list, s) == hipSuccess && ok;
+  ok = hipStreamSynchronize(s) == hipSuccess && ok;
+  return ok ? 0 : -1;
 }
 
 // Autotuner introspection: chosen (G, C, block) for size class cap, or -1.

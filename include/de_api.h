@@ -348,7 +348,8 @@ const char* de_version(void);
 /* ---- Device-memory budget of the launch workspace ------------------------
  * No counterpart in the reference (it allocates per trajectory on the host).
  * Batch solves take their stream-ordered workspace (launch arguments, the
- * longest-first order, the bracket regions of deferred event roots) from a
+ * longest-first order, the bracket regions of deferred event roots, the
+ * spring chains' class lists and one-lane vectors) from a
  * library pool that keeps freed blocks for the next solve.  One budget, 64 GiB
  * by default, bounds each workspace allocation and the blocks the pool keeps;
  * the bracket regions of one event pass stay within 3/4 of it (an ensemble
