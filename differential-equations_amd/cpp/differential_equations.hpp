@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -218,6 +219,10 @@ class EnsembleSolution {
   int32_t max_events = 0;
   std::vector<double> event_t, event_y;
   std::vector<int32_t> n_events;
+  // A point output (all_steps / dense / even): trajectory i's points in slots
+  // points_offset[i] .. points_offset[i] + min(n_points[i], its slots).
+  std::vector<int64_t> points_offset, n_points;
+  std::vector<double> points_t, points_y;
 
   // The reference's Result<Solution, Error> of trajectory i.
   Result at(int64_t i) const {
@@ -256,7 +261,15 @@ class EnsembleSolution {
     }
     r.value.event_t = et;
     r.value.event_y = ey;
-    if (t_eval.empty()) {  // final state (the last point DefaultSolout keeps)
+    if (!points_offset.empty()) {  // the Solution's points
+      const int64_t lo = points_offset[i];
+      const int64_t kept = std::min(n_points[i], points_offset[i + 1] - lo);
+      for (int64_t k = 0; k < kept; ++k) {
+        r.value.t.push_back(points_t[lo + k]);
+        r.value.y.emplace_back(points_y.begin() + (lo + k) * dim,
+                               points_y.begin() + (lo + k + 1) * dim);
+      }
+    } else if (t_eval.empty()) {  // final state (the last point DefaultSolout keeps)
       r.value.t.push_back(t[i]);
       r.value.y.push_back(yi);
     } else {               // TEvalSolout points that were produced
@@ -287,8 +300,16 @@ class IVP {
     IVP p = *this;
     p.t_eval_ = points;
     p.solout_ = DE_SOLOUT_DEFAULT;  // with_solout replaces the whole solout
+    p.points_ = DE_POINTS_NONE;
     return p;
   }
+  // The default Solution of .solve() (src/solout/default.rs:71-91): (t0, y0)
+  // and every accepted step.  Without it (or another output) the ensemble
+  // keeps only each trajectory's final state.
+  IVP all_steps() const { return with_points(DE_POINTS_STEPS, 0, 0.0); }
+  // IVP::dense(n) and IVP::even(dt) (src/ivp.rs:643-655).
+  IVP dense(int32_t n) const { return with_points(DE_POINTS_DENSE, n, 0.0); }
+  IVP even(double dt) const { return with_points(DE_POINTS_EVEN, 0, dt); }
   // Launch schedule (DE_SCHED_*): persistent lane refill by default.
   IVP schedule(int32_t s) const { IVP p = *this; p.schedule_ = s; return p; }
   // IVP::event (src/ivp.rs:662-678): event detection around the current output.
@@ -319,7 +340,39 @@ class IVP {
   // Event points kept per trajectory.
   IVP max_events(int32_t k) const { IVP p = *this; p.max_events_ = k; return p; }
 
+  // With a point output from all_steps() or dense(n), a first solve counts
+  // each trajectory's accepted steps (results are deterministic) and a second
+  // writes the points; even(dt) needs floor(|tf - t0| / dt) + 3 slots.
   EnsembleSolution solve() const {
+    const int64_t n = (int64_t)y0_.size();
+    if (points_ == DE_POINTS_NONE) return solve_once(nullptr);
+    std::vector<int64_t> off((size_t)n + 1, 0);
+    if (points_ == DE_POINTS_EVEN) {
+      const int64_t cnt = (int64_t)std::floor(std::fabs(tf_ - t0_) / points_dt_) + 3;
+      for (int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + cnt;
+    } else {
+      IVP first = *this;
+      first.points_ = DE_POINTS_NONE;
+      const EnsembleSolution s = first.solve_once(nullptr);
+      for (int64_t i = 0; i < n; ++i)
+        off[i + 1] = off[i] + (points_ == DE_POINTS_STEPS ? s.accepted[i] + 1
+                                                          : points_n_ * s.accepted[i] + 1);
+    }
+    return solve_once(&off);
+  }
+
+ private:
+  IVP with_points(int32_t mode, int32_t n, double dt) const {
+    IVP p = *this;
+    p.points_ = mode;
+    p.points_n_ = n;
+    p.points_dt_ = dt;
+    p.t_eval_.clear();
+    p.solout_ = DE_SOLOUT_DEFAULT;
+    return p;
+  }
+
+  EnsembleSolution solve_once(const std::vector<int64_t>* points_offset) const {
     const int64_t n = (int64_t)y0_.size();
     de_ode_batch b = m_.settings();
     b.family = System::family;
@@ -413,12 +466,24 @@ class IVP {
     r.status = s.status.data();
     r.dense = te.empty() ? nullptr : s.dense.data();
     r.n_dense = te.empty() ? nullptr : s.n_dense.data();
+    if (points_offset) {
+      b.points = points_;
+      b.points_n = points_n_;
+      b.points_dt = points_dt_;
+      s.points_offset = *points_offset;
+      s.n_points.resize(n);
+      s.points_t.resize((size_t)points_offset->back());
+      s.points_y.resize((size_t)points_offset->back() * System::dim);
+      r.points_offset = s.points_offset.data();
+      r.n_points = s.n_points.data();
+      r.points_t = s.points_t.empty() ? nullptr : s.points_t.data();
+      r.points_y = s.points_y.empty() ? nullptr : s.points_y.data();
+    }
     int rc = de_ode_batch_solve(&b, &r, nullptr);
     if (rc != DE_OK) throw std::runtime_error(std::string("de_ode_batch_solve: ") + de_last_error());
     return s;
   }
 
- private:
   IVP(const System& sys, double t0, double tf,
       const std::vector<std::array<double, System::dim>>& y0)
       : sys_(sys), t0_(t0), tf_(tf), y0_(y0), m_(ExplicitRungeKutta::dopri5()) {}
@@ -431,6 +496,8 @@ class IVP {
   int32_t solout_ = DE_SOLOUT_DEFAULT;
   Event event_;
   int32_t max_events_ = 16;
+  int32_t points_ = DE_POINTS_NONE, points_n_ = 0;
+  double points_dt_ = 0.0;
 };
 
 }  // namespace differential_equations

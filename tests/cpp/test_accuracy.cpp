@@ -191,6 +191,34 @@ int main() {
   }
   std::printf("events passed\n");
 
+  // The default Solution (every accepted step), dense(3) and even(0.5) on
+  // y'' = -y, y(0) = (1, 0): (t0, y0) first, the final state last, A + 1 and
+  // 3A + 1 points, the grid 0, 0.5, ..., 20, and cos t along the way.
+  {
+    std::vector<std::array<double, 2>> y0s(3, {1.0, 0.0});
+    auto ivp = IVP<HarmonicOscillator>::ode(HarmonicOscillator{{1.0}}, 0.0, 20.0, y0s)
+                   .method(ExplicitRungeKutta::dopri5().rtol(1e-10).atol(1e-12));
+    auto steps = ivp.all_steps().solve();
+    auto dense = ivp.dense(3).solve();
+    auto even = ivp.even(0.5).solve();
+    for (int64_t i = 0; i < steps.n; ++i) {
+      Result a = steps.at(i), d = dense.at(i), e = even.at(i);
+      CHECK(a.ok && (int64_t)a.value.t.size() == a.value.steps.accepted + 1, "steps: A + 1");
+      CHECK(d.ok && (int64_t)d.value.t.size() == 3 * d.value.steps.accepted + 1, "dense: 3A + 1");
+      CHECK(e.ok && e.value.t.size() == 41, "even: 41 points");
+      if (!a.ok || !d.ok || !e.ok) continue;
+      CHECK(a.value.t.front() == 0.0 && a.value.y.front()[0] == 1.0, "steps: t0, y0 first");
+      CHECK(a.value.t.back() == steps.t[i] && a.value.y.back()[0] == steps.y[i], "steps: last");
+      for (size_t k = 0; k < e.value.t.size(); ++k) {
+        CHECK(e.value.t[k] == 0.5 * (double)k, "even grid %zu", k);
+        CHECK(std::fabs(e.value.y[k][0] - std::cos(e.value.t[k])) < 1e-8, "even: cos t");
+      }
+      for (size_t k = 0; k < d.value.t.size(); ++k)
+        CHECK(std::fabs(d.value.y[k][0] - std::cos(d.value.t[k])) < 1e-8, "dense: cos t");
+    }
+  }
+  std::printf("point outputs passed\n");
+
   // Ensemble: per-trajectory parameters (the builder's vectors) on Lorenz.
   {
     const int n = 256;

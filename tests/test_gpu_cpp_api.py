@@ -17,3 +17,4 @@ def test_cpp_builder_runs_reference_tests():
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "ALL PASSED" in out.stdout
+    assert "point outputs passed" in out.stdout
