@@ -119,18 +119,16 @@ __global__ __launch_bounds__(kBlock) void chunk_lane(Queue* q, const Range* rang
 // Occupancy hints: the rarely taken double-double pow fallback raises the
 // peak register count past a wave-slot boundary; the hints trade that for a
 // few spills outside the step block.
-template <int G, int C>
+template <int G, int C, bool OUT = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(C == 1 ? 3 : 2))) void
-chunk_group(Queue* q, const Range* ranges,
-                                                     ReqDev* reqs,
-                                                     LaneSlot<GroupDopri5<G, C>>* slots,
-                                                     int nslots, int k_iters) {
+chunk_group(Queue* q, const Range* ranges, ReqDev* reqs, LaneSlot<GroupDopri5<G, C, OUT>>* slots,
+            int nslots, int k_iters) {
   const int slot = blockIdx.x * kBlock + threadIdx.x;  // one per lane; G lanes per system
   const bool valid = slot < nslots;
   const int lane = __lane_id();
   const int l = lane & (G - 1);
   const int leader_lane = lane - l;
-  LaneSlot<GroupDopri5<G, C>> S;
+  LaneSlot<GroupDopri5<G, C, OUT>> S;
   if (valid) S = slots[slot];
   else S.state = -1;
   const int64_t published = *(volatile int64_t*)&q->published;
@@ -241,11 +239,11 @@ void launch_lane(Family& f, ReqDev* reqs, int k, hipStream_t s) {
                      (LaneSlot<L>*)f.slots, f.nslots, k);
 }
 
-template <int G, int C>
+template <int G, int C, bool OUT = false>
 void launch_group(Family& f, ReqDev* reqs, int k, hipStream_t s) {
   int grid = (f.nslots + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((chunk_group<G, C>), dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges, reqs,
-                     (LaneSlot<GroupDopri5<G, C>>*)f.slots, f.nslots, k);
+  hipLaunchKernelGGL((chunk_group<G, C, OUT>), dim3(grid), dim3(kBlock), 0, s, f.q, f.ranges,
+                     reqs, (LaneSlot<GroupDopri5<G, C, OUT>>*)f.slots, f.nslots, k);
 }
 
 // A pool's slot type: its size, slot initialisation and reset.
@@ -274,6 +272,9 @@ bool lazy_pool_kind_2(int family, int method, int ev, Family& f);
 bool lazy_pool_kind_3(int family, int method, int ev, Family& f);
 bool lazy_pool_kind_4(int family, int method, int ev, Family& f);
 bool lazy_pool_kind_5(int family, int method, int ev, Family& f);
+// Spring chains with DOPRI5 and an event solout on the register group mapping
+// of size class cls (caps 4, 8, ..., 256): slot kind, launcher, cap and G.
+bool spring_event_class_kind(int cls, Family& f);
 
 }  // namespace serve
 }  // namespace de

@@ -222,9 +222,10 @@ enum : int32_t { kServerGraph = 1 };
 // carved from the arena: Lorenz / Van der Pol with RK4, Euler and the
 // embedded family at kLaneLazy + 2 method + (Van der Pol), spring chains one
 // lane per system at kSpringVec + method (every method but DOPRI5, and DOPRI5
-// with an event solout), and Lorenz / Van der Pol with an event solout and a
+// asking for DE_MAP_LANE), and Lorenz / Van der Pol with an event solout and a
 // method other than DOPRI5 at kLaneEvent + 4 method + 2 (crossing) + (Van der
-// Pol).
+// Pol), and the spring classes with DOPRI5 and an event solout at
+// kSpringEvent + class.
 constexpr int kSpringEnd = 9;
 constexpr int kLorenzDop853 = 9, kVdpDop853 = 10;
 constexpr int kLorenzEvent = 11, kLorenzCross = 12, kVdpEvent = 13, kVdpCross = 14;
@@ -233,7 +234,8 @@ constexpr int kMethods = DE_METHOD_RKV989E + 1;
 constexpr int kLaneLazy = kEagerPools;
 constexpr int kSpringVec = kLaneLazy + 2 * kMethods;
 constexpr int kLaneEvent = kSpringVec + kMethods;
-constexpr int kPools = kLaneEvent + 4 * kMethods;
+constexpr int kSpringEvent = kLaneEvent + 4 * kMethods;  // + size class
+constexpr int kPools = kSpringEvent + (kSpringEnd - 2);
 constexpr int kLazyRanges = 1 << 16;  // range table of a lazy pool
 
 bool lane_lazy_method(int method) {
@@ -241,13 +243,15 @@ bool lane_lazy_method(int method) {
 }
 
 // The pool of a request (spring chains: of its systems with nm masses), or -1.
-int fam_index_for(de_server* s, int family, int nm, int method, int solout) {
+// Spring chains asking for one lane per system (DE_MAP_LANE) are served so.
+int fam_index_for(de_server* s, int family, int nm, int method, int solout, int mapping) {
   if (method < 0 || method >= kMethods) return -1;
   const bool d853 = method == DE_METHOD_DOP853;
   if (family == DE_FAMILY_SPRING_CHAIN) {
-    if (method != DE_METHOD_DOPRI5 || solout != DE_SOLOUT_DEFAULT) return kSpringVec + method;
+    if (method != DE_METHOD_DOPRI5 || mapping == DE_MAP_LANE) return kSpringVec + method;
     for (int k = 2; k < kSpringEnd; ++k)
-      if (nm <= s->fams[k].cap) return k;
+      if (nm <= s->fams[k].cap)
+        return solout == DE_SOLOUT_DEFAULT ? k : kSpringEvent + (k - 2);
     return -1;
   }
   if (family != DE_FAMILY_LORENZ && family != DE_FAMILY_VAN_DER_POL) return -1;
@@ -265,6 +269,7 @@ int fam_index_for(de_server* s, int family, int nm, int method, int solout) {
 // The slot kind and chunk kernel of lazy pool k (family and method from its index).
 bool lazy_kind(int k, Family& f) {
   int family, method, ev = 0;
+  if (k >= kSpringEvent) return spring_event_class_kind(k - kSpringEvent, f);
   if (k >= kLaneEvent) {
     const int rel = k - kLaneEvent;
     family = rel & 1 ? DE_FAMILY_VAN_DER_POL : DE_FAMILY_LORENZ;
@@ -540,14 +545,17 @@ void scheduler(de_server* s) {
       std::vector<std::vector<int32_t>> lists(s->fams.size());
       const de_ode_batch& b = r->b;
       std::vector<int> needs;
-      const int k0 = fam_index_for(s, b.family, 0, b.method, b.event_solout);
-      const bool classes = b.family == DE_FAMILY_SPRING_CHAIN && k0 < kSpringVec;
+      const int k0 = fam_index_for(s, b.family, 0, b.method, b.event_solout, b.mapping);
+      // Spring chains with DOPRI5 go to the size-class pools (group mappings).
+      const bool vec = k0 >= kSpringVec && k0 < kSpringVec + kMethods;
+      const bool classes = b.family == DE_FAMILY_SPRING_CHAIN && !vec;
       if (classes) {
         for (int64_t i = 0; i < b.n; ++i)
-          lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i], b.method, DE_SOLOUT_DEFAULT)]
+          lists[fam_index_for(s, DE_FAMILY_SPRING_CHAIN, r->sizes[i], b.method, b.event_solout,
+                              b.mapping)]
               .push_back((int32_t)i);
-        for (int k = 2; k < kSpringEnd; ++k)
-          if (!lists[k].empty()) needs.push_back(k);
+        for (size_t k = 0; k < lists.size(); ++k)
+          if (!lists[k].empty()) needs.push_back((int)k);
       } else {
         needs.push_back(k0);
       }
@@ -568,7 +576,7 @@ void scheduler(de_server* s) {
         s->cv_done.notify_all();
         continue;
       }
-      if (k0 >= kSpringVec)
+      if (vec)
         r->vectors = s->fams[k0].vectors + 1 + (b.event_solout != DE_SOLOUT_DEFAULT ? 1 : 0);
       // A pool whose range table is full takes no new work until it drains;
       // then (no kernel is in flight between iterations) its table restarts.
@@ -613,11 +621,10 @@ void scheduler(de_server* s) {
       r->first_iter = s->launched;
       admitted.push_back(r);
       if (classes) {
-        for (int k = 2; k < kSpringEnd; ++k)
-          if (!lists[k].empty()) {
-            publish(s, s->fams[k], r->slot, 0, (int32_t)lists[k].size(), d_lists[k]);
-            r->pools.push_back(k);
-          }
+        for (int k : needs) {
+          publish(s, s->fams[k], r->slot, 0, (int32_t)lists[k].size(), d_lists[k]);
+          r->pools.push_back(k);
+        }
       } else {
         const int k = needs[0];
         publish(s, s->fams[k], r->slot, 0, (int32_t)b.n, nullptr);

@@ -6,7 +6,8 @@
 //                         (FixedLane) and DOP853 lanes with an event solout,
 //                         and every method on spring chains with one lane per
 //                         system (the Vec* solvers, their vectors in the
-//                         request's device workspace)
+//                         request's device workspace), and DOPRI5 spring
+//                         classes with an event solout on the group mapping
 // A lane with an event solout finds its roots inline (no deferred brackets):
 // EventWrappedSolout (EV 1) keeps the t_eval output, the crossing solouts
 // (EV 2) replace it.
@@ -81,6 +82,28 @@ static bool lane_kind_5(int method, int ev, Family& f) {
   if (method == DE_METHOD_DOP853 && ev == 1) return lane_kind<Dop853Lane<F, true, false, 1>>(f);
   if (method == DE_METHOD_DOP853 && ev == 2) return lane_kind<Dop853Lane<F, false, false, 2>>(f);
   return false;
+}
+
+template <int G, int C>
+static bool group_event_kind(int cap, Family& f) {
+  set_pool_kind<LaneSlot<GroupDopri5<G, C, true>>>(f);
+  f.launch = launch_group<G, C, true>;
+  f.cap = cap;
+  f.g = G;
+  return true;
+}
+
+bool spring_event_class_kind(int cls, Family& f) {
+  switch (cls) {  // the configurations of the DOPRI5 class pools
+    case 0: return group_event_kind<4, 1>(4, f);
+    case 1: return group_event_kind<8, 1>(8, f);
+    case 2: return group_event_kind<16, 1>(16, f);
+    case 3: return group_event_kind<32, 1>(32, f);
+    case 4: return group_event_kind<64, 1>(64, f);
+    case 5: return group_event_kind<64, 2>(128, f);
+    case 6: return group_event_kind<64, 4>(256, f);
+    default: return false;
+  }
 }
 
 bool lazy_pool_kind_5(int family, int method, int ev, Family& f) {

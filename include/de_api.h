@@ -388,7 +388,8 @@ double de_server_latency(de_server* s, uint64_t ticket);
 int de_server_stats(de_server* s, int64_t* iterations, int64_t* range_resets);
 /* Pools made on first use (RK4, Euler and the embedded family on lane families,
  * spring chains on one lane per system, event lanes of the methods other than
- * DOPRI5), their memory taken from the budget's arena: how many are made now,
+ * DOPRI5, spring chains with DOPRI5 and an event solout on the group mapping of
+ * each size class), their memory taken from the budget's arena: how many are made now,
  * and how many times an idle one was released to make room for a request. */
 int de_server_pools(de_server* s, int32_t* made, int64_t* released);
 /* Scheduler wall time so far, in seconds: admitting requests (uploads and

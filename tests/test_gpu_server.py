@@ -374,3 +374,34 @@ def test_idle_lazy_pools_are_released_under_a_small_budget():
             _same_outputs(srv.poll(srv.submit(r)), r.solve())
         made, released = srv.pools()
         assert released > 0 and 0 < made < 25
+
+
+def test_served_spring_events_on_the_group_mapping(orc):
+    # Spring chains with DOPRI5 and an event or crossing solout are served on
+    # the register group mapping of each size class (pools made on first
+    # use): against the oracle, the group mappings' bound -- step and event
+    # counts equal on at least 99.9% of systems, and where they are, final
+    # states within 1e-10 and event times within 1e-8, states within 1e-7.
+    from deamd import CrossingDirection, Event
+    sysm, y0 = deamd.gen_spring(400, seed=61)
+    base = lambda: IVP.ode(sysm, 0.0, 3.0, y0).method(
+        ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9).max_steps(1000000))
+    reqs = [base().event(Event.component(1, 0.0)).options(max_events=6),
+            base().crossing(0, 0.0, CrossingDirection.Positive).options(max_events=3)]
+    with Server(8 * GIB) as srv:
+        ts = [srv.submit(r) for r in reqs]
+        got = [srv.poll(t) for t in ts]
+    for g, r in zip(got, reqs):
+        w = orc.solve_ode(r, threads=16)
+        assert (w.n_events > 0).mean() > 0.5
+        assert (g.status == w.status).all()
+        same = (g.accepted == w.accepted) & (g.rejected == w.rejected) & (g.n_events == w.n_events)
+        assert same.mean() >= 0.999
+        for i in np.nonzero(same)[0]:
+            a, b = sysm.state_offset[i], sysm.state_offset[i + 1]
+            assert np.abs(g.y[a:b] - w.y[a:b]).max() <= 1e-10 * np.abs(w.y[a:b]).max()
+            gt, gy = g.events(i)
+            wt, wy = w.events(i)
+            assert np.abs(gt - wt).max(initial=0.0) <= 1e-8 * 3.0
+            assert np.abs(gy - wy).max(initial=0.0) <= 1e-7 * max(np.abs(wy).max(initial=0.0),
+                                                                    1e-300)
