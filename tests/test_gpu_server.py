@@ -376,7 +376,8 @@ def test_idle_lazy_pools_are_released_under_a_small_budget():
         assert released > 0 and 0 < made < 25
 
 
-def test_served_spring_events_on_the_group_mapping(orc):
+@pytest.mark.parametrize("graph", [False, True])
+def test_served_spring_events_on_the_group_mapping(orc, graph):
     # Spring chains with DOPRI5 and an event or crossing solout are served on
     # the register group mapping of each size class (pools made on first
     # use): against the oracle, the group mappings' bound -- step and event
@@ -388,7 +389,7 @@ def test_served_spring_events_on_the_group_mapping(orc):
         ExplicitRungeKutta.dopri5().rtol(1e-7).atol(1e-9).max_steps(1000000))
     reqs = [base().event(Event.component(1, 0.0)).options(max_events=6),
             base().crossing(0, 0.0, CrossingDirection.Positive).options(max_events=3)]
-    with Server(8 * GIB) as srv:
+    with Server(8 * GIB, graph=graph) as srv:
         ts = [srv.submit(r) for r in reqs]
         got = [srv.poll(t) for t in ts]
     for g, r in zip(got, reqs):
